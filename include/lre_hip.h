@@ -1,0 +1,101 @@
+/*
+ * lre_hip.h — C ABI of the MI355X (gfx950) develop path of libraw_enhanced.
+ *
+ * A HIP backend that sits where the reference's Accelerator dispatches to its
+ * CPU or Metal backends (core/accelerator.h:247-375, core/accelerator.cpp:15-24).
+ * Every entry point returns 0 on success and a negative status on failure
+ * (the caller falls back to the CPU path, as the reference's facade does on a
+ * `false` return, core/accelerator.cpp:141-157); lre_hip_last_error() gives the
+ * message.  Nothing here throws.
+ *
+ * Image layout: row-major H x W.  "rgb" buffers are interleaved float RGB
+ * (the reference's ImageBufferFloat, core/accelerator.h:71-80); "cfa" planes are
+ * one float per site; "img4" is the reference's uint16 x 4 ImageBuffer
+ * (core/accelerator.h:59-68).
+ *
+ * Two families of entry points:
+ *   lre_hip_<stage>(ctx, host pointers...)  copy in, run, copy out, synchronise —
+ *       the drop-in for the corresponding Accelerator method;
+ *   lre_hip_dev_<stage>(ctx, device pointers..., stream)  enqueue only, on the
+ *       given HIP stream (NULL = the context's stream); inputs stay in HBM.
+ */
+#ifndef LRE_HIP_H
+#define LRE_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    LRE_OK = 0,
+    LRE_ERR_ARG = -1,      /* invalid argument (null pointer, bad size, mismatch) */
+    LRE_ERR_HIP = -2,      /* HIP runtime error */
+    LRE_ERR_NODEV = -3,    /* no usable gfx950 device */
+    LRE_ERR_NOMEM = -4,    /* device allocation failed */
+};
+
+typedef struct lre_ctx lre_ctx;
+
+/* 1 when a HIP device is present and usable. */
+int lre_hip_available(void);
+/* Context bound to one device: owns a stream and grow-only device scratch. */
+int lre_hip_create(lre_ctx **ctx, int device);
+int lre_hip_destroy(lre_ctx *ctx);
+/* Message of the last failure on the calling thread ("" if none). */
+const char *lre_hip_last_error(void);
+/* Wait for all work enqueued on the context's stream. */
+int lre_hip_synchronize(lre_ctx *ctx);
+/* The context's HIP stream (hipStream_t), for callers that enqueue their own work. */
+void *lre_hip_stream(lre_ctx *ctx);
+
+/* ---- host-pointer entry points (drop-in for the Accelerator methods) ---- */
+
+/* Accelerator::demosaic_bayer_amaze (core/accelerator.h:279-281,
+ * core/cpu_accelerator.cpp:1635-1653): raw_rgb holds the CFA sample at its
+ * native channel (other channels ignored); the output is clipped at 0 and the
+ * 4-px frame is rebuilt by border interpolation. */
+int lre_hip_amaze(lre_ctx *ctx, const float *raw_rgb, float *rgb, int width, int height,
+                  uint32_t filters, const float cam_mul[4], float maximum_value);
+
+/* Accelerator::tone_mapping (core/accelerator.h:322-323,
+ * core/cpu_accelerator.cpp:3034-3068): ACES per channel times after_scale.
+ * in == out is allowed. */
+int lre_hip_tone_mapping(lre_ctx *ctx, const float *in_rgb, float *out_rgb, int width,
+                         int height, float after_scale);
+
+/* Accelerator::apply_detail_preserving_tonemap_gpu (core/accelerator.h:329-330);
+ * the CPU version is LibRawWrapper's apply_detail_preserving_tonemap
+ * (core/libraw_wrapper.cpp:1398-1471).  in == out is allowed. */
+int lre_hip_detail_tonemap(lre_ctx *ctx, const float *in_rgb, float *out_rgb, int width,
+                           int height);
+
+/* Accelerator::apply_white_balance (core/accelerator.h:304-307,
+ * core/cpu_accelerator.cpp:2789-2869): writes float(sample) * wb[c] to the
+ * native channel of rgb and leaves the other two channels untouched.
+ * filters == 9 selects the X-Trans table xtrans (36 entries, row-major). */
+int lre_hip_white_balance(lre_ctx *ctx, const uint16_t *img4, float *rgb, int width,
+                          int height, const float wb[4], uint32_t filters,
+                          const signed char xtrans[36]);
+
+/* ---- device-resident entry points (enqueue on `stream`, no host sync) ---- */
+
+/* AMaZE from a float CFA plane (one sample per site) to interleaved RGB. */
+int lre_hip_dev_amaze(lre_ctx *ctx, const float *d_cfa, float *d_rgb, int width, int height,
+                      uint32_t filters, const float cam_mul[4], float maximum_value,
+                      void *stream);
+int lre_hip_dev_tone_mapping(lre_ctx *ctx, const float *d_in, float *d_out, int width,
+                             int height, float after_scale, void *stream);
+int lre_hip_dev_detail_tonemap(lre_ctx *ctx, const float *d_in, float *d_out, int width,
+                               int height, void *stream);
+/* Demosaic input planes: extract the native channel of the reference's 3-channel
+ * layout into a CFA plane. */
+int lre_hip_dev_rgb3_to_cfa(lre_ctx *ctx, const float *d_raw_rgb, float *d_cfa, int width,
+                            int height, uint32_t filters, const signed char xtrans[36],
+                            void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LRE_HIP_H */

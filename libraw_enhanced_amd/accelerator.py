@@ -1,0 +1,176 @@
+"""Host-side mirror of the reference's Accelerator / LibRawWrapper methods for
+the develop path, running on the HIP backend through the C ABI of
+liblre_hip.so (include/lre_hip.h).
+
+Method names, argument meaning and defaults follow the reference:
+  Accelerator methods          core/accelerator.h:247-375
+  standalone numpy methods     core/python_bindings.cpp:324-351,
+                               core/libraw_wrapper.cpp:3098-3214
+There is no CPU fallback here: if the native library or the GPU is missing the
+call raises, so a result always comes from the HIP kernels.
+"""
+from __future__ import annotations
+
+import ctypes
+import threading
+
+import numpy as np
+
+from ._libs import load_library
+
+_f32p = ctypes.POINTER(ctypes.c_float)
+_u16p = ctypes.POINTER(ctypes.c_uint16)
+_i8p = ctypes.POINTER(ctypes.c_int8)
+_vp = ctypes.c_void_p
+
+FILTERS_XTRANS = 9
+RGGB = 0x94949494
+
+
+class HIPError(RuntimeError):
+    """A HIP entry point returned a non-zero status."""
+
+
+class LibRawInvalidArgument(ValueError):
+    """Mirror of the reference's LibRawInvalidArgument (python_bindings.cpp:503-505)."""
+
+
+_SIGS = {
+    "lre_hip_available": ([], ctypes.c_int),
+    "lre_hip_create": ([ctypes.POINTER(_vp), ctypes.c_int], ctypes.c_int),
+    "lre_hip_destroy": ([_vp], ctypes.c_int),
+    "lre_hip_last_error": ([], ctypes.c_char_p),
+    "lre_hip_synchronize": ([_vp], ctypes.c_int),
+    "lre_hip_stream": ([_vp], _vp),
+    "lre_hip_amaze": ([_vp, _f32p, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, _f32p,
+                       ctypes.c_float], ctypes.c_int),
+    "lre_hip_tone_mapping": ([_vp, _f32p, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_float],
+                             ctypes.c_int),
+    "lre_hip_detail_tonemap": ([_vp, _f32p, _f32p, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "lre_hip_white_balance": ([_vp, _u16p, _f32p, ctypes.c_int, ctypes.c_int, _f32p,
+                               ctypes.c_uint32, _i8p], ctypes.c_int),
+    "lre_hip_dev_amaze": ([_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, _f32p,
+                           ctypes.c_float, _vp], ctypes.c_int),
+    "lre_hip_dev_tone_mapping": ([_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, _vp],
+                                 ctypes.c_int),
+    "lre_hip_dev_detail_tonemap": ([_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp], ctypes.c_int),
+    "lre_hip_dev_rgb3_to_cfa": ([_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, _i8p,
+                                 _vp], ctypes.c_int),
+}
+
+_lock = threading.Lock()
+
+
+def native():
+    """The loaded liblre_hip.so with its ctypes signatures."""
+    lib = load_library("liblre_hip.so")
+    with _lock:
+        if not getattr(lib, "_lre_sigs", False):
+            for name, (args, res) in _SIGS.items():
+                fn = getattr(lib, name, None)
+                if fn is None:
+                    continue
+                fn.argtypes = args
+                fn.restype = res
+            lib._lre_sigs = True
+    return lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        msg = native().lre_hip_last_error().decode(errors="replace")
+        if rc == -1:
+            raise LibRawInvalidArgument(msg)
+        raise HIPError(f"status {rc}: {msg}")
+
+
+def _rgb(a, name="image"):
+    a = np.asarray(a)
+    if a.ndim != 3 or a.shape[2] != 3 or a.dtype != np.float32:
+        raise LibRawInvalidArgument(f"{name} must be a (H, W, 3) float32 array")
+    return np.ascontiguousarray(a)
+
+
+def _p(a, t=_f32p):
+    return a.ctypes.data_as(t)
+
+
+class HIPAccelerator:
+    """One HIP context (device, stream, scratch) — the drop-in backend."""
+
+    def __init__(self, device: int = 0):
+        lib = native()
+        if not lib.lre_hip_available():
+            raise HIPError("no HIP device available")
+        h = _vp()
+        _check(lib.lre_hip_create(ctypes.byref(h), device))
+        self._h = h
+        self.device = device
+
+    # -- lifecycle --
+    def close(self):
+        if getattr(self, "_h", None):
+            native().lre_hip_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def synchronize(self):
+        _check(native().lre_hip_synchronize(self._h))
+
+    # -- Accelerator methods (host arrays in, new host arrays out) --
+    def demosaic_bayer_amaze(self, raw_rgb, filters: int = RGGB, cam_mul=(1.0, 1.0, 1.0, 1.0),
+                             maximum_value: float = 1.0):
+        raw = _rgb(raw_rgb, "raw_buffer")
+        h, w, _ = raw.shape
+        out = np.empty_like(raw)
+        cm = np.ascontiguousarray(cam_mul, np.float32)
+        _check(native().lre_hip_amaze(self._h, _p(raw), _p(out), w, h, filters, _p(cm),
+                                      maximum_value))
+        return out
+
+    def tone_mapping(self, rgb, after_scale: float = 1.0):
+        src = _rgb(rgb)
+        h, w, _ = src.shape
+        out = np.empty_like(src)
+        _check(native().lre_hip_tone_mapping(self._h, _p(src), _p(out), w, h, after_scale))
+        return out
+
+    def apply_detail_preserving_tonemap(self, rgb):
+        src = _rgb(rgb)
+        h, w, _ = src.shape
+        out = np.empty_like(src)
+        _check(native().lre_hip_detail_tonemap(self._h, _p(src), _p(out), w, h))
+        return out
+
+    def apply_white_balance(self, img4, wb, filters: int = RGGB, xtrans=None, rgb=None):
+        img4 = np.ascontiguousarray(img4, np.uint16)
+        if img4.ndim != 3 or img4.shape[2] != 4:
+            raise LibRawInvalidArgument("raw_buffer must be (H, W, 4) uint16")
+        h, w, _ = img4.shape
+        out = np.zeros((h, w, 3), np.float32) if rgb is None else _rgb(rgb).copy()
+        wbv = np.ascontiguousarray(wb, np.float32)
+        xt = None if xtrans is None else np.ascontiguousarray(np.asarray(xtrans).reshape(-1), np.int8)
+        _check(native().lre_hip_white_balance(self._h, _p(img4, _u16p), _p(out), w, h, _p(wbv),
+                                              filters, None if xt is None else _p(xt, _i8p)))
+        return out
+
+
+class LibRawWrapper:
+    """The standalone float32 methods of the reference's LibRawWrapper
+    (core/python_bindings.cpp:324-351): input (H, W, 3) float32, new array out."""
+
+    def __init__(self, device: int = 0):
+        self._acc = HIPAccelerator(device)
+
+    def tone_mapping(self, image, after_scale: float = 1.0):
+        # core/libraw_wrapper.cpp:3098-3131
+        return self._acc.tone_mapping(_rgb(image), after_scale)

@@ -1,0 +1,706 @@
+// AMaZE Bayer demosaic for gfx950.
+//
+// Semantics: the reference's tiled AMaZE (core/cpu_accelerator.cpp:1145-1630)
+// followed by its 4-px border interpolation (:2984-3028).  The reference's output
+// depends on its tile grid (160x160 tiles, step 128, 16-px reflect-101 border),
+// on in-place raster-order loops, and on several scratch arrays that alias each
+// other, so each workgroup here processes exactly one reference tile at a time
+// in a private arena with the reference's offsets:
+//
+//   * loops without intra-loop dependencies run across the whole workgroup;
+//   * the variance test of stage 3 (:1376-1397) updates hcd left-to-right and
+//     vcd top-to-bottom, each reading its already-updated predecessor two sites
+//     back: it runs as one independent scan per (row, parity) for hcd and per
+//     (column, parity) for vcd;
+//   * the hvwt smoothing (:1446-1448) and the pmwt smoothing (:1537-1539) read
+//     their updated upper-diagonal neighbours: they run row by row, in parallel
+//     within a row;
+//   * the arena is cleared before each tile, so every tile is a pure function
+//     of its input (the reference's per-thread arena carries values from the
+//     previous tile of the same OpenMP thread into a few unfilled cells).
+//
+// The kernel is persistent: a grid of `slots` workgroups walks the tile list,
+// each owning one arena (1.41 MiB).  All arithmetic is FP32 with the operand
+// order of the reference (built with -ffp-contract=off).
+#include "lre_common.h"
+#include "lre_internal.h"
+
+namespace {
+
+constexpr int TS = 160, TSH = 80, N2 = TS * TS;
+constexpr int BLOCK = 256;
+constexpr float EPS = 1e-5f, EPSSQ = 1e-10f, ARTHRESH = 0.75f;
+
+// Arena layout in floats (reference :1222-1240).
+constexpr int O_GREEN = 0, O_DSQ = N2, O_WV = 2 * N2, O_WH = 3 * N2, O_VCD = 4 * N2,
+              O_HCD = 5 * N2, O_VCDALT = 6 * N2, O_HCDALT = 7 * N2, O_CDSQ = 8 * N2,
+              O_HVWT = 9 * N2, O_DGV = 9 * N2 + TS * TSH, O_DGH = 10 * N2 + TS * TSH,
+              O_Q1M = 11 * N2 + TS * TSH, O_Q1P = 11 * N2 + 2 * TS * TSH,
+              O_CFA = 11 * N2 + 3 * TS * TSH, O_NYQ = 12 * N2 + 3 * TS * TSH,
+              O_NYQTEST = O_NYQ + TS * TSH / 4, ARENA_FLOATS = O_NYQTEST + TS * TSH;
+static_assert(ARENA_FLOATS % 64 == 0, "arena stride keeps 256-byte alignment");
+
+struct Args {
+    const float *cfa_plane;
+    float *rgb;
+    int W, H;
+    unsigned cf[2][2];
+    int ex, ey;
+    float clip_pt, clip_pt8, scale;
+    int ntc, ntiles;
+    float *arenas;
+};
+
+__device__ const float GODD[4] = {0.14659727707323927f, 0.103592713382435f, 0.0732036125103057f,
+                                  0.0365543548389495f};
+__device__ const float GGRAD[6] = {0.5f * 0.07384411893421103f, 0.5f * 0.06207511968171489f,
+                                   0.5f * 0.0521818194747806f, 0.5f * 0.03687419286733595f,
+                                   0.5f * 0.03099732204057846f, 0.5f * 0.018413194161458882f};
+__device__ const float GEVEN[2] = {0.13719494435797422f, 0.05640252782101291f};
+__device__ const float GQUINC[4] = {0.169917f, 0.108947f, 0.069855f, 0.0287182f};
+
+LRE_DEV float half(float x) { return x * 0.5f; }
+LRE_DEV float twice(float x) { return x + x; }
+LRE_DEV int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+template <class F>
+LRE_DEV void for_rect(int r0, int r1, int c0, int c1, F f) {
+    const int nc = c1 - c0, nr = r1 - r0;
+    if (nc <= 0 || nr <= 0) return;
+    for (int t = threadIdx.x; t < nr * nc; t += BLOCK) f(r0 + t / nc, c0 + t % nc);
+}
+
+// Non-green ("site") positions of rows [r0,r1): cc = c0 + parity(row) + 2j < c1.
+template <class F>
+LRE_DEV void for_sites(const Args &A, int r0, int r1, int c0, int c1, F f) {
+    const int ns = (c1 - c0 + 1) / 2, nr = r1 - r0;
+    if (ns <= 0 || nr <= 0) return;
+    for (int t = threadIdx.x; t < nr * ns; t += BLOCK) {
+        int rr = r0 + t / ns;
+        int cc = c0 + (int)(A.cf[rr & 1][0] & 1) + 2 * (t % ns);
+        if (cc < c1) f(rr, cc);
+    }
+}
+
+LRE_DEV unsigned fct(const Args &A, int r, int c) { return A.cf[r & 1][c & 1]; }
+
+__device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
+    constexpr int v1 = TS, v2 = 2 * TS, v3 = 3 * TS;
+    constexpr int p1 = -TS + 1, p2 = -2 * TS + 2, p3 = -3 * TS + 3;
+    constexpr int m1 = TS + 1, m2 = 2 * TS + 2, m3 = 3 * TS + 3;
+    const int W = A.W, H = A.H;
+    float *cfa = ar + O_CFA, *green = ar + O_GREEN;
+    float *wv = ar + O_WV, *wh = ar + O_WH, *dsq = ar + O_DSQ;
+    float *vcd = ar + O_VCD, *hcd = ar + O_HCD, *vcdalt = ar + O_VCDALT, *hcdalt = ar + O_HCDALT;
+    float *dgv = ar + O_DGV, *dgh = ar + O_DGH, *cdsq = ar + O_CDSQ, *hvwt = ar + O_HVWT;
+    float *nyqtest = ar + O_NYQTEST;
+    unsigned char *nyq = reinterpret_cast<unsigned char *>(ar + O_NYQ);
+    unsigned char *nyq2 = reinterpret_cast<unsigned char *>(cdsq);
+
+    const int bottom = min(top + TS, H + 16), right = min(left + TS, W + 16);
+    const int rr1 = bottom - top, cc1 = right - left;
+    const int rrmin = top < 0 ? 16 : 0, ccmin = left < 0 ? 16 : 0;
+    const int rrmax = bottom > H ? H - top : rr1;
+    const int ccmax = right > W ? W - left : cc1;
+
+    // clear the arena (clean-tile contract)
+    {
+        float4 *a4 = reinterpret_cast<float4 *>(ar);
+        const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int t = threadIdx.x; t < ARENA_FLOATS / 4; t += BLOCK) a4[t] = z;
+    }
+    __syncthreads();
+
+    // tile fill (:1256-1325), replaying the reference's writes by linear index:
+    // the right border of a row starting past column 144 runs into the first
+    // columns of the next row, and the lower border of a 160-row tile runs past
+    // the end of the tile arrays; the right border is written last and wins.
+    // The four border corners stay empty.
+    {
+        const int nc = ccmax - ccmin, nr = rrmax - rrmin;
+        const int n_up = rrmin > 0 ? 16 * nc : 0;
+        const int n_in = nr * nc;
+        const int n_lo = rrmax < rr1 ? 16 * nc : 0;
+        const int n_le = ccmin > 0 ? 16 * nr : 0;
+        for (int t = threadIdx.x; t < n_up + n_in + n_lo + n_le; t += BLOCK) {
+            int row, col, L;
+            if (t < n_up) {
+                const int rr = t / nc, cc = ccmin + t % nc;
+                row = clampi(32 - rr + top, 0, H - 1); col = clampi(cc + left, 0, W - 1);
+                L = rr * TS + cc;
+            } else if (t < n_up + n_in) {
+                const int u = t - n_up, rr = rrmin + u / nc, cc = ccmin + u % nc;
+                row = rr + top; col = cc + left;
+                L = rr * TS + cc;
+            } else if (t < n_up + n_in + n_lo) {
+                const int u = t - n_up - n_in, k = u / nc, cc = ccmin + u % nc;
+                row = clampi(H - k - 2, 0, H - 1); col = clampi(left + cc, 0, W - 1);
+                L = (rrmax + k) * TS + cc;
+            } else {
+                const int u = t - n_up - n_in - n_lo, rr = rrmin + u / 16, cc = u % 16;
+                row = clampi(rr + top, 0, H - 1); col = clampi(32 - cc + left, 0, W - 1);
+                L = rr * TS + cc;
+            }
+            const float v = A.cfa_plane[(size_t)row * W + col] * A.scale;
+            cfa[L] = v;
+            green[L] = v;
+        }
+        __syncthreads();
+        if (ccmax < cc1)
+            for (int t = threadIdx.x; t < 16 * nr; t += BLOCK) {
+                const int rr = rrmin + t / 16, k = t % 16;
+                const int row = clampi(top + rr, 0, H - 1), col = clampi(W - k - 2, 0, W - 1);
+                const float v = A.cfa_plane[(size_t)row * W + col] * A.scale;
+                cfa[rr * TS + ccmax + k] = v;
+                green[rr * TS + ccmax + k] = v;
+            }
+    }
+    __syncthreads();
+
+    // stage 1 (:1331-1339)
+    for_rect(2, rr1 - 2, 2, cc1 - 2, [&](int rr, int cc) {
+        int i = rr * TS + cc;
+        float dh = fabsf(cfa[i + 1] - cfa[i - 1]);
+        float dv = fabsf(cfa[i + v1] - cfa[i - v1]);
+        wv[i] = EPS + fabsf(cfa[i + v2] - cfa[i]) + fabsf(cfa[i] - cfa[i - v2]) + dv;
+        wh[i] = EPS + fabsf(cfa[i + 2] - cfa[i]) + fabsf(cfa[i] - cfa[i - 2]) + dh;
+        dsq[i] = lre::sq(dh) + lre::sq(dv);
+    });
+    __syncthreads();
+
+    // stage 2 (:1342-1373)
+    for_rect(4, rr1 - 4, 4, cc1 - 4, [&](int rr, int cc) {
+        int i = rr * TS + cc;
+        float sgn = (fct(A, rr, cc) & 1) ? -1.f : 1.f;
+        float c0 = cfa[i];
+        float ru = cfa[i - v1] * (wv[i - v2] + wv[i]) / (wv[i - v2] * (EPS + c0) + wv[i] * (EPS + cfa[i - v2]));
+        float rd = cfa[i + v1] * (wv[i + v2] + wv[i]) / (wv[i + v2] * (EPS + c0) + wv[i] * (EPS + cfa[i + v2]));
+        float rl = cfa[i - 1] * (wh[i - 2] + wh[i]) / (wh[i - 2] * (EPS + c0) + wh[i] * (EPS + cfa[i - 2]));
+        float rr_ = cfa[i + 1] * (wh[i + 2] + wh[i]) / (wh[i + 2] * (EPS + c0) + wh[i] * (EPS + cfa[i + 2]));
+        float gu_h = cfa[i - v1] + half(c0 - cfa[i - v2]);
+        float gd_h = cfa[i + v1] + half(c0 - cfa[i + v2]);
+        float gl_h = cfa[i - 1] + half(c0 - cfa[i - 2]);
+        float gr_h = cfa[i + 1] + half(c0 - cfa[i + 2]);
+        float gu_a = fabsf(1.f - ru) < ARTHRESH ? c0 * ru : gu_h;
+        float gd_a = fabsf(1.f - rd) < ARTHRESH ? c0 * rd : gd_h;
+        float gl_a = fabsf(1.f - rl) < ARTHRESH ? c0 * rl : gl_h;
+        float gr_a = fabsf(1.f - rr_) < ARTHRESH ? c0 * rr_ : gr_h;
+        float hw = wh[i - 1] / (wh[i - 1] + wh[i + 1]);
+        float vw = wv[i - v1] / (wv[i + v1] + wv[i - v1]);
+        float gv_h = vw * gd_h + (1.f - vw) * gu_h;
+        float gh_h = hw * gr_h + (1.f - hw) * gl_h;
+        float va = sgn * (gv_h - c0), ha = sgn * (gh_h - c0);
+        vcdalt[i] = va;
+        hcdalt[i] = ha;
+        if (c0 > A.clip_pt8 || gv_h > A.clip_pt8 || gh_h > A.clip_pt8) {
+            vcd[i] = va;
+            hcd[i] = ha;
+            gu_a = gu_h; gd_a = gd_h; gl_a = gl_h; gr_a = gr_h;
+        } else {
+            vcd[i] = sgn * (vw * gd_a + (1.f - vw) * gu_a - c0);
+            hcd[i] = sgn * (hw * gr_a + (1.f - hw) * gl_a - c0);
+        }
+        dgv[i] = lre::min_std(lre::sq(gu_h - gd_h), lre::sq(gu_a - gd_a));
+        dgh[i] = lre::min_std(lre::sq(gl_h - gr_h), lre::sq(gl_a - gr_a));
+    });
+    __syncthreads();
+
+    // stage 3 (:1376-1397): hcd scans along rows, vcd scans down columns
+    {
+        const int nh = 2 * max(rr1 - 8, 0), nv = 2 * max(cc1 - 8, 0);
+        for (int t = threadIdx.x; t < nh + nv; t += BLOCK) {
+            if (t < nh) {
+                const int rr = 4 + t / 2, par = t & 1;
+                float prev = hcd[rr * TS + 2 + par];
+                for (int cc = 4 + par; cc < cc1 - 4; cc += 2) {
+                    const int i = rr * TS + cc;
+                    const float c0 = cfa[i], cur = hcd[i], nxt = hcd[i + 2];
+                    const float a0 = hcdalt[i - 2], a1 = hcdalt[i], a2 = hcdalt[i + 2];
+                    float hv = 3.f * (lre::sq(prev) + lre::sq(cur) + lre::sq(nxt)) - lre::sq(prev + cur + nxt);
+                    float hav = 3.f * (lre::sq(a0) + lre::sq(a1) + lre::sq(a2)) - lre::sq(a0 + a1 + a2);
+                    float h = hav < hv ? a1 : cur;
+                    float gh;
+                    const bool is_g = fct(A, rr, cc) & 1;
+                    const float lf = cfa[i - 1], rt = cfa[i + 1];
+                    if (is_g) {
+                        gh = -h + c0;
+                        if (h > 0) {
+                            if (3.f * h > (gh + c0)) h = -lre::med3(gh, lf, rt) + c0;
+                            else {
+                                float w2 = 1.f - 3.f * h / (EPS + gh + c0);
+                                h = w2 * h + (1.f - w2) * (-lre::med3(gh, lf, rt) + c0);
+                            }
+                        }
+                    } else {
+                        gh = h + c0;
+                        if (h < 0) {
+                            if (3.f * h < -(gh + c0)) h = lre::med3(gh, lf, rt) - c0;
+                            else {
+                                float w2 = 1.f + 3.f * h / (EPS + gh + c0);
+                                h = w2 * h + (1.f - w2) * (lre::med3(gh, lf, rt) - c0);
+                            }
+                        }
+                    }
+                    if (gh > A.clip_pt) h = (is_g ? -1.f : 1.f) * (lre::med3(gh, lf, rt) - c0);
+                    hcd[i] = h;
+                    prev = h;
+                }
+            } else {
+                const int u = t - nh, cc = 4 + u / 2, par = u & 1;
+                float prev = vcd[(2 + par) * TS + cc];
+                for (int rr = 4 + par; rr < rr1 - 4; rr += 2) {
+                    const int i = rr * TS + cc;
+                    const float c0 = cfa[i], cur = vcd[i], nxt = vcd[i + v2];
+                    const float a0 = vcdalt[i - v2], a1 = vcdalt[i], a2 = vcdalt[i + v2];
+                    float vv = 3.f * (lre::sq(prev) + lre::sq(cur) + lre::sq(nxt)) - lre::sq(prev + cur + nxt);
+                    float vav = 3.f * (lre::sq(a0) + lre::sq(a1) + lre::sq(a2)) - lre::sq(a0 + a1 + a2);
+                    float v = vav < vv ? a1 : cur;
+                    float gv;
+                    const bool is_g = fct(A, rr, cc) & 1;
+                    const float up = cfa[i - v1], dn = cfa[i + v1];
+                    if (is_g) {
+                        gv = -v + c0;
+                        if (v > 0) {
+                            if (3.f * v > (gv + c0)) v = -lre::med3(gv, up, dn) + c0;
+                            else {
+                                float w2 = 1.f - 3.f * v / (EPS + gv + c0);
+                                v = w2 * v + (1.f - w2) * (-lre::med3(gv, up, dn) + c0);
+                            }
+                        }
+                    } else {
+                        gv = v + c0;
+                        if (v < 0) {
+                            if (3.f * v < -(gv + c0)) v = lre::med3(gv, up, dn) - c0;
+                            else {
+                                float w2 = 1.f + 3.f * v / (EPS + gv + c0);
+                                v = w2 * v + (1.f - w2) * (lre::med3(gv, up, dn) - c0);
+                            }
+                        }
+                    }
+                    if (gv > A.clip_pt) v = (is_g ? -1.f : 1.f) * (lre::med3(gv, up, dn) - c0);
+                    vcd[i] = v;
+                    prev = v;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for_rect(4, rr1 - 4, 4, cc1 - 4, [&](int rr, int cc) {
+        int i = rr * TS + cc;
+        cdsq[i] = lre::sq(vcd[i] - hcd[i]);
+    });
+    __syncthreads();
+
+    // stages 4-5 (:1400-1428)
+    for_sites(A, 6, rr1 - 6, 6, cc1 - 6, [&](int rr, int cc) {
+        int i = rr * TS + cc;
+        float uave = vcd[i] + vcd[i - v1] + vcd[i - v2] + vcd[i - v3];
+        float dave = vcd[i] + vcd[i + v1] + vcd[i + v2] + vcd[i + v3];
+        float lave = hcd[i] + hcd[i - 1] + hcd[i - 2] + hcd[i - 3];
+        float rave = hcd[i] + hcd[i + 1] + hcd[i + 2] + hcd[i + 3];
+        using lre::sq;
+        float vu = sq(vcd[i] - uave) + sq(vcd[i - v1] - uave) + sq(vcd[i - v2] - uave) + sq(vcd[i - v3] - uave);
+        float vd = sq(vcd[i] - dave) + sq(vcd[i + v1] - dave) + sq(vcd[i + v2] - dave) + sq(vcd[i + v3] - dave);
+        float hl = sq(hcd[i] - lave) + sq(hcd[i - 1] - lave) + sq(hcd[i - 2] - lave) + sq(hcd[i - 3] - lave);
+        float hr = sq(hcd[i] - rave) + sq(hcd[i + 1] - rave) + sq(hcd[i + 2] - rave) + sq(hcd[i + 3] - rave);
+        float hw = wh[i - 1] / (wh[i - 1] + wh[i + 1]);
+        float vw = wv[i - v1] / (wv[i + v1] + wv[i - v1]);
+        float vvar = EPSSQ + vw * vd + (1.f - vw) * vu;
+        float hvar = EPSSQ + hw * hr + (1.f - hw) * hl;
+        float vu2 = dgv[i] + dgv[i - v1] + dgv[i - v2];
+        float vd2 = dgv[i] + dgv[i + v1] + dgv[i + v2];
+        float hl2 = dgh[i] + dgh[i - 1] + dgh[i - 2];
+        float hr2 = dgh[i] + dgh[i + 1] + dgh[i + 2];
+        float vvar1 = EPSSQ + vw * vd2 + (1.f - vw) * vu2;
+        float hvar1 = EPSSQ + hw * hr2 + (1.f - hw) * hl2;
+        float varwt = hvar / (vvar + hvar);
+        float diffwt = hvar1 / (vvar1 + hvar1);
+        hvwt[i >> 1] = ((0.5f - varwt) * (0.5f - diffwt) > 0.f && fabsf(0.5f - diffwt) < fabsf(0.5f - varwt)) ? varwt : diffwt;
+        const float *d = cdsq, *g = dsq;
+        nyqtest[i >> 1] =
+            (GODD[0] * d[i] + GODD[1] * (d[i - m1] + d[i + p1] + d[i - p1] + d[i + m1]) +
+             GODD[2] * (d[i - v2] + d[i - 2] + d[i + 2] + d[i + v2]) +
+             GODD[3] * (d[i - m2] + d[i + p2] + d[i - p2] + d[i + m2])) -
+            (GGRAD[0] * g[i] + GGRAD[1] * (g[i - v1] + g[i + 1] + g[i - 1] + g[i + v1]) +
+             GGRAD[2] * (g[i - m1] + g[i + p1] + g[i - p1] + g[i + m1]) +
+             GGRAD[3] * (g[i - v2] + g[i - 2] + g[i + 2] + g[i + v2]) +
+             GGRAD[4] * (g[i - v2 - 1] + g[i - v2 + 1] + g[i - TS - 2] + g[i - TS + 2] + g[i + TS - 2] + g[i + TS + 2] + g[i + v2 - 1] + g[i + v2 + 1]) +
+             GGRAD[5] * (g[i - m2] + g[i + p2] + g[i - p2] + g[i + m2]));
+    });
+    __syncthreads();
+    int local_nyq = 0;
+    for_sites(A, 6, rr1 - 6, 6, cc1 - 6, [&](int rr, int cc) {
+        int i = rr * TS + cc;
+        if (nyqtest[i >> 1] > 0.f) { nyq[i >> 1] = 1; local_nyq = 1; }
+    });
+    const int any_nyq = __syncthreads_or(local_nyq);
+
+    // stage 6 (:1436-1481)
+    if (any_nyq) {
+        for (int t = threadIdx.x; t < TS * TSH / 4; t += BLOCK) reinterpret_cast<unsigned *>(nyq2)[t] = 0u;
+        __syncthreads();
+        for_sites(A, 8, rr1 - 8, 8, cc1 - 8, [&](int rr, int cc) {
+            int i = rr * TS + cc;
+            unsigned s = nyq[(i - v2) >> 1] + nyq[(i - m1) >> 1] + nyq[(i + p1) >> 1] + nyq[(i - 2) >> 1] +
+                         nyq[(i + 2) >> 1] + nyq[(i - p1) >> 1] + nyq[(i + m1) >> 1] + nyq[(i + v2) >> 1];
+            nyq2[i >> 1] = s > 4 ? 1 : (s < 4 ? 0 : nyq[i >> 1]);
+        });
+        __syncthreads();
+        for_sites(A, 8, rr1 - 8, 8, cc1 - 8, [&](int rr, int cc) {
+            int i = rr * TS + cc;
+            if (!nyq2[i >> 1]) return;
+            float scfa = 0, sh = 0, sv = 0, sqh = 0, sqv = 0, area = 0;
+            for (int a = -6; a < 7; a += 2)
+                for (int b = -6; b < 7; b += 2) {
+                    int j = i + a * TS + b;
+                    if (j >= 0 && j < N2 && nyq2[j >> 1]) {
+                        float t = cfa[j];
+                        scfa += t;
+                        sh += (cfa[j - 1] + cfa[j + 1]);
+                        sv += (cfa[j - v1] + cfa[j + v1]);
+                        sqh += lre::sq(t - cfa[j - 1]) + lre::sq(t - cfa[j + 1]);
+                        sqv += lre::sq(t - cfa[j - v1]) + lre::sq(t - cfa[j + v1]);
+                        area += 1;
+                    }
+                }
+            sh = scfa - half(sh);
+            sv = scfa - half(sv);
+            area = half(area);
+            float hv2 = EPSSQ + fabsf(area * sqh - sh * sh), vv2 = EPSSQ + fabsf(area * sqv - sv * sv);
+            hvwt[i >> 1] = hv2 / (vv2 + hv2);
+        });
+        __syncthreads();
+    }
+    float *dg0 = vcdalt;        // Dgrb0 over vcdalt (half-index)
+    float *d2 = ar + O_Q1M;     // Dgrb2 {h,v} pairs over Dgrbsq1m/Dgrbsq1p
+    // hvwt smoothing reads updated upper-diagonal neighbours: row by row
+    for (int rr = 8; rr < rr1 - 8; rr++) {
+        const int c0 = 8 + (int)(fct(A, rr, 2) & 1);
+        for (int cc = c0 + 2 * (int)threadIdx.x; cc < cc1 - 8; cc += 2 * BLOCK) {
+            int i = rr * TS + cc, k = i >> 1;
+            float alt = 0.25f * (hvwt[(i - m1) >> 1] + hvwt[(i + p1) >> 1] + hvwt[(i - p1) >> 1] + hvwt[(i + m1) >> 1]);
+            float hk = hvwt[k];
+            hk = fabsf(0.5f - hk) < fabsf(0.5f - alt) ? alt : hk;
+            hvwt[k] = hk;
+        }
+        __syncthreads();
+    }
+    for_sites(A, 8, rr1 - 8, 8, cc1 - 8, [&](int rr, int cc) {
+        int i = rr * TS + cc, k = i >> 1;
+        float hk = hvwt[k];
+        float d0 = hk * vcd[i] + (1.f - hk) * hcd[i];
+        dg0[k] = d0;
+        float g = cfa[i] + d0;
+        green[i] = g;
+        d2[2 * k] = nyq2[k] ? lre::sq(g - half(green[i - 1] + green[i + 1])) : 0.f;
+        d2[2 * k + 1] = nyq2[k] ? lre::sq(g - half(green[i - v1] + green[i + v1])) : 0.f;
+    });
+    __syncthreads();
+    if (any_nyq) {
+        for_sites(A, 8, rr1 - 8, 8, cc1 - 8, [&](int rr, int cc) {
+            int i = rr * TS + cc, k = i >> 1;
+            if (!nyq2[k]) return;
+#define D2H(o) d2[2 * ((i + (o)) >> 1)]
+#define D2V(o) d2[2 * ((i + (o)) >> 1) + 1]
+            float gvh = EPSSQ + (GQUINC[0] * D2H(0) + GQUINC[1] * (D2H(-m1) + D2H(p1) + D2H(-p1) + D2H(m1)) +
+                                 GQUINC[2] * (D2H(-v2) + D2H(-2) + D2H(2) + D2H(v2)) +
+                                 GQUINC[3] * (D2H(-m2) + D2H(p2) + D2H(-p2) + D2H(m2)));
+            float gvv = EPSSQ + (GQUINC[0] * D2V(0) + GQUINC[1] * (D2V(-m1) + D2V(p1) + D2V(-p1) + D2V(m1)) +
+                                 GQUINC[2] * (D2V(-v2) + D2V(-2) + D2V(2) + D2V(v2)) +
+                                 GQUINC[3] * (D2V(-m2) + D2V(p2) + D2V(-p2) + D2V(m2)));
+#undef D2H
+#undef D2V
+            float d0 = (hcd[i] * gvv + vcd[i] * gvh) / (gvv + gvh);
+            dg0[k] = d0;
+            green[i] = cfa[i] + d0;
+        });
+        __syncthreads();
+    }
+
+    // stage 7 (:1484-1502)
+    float *delp = cdsq, *delm = cdsq + TS * TSH, *q1p = ar + O_Q1P, *q1m = ar + O_Q1M;
+    for_rect(6, rr1 - 6, 0, (cc1 - 6 - 6 + 1) / 2, [&](int rr, int j) {
+        int cc = 6 + 2 * j;
+        int i = rr * TS + cc;
+        if ((fct(A, rr, 2) & 1) == 0) {
+            delp[i >> 1] = fabsf(cfa[i + p1] - cfa[i - p1]);
+            delm[i >> 1] = fabsf(cfa[i + m1] - cfa[i - m1]);
+            q1p[i >> 1] = lre::sq(cfa[i + 1] - cfa[i + 1 - p1]) + lre::sq(cfa[i + 1] - cfa[i + 1 + p1]);
+            q1m[i >> 1] = lre::sq(cfa[i + 1] - cfa[i + 1 - m1]) + lre::sq(cfa[i + 1] - cfa[i + 1 + m1]);
+        } else {
+            q1p[i >> 1] = lre::sq(cfa[i] - cfa[i - p1]) + lre::sq(cfa[i] - cfa[i + p1]);
+            q1m[i >> 1] = lre::sq(cfa[i] - cfa[i - m1]) + lre::sq(cfa[i] - cfa[i + m1]);
+            delp[i >> 1] = fabsf(cfa[i + 1 + p1] - cfa[i + 1 - p1]);
+            delm[i >> 1] = fabsf(cfa[i + 1 + m1] - cfa[i + 1 - m1]);
+        }
+    });
+    __syncthreads();
+
+    // stage 8 (:1505-1532)
+    float *rbm = vcd, *rbp = hcdalt, *pmwt = wh;
+    for_sites(A, 8, rr1 - 8, 8, cc1 - 8, [&](int rr, int cc) {
+        int i = rr * TS + cc, k = i >> 1;
+        float c0 = cfa[i];
+        float rse = twice(cfa[i + m1]) / (EPS + c0 + cfa[i + m2]);
+        float rnw = twice(cfa[i - m1]) / (EPS + c0 + cfa[i - m2]);
+        float rne = twice(cfa[i + p1]) / (EPS + c0 + cfa[i + p2]);
+        float rsw = twice(cfa[i - p1]) / (EPS + c0 + cfa[i - p2]);
+        float bse = fabsf(1.f - rse) < ARTHRESH ? c0 * rse : (cfa[i + m1]) + half(c0 - cfa[i + m2]);
+        float bnw = fabsf(1.f - rnw) < ARTHRESH ? c0 * rnw : (cfa[i - m1]) + half(c0 - cfa[i - m2]);
+        float bne = fabsf(1.f - rne) < ARTHRESH ? c0 * rne : (cfa[i + p1]) + half(c0 - cfa[i + p2]);
+        float bsw = fabsf(1.f - rsw) < ARTHRESH ? c0 * rsw : (cfa[i - p1]) + half(c0 - cfa[i - p2]);
+        float wse = EPS + delm[k] + delm[(i + m1) >> 1] + delm[(i + m2) >> 1];
+        float wnw = EPS + delm[k] + delm[(i - m1) >> 1] + delm[(i - m2) >> 1];
+        float wne = EPS + delp[k] + delp[(i + p1) >> 1] + delp[(i + p2) >> 1];
+        float wsw = EPS + delp[k] + delp[(i - p1) >> 1] + delp[(i - p2) >> 1];
+        float m = (wse * bnw + wnw * bse) / (wse + wnw);
+        float p = (wne * bsw + wsw * bne) / (wne + wsw);
+        const float cp = cfa[i - p1], cp2 = cfa[i + p1], cm = cfa[i - m1], cm2 = cfa[i + m1];
+        if (p < c0) {
+            if (twice(p) < c0) p = lre::med3(p, cp, cp2);
+            else {
+                float w = twice(c0 - p) / (EPS + p + c0);
+                p = w * p + (1.f - w) * lre::med3(p, cp, cp2);
+            }
+        }
+        if (m < c0) {
+            if (twice(m) < c0) m = lre::med3(m, cm, cm2);
+            else {
+                float w = twice(c0 - m) / (EPS + m + c0);
+                m = w * m + (1.f - w) * lre::med3(m, cm, cm2);
+            }
+        }
+        if (p > A.clip_pt) p = lre::med3(p, cp, cp2);
+        if (m > A.clip_pt) m = lre::med3(m, cm, cm2);
+        rbp[k] = p;
+        rbm[k] = m;
+#define QS(a, o) a[(i + (o)) >> 1]
+        float varm = EPSSQ + (GEVEN[0] * (QS(q1m, -v1) + QS(q1m, -1) + QS(q1m, 1) + QS(q1m, v1)) +
+                              GEVEN[1] * (QS(q1m, -v2 - 1) + QS(q1m, -v2 + 1) + QS(q1m, -2 - v1) + QS(q1m, 2 - v1) +
+                                          QS(q1m, -2 + v1) + QS(q1m, 2 + v1) + QS(q1m, v2 - 1) + QS(q1m, v2 + 1)));
+        float varp = EPSSQ + (GEVEN[0] * (QS(q1p, -v1) + QS(q1p, -1) + QS(q1p, 1) + QS(q1p, v1)) +
+                              GEVEN[1] * (QS(q1p, -v2 - 1) + QS(q1p, -v2 + 1) + QS(q1p, -2 - v1) + QS(q1p, 2 - v1) +
+                                          QS(q1p, -2 + v1) + QS(q1p, 2 + v1) + QS(q1p, v2 - 1) + QS(q1p, v2 + 1)));
+#undef QS
+        pmwt[k] = varm / (varp + varm);
+    });
+    __syncthreads();
+
+    // stage 9 (:1535-1540): pmwt smoothing reads updated upper-diagonal neighbours
+    float *rbint = dsq;
+    for (int rr = 10; rr < rr1 - 10; rr++) {
+        const int c0 = 10 + (int)(fct(A, rr, 2) & 1);
+        for (int cc = c0 + 2 * (int)threadIdx.x; cc < cc1 - 10; cc += 2 * BLOCK) {
+            int i = rr * TS + cc, k = i >> 1;
+            float alt = 0.25f * (pmwt[(i - m1) >> 1] + pmwt[(i + p1) >> 1] + pmwt[(i - p1) >> 1] + pmwt[(i + m1) >> 1]);
+            float pk = pmwt[k];
+            if (fabsf(0.5f - pk) < fabsf(0.5f - alt)) pk = alt;
+            pmwt[k] = pk;
+            rbint[k] = half(cfa[i] + rbm[k] * (1.f - pk) + rbp[k] * pk);
+        }
+        __syncthreads();
+    }
+    // (:1541-1566)
+    for_sites(A, 12, rr1 - 12, 12, cc1 - 12, [&](int rr, int cc) {
+        int i = rr * TS + cc, k = i >> 1;
+        if (fabsf(0.5f - pmwt[k]) >= fabsf(0.5f - hvwt[k])) return;
+        float rb = rbint[k];
+        float ru = twice(cfa[i - v1]) / (EPS + rb + rbint[k - v1]);
+        float rd = twice(cfa[i + v1]) / (EPS + rb + rbint[k + v1]);
+        float rl = twice(cfa[i - 1]) / (EPS + rb + rbint[k - 1]);
+        float rr_ = twice(cfa[i + 1]) / (EPS + rb + rbint[k + 1]);
+        float gu = fabsf(1.f - ru) < ARTHRESH ? rb * ru : cfa[i - v1] + half(rb - rbint[k - v1]);
+        float gd = fabsf(1.f - rd) < ARTHRESH ? rb * rd : cfa[i + v1] + half(rb - rbint[k + v1]);
+        float gl = fabsf(1.f - rl) < ARTHRESH ? rb * rl : cfa[i - 1] + half(rb - rbint[k - 1]);
+        float gr = fabsf(1.f - rr_) < ARTHRESH ? rb * rr_ : cfa[i + 1] + half(rb - rbint[k + 1]);
+        float gv = (wv[i - v1] * gd + wv[i + v1] * gu) / (wv[i + v1] + wv[i - v1]);
+        float gh = (wh[i - 1] * gr + wh[i + 1] * gl) / (wh[i - 1] + wh[i + 1]);
+        const float up = cfa[i - v1], dn = cfa[i + v1], lf = cfa[i - 1], rt = cfa[i + 1];
+        if (gv < rb) {
+            if (twice(gv) < rb) gv = lre::med3(gv, up, dn);
+            else {
+                float w = twice(rb - gv) / (EPS + gv + rb);
+                gv = w * gv + (1.f - w) * lre::med3(gv, up, dn);
+            }
+        }
+        if (gh < rb) {
+            if (twice(gh) < rb) gh = lre::med3(gh, lf, rt);
+            else {
+                float w = twice(rb - gh) / (EPS + gh + rb);
+                gh = w * gh + (1.f - w) * lre::med3(gh, lf, rt);
+            }
+        }
+        if (gh > A.clip_pt) gh = lre::med3(gh, lf, rt);
+        if (gv > A.clip_pt) gv = lre::med3(gv, up, dn);
+        float hk = hvwt[k];
+        float g = gh * (1.f - hk) + gv * hk;
+        green[i] = g;
+        dg0[k] = g - cfa[i];
+    });
+    __syncthreads();
+
+    // second chroma difference (:1571-1595)
+    float *dg1 = hcd;  // Dgrb1 over hcd (half-index)
+    {
+        const int r0 = 13 - A.ey;
+        const int nr = rr1 - 12 - r0 > 0 ? (rr1 - 12 - r0 + 1) / 2 : 0;
+        const int k0 = (13 - A.ex) >> 1;  // offsets within a row are row-independent (TS even)
+        const int nk = ((cc1 - 12) >> 1) - k0;
+        if (nk > 0)
+            for (int t = threadIdx.x; t < nr * nk; t += BLOCK) {
+                int rr = r0 + 2 * (t / nk);
+                int k = ((rr * TS + 13 - A.ex) >> 1) + t % nk;
+                if (k < ((rr * TS + cc1 - 12) >> 1)) {
+                    dg1[k] = dg0[k];
+                    dg0[k] = 0;
+                }
+            }
+    }
+    __syncthreads();
+    for_sites(A, 14, rr1 - 14, 14, cc1 - 14, [&](int rr, int cc) {
+        int i = rr * TS + cc;
+        float *D = (1 - (int)fct(A, rr, cc) / 2) ? dg1 : dg0;
+#define DD(o) D[(i + (o)) >> 1]
+        float wnw = 1.f / (EPS + fabsf(DD(-m1) - DD(m1)) + fabsf(DD(-m1) - DD(-m3)) + fabsf(DD(m1) - DD(-m3)));
+        float wne = 1.f / (EPS + fabsf(DD(p1) - DD(-p1)) + fabsf(DD(p1) - DD(p3)) + fabsf(DD(-p1) - DD(p3)));
+        float wsw = 1.f / (EPS + fabsf(DD(-p1) - DD(p1)) + fabsf(DD(-p1) - DD(m3)) + fabsf(DD(p1) - DD(-p3)));
+        float wse = 1.f / (EPS + fabsf(DD(m1) - DD(-m1)) + fabsf(DD(m1) - DD(-p3)) + fabsf(DD(-m1) - DD(m3)));
+        D[i >> 1] = (wnw * (1.325f * DD(-m1) - 0.175f * DD(-m3) - 0.075f * (DD(-m1 - 2) + DD(-m1 - v2))) +
+                     wne * (1.325f * DD(p1) - 0.175f * DD(p3) - 0.075f * (DD(p1 + 2) + DD(p1 + v2))) +
+                     wsw * (1.325f * DD(-p1) - 0.175f * DD(-p3) - 0.075f * (DD(-p1 - 2) + DD(-p1 - v2))) +
+                     wse * (1.325f * DD(m1) - 0.175f * DD(m3) - 0.075f * (DD(m1 + 2) + DD(m1 + v2)))) /
+                    (wnw + wne + wsw + wse);
+#undef DD
+    });
+    __syncthreads();
+
+    // stage 10 (:1597-1625)
+    for_rect(16, rr1 - 16, 16, cc1 - 16, [&](int rr, int cc) {
+        int row = rr + top, col = cc + left;
+        if (row < 0 || row >= H || col < 0 || col >= W) return;
+        int t = rr * TS + cc;
+        float r, g, b;
+        if (fct(A, row, col) == 1) {
+            g = cfa[t];
+            float hu = hvwt[(t - v1) >> 1], hr = hvwt[(t + 1) >> 1], hl = hvwt[(t - 1) >> 1], hd = hvwt[(t + v1) >> 1];
+            float inv = 1.0f / (hu + 2.f - hr - hl + hd);
+            float rd = (hu * dg0[(t - v1) >> 1] + (1.f - hr) * dg0[(t + 1) >> 1] + (1.f - hl) * dg0[(t - 1) >> 1] + hd * dg0[(t + v1) >> 1]) * inv;
+            float bd = (hu * dg1[(t - v1) >> 1] + (1.f - hr) * dg1[(t + 1) >> 1] + (1.f - hl) * dg1[(t - 1) >> 1] + hd * dg1[(t + v1) >> 1]) * inv;
+            r = g - rd;
+            b = g - bd;
+        } else {
+            g = green[t];
+            r = g - dg0[t >> 1];
+            b = g - dg1[t >> 1];
+        }
+        float *o = A.rgb + ((size_t)row * W + col) * 3;
+        o[0] = (0.f < r) ? r : 0.f;
+        o[1] = (0.f < g) ? g : 0.f;
+        o[2] = (0.f < b) ? b : 0.f;
+    });
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(BLOCK) void amaze_kernel(Args A) {
+    float *ar = A.arenas + (size_t)blockIdx.x * ARENA_FLOATS;
+    for (int t = blockIdx.x; t < A.ntiles; t += gridDim.x) {
+        const int tr = t / A.ntc, tc = t % A.ntc;
+        amaze_tile(A, ar, -16 + tr * (TS - 32), -16 + tc * (TS - 32));
+    }
+}
+
+// LibRaw-style border interpolation (:2984-3028): frame pixels get the 3x3 mean
+// of each non-native colour.  Reads only native channels, writes only
+// non-native ones, so it runs in place.
+__global__ void border_kernel(float *img, int w, int h, uint32_t filters, int border) {
+    const int nframe_rows = min(border, h);
+    // enumerate frame pixels: full rows at top/bottom, then the left/right strips
+    const int top_n = nframe_rows * w;
+    const int bot_rows = max(0, min(border, h - border));
+    const int bot_n = bot_rows * w;
+    const int mid_rows = max(0, h - 2 * border);
+    const int strip = min(border, w);
+    const int side_n = mid_rows * 2 * strip;
+    const int total = top_n + bot_n + side_n;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+        int row, col;
+        if (t < top_n) { row = t / w; col = t % w; }
+        else if (t < top_n + bot_n) { int u = t - top_n; row = h - bot_rows + u / w; col = u % w; }
+        else {
+            int u = t - top_n - bot_n;
+            row = border + u / (2 * strip);
+            int s = u % (2 * strip);
+            col = s < strip ? s : w - strip + (s - strip);
+        }
+        if (col >= border && col < w - border && row >= border && row < h - border) continue;
+        float sum[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int y = row > 0 ? row - 1 : 0; y < row + 2 && y < h; y++)
+            for (int x = col > 0 ? col - 1 : 0; x < col + 2 && x < w; x++) {
+                unsigned f = lre::bayer3(filters, (unsigned)y, (unsigned)x);
+                sum[f] += img[((size_t)y * w + x) * 3 + f];
+                sum[f + 4] += 1.f;
+            }
+        unsigned f = lre::bayer3(filters, (unsigned)row, (unsigned)col);
+        for (unsigned c = 0; c < 3; c++)
+            if (c != f && sum[c + 4] > 0) img[((size_t)row * w + col) * 3 + c] = sum[c] / sum[c + 4];
+    }
+}
+
+int tiles_for(int n) {
+    int k = 0;
+    for (int o = -16; o < n; o += TS - 32) k++;
+    return k;
+}
+
+}  // namespace
+
+static int amaze_slots(lre_ctx *ctx, int w, int h) {
+    const int ntiles = tiles_for(w) * tiles_for(h);
+    const int cap = (ctx->num_cus > 0 ? ctx->num_cus : 256) * 8;
+    return ntiles < cap ? ntiles : cap;
+}
+
+size_t lre_amaze_scratch_bytes(lre_ctx *ctx, int w, int h) {
+    // one spare arena of padding behind the last slot
+    return (size_t)(amaze_slots(ctx, w, h) + 1) * ARENA_FLOATS * sizeof(float);
+}
+
+int lre_launch_amaze(lre_ctx *ctx, const float *d_cfa, float *d_rgb, int w, int h,
+                     uint32_t filters, const float cam_mul[4], float maximum_value,
+                     void *d_scratch, hipStream_t s) {
+    Args A;
+    A.cfa_plane = d_cfa;
+    A.rgb = d_rgb;
+    A.W = w;
+    A.H = h;
+    float mx = cam_mul[0] < cam_mul[2] ? cam_mul[2] : cam_mul[0];
+    float mn = cam_mul[2] < cam_mul[0] ? cam_mul[2] : cam_mul[0];
+    float gain = (mn > 1e-6f) ? (mx / mn) : 1.0f;
+    A.clip_pt = 1.0f / gain;
+    A.clip_pt8 = 0.8f / gain;
+    A.scale = 1.0f / maximum_value;
+    for (int r = 0; r < 2; r++)
+        for (int c = 0; c < 2; c++) A.cf[r][c] = lre::bayer3(filters, r, c);
+    if (A.cf[0][0] == 1) {
+        if (A.cf[0][1] == 0) { A.ey = 0; A.ex = 1; } else { A.ey = 1; A.ex = 0; }
+    } else {
+        if (A.cf[0][0] == 0) { A.ey = 0; A.ex = 0; } else { A.ey = 1; A.ex = 1; }
+    }
+    A.ntc = tiles_for(w);
+    A.ntiles = A.ntc * tiles_for(h);
+    A.arenas = static_cast<float *>(d_scratch);
+    const int slots = amaze_slots(ctx, w, h);
+    hipLaunchKernelGGL(amaze_kernel, dim3(slots), dim3(BLOCK), 0, s, A);
+    LRE_HIP_CHECK(hipGetLastError());
+    return lre_launch_border_interpolate(nullptr, d_rgb, w, h, filters, 4, s);
+}
+
+int lre_launch_border_interpolate(const float *, float *d_rgb, int w, int h, uint32_t filters,
+                                  int border, hipStream_t s) {
+    const int total = 2 * border * w + 2 * border * h;
+    const int blocks = (total + 255) / 256;
+    hipLaunchKernelGGL(border_kernel, dim3(blocks > 0 ? blocks : 1), dim3(256), 0, s, d_rgb, w, h,
+                       filters, border);
+    LRE_HIP_CHECK(hipGetLastError());
+    return LRE_OK;
+}

@@ -1,0 +1,266 @@
+// C ABI of the HIP develop path: context management, host-pointer entry points
+// (copy in, run, copy out) and device-resident entry points (enqueue only).
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "lre_common.h"
+#include "lre_internal.h"
+
+namespace {
+thread_local std::string g_last_error;
+
+bool valid_dims(int w, int h) { return w > 0 && h > 0 && (size_t)w * h < ((size_t)1 << 31); }
+
+int set_arg_error(const char *what) {
+    lre_set_error("invalid argument: %s", what);
+    return LRE_ERR_ARG;
+}
+
+int ensure_pinned(lre_ctx *ctx, size_t bytes) {
+    if (ctx->pinned_bytes >= bytes) return LRE_OK;
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    ctx->pinned = nullptr;
+    ctx->pinned_bytes = 0;
+    LRE_HIP_CHECK(hipHostMalloc(&ctx->pinned, bytes, hipHostMallocDefault));
+    ctx->pinned_bytes = bytes;
+    return LRE_OK;
+}
+
+// Device buffers for one host-pointer call (kept in the context's scratch).
+struct Carve {
+    char *base;
+    size_t off = 0;
+    explicit Carve(void *b) : base(static_cast<char *>(b)) {}
+    template <class T>
+    T *take(size_t count) {
+        T *p = reinterpret_cast<T *>(base + off);
+        off += (count * sizeof(T) + 255) & ~(size_t)255;
+        return p;
+    }
+};
+
+size_t aligned(size_t b) { return (b + 255) & ~(size_t)255; }
+
+int copy_in(lre_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    int rc = ensure_pinned(ctx, bytes);
+    if (rc) return rc;
+    std::memcpy(ctx->pinned, src, bytes);
+    LRE_HIP_CHECK(hipMemcpyAsync(dst, ctx->pinned, bytes, hipMemcpyHostToDevice, ctx->stream));
+    // the staging buffer is reused by the next copy
+    LRE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return LRE_OK;
+}
+
+int copy_out(lre_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    int rc = ensure_pinned(ctx, bytes);
+    if (rc) return rc;
+    LRE_HIP_CHECK(hipMemcpyAsync(ctx->pinned, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    LRE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    std::memcpy(dst, ctx->pinned, bytes);
+    return LRE_OK;
+}
+}  // namespace
+
+void lre_set_error(const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+}
+
+int lre_ensure_scratch(lre_ctx *ctx, size_t bytes) {
+    if (ctx->scratch_bytes >= bytes) return LRE_OK;
+    if (ctx->scratch) {
+        LRE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        (void)hipFree(ctx->scratch);
+    }
+    ctx->scratch = nullptr;
+    ctx->scratch_bytes = 0;
+    hipError_t e = hipMalloc(&ctx->scratch, bytes);
+    if (e != hipSuccess) {
+        lre_set_error("device scratch allocation of %zu bytes failed: %s", bytes,
+                      hipGetErrorString(e));
+        return LRE_ERR_NOMEM;
+    }
+    ctx->scratch_bytes = bytes;
+    return LRE_OK;
+}
+
+extern "C" {
+
+const char *lre_hip_last_error(void) { return g_last_error.c_str(); }
+
+int lre_hip_available(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n > 0 ? 1 : 0;
+}
+
+int lre_hip_create(lre_ctx **out, int device) {
+    if (!out) return set_arg_error("ctx");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        lre_set_error("no HIP device");
+        return LRE_ERR_NODEV;
+    }
+    if (device < 0 || device >= n) return set_arg_error("device index");
+    LRE_HIP_CHECK(hipSetDevice(device));
+    auto *ctx = new lre_ctx();
+    ctx->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
+    hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete ctx;
+        lre_set_error("hipStreamCreate failed: %s", hipGetErrorString(e));
+        return LRE_ERR_HIP;
+    }
+    *out = ctx;
+    return LRE_OK;
+}
+
+int lre_hip_destroy(lre_ctx *ctx) {
+    if (!ctx) return LRE_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->scratch) (void)hipFree(ctx->scratch);
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return LRE_OK;
+}
+
+int lre_hip_synchronize(lre_ctx *ctx) {
+    if (!ctx) return set_arg_error("ctx");
+    LRE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return LRE_OK;
+}
+
+void *lre_hip_stream(lre_ctx *ctx) { return ctx ? ctx->stream : nullptr; }
+
+// ---------------- device-resident entry points ----------------
+
+int lre_hip_dev_amaze(lre_ctx *ctx, const float *d_cfa, float *d_rgb, int w, int h,
+                      uint32_t filters, const float cam_mul[4], float maximum_value,
+                      void *stream) {
+    if (!ctx || !d_cfa || !d_rgb || !cam_mul) return set_arg_error("null pointer");
+    if (!valid_dims(w, h)) return set_arg_error("image size");
+    if (!(maximum_value > 0.f)) return set_arg_error("maximum_value must be > 0");
+    if (filters == 9) return set_arg_error("AMaZE needs a Bayer pattern");
+    int rc = lre_ensure_scratch(ctx, lre_amaze_scratch_bytes(ctx, w, h));
+    if (rc) return rc;
+    return lre_launch_amaze(ctx, d_cfa, d_rgb, w, h, filters, cam_mul, maximum_value,
+                            ctx->scratch, lre_pick_stream(ctx, stream));
+}
+
+int lre_hip_dev_tone_mapping(lre_ctx *ctx, const float *d_in, float *d_out, int w, int h,
+                             float after_scale, void *stream) {
+    if (!ctx || !d_in || !d_out) return set_arg_error("null pointer");
+    if (!valid_dims(w, h)) return set_arg_error("image size");
+    return lre_launch_tone_mapping(d_in, d_out, (size_t)w * h * 3, after_scale,
+                                   lre_pick_stream(ctx, stream));
+}
+
+int lre_hip_dev_detail_tonemap(lre_ctx *ctx, const float *d_in, float *d_out, int w, int h,
+                               void *stream) {
+    if (!ctx || !d_in || !d_out) return set_arg_error("null pointer");
+    if (!valid_dims(w, h)) return set_arg_error("image size");
+    int rc = lre_ensure_scratch(ctx, lre_detail_tm_scratch_bytes(w, h));
+    if (rc) return rc;
+    return lre_launch_detail_tonemap(d_in, d_out, w, h, ctx->scratch, lre_pick_stream(ctx, stream));
+}
+
+int lre_hip_dev_rgb3_to_cfa(lre_ctx *ctx, const float *d_raw_rgb, float *d_cfa, int w, int h,
+                            uint32_t filters, const signed char xtrans[36], void *stream) {
+    if (!ctx || !d_raw_rgb || !d_cfa) return set_arg_error("null pointer");
+    if (!valid_dims(w, h)) return set_arg_error("image size");
+    if (filters == 9 && !xtrans) return set_arg_error("X-Trans table");
+    return lre_launch_rgb3_to_cfa(d_raw_rgb, d_cfa, w, h, filters, xtrans,
+                                  lre_pick_stream(ctx, stream));
+}
+
+// ---------------- host-pointer entry points ----------------
+// Device buffers live at the front of the context scratch; the stage's own
+// scratch follows them.
+
+int lre_hip_amaze(lre_ctx *ctx, const float *raw_rgb, float *rgb, int w, int h,
+                  uint32_t filters, const float cam_mul[4], float maximum_value) {
+    if (!ctx || !raw_rgb || !rgb || !cam_mul) return set_arg_error("null pointer");
+    if (!valid_dims(w, h)) return set_arg_error("image size");
+    if (!(maximum_value > 0.f)) return set_arg_error("maximum_value must be > 0");
+    if (filters == 9) return set_arg_error("AMaZE needs a Bayer pattern");
+    LRE_HIP_CHECK(hipSetDevice(ctx->device));
+    const size_t n = (size_t)w * h;
+    const size_t io = aligned(n * 3 * 4) * 2 + aligned(n * 4);
+    int rc = lre_ensure_scratch(ctx, io + lre_amaze_scratch_bytes(ctx, w, h));
+    if (rc) return rc;
+    Carve c(ctx->scratch);
+    float *d_in = c.take<float>(n * 3), *d_out = c.take<float>(n * 3), *d_cfa = c.take<float>(n);
+    void *d_ws = static_cast<char *>(ctx->scratch) + c.off;
+    if ((rc = copy_in(ctx, d_in, raw_rgb, n * 12))) return rc;
+    if ((rc = lre_launch_rgb3_to_cfa(d_in, d_cfa, w, h, filters, nullptr, ctx->stream))) return rc;
+    // the reference writes every output pixel; start from a copy of the input so
+    // that nothing is left undefined
+    LRE_HIP_CHECK(hipMemcpyAsync(d_out, d_in, n * 12, hipMemcpyDeviceToDevice, ctx->stream));
+    if ((rc = lre_launch_amaze(ctx, d_cfa, d_out, w, h, filters, cam_mul, maximum_value, d_ws,
+                               ctx->stream)))
+        return rc;
+    return copy_out(ctx, rgb, d_out, n * 12);
+}
+
+int lre_hip_tone_mapping(lre_ctx *ctx, const float *in_rgb, float *out_rgb, int w, int h,
+                         float after_scale) {
+    if (!ctx || !in_rgb || !out_rgb) return set_arg_error("null pointer");
+    if (!valid_dims(w, h)) return set_arg_error("image size");
+    LRE_HIP_CHECK(hipSetDevice(ctx->device));
+    const size_t n = (size_t)w * h * 3;
+    int rc = lre_ensure_scratch(ctx, aligned(n * 4));
+    if (rc) return rc;
+    float *d = static_cast<float *>(ctx->scratch);
+    if ((rc = copy_in(ctx, d, in_rgb, n * 4))) return rc;
+    if ((rc = lre_launch_tone_mapping(d, d, n, after_scale, ctx->stream))) return rc;
+    return copy_out(ctx, out_rgb, d, n * 4);
+}
+
+int lre_hip_detail_tonemap(lre_ctx *ctx, const float *in_rgb, float *out_rgb, int w, int h) {
+    if (!ctx || !in_rgb || !out_rgb) return set_arg_error("null pointer");
+    if (!valid_dims(w, h)) return set_arg_error("image size");
+    LRE_HIP_CHECK(hipSetDevice(ctx->device));
+    const size_t n = (size_t)w * h * 3;
+    int rc = lre_ensure_scratch(ctx, aligned(n * 4) + lre_detail_tm_scratch_bytes(w, h));
+    if (rc) return rc;
+    float *d = static_cast<float *>(ctx->scratch);
+    void *ws = static_cast<char *>(ctx->scratch) + aligned(n * 4);
+    if ((rc = copy_in(ctx, d, in_rgb, n * 4))) return rc;
+    if ((rc = lre_launch_detail_tonemap(d, d, w, h, ws, ctx->stream))) return rc;
+    return copy_out(ctx, out_rgb, d, n * 4);
+}
+
+int lre_hip_white_balance(lre_ctx *ctx, const uint16_t *img4, float *rgb, int w, int h,
+                          const float wb[4], uint32_t filters, const signed char xtrans[36]) {
+    if (!ctx || !img4 || !rgb || !wb) return set_arg_error("null pointer");
+    if (!valid_dims(w, h)) return set_arg_error("image size");
+    if (filters == 9 && !xtrans) return set_arg_error("X-Trans table");
+    LRE_HIP_CHECK(hipSetDevice(ctx->device));
+    const size_t n = (size_t)w * h;
+    int rc = lre_ensure_scratch(ctx, aligned(n * 8) + aligned(n * 12));
+    if (rc) return rc;
+    Carve c(ctx->scratch);
+    uint16_t *d_in = c.take<uint16_t>(n * 4);
+    float *d_out = c.take<float>(n * 3);
+    if ((rc = copy_in(ctx, d_in, img4, n * 8))) return rc;
+    // channels other than the native one keep the caller's values
+    if ((rc = copy_in(ctx, d_out, rgb, n * 12))) return rc;
+    if ((rc = lre_launch_white_balance(d_in, d_out, w, h, wb, filters, xtrans, ctx->stream)))
+        return rc;
+    return copy_out(ctx, rgb, d_out, n * 12);
+}
+
+}  // extern "C"

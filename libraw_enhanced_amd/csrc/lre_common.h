@@ -1,0 +1,56 @@
+// Shared device/host helpers of the HIP develop path (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+
+#define LRE_DEV __device__ __forceinline__
+
+namespace lre {
+
+// Colour of a Bayer site from LibRaw's 32-bit `filters` word (0 R, 1 G, 2 B; the
+// second green, 3, folds to 1).  Same bit formula as the reference's
+// fcol_bayer_native / fcol_bayer (core/metal/shader_common.h:25-31).
+LRE_DEV __host__ unsigned bayer_native(uint32_t filters, unsigned row, unsigned col) {
+    return filters >> ((((row << 1) & 14) | (col & 1)) << 1) & 3;
+}
+LRE_DEV __host__ unsigned bayer3(uint32_t filters, unsigned row, unsigned col) {
+    unsigned n = bayer_native(filters, row, col);
+    return n == 3 ? 1 : n;
+}
+
+LRE_DEV float sq(float x) { return x * x; }
+// std::min / std::max operand semantics (NaN handling follows the comparison).
+LRE_DEV float min_std(float a, float b) { return (b < a) ? b : a; }
+LRE_DEV float max_std(float a, float b) { return (a < b) ? b : a; }
+LRE_DEV float clamp_std(float v, float lo, float hi) { return v < lo ? lo : (hi < v ? hi : v); }
+// Median of three by the reference's three compare-swaps (shader_common.h:55-61).
+LRE_DEV float med3(float a, float b, float c) {
+    float t;
+    if (a > b) { t = a; a = b; b = t; }
+    if (b > c) { t = b; b = c; c = t; }
+    if (a > b) { t = a; a = b; b = t; }
+    return b;
+}
+
+LRE_DEV float aces(float x) {
+    const float a = 2.51f, b = 0.03f, c = 2.43f, d = 0.59f, e = 0.14f;
+    return (x * (a * x + b)) / (x * (c * x + d) + e);
+}
+
+}  // namespace lre
+
+// Error plumbing of the C-ABI: every HIP call is checked, the message is kept
+// per thread and the entry point returns a negative status.
+void lre_set_error(const char *fmt, ...);
+
+#define LRE_HIP_CHECK(expr)                                                                   \
+    do {                                                                                      \
+        hipError_t _e = (expr);                                                               \
+        if (_e != hipSuccess) {                                                               \
+            lre_set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), __FILE__, \
+                          __LINE__);                                                          \
+            return LRE_ERR_HIP;                                                               \
+        }                                                                                     \
+    } while (0)

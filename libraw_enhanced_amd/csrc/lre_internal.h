@@ -1,0 +1,49 @@
+// Internal host-side declarations shared by the C-ABI and the kernel files.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <mutex>
+
+#include "lre_hip.h"
+
+struct lre_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // Grow-only device scratch, carved by each stage.  A context is used by one
+    // host thread at a time (the same contract as the reference's Accelerator).
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
+    // Pinned staging for the host-pointer entry points.
+    void *pinned = nullptr;
+    size_t pinned_bytes = 0;
+    int num_cus = 0;
+};
+
+// Scratch helpers (stream-ordered: the caller must not reuse scratch across
+// streams without synchronising).
+int lre_ensure_scratch(lre_ctx *ctx, size_t bytes);
+inline hipStream_t lre_pick_stream(lre_ctx *ctx, void *stream) {
+    return stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+}
+
+// Bytes of device scratch each stage needs for an image of w x h.
+size_t lre_amaze_scratch_bytes(lre_ctx *ctx, int w, int h);
+size_t lre_detail_tm_scratch_bytes(int w, int h);
+
+// Kernel launchers (device pointers; scratch must already be large enough).
+int lre_launch_amaze(lre_ctx *ctx, const float *d_cfa, float *d_rgb, int w, int h,
+                     uint32_t filters, const float cam_mul[4], float maximum_value,
+                     void *d_scratch, hipStream_t s);
+int lre_launch_border_interpolate(const float *dummy, float *d_rgb, int w, int h,
+                                  uint32_t filters, int border, hipStream_t s);
+int lre_launch_tone_mapping(const float *d_in, float *d_out, size_t n, float after_scale,
+                            hipStream_t s);
+int lre_launch_detail_tonemap(const float *d_in, float *d_out, int w, int h, void *d_scratch,
+                              hipStream_t s);
+int lre_launch_white_balance(const uint16_t *d_img4, float *d_rgb, int w, int h,
+                             const float wb[4], uint32_t filters, const signed char *xtrans,
+                             hipStream_t s);
+int lre_launch_rgb3_to_cfa(const float *d_rgb3, float *d_cfa, int w, int h, uint32_t filters,
+                           const signed char *xtrans, hipStream_t s);

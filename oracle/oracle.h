@@ -1,0 +1,38 @@
+/*
+ * TEST INFRASTRUCTURE ONLY — CPU restatement ("oracle") of the reference's
+ * develop path.  Only tests/, the smoke check and bench.py's CPU-baseline leg
+ * may load liblre_oracle.so, and only as the checker or as a timed baseline;
+ * the product never calls it.
+ *
+ * Every function names the reference file:line it restates.  Images are
+ * caller-owned, row-major, interleaved float RGB (H x W x 3) unless stated.
+ */
+#ifndef LRE_ORACLE_H
+#define LRE_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Scratch semantics of the tiled demosaics.
+ *  ORACLE_TILES_CLEAN:      every tile starts from zeroed scratch (deterministic,
+ *                           thread-count independent; the contract of the HIP path).
+ *  ORACLE_TILES_SEQUENTIAL: one scratch buffer, zeroed once, reused by the tiles
+ *                           in raster order — what the reference does with one
+ *                           OpenMP thread (its per-thread buffer is reused and
+ *                           some reads see the previous tile's values). */
+enum { ORACLE_TILES_CLEAN = 0, ORACLE_TILES_SEQUENTIAL = 1 };
+
+int oracle_amaze(const float *raw3, float *rgb3, int w, int h, uint32_t filters,
+                 const float *cam_mul4, float maximum_value, int tile_mode);
+void oracle_border_interpolate(float *rgb3, int w, int h, uint32_t filters, int border);
+int oracle_tone_mapping(const float *in3, float *out3, int w, int h, float after_scale);
+int oracle_detail_tonemap(float *rgb3, int w, int h);
+int oracle_white_balance(const uint16_t *img4, float *rgb3, int w, int h, const float *wb4,
+                         uint32_t filters, const signed char *xtrans36);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
