@@ -50,6 +50,12 @@ int lre_hip_synchronize(lre_ctx *ctx);
 /* The context's HIP stream (hipStream_t), for callers that enqueue their own work. */
 void *lre_hip_stream(lre_ctx *ctx);
 
+/* Device memory plumbing for callers without their own allocator. */
+int lre_hip_dev_malloc(lre_ctx *ctx, void **d_ptr, size_t bytes);
+int lre_hip_dev_free(lre_ctx *ctx, void *d_ptr);
+int lre_hip_memcpy_htod(lre_ctx *ctx, void *d_dst, const void *h_src, size_t bytes);
+int lre_hip_memcpy_dtoh(lre_ctx *ctx, void *h_dst, const void *d_src, size_t bytes);
+
 /* ---- host-pointer entry points (drop-in for the Accelerator methods) ---- */
 
 /* Accelerator::demosaic_bayer_amaze (core/accelerator.h:279-281,

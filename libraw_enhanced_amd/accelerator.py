@@ -42,6 +42,10 @@ _SIGS = {
     "lre_hip_last_error": ([], ctypes.c_char_p),
     "lre_hip_synchronize": ([_vp], ctypes.c_int),
     "lre_hip_stream": ([_vp], _vp),
+    "lre_hip_dev_malloc": ([_vp, ctypes.POINTER(_vp), ctypes.c_size_t], ctypes.c_int),
+    "lre_hip_dev_free": ([_vp, _vp], ctypes.c_int),
+    "lre_hip_memcpy_htod": ([_vp, _vp, _vp, ctypes.c_size_t], ctypes.c_int),
+    "lre_hip_memcpy_dtoh": ([_vp, _vp, _vp, ctypes.c_size_t], ctypes.c_int),
     "lre_hip_amaze": ([_vp, _f32p, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, _f32p,
                        ctypes.c_float], ctypes.c_int),
     "lre_hip_tone_mapping": ([_vp, _f32p, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_float],
@@ -126,6 +130,16 @@ class HIPAccelerator:
     def synchronize(self):
         _check(native().lre_hip_synchronize(self._h))
 
+    # -- device buffers --
+    def dev_alloc(self, nbytes: int) -> "DeviceBuffer":
+        return DeviceBuffer(self, nbytes)
+
+    def dev_from_host(self, arr: np.ndarray) -> "DeviceBuffer":
+        arr = np.ascontiguousarray(arr)
+        buf = DeviceBuffer(self, arr.nbytes)
+        buf.upload(arr)
+        return buf
+
     # -- Accelerator methods (host arrays in, new host arrays out) --
     def demosaic_bayer_amaze(self, raw_rgb, filters: int = RGGB, cam_mul=(1.0, 1.0, 1.0, 1.0),
                              maximum_value: float = 1.0):
@@ -162,6 +176,41 @@ class HIPAccelerator:
         _check(native().lre_hip_white_balance(self._h, _p(img4, _u16p), _p(out), w, h, _p(wbv),
                                               filters, None if xt is None else _p(xt, _i8p)))
         return out
+
+
+class DeviceBuffer:
+    """A device allocation owned by a HIPAccelerator context."""
+
+    def __init__(self, acc: HIPAccelerator, nbytes: int):
+        self._acc = acc
+        self.nbytes = int(nbytes)
+        p = _vp()
+        _check(native().lre_hip_dev_malloc(acc.handle, ctypes.byref(p), self.nbytes))
+        self.ptr = p
+
+    def upload(self, arr: np.ndarray):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        _check(native().lre_hip_memcpy_htod(self._acc.handle, self.ptr,
+                                            arr.ctypes.data_as(_vp), arr.nbytes))
+
+    def download(self, shape, dtype=np.float32) -> np.ndarray:
+        out = np.empty(shape, dtype)
+        assert out.nbytes <= self.nbytes
+        _check(native().lre_hip_memcpy_dtoh(self._acc.handle, out.ctypes.data_as(_vp), self.ptr,
+                                            out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            native().lre_hip_dev_free(self._acc.handle, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class LibRawWrapper:

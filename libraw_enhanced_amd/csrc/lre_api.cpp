@@ -77,7 +77,8 @@ void lre_set_error(const char *fmt, ...) {
 int lre_ensure_scratch(lre_ctx *ctx, size_t bytes) {
     if (ctx->scratch_bytes >= bytes) return LRE_OK;
     if (ctx->scratch) {
-        LRE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        // work on any stream may still use the old scratch
+        LRE_HIP_CHECK(hipDeviceSynchronize());
         (void)hipFree(ctx->scratch);
     }
     ctx->scratch = nullptr;
@@ -144,6 +145,39 @@ int lre_hip_synchronize(lre_ctx *ctx) {
 }
 
 void *lre_hip_stream(lre_ctx *ctx) { return ctx ? ctx->stream : nullptr; }
+
+int lre_hip_dev_malloc(lre_ctx *ctx, void **d_ptr, size_t bytes) {
+    if (!ctx || !d_ptr) return set_arg_error("null pointer");
+    LRE_HIP_CHECK(hipSetDevice(ctx->device));
+    hipError_t e = hipMalloc(d_ptr, bytes ? bytes : 1);
+    if (e != hipSuccess) {
+        lre_set_error("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+        return LRE_ERR_NOMEM;
+    }
+    return LRE_OK;
+}
+
+int lre_hip_dev_free(lre_ctx *ctx, void *d_ptr) {
+    if (!ctx) return set_arg_error("ctx");
+    if (!d_ptr) return LRE_OK;
+    LRE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    LRE_HIP_CHECK(hipFree(d_ptr));
+    return LRE_OK;
+}
+
+int lre_hip_memcpy_htod(lre_ctx *ctx, void *d_dst, const void *h_src, size_t bytes) {
+    if (!ctx || !d_dst || !h_src) return set_arg_error("null pointer");
+    LRE_HIP_CHECK(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    LRE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return LRE_OK;
+}
+
+int lre_hip_memcpy_dtoh(lre_ctx *ctx, void *h_dst, const void *d_src, size_t bytes) {
+    if (!ctx || !h_dst || !d_src) return set_arg_error("null pointer");
+    LRE_HIP_CHECK(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    LRE_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return LRE_OK;
+}
 
 // ---------------- device-resident entry points ----------------
 

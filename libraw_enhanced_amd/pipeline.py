@@ -1,0 +1,75 @@
+"""Device-resident develop chain on one image: the CFA plane and every
+intermediate stay in HBM, stages are enqueued back to back on the context's
+stream with no host synchronisation between them.
+
+Stage names follow the reference's Accelerator methods; the default chain is
+the 24 MP Bayer develop: AMaZE demosaic -> defringe -> detail tone map.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from .accelerator import RGGB, HIPAccelerator, _check, native
+
+_vp = ctypes.c_void_p
+_f32p = ctypes.POINTER(ctypes.c_float)
+
+
+class DevelopChain:
+    """Enqueue a fixed stage list over one resident image."""
+
+    def __init__(self, acc: HIPAccelerator, height: int, width: int, stages=None,
+                 filters: int = RGGB, cam_mul=(1.0, 1.0, 1.0, 1.0), maximum_value: float = 1.0,
+                 defringe_params=None):
+        self.acc = acc
+        self.h, self.w = height, width
+        self.filters = filters
+        self.cam_mul = np.ascontiguousarray(cam_mul, np.float32)
+        self.maximum_value = float(maximum_value)
+        self.defringe_params = dict(radius=10.0, edge_threshold=0.1, chroma_threshold=0.15,
+                                    strength=10.0, defringe_green=False, green_strength_scale=0.3)
+        if defringe_params:
+            self.defringe_params.update(defringe_params)
+        n = height * width
+        self.rgb = acc.dev_alloc(n * 12)
+        self.rgb2 = acc.dev_alloc(n * 12)
+        self.stages = list(stages or available_chain())
+
+    def run_stage(self, name: str, d_cfa, stream=None):
+        lib, ctx, h, w = native(), self.acc.handle, self.h, self.w
+        if name == "amaze":
+            _check(lib.lre_hip_dev_amaze(ctx, d_cfa.ptr, self.rgb.ptr, w, h, self.filters,
+                                         self.cam_mul.ctypes.data_as(_f32p), self.maximum_value,
+                                         stream))
+        elif name == "defringe":
+            p = self.defringe_params
+            _check(lib.lre_hip_dev_defringe(ctx, self.rgb.ptr, self.rgb2.ptr, w, h, p["radius"],
+                                            p["edge_threshold"], p["chroma_threshold"],
+                                            p["strength"], int(p["defringe_green"]),
+                                            p["green_strength_scale"], stream))
+            self.rgb, self.rgb2 = self.rgb2, self.rgb
+        elif name == "detail_tm":
+            _check(lib.lre_hip_dev_detail_tonemap(ctx, self.rgb.ptr, self.rgb.ptr, w, h, stream))
+        elif name == "tone_mapping":
+            _check(lib.lre_hip_dev_tone_mapping(ctx, self.rgb.ptr, self.rgb.ptr, w, h, 1.0, stream))
+        else:
+            raise ValueError(f"unknown stage {name}")
+
+    def run(self, d_cfa, stream=None):
+        for s in self.stages:
+            self.run_stage(s, d_cfa, stream)
+
+    def result(self) -> np.ndarray:
+        return self.rgb.download((self.h, self.w, 3), np.float32)
+
+
+def available_chain():
+    """The 24 MP Bayer chain, limited to the stages the loaded library exports."""
+    lib = native()
+    chain = ["amaze"]
+    if hasattr(lib, "lre_hip_dev_defringe"):
+        chain.append("defringe")
+    chain.append("detail_tm")
+    return chain
