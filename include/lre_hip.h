@@ -77,6 +77,16 @@ int lre_hip_tone_mapping(lre_ctx *ctx, const float *in_rgb, float *out_rgb, int 
 int lre_hip_detail_tonemap(lre_ctx *ctx, const float *in_rgb, float *out_rgb, int width,
                            int height);
 
+/* Accelerator::defringe (core/accelerator.h:344-351, core/cpu_accelerator.cpp:3939-5064),
+ * defaults radius 10, edge_threshold 0.1, chroma_threshold 0.15, strength 10,
+ * defringe_green 0, green_strength_scale 0.3.  in_rgb == out_rgb selects the
+ * reference's in-place semantics (the solo cleanup then sees the pixel's
+ * corrected values, core/libraw_wrapper.cpp:1810); distinct buffers select the
+ * standalone semantics (core/libraw_wrapper.cpp:3211). */
+int lre_hip_defringe(lre_ctx *ctx, const float *in_rgb, float *out_rgb, int width, int height,
+                     float radius, float edge_threshold, float chroma_threshold, float strength,
+                     int defringe_green, float green_strength_scale);
+
 /* Accelerator::apply_white_balance (core/accelerator.h:304-307,
  * core/cpu_accelerator.cpp:2789-2869): writes float(sample) * wb[c] to the
  * native channel of rgb and leaves the other two channels untouched.
@@ -95,6 +105,11 @@ int lre_hip_dev_tone_mapping(lre_ctx *ctx, const float *d_in, float *d_out, int 
                              int height, float after_scale, void *stream);
 int lre_hip_dev_detail_tonemap(lre_ctx *ctx, const float *d_in, float *d_out, int width,
                                int height, void *stream);
+/* Defringe on resident buffers; d_in == d_out selects the in-place semantics. */
+int lre_hip_dev_defringe(lre_ctx *ctx, const float *d_in, float *d_out, int width, int height,
+                         float radius, float edge_threshold, float chroma_threshold,
+                         float strength, int defringe_green, float green_strength_scale,
+                         void *stream);
 /* Demosaic input planes: extract the native channel of the reference's 3-channel
  * layout into a CFA plane. */
 int lre_hip_dev_rgb3_to_cfa(lre_ctx *ctx, const float *d_raw_rgb, float *d_cfa, int width,

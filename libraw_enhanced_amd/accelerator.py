@@ -51,6 +51,12 @@ _SIGS = {
     "lre_hip_tone_mapping": ([_vp, _f32p, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_float],
                              ctypes.c_int),
     "lre_hip_detail_tonemap": ([_vp, _f32p, _f32p, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "lre_hip_defringe": ([_vp, _f32p, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                          ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int,
+                          ctypes.c_float], ctypes.c_int),
+    "lre_hip_dev_defringe": ([_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                              ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int,
+                              ctypes.c_float, _vp], ctypes.c_int),
     "lre_hip_white_balance": ([_vp, _u16p, _f32p, ctypes.c_int, ctypes.c_int, _f32p,
                                ctypes.c_uint32, _i8p], ctypes.c_int),
     "lre_hip_dev_amaze": ([_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, _f32p,
@@ -165,6 +171,22 @@ class HIPAccelerator:
         _check(native().lre_hip_detail_tonemap(self._h, _p(src), _p(out), w, h))
         return out
 
+    def defringe(self, rgb_input, rgb_output=None, radius: float = 10.0,
+                 edge_threshold: float = 0.1, chroma_threshold: float = 0.15,
+                 strength: float = 10.0, defringe_green: bool = False,
+                 green_strength_scale: float = 0.3):
+        """Accelerator::defringe.  Passing the same array as rgb_input and
+        rgb_output (or rgb_output=None with inplace semantics requested by
+        passing rgb_output=rgb_input) runs the reference's in-place form."""
+        src = _rgb(rgb_input)
+        h, w, _ = src.shape
+        inplace = rgb_output is not None and rgb_output is rgb_input
+        out = src if inplace else np.empty_like(src)
+        _check(native().lre_hip_defringe(self._h, _p(src), _p(out), w, h, radius, edge_threshold,
+                                         chroma_threshold, strength, int(defringe_green),
+                                         green_strength_scale))
+        return out
+
     def apply_white_balance(self, img4, wb, filters: int = RGGB, xtrans=None, rgb=None):
         img4 = np.ascontiguousarray(img4, np.uint16)
         if img4.ndim != 3 or img4.shape[2] != 4:
@@ -223,3 +245,10 @@ class LibRawWrapper:
     def tone_mapping(self, image, after_scale: float = 1.0):
         # core/libraw_wrapper.cpp:3098-3131
         return self._acc.tone_mapping(_rgb(image), after_scale)
+
+    def defringe(self, image, radius: float = 10.0, strength: float = 10.0,
+                 defringe_green: bool = False):
+        # core/libraw_wrapper.cpp:3176-3214: copies the input and runs with
+        # distinct buffers, edge 0.1, chroma 0.15, green scale 0.3
+        return self._acc.defringe(_rgb(image), None, radius, 0.1, 0.15, strength,
+                                  defringe_green, 0.3)

@@ -1,0 +1,607 @@
+// Edge-gated Gaussian defringe for gfx950 (core/cpu_accelerator.cpp:3939-5064).
+//
+// Pipeline (each box is one kernel; planes are float, H x W, in the context scratch):
+//   max(G) ─► guide / guide_signal / log R/G / log B/G ─► Sobel |∇| + max ─►
+//   5 Gaussian blurs (horizontal + vertical pass) ─► support seed ─► blur ─►
+//   8-direction line link ─► solo seeds ─► 2 blurs ─► per-pixel correction.
+// The two global maxima stay on the device (atomicMax on the bit pattern of a
+// non-negative float), so the whole stage is enqueued without a host sync.
+//
+// Blurs sum the taps in the reference's order (k = -ks..ks, product then add,
+// no contraction), so blurred planes are bit-identical to the reference's.  The
+// logarithms and exponentials are evaluated in double and rounded to float
+// (correctly rounded); the reference's glibc logf/expf differ from that in the
+// last bit on ~0.1 % of arguments, which moves the result by ~1e-7.
+#include "lre_common.h"
+#include "lre_internal.h"
+
+namespace {
+
+constexpr int BLOCK = 256;
+constexpr int MAX_TAPS = 129;  // radius <= 64 px
+
+struct Scal {
+    int W, H, ks, inplace, enable_green;
+    float edge_thr, chroma_thr, strength, green_scale;
+    float isens, mask_cutoff, solo_enable, threshold_soften;
+};
+
+LRE_DEV float sstep(float e0, float e1, float x) {  // :3322-3325
+    float t = lre::clamp_std((x - e0) / (e1 - e0), 0.0f, 1.0f);
+    return t * t * (3.0f - 2.0f * t);
+}
+LRE_DEV float mx(float a, float b) { return lre::max_std(a, b); }
+LRE_DEV float mn(float a, float b) { return lre::min_std(a, b); }
+LRE_DEV float expf_cr(float x) { return (float)exp((double)x); }
+LRE_DEV float logf_cr(float x) { return (float)log((double)x); }
+
+// block-wide max of non-negative floats folded into *dst
+LRE_DEV void block_max_to(float v, unsigned *dst) {
+    __shared__ float red[BLOCK / 64];
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float m = red[0];
+        for (int k = 1; k < BLOCK / 64; k++) m = fmaxf(m, red[k]);
+        atomicMax(dst, __float_as_uint(m));
+    }
+}
+
+__global__ void init_stats(unsigned *stats) {
+    if (threadIdx.x < 4) stats[threadIdx.x] = 0u;
+}
+
+// max over pixels of max(0, G)  (:3961-3969)
+__global__ __launch_bounds__(BLOCK) void max_green(const float *__restrict__ src, size_t n,
+                                                   unsigned *stats) {
+    float m = 0.f;
+    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK) {
+        float g = mx(0.f, src[3 * i + 1]);
+        if (g > m) m = g;
+    }
+    block_max_to(m, &stats[0]);
+}
+
+LRE_DEV float max_guide_of(const unsigned *stats) { return mx(__uint_as_float(stats[0]), 1e-6f); }
+
+// guide planes and log chroma ratios (:3970-3991); out of place also seeds dst
+__global__ __launch_bounds__(BLOCK) void prep(const float *src, float *__restrict__ gl,
+                                              float *__restrict__ gs, float *__restrict__ lrg,
+                                              float *__restrict__ lbg, size_t n,
+                                              const unsigned *stats) {
+    const float max_guide = max_guide_of(stats);
+    const float ratio_eps = mx(max_guide * 1e-4f, 1e-6f);
+    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK) {
+        const float r = src[3 * i], g = src[3 * i + 1], b = src[3 * i + 2];
+        const float gg = mx(g, 0.f);
+        gl[i] = gg;
+        gs[i] = sqrtf(lre::clamp_std(gg / max_guide, 0.f, 1.f));
+        lrg[i] = logf_cr((mx(r, 0.f) + ratio_eps) / (gg + ratio_eps));
+        lbg[i] = logf_cr((mx(b, 0.f) + ratio_eps) / (gg + ratio_eps));
+    }
+}
+
+// 3x3 Sobel magnitude of guide_signal, 0 on the frame (:3996-4027)
+__global__ __launch_bounds__(BLOCK) void sobel(const float *__restrict__ s, float *__restrict__ er,
+                                               int W, int H, unsigned *stats) {
+    float m = 0.f;
+    const size_t n = (size_t)W * H;
+    const int ly = H > 1 ? H - 1 : 1, lx = W > 1 ? W - 1 : 1;
+    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK) {
+        const int y = (int)(i / W), x = (int)(i % W);
+        float mag = 0.f;
+        if (y >= 1 && y < ly && x >= 1 && x < lx) {
+            const size_t u = (size_t)(y - 1) * W, c = (size_t)y * W, d = (size_t)(y + 1) * W;
+            float gx = -s[u + x - 1] + s[u + x + 1] - 2.f * s[c + x - 1] + 2.f * s[c + x + 1] -
+                       s[d + x - 1] + s[d + x + 1];
+            float gy = -s[u + x - 1] - 2.f * s[u + x] - s[u + x + 1] + s[d + x - 1] +
+                       2.f * s[d + x] + s[d + x + 1];
+            mag = sqrtf(gx * gx + gy * gy);
+            if (mag > m) m = mag;
+        }
+        er[i] = mag;
+    }
+    block_max_to(m, &stats[1]);
+}
+
+// ---- separable Gaussian, clamp-to-edge, taps in the reference's order ----
+struct BlurSet {
+    const float *in[5];
+    float *tmp[5];
+    float *out[5];
+    int scale_by_edge[5];  // multiply the input by 1/max(edge) (the normalised edge map)
+    int count;
+};
+
+constexpr int HB_W = 256;  // outputs per block along x
+
+__global__ __launch_bounds__(BLOCK) void blur_h(BlurSet S, const float *__restrict__ taps, int W,
+                                                int H, int ks, const unsigned *stats) {
+    __shared__ float seg[HB_W + 2 * (MAX_TAPS / 2)];
+    __shared__ float kt[MAX_TAPS];
+    const int y = blockIdx.y, x0 = blockIdx.x * HB_W;
+    for (int t = threadIdx.x; t < 2 * ks + 1; t += BLOCK) kt[t] = taps[t];
+    const float inv_edge = 1.f / mx(__uint_as_float(stats[1]), 1e-6f);
+    for (int p = 0; p < S.count; p++) {
+        const float *in = S.in[p] + (size_t)y * W;
+        const float sc = S.scale_by_edge[p] ? inv_edge : 1.f;
+        __syncthreads();
+        for (int t = threadIdx.x; t < HB_W + 2 * ks; t += BLOCK) {
+            int xx = x0 - ks + t;
+            xx = xx < 0 ? 0 : (xx > W - 1 ? W - 1 : xx);
+            seg[t] = S.scale_by_edge[p] ? in[xx] * sc : in[xx];
+        }
+        __syncthreads();
+        const int x = x0 + threadIdx.x;
+        if (x < W) {
+            float acc = 0.f;
+            for (int k = 0; k <= 2 * ks; k++) acc += kt[k] * seg[threadIdx.x + k];
+            S.tmp[p][(size_t)y * W + x] = acc;
+        }
+    }
+}
+
+constexpr int VB_W = 64, VB_H = 32;
+
+__global__ __launch_bounds__(BLOCK) void blur_v(BlurSet S, const float *__restrict__ taps, int W,
+                                                int H, int ks) {
+    __shared__ float kt[MAX_TAPS];
+    for (int t = threadIdx.x; t < 2 * ks + 1; t += BLOCK) kt[t] = taps[t];
+    __syncthreads();
+    const int x = blockIdx.x * VB_W + (threadIdx.x % VB_W);
+    const int y0 = blockIdx.y * VB_H;
+    if (x >= W) return;
+    for (int p = 0; p < S.count; p++) {
+        const float *tm = S.tmp[p];
+        for (int r = threadIdx.x / VB_W; r < VB_H; r += BLOCK / VB_W) {
+            const int y = y0 + r;
+            if (y >= H) break;
+            float acc = 0.f;
+            for (int k = -ks; k <= ks; k++) {
+                int yy = y + k;
+                yy = yy < 0 ? 0 : (yy > H - 1 ? H - 1 : yy);
+                acc += kt[k + ks] * tm[(size_t)yy * W + x];
+            }
+            S.out[p][(size_t)y * W + x] = acc;
+        }
+    }
+}
+
+struct Planes {
+    const float *src;
+    float *dst;
+    float *lrg, *lbg, *bg, *brg, *bbg, *be, *ber, *fs, *bfs, *lfs, *sr, *sb, *srb, *sbb;
+};
+
+// fringe support seed (:4162-4248)
+__global__ __launch_bounds__(BLOCK) void support(Planes P, Scal C, const unsigned *stats) {
+    const size_t n = (size_t)C.W * C.H;
+    const float isens = C.isens, et = C.edge_thr, ct = C.chroma_thr;
+    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK) {
+        const float rg = P.lrg[i], bg = P.lbg[i];
+        const float dr = rg - P.brg[i], db = bg - P.bbg[i];
+        const float r0 = mx(P.src[3 * i], 0.f), g0 = mx(P.src[3 * i + 1], 0.f), b0 = mx(P.src[3 * i + 2], 0.f);
+        const float purple_excess = mn(mx(0.f, dr), mx(0.f, db));
+        const float denom = mx(P.bg[i], 0.05f);
+        const float g_over_min = mx(0.f, g0 - mn(r0, b0)) / denom;
+        const float green_dom = sstep(0.010f, 0.060f, g_over_min);
+        const float green_excess = mx(mx(0.f, -dr), mx(0.f, -db)) * green_dom;
+        const bool is_purple = purple_excess >= green_excess;
+        const float excess = is_purple ? purple_excess : green_excess;
+        const float min_dir = is_purple ? mx(0.f, mn(rg, bg)) : mx(0.f, mn(-rg, -bg));
+        const float max_dir = is_purple ? mx(1e-4f, mx(rg, bg)) : mx(1e-4f, mx(-rg, -bg));
+        const float balance = min_dir / max_dir;
+        const float c_edge = sstep(et * 0.35f * isens, et * 1.50f * isens, P.be[i]) *
+                             sstep(0.55f * isens, 1.35f * isens, P.ber[i]);
+        const float c_bright = sstep(0.24f * isens, 0.52f * isens, P.bg[i]);
+        const float c_chroma = sstep(ct * 0.45f * isens, ct * 1.60f * isens, excess);
+        const float red_strength = mx(0.f, r0 - mx(g0, b0)) / mx(r0, 1e-4f);
+        const float rb_balance = b0 / mx(r0, 1e-4f);
+        const float blue_lift = mx(0.f, b0 - g0) / mx(r0 - g0, 1e-4f);
+        const float red_purple = mx(sstep(0.55f, 0.78f, rb_balance), sstep(0.16f, 0.34f, blue_lift));
+        const float rp_signal = sstep(0.50f, 0.72f, rb_balance) * sstep(0.24f, 0.42f, blue_lift);
+        const float balanced_hue = sstep(0.06f * isens, 0.22f * isens, min_dir) * sstep(0.35f, 0.75f, balance);
+        const float rp_hue = is_purple ? mn(1.f, rp_signal * 1.50f) * sstep(0.035f * isens, 0.16f * isens, min_dir) *
+                                             sstep(0.18f, 0.58f, balance)
+                                       : 0.f;
+        const float c_hue = mx(balanced_hue, rp_hue);
+        const float red_guard = 1.f - sstep(0.08f, 0.25f, red_strength) * (1.f - red_purple);
+        const float green_enable = (C.enable_green || is_purple) ? 1.f : 0.f;
+        P.fs[i] = c_edge * c_bright * c_hue * c_chroma * red_guard * green_enable;
+    }
+}
+
+// line-linked support (:4266-4323): ls = max(fs, 0.55 * blur(fs)) sampled with clamp
+LRE_DEV float ls_at(const Planes &P, int W, int H, int x, int y) {
+    x = x < 0 ? 0 : (x > W - 1 ? W - 1 : x);
+    y = y < 0 ? 0 : (y > H - 1 ? H - 1 : y);
+    const size_t i = (size_t)y * W + x;
+    return mx(P.fs[i], P.bfs[i] * 0.55f);
+}
+LRE_DEV float fs_at(const Planes &P, int W, int H, int x, int y) {
+    x = x < 0 ? 0 : (x > W - 1 ? W - 1 : x);
+    y = y < 0 ? 0 : (y > H - 1 ? H - 1 : y);
+    return P.fs[(size_t)y * W + x];
+}
+
+__global__ __launch_bounds__(BLOCK) void line_link(Planes P, int W, int H) {
+    const int dtx[8] = {1, 0, 1, 1, 2, 1, 2, 1}, dty[8] = {0, 1, 1, -1, 1, 2, -1, -2};
+    const size_t n = (size_t)W * H;
+    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK) {
+        const int y = (int)(i / W), x = (int)(i % W);
+        float best = 0.f;
+#pragma unroll
+        for (int d = 0; d < 8; d++) {
+            const int tx = dtx[d], ty = dty[d], nx = -ty, ny = tx;
+            float along = 0.f;
+            const int max_step = d < 4 ? 14 : 6;
+            for (int s = 1; s <= max_step; s++) {
+                along = mx(along, ls_at(P, W, H, x + tx * s, y + ty * s));
+                along = mx(along, ls_at(P, W, H, x - tx * s, y - ty * s));
+            }
+            const float across = mx(fs_at(P, W, H, x + nx * 2, y + ny * 2), fs_at(P, W, H, x - nx * 2, y - ny * 2));
+            const float thin = 1.f - sstep(along * 0.45f, along * 0.95f, across);
+            best = mx(best, along * thin);
+        }
+        P.lfs[i] = best;
+    }
+}
+
+// solo seeds (:4325-4376)
+__global__ __launch_bounds__(BLOCK) void solo_seed(Planes P, Scal C) {
+    const size_t n = (size_t)C.W * C.H;
+    const float isens = C.isens, et = C.edge_thr, ct = C.chroma_thr;
+    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK) {
+        const float s0 = P.src[3 * i], s1 = P.src[3 * i + 1], s2 = P.src[3 * i + 2];
+        const float g = mx(s1, 0.f);
+        const float tr = g * expf_cr(P.brg[i]);
+        const float tb = g * expf_cr(P.bbg[i]);
+        const float solo_r = mx(0.f, s0 - tr) / mx(P.bg[i], 0.05f);
+        const float solo_b = mx(0.f, s2 - tb) / mx(P.bg[i], 0.05f);
+        const float linked = sstep(0.004f, 0.045f, mx(P.bfs[i], P.lfs[i] * 0.90f));
+        const float c_edge = sstep(et * 0.35f * isens, et * 1.50f * isens, P.be[i]) *
+                             sstep(0.55f * isens, 1.35f * isens, P.ber[i]);
+        const float c_bright = sstep(0.24f * isens, 0.52f * isens, P.bg[i]);
+        const float r0 = mx(s0, 0.f), g0 = mx(s1, 0.f), b0 = mx(s2, 0.f);
+        const float red_strength = mx(0.f, r0 - mx(g0, b0)) / mx(r0, 1e-4f);
+        const float red_guard = 1.f - 0.90f * sstep(0.12f, 0.32f, red_strength);
+        P.sr[i] = C.solo_enable * c_edge * c_bright * linked * red_guard *
+                  sstep(ct * 0.22f * isens, ct * 0.95f * isens, solo_r);
+        P.sb[i] = C.solo_enable * c_edge * c_bright * sstep(ct * 0.22f * isens, ct * 0.95f * isens, solo_b);
+    }
+}
+
+// main correction and solo cleanup (:4377-4987)
+__global__ __launch_bounds__(BLOCK) void correct(Planes P, Scal C, const unsigned *stats) {
+    const size_t n = (size_t)C.W * C.H;
+    const float isens = C.isens, et = C.edge_thr, ct = C.chroma_thr, strength = C.strength;
+    const bool eg = C.enable_green != 0;
+    const float scene_gate = sstep(0.55f, 0.75f, max_guide_of(stats));
+    const float mask_cutoff = C.mask_cutoff;
+    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK) {
+        const float s0 = P.src[3 * i], s1 = P.src[3 * i + 1], s2 = P.src[3 * i + 2];
+        const float bguide = P.bg[i], brg = P.brg[i], bbg = P.bbg[i], bfs = P.bfs[i], lfs = P.lfs[i];
+        const float rg = P.lrg[i], bg = P.lbg[i];
+        const float dr = rg - brg, db = bg - bbg;
+        const float r0 = mx(s0, 0.f), g0 = mx(s1, 0.f), b0 = mx(s2, 0.f);
+        const float rb_avg = 0.5f * (r0 + b0);
+        const float purple_excess = mn(mx(0.f, dr), mx(0.f, db));
+        const float denom = mx(bguide, 0.05f);
+        const float g_over_min = mx(0.f, g0 - mn(r0, b0)) / denom;
+        const float green_dom = sstep(0.010f, 0.060f, g_over_min);
+        const float green_excess = mx(mx(0.f, -dr), mx(0.f, -db)) * green_dom;
+        const bool is_purple = purple_excess >= green_excess;
+        const float excess = is_purple ? purple_excess : green_excess;
+        const float min_dir = is_purple ? mx(0.f, mn(rg, bg)) : mx(0.f, mn(-rg, -bg));
+        const float max_dir = is_purple ? mx(1e-4f, mx(rg, bg)) : mx(1e-4f, mx(-rg, -bg));
+        const float balance = min_dir / max_dir;
+        const float c_edge_core = sstep(et * 0.35f * isens, et * 1.50f * isens, P.be[i]) *
+                                  sstep(0.55f * isens, 1.35f * isens, P.ber[i]);
+        const float c_edge_low = sstep(et * 0.10f * isens, et * 0.80f * isens, P.be[i]) *
+                                 sstep(0.20f * isens, 0.90f * isens, P.ber[i]);
+        const float linked = sstep(0.004f, 0.045f, mx(bfs, lfs * 0.90f));
+        const float c_bright = sstep(0.24f * isens, 0.52f * isens, bguide);
+        const float c_chroma = sstep(ct * 0.45f * isens, ct * 1.60f * isens, excess);
+        const float rb_balance_hue = b0 / mx(r0, 1e-4f);
+        const float blue_lift_hue = mx(0.f, b0 - g0) / mx(r0 - g0, 1e-4f);
+        const float rp_signal = sstep(0.50f, 0.72f, rb_balance_hue) * sstep(0.24f, 0.42f, blue_lift_hue);
+        const float balanced_hue = sstep(0.06f * isens, 0.22f * isens, min_dir) * sstep(0.35f, 0.75f, balance);
+        const float rp_gate = is_purple ? mn(1.f, rp_signal * 1.50f) * sstep(0.035f * isens, 0.16f * isens, min_dir) *
+                                              sstep(0.18f, 0.58f, balance)
+                                        : 0.f;
+        const float green_unbal = (eg && !is_purple) ? sstep(0.035f * isens, 0.16f * isens, max_dir) : 0.f;
+        const float c_hue = mx(balanced_hue, mx(rp_gate, green_unbal));
+        const float purple_abs = mn(mx(0.f, r0 - g0), mx(0.f, b0 - g0)) / mx(bguide, 0.05f);
+        const float green_abs_base = mx(0.f, g0 - rb_avg) / mx(bguide, 0.05f);
+        const float green_abs_alt = mx(0.f, g0 - mn(r0, b0)) / mx(bguide, 0.05f);
+        const float green_abs = mx(green_abs_base, 0.55f * green_abs_alt);
+        const bool abs_is_purple = purple_abs >= green_abs;
+        const float abs_excess = abs_is_purple ? purple_abs : green_abs;
+        const float c_abs = sstep(ct * 0.35f * isens, ct * 1.20f * isens, abs_excess);
+        const float low_contrast = sstep(ct * 1.20f * isens, ct * 2.40f * isens, abs_excess);
+        const float green_soft = (eg && !is_purple) ? c_edge_low * sstep(0.05f * isens, 0.20f * isens, max_dir) : 0.f;
+        const float c_edge = mx(c_edge_core, mx(c_edge_low * linked * 1.45f, mx(c_edge_low * low_contrast * 0.35f, green_soft)));
+        const float ratio_mask = c_edge * c_bright * c_hue * c_chroma;
+        const float direct_mask = c_edge * c_bright * c_hue * c_abs;
+        const bool use_direct = direct_mask > ratio_mask;
+        float mask = mx(ratio_mask, direct_mask) * scene_gate;
+        const bool correct_purple = use_direct ? abs_is_purple : is_purple;
+        if (correct_purple) {
+            mask *= sstep(ct * 0.10f * isens, ct * 0.55f * isens, purple_abs);
+        } else {
+            const float local_gate = sstep(ct * 0.10f * isens, ct * 0.55f * isens, green_abs);
+            if (eg) {
+                const float peak_chroma = sstep(ct * 0.80f * isens, ct * 2.50f * isens, green_abs);
+                const float peak_nb = sstep(0.005f, 0.030f, bfs);
+                mask = mx(mask, peak_chroma * peak_nb * c_bright * c_hue * scene_gate);
+            }
+            const float hl_protect = 1.f - 0.85f * sstep(0.62f, 0.88f, bguide);
+            const float rb_to_green = rb_avg / mx(g0, 1e-4f);
+            const float natural_hl = sstep(0.22f, 0.46f, rb_to_green) * sstep(0.30f, 0.55f, g0);
+            const float nat_suppress = 1.f - 0.99f * natural_hl;
+            const float nat_suppress_sq = nat_suppress * nat_suppress;
+            const float halo = sstep(ct * 0.55f * isens, ct * 1.65f * isens, green_abs);
+            const float green_gain = (eg ? 0.60f : 0.35f) + (eg ? 1.10f : 0.95f) * halo;
+            const float green_nb = sstep(0.02f, 0.10f, mx(0.f, -mn(brg, bbg)));
+            const float smax = mx(r0, mx(g0, b0)), smin = mn(r0, mn(g0, b0));
+            const float ssat = (smax - smin) / mx(smax, 1e-4f);
+            const float whiteish = 1.f - sstep(0.05f, 0.13f, ssat);
+            const float foliage_guard = green_nb * whiteish * sstep(0.35f, 0.75f, bguide) * (1.f - halo);
+            const float strength_release = eg ? sstep(2.0f, 6.0f, strength) : 0.f;
+            const float line_evidence = sstep(0.004f, 0.030f, lfs) * sstep(0.20f, 0.55f, halo) * sstep(0.05f, 0.15f, ssat);
+            const float release = strength_release * line_evidence * (1.f - natural_hl);
+            const float rel_hl = hl_protect + (1.f - hl_protect) * release;
+            const float rel_nat = nat_suppress_sq + (1.f - nat_suppress_sq) * release;
+            const float foliage_term = 1.f - 0.999f * foliage_guard;
+            const float rel_foliage = foliage_term + (1.f - foliage_term) * release;
+            mask *= local_gate * green_gain * rel_hl * rel_nat * rel_foliage;
+            const float kill_evidence = sstep(0.04f, 0.15f, lfs);
+            const float natural_blob = sqrtf(mx(0.f, natural_hl)) * (1.f - kill_evidence) * green_nb;
+            mask *= 1.f - 0.999f * natural_blob;
+            if (!eg) mask = 0.f;
+        }
+
+        float o0 = s0, o1 = s1, o2 = s2;
+        bool skip_solo = false;
+        float e_rg = 0.f, e_bg = 0.f;
+        bool have_exp = false;
+        if (mask > mask_cutoff) {
+            const float g = mx(s1, 0.f);
+            const float red_strength = mx(0.f, r0 - mx(g0, b0)) / mx(r0, 1e-4f);
+            const float rb_balance = b0 / mx(r0, 1e-4f);
+            const float blue_lift = mx(0.f, b0 - g0) / mx(r0 - g0, 1e-4f);
+            const float red_purple = mx(sstep(0.55f, 0.78f, rb_balance), sstep(0.16f, 0.34f, blue_lift));
+            const float rsg = sstep(0.08f, 0.25f, red_strength);
+            float rp_context = 0.f, ctx_rp = 0.f, red_magenta = 0.f;
+            if (red_purple > 0.50f) {
+                rp_context = linked;
+                ctx_rp = red_purple * rp_context;
+                if (rp_context < 0.20f && rsg < 0.50f && red_purple > 0.50f)
+                    red_magenta = sstep(0.02f, 0.12f, purple_abs) * (1.f - rp_context);
+            }
+            float ngs = 0.f;
+            if (strength > 3.f && rp_context > 0.80f && lfs > 0.012f && rsg < 0.98f && blue_lift > 1.10f &&
+                rb_balance > 0.82f && purple_abs > 0.015f) {
+                const float ng_edge = sstep(1.10f, 1.80f, blue_lift) * sstep(0.82f, 1.05f, rb_balance) * (1.f - rsg);
+                ngs = ng_edge * sstep(0.015f, 0.08f, purple_abs) * sstep(3.f, 8.f, strength);
+            }
+            const float red_object = mx(rsg, red_magenta);
+            const float nwg_base = 1.f - red_object * (1.f - ctx_rp);
+            const float nwg_sq = nwg_base * nwg_base * nwg_base * nwg_base;
+            const float nwg = nwg_sq * nwg_sq;
+            const float line_relief = sstep(0.012f, 0.06f, lfs) * sstep(0.10f, 0.24f, blue_lift);
+            const float rog = sstep(0.55f, 0.90f, red_object);
+            const float linked_override = linked * 1.05f * (1.f - 0.85f * rog);
+            const float line_override = line_relief * (1.f - 0.85f * rog);
+            const bool force_warm = (rog > 0.75f) && (linked < 0.65f) && (line_relief < 0.45f);
+            const float warm_protect = correct_purple ? (force_warm ? nwg : mx(nwg, mx(linked_override, line_override))) : 1.f;
+            const float red_wo_support = sstep(0.22f, 0.45f, red_strength) * (1.f - sstep(0.02f, 0.12f, bfs));
+            const float protect = correct_purple ? red_wo_support : 0.f;
+            const float smooth_mask = sstep(mask_cutoff, mask_cutoff * 5.0f, mask);
+            const float smax = mx(r0, mx(g0, b0)), smin = mn(r0, mn(g0, b0));
+            const float ssat = (smax - smin) / mx(smax, 1e-4f);
+            const float ng_base = sstep(0.008f, 0.030f, ssat);
+            const float neutral_guard = 1.f - (1.f - ng_base) * sstep(0.35f, 0.80f, red_object);
+            float amount = lre::clamp_std(smooth_mask * strength * 1.90f * (1.f - protect) * warm_protect *
+                                              neutral_guard * C.threshold_soften,
+                                          0.f, 1.f);
+            const float warm_edge = sstep(0.006f, 0.035f, purple_abs) * sstep(0.25f, 0.65f, red_purple) *
+                                    sstep(0.15f, 0.65f, rp_context) * (1.f - 0.55f * rsg) * sstep(3.f, 8.f, strength);
+            const float safe_surface = mx(ngs, warm_edge);
+            if (correct_purple && safe_surface > 1e-4f) amount = mn(amount, 1.f - 0.99f * safe_surface);
+            if (amount <= 1e-4f) {
+                skip_solo = true;
+            } else {
+                e_rg = expf_cr(brg);
+                e_bg = expf_cr(bbg);
+                have_exp = true;
+                const float target_r = mn(g, g * e_rg);
+                const float target_b = mn(g, g * e_bg);
+                if (correct_purple) {
+                    float cr = lre::clamp_std(s0 * (1.f - amount) + target_r * amount, 0.f, s0);
+                    float cg = s1;
+                    float cb = lre::clamp_std(s2 * (1.f - amount) + target_b * amount, 0.f, s2);
+                    if (safe_surface > 1e-4f) {
+                        const float luma_before = 0.2126f * s0 + 0.7152f * s1 + 0.0722f * s2;
+                        const float luma_after = 0.2126f * cr + 0.7152f * cg + 0.0722f * cb;
+                        const float max_drop = luma_before * 0.02f * (1.f - ngs);
+                        const float min_luma = mx(0.f, luma_before - max_drop);
+                        if (luma_after < min_luma) {
+                            const float lift = min_luma - luma_after;
+                            const float ceil_ = mx(s0, mx(s1, s2));
+                            cr = mn(s0, cr + lift);
+                            cg = mn(ceil_, cg + lift);
+                            cb = mn(s2, cb + lift);
+                        }
+                        const float smx = mx(s0, mx(s1, s2)), smn = mn(s0, mn(s1, s2));
+                        const float cmx = mx(cr, mx(cg, cb)), cmn = mn(cr, mn(cg, cb));
+                        const float sch = smx - smn, cch = cmx - cmn;
+                        const float min_chroma = sch * (1.f - 0.12f * safe_surface);
+                        if (sch > 1e-4f && cch < min_chroma) {
+                            const float dn = mx(sch - cch, 1e-4f);
+                            const float keep = lre::clamp_std((sch - min_chroma) / dn, 0.f, 1.f);
+                            cr = s0 * (1.f - keep) + cr * keep;
+                            cg = s1 * (1.f - keep) + cg * keep;
+                            cb = s2 * (1.f - keep) + cb * keep;
+                        }
+                    }
+                    o0 = cr; o1 = cg; o2 = cb;
+                } else {
+                    float target_g = s1;
+                    if (e_rg > 1e-4f && e_bg > 1e-4f) {
+                        const float g_from_r = mx(s0, 0.f) / e_rg;
+                        const float g_from_b = mx(s2, 0.f) / e_bg;
+                        target_g = 0.5f * (g_from_r + g_from_b);
+                    }
+                    target_g = mx(target_g, 0.5f * (mx(s0, 0.f) + mx(s2, 0.f)));
+                    const float green_amount = lre::clamp_std(amount * C.green_scale, 0.f, 1.f);
+                    o1 = lre::clamp_std(s1 * (1.f - green_amount) + target_g * green_amount, 0.f, s1);
+                }
+            }
+        }
+        if (!skip_solo && C.solo_enable > 0.f) {
+            const float q0 = C.inplace ? o0 : s0, q1 = C.inplace ? o1 : s1, q2 = C.inplace ? o2 : s2;
+            if (!have_exp) { e_rg = expf_cr(brg); e_bg = expf_cr(bbg); }
+            const float g = mx(q1, 0.f);
+            const float target_r = g * e_rg;
+            const float target_b = g * e_bg;
+            const float solo_r = mx(0.f, q0 - target_r) / mx(bguide, 0.05f);
+            const float solo_b = mx(0.f, q2 - target_b) / mx(bguide, 0.05f);
+            const float local_r = sstep(ct * 0.12f * isens, ct * 0.65f * isens, solo_r);
+            const float local_b = sstep(ct * 0.12f * isens, ct * 0.65f * isens, solo_b);
+            const float qr0 = mx(q0, 0.f), qb0 = mx(q2, 0.f);
+            const float gom = mx(0.f, g - mn(qr0, qb0)) / mx(bguide, 0.05f);
+            const float gdom = sstep(0.010f, 0.060f, gom);
+            const float purpleish = sstep(0.010f, 0.060f, mx(mx(0.f, qr0 - g), mx(0.f, qb0 - g)) / mx(bguide, 0.05f));
+            const float allow = 1.f - gdom * (1.f - purpleish);
+            const float mr = P.srb[i] * local_r * allow;
+            const float mb = P.sbb[i] * local_b * allow;
+            if (mr > mask_cutoff) {
+                const float a = lre::clamp_std(mr * strength * 0.85f, 0.f, 0.85f);
+                o0 = lre::clamp_std(o0 * (1.f - a) + target_r * a, 0.f, o0);
+            }
+            if (mb > mask_cutoff) {
+                const float a = lre::clamp_std(mb * strength * 0.95f, 0.f, 0.90f);
+                o2 = lre::clamp_std(o2 * (1.f - a) + target_b * a, 0.f, o2);
+            }
+        }
+        P.dst[3 * i] = o0;
+        P.dst[3 * i + 1] = o1;
+        P.dst[3 * i + 2] = o2;
+    }
+}
+
+int grid_for(size_t n) {
+    size_t b = (n + BLOCK - 1) / BLOCK;
+    if (b > 8192) b = 8192;
+    return (int)(b > 0 ? b : 1);
+}
+
+}  // namespace
+
+// planes: 20 full planes + taps + stats
+size_t lre_defringe_scratch_bytes(int w, int h) {
+    const size_t plane = (((size_t)w * h * 4) + 255) & ~(size_t)255;
+    return plane * 20 + 4096;
+}
+
+int lre_launch_defringe(const float *d_in, float *d_out, int w, int h, float radius,
+                        float edge_threshold, float chroma_threshold, float strength,
+                        int defringe_green, float green_strength_scale, void *d_scratch,
+                        hipStream_t s) {
+    const size_t n = (size_t)w * h;
+    const size_t plane = ((n * 4) + 255) & ~(size_t)255;
+    char *base = static_cast<char *>(d_scratch);
+    auto take = [&](int k) { return reinterpret_cast<float *>(base + plane * k); };
+    float *gl = take(0), *gs = take(1), *lrg = take(2), *lbg = take(3), *er = take(4);
+    float *t0 = take(5), *t1 = take(6), *t2 = take(7), *t3 = take(8), *t4 = take(9);
+    float *bg = take(10), *brg = take(11), *bbg = take(12), *be = take(13), *ber = take(14);
+    float *fs = take(15), *bfs = take(16), *lfs = take(17);
+    float *sr = gl, *sb = gs, *srb = t0, *sbb = t1;  // reuse: the guide planes are dead by then
+    unsigned *stats = reinterpret_cast<unsigned *>(base + plane * 20);
+    float *d_taps = reinterpret_cast<float *>(base + plane * 20 + 256);
+
+    // Gaussian taps (:4033-4048), computed on the host with the C library's expf
+    green_strength_scale = green_strength_scale < 0.f ? 0.f : (1.f < green_strength_scale ? 1.f : green_strength_scale);
+    const float sigma = (radius / 3.f < 0.5f) ? 0.5f : radius / 3.f;
+    const int ks = (int)std::ceil(3.f * sigma);
+    if (2 * ks + 1 > MAX_TAPS) {
+        lre_set_error("defringe radius %g too large (at most %d taps)", radius, MAX_TAPS);
+        return LRE_ERR_ARG;
+    }
+    float taps[MAX_TAPS];
+    {
+        float ksum = 0.f;
+        for (int k = -ks; k <= ks; k++) {
+            float v = std::exp(-0.5f * (float)(k * k) / (sigma * sigma));
+            taps[k + ks] = v;
+            ksum += v;
+        }
+        for (int k = 0; k < 2 * ks + 1; k++) taps[k] /= ksum;
+    }
+    LRE_HIP_CHECK(hipMemcpyAsync(d_taps, taps, sizeof(float) * (2 * ks + 1), hipMemcpyHostToDevice, s));
+
+    Scal C;
+    C.W = w; C.H = h; C.ks = ks;
+    C.inplace = d_in == d_out;
+    C.enable_green = defringe_green ? 1 : 0;
+    C.edge_thr = edge_threshold; C.chroma_thr = chroma_threshold; C.strength = strength;
+    C.green_scale = green_strength_scale;
+    {
+        float sv = std::sqrt(strength < 1.0f ? 1.0f : strength);
+        const float sens = sv < 1.0f ? 1.0f : (2.5f < sv ? 2.5f : sv);
+        C.isens = 1.0f / sens;
+        C.mask_cutoff = 0.02f * C.isens;
+        float t = (strength - 1.25f) / (3.0f - 1.25f);
+        t = t < 0.f ? 0.f : (1.f < t ? 1.f : t);
+        C.solo_enable = t * t * (3.0f - 2.0f * t);
+        float u = (chroma_threshold - 0.22f) / (0.40f - 0.22f);
+        u = u < 0.f ? 0.f : (1.f < u ? 1.f : u);
+        C.threshold_soften = 1.f - 0.86f * (u * u * (3.0f - 2.0f * u));
+    }
+
+    const int g = grid_for(n);
+    hipLaunchKernelGGL(init_stats, dim3(1), dim3(64), 0, s, stats);
+    hipLaunchKernelGGL(max_green, dim3(g), dim3(BLOCK), 0, s, d_in, n, stats);
+    hipLaunchKernelGGL(prep, dim3(g), dim3(BLOCK), 0, s, d_in, gl, gs, lrg, lbg, n, stats);
+    hipLaunchKernelGGL(sobel, dim3(g), dim3(BLOCK), 0, s, gs, er, w, h, stats);
+    LRE_HIP_CHECK(hipGetLastError());
+
+    auto blur = [&](BlurSet S) -> int {
+        dim3 gh((w + HB_W - 1) / HB_W, h);
+        hipLaunchKernelGGL(blur_h, gh, dim3(BLOCK), 0, s, S, d_taps, w, h, ks, stats);
+        dim3 gv((w + VB_W - 1) / VB_W, (h + VB_H - 1) / VB_H);
+        hipLaunchKernelGGL(blur_v, gv, dim3(BLOCK), 0, s, S, d_taps, w, h, ks);
+        LRE_HIP_CHECK(hipGetLastError());
+        return LRE_OK;
+    };
+    BlurSet S5{};
+    S5.count = 5;
+    const float *ins[5] = {gl, lrg, lbg, er, er};
+    float *tms[5] = {t0, t1, t2, t3, t4};
+    float *outs[5] = {bg, brg, bbg, be, ber};
+    for (int k = 0; k < 5; k++) { S5.in[k] = ins[k]; S5.tmp[k] = tms[k]; S5.out[k] = outs[k]; S5.scale_by_edge[k] = 0; }
+    S5.scale_by_edge[3] = 1;  // blur of the normalised edge map
+    int rc = blur(S5);
+    if (rc) return rc;
+
+    Planes P;
+    P.src = d_in; P.dst = d_out;
+    P.lrg = lrg; P.lbg = lbg; P.bg = bg; P.brg = brg; P.bbg = bbg; P.be = be; P.ber = ber;
+    P.fs = fs; P.bfs = bfs; P.lfs = lfs; P.sr = sr; P.sb = sb; P.srb = srb; P.sbb = sbb;
+    hipLaunchKernelGGL(support, dim3(g), dim3(BLOCK), 0, s, P, C, stats);
+    BlurSet S1{};
+    S1.count = 1; S1.in[0] = fs; S1.tmp[0] = t2; S1.out[0] = bfs;
+    if ((rc = blur(S1))) return rc;
+    hipLaunchKernelGGL(line_link, dim3(g), dim3(BLOCK), 0, s, P, w, h);
+    hipLaunchKernelGGL(solo_seed, dim3(g), dim3(BLOCK), 0, s, P, C);
+    BlurSet S2{};
+    S2.count = 2;
+    S2.in[0] = sr; S2.tmp[0] = t3; S2.out[0] = srb;
+    S2.in[1] = sb; S2.tmp[1] = t4; S2.out[1] = sbb;
+    if ((rc = blur(S2))) return rc;
+    hipLaunchKernelGGL(correct, dim3(g), dim3(BLOCK), 0, s, P, C, stats);
+    LRE_HIP_CHECK(hipGetLastError());
+    return LRE_OK;
+}

@@ -211,6 +211,19 @@ int lre_hip_dev_detail_tonemap(lre_ctx *ctx, const float *d_in, float *d_out, in
     return lre_launch_detail_tonemap(d_in, d_out, w, h, ctx->scratch, lre_pick_stream(ctx, stream));
 }
 
+int lre_hip_dev_defringe(lre_ctx *ctx, const float *d_in, float *d_out, int w, int h,
+                         float radius, float edge_threshold, float chroma_threshold,
+                         float strength, int defringe_green, float green_strength_scale,
+                         void *stream) {
+    if (!ctx || !d_in || !d_out) return set_arg_error("null pointer");
+    if (!valid_dims(w, h)) return set_arg_error("image size");
+    int rc = lre_ensure_scratch(ctx, lre_defringe_scratch_bytes(w, h));
+    if (rc) return rc;
+    return lre_launch_defringe(d_in, d_out, w, h, radius, edge_threshold, chroma_threshold,
+                               strength, defringe_green, green_strength_scale, ctx->scratch,
+                               lre_pick_stream(ctx, stream));
+}
+
 int lre_hip_dev_rgb3_to_cfa(lre_ctx *ctx, const float *d_raw_rgb, float *d_cfa, int w, int h,
                             uint32_t filters, const signed char xtrans[36], void *stream) {
     if (!ctx || !d_raw_rgb || !d_cfa) return set_arg_error("null pointer");
@@ -275,6 +288,27 @@ int lre_hip_detail_tonemap(lre_ctx *ctx, const float *in_rgb, float *out_rgb, in
     if ((rc = copy_in(ctx, d, in_rgb, n * 4))) return rc;
     if ((rc = lre_launch_detail_tonemap(d, d, w, h, ws, ctx->stream))) return rc;
     return copy_out(ctx, out_rgb, d, n * 4);
+}
+
+int lre_hip_defringe(lre_ctx *ctx, const float *in_rgb, float *out_rgb, int w, int h,
+                     float radius, float edge_threshold, float chroma_threshold, float strength,
+                     int defringe_green, float green_strength_scale) {
+    if (!ctx || !in_rgb || !out_rgb) return set_arg_error("null pointer");
+    if (!valid_dims(w, h)) return set_arg_error("image size");
+    LRE_HIP_CHECK(hipSetDevice(ctx->device));
+    const size_t n = (size_t)w * h;
+    const bool inplace = in_rgb == out_rgb;
+    int rc = lre_ensure_scratch(ctx, aligned(n * 12) * 2 + lre_defringe_scratch_bytes(w, h));
+    if (rc) return rc;
+    Carve c(ctx->scratch);
+    float *d_in = c.take<float>(n * 3), *d_out = c.take<float>(n * 3);
+    void *ws = static_cast<char *>(ctx->scratch) + c.off;
+    if ((rc = copy_in(ctx, d_in, in_rgb, n * 12))) return rc;
+    if ((rc = lre_launch_defringe(d_in, inplace ? d_in : d_out, w, h, radius, edge_threshold,
+                                  chroma_threshold, strength, defringe_green,
+                                  green_strength_scale, ws, ctx->stream)))
+        return rc;
+    return copy_out(ctx, out_rgb, inplace ? d_in : d_out, n * 12);
 }
 
 int lre_hip_white_balance(lre_ctx *ctx, const uint16_t *img4, float *rgb, int w, int h,

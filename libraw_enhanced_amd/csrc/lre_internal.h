@@ -31,6 +31,7 @@ inline hipStream_t lre_pick_stream(lre_ctx *ctx, void *stream) {
 // Bytes of device scratch each stage needs for an image of w x h.
 size_t lre_amaze_scratch_bytes(lre_ctx *ctx, int w, int h);
 size_t lre_detail_tm_scratch_bytes(int w, int h);
+size_t lre_defringe_scratch_bytes(int w, int h);
 
 // Kernel launchers (device pointers; scratch must already be large enough).
 int lre_launch_amaze(lre_ctx *ctx, const float *d_cfa, float *d_rgb, int w, int h,
@@ -42,6 +43,10 @@ int lre_launch_tone_mapping(const float *d_in, float *d_out, size_t n, float aft
                             hipStream_t s);
 int lre_launch_detail_tonemap(const float *d_in, float *d_out, int w, int h, void *d_scratch,
                               hipStream_t s);
+int lre_launch_defringe(const float *d_in, float *d_out, int w, int h, float radius,
+                        float edge_threshold, float chroma_threshold, float strength,
+                        int defringe_green, float green_strength_scale, void *d_scratch,
+                        hipStream_t s);
 int lre_launch_white_balance(const uint16_t *d_img4, float *d_rgb, int w, int h,
                              const float wb[4], uint32_t filters, const signed char *xtrans,
                              hipStream_t s);

@@ -34,7 +34,6 @@ class DevelopChain:
             self.defringe_params.update(defringe_params)
         n = height * width
         self.rgb = acc.dev_alloc(n * 12)
-        self.rgb2 = acc.dev_alloc(n * 12)
         self.stages = list(stages or available_chain())
 
     def run_stage(self, name: str, d_cfa, stream=None):
@@ -45,11 +44,11 @@ class DevelopChain:
                                          stream))
         elif name == "defringe":
             p = self.defringe_params
-            _check(lib.lre_hip_dev_defringe(ctx, self.rgb.ptr, self.rgb2.ptr, w, h, p["radius"],
+            # in place, as the develop pipeline calls it (core/libraw_wrapper.cpp:1804-1818)
+            _check(lib.lre_hip_dev_defringe(ctx, self.rgb.ptr, self.rgb.ptr, w, h, p["radius"],
                                             p["edge_threshold"], p["chroma_threshold"],
                                             p["strength"], int(p["defringe_green"]),
                                             p["green_strength_scale"], stream))
-            self.rgb, self.rgb2 = self.rgb2, self.rgb
         elif name == "detail_tm":
             _check(lib.lre_hip_dev_detail_tonemap(ctx, self.rgb.ptr, self.rgb.ptr, w, h, stream))
         elif name == "tone_mapping":

@@ -29,6 +29,10 @@ int oracle_amaze(const float *raw3, float *rgb3, int w, int h, uint32_t filters,
 void oracle_border_interpolate(float *rgb3, int w, int h, uint32_t filters, int border);
 int oracle_tone_mapping(const float *in3, float *out3, int w, int h, float after_scale);
 int oracle_detail_tonemap(float *rgb3, int w, int h);
+/* in == out selects the in-place semantics of the reference */
+int oracle_defringe(const float *in3, float *out3, int w, int h, float radius, float edge_threshold,
+                    float chroma_threshold, float strength, int defringe_green,
+                    float green_strength_scale);
 int oracle_white_balance(const uint16_t *img4, float *rgb3, int w, int h, const float *wb4,
                          uint32_t filters, const signed char *xtrans36);
 

@@ -78,3 +78,63 @@ def test_white_balance_bit_exact(acc):
     ximg4 = place_native_u16(x16, 9, scene.XTRANS)
     got = acc.apply_white_balance(ximg4, wb, 9, scene.XTRANS)
     assert np.array_equal(got, oracle_white_balance(ximg4, wb, 9, scene.XTRANS))
+
+
+# ---------------- defringe ----------------
+import ctypes  # noqa: E402
+
+import _defringe_cases as cases  # noqa: E402
+from _common import oracle, p  # noqa: E402
+
+
+def _oracle_defringe(img, inplace, radius=10.0, edge=0.1, chroma=0.15, strength=10.0, green=False, gscale=0.3):
+    h, w, _ = img.shape
+    a = np.ascontiguousarray(img, np.float32).copy()
+    out = a if inplace else np.zeros_like(a)
+    assert oracle().oracle_defringe(p(a), p(out), w, h, ctypes.c_float(radius), ctypes.c_float(edge),
+                                    ctypes.c_float(chroma), ctypes.c_float(strength), int(green),
+                                    ctypes.c_float(gscale)) == 0
+    return out
+
+
+def _gpu_defringe(acc, img, inplace, radius=10.0, edge=0.1, chroma=0.15, strength=10.0, green=False, gscale=0.3):
+    a = np.ascontiguousarray(img, np.float32).copy()
+    return acc.defringe(a, a if inplace else None, radius, edge, chroma, strength, green, gscale)
+
+
+@pytest.mark.parametrize("inplace", [False, True])
+@pytest.mark.parametrize("kw", [dict(), dict(green=True), dict(radius=6.0, strength=2.0, green=True, gscale=0.5)])
+def test_defringe_tolerance(acc, inplace, kw):
+    h, w = 480, 720
+    img = oracle_amaze(scene.cfa_to_rgb3(scene.bayer_scene(h, w, scene.BASE_SEED + 21))) * 1.8
+    want = _oracle_defringe(img, inplace, **kw)
+    got = _gpu_defringe(acc, img, inplace, **kw)
+    err = float(np.abs(got - want).max())
+    ndiff = int(np.count_nonzero(np.any(got != want, axis=2)))
+    print(f"defringe {kw} inplace={inplace}: max-abs {err:.3g}, {ndiff} px differ in the last bits")
+    assert err <= 2e-4
+    assert psnr(got, want) >= 70.0
+    assert ndiff < 0.01 * h * w
+
+
+def test_defringe_reference_invariants_on_gpu(acc):
+    def f(img, radius=10.0, strength=10.0, defringe_green=False):
+        return _gpu_defringe(acc, img, False, radius=radius, strength=strength, green=defringe_green)
+    assert cases.check_all(f) == []
+
+
+def test_develop_chain_matches_oracle(acc):
+    """AMaZE -> defringe (in place) -> detail tone map, all resident on the device."""
+    from libraw_enhanced_amd.pipeline import DevelopChain
+    h, w = 600, 900
+    cfa = scene.bayer_scene(h, w, scene.BASE_SEED + 5)
+    d_cfa = acc.dev_from_host(cfa)
+    chain = DevelopChain(acc, h, w, ["amaze", "defringe", "detail_tm"])
+    chain.run(d_cfa)
+    got = chain.result()
+    want = oracle_amaze(scene.cfa_to_rgb3(cfa))
+    want = _oracle_defringe(want, True)
+    want = oracle_detail_tonemap(want)
+    err = float(np.abs(got - want).max())
+    assert err <= 2e-4, err
+    assert psnr(got, want) >= 70.0
