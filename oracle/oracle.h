@@ -33,6 +33,10 @@ int oracle_detail_tonemap(float *rgb3, int w, int h);
 int oracle_defringe(const float *in3, float *out3, int w, int h, float radius, float edge_threshold,
                     float chroma_threshold, float strength, int defringe_green,
                     float green_strength_scale);
+int oracle_axial_ca(const float *in3, float *out3, int w, int h, int radius, float epsilon,
+                    float strength);
+int oracle_lateral_ca(const float *in3, float *out3, int w, int h, int cell_size, int max_iterations,
+                      float max_shift, float min_confidence, int pyramid_levels);
 int oracle_white_balance(const uint16_t *img4, float *rgb3, int w, int h, const float *wb4,
                          uint32_t filters, const signed char *xtrans36);
 
