@@ -87,6 +87,21 @@ int lre_hip_defringe(lre_ctx *ctx, const float *in_rgb, float *out_rgb, int widt
                      float radius, float edge_threshold, float chroma_threshold, float strength,
                      int defringe_green, float green_strength_scale);
 
+/* Accelerator::ca_axial_cleanup (core/accelerator.h:363-367,
+ * core/cpu_accelerator.cpp:3799-3928): guided filter of R and B with G as the
+ * guide; defaults radius 6 (at most 16 here), epsilon 1e-4, strength 0.3.
+ * in == out is allowed. */
+int lre_hip_axial_ca(lre_ctx *ctx, const float *in_rgb, float *out_rgb, int width, int height,
+                     int radius, float epsilon, float strength);
+
+/* Accelerator::ca_register_lateral (core/accelerator.h:354-360,
+ * core/cpu_accelerator.cpp:3496-3673): pyramidal Lucas-Kanade registration of R
+ * and B to G per cell, then a bilinear warp; defaults cell 96, 3 iterations,
+ * max shift 6, min confidence 0.02, 3 levels.  in == out is allowed. */
+int lre_hip_lateral_ca(lre_ctx *ctx, const float *in_rgb, float *out_rgb, int width, int height,
+                       int cell_size, int max_iterations, float max_shift, float min_confidence,
+                       int pyramid_levels);
+
 /* Accelerator::apply_white_balance (core/accelerator.h:304-307,
  * core/cpu_accelerator.cpp:2789-2869): writes float(sample) * wb[c] to the
  * native channel of rgb and leaves the other two channels untouched.
@@ -110,6 +125,11 @@ int lre_hip_dev_defringe(lre_ctx *ctx, const float *d_in, float *d_out, int widt
                          float radius, float edge_threshold, float chroma_threshold,
                          float strength, int defringe_green, float green_strength_scale,
                          void *stream);
+int lre_hip_dev_axial_ca(lre_ctx *ctx, const float *d_in, float *d_out, int width, int height,
+                         int radius, float epsilon, float strength, void *stream);
+int lre_hip_dev_lateral_ca(lre_ctx *ctx, const float *d_in, float *d_out, int width, int height,
+                           int cell_size, int max_iterations, float max_shift,
+                           float min_confidence, int pyramid_levels, void *stream);
 /* Demosaic input planes: extract the native channel of the reference's 3-channel
  * layout into a CFA plane. */
 int lre_hip_dev_rgb3_to_cfa(lre_ctx *ctx, const float *d_raw_rgb, float *d_cfa, int width,

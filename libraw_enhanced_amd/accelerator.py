@@ -57,6 +57,15 @@ _SIGS = {
     "lre_hip_dev_defringe": ([_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_float,
                               ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int,
                               ctypes.c_float, _vp], ctypes.c_int),
+    "lre_hip_axial_ca": ([_vp, _f32p, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                          ctypes.c_float, ctypes.c_float], ctypes.c_int),
+    "lre_hip_lateral_ca": ([_vp, _f32p, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                            ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int], ctypes.c_int),
+    "lre_hip_dev_axial_ca": ([_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                              ctypes.c_float, ctypes.c_float, _vp], ctypes.c_int),
+    "lre_hip_dev_lateral_ca": ([_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int, _vp],
+                               ctypes.c_int),
     "lre_hip_white_balance": ([_vp, _u16p, _f32p, ctypes.c_int, ctypes.c_int, _f32p,
                                ctypes.c_uint32, _i8p], ctypes.c_int),
     "lre_hip_dev_amaze": ([_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, _f32p,
@@ -185,6 +194,25 @@ class HIPAccelerator:
         _check(native().lre_hip_defringe(self._h, _p(src), _p(out), w, h, radius, edge_threshold,
                                          chroma_threshold, strength, int(defringe_green),
                                          green_strength_scale))
+        return out
+
+    def ca_axial_cleanup(self, rgb_input, radius: int = 6, epsilon: float = 1e-4,
+                         strength: float = 0.3):
+        src = _rgb(rgb_input)
+        h, w, _ = src.shape
+        out = np.empty_like(src)
+        _check(native().lre_hip_axial_ca(self._h, _p(src), _p(out), w, h, radius, epsilon, strength))
+        return out
+
+    def ca_register_lateral(self, rgb_input, cell_size: int = 96, max_iterations: int = 3,
+                            max_shift: float = 6.0, min_confidence: float = 0.02,
+                            pyramid_levels: int = 3):
+        src = _rgb(rgb_input)
+        h, w, _ = src.shape
+        out = np.empty_like(src)
+        _check(native().lre_hip_lateral_ca(self._h, _p(src), _p(out), w, h, cell_size,
+                                           max_iterations, max_shift, min_confidence,
+                                           pyramid_levels))
         return out
 
     def apply_white_balance(self, img4, wb, filters: int = RGGB, xtrans=None, rgb=None):

@@ -63,6 +63,24 @@ int copy_out(lre_ctx *ctx, void *dst, const void *src, size_t bytes) {
     std::memcpy(dst, ctx->pinned, bytes);
     return LRE_OK;
 }
+// host-pointer form of an RGB -> RGB stage: copy in, run, copy out
+template <class F>
+int rgb_stage_host(lre_ctx *ctx, const float *in_rgb, float *out_rgb, int w, int h,
+                          size_t stage_scratch, F launch) {
+    if (!ctx || !in_rgb || !out_rgb) return set_arg_error("null pointer");
+    if (!valid_dims(w, h)) return set_arg_error("image size");
+    LRE_HIP_CHECK(hipSetDevice(ctx->device));
+    const size_t n = (size_t)w * h;
+    int rc = lre_ensure_scratch(ctx, aligned(n * 12) * 2 + stage_scratch);
+    if (rc) return rc;
+    Carve c(ctx->scratch);
+    float *d_in = c.take<float>(n * 3), *d_out = c.take<float>(n * 3);
+    void *ws = static_cast<char *>(ctx->scratch) + c.off;
+    if ((rc = copy_in(ctx, d_in, in_rgb, n * 12))) return rc;
+    if ((rc = launch(d_in, d_out, ws))) return rc;
+    return copy_out(ctx, out_rgb, d_out, n * 12);
+}
+
 }  // namespace
 
 void lre_set_error(const char *fmt, ...) {
@@ -224,6 +242,28 @@ int lre_hip_dev_defringe(lre_ctx *ctx, const float *d_in, float *d_out, int w, i
                                lre_pick_stream(ctx, stream));
 }
 
+int lre_hip_dev_axial_ca(lre_ctx *ctx, const float *d_in, float *d_out, int w, int h, int radius,
+                         float epsilon, float strength, void *stream) {
+    if (!ctx || !d_in || !d_out) return set_arg_error("null pointer");
+    if (!valid_dims(w, h)) return set_arg_error("image size");
+    int rc = lre_ensure_scratch(ctx, lre_axial_scratch_bytes(w, h));
+    if (rc) return rc;
+    return lre_launch_axial_ca(d_in, d_out, w, h, radius, epsilon, strength, ctx->scratch,
+                               lre_pick_stream(ctx, stream));
+}
+
+int lre_hip_dev_lateral_ca(lre_ctx *ctx, const float *d_in, float *d_out, int w, int h,
+                           int cell_size, int max_iterations, float max_shift,
+                           float min_confidence, int pyramid_levels, void *stream) {
+    if (!ctx || !d_in || !d_out) return set_arg_error("null pointer");
+    if (!valid_dims(w, h)) return set_arg_error("image size");
+    int rc = lre_ensure_scratch(ctx, lre_lateral_scratch_bytes(w, h, cell_size));
+    if (rc) return rc;
+    return lre_launch_lateral_ca(d_in, d_out, w, h, cell_size, max_iterations, max_shift,
+                                 min_confidence, pyramid_levels, ctx->scratch,
+                                 lre_pick_stream(ctx, stream));
+}
+
 int lre_hip_dev_rgb3_to_cfa(lre_ctx *ctx, const float *d_raw_rgb, float *d_cfa, int w, int h,
                             uint32_t filters, const signed char xtrans[36], void *stream) {
     if (!ctx || !d_raw_rgb || !d_cfa) return set_arg_error("null pointer");
@@ -309,6 +349,27 @@ int lre_hip_defringe(lre_ctx *ctx, const float *in_rgb, float *out_rgb, int w, i
                                   green_strength_scale, ws, ctx->stream)))
         return rc;
     return copy_out(ctx, out_rgb, inplace ? d_in : d_out, n * 12);
+}
+
+int lre_hip_axial_ca(lre_ctx *ctx, const float *in_rgb, float *out_rgb, int w, int h, int radius,
+                     float epsilon, float strength) {
+    return rgb_stage_host(ctx, in_rgb, out_rgb, w, h, lre_axial_scratch_bytes(w, h),
+                          [&](const float *di, float *dout, void *ws) {
+                              return lre_launch_axial_ca(di, dout, w, h, radius, epsilon, strength,
+                                                         ws, ctx->stream);
+                          });
+}
+
+int lre_hip_lateral_ca(lre_ctx *ctx, const float *in_rgb, float *out_rgb, int w, int h,
+                       int cell_size, int max_iterations, float max_shift, float min_confidence,
+                       int pyramid_levels) {
+    return rgb_stage_host(ctx, in_rgb, out_rgb, w, h, lre_lateral_scratch_bytes(w, h, cell_size),
+                          [&](const float *di, float *dout, void *ws) {
+                              return lre_launch_lateral_ca(di, dout, w, h, cell_size,
+                                                           max_iterations, max_shift,
+                                                           min_confidence, pyramid_levels, ws,
+                                                           ctx->stream);
+                          });
 }
 
 int lre_hip_white_balance(lre_ctx *ctx, const uint16_t *img4, float *rgb, int w, int h,

@@ -32,6 +32,8 @@ inline hipStream_t lre_pick_stream(lre_ctx *ctx, void *stream) {
 size_t lre_amaze_scratch_bytes(lre_ctx *ctx, int w, int h);
 size_t lre_detail_tm_scratch_bytes(int w, int h);
 size_t lre_defringe_scratch_bytes(int w, int h);
+size_t lre_axial_scratch_bytes(int w, int h);
+size_t lre_lateral_scratch_bytes(int w, int h, int cell_size);
 
 // Kernel launchers (device pointers; scratch must already be large enough).
 int lre_launch_amaze(lre_ctx *ctx, const float *d_cfa, float *d_rgb, int w, int h,
@@ -47,6 +49,11 @@ int lre_launch_defringe(const float *d_in, float *d_out, int w, int h, float rad
                         float edge_threshold, float chroma_threshold, float strength,
                         int defringe_green, float green_strength_scale, void *d_scratch,
                         hipStream_t s);
+int lre_launch_axial_ca(const float *d_in, float *d_out, int w, int h, int radius, float epsilon,
+                        float strength, void *d_scratch, hipStream_t s);
+int lre_launch_lateral_ca(const float *d_in, float *d_out, int w, int h, int cell_size,
+                          int max_iterations, float max_shift, float min_confidence,
+                          int pyramid_levels, void *d_scratch, hipStream_t s);
 int lre_launch_white_balance(const uint16_t *d_img4, float *d_rgb, int w, int h,
                              const float wb[4], uint32_t filters, const signed char *xtrans,
                              hipStream_t s);

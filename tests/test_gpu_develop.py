@@ -138,3 +138,28 @@ def test_develop_chain_matches_oracle(acc):
     err = float(np.abs(got - want).max())
     assert err <= 2e-4, err
     assert psnr(got, want) >= 70.0
+
+
+# ---------------- chromatic aberration ----------------
+from test_ca_cpu import axial as _axial, ca_scene as _ca_scene, lateral as _lateral  # noqa: E402
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(radius=3, eps=1e-3, strength=1.0)])
+def test_axial_ca_tolerance(acc, kw):
+    img = _ca_scene(520, 780)
+    want = _axial(oracle(), "oracle_axial_ca", img, **kw)
+    got = acc.ca_axial_cleanup(img, kw.get("radius", 6), kw.get("eps", 1e-4), kw.get("strength", 0.3))
+    err = float(np.abs(got - want).max())
+    print(f"axial {kw}: max-abs {err:.3g}, {int(np.count_nonzero(got != want))} values differ")
+    assert err <= 2e-4 and psnr(got, want) >= 70.0
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(cell=48, iters=5, shift=3.0, levels=2)])
+def test_lateral_ca_tolerance(acc, kw):
+    img = _ca_scene(800, 1200)
+    want = _lateral(oracle(), "oracle_lateral_ca", img, **kw)
+    got = acc.ca_register_lateral(img, kw.get("cell", 96), kw.get("iters", 3), kw.get("shift", 6.0),
+                                  kw.get("conf", 0.02), kw.get("levels", 3))
+    err = float(np.abs(got - want).max())
+    print(f"lateral {kw}: max-abs {err:.3g}, {int(np.count_nonzero(got != want))} values differ")
+    assert err <= 2e-4 and psnr(got, want) >= 70.0
