@@ -65,6 +65,15 @@ int lre_hip_memcpy_dtoh(lre_ctx *ctx, void *h_dst, const void *d_src, size_t byt
 int lre_hip_amaze(lre_ctx *ctx, const float *raw_rgb, float *rgb, int width, int height,
                   uint32_t filters, const float cam_mul[4], float maximum_value);
 
+/* Accelerator::demosaic_xtrans_3pass (core/accelerator.h:291-295,
+ * core/cpu_accelerator.cpp:1870-2676 and :2776-2783): Markesteijn 3-pass
+ * with YPbPr homogeneity, then the weighted border interpolation of the 8-px
+ * frame.  xtrans is the 6x6 colour table (0 R, 1 G, 2 B), row-major.  The
+ * reference's colour matrix argument only feeds its disabled CIELab branch and
+ * is not needed.  Width at most 16382 (16-bit hexagon offsets, as the reference). */
+int lre_hip_xtrans3(lre_ctx *ctx, const float *raw_rgb, float *rgb, int width, int height,
+                    const signed char xtrans[36], float maximum_value);
+
 /* Accelerator::tone_mapping (core/accelerator.h:322-323,
  * core/cpu_accelerator.cpp:3034-3068): ACES per channel times after_scale.
  * in == out is allowed. */
@@ -116,6 +125,9 @@ int lre_hip_white_balance(lre_ctx *ctx, const uint16_t *img4, float *rgb, int wi
 int lre_hip_dev_amaze(lre_ctx *ctx, const float *d_cfa, float *d_rgb, int width, int height,
                       uint32_t filters, const float cam_mul[4], float maximum_value,
                       void *stream);
+/* X-Trans 3-pass from a float CFA plane to interleaved RGB. */
+int lre_hip_dev_xtrans3(lre_ctx *ctx, const float *d_cfa, float *d_rgb, int width, int height,
+                        const signed char xtrans[36], float maximum_value, void *stream);
 int lre_hip_dev_tone_mapping(lre_ctx *ctx, const float *d_in, float *d_out, int width,
                              int height, float after_scale, void *stream);
 int lre_hip_dev_detail_tonemap(lre_ctx *ctx, const float *d_in, float *d_out, int width,

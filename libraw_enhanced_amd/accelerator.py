@@ -48,6 +48,10 @@ _SIGS = {
     "lre_hip_memcpy_dtoh": ([_vp, _vp, _vp, ctypes.c_size_t], ctypes.c_int),
     "lre_hip_amaze": ([_vp, _f32p, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, _f32p,
                        ctypes.c_float], ctypes.c_int),
+    "lre_hip_xtrans3": ([_vp, _f32p, _f32p, ctypes.c_int, ctypes.c_int, _i8p, ctypes.c_float],
+                        ctypes.c_int),
+    "lre_hip_dev_xtrans3": ([_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _i8p, ctypes.c_float, _vp],
+                            ctypes.c_int),
     "lre_hip_tone_mapping": ([_vp, _f32p, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_float],
                              ctypes.c_int),
     "lre_hip_detail_tonemap": ([_vp, _f32p, _f32p, ctypes.c_int, ctypes.c_int], ctypes.c_int),
@@ -164,6 +168,14 @@ class HIPAccelerator:
         cm = np.ascontiguousarray(cam_mul, np.float32)
         _check(native().lre_hip_amaze(self._h, _p(raw), _p(out), w, h, filters, _p(cm),
                                       maximum_value))
+        return out
+
+    def demosaic_xtrans_3pass(self, raw_rgb, xtrans, maximum_value: float = 1.0):
+        raw = _rgb(raw_rgb, "raw_buffer")
+        h, w, _ = raw.shape
+        out = np.empty_like(raw)
+        xt = np.ascontiguousarray(np.asarray(xtrans).reshape(-1), np.int8)
+        _check(native().lre_hip_xtrans3(self._h, _p(raw), _p(out), w, h, _p(xt, _i8p), maximum_value))
         return out
 
     def tone_mapping(self, rgb, after_scale: float = 1.0):

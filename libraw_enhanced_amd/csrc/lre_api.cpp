@@ -212,6 +212,17 @@ int lre_hip_dev_amaze(lre_ctx *ctx, const float *d_cfa, float *d_rgb, int w, int
                             ctx->scratch, lre_pick_stream(ctx, stream));
 }
 
+int lre_hip_dev_xtrans3(lre_ctx *ctx, const float *d_cfa, float *d_rgb, int w, int h,
+                        const signed char xtrans[36], float maximum_value, void *stream) {
+    if (!ctx || !d_cfa || !d_rgb || !xtrans) return set_arg_error("null pointer");
+    if (!valid_dims(w, h)) return set_arg_error("image size");
+    if (!(maximum_value > 0.f)) return set_arg_error("maximum_value must be > 0");
+    int rc = lre_ensure_scratch(ctx, lre_xtrans_scratch_bytes(ctx, w, h));
+    if (rc) return rc;
+    return lre_launch_xtrans3(ctx, d_cfa, d_rgb, w, h, xtrans, maximum_value, ctx->scratch,
+                              lre_pick_stream(ctx, stream));
+}
+
 int lre_hip_dev_tone_mapping(lre_ctx *ctx, const float *d_in, float *d_out, int w, int h,
                              float after_scale, void *stream) {
     if (!ctx || !d_in || !d_out) return set_arg_error("null pointer");
@@ -298,6 +309,27 @@ int lre_hip_amaze(lre_ctx *ctx, const float *raw_rgb, float *rgb, int w, int h,
     LRE_HIP_CHECK(hipMemcpyAsync(d_out, d_in, n * 12, hipMemcpyDeviceToDevice, ctx->stream));
     if ((rc = lre_launch_amaze(ctx, d_cfa, d_out, w, h, filters, cam_mul, maximum_value, d_ws,
                                ctx->stream)))
+        return rc;
+    return copy_out(ctx, rgb, d_out, n * 12);
+}
+
+int lre_hip_xtrans3(lre_ctx *ctx, const float *raw_rgb, float *rgb, int w, int h,
+                    const signed char xtrans[36], float maximum_value) {
+    if (!ctx || !raw_rgb || !rgb || !xtrans) return set_arg_error("null pointer");
+    if (!valid_dims(w, h)) return set_arg_error("image size");
+    if (!(maximum_value > 0.f)) return set_arg_error("maximum_value must be > 0");
+    LRE_HIP_CHECK(hipSetDevice(ctx->device));
+    const size_t n = (size_t)w * h;
+    const size_t io = aligned(n * 3 * 4) * 2 + aligned(n * 4);
+    int rc = lre_ensure_scratch(ctx, io + lre_xtrans_scratch_bytes(ctx, w, h));
+    if (rc) return rc;
+    Carve c(ctx->scratch);
+    float *d_in = c.take<float>(n * 3), *d_out = c.take<float>(n * 3), *d_cfa = c.take<float>(n);
+    void *d_ws = static_cast<char *>(ctx->scratch) + c.off;
+    if ((rc = copy_in(ctx, d_in, raw_rgb, n * 12))) return rc;
+    if ((rc = lre_launch_rgb3_to_cfa(d_in, d_cfa, w, h, 9, xtrans, ctx->stream))) return rc;
+    LRE_HIP_CHECK(hipMemcpyAsync(d_out, d_in, n * 12, hipMemcpyDeviceToDevice, ctx->stream));
+    if ((rc = lre_launch_xtrans3(ctx, d_cfa, d_out, w, h, xtrans, maximum_value, d_ws, ctx->stream)))
         return rc;
     return copy_out(ctx, rgb, d_out, n * 12);
 }

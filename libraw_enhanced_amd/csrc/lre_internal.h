@@ -33,6 +33,7 @@ size_t lre_amaze_scratch_bytes(lre_ctx *ctx, int w, int h);
 size_t lre_detail_tm_scratch_bytes(int w, int h);
 size_t lre_defringe_scratch_bytes(int w, int h);
 size_t lre_axial_scratch_bytes(int w, int h);
+size_t lre_xtrans_scratch_bytes(lre_ctx *ctx, int w, int h);
 size_t lre_lateral_scratch_bytes(int w, int h, int cell_size);
 
 // Kernel launchers (device pointers; scratch must already be large enough).
@@ -49,6 +50,9 @@ int lre_launch_defringe(const float *d_in, float *d_out, int w, int h, float rad
                         float edge_threshold, float chroma_threshold, float strength,
                         int defringe_green, float green_strength_scale, void *d_scratch,
                         hipStream_t s);
+int lre_launch_xtrans3(lre_ctx *ctx, const float *d_cfa, float *d_rgb, int w, int h,
+                       const signed char xtrans36[36], float maximum_value, void *d_scratch,
+                       hipStream_t s);
 int lre_launch_axial_ca(const float *d_in, float *d_out, int w, int h, int radius, float epsilon,
                         float strength, void *d_scratch, hipStream_t s);
 int lre_launch_lateral_ca(const float *d_in, float *d_out, int w, int h, int cell_size,

@@ -1,0 +1,508 @@
+// X-Trans 3-pass (Markesteijn) demosaic for gfx950
+// (core/cpu_accelerator.cpp:1870-2676, border :1806-1861).
+//
+// Each workgroup processes one reference tile (114 x 114, step 98, first tile
+// at (3,3)) in a private arena holding the 8 direction buffers, the derivative
+// and homogeneity maps and the green bounds.  Every step of the reference is a
+// loop over one class of CFA sites whose iterations neither read nor write each
+// other's cells (checked on the X-Trans pattern), so each step runs as one
+// parallel pass over its sites with a barrier after it.  Site positions along a
+// row follow the reference's column walks in closed form: a stride of 3, or
+// strides alternating between 1 and 2.  The green bounds keep the reference's
+// per-row walk (one thread per row), because pairs of columns share a slot and
+// the last write wins.  The homogeneity map is cleared per tile, so rows the
+// tile does not write read as zero (the reference's per-thread buffers keep
+// the previous tile's values there).
+#include <cfloat>
+
+#include "lre_common.h"
+#include "lre_internal.h"
+
+namespace {
+
+constexpr int TS = 114;
+constexpr int BLOCK = 256;
+constexpr int LW = TS - 8, DW = TS - 10;
+constexpr size_t DIRSZ = (size_t)TS * TS * 3;  // floats per direction buffer
+
+// arena layout (bytes)
+constexpr size_t A_RGB = 0;
+constexpr size_t A_DRV = A_RGB + 8 * DIRSZ * 4;
+constexpr size_t A_GM = A_DRV + (size_t)8 * DW * DW * 4;
+constexpr size_t A_HOMO = A_GM + (size_t)TS * (TS / 2) * 8;
+constexpr size_t A_HSUM = A_HOMO + (size_t)8 * TS * TS;
+constexpr size_t A_HMAX = A_HSUM + (size_t)8 * TS * TS;
+constexpr size_t ARENA = ((A_HMAX + (size_t)TS * TS) + 255) & ~(size_t)255;
+
+struct XArgs {
+    const float *cfa;  // CFA plane (one sample per site)
+    float *out;
+    int W, H;
+    float maxv;
+    signed char xt[6][6];
+    short hex[2][3][3][8];
+    int sgrow, sgcol;
+    int rshift[3];
+    int ntc, ntiles;
+    char *arenas;
+};
+
+LRE_DEV int fcolx(const XArgs &A, int r, int c) { return A.xt[(r + 6) % 6][(c + 6) % 6]; }
+LRE_DEV int isgreen(const XArgs &A, int r, int c) { return A.xt[r % 3][c % 3] & 1; }
+// raw sample / maximum (division, as the reference), at a linear offset
+LRE_DEV float rawv(const XArgs &A, int r, int c) { return A.cfa[(size_t)r * A.W + c] / A.maxv; }
+LRE_DEV float rawhex(const XArgs &A, int r, int c, int hex) {
+    return A.cfa[(size_t)(r * A.W + c + hex)] / A.maxv;
+}
+LRE_DEV float limf(float v, float lo, float hi) {
+    float m = (hi < v) ? hi : v;
+    return (lo < m) ? m : lo;
+}
+LRE_DEV float sq(float x) { return x * x; }
+
+struct MM { float min, max; };
+
+// first column >= c0 (and < lim) that is not green
+LRE_DEV int first_nongreen(const XArgs &A, int row, int c0, int lim) {
+    int c = c0;
+    for (; c < lim; c++)
+        if (!isgreen(A, row, c)) break;
+    return c;
+}
+
+// k-th site of a walk starting at ls with stride pattern: fixed 3, or
+// alternating c0, c0^3, ...
+LRE_DEV int walk_col(int ls, int coff0, int k) {
+    if (coff0 == 3) return ls + 3 * k;
+    return ls + 3 * (k >> 1) + ((k & 1) ? coff0 : 0);
+}
+
+__device__ void xtrans_tile(const XArgs &A, char *ar, int top, int left) {
+    const int W = A.W, H = A.H;
+    float *rgb0 = reinterpret_cast<float *>(ar + A_RGB);
+    float *drv = reinterpret_cast<float *>(ar + A_DRV);
+    MM *gm = reinterpret_cast<MM *>(ar + A_GM);
+    unsigned char *homo = reinterpret_cast<unsigned char *>(ar + A_HOMO);
+    unsigned char *hsum = reinterpret_cast<unsigned char *>(ar + A_HSUM);
+    unsigned char *hmax = reinterpret_cast<unsigned char *>(ar + A_HMAX);
+    const int mrow0 = min(top + TS, H - 3), mcol0 = min(left + TS, W - 3);
+    const int nr = mrow0 - top, nc = mcol0 - left;
+#define RGB(base, d, r, c) ((base) + ((size_t)(d) * DIRSZ + ((size_t)(r) * TS + (c)) * 3))
+#define GM(r, c) gm[(size_t)(r) * (TS / 2) + (c)]
+
+    // clear the homogeneity map (rows the tile leaves unwritten read as zero)
+    for (int t = threadIdx.x; t < 8 * TS * TS / 4; t += BLOCK) reinterpret_cast<unsigned *>(homo)[t] = 0u;
+
+    // green min/max (:1973-2059): one thread per row, the reference's walk
+    for (int row = top + (int)threadIdx.x; row < mrow0; row += BLOCK) {
+        const int ls = first_nongreen(A, row, left, mcol0);
+        const int coff = A.rshift[row % 3] == 1 ? 3 : 1 + (fcolx(A, row, ls + 1) & 1);
+        auto mmhex = [&](int col, const short *hx) {
+            float lo = FLT_MAX, hi = 0.f;
+            for (int c = 0; c < 6; c++) {
+                const float v = rawhex(A, row, col, hx[c]);
+                lo = lo < v ? lo : v;
+                hi = hi > v ? hi : v;
+            }
+            return MM{lo, hi};
+        };
+        if (coff == 3) {
+            const short *hx = A.hex[0][row % 3][ls % 3];
+            for (int col = ls; col < mcol0; col += 3) GM(row - top, (col - left) >> 1) = mmhex(col, hx);
+        } else {
+            int col = ls;
+            if (coff == 2) {
+                GM(row - top, (col - left) >> 1) = mmhex(col, A.hex[0][row % 3][col % 3]);
+                col += 2;
+            }
+            const short *hx = A.hex[0][row % 3][col % 3];
+            for (; col < mcol0 - 1; col += 3) {
+                const MM m = mmhex(col, hx);
+                GM(row - top, (col - left) >> 1) = m;
+                GM(row - top, (col + 1 - left) >> 1) = m;
+            }
+            if (col < mcol0) GM(row - top, (col - left) >> 1) = mmhex(col, hx);
+        }
+    }
+    // direction buffers 0-3: the raw sample at its colour, zero elsewhere; the
+    // reference copies buffer 0 into 1-3 one float short, so that last float is 0
+    for (int t = threadIdx.x; t < TS * TS; t += BLOCK) {
+        const int r = t / TS, c = t % TS;
+        float v[3] = {0.f, 0.f, 0.f};
+        if (r < nr && c < nc) v[fcolx(A, r + top, c + left)] = rawv(A, r + top, c + left);
+        for (int d = 0; d < 4; d++) {
+            float *p = RGB(rgb0, d, r, c);
+            p[0] = v[0];
+            p[1] = v[1];
+            p[2] = (d > 0 && t == TS * TS - 1) ? 0.f : v[2];
+        }
+    }
+    __syncthreads();
+
+    // green along four directions (:2075-2128)
+    for (int t = threadIdx.x; t < nr * TS; t += BLOCK) {
+        const int row = top + t / TS, k = t % TS;
+        const int ls = first_nongreen(A, row, left, mcol0);
+        const int coff0 = A.rshift[row % 3] == 1 ? 3 : 1 + (fcolx(A, row, ls + 1) & 1);
+        const int col = walk_col(ls, coff0, k);
+        if (col >= mcol0) continue;
+        const short *hx = coff0 == 3 ? A.hex[0][row % 3][ls % 3]
+                                     : A.hex[0][row % 3][((k & 1) ? ls + coff0 : ls) % 3];
+        const int swap = coff0 == 3 ? 0 : 1;
+        float color[4];
+        color[0] = 0.6796875f * (rawhex(A, row, col, hx[1]) + rawhex(A, row, col, hx[0])) -
+                   0.1796875f * (rawhex(A, row, col, 2 * hx[1]) + rawhex(A, row, col, 2 * hx[0]));
+        color[1] = 0.87109375f * rawhex(A, row, col, hx[3]) + rawhex(A, row, col, hx[2]) * 0.12890625f +
+                   0.359375f * (rawhex(A, row, col, 0) - rawhex(A, row, col, -hx[2]));
+        for (int c = 0; c < 2; c++)
+            color[2 + c] = 0.640625f * rawhex(A, row, col, hx[4 + c]) + 0.359375f * rawhex(A, row, col, -2 * hx[4 + c]) +
+                           0.12890625f * (2.f * rawhex(A, row, col, 0) - rawhex(A, row, col, 3 * hx[4 + c]) -
+                                          rawhex(A, row, col, -3 * hx[4 + c]));
+        const MM m = GM(row - top, (col - left) >> 1);
+        for (int c = 0; c < 4; c++) RGB(rgb0, c ^ swap, row - top, col - left)[1] = limf(color[c], m.min, m.max);
+    }
+    __syncthreads();
+
+    float *rgb = rgb0;
+    for (int pass = 0; pass < 3; pass++) {
+        if (pass == 1) {
+            // buffers 0-3 copied into 4-7 one float short; later passes refine 4-7
+            float *dst = rgb0 + 4 * DIRSZ;
+            const int n4 = (int)(4 * DIRSZ);
+            for (int t = threadIdx.x; t < n4; t += BLOCK) dst[t] = (t == n4 - 1) ? 0.f : rgb0[t];
+            __syncthreads();
+            rgb = rgb0 + 4 * DIRSZ;
+        }
+        if (pass) {  // green from closer pixels (:2141-2182)
+            const int r0 = top + 2, r1 = mrow0 - 2;
+            for (int t = threadIdx.x; t < max(r1 - r0, 0) * TS; t += BLOCK) {
+                const int row = r0 + t / TS, k = t % TS;
+                const int ls = first_nongreen(A, row, left + 2, mcol0 - 2);
+                const int coff0 = A.rshift[row % 3] == 1 ? 3 : 1 + (fcolx(A, row, ls + 1) & 1);
+                const int col = walk_col(ls, coff0, k);
+                if (col >= mcol0 - 2) continue;
+                const int f = fcolx(A, row, col);
+                const short *hx = coff0 == 3 ? A.hex[1][row % 3][ls % 3]
+                                             : A.hex[1][row % 3][((k & 1) ? ls + coff0 : ls) % 3];
+                const MM m = GM(row - top, (col - left) >> 1);
+                for (int d = 3; d < 6; d++) {
+                    const int dd = coff0 == 3 ? d - 2 : (d - 2) ^ 1;
+                    float *rix = RGB(rgb, dd, row - top, col - left);
+                    const int o1 = 3 * hx[d], o2 = 3 * (-2 * hx[d]);
+                    const float val = 0.33333333f * (rix[o2 + 1] + 2 * (rix[o1 + 1] - rix[o1 + f]) - rix[o2 + f]) + rix[f];
+                    rix[1] = limf(val, m.min, m.max);
+                }
+            }
+            __syncthreads();
+        }
+        // R and B at solitary green sites (:2184-2222)
+        {
+            const int rstart = (top - A.sgrow + 4) / 3 * 3 + A.sgrow;
+            const int cstart = (left - A.sgcol + 4) / 3 * 3 + A.sgcol;
+            const int nrs = mrow0 - 2 > rstart ? (mrow0 - 2 - rstart + 2) / 3 : 0;
+            const int ncs = mcol0 - 2 > cstart ? (mcol0 - 2 - cstart + 2) / 3 : 0;
+            for (int t = threadIdx.x; t < nrs * ncs; t += BLOCK) {
+                const int row = rstart + 3 * (t / ncs), col = cstart + 3 * (t % ncs);
+                int h = fcolx(A, row, cstart + 1) ^ (((t % ncs) & 1) ? 2 : 0);
+                float *rix = RGB(rgb, 0, row - top, col - left);
+                float diff[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                float color[3][6];
+                for (int i = 1, d = 0; d < 6; d++, i ^= TS ^ 1, h ^= 2) {
+                    for (int c = 0; c < 2; c++, h ^= 2) {
+                        const int o = i << c;
+                        const float g = rix[1] + rix[1] - rix[3 * o + 1] - rix[3 * -o + 1];
+                        color[h][d] = g + rix[3 * o + h] + rix[3 * -o + h];
+                        if (d > 1) diff[d] += sq(rix[3 * o + 1] - rix[3 * -o + 1] - rix[3 * o + h] + rix[3 * -o + h]) + sq(g);
+                    }
+                    if (d > 2 && (d & 1))
+                        if (diff[d - 1] < diff[d])
+                            for (int c = 0; c < 2; c++) color[c * 2][d] = color[c * 2][d - 1];
+                    if ((d & 1) || d < 2) {
+                        for (int c = 0; c < 2; c++) rix[c * 2] = 0.5f * color[c * 2][d];
+                        rix += DIRSZ;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // R at B sites and B at R sites (:2224-2266)
+        {
+            const int r0 = top + 3, r1 = mrow0 - 3;
+            for (int t = threadIdx.x; t < max(r1 - r0, 0) * TS; t += BLOCK) {
+                const int row = r0 + t / TS, k = t % TS;
+                const int ls = first_nongreen(A, row, left + 3, mcol0 - 1);
+                int coff0 = A.rshift[row % 3] == 1 ? 3 : (fcolx(A, row, ls + 1) == 1 ? 2 : 1);
+                const int col = walk_col(ls, coff0, k);
+                if (col >= mcol0 - 3) continue;
+                const int c = ((row - A.sgrow) % 3) ? TS : 1;
+                const int h = 3 * (c ^ TS ^ 1);
+                const int f = 2 - fcolx(A, row, col);
+                float *rix = RGB(rgb, 0, row - top, col - left);
+                for (int d = 0; d < 4; d++, rix += DIRSZ) {
+                    const int i = d > 1 || ((d ^ c) & 1) ||
+                                          ((fabsf(rix[1] - rix[3 * c + 1]) + fabsf(rix[1] - rix[3 * -c + 1])) <
+                                           2.f * (fabsf(rix[1] - rix[3 * h + 1]) + fabsf(rix[1] - rix[3 * -h + 1])))
+                                      ? c : h;
+                    rix[f] = rix[1] + 0.5f * (rix[3 * i + f] + rix[3 * -i + f] - rix[3 * i + 1] - rix[3 * -i + 1]);
+                }
+            }
+        }
+        __syncthreads();
+        // R and B for 2x2 green blocks (:2268-2314)
+        {
+            int topstart = top + 2;
+            for (; topstart < mrow0 - 2; topstart++)
+                if ((topstart - A.sgrow) % 3) break;
+            int ls = left + 2;
+            for (; ls < mcol0 - 2; ls++)
+                if ((ls - A.sgcol) % 3) break;
+            const int coffstart = 2 - (fcolx(A, topstart, ls + 1) & 1);
+            const int r0 = topstart, r1 = mrow0 - 2;
+            for (int t = threadIdx.x; t < max(r1 - r0, 0) * TS; t += BLOCK) {
+                const int row = r0 + t / TS, k = t % TS;
+                if (!((row - A.sgrow) % 3)) continue;
+                const int col = walk_col(ls, coffstart, k);
+                if (col >= mcol0 - 2) continue;
+                const short *hx = A.hex[1][row % 3][((k & 1) ? ls + coffstart : ls) % 3];
+                float *rix = RGB(rgb, 0, row - top, col - left);
+                for (int d = 0; d < 8; d += 2, rix += DIRSZ) {
+                    if (hx[d] + hx[d + 1]) {
+                        const float g = 3 * rix[1] - 2 * rix[3 * hx[d] + 1] - rix[3 * hx[d + 1] + 1];
+                        for (int c = 0; c < 4; c += 2)
+                            rix[c] = (g + 2 * rix[3 * hx[d] + c] + rix[3 * hx[d + 1] + c]) * 0.33333333f;
+                    } else {
+                        const float g = 2 * rix[1] - rix[3 * hx[d] + 1] - rix[3 * hx[d + 1] + 1];
+                        for (int c = 0; c < 4; c += 2) rix[c] = (g + rix[3 * hx[d] + c] + rix[3 * hx[d + 1] + c]) * 0.5f;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // YPbPr derivatives (:2346-2419), luma/chroma evaluated where needed
+    int mrow = mrow0 - top, mcol = mcol0 - left;
+    {
+        const int rows = max(mrow - 10, 0), cols = max(mcol - 10, 0);
+        for (int t = threadIdx.x; t < 8 * rows * cols; t += BLOCK) {
+            const int d = t / (rows * cols), u = t % (rows * cols);
+            const int row = 5 + u / cols, col = 5 + u % cols;
+            const int fo = (d & 3) == 0 ? 1 : (d & 3) == 1 ? TS : (d & 3) == 2 ? TS + 1 : TS - 1;  // :1878
+            auto ypp = [&](int r, int c, float &y, float &pb, float &pr) {
+                const float *p = RGB(rgb0, d, r, c);
+                y = 0.2627f * p[0] + 0.6780f * p[1] + 0.0593f * p[2];
+                pb = (p[2] - y) * 0.56433f;
+                pr = (p[0] - y) * 0.67815f;
+            };
+            // f offsets in the reference index a (TS-8)-wide plane; here rows are TS wide
+            const int dr = fo == 1 ? 0 : 1, dc = fo == 1 ? 1 : fo - TS;
+            float y0, u0, v0, y1, u1, v1, y2, u2, v2;
+            ypp(row, col, y0, u0, v0);
+            ypp(row + dr, col + dc, y1, u1, v1);
+            ypp(row - dr, col - dc, y2, u2, v2);
+            drv[(size_t)d * DW * DW + (size_t)(row - 5) * DW + col - 5] =
+                sq(2 * y0 - y1 - y2) + sq(2 * u0 - u1 - u2) + sq(2 * v0 - v1 - v2);
+        }
+    }
+    __syncthreads();
+#define DRV(d, r, c) drv[(size_t)(d) * DW * DW + (size_t)(r) * DW + (c)]
+#define HOMO(d, r, c) homo[((size_t)(d) * TS + (r)) * TS + (c)]
+#define HSUM(d, r, c) hsum[((size_t)(d) * TS + (r)) * TS + (c)]
+    {
+        const int rows = max(mrow - 12, 0), cols = max(mcol - 12, 0);
+        for (int t = threadIdx.x; t < rows * cols; t += BLOCK) {
+            const int row = 6 + t / cols, col = 6 + t % cols;
+            float tr = DRV(0, row - 5, col - 5) < DRV(1, row - 5, col - 5) ? DRV(0, row - 5, col - 5) : DRV(1, row - 5, col - 5);
+            for (int d = 2; d < 8; d++) tr = DRV(d, row - 5, col - 5) < tr ? DRV(d, row - 5, col - 5) : tr;
+            tr *= 8;
+            for (int d = 0; d < 8; d++) {
+                unsigned char s = 0;
+                for (int v = -1; v <= 1; v++)
+                    for (int h = -1; h <= 1; h++) s += (DRV(d, row + v - 5, col + h - 5) <= tr ? 1 : 0);
+                HOMO(d, row, col) = s;
+            }
+        }
+    }
+    __syncthreads();
+    if (H - top < TS + 4) mrow = H - top + 2;
+    if (W - left < TS + 4) mcol = W - left + 2;
+    const int rs = top < 8 ? top : 8, cs = left < 8 ? left : 8;
+    {
+        const int rows = max(mrow - 8 - rs, 0), cols = max(mcol - 8 - cs, 0);
+        for (int t = threadIdx.x; t < rows * cols; t += BLOCK) {
+            const int row = rs + t / cols, col = cs + t % cols;
+            unsigned char m = 0;
+            for (int d = 0; d < 8; d++) {
+                int s = 0;  // exact integer sums: the running form of the reference gives the same
+                for (int v = -2; v <= 2; v++)
+                    for (int h = -2; h <= 2; h++) s += HOMO(d, row + v, col + h);
+                const unsigned char us = (unsigned char)s;
+                HSUM(d, row, col) = us;
+                m = d == 0 ? us : (m < us ? us : m);
+            }
+            m -= m >> 3;
+            hmax[(size_t)row * TS + col] = m;
+        }
+    }
+    __syncthreads();
+    {
+        const int rows = max(mrow - 8 - rs, 0), cols = max(mcol - 8 - cs, 0);
+        for (int t = threadIdx.x; t < rows * cols; t += BLOCK) {
+            const int row = rs + t / cols, col = cs + t % cols;
+            unsigned char hm[8];
+            for (int d = 0; d < 4; d++) hm[d] = HSUM(d, row, col);
+            for (int d = 4; d < 8; d++) {
+                hm[d] = HSUM(d, row, col);
+                if (hm[d - 4] < hm[d]) hm[d - 4] = 0;
+                else if (hm[d - 4] > hm[d]) hm[d] = 0;
+            }
+            float avg[4] = {0.f, 0.f, 0.f, 0.f};
+            const unsigned char mx = hmax[(size_t)row * TS + col];
+            for (int d = 0; d < 8; d++)
+                if (hm[d] >= mx) {
+                    const float *p = RGB(rgb0, d, row, col);
+                    for (int c = 0; c < 3; c++) avg[c] += p[c];
+                    avg[3] += 1.f;
+                }
+            float *o = A.out + ((size_t)(row + top) * W + col + left) * 3;
+            const float a0 = avg[0] / avg[3], a1 = avg[1] / avg[3], a2 = avg[2] / avg[3];
+            o[0] = (0.f < a0) ? a0 : 0.f;
+            o[1] = (0.f < a1) ? a1 : 0.f;
+            o[2] = (0.f < a2) ? a2 : 0.f;
+        }
+    }
+    __syncthreads();
+#undef DRV
+#undef HOMO
+#undef HSUM
+#undef RGB
+#undef GM
+}
+
+__global__ __launch_bounds__(BLOCK) void xtrans_kernel(XArgs A) {
+    char *ar = A.arenas + (size_t)blockIdx.x * ARENA;
+    for (int t = blockIdx.x; t < A.ntiles; t += gridDim.x)
+        xtrans_tile(A, ar, 3 + (t / A.ntc) * (TS - 16), 3 + (t % A.ntc) * (TS - 16));
+}
+
+// weighted 3x3 border interpolation (:1806-1861) on the 8-px frame
+__global__ void xtrans_border(XArgs A, int border) {
+    const int W = A.W, H = A.H;
+    const size_t n = (size_t)W * H;
+    const float wt[3][3] = {{0.25f, 0.5f, 0.25f}, {0.5f, 0.f, 0.5f}, {0.25f, 0.5f, 0.25f}};
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int row = (int)(i / W), col = (int)(i % W);
+        if (col >= border && col < W - border && row >= border && row < H - border) continue;
+        float sum[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        for (int y = row > 0 ? row - 1 : 0, v = row == 0 ? 0 : -1; y < row + 2 && y < H; y++, v++)
+            for (int x = col > 0 ? col - 1 : 0, h = col == 0 ? 0 : -1; x < col + 2 && x < W; x++, h++) {
+                const int f = fcolx(A, y, x);
+                sum[f] += rawv(A, y, x) * wt[v + 1][h + 1];
+                sum[f + 3] += wt[v + 1][h + 1];
+            }
+        float *o = A.out + i * 3;
+        switch (fcolx(A, row, col)) {
+        case 0:
+            o[0] = rawv(A, row, col);
+            o[1] = sum[1] / sum[4];
+            o[2] = sum[2] / sum[5];
+            break;
+        case 1:
+            if (sum[3] == 0.f) {
+                const float v = rawv(A, row, col);
+                o[0] = o[1] = o[2] = v;
+            } else {
+                o[0] = sum[0] / sum[3];
+                o[1] = rawv(A, row, col);
+                o[2] = sum[2] / sum[5];
+            }
+            break;
+        default:
+            o[0] = sum[0] / sum[3];
+            o[1] = sum[1] / sum[4];
+            o[2] = rawv(A, row, col);
+        }
+    }
+}
+
+int tiles_for(int n) {
+    int k = 0;
+    for (int o = 3; o < n - 19; o += TS - 16) k++;
+    return k;
+}
+
+void setup(XArgs &A, const signed char *xt36) {
+    for (int i = 0; i < 36; i++) A.xt[i / 6][i % 6] = xt36[i];
+    auto isg = [&](int r, int c) { return A.xt[r % 3][c % 3] & 1; };
+    static const short orth[12] = {1, 0, 0, 1, -1, 0, 0, -1, 1, 0, 0, 1};
+    static const short patt[2][16] = {{0, 1, 0, -1, 2, 0, -1, 0, 1, 1, 1, -1, 0, 0, 0, 0},
+                                      {0, 1, 0, -2, 1, 0, -2, 0, 1, 1, -2, -2, 1, -1, -1, 1}};
+    A.sgrow = A.sgcol = 0;
+    for (int a = 0; a < 2; a++)
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++)
+                for (int k = 0; k < 8; k++) A.hex[a][r][c][k] = 0;
+    for (int row = 0; row < 3; row++)
+        for (int col = 0; col < 3; col++) {
+            const int gint = isg(row, col);
+            int ng = 0;
+            for (int d = 0; d < 10; d += 2) {
+                if (isg(row + orth[d] + 6, col + orth[d + 2] + 6)) ng = 0;
+                else ng++;
+                if (ng == 4) { A.sgrow = row; A.sgcol = col; }
+                if (ng == gint + 1)
+                    for (int c = 0; c < 8; c++) {
+                        const int v = orth[d] * patt[gint][c * 2] + orth[d + 1] * patt[gint][c * 2 + 1];
+                        const int h = orth[d + 2] * patt[gint][c * 2] + orth[d + 3] * patt[gint][c * 2 + 1];
+                        A.hex[0][row][col][c ^ (gint * 2 & d)] = (short)(h + v * A.W);
+                        A.hex[1][row][col][c ^ (gint * 2 & d)] = (short)(h + v * TS);
+                    }
+            }
+        }
+    for (int row = 0; row < 3; row++) {
+        int gc = 0;
+        for (int col = 0; col < 3; col++) gc += isg(row, col);
+        A.rshift[row] = gc == 2;
+    }
+}
+
+}  // namespace
+
+static int xtrans_slots(lre_ctx *ctx, int w, int h) {
+    const int ntiles = tiles_for(w) * tiles_for(h);
+    const int cap = (ctx->num_cus > 0 ? ctx->num_cus : 256) * 6;
+    return ntiles < cap ? (ntiles > 0 ? ntiles : 1) : cap;
+}
+
+size_t lre_xtrans_scratch_bytes(lre_ctx *ctx, int w, int h) {
+    return (size_t)(xtrans_slots(ctx, w, h) + 1) * ARENA;
+}
+
+int lre_launch_xtrans3(lre_ctx *ctx, const float *d_cfa, float *d_rgb, int w, int h,
+                       const signed char xtrans36[36], float maximum_value, void *d_scratch,
+                       hipStream_t s) {
+    if (2 * w + 2 > 32767 || h < 1) {
+        lre_set_error("X-Trans width %d exceeds the 16-bit hexagon offsets", w);
+        return LRE_ERR_ARG;
+    }
+    XArgs A;
+    A.cfa = d_cfa;
+    A.out = d_rgb;
+    A.W = w;
+    A.H = h;
+    A.maxv = maximum_value;
+    setup(A, xtrans36);
+    A.ntc = tiles_for(w);
+    A.ntiles = A.ntc * tiles_for(h);
+    A.arenas = static_cast<char *>(d_scratch);
+    if (A.ntiles > 0) {
+        hipLaunchKernelGGL(xtrans_kernel, dim3(xtrans_slots(ctx, w, h)), dim3(BLOCK), 0, s, A);
+        LRE_HIP_CHECK(hipGetLastError());
+    }
+    const size_t frame = (size_t)w * h;
+    size_t blocks = (frame + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(xtrans_border, dim3((unsigned)blocks), dim3(256), 0, s, A, 8);
+    LRE_HIP_CHECK(hipGetLastError());
+    return LRE_OK;
+}

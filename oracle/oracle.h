@@ -26,6 +26,8 @@ enum { ORACLE_TILES_CLEAN = 0, ORACLE_TILES_SEQUENTIAL = 1 };
 
 int oracle_amaze(const float *raw3, float *rgb3, int w, int h, uint32_t filters,
                  const float *cam_mul4, float maximum_value, int tile_mode);
+int oracle_xtrans3(const float *raw3, float *rgb3, int w, int h, const signed char *xtrans36,
+                   float maximum_value, int tile_mode);
 void oracle_border_interpolate(float *rgb3, int w, int h, uint32_t filters, int border);
 int oracle_tone_mapping(const float *in3, float *out3, int w, int h, float after_scale);
 int oracle_detail_tonemap(float *rgb3, int w, int h);

@@ -163,3 +163,17 @@ def test_lateral_ca_tolerance(acc, kw):
     err = float(np.abs(got - want).max())
     print(f"lateral {kw}: max-abs {err:.3g}, {int(np.count_nonzero(got != want))} values differ")
     assert err <= 2e-4 and psnr(got, want) >= 70.0
+
+
+# ---------------- X-Trans ----------------
+from test_xtrans_cpu import oracle_xtrans as _oracle_xtrans, xtrans_raw3 as _xtrans_raw3  # noqa: E402
+
+
+@pytest.mark.parametrize("h,w", [(120, 150), (300, 500), (640, 960), (1040, 1560), (777, 1001)])
+def test_xtrans_bit_exact(acc, h, w):
+    raw3 = _xtrans_raw3(h, w, scene.BASE_SEED + w)
+    print(f"xtrans {h}x{w}", flush=True)
+    got = acc.demosaic_xtrans_3pass(raw3, scene.XTRANS, 1.0)
+    want = _oracle_xtrans(raw3)
+    bad = np.any(got != want, axis=2)
+    assert not bad.any(), f"{int(bad.sum())} px differ, max-abs {np.abs(got - want).max()}"
