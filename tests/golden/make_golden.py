@@ -69,6 +69,32 @@ def main():
         xtv = (xt if xt is not None else scene.XTRANS).reshape(-1).astype(np.int8)
         assert ref.ref_white_balance(P(img4, U16P), P(res), 36, 24, P(wb), ctypes.c_uint32(filters), P(xtv, I8P)) == 0
         out[name + "/img4"], out[name + "/out"] = img4, res
+    # defringe, in and out of place, on an AMaZE output (the AMaZE fixture above)
+    img = out["amaze_rggb_64x96/out"] * 1.8
+    for name, inplace, params in (("defringe_oop_64x96", False, (10.0, 0.1, 0.15, 10.0, 0, 0.3)),
+                                  ("defringe_inplace_green_64x96", True, (6.0, 0.1, 0.15, 2.0, 1, 0.5))):
+        a = img.copy()
+        res = a if inplace else np.zeros_like(a)
+        r_, e_, c_, s_, g_, gs_ = params
+        assert ref.ref_defringe(P(a), P(res), 96, 64, ctypes.c_float(r_), ctypes.c_float(e_), ctypes.c_float(c_),
+                                ctypes.c_float(s_), g_, ctypes.c_float(gs_)) == 0
+        out[name + "/in"], out[name + "/out"] = img, res
+        out[name + "/params"] = np.array([r_, e_, c_, s_, g_, gs_, int(inplace)], np.float64)
+    # chromatic aberration on a radially pre-warped scene
+    rgb = scene.lateral_warp(scene.render(96, 128, scene.BASE_SEED + 6), 1.003, 0.997)
+    res = np.zeros_like(rgb)
+    assert ref.ref_axial_ca(P(rgb), P(res), 128, 96, 6, ctypes.c_float(1e-4), ctypes.c_float(0.3)) == 0
+    out["axial_96x128/in"], out["axial_96x128/out"] = rgb, res
+    res = np.zeros_like(rgb)
+    assert ref.ref_lateral_ca(P(rgb), P(res), 128, 96, 32, 3, ctypes.c_float(6.0), ctypes.c_float(0.02), 2) == 0
+    out["lateral_96x128/in"], out["lateral_96x128/out"] = rgb, res
+    # X-Trans 3-pass on a single tile
+    xt = scene.XTRANS.reshape(-1).astype(np.int8).copy()
+    cfa = scene.xtrans_scene(96, 120, scene.BASE_SEED + 7)
+    raw3 = scene.cfa_to_rgb3(cfa, 0, scene.XTRANS)
+    res = np.zeros_like(raw3)
+    assert ref.ref_xtrans3(P(raw3), P(res), 120, 96, P(xt, I8P), ctypes.c_float(1.0)) == 0
+    out["xtrans_96x120/cfa"], out["xtrans_96x120/out"] = cfa, res
     path = os.path.join(HERE, "develop_golden.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, os.path.getsize(path), "bytes")

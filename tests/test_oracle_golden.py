@@ -48,6 +48,43 @@ def test_white_balance_golden(gold, name, filters):
     assert np.array_equal(got, gold[name + "/out"])
 
 
+@pytest.mark.parametrize("name", ["defringe_oop_64x96", "defringe_inplace_green_64x96"])
+def test_defringe_golden(gold, name):
+    import ctypes
+    from _common import oracle, p
+    r_, e_, c_, s_, g_, gs_, inplace = gold[name + "/params"]
+    a = gold[name + "/in"].copy()
+    res = a if inplace else np.zeros_like(a)
+    assert oracle().oracle_defringe(p(a), p(res), 96, 64, ctypes.c_float(r_), ctypes.c_float(e_),
+                                    ctypes.c_float(c_), ctypes.c_float(s_), int(g_), ctypes.c_float(gs_)) == 0
+    assert np.array_equal(res, gold[name + "/out"])
+
+
+def test_ca_golden(gold):
+    import ctypes
+    from _common import oracle, p
+    src = gold["axial_96x128/in"]
+    res = np.zeros_like(src)
+    assert oracle().oracle_axial_ca(p(src), p(res), 128, 96, 6, ctypes.c_float(1e-4), ctypes.c_float(0.3)) == 0
+    assert np.array_equal(res, gold["axial_96x128/out"])
+    src = gold["lateral_96x128/in"]
+    res = np.zeros_like(src)
+    assert oracle().oracle_lateral_ca(p(src), p(res), 128, 96, 32, 3, ctypes.c_float(6.0),
+                                      ctypes.c_float(0.02), 2) == 0
+    assert np.array_equal(res, gold["lateral_96x128/out"])
+
+
+def test_xtrans_golden(gold):
+    import ctypes
+    from _common import I8P, oracle, p
+    cfa = gold["xtrans_96x120/cfa"]
+    raw3 = scene.cfa_to_rgb3(cfa, 0, scene.XTRANS)
+    res = np.zeros_like(raw3)
+    xt = scene.XTRANS.reshape(-1).astype(np.int8).copy()
+    assert oracle().oracle_xtrans3(p(raw3), p(res), 120, 96, p(xt, I8P), ctypes.c_float(1.0), TILES_SEQUENTIAL) == 0
+    assert np.array_equal(res, gold["xtrans_96x120/out"])
+
+
 def test_amaze_flat_field_stays_flat():
     """A uniform grey CFA is reproduced on every channel (to the eps terms of the
     colour-ratio interpolation, ~1e-5 relative)."""
