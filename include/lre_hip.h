@@ -111,6 +111,15 @@ int lre_hip_lateral_ca(lre_ctx *ctx, const float *in_rgb, float *out_rgb, int wi
                        int cell_size, int max_iterations, float max_shift, float min_confidence,
                        int pyramid_levels);
 
+/* LibRawWrapper's apply_black_level_correction (core/libraw_wrapper.cpp:401-491)
+ * on the uint16 x 4 image, in place: every channel of every pixel becomes
+ * clamp(v - pos_black - cblack[c], 0, maximum) when a positional table
+ * (pos_rows x pos_cols, LibRaw's cblack[6..]) is given, else
+ * clamp(v - cblack[c], 0, maximum) when any cblack[c] is non-zero, else
+ * clamp(v - black, 0, maximum). */
+int lre_hip_black_level(lre_ctx *ctx, uint16_t *img4, int width, int height, const int cblack[4],
+                        const int *pos_table, int pos_rows, int pos_cols, int black, int maximum);
+
 /* Accelerator::apply_white_balance (core/accelerator.h:304-307,
  * core/cpu_accelerator.cpp:2789-2869): writes float(sample) * wb[c] to the
  * native channel of rgb and leaves the other two channels untouched.
@@ -142,6 +151,16 @@ int lre_hip_dev_axial_ca(lre_ctx *ctx, const float *d_in, float *d_out, int widt
 int lre_hip_dev_lateral_ca(lre_ctx *ctx, const float *d_in, float *d_out, int width, int height,
                            int cell_size, int max_iterations, float max_shift,
                            float min_confidence, int pyramid_levels, void *stream);
+/* Fused ingest of a uint16 CFA plane (one sample per site): black subtraction
+ * and clip as lre_hip_black_level (per-channel or global black, no positional
+ * table), white balance float(v) * wb[c] at the native channel (as
+ * apply_white_balance), into a float CFA plane.  When d_data_max is not NULL it
+ * receives the data maximum that adjust_maximum tests (the maximum of each
+ * pixel's fcol_bayer channel after white balance, core/libraw_wrapper.cpp:1037-1062). */
+int lre_hip_dev_ingest_u16(lre_ctx *ctx, const uint16_t *d_raw, float *d_cfa, int width,
+                           int height, uint32_t filters, const signed char xtrans[36],
+                           const int cblack[4], int black, int maximum, const float wb[4],
+                           float *d_data_max, void *stream);
 /* Demosaic input planes: extract the native channel of the reference's 3-channel
  * layout into a CFA plane. */
 int lre_hip_dev_rgb3_to_cfa(lre_ctx *ctx, const float *d_raw_rgb, float *d_cfa, int width,

@@ -70,6 +70,12 @@ _SIGS = {
     "lre_hip_dev_lateral_ca": ([_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                 ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int, _vp],
                                ctypes.c_int),
+    "lre_hip_black_level": ([_vp, _u16p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                             ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                             ctypes.c_int], ctypes.c_int),
+    "lre_hip_dev_ingest_u16": ([_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, _i8p,
+                                ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int, _f32p, _vp,
+                                _vp], ctypes.c_int),
     "lre_hip_white_balance": ([_vp, _u16p, _f32p, ctypes.c_int, ctypes.c_int, _f32p,
                                ctypes.c_uint32, _i8p], ctypes.c_int),
     "lre_hip_dev_amaze": ([_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, _f32p,
@@ -226,6 +232,23 @@ class HIPAccelerator:
                                            max_iterations, max_shift, min_confidence,
                                            pyramid_levels))
         return out
+
+    def apply_black_level_correction(self, img4, cblack=(0, 0, 0, 0), black: int = 0,
+                                     maximum: int = 16383, pos_table=None):
+        """In place on a (H, W, 4) uint16 image; returns it."""
+        if img4.dtype != np.uint16 or img4.ndim != 3 or img4.shape[2] != 4 or not img4.flags.c_contiguous:
+            raise LibRawInvalidArgument("raw_buffer must be a C-contiguous (H, W, 4) uint16 array")
+        h, w, _ = img4.shape
+        cb = (ctypes.c_int * 4)(*[int(v) for v in cblack])
+        if pos_table is not None:
+            pt = np.ascontiguousarray(pos_table, np.int32)
+            pr, pc = pt.shape
+            ptr = pt.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        else:
+            pr = pc = 0
+            ptr = None
+        _check(native().lre_hip_black_level(self._h, _p(img4, _u16p), w, h, cb, ptr, pr, pc, black, maximum))
+        return img4
 
     def apply_white_balance(self, img4, wb, filters: int = RGGB, xtrans=None, rgb=None):
         img4 = np.ascontiguousarray(img4, np.uint16)

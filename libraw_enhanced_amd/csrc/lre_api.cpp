@@ -275,6 +275,17 @@ int lre_hip_dev_lateral_ca(lre_ctx *ctx, const float *d_in, float *d_out, int w,
                                  lre_pick_stream(ctx, stream));
 }
 
+int lre_hip_dev_ingest_u16(lre_ctx *ctx, const uint16_t *d_raw, float *d_cfa, int w, int h,
+                           uint32_t filters, const signed char xtrans[36], const int cblack[4],
+                           int black, int maximum, const float wb[4], float *d_data_max,
+                           void *stream) {
+    if (!ctx || !d_raw || !d_cfa || !wb) return set_arg_error("null pointer");
+    if (!valid_dims(w, h)) return set_arg_error("image size");
+    if (filters == 9 && !xtrans) return set_arg_error("X-Trans table");
+    return lre_launch_ingest_u16(d_raw, d_cfa, w, h, filters, xtrans, cblack, black, maximum, wb,
+                                 d_data_max, lre_pick_stream(ctx, stream));
+}
+
 int lre_hip_dev_rgb3_to_cfa(lre_ctx *ctx, const float *d_raw_rgb, float *d_cfa, int w, int h,
                             uint32_t filters, const signed char xtrans[36], void *stream) {
     if (!ctx || !d_raw_rgb || !d_cfa) return set_arg_error("null pointer");
@@ -402,6 +413,27 @@ int lre_hip_lateral_ca(lre_ctx *ctx, const float *in_rgb, float *out_rgb, int w,
                                                            min_confidence, pyramid_levels, ws,
                                                            ctx->stream);
                           });
+}
+
+int lre_hip_black_level(lre_ctx *ctx, uint16_t *img4, int w, int h, const int cblack[4],
+                        const int *pos_table, int pos_rows, int pos_cols, int black, int maximum) {
+    if (!ctx || !img4 || !cblack) return set_arg_error("null pointer");
+    if (!valid_dims(w, h)) return set_arg_error("image size");
+    const bool positional = pos_table && pos_rows > 0 && pos_cols > 0;
+    if (positional && (size_t)pos_rows * pos_cols > 4096) return set_arg_error("positional table size");
+    LRE_HIP_CHECK(hipSetDevice(ctx->device));
+    const size_t n = (size_t)w * h;
+    int rc = lre_ensure_scratch(ctx, aligned(n * 8) + aligned(4096 * 4));
+    if (rc) return rc;
+    Carve c(ctx->scratch);
+    uint16_t *d_img = c.take<uint16_t>(n * 4);
+    int *d_pos = c.take<int>(4096);
+    if ((rc = copy_in(ctx, d_img, img4, n * 8))) return rc;
+    if (positional && (rc = copy_in(ctx, d_pos, pos_table, sizeof(int) * pos_rows * pos_cols))) return rc;
+    if ((rc = lre_launch_black_level(d_img, w, h, cblack, d_pos, positional ? pos_rows : 0,
+                                     positional ? pos_cols : 0, black, maximum, ctx->stream)))
+        return rc;
+    return copy_out(ctx, img4, d_img, n * 8);
 }
 
 int lre_hip_white_balance(lre_ctx *ctx, const uint16_t *img4, float *rgb, int w, int h,

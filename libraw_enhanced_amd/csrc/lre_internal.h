@@ -58,6 +58,11 @@ int lre_launch_axial_ca(const float *d_in, float *d_out, int w, int h, int radiu
 int lre_launch_lateral_ca(const float *d_in, float *d_out, int w, int h, int cell_size,
                           int max_iterations, float max_shift, float min_confidence,
                           int pyramid_levels, void *d_scratch, hipStream_t s);
+int lre_launch_black_level(uint16_t *d_img4, int w, int h, const int cblack[4], const int *d_pos,
+                           int pos_rows, int pos_cols, int black, int maximum, hipStream_t s);
+int lre_launch_ingest_u16(const uint16_t *d_raw, float *d_cfa, int w, int h, uint32_t filters,
+                          const signed char *xtrans, const int cblack[4], int black, int maximum,
+                          const float wb[4], float *d_data_max, hipStream_t s);
 int lre_launch_white_balance(const uint16_t *d_img4, float *d_rgb, int w, int h,
                              const float wb[4], uint32_t filters, const signed char *xtrans,
                              hipStream_t s);

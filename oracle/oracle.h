@@ -39,6 +39,12 @@ int oracle_axial_ca(const float *in3, float *out3, int w, int h, int radius, flo
                     float strength);
 int oracle_lateral_ca(const float *in3, float *out3, int w, int h, int cell_size, int max_iterations,
                       float max_shift, float min_confidence, int pyramid_levels);
+int oracle_black_level(uint16_t *img4, int w, int h, const int *cblack4, const int *pos_table,
+                       int pos_rows, int pos_cols, int black, int maximum);
+int oracle_ingest_u16(const uint16_t *raw, float *cfa, int w, int h, uint32_t filters,
+                      const signed char *xtrans36, const int *cblack4, int black, int maximum,
+                      const float *wb4, float *data_max);
+float oracle_adjust_maximum(float data_max, float maximum, float threshold);
 int oracle_white_balance(const uint16_t *img4, float *rgb3, int w, int h, const float *wb4,
                          uint32_t filters, const signed char *xtrans36);
 

@@ -177,3 +177,41 @@ def test_xtrans_bit_exact(acc, h, w):
     want = _oracle_xtrans(raw3)
     bad = np.any(got != want, axis=2)
     assert not bad.any(), f"{int(bad.sum())} px differ, max-abs {np.abs(got - want).max()}"
+
+
+# ---------------- integer preprocessing ----------------
+from test_ingest_cpu import black_level as _black_level, ingest as _ingest  # noqa: E402
+
+
+def test_black_level_bit_exact(acc):
+    raw = scene.quantize_u16(scene.bayer_scene(150, 210, 4))
+    img4 = place_native_u16(raw, RGGB)
+    for kw in (dict(black=512, maximum=16383), dict(cblack=(500, 510, 520, 530), maximum=16000),
+               dict(cblack=(3, 0, 1, 0), pos=np.array([[1, 2, 3], [4, 5, 6]]), maximum=16383)):
+        want = _black_level(img4, **kw)
+        got = acc.apply_black_level_correction(img4.copy(), kw.get("cblack", (0, 0, 0, 0)), kw.get("black", 0),
+                                               kw["maximum"], kw.get("pos"))
+        assert np.array_equal(got, want), kw
+
+
+@pytest.mark.parametrize("filters,xtrans", [(RGGB, None), (9, scene.XTRANS)])
+def test_ingest_u16_bit_exact(acc, filters, xtrans):
+    import ctypes as ct
+    from libraw_enhanced_amd.accelerator import native
+    h, w = 333, 517
+    cfa = scene.bayer_scene(h, w, 9) if xtrans is None else scene.xtrans_scene(h, w, 9, lateral=None)
+    raw = scene.quantize_u16(cfa, filters if xtrans is None else 0, xtrans)
+    want, want_max = _ingest(raw, filters, xtrans)
+    d_raw = acc.dev_from_host(raw)
+    d_cfa = acc.dev_alloc(h * w * 4)
+    d_max = acc.dev_alloc(4)
+    cb = (ct.c_int * 4)(0, 0, 0, 0)
+    wb = np.array([2.1, 1.0, 1.5, 1.0], np.float32)
+    xt = None if xtrans is None else np.ascontiguousarray(xtrans.reshape(-1), np.int8)
+    rc = native().lre_hip_dev_ingest_u16(acc.handle, d_raw.ptr, d_cfa.ptr, w, h, filters,
+                                        None if xt is None else xt.ctypes.data_as(ct.POINTER(ct.c_int8)),
+                                        cb, 512, 16383, wb.ctypes.data_as(ct.POINTER(ct.c_float)), d_max.ptr, None)
+    assert rc == 0
+    acc.synchronize()
+    assert np.array_equal(d_cfa.download((h, w)), want)
+    assert float(d_max.download((1,))[0]) == want_max
