@@ -142,29 +142,38 @@ __global__ __launch_bounds__(BLOCK) void blur_h(BlurSet S, const float *__restri
     }
 }
 
-constexpr int VB_W = 64, VB_H = 32;
+constexpr int VB_W = 64, VB_H = 64;
 
+// vertical pass: each workgroup stages (VB_H + 2 ks) rows x 64 columns of the
+// horizontal result in LDS (rows clamped at the frame) and sums the taps
 __global__ __launch_bounds__(BLOCK) void blur_v(BlurSet S, const float *__restrict__ taps, int W,
                                                 int H, int ks) {
+    extern __shared__ float col_tile[];  // (VB_H + 2 ks) x VB_W
     __shared__ float kt[MAX_TAPS];
     for (int t = threadIdx.x; t < 2 * ks + 1; t += BLOCK) kt[t] = taps[t];
-    __syncthreads();
-    const int x = blockIdx.x * VB_W + (threadIdx.x % VB_W);
-    const int y0 = blockIdx.y * VB_H;
-    if (x >= W) return;
+    const int x0 = blockIdx.x * VB_W, y0 = blockIdx.y * VB_H;
+    const int rows = VB_H + 2 * ks;
+    const int tx = threadIdx.x % VB_W, ty = threadIdx.x / VB_W;
+    const int x = x0 + tx;
     for (int p = 0; p < S.count; p++) {
         const float *tm = S.tmp[p];
-        for (int r = threadIdx.x / VB_W; r < VB_H; r += BLOCK / VB_W) {
-            const int y = y0 + r;
-            if (y >= H) break;
-            float acc = 0.f;
-            for (int k = -ks; k <= ks; k++) {
-                int yy = y + k;
-                yy = yy < 0 ? 0 : (yy > H - 1 ? H - 1 : yy);
-                acc += kt[k + ks] * tm[(size_t)yy * W + x];
-            }
-            S.out[p][(size_t)y * W + x] = acc;
+        __syncthreads();
+        for (int t = threadIdx.x; t < rows * VB_W; t += BLOCK) {
+            const int r = t / VB_W, c = t % VB_W;
+            int yy = y0 - ks + r;
+            yy = yy < 0 ? 0 : (yy > H - 1 ? H - 1 : yy);
+            const int xx = x0 + c;
+            col_tile[t] = xx < W ? tm[(size_t)yy * W + xx] : 0.f;
         }
+        __syncthreads();
+        if (x < W)
+            for (int r = ty; r < VB_H; r += BLOCK / VB_W) {
+                const int y = y0 + r;
+                if (y >= H) break;
+                float acc = 0.f;
+                for (int k = 0; k <= 2 * ks; k++) acc += kt[k] * col_tile[(r + k) * VB_W + tx];
+                S.out[p][(size_t)y * W + x] = acc;
+            }
     }
 }
 
@@ -212,39 +221,47 @@ __global__ __launch_bounds__(BLOCK) void support(Planes P, Scal C, const unsigne
     }
 }
 
-// line-linked support (:4266-4323): ls = max(fs, 0.55 * blur(fs)) sampled with clamp
-LRE_DEV float ls_at(const Planes &P, int W, int H, int x, int y) {
-    x = x < 0 ? 0 : (x > W - 1 ? W - 1 : x);
-    y = y < 0 ? 0 : (y > H - 1 ? H - 1 : y);
-    const size_t i = (size_t)y * W + x;
-    return mx(P.fs[i], P.bfs[i] * 0.55f);
-}
-LRE_DEV float fs_at(const Planes &P, int W, int H, int x, int y) {
-    x = x < 0 ? 0 : (x > W - 1 ? W - 1 : x);
-    y = y < 0 ? 0 : (y > H - 1 ? H - 1 : y);
-    return P.fs[(size_t)y * W + x];
-}
+// line-linked support (:4266-4323): ls = max(fs, 0.55 * blur(fs)) sampled with
+// clamp-to-edge.  Each workgroup stages ls and fs for its 32x32 outputs plus a
+// 14-px halo in LDS (coordinates clamped at load time, which equals clamping
+// every sample).
+constexpr int LL_T = 32, LL_HALO = 14, LL_R = LL_T + 2 * LL_HALO;
 
 __global__ __launch_bounds__(BLOCK) void line_link(Planes P, int W, int H) {
+    __shared__ float s_ls[LL_R * LL_R], s_fs[LL_R * LL_R];
+    const int x0 = blockIdx.x * LL_T, y0 = blockIdx.y * LL_T;
+    for (int t = threadIdx.x; t < LL_R * LL_R; t += BLOCK) {
+        int y = y0 - LL_HALO + t / LL_R, x = x0 - LL_HALO + t % LL_R;
+        x = x < 0 ? 0 : (x > W - 1 ? W - 1 : x);
+        y = y < 0 ? 0 : (y > H - 1 ? H - 1 : y);
+        const size_t i = (size_t)y * W + x;
+        const float f = P.fs[i];
+        s_fs[t] = f;
+        s_ls[t] = mx(f, P.bfs[i] * 0.55f);
+    }
+    __syncthreads();
     const int dtx[8] = {1, 0, 1, 1, 2, 1, 2, 1}, dty[8] = {0, 1, 1, -1, 1, 2, -1, -2};
-    const size_t n = (size_t)W * H;
-    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK) {
-        const int y = (int)(i / W), x = (int)(i % W);
+    for (int o = threadIdx.x; o < LL_T * LL_T; o += BLOCK) {
+        const int ly = o / LL_T, lx = o % LL_T;
+        const int x = x0 + lx, y = y0 + ly;
+        if (x >= W || y >= H) continue;
+        const int c = (ly + LL_HALO) * LL_R + (lx + LL_HALO);
         float best = 0.f;
 #pragma unroll
         for (int d = 0; d < 8; d++) {
-            const int tx = dtx[d], ty = dty[d], nx = -ty, ny = tx;
+            const int step = dty[d] * LL_R + dtx[d];
+            const int nstep = dtx[d] * LL_R - dty[d];  // (nx, ny) = (-ty, tx)
             float along = 0.f;
             const int max_step = d < 4 ? 14 : 6;
             for (int s = 1; s <= max_step; s++) {
-                along = mx(along, ls_at(P, W, H, x + tx * s, y + ty * s));
-                along = mx(along, ls_at(P, W, H, x - tx * s, y - ty * s));
+                along = mx(along, s_ls[c + step * s]);
+                along = mx(along, s_ls[c - step * s]);
             }
-            const float across = mx(fs_at(P, W, H, x + nx * 2, y + ny * 2), fs_at(P, W, H, x - nx * 2, y - ny * 2));
+            const float across = mx(s_fs[c + nstep * 2], s_fs[c - nstep * 2]);
             const float thin = 1.f - sstep(along * 0.45f, along * 0.95f, across);
             best = mx(best, along * thin);
         }
-        P.lfs[i] = best;
+        P.lfs[(size_t)y * W + x] = best;
     }
 }
 
@@ -572,7 +589,7 @@ int lre_launch_defringe(const float *d_in, float *d_out, int w, int h, float rad
         dim3 gh((w + HB_W - 1) / HB_W, h);
         hipLaunchKernelGGL(blur_h, gh, dim3(BLOCK), 0, s, S, d_taps, w, h, ks, stats);
         dim3 gv((w + VB_W - 1) / VB_W, (h + VB_H - 1) / VB_H);
-        hipLaunchKernelGGL(blur_v, gv, dim3(BLOCK), 0, s, S, d_taps, w, h, ks);
+        hipLaunchKernelGGL(blur_v, gv, dim3(BLOCK), sizeof(float) * (VB_H + 2 * ks) * VB_W, s, S, d_taps, w, h, ks);
         LRE_HIP_CHECK(hipGetLastError());
         return LRE_OK;
     };
@@ -594,7 +611,7 @@ int lre_launch_defringe(const float *d_in, float *d_out, int w, int h, float rad
     BlurSet S1{};
     S1.count = 1; S1.in[0] = fs; S1.tmp[0] = t2; S1.out[0] = bfs;
     if ((rc = blur(S1))) return rc;
-    hipLaunchKernelGGL(line_link, dim3(g), dim3(BLOCK), 0, s, P, w, h);
+    hipLaunchKernelGGL(line_link, dim3((w + LL_T - 1) / LL_T, (h + LL_T - 1) / LL_T), dim3(BLOCK), 0, s, P, w, h);
     hipLaunchKernelGGL(solo_seed, dim3(g), dim3(BLOCK), 0, s, P, C);
     BlurSet S2{};
     S2.count = 2;

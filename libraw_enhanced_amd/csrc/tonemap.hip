@@ -38,7 +38,8 @@ constexpr int DBLOCK = 256;
 LRE_DEV int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 // Box-filter two planes held in LDS (in0/in1: DIY x DIX, zero outside the
-// image) into out0/out1 (DTY x DTX, in LDS).  hs0/hs1: DIY x DTX scratch.
+// image) into out0/out1 (DTY x DTX, in LDS; may alias in0/in1, which are dead
+// after the horizontal pass).  hs0/hs1: DIY x DTX scratch.
 LRE_DEV void box2(const float *in0, const float *in1, float *hs0, float *hs1, float *out0,
                   float *out1, int x0, int y0, int W, int H) {
     for (int t = threadIdx.x; t < DIY * DTX; t += DBLOCK) {
@@ -72,7 +73,7 @@ __global__ __launch_bounds__(DBLOCK) void dtm_ab_kernel(const float *__restrict_
                                                         float *__restrict__ pb, int W, int H) {
     __shared__ float in0[DIY * DIX], in1[DIY * DIX];
     __shared__ float hs0[DIY * DTX], hs1[DIY * DTX];
-    __shared__ float o0[DTY * DTX], o1[DTY * DTX];
+    float *o0 = in0, *o1 = in1;
     const int x0 = blockIdx.x * DTX, y0 = blockIdx.y * DTY;
     for (int t = threadIdx.x; t < DIY * DIX; t += DBLOCK) {
         const int y = y0 - DR + t / DIX, x = x0 - DR + t % DIX;
@@ -105,7 +106,7 @@ __global__ __launch_bounds__(DBLOCK) void dtm_apply_kernel(const float *__restri
                                                            int H) {
     __shared__ float in0[DIY * DIX], in1[DIY * DIX];
     __shared__ float hs0[DIY * DTX], hs1[DIY * DTX];
-    __shared__ float o0[DTY * DTX], o1[DTY * DTX];
+    float *o0 = in0, *o1 = in1;
     const int x0 = blockIdx.x * DTX, y0 = blockIdx.y * DTY;
     for (int t = threadIdx.x; t < DIY * DIX; t += DBLOCK) {
         const int y = y0 - DR + t / DIX, x = x0 - DR + t % DIX;
