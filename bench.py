@@ -1,15 +1,25 @@
 #!/usr/bin/env python3
 """Throughput of the develop chain on one MI355X.
 
-Workload: one 24 MP (4000 x 6000) RGGB Bayer image, float32, from the seeded
-scene generator (gradient + 200 polygons + zone plate, Poisson-Gaussian
-noise), developed by AMaZE demosaic -> defringe (radius 10, strength 10, green
-off) -> detail tone map.  The CFA plane is resident in HBM before timing; a
-step is one pass of the chain over the image.
-
+Default workload (--workload chain24): one 24 MP (4000 x 6000) RGGB Bayer
+image, float32, from the seeded scene generator (gradient + 200 polygons +
+zone plate, Poisson-Gaussian noise), developed by AMaZE demosaic -> defringe
+(radius 10, strength 10, green off) -> detail tone map.  The CFA plane is
+resident in HBM before timing; a step is one pass of the chain over the image.
 Prints one JSON line: MP/s (value), ms per step, the roofline of the dominant
 kernel (HIP events on the stream the stages run on), and the reference's own
 CPU code timed on this host's cores on the same image.
+
+Other workloads, one JSON line each:
+  xtrans26  one 26 MP (4160 x 6240) X-Trans image, pre-warped with radial
+            lateral CA: X-Trans 3-pass -> lateral CA -> axial CA
+  batch64   64 images (16x 12 MP, 16x 24 MP, 12x 45 MP, 8x 61 MP Bayer, 12x
+            40 MP X-Trans), float32: demosaic -> axial CA -> defringe -> detail
+            tone map as one batch (multi-image tile queues + HIP graph), against
+            the plain per-image path (host arrays in and out of every stage)
+  serve     Poisson arrivals of uint16 requests (40% 24 MP, 30% 45 MP, 20%
+            61 MP Bayer, 10% 40 MP X-Trans; black 512, white 16383, WB
+            2.1/1.0/1.5) through the request loop; p50/p99 latency and MP/s
 """
 from __future__ import annotations
 
@@ -112,11 +122,19 @@ def main():
     ap.add_argument("--width", type=int, default=6000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--workload", default="chain24", choices=["chain24", "xtrans26", "batch64", "serve"])
+    ap.add_argument("--rates", default="4,8,16", help="serve: arrival rates (requests/s)")
+    ap.add_argument("--duration", type=float, default=20.0, help="serve: seconds of arrivals per rate")
+    ap.add_argument("--lanes", type=int, default=3, help="serve: requests in flight")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): images are independent, "
               "so N GPUs would each run this same single-GPU workload", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "chain24":
+        run = {"xtrans26": bench_xtrans26, "batch64": bench_batch64, "serve": bench_serve}[args.workload]
+        print(json.dumps(run(args)), flush=True)
+        return
 
     from libraw_enhanced_amd import scene
     from libraw_enhanced_amd.accelerator import HIPAccelerator, native
@@ -187,6 +205,329 @@ def main():
         line["cpu_baseline"] = None
     acc.close()
     print(json.dumps(line), flush=True)
+
+
+# ---------------------------------------------------------------------------
+# other workloads
+
+def _log(msg):
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
+def _threads(args):
+    return args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+
+
+def _ref_lib(threads):
+    """oracle/_ref (the reference's CPU code built from its sources), else the restatement."""
+    ref_path = os.path.join(ROOT, "oracle", "_ref", "libref_cpu.so")
+    if os.path.exists(ref_path):
+        lib = ctypes.CDLL(ref_path)
+        lib.ref_set_threads(threads)
+        return lib, "reference"
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    return ctypes.CDLL(os.path.join(ROOT, "oracle", "liblre_oracle.so")), "port"
+
+
+def cpu_baseline_stages(raw3, filters, xtrans, stages, threads, label):
+    """One image through `stages` on the CPU (reference names, in-place as the
+    develop pipeline calls them); returns the cpu_baseline object."""
+    f32p = ctypes.POINTER(ctypes.c_float)
+    i8p = ctypes.POINTER(ctypes.c_int8)
+    lib, kind = _ref_lib(threads)
+    R = kind == "reference"
+    h, w, _ = raw3.shape
+    rgb = np.zeros_like(raw3)
+    tmp = np.zeros_like(raw3)
+    cm = np.ones(4, np.float32)
+    xt = None if xtrans is None else np.ascontiguousarray(np.asarray(xtrans).reshape(36), np.int8)
+    P = lambda a: a.ctypes.data_as(f32p)
+    t0 = time.perf_counter()
+    for s in stages:
+        if s == "demosaic" and filters != 9:
+            if R:
+                lib.ref_amaze(P(raw3), P(rgb), w, h, ctypes.c_uint32(filters), P(cm), ctypes.c_float(1.0))
+            else:
+                lib.oracle_amaze(P(raw3), P(rgb), w, h, ctypes.c_uint32(filters), P(cm), ctypes.c_float(1.0), 0)
+        elif s == "demosaic":
+            fn = lib.ref_xtrans3 if R else lib.oracle_xtrans3
+            args = (P(raw3), P(rgb), w, h, xt.ctypes.data_as(i8p), ctypes.c_float(1.0))
+            fn(*args) if R else fn(*args, 0)
+        elif s == "lateral_ca":
+            fn = lib.ref_lateral_ca if R else lib.oracle_lateral_ca
+            fn(P(rgb), P(tmp), w, h, 96, 3, ctypes.c_float(6.0), ctypes.c_float(0.02), 3)
+            rgb, tmp = tmp, rgb
+        elif s == "axial_ca":
+            fn = lib.ref_axial_ca if R else lib.oracle_axial_ca
+            fn(P(rgb), P(tmp), w, h, 6, ctypes.c_float(1e-4), ctypes.c_float(0.3))
+            rgb, tmp = tmp, rgb
+        elif s == "defringe":
+            fn = lib.ref_defringe if R else lib.oracle_defringe
+            fn(P(rgb), P(rgb), w, h, ctypes.c_float(10.0), ctypes.c_float(0.1), ctypes.c_float(0.15),
+               ctypes.c_float(10.0), 0, ctypes.c_float(0.3))
+        elif s == "detail_tm":
+            fn = lib.ref_detail_tonemap if R else lib.oracle_detail_tonemap
+            fn(P(rgb), w, h)
+    dt = time.perf_counter() - t0
+    return {"value": round(h * w / 1e6 / dt, 3), "unit": "MP/s", "cores": threads, "kind": kind,
+            "sample": f"one {w}x{h} {label} image through {' -> '.join(stages)} ({dt:.2f} s)"}
+
+
+def bench_xtrans26(args):
+    from libraw_enhanced_amd import scene
+    from libraw_enhanced_amd.accelerator import HIPAccelerator, native
+    from libraw_enhanced_amd.pipeline import DevelopChain
+    h, w = 4160, 6240
+    _log(f"generating {w}x{h} X-Trans scene (radial pre-warp R 1.0008, B 0.9993)")
+    cfa = scene.xtrans_scene(h, w, scene.BASE_SEED)
+    acc = HIPAccelerator(0)
+    d_cfa = acc.dev_from_host(cfa)
+    stages = ["xtrans3", "lateral_ca", "axial_ca"]
+    chain = DevelopChain(acc, h, w, stages, filters=9, xtrans=scene.XTRANS)
+    stream = native().lre_hip_stream(acc.handle)
+    ev = HipEvents()
+    for _ in range(args.warmup):
+        chain.run(d_cfa)
+    acc.synchronize()
+    marks = [[ev.create() for _ in range(len(stages) + 1)] for _ in range(args.steps)]
+    _log(f"timing {args.steps} steps of {' -> '.join(stages)}")
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        ev.record(marks[k][0], stream)
+        for i, s in enumerate(stages):
+            chain.run_stage(s, d_cfa)
+            ev.record(marks[k][i + 1], stream)
+    acc.synchronize()
+    elapsed = time.perf_counter() - t0
+    stage_ms = {s: float(np.mean([ev.elapsed_ms(marks[k][i], marks[k][i + 1]) for k in range(args.steps)]))
+                for i, s in enumerate(stages)}
+    mp = h * w / 1e6
+    line = {"metric": "MP/s X-Trans develop chain (26 MP)", "value": round(mp * args.steps / elapsed, 2),
+            "unit": "MP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(elapsed * 1e3 / args.steps, 3), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic (seeded scene, radial lateral CA pre-warp, Poisson-Gaussian noise)",
+            "config": {"workload": f"{' -> '.join(stages)} on one {w}x{h} X-Trans float32 mosaic",
+                       "image_mp": round(mp, 2), "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()}}}
+    if not args.no_cpu:
+        threads = _threads(args)
+        _log(f"CPU baseline on {threads} threads")
+        raw3 = scene.cfa_to_rgb3(cfa, 0, scene.XTRANS)
+        line["cpu_baseline"] = cpu_baseline_stages(raw3, 9, scene.XTRANS, ["demosaic", "lateral_ca", "axial_ca"],
+                                                   threads, "X-Trans")
+    acc.close()
+    return line
+
+
+# (count, height, width, filters) of the 64-image batch
+BATCH64 = [(16, 3000, 4000, "bayer"), (16, 4000, 6000, "bayer"), (12, 5504, 8256, "bayer"),
+           (8, 6336, 9504, "bayer"), (12, 5152, 7728, "xtrans")]
+
+
+def _plain_path(acc, raw3, filters, xtrans, stages):
+    """The reference's call pattern on the HIP backend: host arrays into and out
+    of every Accelerator method (the per-image plain path)."""
+    if filters == 9:
+        rgb = acc.demosaic_xtrans_3pass(raw3, xtrans, 1.0)
+    else:
+        rgb = acc.demosaic_bayer_amaze(raw3, filters, (1, 1, 1, 1), 1.0)
+    for s in stages[1:]:
+        if s == "axial_ca":
+            rgb = acc.ca_axial_cleanup(rgb)
+        elif s == "defringe":
+            rgb = acc.defringe(rgb, rgb)
+        elif s == "detail_tm":
+            rgb = acc.apply_detail_preserving_tonemap(rgb)
+    return rgb
+
+
+def bench_batch64(args):
+    from libraw_enhanced_amd import develop as dv
+    from libraw_enhanced_amd import scene
+    from libraw_enhanced_amd.accelerator import HIPAccelerator
+    acc = HIPAccelerator(0)
+    stages = list(dv.BATCH_CHAIN)
+    images, bufs, total_mp, k = [], [], 0.0, 0
+    _log("generating 64 images (2080.8 MP) and uploading them")
+    t_gen = time.perf_counter()
+    samples = {}
+    for count, h, w, kind in BATCH64:
+        for _ in range(count):
+            seed = scene.BASE_SEED + k
+            if kind == "bayer":
+                cfa, filters, xt = scene.bayer_scene(h, w, seed), scene.RGGB, None
+            else:
+                cfa, filters, xt = scene.xtrans_scene(h, w, seed, lateral=None), 9, scene.XTRANS
+            d_cfa = acc.dev_from_host(cfa)
+            d_rgb = acc.dev_alloc(h * w * 12)
+            bufs.append((d_cfa, d_rgb))
+            images.append(dv.make_image(w, h, filters=filters, xtrans=xt, d_cfa=d_cfa, maximum_value=1.0,
+                                        d_rgb=d_rgb))
+            samples.setdefault((h, w, kind), (cfa, filters, xt))
+            total_mp += h * w / 1e6
+            k += 1
+    _log(f"generated in {time.perf_counter() - t_gen:.1f} s; autotuning on one image per size class")
+    b = dv.BatchDeveloper(0, 2)
+    tune_set = [images[sum(c for c, *_ in BATCH64[:i])] for i in range(len(BATCH64))]
+    tune_ms = b.autotune(tune_set, stages, reps=1)
+    cfg = b.config()
+    _log(f"autotune: {cfg} ({tune_ms:.1f} ms for the 5-image sample)")
+    b.run(images, stages)  # eager warm-up (sizes every buffer)
+    b.synchronize()
+    eager_t0 = time.perf_counter()
+    b.run(images, stages)
+    b.synchronize()
+    eager_s = time.perf_counter() - eager_t0
+    b.capture(images, stages)
+    for _ in range(args.warmup):
+        b.replay()
+    b.synchronize()
+    steps = max(1, args.steps)
+    _log(f"timing {steps} graph replays of the 64-image batch")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.replay()
+    b.synchronize()
+    elapsed = time.perf_counter() - t0
+    value = total_mp * steps / elapsed
+    # the plain per-image path on one image of each size class
+    _log("timing the plain per-image path on one image per size class")
+    plain_mp, plain_s = 0.0, 0.0
+    for (h, w, kind), (cfa, filters, xt) in samples.items():
+        raw3 = scene.cfa_to_rgb3(cfa, filters if xt is None else 0, xt)
+        _plain_path(acc, raw3[:64, :64].copy(), filters, xt, stages)  # warm
+        t = time.perf_counter()
+        _plain_path(acc, raw3, filters, xt, stages)
+        plain_s += time.perf_counter() - t
+        plain_mp += h * w / 1e6
+    plain = plain_mp / plain_s
+    line = {"metric": "MP/s batched develop (64 uneven images)", "value": round(value, 2), "unit": "MP/s",
+            "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+            "ms_per_step": round(elapsed * 1e3 / steps, 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic (seeded scenes base..base+63, Poisson-Gaussian noise)",
+            "config": {"workload": "demosaic -> axial_ca -> defringe -> detail_tm over 16x12 MP + 16x24 MP + "
+                                   "12x45.4 MP + 8x60.2 MP RGGB + 12x39.8 MP X-Trans, float32, one batch",
+                       "total_mp": round(total_mp, 1), "autotuned": cfg, "eager_batch_s": round(eager_s, 3),
+                       "plain_per_image_mp_s": round(plain, 2), "batched_vs_plain": round(value / plain, 2),
+                       "plain_sample_mp": round(plain_mp, 1)}}
+    if not args.no_cpu:
+        threads = _threads(args)
+        _log(f"CPU baseline on {threads} threads (one 12 MP Bayer image)")
+        cfa, filters, xt = samples[(3000, 4000, "bayer")]
+        line["cpu_baseline"] = cpu_baseline_stages(scene.cfa_to_rgb3(cfa), filters, None, stages, threads, "RGGB")
+    b.close()
+    acc.close()
+    return line
+
+
+# request mix of the served loop: (share, height, width, kind)
+SERVE_MIX = [(0.4, 4000, 6000, "bayer"), (0.3, 5504, 8256, "bayer"), (0.2, 6336, 9504, "bayer"),
+             (0.1, 5152, 7728, "xtrans")]
+WB = (2.1, 1.0, 1.5, 1.0)
+
+
+def bench_serve(args):
+    from libraw_enhanced_amd import develop as dv
+    from libraw_enhanced_amd import scene
+    from libraw_enhanced_amd.accelerator import HIPAccelerator
+    rates = [float(r) for r in args.rates.split(",") if r]
+    _log("generating uint16 requests (2 scenes per size class) into pinned memory")
+    inputs, outputs = {}, {}
+    for ci, (_, h, w, kind) in enumerate(SERVE_MIX):
+        for v in range(2):
+            seed = scene.BASE_SEED + 1000 + 10 * ci + v
+            if kind == "bayer":
+                cfa = scene.bayer_scene(h, w, seed)
+                raw = scene.quantize_u16(cfa, scene.RGGB, None)
+            else:
+                cfa = scene.xtrans_scene(h, w, seed, lateral=None)
+                raw = scene.quantize_u16(cfa, 0, scene.XTRANS)
+            pa = dv.PinnedArray(raw.shape, np.uint16)
+            pa.array[...] = raw
+            inputs[(ci, v)] = pa
+        # one pinned output per class (the benchmark does not read the results)
+        outputs[ci] = dv.PinnedArray((h, w, 3), np.float32)
+    shares = np.cumsum([m[0] for m in SERVE_MIX])
+
+    def image_for(ci):
+        _, h, w, kind = SERVE_MIX[ci]
+        if kind == "bayer":
+            return dv.make_image(w, h, filters=scene.RGGB, cam_mul=WB, black=512, white=16383, wb=WB)
+        return dv.make_image(w, h, filters=9, xtrans=scene.XTRANS, cam_mul=WB, black=512, white=16383, wb=WB)
+
+    acc = HIPAccelerator(0)  # device init before the server threads
+    server = dv.DevelopServer(0, lanes=args.lanes, hbm_budget_bytes=96 << 30)
+    # warm-up: one request of each class (pool and code objects)
+    for ci in range(len(SERVE_MIX)):
+        server.submit(10_000 + ci, image_for(ci), inputs[(ci, 0)].ptr, outputs[ci].ptr)
+    got = 0
+    while got < len(SERVE_MIX):
+        got += len(server.poll(timeout_ms=1000))
+    results = []
+    for rate in rates:
+        n = max(1, int(round(rate * args.duration)))
+        arr = dv.poisson_schedule(rate, n, seed=int(rate * 1000))
+        rng = np.random.Generator(np.random.PCG64(int(rate * 7)))
+        classes = np.searchsorted(shares, rng.random(n), side="right").clip(0, len(SERVE_MIX) - 1)
+        variants = rng.integers(0, 2, n)
+
+        def make(k):
+            ci = int(classes[k])
+            _, h, w, _ = SERVE_MIX[ci]
+            return image_for(ci), inputs[(ci, int(variants[k]))].ptr, outputs[ci].ptr, h * w / 1e6, None
+
+        _log(f"rate {rate}/s: {n} requests over {arr[-1]:.1f} s")
+        wall, lat, mp_total = dv.run_open_loop(server, arr, make)
+        rep = dv.ServeReport(rate=rate, requests=n, duration_s=wall, mp_total=mp_total, latencies_s=lat)
+        summ = rep.summary()
+        summ["offered_mp_s"] = round(mp_total / arr[-1], 2)
+        results.append(summ)
+        _log(f"rate {rate}/s: {summ}")
+    _, peak = server.footprint()
+    server.close()
+    # the plain per-image path: one request of each class, host arrays through every stage
+    _log("timing the plain per-image path (one request per class)")
+    plain_lat = {}
+    for ci, (_, h, w, kind) in enumerate(SERVE_MIX):
+        raw = inputs[(ci, 0)].array
+        t = time.perf_counter()
+        img4 = np.zeros((h, w, 4), np.uint16)
+        if kind == "bayer":  # LibRaw's 4-channel layout: the native channel (G2 = 3)
+            r, c = np.arange(h)[:, None], np.arange(w)[None, :]
+            f = (scene.RGGB >> ((((r << 1) & 14) | (c & 1)) << 1)) & 3
+        else:
+            f = scene.fcol_xtrans(h, w)
+        np.put_along_axis(img4, f[..., None].astype(np.int64), raw[..., None], axis=2)
+        img4 = acc.apply_black_level_correction(img4, (0, 0, 0, 0), 512, 16383, None)
+        rgb = acc.apply_white_balance(img4, WB, scene.RGGB if kind == "bayer" else 9,
+                                      None if kind == "bayer" else scene.XTRANS)
+        if kind == "bayer":
+            rgb = acc.demosaic_bayer_amaze(rgb, scene.RGGB, WB, 15871.0)
+        else:
+            rgb = acc.demosaic_xtrans_3pass(rgb, scene.XTRANS, 15871.0)
+        rgb = acc.ca_axial_cleanup(rgb)
+        rgb = acc.defringe(rgb, rgb)
+        rgb = acc.apply_detail_preserving_tonemap(rgb)
+        plain_lat[ci] = time.perf_counter() - t
+    mean_mp = sum(m[0] * m[1] * m[2] / 1e6 for m in SERVE_MIX)
+    mean_plain_s = sum(m[0] * plain_lat[ci] for ci, m in enumerate(SERVE_MIX))
+    plain_mp_s = mean_mp / mean_plain_s
+    best = max(r["mp_per_s"] for r in results)
+    line = {"metric": "p99 latency of the served develop loop", "value": results[-1]["p99_ms"], "unit": "ms",
+            "n_gpus": 1, "steps": sum(r["requests"] for r in results), "warmup": len(SERVE_MIX),
+            "ms_per_step": None, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+            "dtype": "u16->f32", "data": "synthetic uint16 CFA (seeded scenes, black 512, white 16383)",
+            "config": {"workload": "ingest -> demosaic -> axial_ca -> defringe -> detail_tm, Poisson arrivals, "
+                                   "40% 24 MP / 30% 45.4 MP / 20% 60.2 MP Bayer / 10% 39.8 MP X-Trans",
+                       "rates": results, "lanes": args.lanes, "duration_s_per_rate": args.duration,
+                       "peak_device_bytes": int(peak),
+                       "plain_per_image_ms": {f"{m[1]}x{m[2]}{'x' if m[3] == 'xtrans' else ''}":
+                                              round(plain_lat[ci] * 1e3, 1) for ci, m in enumerate(SERVE_MIX)},
+                       "plain_per_image_mp_s": round(plain_mp_s, 2),
+                       "served_vs_plain_mp_s": round(best / plain_mp_s, 2)}}
+    acc.close()
+    return line
 
 
 if __name__ == "__main__":

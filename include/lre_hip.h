@@ -167,6 +167,119 @@ int lre_hip_dev_rgb3_to_cfa(lre_ctx *ctx, const float *d_raw_rgb, float *d_cfa, 
                             int height, uint32_t filters, const signed char xtrans[36],
                             void *stream);
 
+/* ---- many images: batched develop and the request loop --------------------
+ * The reference develops one image per postprocess() call
+ * (core/libraw_wrapper.cpp:1477-1656) and has no batch or serving layer; these
+ * entry points run the same stage functions (the ones above, same parameters
+ * and defaults) over many device-resident images. */
+
+/* Stage codes of an ordered stage list.  LRE_STAGE_DEMOSAIC, when present, must
+ * come first: AMaZE for Bayer images, X-Trans 3-pass for filters == 9. */
+enum {
+    LRE_STAGE_DEMOSAIC = 1,
+    LRE_STAGE_LATERAL_CA = 2,
+    LRE_STAGE_AXIAL_CA = 3,
+    LRE_STAGE_DEFRINGE = 4,
+    LRE_STAGE_DETAIL_TM = 5,
+    LRE_STAGE_TONE_MAPPING = 6
+};
+
+/* Parameters of the post-demosaic stages. */
+typedef struct lre_chain_params {
+    float defringe_radius, defringe_edge_threshold, defringe_chroma_threshold, defringe_strength;
+    int defringe_green;
+    float defringe_green_scale;
+    int axial_radius;
+    float axial_epsilon, axial_strength;
+    int lateral_cell, lateral_iterations;
+    float lateral_max_shift, lateral_min_confidence;
+    int lateral_levels;
+    float tone_after_scale;
+} lre_chain_params;
+
+/* The reference's defaults (core/accelerator.h:344-367, :322-323). */
+void lre_chain_params_default(lre_chain_params *p);
+
+/* One image.  Input is either a float CFA plane (d_cfa, with maximum_value as
+ * for lre_hip_dev_amaze) or, when d_cfa is NULL, a uint16 CFA plane d_raw that
+ * is preprocessed on the device exactly as lre_hip_dev_ingest_u16 (black,
+ * cblack, white = clip level, wb) followed by the adjust_maximum rule
+ * (core/libraw_wrapper.cpp:1070-1077) with maximum = white - black and
+ * adjust_threshold.  All stages run in place on d_rgb (H x W x 3 float). */
+typedef struct lre_image {
+    int width, height;
+    uint32_t filters;          /* Bayer filters word, or 9 for X-Trans */
+    signed char xtrans[36];    /* X-Trans pattern when filters == 9 */
+    float cam_mul[4];          /* AMaZE clip point, as lre_hip_amaze */
+    const float *d_cfa;
+    float maximum_value;
+    const uint16_t *d_raw;
+    int black, white;
+    int cblack[4];
+    float wb[4];
+    float adjust_threshold;
+    float *d_rgb;
+} lre_image;
+
+/* Batched develop over resident images.  One multi-image tile queue per
+ * demosaic kind (all Bayer images in one AMaZE launch, all X-Trans images in
+ * one X-Trans launch), then the remaining stages per image spread over `lanes`
+ * HIP streams.  lre_hip_batch_run only enqueues; lre_hip_batch_synchronize
+ * waits.  A batch object is used by one host thread at a time. */
+typedef struct lre_batch lre_batch;
+int lre_hip_batch_create(lre_batch **out, int device, int lanes);
+int lre_hip_batch_destroy(lre_batch *b);
+int lre_hip_batch_run(lre_batch *b, const lre_image *images, int n, const int *stages,
+                      int nstages, const lre_chain_params *params);
+int lre_hip_batch_synchronize(lre_batch *b);
+/* Record the batch (same arguments as lre_hip_batch_run) into a HIP graph after
+ * one eager run that sizes every buffer; lre_hip_batch_replay launches it.  The
+ * graph holds kernel nodes only and refers to the images' device buffers. */
+int lre_hip_batch_capture(lre_batch *b, const lre_image *images, int n, const int *stages,
+                          int nstages, const lre_chain_params *params);
+int lre_hip_batch_replay(lre_batch *b);
+/* Launch shape: lanes (1..8) and persistent workgroups per CU of the AMaZE and
+ * X-Trans tile queues (1..16). */
+int lre_hip_batch_set_config(lre_batch *b, int lanes, int amaze_slots_per_cu,
+                             int xtrans_slots_per_cu);
+int lre_hip_batch_get_config(lre_batch *b, int *lanes, int *amaze_slots_per_cu,
+                             int *xtrans_slots_per_cu);
+/* Time `reps` runs of each candidate launch shape on this batch and keep the
+ * fastest; *best_ms receives its time per batch. */
+int lre_hip_batch_autotune(lre_batch *b, const lre_image *images, int n, const int *stages,
+                           int nstages, const lre_chain_params *params, int reps, float *best_ms);
+
+/* Request loop.  Requests carry host pointers (uint16 CFA in, float RGB out);
+ * lre_hip_host_alloc gives pinned buffers for the fastest copies.  A request is
+ * admitted when one of `lanes` worker streams is free and its device footprint
+ * (input, float CFA, RGB and stage scratch) fits in hbm_budget_bytes beside the
+ * requests already admitted; admitted requests run concurrently on the device. */
+typedef struct lre_server lre_server;
+typedef struct lre_request {
+    uint64_t id;
+    lre_image image;           /* geometry and preprocessing; device pointers ignored */
+    const uint16_t *h_raw;     /* width * height uint16 CFA samples */
+    float *h_rgb;              /* width * height * 3 floats */
+} lre_request;
+typedef struct lre_completion {
+    uint64_t id;
+    int status;                /* LRE_OK or an error code */
+    double t_submit, t_admit, t_done;  /* seconds on the server's monotonic clock */
+} lre_completion;
+int lre_hip_serve_create(lre_server **out, int device, int lanes, size_t hbm_budget_bytes,
+                         const int *stages, int nstages, const lre_chain_params *params);
+/* Queue a request (FIFO).  The host buffers must stay valid until its completion. */
+int lre_hip_serve_submit(lre_server *s, const lre_request *req);
+/* Up to max completions; waits up to timeout_ms for the first one.  Returns the count. */
+int lre_hip_serve_poll(lre_server *s, lre_completion *out, int max, int timeout_ms);
+/* The server's monotonic clock (seconds), for arrival schedules. */
+double lre_hip_serve_now(lre_server *s);
+/* Device bytes the server holds for admitted requests, and the peak so far. */
+int lre_hip_serve_footprint(lre_server *s, size_t *current, size_t *peak);
+int lre_hip_serve_destroy(lre_server *s);
+void *lre_hip_host_alloc(size_t bytes);
+int lre_hip_host_free(void *p);
+
 #ifdef __cplusplus
 }
 #endif

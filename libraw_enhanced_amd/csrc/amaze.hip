@@ -47,7 +47,9 @@ struct Args {
     unsigned cf[2][2];
     int ex, ey;
     float clip_pt, clip_pt8, scale;
+    const float *d_maxv;  // when set, the demosaic maximum is read from the device
     int ntc, ntiles;
+    int tile_base;        // first tile of this image in a multi-image tile list
     float *arenas;
 };
 
@@ -102,6 +104,7 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
     const int rrmin = top < 0 ? 16 : 0, ccmin = left < 0 ? 16 : 0;
     const int rrmax = bottom > H ? H - top : rr1;
     const int ccmax = right > W ? W - left : cc1;
+    const float scale = A.d_maxv ? 1.0f / *A.d_maxv : A.scale;
 
     // clear the arena (clean-tile contract)
     {
@@ -141,7 +144,7 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
                 row = clampi(rr + top, 0, H - 1); col = clampi(32 - cc + left, 0, W - 1);
                 L = rr * TS + cc;
             }
-            const float v = A.cfa_plane[(size_t)row * W + col] * A.scale;
+            const float v = A.cfa_plane[(size_t)row * W + col] * scale;
             cfa[L] = v;
             green[L] = v;
         }
@@ -150,7 +153,7 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
             for (int t = threadIdx.x; t < 16 * nr; t += BLOCK) {
                 const int rr = rrmin + t / 16, k = t % 16;
                 const int row = clampi(top + rr, 0, H - 1), col = clampi(W - k - 2, 0, W - 1);
-                const float v = A.cfa_plane[(size_t)row * W + col] * A.scale;
+                const float v = A.cfa_plane[(size_t)row * W + col] * scale;
                 cfa[rr * TS + ccmax + k] = v;
                 green[rr * TS + ccmax + k] = v;
             }
@@ -609,6 +612,26 @@ __global__ __launch_bounds__(BLOCK) void amaze_kernel(Args A) {
     }
 }
 
+// One tile queue over several images: tile t belongs to the last image whose
+// tile_base is <= t.  Every workgroup walks the queue with a fixed stride and
+// ends when it passes total_tiles.
+__global__ __launch_bounds__(BLOCK) void amaze_multi_kernel(const Args *__restrict__ tab, int nimg,
+                                                            int total_tiles, float *arenas) {
+    float *ar = arenas + (size_t)blockIdx.x * ARENA_FLOATS;
+    for (int t = blockIdx.x; t < total_tiles; t += gridDim.x) {
+        int lo = 0, hi = nimg - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (tab[mid].tile_base <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        const Args &A = tab[lo];
+        const int lt = t - A.tile_base;
+        const int tr = lt / A.ntc, tc = lt % A.ntc;
+        amaze_tile(A, ar, -16 + tr * (TS - 32), -16 + tc * (TS - 32));
+    }
+}
+
 // LibRaw-style border interpolation (:2984-3028): frame pixels get the 3x3 mean
 // of each non-native colour.  Reads only native channels, writes only
 // non-native ones, so it runs in place.
@@ -665,10 +688,8 @@ size_t lre_amaze_scratch_bytes(lre_ctx *ctx, int w, int h) {
     return (size_t)(amaze_slots(ctx, w, h) + 1) * ARENA_FLOATS * sizeof(float);
 }
 
-int lre_launch_amaze(lre_ctx *ctx, const float *d_cfa, float *d_rgb, int w, int h,
-                     uint32_t filters, const float cam_mul[4], float maximum_value,
-                     void *d_scratch, hipStream_t s) {
-    Args A;
+static void amaze_args(Args &A, const float *d_cfa, float *d_rgb, int w, int h, uint32_t filters,
+                       const float cam_mul[4], float maximum_value, const float *d_maxv) {
     A.cfa_plane = d_cfa;
     A.rgb = d_rgb;
     A.W = w;
@@ -686,13 +707,62 @@ int lre_launch_amaze(lre_ctx *ctx, const float *d_cfa, float *d_rgb, int w, int 
     } else {
         if (A.cf[0][0] == 0) { A.ey = 0; A.ex = 0; } else { A.ey = 1; A.ex = 1; }
     }
+    A.d_maxv = d_maxv;
     A.ntc = tiles_for(w);
     A.ntiles = A.ntc * tiles_for(h);
+    A.tile_base = 0;
+    A.arenas = nullptr;
+}
+
+int lre_launch_amaze(lre_ctx *ctx, const float *d_cfa, float *d_rgb, int w, int h,
+                     uint32_t filters, const float cam_mul[4], float maximum_value,
+                     void *d_scratch, hipStream_t s, const float *d_maxv) {
+    Args A;
+    amaze_args(A, d_cfa, d_rgb, w, h, filters, cam_mul, maximum_value, d_maxv);
     A.arenas = static_cast<float *>(d_scratch);
     const int slots = amaze_slots(ctx, w, h);
     hipLaunchKernelGGL(amaze_kernel, dim3(slots), dim3(BLOCK), 0, s, A);
     LRE_HIP_CHECK(hipGetLastError());
     return lre_launch_border_interpolate(nullptr, d_rgb, w, h, filters, 4, s);
+}
+
+size_t lre_amaze_multi_table_bytes(int n) { return sizeof(Args) * (size_t)n; }
+
+size_t lre_amaze_multi_scratch_bytes(int slots) {
+    return (size_t)(slots + 1) * ARENA_FLOATS * sizeof(float);
+}
+
+int lre_amaze_tiles(int w, int h) { return tiles_for(w) * tiles_for(h); }
+
+int lre_amaze_multi_fill(const lre_demosaic_job *jobs, int n, void *h_table) {
+    Args *tab = static_cast<Args *>(h_table);
+    int total = 0;
+    for (int k = 0; k < n; k++) {
+        const lre_demosaic_job &j = jobs[k];
+        amaze_args(tab[k], j.d_cfa, j.d_rgb, j.w, j.h, j.filters, j.cam_mul, j.maximum_value,
+                   j.d_maxv);
+        tab[k].tile_base = total;
+        total += tab[k].ntiles;
+    }
+    return total;
+}
+
+int lre_launch_amaze_multi(const lre_demosaic_job *jobs, int n, const void *d_table, int total_tiles,
+                           int slots, void *d_scratch, hipStream_t s) {
+    if (n == 0) return LRE_OK;
+    if (total_tiles > 0) {
+        const int grid = slots < total_tiles ? slots : total_tiles;
+        hipLaunchKernelGGL(amaze_multi_kernel, dim3(grid), dim3(BLOCK), 0, s,
+                           static_cast<const Args *>(d_table), n, total_tiles,
+                           static_cast<float *>(d_scratch));
+        LRE_HIP_CHECK(hipGetLastError());
+    }
+    for (int k = 0; k < n; k++) {
+        int rc = lre_launch_border_interpolate(nullptr, jobs[k].d_rgb, jobs[k].w, jobs[k].h,
+                                               jobs[k].filters, 4, s);
+        if (rc) return rc;
+    }
+    return LRE_OK;
 }
 
 int lre_launch_border_interpolate(const float *, float *d_rgb, int w, int h, uint32_t filters,

@@ -489,14 +489,16 @@ int lre_launch_axial_ca(const float *d_in, float *d_out, int w, int h, int radiu
     hipLaunchKernelGGL(init_u32, dim3(1), dim3(64), 0, s, stats, 4);
     hipLaunchKernelGGL(axial_max_g, dim3(grid_for(n)), dim3(BLOCK), 0, s, d_in, n, stats);
     dim3 grid((w + AX_TX - 1) / AX_TX, (h + AX_TY - 1) / AX_TY);
-    static bool attr_set = false;
-    if (!attr_set) {
+    // raise the dynamic LDS limit once per process (thread-safe static initialisation)
+    static const hipError_t attr_rc = [] {
         const int RX = AX_TX + 2 * AX_RMAX + 2, RY = AX_TY + 2 * AX_RMAX + 2;
         const int lds_max = (int)(sizeof(float) * RX * RY * 4 + sizeof(double) * RY * AX_TX);
-        LRE_HIP_CHECK(hipFuncSetAttribute((const void *)axial_ab, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
-        LRE_HIP_CHECK(hipFuncSetAttribute((const void *)axial_apply, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
-        attr_set = true;
-    }
+        hipError_t e = hipFuncSetAttribute((const void *)axial_ab, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute((const void *)axial_apply, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
+        return e;
+    }();
+    LRE_HIP_CHECK(attr_rc);
     {
         const int RX = AX_TX + 2 * radius + 2, RY = AX_TY + 2 * radius + 2;
         const size_t lds = sizeof(float) * RX * RY * 4 + sizeof(double) * (RY - 2) * AX_TX;

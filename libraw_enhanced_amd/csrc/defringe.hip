@@ -114,14 +114,19 @@ struct BlurSet {
     int count;
 };
 
+// normalised taps, passed by value (no host-to-device copy per call)
+struct Taps {
+    float v[MAX_TAPS];
+};
+
 constexpr int HB_W = 256;  // outputs per block along x
 
-__global__ __launch_bounds__(BLOCK) void blur_h(BlurSet S, const float *__restrict__ taps, int W,
-                                                int H, int ks, const unsigned *stats) {
+__global__ __launch_bounds__(BLOCK) void blur_h(BlurSet S, Taps taps, int W, int H, int ks,
+                                                const unsigned *stats) {
     __shared__ float seg[HB_W + 2 * (MAX_TAPS / 2)];
     __shared__ float kt[MAX_TAPS];
     const int y = blockIdx.y, x0 = blockIdx.x * HB_W;
-    for (int t = threadIdx.x; t < 2 * ks + 1; t += BLOCK) kt[t] = taps[t];
+    for (int t = threadIdx.x; t < 2 * ks + 1; t += BLOCK) kt[t] = taps.v[t];
     const float inv_edge = 1.f / mx(__uint_as_float(stats[1]), 1e-6f);
     for (int p = 0; p < S.count; p++) {
         const float *in = S.in[p] + (size_t)y * W;
@@ -146,11 +151,10 @@ constexpr int VB_W = 64, VB_H = 64;
 
 // vertical pass: each workgroup stages (VB_H + 2 ks) rows x 64 columns of the
 // horizontal result in LDS (rows clamped at the frame) and sums the taps
-__global__ __launch_bounds__(BLOCK) void blur_v(BlurSet S, const float *__restrict__ taps, int W,
-                                                int H, int ks) {
+__global__ __launch_bounds__(BLOCK) void blur_v(BlurSet S, Taps taps, int W, int H, int ks) {
     extern __shared__ float col_tile[];  // (VB_H + 2 ks) x VB_W
     __shared__ float kt[MAX_TAPS];
-    for (int t = threadIdx.x; t < 2 * ks + 1; t += BLOCK) kt[t] = taps[t];
+    for (int t = threadIdx.x; t < 2 * ks + 1; t += BLOCK) kt[t] = taps.v[t];
     const int x0 = blockIdx.x * VB_W, y0 = blockIdx.y * VB_H;
     const int rows = VB_H + 2 * ks;
     const int tx = threadIdx.x % VB_W, ty = threadIdx.x / VB_W;
@@ -537,7 +541,6 @@ int lre_launch_defringe(const float *d_in, float *d_out, int w, int h, float rad
     float *fs = take(15), *bfs = take(16), *lfs = take(17);
     float *sr = gl, *sb = gs, *srb = t0, *sbb = t1;  // reuse: the guide planes are dead by then
     unsigned *stats = reinterpret_cast<unsigned *>(base + plane * 20);
-    float *d_taps = reinterpret_cast<float *>(base + plane * 20 + 256);
 
     // Gaussian taps (:4033-4048), computed on the host with the C library's expf
     green_strength_scale = green_strength_scale < 0.f ? 0.f : (1.f < green_strength_scale ? 1.f : green_strength_scale);
@@ -547,17 +550,16 @@ int lre_launch_defringe(const float *d_in, float *d_out, int w, int h, float rad
         lre_set_error("defringe radius %g too large (at most %d taps)", radius, MAX_TAPS);
         return LRE_ERR_ARG;
     }
-    float taps[MAX_TAPS];
+    Taps taps{};
     {
         float ksum = 0.f;
         for (int k = -ks; k <= ks; k++) {
             float v = std::exp(-0.5f * (float)(k * k) / (sigma * sigma));
-            taps[k + ks] = v;
+            taps.v[k + ks] = v;
             ksum += v;
         }
-        for (int k = 0; k < 2 * ks + 1; k++) taps[k] /= ksum;
+        for (int k = 0; k < 2 * ks + 1; k++) taps.v[k] /= ksum;
     }
-    LRE_HIP_CHECK(hipMemcpyAsync(d_taps, taps, sizeof(float) * (2 * ks + 1), hipMemcpyHostToDevice, s));
 
     Scal C;
     C.W = w; C.H = h; C.ks = ks;
@@ -587,9 +589,9 @@ int lre_launch_defringe(const float *d_in, float *d_out, int w, int h, float rad
 
     auto blur = [&](BlurSet S) -> int {
         dim3 gh((w + HB_W - 1) / HB_W, h);
-        hipLaunchKernelGGL(blur_h, gh, dim3(BLOCK), 0, s, S, d_taps, w, h, ks, stats);
+        hipLaunchKernelGGL(blur_h, gh, dim3(BLOCK), 0, s, S, taps, w, h, ks, stats);
         dim3 gv((w + VB_W - 1) / VB_W, (h + VB_H - 1) / VB_H);
-        hipLaunchKernelGGL(blur_v, gv, dim3(BLOCK), sizeof(float) * (VB_H + 2 * ks) * VB_W, s, S, d_taps, w, h, ks);
+        hipLaunchKernelGGL(blur_v, gv, dim3(BLOCK), sizeof(float) * (VB_H + 2 * ks) * VB_W, s, S, taps, w, h, ks);
         LRE_HIP_CHECK(hipGetLastError());
         return LRE_OK;
     };

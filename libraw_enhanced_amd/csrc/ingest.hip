@@ -95,7 +95,29 @@ Black make_black(const int cblack[4], int black, int maximum) {
     return B;
 }
 
+// adjust_maximum (core/libraw_wrapper.cpp:1070-1077) on the device-side data
+// maximum, so the demosaic can follow the ingest without a host round trip
+__global__ void adjust_maximum_kernel(const float *d_data_max, float maximum, float threshold,
+                                      float *d_maxv) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    float m = maximum;
+    if (!(threshold < 0.00001f)) {
+        const float thr = threshold > 0.99999f ? 0.75f : threshold;
+        const float d = *d_data_max;
+        if (d > 0 && d < maximum && d > maximum * thr) m = d;
+    }
+    *d_maxv = m;
+}
+
 }  // namespace
+
+int lre_launch_adjust_maximum(const float *d_data_max, float maximum, float threshold,
+                              float *d_maxv, hipStream_t s) {
+    hipLaunchKernelGGL(adjust_maximum_kernel, dim3(1), dim3(64), 0, s, d_data_max, maximum,
+                       threshold, d_maxv);
+    LRE_HIP_CHECK(hipGetLastError());
+    return LRE_OK;
+}
 
 int lre_launch_black_level(uint16_t *d_img4, int w, int h, const int cblack[4], const int *d_pos,
                            int pos_rows, int pos_cols, int black, int maximum, hipStream_t s) {

@@ -36,10 +36,42 @@ size_t lre_axial_scratch_bytes(int w, int h);
 size_t lre_xtrans_scratch_bytes(lre_ctx *ctx, int w, int h);
 size_t lre_lateral_scratch_bytes(int w, int h, int cell_size);
 
+// One image of a multi-image demosaic launch.  d_maxv, when set, points at the
+// demosaic maximum computed on the device (it then replaces maximum_value).
+struct lre_demosaic_job {
+    const float *d_cfa;
+    float *d_rgb;
+    int w, h;
+    uint32_t filters;
+    signed char xtrans[36];
+    float cam_mul[4];
+    float maximum_value;
+    const float *d_maxv;
+};
+
 // Kernel launchers (device pointers; scratch must already be large enough).
 int lre_launch_amaze(lre_ctx *ctx, const float *d_cfa, float *d_rgb, int w, int h,
                      uint32_t filters, const float cam_mul[4], float maximum_value,
-                     void *d_scratch, hipStream_t s);
+                     void *d_scratch, hipStream_t s, const float *d_maxv = nullptr);
+// Multi-image tile queues.  *_fill writes the per-image table on the host and
+// returns the total tile count (<0 on error); the caller copies it to d_table
+// before the launch.  Table and arena sizes come from the *_bytes helpers.
+size_t lre_amaze_multi_table_bytes(int n);
+size_t lre_amaze_multi_scratch_bytes(int slots);
+int lre_amaze_tiles(int w, int h);
+int lre_amaze_multi_fill(const lre_demosaic_job *jobs, int n, void *h_table);
+int lre_launch_amaze_multi(const lre_demosaic_job *jobs, int n, const void *d_table, int total_tiles,
+                           int slots, void *d_scratch, hipStream_t s);
+size_t lre_xtrans_multi_table_bytes(int n);
+size_t lre_xtrans_multi_scratch_bytes(int slots);
+int lre_xtrans_tiles(int w, int h);
+int lre_xtrans_multi_fill(const lre_demosaic_job *jobs, int n, void *h_table);
+// h_table: the filled host table (read for the per-image border launches)
+int lre_launch_xtrans_multi(const lre_demosaic_job *jobs, int n, const void *h_table, void *d_table,
+                            int total_tiles, int slots, void *d_scratch, hipStream_t s);
+// adjust_maximum on the device: d_maxv = rule(data_max, maximum, threshold)
+int lre_launch_adjust_maximum(const float *d_data_max, float maximum, float threshold,
+                              float *d_maxv, hipStream_t s);
 int lre_launch_border_interpolate(const float *dummy, float *d_rgb, int w, int h,
                                   uint32_t filters, int border, hipStream_t s);
 int lre_launch_tone_mapping(const float *d_in, float *d_out, size_t n, float after_scale,

@@ -1,0 +1,958 @@
+// Many images on one device: the batched develop (multi-image tile queues for
+// the demosaics, remaining stages spread over streams, optional HIP graph) and
+// the request loop (FIFO admission by device footprint, one worker per stream).
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "lre_common.h"
+#include "lre_internal.h"
+
+namespace {
+
+constexpr int MAX_LANES = 8;
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+bool valid_stage(int s) { return s >= LRE_STAGE_DEMOSAIC && s <= LRE_STAGE_TONE_MAPPING; }
+
+int check_stages(const int *stages, int ns) {
+    if (ns < 0 || (ns > 0 && !stages)) {
+        lre_set_error("invalid stage list");
+        return LRE_ERR_ARG;
+    }
+    for (int i = 0; i < ns; i++) {
+        if (!valid_stage(stages[i])) {
+            lre_set_error("unknown stage code %d", stages[i]);
+            return LRE_ERR_ARG;
+        }
+        if (stages[i] == LRE_STAGE_DEMOSAIC && i != 0) {
+            lre_set_error("the demosaic stage must come first");
+            return LRE_ERR_ARG;
+        }
+    }
+    return LRE_OK;
+}
+
+int check_image(const lre_image &im, bool demosaic) {
+    if (im.width < 1 || im.height < 1 || !im.d_rgb) {
+        lre_set_error("image needs a positive size and an output buffer");
+        return LRE_ERR_ARG;
+    }
+    if (demosaic) {
+        if (!im.d_cfa && !im.d_raw) {
+            lre_set_error("image has neither a float nor a uint16 CFA plane");
+            return LRE_ERR_ARG;
+        }
+        if (im.d_cfa && !(im.maximum_value > 0.f)) {
+            lre_set_error("maximum_value must be positive");
+            return LRE_ERR_ARG;
+        }
+        if (!im.d_cfa && !(im.white > im.black)) {
+            lre_set_error("white level must exceed the black level");
+            return LRE_ERR_ARG;
+        }
+    }
+    return LRE_OK;
+}
+
+// device scratch one image needs for the stages after the demosaic
+size_t post_scratch(const lre_image &im, const int *stages, int ns, const lre_chain_params &p) {
+    size_t m = 0;
+    for (int i = 0; i < ns; i++) {
+        size_t b = 0;
+        switch (stages[i]) {
+        case LRE_STAGE_LATERAL_CA: b = lre_lateral_scratch_bytes(im.width, im.height, p.lateral_cell); break;
+        case LRE_STAGE_AXIAL_CA: b = lre_axial_scratch_bytes(im.width, im.height); break;
+        case LRE_STAGE_DEFRINGE: b = lre_defringe_scratch_bytes(im.width, im.height); break;
+        case LRE_STAGE_DETAIL_TM: b = lre_detail_tm_scratch_bytes(im.width, im.height); break;
+        default: break;
+        }
+        m = std::max(m, b);
+    }
+    return m;
+}
+
+// one post-demosaic stage, in place on the image's RGB buffer (the develop
+// pipeline calls every one of them in place, core/libraw_wrapper.cpp:1776-1818)
+int run_post_stage(int stage, const lre_image &im, const lre_chain_params &p, void *scratch,
+                   hipStream_t s) {
+    float *rgb = im.d_rgb;
+    const int w = im.width, h = im.height;
+    switch (stage) {
+    case LRE_STAGE_LATERAL_CA:
+        return lre_launch_lateral_ca(rgb, rgb, w, h, p.lateral_cell, p.lateral_iterations,
+                                     p.lateral_max_shift, p.lateral_min_confidence,
+                                     p.lateral_levels, scratch, s);
+    case LRE_STAGE_AXIAL_CA:
+        return lre_launch_axial_ca(rgb, rgb, w, h, p.axial_radius, p.axial_epsilon,
+                                   p.axial_strength, scratch, s);
+    case LRE_STAGE_DEFRINGE:
+        return lre_launch_defringe(rgb, rgb, w, h, p.defringe_radius, p.defringe_edge_threshold,
+                                   p.defringe_chroma_threshold, p.defringe_strength,
+                                   p.defringe_green, p.defringe_green_scale, scratch, s);
+    case LRE_STAGE_DETAIL_TM:
+        return lre_launch_detail_tonemap(rgb, rgb, w, h, scratch, s);
+    case LRE_STAGE_TONE_MAPPING:
+        return lre_launch_tone_mapping(rgb, rgb, (size_t)w * h * 3, p.tone_after_scale, s);
+    default:
+        return LRE_OK;
+    }
+}
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    // grow-only; the caller has made sure no queued work still uses the old buffer
+    int ensure(size_t want) {
+        if (want <= bytes) return LRE_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        hipError_t e = hipMalloc(&p, want);
+        if (e != hipSuccess) {
+            p = nullptr;
+            lre_set_error("hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
+            return LRE_ERR_NOMEM;
+        }
+        bytes = want;
+        return LRE_OK;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
+struct HostBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t want) {
+        if (want <= bytes) return LRE_OK;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+        hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            p = nullptr;
+            lre_set_error("hipHostMalloc(%zu) failed: %s", want, hipGetErrorString(e));
+            return LRE_ERR_NOMEM;
+        }
+        bytes = want;
+        return LRE_OK;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
+struct Lane {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    DevBuf scratch;
+};
+
+int lane_open(Lane &L) {
+    LRE_HIP_CHECK(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
+    LRE_HIP_CHECK(hipEventCreateWithFlags(&L.done, hipEventDisableTiming));
+    return LRE_OK;
+}
+
+void lane_close(Lane &L) {
+    if (L.stream) (void)hipStreamSynchronize(L.stream);
+    L.scratch.release();
+    if (L.done) (void)hipEventDestroy(L.done);
+    if (L.stream) (void)hipStreamDestroy(L.stream);
+    L.stream = nullptr;
+    L.done = nullptr;
+}
+
+int device_cus(int device) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+        cus = 256;
+    return cus > 0 ? cus : 256;
+}
+
+// longest-processing-time-first assignment of images to lanes
+std::vector<std::vector<int>> assign_lanes(const lre_image *imgs, int n, int lanes) {
+    std::vector<int> order(n);
+    for (int i = 0; i < n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return (size_t)imgs[a].width * imgs[a].height > (size_t)imgs[b].width * imgs[b].height;
+    });
+    std::vector<std::vector<int>> out(lanes);
+    std::vector<size_t> load(lanes, 0);
+    for (int i : order) {
+        int best = 0;
+        for (int k = 1; k < lanes; k++)
+            if (load[k] < load[best]) best = k;
+        out[best].push_back(i);
+        load[best] += (size_t)imgs[i].width * imgs[i].height;
+    }
+    return out;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// batch
+
+struct lre_batch {
+    int device = 0;
+    int num_cus = 256;
+    int lanes = 2;
+    int amaze_per_cu = 8;
+    int xtrans_per_cu = 6;
+    Lane lane[MAX_LANES];
+    DevBuf amaze_arena, xtrans_arena, ingest_ws, d_tab_a, d_tab_x;
+    HostBuf h_tab_a, h_tab_x;
+    hipEvent_t ev_fork = nullptr, ev_ingest = nullptr, ev_demosaic[2] = {nullptr, nullptr};
+    hipEvent_t ev_tables = nullptr;  // the last table upload has run
+    bool tables_pending = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    // tables of the captured graph (kept apart from those of eager runs)
+    DevBuf g_tab_a, g_tab_x;
+};
+
+namespace {
+
+int batch_sync_all(lre_batch *b) {
+    for (int k = 0; k < MAX_LANES; k++)
+        if (b->lane[k].stream) LRE_HIP_CHECK(hipStreamSynchronize(b->lane[k].stream));
+    return LRE_OK;
+}
+
+void drop_graph(lre_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+struct Plan {
+    std::vector<lre_demosaic_job> bayer, xtrans;
+    int tiles_a = 0, tiles_x = 0, slots_a = 0, slots_x = 0;
+    bool demosaic = false;
+    std::vector<size_t> ingest_off;  // per image offset into the ingest workspace (or SIZE_MAX)
+    size_t ingest_bytes = 0;
+    size_t lane_scratch = 0;
+};
+
+int make_plan(lre_batch *b, const lre_image *imgs, int n, const int *stages, int ns,
+              const lre_chain_params &p, Plan &P) {
+    int rc = check_stages(stages, ns);
+    if (rc) return rc;
+    if (n < 0 || (n > 0 && !imgs)) {
+        lre_set_error("invalid image list");
+        return LRE_ERR_ARG;
+    }
+    P.demosaic = ns > 0 && stages[0] == LRE_STAGE_DEMOSAIC;
+    P.ingest_off.assign(n, SIZE_MAX);
+    for (int i = 0; i < n; i++) {
+        if ((rc = check_image(imgs[i], P.demosaic))) return rc;
+        P.lane_scratch = std::max(P.lane_scratch, post_scratch(imgs[i], stages, ns, p));
+        if (!P.demosaic) continue;
+        const lre_image &im = imgs[i];
+        lre_demosaic_job j{};
+        j.d_rgb = im.d_rgb;
+        j.w = im.width;
+        j.h = im.height;
+        j.filters = im.filters;
+        std::memcpy(j.xtrans, im.xtrans, 36);
+        std::memcpy(j.cam_mul, im.cam_mul, sizeof(j.cam_mul));
+        if (im.d_cfa) {
+            j.d_cfa = im.d_cfa;
+            j.maximum_value = im.maximum_value;
+            j.d_maxv = nullptr;
+        } else {
+            // workspace: float CFA plane + {data_max, maximum}
+            P.ingest_off[i] = P.ingest_bytes;
+            P.ingest_bytes += align256((size_t)im.width * im.height * 4) + 256;
+            j.maximum_value = (float)(im.white - im.black);
+        }
+        (im.filters == 9 ? P.xtrans : P.bayer).push_back(j);
+    }
+    for (auto &j : P.bayer) P.tiles_a += lre_amaze_tiles(j.w, j.h);
+    for (auto &j : P.xtrans) P.tiles_x += lre_xtrans_tiles(j.w, j.h);
+    P.slots_a = std::max(1, std::min(P.tiles_a, b->num_cus * b->amaze_per_cu));
+    P.slots_x = std::max(1, std::min(P.tiles_x, b->num_cus * b->xtrans_per_cu));
+    return LRE_OK;
+}
+
+// Sizes every buffer for the plan.  Growing a buffer waits for the device.
+int ensure_buffers(lre_batch *b, const Plan &P, int n) {
+    const size_t need_a = P.bayer.empty() ? 0 : lre_amaze_multi_scratch_bytes(P.slots_a);
+    const size_t need_x = P.xtrans.empty() ? 0 : lre_xtrans_multi_scratch_bytes(P.slots_x);
+    bool grow = P.demosaic && (need_a > b->amaze_arena.bytes || need_x > b->xtrans_arena.bytes ||
+                               P.ingest_bytes > b->ingest_ws.bytes ||
+                               lre_amaze_multi_table_bytes(std::max(n, 1)) > b->d_tab_a.bytes ||
+                               lre_xtrans_multi_table_bytes(std::max(n, 1)) > b->d_tab_x.bytes);
+    for (int k = 0; k < b->lanes; k++) grow = grow || P.lane_scratch > b->lane[k].scratch.bytes;
+    if (!grow) return LRE_OK;
+    int rc = batch_sync_all(b);
+    if (rc) return rc;
+    for (int k = 0; k < b->lanes; k++)
+        if ((rc = b->lane[k].scratch.ensure(P.lane_scratch))) return rc;
+    if (P.demosaic) {
+        if ((rc = b->amaze_arena.ensure(need_a))) return rc;
+        if ((rc = b->xtrans_arena.ensure(need_x))) return rc;
+        if (P.ingest_bytes && (rc = b->ingest_ws.ensure(P.ingest_bytes))) return rc;
+        if ((rc = b->d_tab_a.ensure(lre_amaze_multi_table_bytes(std::max(n, 1))))) return rc;
+        if ((rc = b->d_tab_x.ensure(lre_xtrans_multi_table_bytes(std::max(n, 1))))) return rc;
+        if ((rc = b->h_tab_a.ensure(lre_amaze_multi_table_bytes(std::max(n, 1))))) return rc;
+        if ((rc = b->h_tab_x.ensure(lre_xtrans_multi_table_bytes(std::max(n, 1))))) return rc;
+    }
+    return LRE_OK;
+}
+
+// Fill the per-image tables and copy them to d_a / d_x (stream-ordered on lane 0).
+int upload_tables(lre_batch *b, Plan &P, const lre_image *imgs, int n, void *d_a, void *d_x,
+                  bool synchronous) {
+    // the pinned tables may still be read by the previous upload
+    if (b->tables_pending) LRE_HIP_CHECK(hipEventSynchronize(b->ev_tables));
+    b->tables_pending = false;
+    // point u16 images at their workspace planes
+    char *ws = static_cast<char *>(b->ingest_ws.p);
+    size_t ib = 0, ix = 0;
+    for (int i = 0; i < n; i++) {
+        lre_demosaic_job &j = imgs[i].filters == 9 ? P.xtrans[ix++] : P.bayer[ib++];
+        if (P.ingest_off[i] == SIZE_MAX) continue;
+        // the float CFA plane, then {data maximum, demosaic maximum}
+        j.d_cfa = reinterpret_cast<float *>(ws + P.ingest_off[i]);
+        j.d_maxv = reinterpret_cast<float *>(ws + P.ingest_off[i] + align256((size_t)j.w * j.h * 4)) + 1;
+    }
+    hipStream_t s = b->lane[0].stream;
+    if (!P.bayer.empty()) {
+        lre_amaze_multi_fill(P.bayer.data(), (int)P.bayer.size(), b->h_tab_a.p);
+        LRE_HIP_CHECK(hipMemcpyAsync(d_a, b->h_tab_a.p, lre_amaze_multi_table_bytes((int)P.bayer.size()),
+                                     hipMemcpyHostToDevice, s));
+    }
+    if (!P.xtrans.empty()) {
+        int t = lre_xtrans_multi_fill(P.xtrans.data(), (int)P.xtrans.size(), b->h_tab_x.p);
+        if (t < 0) return t;
+        LRE_HIP_CHECK(hipMemcpyAsync(d_x, b->h_tab_x.p, lre_xtrans_multi_table_bytes((int)P.xtrans.size()),
+                                     hipMemcpyHostToDevice, s));
+    }
+    LRE_HIP_CHECK(hipEventRecord(b->ev_tables, s));
+    b->tables_pending = true;
+    if (synchronous) {
+        LRE_HIP_CHECK(hipEventSynchronize(b->ev_tables));
+        b->tables_pending = false;
+    }
+    return LRE_OK;
+}
+
+// Enqueue the whole batch.  Tables must already be on the device (d_a, d_x).
+int enqueue(lre_batch *b, const Plan &P, const lre_image *imgs, int n, const int *stages, int ns,
+            const lre_chain_params &p, const void *d_a, void *d_x) {
+    Lane *L = b->lane;
+    const int lanes = b->lanes;
+    hipStream_t s0 = L[0].stream;
+    LRE_HIP_CHECK(hipEventRecord(b->ev_fork, s0));
+    for (int k = 1; k < lanes; k++) LRE_HIP_CHECK(hipStreamWaitEvent(L[k].stream, b->ev_fork, 0));
+    int rc;
+    if (P.demosaic) {
+        // uint16 ingest + device-side maximum (lane 0)
+        char *ws = static_cast<char *>(b->ingest_ws.p);
+        for (int i = 0; i < n; i++) {
+            if (P.ingest_off[i] == SIZE_MAX) continue;
+            const lre_image &im = imgs[i];
+            float *cfa = reinterpret_cast<float *>(ws + P.ingest_off[i]);
+            float *stat = reinterpret_cast<float *>(ws + P.ingest_off[i] + align256((size_t)im.width * im.height * 4));
+            if ((rc = lre_launch_ingest_u16(im.d_raw, cfa, im.width, im.height, im.filters,
+                                            im.filters == 9 ? im.xtrans : nullptr, im.cblack,
+                                            im.black, im.white, im.wb, stat, s0)))
+                return rc;
+            if ((rc = lre_launch_adjust_maximum(stat, (float)(im.white - im.black),
+                                                im.adjust_threshold, stat + 1, s0)))
+                return rc;
+        }
+        hipStream_t sx = lanes > 1 ? L[1].stream : s0;
+        if (!P.xtrans.empty() && sx != s0) {
+            LRE_HIP_CHECK(hipEventRecord(b->ev_ingest, s0));
+            LRE_HIP_CHECK(hipStreamWaitEvent(sx, b->ev_ingest, 0));
+        }
+        if (!P.bayer.empty() &&
+            (rc = lre_launch_amaze_multi(P.bayer.data(), (int)P.bayer.size(), d_a, P.tiles_a,
+                                         P.slots_a, b->amaze_arena.p, s0)))
+            return rc;
+        if (!P.xtrans.empty() &&
+            (rc = lre_launch_xtrans_multi(P.xtrans.data(), (int)P.xtrans.size(), b->h_tab_x.p, d_x,
+                                          P.tiles_x, P.slots_x, b->xtrans_arena.p, sx)))
+            return rc;
+        // every lane waits for both demosaic queues
+        LRE_HIP_CHECK(hipEventRecord(b->ev_demosaic[0], s0));
+        if (sx != s0) LRE_HIP_CHECK(hipEventRecord(b->ev_demosaic[1], sx));
+        for (int k = 0; k < lanes; k++) {
+            if (k != 0) LRE_HIP_CHECK(hipStreamWaitEvent(L[k].stream, b->ev_demosaic[0], 0));
+            if (sx != s0 && L[k].stream != sx)
+                LRE_HIP_CHECK(hipStreamWaitEvent(L[k].stream, b->ev_demosaic[1], 0));
+        }
+    }
+    const int first = P.demosaic ? 1 : 0;
+    if (ns > first) {
+        auto plan = assign_lanes(imgs, n, lanes);
+        for (int k = 0; k < lanes; k++)
+            for (int i : plan[k])
+                for (int si = first; si < ns; si++)
+                    if ((rc = run_post_stage(stages[si], imgs[i], p, L[k].scratch.p, L[k].stream)))
+                        return rc;
+    }
+    for (int k = 1; k < lanes; k++) {
+        LRE_HIP_CHECK(hipEventRecord(L[k].done, L[k].stream));
+        LRE_HIP_CHECK(hipStreamWaitEvent(s0, L[k].done, 0));
+    }
+    return LRE_OK;
+}
+
+int batch_run_eager(lre_batch *b, const lre_image *imgs, int n, const int *stages, int ns,
+                    const lre_chain_params *params) {
+    lre_chain_params p;
+    if (params) p = *params;
+    else lre_chain_params_default(&p);
+    Plan P;
+    int rc = make_plan(b, imgs, n, stages, ns, p, P);
+    if (rc) return rc;
+    if ((rc = ensure_buffers(b, P, n))) return rc;
+    if (P.demosaic && (rc = upload_tables(b, P, imgs, n, b->d_tab_a.p, b->d_tab_x.p, false))) return rc;
+    return enqueue(b, P, imgs, n, stages, ns, p, b->d_tab_a.p, b->d_tab_x.p);
+}
+
+}  // namespace
+
+extern "C" {
+
+void lre_chain_params_default(lre_chain_params *p) {
+    if (!p) return;
+    p->defringe_radius = 10.0f;
+    p->defringe_edge_threshold = 0.1f;
+    p->defringe_chroma_threshold = 0.15f;
+    p->defringe_strength = 10.0f;
+    p->defringe_green = 0;
+    p->defringe_green_scale = 0.3f;
+    p->axial_radius = 6;
+    p->axial_epsilon = 1e-4f;
+    p->axial_strength = 0.3f;
+    p->lateral_cell = 96;
+    p->lateral_iterations = 3;
+    p->lateral_max_shift = 6.0f;
+    p->lateral_min_confidence = 0.02f;
+    p->lateral_levels = 3;
+    p->tone_after_scale = 1.0f;
+}
+
+int lre_hip_batch_create(lre_batch **out, int device, int lanes) {
+    if (!out) return LRE_ERR_ARG;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) {
+        lre_set_error("no HIP device %d", device);
+        return LRE_ERR_NODEV;
+    }
+    if (lanes < 1 || lanes > MAX_LANES) {
+        lre_set_error("lanes must be 1..%d", MAX_LANES);
+        return LRE_ERR_ARG;
+    }
+    LRE_HIP_CHECK(hipSetDevice(device));
+    auto *b = new lre_batch();
+    b->device = device;
+    b->num_cus = device_cus(device);
+    b->lanes = lanes;
+    int rc = LRE_OK;
+    for (int k = 0; k < MAX_LANES && !rc; k++) rc = lane_open(b->lane[k]);
+    hipEvent_t *evs[] = {&b->ev_fork, &b->ev_ingest, &b->ev_demosaic[0], &b->ev_demosaic[1], &b->ev_tables};
+    for (auto *e : evs)
+        if (!rc && hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) rc = LRE_ERR_HIP;
+    if (rc) {
+        lre_hip_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return LRE_OK;
+}
+
+int lre_hip_batch_destroy(lre_batch *b) {
+    if (!b) return LRE_OK;
+    (void)hipSetDevice(b->device);
+    batch_sync_all(b);
+    drop_graph(b);
+    for (int k = 0; k < MAX_LANES; k++) lane_close(b->lane[k]);
+    DevBuf *bufs[] = {&b->amaze_arena, &b->xtrans_arena, &b->ingest_ws, &b->d_tab_a, &b->d_tab_x,
+                      &b->g_tab_a, &b->g_tab_x};
+    for (auto *d : bufs) d->release();
+    b->h_tab_a.release();
+    b->h_tab_x.release();
+    hipEvent_t evs[] = {b->ev_fork, b->ev_ingest, b->ev_demosaic[0], b->ev_demosaic[1], b->ev_tables};
+    for (auto e : evs)
+        if (e) (void)hipEventDestroy(e);
+    delete b;
+    return LRE_OK;
+}
+
+int lre_hip_batch_run(lre_batch *b, const lre_image *imgs, int n, const int *stages, int ns,
+                      const lre_chain_params *params) {
+    if (!b) return LRE_ERR_ARG;
+    LRE_HIP_CHECK(hipSetDevice(b->device));
+    return batch_run_eager(b, imgs, n, stages, ns, params);
+}
+
+int lre_hip_batch_synchronize(lre_batch *b) {
+    if (!b) return LRE_ERR_ARG;
+    return batch_sync_all(b);
+}
+
+int lre_hip_batch_capture(lre_batch *b, const lre_image *imgs, int n, const int *stages, int ns,
+                          const lre_chain_params *params) {
+    if (!b) return LRE_ERR_ARG;
+    LRE_HIP_CHECK(hipSetDevice(b->device));
+    lre_chain_params p;
+    if (params) p = *params;
+    else lre_chain_params_default(&p);
+    // one eager run sizes every buffer (no allocation may happen under capture)
+    int rc = batch_run_eager(b, imgs, n, stages, ns, &p);
+    if (rc) return rc;
+    if ((rc = batch_sync_all(b))) return rc;
+    drop_graph(b);
+    Plan P;
+    if ((rc = make_plan(b, imgs, n, stages, ns, p, P))) return rc;
+    if ((rc = ensure_buffers(b, P, n))) return rc;
+    if (P.demosaic) {
+        if ((rc = b->g_tab_a.ensure(lre_amaze_multi_table_bytes(std::max(n, 1))))) return rc;
+        if ((rc = b->g_tab_x.ensure(lre_xtrans_multi_table_bytes(std::max(n, 1))))) return rc;
+        // the graph's own tables, uploaded now and read by every replay
+        if ((rc = upload_tables(b, P, imgs, n, b->g_tab_a.p, b->g_tab_x.p, true))) return rc;
+    }
+    hipStream_t s0 = b->lane[0].stream;
+    LRE_HIP_CHECK(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
+    rc = enqueue(b, P, imgs, n, stages, ns, p, b->g_tab_a.p, b->g_tab_x.p);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s0, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) {
+        lre_set_error("stream capture failed: %s", hipGetErrorString(e));
+        return LRE_ERR_HIP;
+    }
+    b->graph = g;
+    e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        drop_graph(b);
+        lre_set_error("graph instantiation failed: %s", hipGetErrorString(e));
+        return LRE_ERR_HIP;
+    }
+    return LRE_OK;
+}
+
+int lre_hip_batch_replay(lre_batch *b) {
+    if (!b || !b->exec) {
+        lre_set_error("no captured batch");
+        return LRE_ERR_ARG;
+    }
+    LRE_HIP_CHECK(hipSetDevice(b->device));
+    LRE_HIP_CHECK(hipGraphLaunch(b->exec, b->lane[0].stream));
+    return LRE_OK;
+}
+
+int lre_hip_batch_set_config(lre_batch *b, int lanes, int amaze_per_cu, int xtrans_per_cu) {
+    if (!b) return LRE_ERR_ARG;
+    if (lanes < 1 || lanes > MAX_LANES || amaze_per_cu < 1 || amaze_per_cu > 16 ||
+        xtrans_per_cu < 1 || xtrans_per_cu > 16) {
+        lre_set_error("config out of range (lanes 1..%d, slots per CU 1..16)", MAX_LANES);
+        return LRE_ERR_ARG;
+    }
+    int rc = batch_sync_all(b);
+    if (rc) return rc;
+    drop_graph(b);
+    b->lanes = lanes;
+    b->amaze_per_cu = amaze_per_cu;
+    b->xtrans_per_cu = xtrans_per_cu;
+    return LRE_OK;
+}
+
+int lre_hip_batch_get_config(lre_batch *b, int *lanes, int *amaze_per_cu, int *xtrans_per_cu) {
+    if (!b) return LRE_ERR_ARG;
+    if (lanes) *lanes = b->lanes;
+    if (amaze_per_cu) *amaze_per_cu = b->amaze_per_cu;
+    if (xtrans_per_cu) *xtrans_per_cu = b->xtrans_per_cu;
+    return LRE_OK;
+}
+
+int lre_hip_batch_autotune(lre_batch *b, const lre_image *imgs, int n, const int *stages, int ns,
+                           const lre_chain_params *params, int reps, float *best_ms) {
+    if (!b) return LRE_ERR_ARG;
+    LRE_HIP_CHECK(hipSetDevice(b->device));
+    if (reps < 1) reps = 1;
+    struct Cand { int lanes, a, x; };
+    std::vector<Cand> cands;
+    for (int lanes : {1, 2, 3, 4})
+        for (int a : {4, 6, 8, 12})
+            cands.push_back({lanes, a, a <= 8 ? a : 8});
+    hipEvent_t e0, e1;
+    LRE_HIP_CHECK(hipEventCreate(&e0));
+    LRE_HIP_CHECK(hipEventCreate(&e1));
+    float best = 1e30f;
+    Cand pick = {b->lanes, b->amaze_per_cu, b->xtrans_per_cu};
+    int rc = LRE_OK;
+    for (const Cand &c : cands) {
+        if ((rc = lre_hip_batch_set_config(b, c.lanes, c.a, c.x))) break;
+        // warm run (sizes buffers), then timed runs
+        if ((rc = batch_run_eager(b, imgs, n, stages, ns, params))) break;
+        if ((rc = batch_sync_all(b))) break;
+        if (hipEventRecord(e0, b->lane[0].stream) != hipSuccess) { rc = LRE_ERR_HIP; break; }
+        for (int r = 0; r < reps && !rc; r++) rc = batch_run_eager(b, imgs, n, stages, ns, params);
+        if (rc) break;
+        if (hipEventRecord(e1, b->lane[0].stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+            rc = LRE_ERR_HIP;
+            break;
+        }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        ms /= (float)reps;
+        if (ms < best) {
+            best = ms;
+            pick = c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    if (best_ms) *best_ms = best;
+    return lre_hip_batch_set_config(b, pick.lanes, pick.a, pick.x);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// request loop
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+// Device memory for admitted requests, cached by exact size so a steady mix of
+// request sizes stops calling hipMalloc after warm-up.
+struct Pool {
+    std::multimap<size_t, void *> free_blocks;
+    size_t cached = 0;
+    // cap: bytes the cache plus admitted requests may hold before the cache is given back
+    void *get(size_t bytes, size_t in_use, size_t cap) {
+        auto it = free_blocks.find(bytes);
+        if (it != free_blocks.end()) {
+            void *p = it->second;
+            free_blocks.erase(it);
+            cached -= bytes;
+            return p;
+        }
+        if (cached + in_use > cap) trim();
+        void *p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            // give back the cache and retry once
+            trim();
+            if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+        }
+        return p;
+    }
+    void put(void *p, size_t bytes) {
+        free_blocks.emplace(bytes, p);
+        cached += bytes;
+    }
+    void trim() {
+        for (auto &kv : free_blocks) (void)hipFree(kv.second);
+        free_blocks.clear();
+        cached = 0;
+    }
+};
+
+}  // namespace
+
+struct lre_server {
+    int device = 0;
+    int lanes = 1;
+    size_t budget = 0;
+    std::vector<int> stages;
+    lre_chain_params params{};
+    int demosaic_per_cu = 4;  // persistent workgroups per CU of one request's tile queue
+    Clock::time_point t0;
+
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<std::pair<lre_request, double>> pending;
+    std::deque<lre_completion> done;
+    size_t in_use = 0, peak = 0;
+    bool stop = false;
+    Pool pool;
+    std::vector<std::thread> workers;
+    std::vector<Lane> lane;
+
+    double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+};
+
+namespace {
+
+struct Footprint {
+    size_t raw, cfa, rgb, scratch, total;
+};
+
+Footprint footprint(const lre_server *S, const lre_image &im) {
+    Footprint f{};
+    const size_t px = (size_t)im.width * im.height;
+    f.raw = align256(px * 2);
+    f.cfa = align256(px * 4) + 256;
+    f.rgb = align256(px * 12);
+    f.scratch = align256(post_scratch(im, S->stages.data(), (int)S->stages.size(), S->params));
+    f.total = f.raw + f.cfa + f.rgb + f.scratch;
+    return f;
+}
+
+// the demosaic queue of one request: its own arena, sized by tiles
+size_t demosaic_arena(const lre_image &im, int num_cus, int per_cu, int *slots) {
+    if (im.filters == 9) {
+        const int t = lre_xtrans_tiles(im.width, im.height);
+        *slots = std::max(1, std::min(t, num_cus * per_cu));
+        return align256(lre_xtrans_multi_scratch_bytes(*slots)) +
+               align256(lre_xtrans_multi_table_bytes(1));
+    }
+    const int t = lre_amaze_tiles(im.width, im.height);
+    *slots = std::max(1, std::min(t, num_cus * per_cu));
+    return align256(lre_amaze_multi_scratch_bytes(*slots)) + align256(lre_amaze_multi_table_bytes(1));
+}
+
+// One request on one stream, start to finish.  Returns an LRE status.
+int serve_one(lre_server *S, Lane &L, const lre_request &rq, char *dev, const Footprint &f,
+              size_t arena_bytes, int slots, void *h_tab) {
+    const lre_image &im = rq.image;
+    hipStream_t s = L.stream;
+    const size_t px = (size_t)im.width * im.height;
+    uint16_t *d_raw = reinterpret_cast<uint16_t *>(dev);
+    float *d_cfa = reinterpret_cast<float *>(dev + f.raw);
+    float *d_stat = reinterpret_cast<float *>(dev + f.raw + align256(px * 4));
+    float *d_rgb = reinterpret_cast<float *>(dev + f.raw + f.cfa);
+    void *d_scr = dev + f.raw + f.cfa + f.rgb;
+    char *d_arena = dev + f.total;
+    void *d_tab = d_arena + (arena_bytes - align256(im.filters == 9 ? lre_xtrans_multi_table_bytes(1)
+                                                                    : lre_amaze_multi_table_bytes(1)));
+    int rc;
+    LRE_HIP_CHECK(hipMemcpyAsync(d_raw, rq.h_raw, px * 2, hipMemcpyHostToDevice, s));
+    if ((rc = lre_launch_ingest_u16(d_raw, d_cfa, im.width, im.height, im.filters,
+                                    im.filters == 9 ? im.xtrans : nullptr, im.cblack, im.black,
+                                    im.white, im.wb, d_stat, s)))
+        return rc;
+    if ((rc = lre_launch_adjust_maximum(d_stat, (float)(im.white - im.black), im.adjust_threshold,
+                                        d_stat + 1, s)))
+        return rc;
+    lre_demosaic_job j{};
+    j.d_cfa = d_cfa;
+    j.d_rgb = d_rgb;
+    j.w = im.width;
+    j.h = im.height;
+    j.filters = im.filters;
+    std::memcpy(j.xtrans, im.xtrans, 36);
+    std::memcpy(j.cam_mul, im.cam_mul, sizeof(j.cam_mul));
+    j.maximum_value = (float)(im.white - im.black);
+    j.d_maxv = d_stat + 1;
+    const bool has_demosaic = !S->stages.empty() && S->stages[0] == LRE_STAGE_DEMOSAIC;
+    if (has_demosaic) {
+        if (im.filters == 9) {
+            const int t = lre_xtrans_multi_fill(&j, 1, h_tab);
+            if (t < 0) return t;
+            LRE_HIP_CHECK(hipMemcpyAsync(d_tab, h_tab, lre_xtrans_multi_table_bytes(1), hipMemcpyHostToDevice, s));
+            if ((rc = lre_launch_xtrans_multi(&j, 1, h_tab, d_tab, t, slots, d_arena, s))) return rc;
+        } else {
+            const int t = lre_amaze_multi_fill(&j, 1, h_tab);
+            LRE_HIP_CHECK(hipMemcpyAsync(d_tab, h_tab, lre_amaze_multi_table_bytes(1), hipMemcpyHostToDevice, s));
+            if ((rc = lre_launch_amaze_multi(&j, 1, d_tab, t, slots, d_arena, s))) return rc;
+        }
+    }
+    lre_image out = im;
+    out.d_rgb = d_rgb;
+    for (size_t si = has_demosaic ? 1 : 0; si < S->stages.size(); si++)
+        if ((rc = run_post_stage(S->stages[si], out, S->params, d_scr, s))) return rc;
+    LRE_HIP_CHECK(hipMemcpyAsync(rq.h_rgb, d_rgb, px * 12, hipMemcpyDeviceToHost, s));
+    LRE_HIP_CHECK(hipStreamSynchronize(s));
+    return LRE_OK;
+}
+
+void worker_main(lre_server *S, int k) {
+    (void)hipSetDevice(S->device);
+    Lane &L = S->lane[k];
+    const int num_cus = device_cus(S->device);
+    void *h_tab = nullptr;
+    const size_t tab_bytes = std::max(lre_amaze_multi_table_bytes(1), lre_xtrans_multi_table_bytes(1));
+    if (hipHostMalloc(&h_tab, tab_bytes, hipHostMallocDefault) != hipSuccess) h_tab = nullptr;
+    for (;;) {
+        lre_request rq;
+        double t_submit;
+        Footprint f;
+        size_t arena = 0, need = 0;
+        int slots = 1;
+        void *dev = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            // FIFO: only the head request may be admitted
+            for (;;) {
+                if (S->stop) {
+                    if (h_tab) (void)hipHostFree(h_tab);
+                    return;
+                }
+                if (!S->pending.empty()) {
+                    const lre_request &head = S->pending.front().first;
+                    f = footprint(S, head.image);
+                    arena = demosaic_arena(head.image, num_cus, S->demosaic_per_cu, &slots);
+                    need = f.total + arena;
+                    if (S->in_use + need <= S->budget || S->in_use == 0) break;
+                }
+                S->cv_work.wait(lk);
+            }
+            rq = S->pending.front().first;
+            t_submit = S->pending.front().second;
+            S->pending.pop_front();
+            dev = S->pool.get(need, S->in_use + need, S->budget);
+            S->in_use += need;
+            S->peak = std::max(S->peak, S->in_use);
+        }
+        lre_completion c{};
+        c.id = rq.id;
+        c.t_submit = t_submit;
+        c.t_admit = S->now();
+        if (!dev) {
+            lre_set_error("device allocation of %zu bytes failed", need);
+            c.status = LRE_ERR_NOMEM;
+        } else if (!h_tab) {
+            c.status = LRE_ERR_NOMEM;
+        } else {
+            c.status = serve_one(S, L, rq, static_cast<char *>(dev), f, arena, slots, h_tab);
+        }
+        c.t_done = S->now();
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            if (dev) S->pool.put(dev, need);
+            S->in_use -= need;
+            S->done.push_back(c);
+        }
+        S->cv_work.notify_all();
+        S->cv_done.notify_all();
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int lre_hip_serve_create(lre_server **out, int device, int lanes, size_t budget, const int *stages,
+                         int ns, const lre_chain_params *params) {
+    if (!out) return LRE_ERR_ARG;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) {
+        lre_set_error("no HIP device %d", device);
+        return LRE_ERR_NODEV;
+    }
+    if (lanes < 1 || lanes > MAX_LANES || budget == 0) {
+        lre_set_error("lanes must be 1..%d and the budget positive", MAX_LANES);
+        return LRE_ERR_ARG;
+    }
+    int rc = check_stages(stages, ns);
+    if (rc) return rc;
+    LRE_HIP_CHECK(hipSetDevice(device));
+    auto *S = new lre_server();
+    S->device = device;
+    S->lanes = lanes;
+    S->budget = budget;
+    S->stages.assign(stages, stages + ns);
+    if (params) S->params = *params;
+    else lre_chain_params_default(&S->params);
+    S->t0 = Clock::now();
+    S->lane.resize(lanes);
+    for (int k = 0; k < lanes && !rc; k++) rc = lane_open(S->lane[k]);
+    if (rc) {
+        for (auto &L : S->lane) lane_close(L);
+        delete S;
+        return rc;
+    }
+    for (int k = 0; k < lanes; k++) S->workers.emplace_back(worker_main, S, k);
+    *out = S;
+    return LRE_OK;
+}
+
+int lre_hip_serve_submit(lre_server *S, const lre_request *rq) {
+    if (!S || !rq || !rq->h_raw || !rq->h_rgb || rq->image.width < 1 || rq->image.height < 1 ||
+        !(rq->image.white > rq->image.black)) {
+        lre_set_error("invalid request");
+        return LRE_ERR_ARG;
+    }
+    if (rq->image.filters == 9 && 2 * rq->image.width + 2 > 32767) {
+        lre_set_error("X-Trans width %d exceeds the 16-bit hexagon offsets", rq->image.width);
+        return LRE_ERR_ARG;
+    }
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->pending.emplace_back(*rq, S->now());
+    }
+    S->cv_work.notify_all();
+    return LRE_OK;
+}
+
+int lre_hip_serve_poll(lre_server *S, lre_completion *out, int max, int timeout_ms) {
+    if (!S || !out || max < 1) return LRE_ERR_ARG;
+    std::unique_lock<std::mutex> lk(S->mu);
+    if (S->done.empty() && timeout_ms > 0)
+        S->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), [&] { return !S->done.empty(); });
+    int k = 0;
+    while (k < max && !S->done.empty()) {
+        out[k++] = S->done.front();
+        S->done.pop_front();
+    }
+    return k;
+}
+
+double lre_hip_serve_now(lre_server *S) { return S ? S->now() : 0.0; }
+
+int lre_hip_serve_footprint(lre_server *S, size_t *current, size_t *peak) {
+    if (!S) return LRE_ERR_ARG;
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (current) *current = S->in_use;
+    if (peak) *peak = S->peak;
+    return LRE_OK;
+}
+
+int lre_hip_serve_destroy(lre_server *S) {
+    if (!S) return LRE_OK;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stop = true;
+    }
+    S->cv_work.notify_all();
+    for (auto &t : S->workers) t.join();
+    (void)hipSetDevice(S->device);
+    for (auto &L : S->lane) lane_close(L);
+    S->pool.trim();
+    delete S;
+    return LRE_OK;
+}
+
+void *lre_hip_host_alloc(size_t bytes) {
+    void *p = nullptr;
+    if (hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) return nullptr;
+    return p;
+}
+
+int lre_hip_host_free(void *p) {
+    if (p) LRE_HIP_CHECK(hipHostFree(p));
+    return LRE_OK;
+}
+
+}  // extern "C"

@@ -44,6 +44,7 @@ struct XArgs {
     int sgrow, sgcol;
     int rshift[3];
     int ntc, ntiles;
+    int tile_base;  // first tile of this image in a multi-image tile list
     char *arenas;
 };
 
@@ -385,8 +386,32 @@ __global__ __launch_bounds__(BLOCK) void xtrans_kernel(XArgs A) {
         xtrans_tile(A, ar, 3 + (t / A.ntc) * (TS - 16), 3 + (t % A.ntc) * (TS - 16));
 }
 
+// One tile queue over several images (table in device memory): tile t belongs
+// to the last image whose tile_base is <= t.
+__global__ __launch_bounds__(BLOCK) void xtrans_multi_kernel(const XArgs *__restrict__ tab, int nimg,
+                                                             int total_tiles, char *arenas) {
+    char *ar = arenas + (size_t)blockIdx.x * ARENA;
+    for (int t = blockIdx.x; t < total_tiles; t += gridDim.x) {
+        int lo = 0, hi = nimg - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (tab[mid].tile_base <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        const XArgs &A = tab[lo];
+        const int lt = t - A.tile_base;
+        xtrans_tile(A, ar, 3 + (lt / A.ntc) * (TS - 16), 3 + (lt % A.ntc) * (TS - 16));
+    }
+}
+
+// copies a demosaic maximum produced on the device into an image's table entry
+__global__ void xtrans_set_maxv(XArgs *entry, const float *d_maxv) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) entry->maxv = *d_maxv;
+}
+
 // weighted 3x3 border interpolation (:1806-1861) on the 8-px frame
-__global__ void xtrans_border(XArgs A, int border) {
+__global__ void xtrans_border(XArgs A, int border, const float *d_maxv) {
+    if (d_maxv) A.maxv = *d_maxv;
     const int W = A.W, H = A.H;
     const size_t n = (size_t)W * H;
     const float wt[3][3] = {{0.25f, 0.5f, 0.25f}, {0.5f, 0.f, 0.5f}, {0.25f, 0.5f, 0.25f}};
@@ -494,6 +519,7 @@ int lre_launch_xtrans3(lre_ctx *ctx, const float *d_cfa, float *d_rgb, int w, in
     setup(A, xtrans36);
     A.ntc = tiles_for(w);
     A.ntiles = A.ntc * tiles_for(h);
+    A.tile_base = 0;
     A.arenas = static_cast<char *>(d_scratch);
     if (A.ntiles > 0) {
         hipLaunchKernelGGL(xtrans_kernel, dim3(xtrans_slots(ctx, w, h)), dim3(BLOCK), 0, s, A);
@@ -502,7 +528,65 @@ int lre_launch_xtrans3(lre_ctx *ctx, const float *d_cfa, float *d_rgb, int w, in
     const size_t frame = (size_t)w * h;
     size_t blocks = (frame + 255) / 256;
     if (blocks > 8192) blocks = 8192;
-    hipLaunchKernelGGL(xtrans_border, dim3((unsigned)blocks), dim3(256), 0, s, A, 8);
+    hipLaunchKernelGGL(xtrans_border, dim3((unsigned)blocks), dim3(256), 0, s, A, 8,
+                       static_cast<const float *>(nullptr));
     LRE_HIP_CHECK(hipGetLastError());
+    return LRE_OK;
+}
+
+size_t lre_xtrans_multi_table_bytes(int n) { return sizeof(XArgs) * (size_t)n; }
+
+size_t lre_xtrans_multi_scratch_bytes(int slots) { return (size_t)(slots + 1) * ARENA; }
+
+int lre_xtrans_tiles(int w, int h) { return tiles_for(w) * tiles_for(h); }
+
+int lre_xtrans_multi_fill(const lre_demosaic_job *jobs, int n, void *h_table) {
+    XArgs *tab = static_cast<XArgs *>(h_table);
+    int total = 0;
+    for (int k = 0; k < n; k++) {
+        const lre_demosaic_job &j = jobs[k];
+        if (2 * j.w + 2 > 32767 || j.h < 1) {
+            lre_set_error("X-Trans width %d exceeds the 16-bit hexagon offsets", j.w);
+            return LRE_ERR_ARG;
+        }
+        XArgs &A = tab[k];
+        A.cfa = j.d_cfa;
+        A.out = j.d_rgb;
+        A.W = j.w;
+        A.H = j.h;
+        A.maxv = j.maximum_value;
+        setup(A, j.xtrans);
+        A.ntc = tiles_for(j.w);
+        A.ntiles = A.ntc * tiles_for(j.h);
+        A.tile_base = total;
+        A.arenas = nullptr;
+        total += A.ntiles;
+    }
+    return total;
+}
+
+int lre_launch_xtrans_multi(const lre_demosaic_job *jobs, int n, const void *h_table, void *d_table,
+                            int total_tiles, int slots, void *d_scratch, hipStream_t s) {
+    if (n == 0) return LRE_OK;
+    const XArgs *tab = static_cast<const XArgs *>(h_table);
+    XArgs *dtab = static_cast<XArgs *>(d_table);
+    for (int k = 0; k < n; k++)
+        if (jobs[k].d_maxv)
+            hipLaunchKernelGGL(xtrans_set_maxv, dim3(1), dim3(64), 0, s, dtab + k, jobs[k].d_maxv);
+    if (total_tiles > 0) {
+        const int grid = slots < total_tiles ? slots : total_tiles;
+        hipLaunchKernelGGL(xtrans_multi_kernel, dim3(grid), dim3(BLOCK), 0, s,
+                           static_cast<const XArgs *>(dtab), n, total_tiles,
+                           static_cast<char *>(d_scratch));
+        LRE_HIP_CHECK(hipGetLastError());
+    }
+    for (int k = 0; k < n; k++) {
+        const size_t frame = (size_t)jobs[k].w * jobs[k].h;
+        size_t blocks = (frame + 255) / 256;
+        if (blocks > 8192) blocks = 8192;
+        hipLaunchKernelGGL(xtrans_border, dim3((unsigned)blocks), dim3(256), 0, s, tab[k], 8,
+                           jobs[k].d_maxv);
+        LRE_HIP_CHECK(hipGetLastError());
+    }
     return LRE_OK;
 }

@@ -22,10 +22,11 @@ class DevelopChain:
 
     def __init__(self, acc: HIPAccelerator, height: int, width: int, stages=None,
                  filters: int = RGGB, cam_mul=(1.0, 1.0, 1.0, 1.0), maximum_value: float = 1.0,
-                 defringe_params=None):
+                 defringe_params=None, xtrans=None):
         self.acc = acc
         self.h, self.w = height, width
         self.filters = filters
+        self.xtrans = None if xtrans is None else np.ascontiguousarray(np.asarray(xtrans).reshape(36), np.int8)
         self.cam_mul = np.ascontiguousarray(cam_mul, np.float32)
         self.maximum_value = float(maximum_value)
         self.defringe_params = dict(radius=10.0, edge_threshold=0.1, chroma_threshold=0.15,
@@ -42,6 +43,16 @@ class DevelopChain:
             _check(lib.lre_hip_dev_amaze(ctx, d_cfa.ptr, self.rgb.ptr, w, h, self.filters,
                                          self.cam_mul.ctypes.data_as(_f32p), self.maximum_value,
                                          stream))
+        elif name == "xtrans3":
+            _check(lib.lre_hip_dev_xtrans3(ctx, d_cfa.ptr, self.rgb.ptr, w, h,
+                                           self.xtrans.ctypes.data_as(ctypes.POINTER(ctypes.c_int8)),
+                                           self.maximum_value, stream))
+        elif name == "lateral_ca":
+            # in place, as the develop pipeline calls it (core/libraw_wrapper.cpp:1776)
+            _check(lib.lre_hip_dev_lateral_ca(ctx, self.rgb.ptr, self.rgb.ptr, w, h, 96, 3, 6.0, 0.02, 3,
+                                              stream))
+        elif name == "axial_ca":
+            _check(lib.lre_hip_dev_axial_ca(ctx, self.rgb.ptr, self.rgb.ptr, w, h, 6, 1e-4, 0.3, stream))
         elif name == "defringe":
             p = self.defringe_params
             # in place, as the develop pipeline calls it (core/libraw_wrapper.cpp:1804-1818)

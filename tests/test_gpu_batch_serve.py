@@ -1,0 +1,270 @@
+"""Batched develop and request loop on the GPU, through the C ABI.
+
+Every image of a mixed batch (Bayer and X-Trans, float and uint16 input,
+several sizes) must come out bit-identical to the same stages run one image at
+a time through the per-image entry points; a captured graph must reproduce the
+eager batch on every replay; a served request must give the same bytes alone
+and in a mix of concurrent requests; one batch image is also checked against
+the oracle chain within max-abs 2e-4 and PSNR 70 dB.
+"""
+import ctypes as ct
+
+import numpy as np
+import pytest
+
+from _common import RGGB, oracle, oracle_amaze, oracle_detail_tonemap, psnr
+from libraw_enhanced_amd import scene
+
+pytestmark = pytest.mark.gpu
+
+WB = (2.1, 1.0, 1.5, 1.0)
+BLACK, WHITE = 512, 16383
+
+
+@pytest.fixture(scope="module")
+def acc():
+    from libraw_enhanced_amd.accelerator import HIPAccelerator
+    a = HIPAccelerator(0)
+    yield a
+    a.close()
+
+
+def _spec():
+    # (kind, h, w, filters, input)
+    return [("bayer", 240, 320, RGGB, "f32"), ("bayer", 300, 420, 0x16161616, "f32"),
+            ("xtrans", 240, 330, 9, "f32"), ("bayer", 256, 384, RGGB, "u16"),
+            ("xtrans", 252, 330, 9, "u16"), ("bayer", 130, 700, RGGB, "f32")]
+
+
+def _inputs(k, kind, h, w, filters, inp):
+    seed = scene.BASE_SEED + 100 + k
+    if kind == "bayer":
+        cfa = scene.mosaic(scene.render(h, w, seed), seed, filters)
+        xt = None
+    else:
+        cfa = scene.xtrans_scene(h, w, seed, lateral=None)
+        xt = scene.XTRANS
+    if inp == "u16":
+        return scene.quantize_u16(cfa, filters if xt is None else 0, xt), xt
+    return np.ascontiguousarray(cfa, np.float32), xt
+
+
+class _Set:
+    """Device buffers of the mixed batch."""
+
+    def __init__(self, acc, spec=None):
+        from libraw_enhanced_amd import develop as dv
+        self.items = []
+        for k, (kind, h, w, filters, inp) in enumerate(spec or _spec()):
+            data, xt = _inputs(k, kind, h, w, filters, inp)
+            d_in = acc.dev_from_host(data)
+            d_rgb = acc.dev_alloc(h * w * 12)
+            if inp == "u16":
+                im = dv.make_image(w, h, filters=filters, xtrans=xt, cam_mul=WB, d_raw=d_in,
+                                   black=BLACK, white=WHITE, wb=WB, d_rgb=d_rgb)
+            else:
+                im = dv.make_image(w, h, filters=filters, xtrans=xt, d_cfa=d_in, maximum_value=1.0,
+                                   d_rgb=d_rgb)
+            self.items.append(dict(kind=kind, h=h, w=w, filters=filters, inp=inp, data=data, xt=xt,
+                                   d_in=d_in, d_rgb=d_rgb, im=im))
+
+    def images(self):
+        return [it["im"] for it in self.items]
+
+    def results(self):
+        return [it["d_rgb"].download((it["h"], it["w"], 3)) for it in self.items]
+
+
+def _single(acc, it, stages, p):
+    """The same stages one image at a time through the per-image entry points."""
+    from libraw_enhanced_amd.accelerator import native
+    lib, ctx = native(), acc.handle
+    h, w, filters = it["h"], it["w"], it["filters"]
+    d_rgb = acc.dev_alloc(h * w * 12)
+    xt = None if it["xt"] is None else np.ascontiguousarray(it["xt"].reshape(-1), np.int8)
+    xtp = None if xt is None else xt.ctypes.data_as(ct.POINTER(ct.c_int8))
+    if it["inp"] == "u16":
+        d_cfa = acc.dev_alloc(h * w * 4)
+        d_max = acc.dev_alloc(4)
+        wb = np.array(WB, np.float32)
+        cb = (ct.c_int * 4)(0, 0, 0, 0)
+        assert lib.lre_hip_dev_ingest_u16(ctx, it["d_in"].ptr, d_cfa.ptr, w, h, filters, xtp, cb, BLACK,
+                                          WHITE, wb.ctypes.data_as(ct.POINTER(ct.c_float)), d_max.ptr,
+                                          None) == 0
+        acc.synchronize()
+        dmax = float(d_max.download((1,))[0])
+        maxv = float(np.float32(WHITE - BLACK))
+        if 0 < dmax < maxv and dmax > maxv * np.float32(0.75):
+            maxv = dmax
+        cam = np.array(WB, np.float32)
+    else:
+        d_cfa, maxv, cam = it["d_in"], 1.0, np.ones(4, np.float32)
+    for s in stages:
+        if s == "demosaic":
+            if filters == 9:
+                rc = lib.lre_hip_dev_xtrans3(ctx, d_cfa.ptr, d_rgb.ptr, w, h, xtp, maxv, None)
+            else:
+                rc = lib.lre_hip_dev_amaze(ctx, d_cfa.ptr, d_rgb.ptr, w, h, filters,
+                                           cam.ctypes.data_as(ct.POINTER(ct.c_float)), maxv, None)
+        elif s == "axial_ca":
+            rc = lib.lre_hip_dev_axial_ca(ctx, d_rgb.ptr, d_rgb.ptr, w, h, p.axial_radius,
+                                          p.axial_epsilon, p.axial_strength, None)
+        elif s == "lateral_ca":
+            rc = lib.lre_hip_dev_lateral_ca(ctx, d_rgb.ptr, d_rgb.ptr, w, h, p.lateral_cell,
+                                            p.lateral_iterations, p.lateral_max_shift,
+                                            p.lateral_min_confidence, p.lateral_levels, None)
+        elif s == "defringe":
+            rc = lib.lre_hip_dev_defringe(ctx, d_rgb.ptr, d_rgb.ptr, w, h, p.defringe_radius,
+                                          p.defringe_edge_threshold, p.defringe_chroma_threshold,
+                                          p.defringe_strength, p.defringe_green, p.defringe_green_scale,
+                                          None)
+        elif s == "detail_tm":
+            rc = lib.lre_hip_dev_detail_tonemap(ctx, d_rgb.ptr, d_rgb.ptr, w, h, None)
+        elif s == "tone_mapping":
+            rc = lib.lre_hip_dev_tone_mapping(ctx, d_rgb.ptr, d_rgb.ptr, w, h, p.tone_after_scale, None)
+        assert rc == 0, s
+    acc.synchronize()
+    return d_rgb.download((h, w, 3))
+
+
+CHAIN = ("demosaic", "axial_ca", "defringe", "detail_tm")
+
+
+@pytest.mark.parametrize("lanes", [1, 3])
+def test_batch_matches_per_image(acc, lanes):
+    from libraw_enhanced_amd import develop as dv
+    print(f"batch lanes={lanes}", flush=True)
+    S = _Set(acc)
+    p = dv.ChainParams.defaults()
+    b = dv.BatchDeveloper(0, lanes)
+    b.run(S.images(), CHAIN, p)
+    b.synchronize()
+    got = S.results()
+    for it, g in zip(S.items, got):
+        want = _single(acc, it, CHAIN, p)
+        bad = int(np.count_nonzero(g != want))
+        assert bad == 0, f"{it['kind']} {it['h']}x{it['w']} {it['inp']}: {bad} values differ"
+    b.close()
+
+
+def test_batch_other_stage_lists(acc):
+    from libraw_enhanced_amd import develop as dv
+    p = dv.ChainParams.defaults(defringe_green=1, axial_radius=3)
+    spec = _spec()[:3]
+    for stages in [("demosaic",), ("demosaic", "lateral_ca", "tone_mapping"),
+                   ("demosaic", "defringe", "axial_ca", "detail_tm")]:
+        print(f"stages {stages}", flush=True)
+        S = _Set(acc, spec)
+        b = dv.BatchDeveloper(0, 2)
+        b.run(S.images(), stages, p)
+        b.synchronize()
+        for it, g in zip(S.items, S.results()):
+            assert np.array_equal(g, _single(acc, it, stages, p)), (stages, it["kind"])
+        b.close()
+
+
+def test_batch_graph_replays(acc):
+    from libraw_enhanced_amd import develop as dv
+    S = _Set(acc)
+    b = dv.BatchDeveloper(0, 2)
+    b.run(S.images(), CHAIN)
+    b.synchronize()
+    eager = S.results()
+    b.capture(S.images(), CHAIN)
+    for rep in range(3):
+        print(f"replay {rep}", flush=True)
+        for it in S.items:  # clobber the outputs between replays
+            it["d_rgb"].upload(np.full((it["h"], it["w"], 3), np.nan, np.float32))
+        acc.synchronize()
+        b.replay()
+        b.synchronize()
+        for e, g in zip(eager, S.results()):
+            assert np.array_equal(e, g), rep
+        # eager work between replays
+        b.run(S.images()[:2], ("demosaic",))
+        b.synchronize()
+    b.close()
+
+
+def test_batch_autotune_and_config(acc):
+    from libraw_enhanced_amd import develop as dv
+    S = _Set(acc, _spec()[:3])
+    b = dv.BatchDeveloper(0, 2)
+    ms = b.autotune(S.images(), CHAIN)
+    cfg = b.config()
+    assert ms > 0 and 1 <= cfg["lanes"] <= 4 and cfg["amaze_slots_per_cu"] in (4, 6, 8, 12)
+    from libraw_enhanced_amd.accelerator import LibRawInvalidArgument
+    with pytest.raises(LibRawInvalidArgument):
+        b.set_config(0, 8, 6)
+    with pytest.raises(LibRawInvalidArgument):
+        b.run(S.images(), ("axial_ca", "demosaic"))
+    b.close()
+
+
+def test_batch_image_matches_oracle_chain(acc):
+    """u16 Bayer input: ingest + adjust_maximum + AMaZE + axial + defringe + detail TM."""
+    from libraw_enhanced_amd import develop as dv
+    from test_ca_cpu import axial as _axial
+    from test_gpu_develop import _oracle_defringe
+    from test_ingest_cpu import ingest as _ingest
+    spec = [("bayer", 256, 384, RGGB, "u16")]
+    S = _Set(acc, spec)
+    b = dv.BatchDeveloper(0, 1)
+    b.run(S.images(), CHAIN)
+    b.synchronize()
+    got = S.results()[0]
+    it = S.items[0]
+    cfa, dmax = _ingest(it["data"], RGGB, None, black=BLACK, maximum=WHITE, wb=WB)
+    o = oracle()
+    o.oracle_adjust_maximum.restype = ct.c_float
+    maxv = o.oracle_adjust_maximum(ct.c_float(dmax), ct.c_float(WHITE - BLACK), ct.c_float(0.75))
+    want = oracle_amaze(scene.cfa_to_rgb3(cfa), RGGB, WB, maxv)
+    want = _axial(o, "oracle_axial_ca", want, inplace=True)
+    want = _oracle_defringe(want, True)
+    want = oracle_detail_tonemap(want)
+    err = float(np.abs(got - want).max())
+    print(f"batch u16 chain vs oracle: max-abs {err:.3g}")
+    assert err <= 2e-4 and psnr(got, want) >= 70.0
+
+
+def test_serve_alone_and_in_the_mix(acc):
+    from libraw_enhanced_amd import develop as dv
+    spec = _spec()
+    reqs = []
+    for k, (kind, h, w, filters, _) in enumerate(spec):
+        raw, xt = _inputs(k, kind, h, w, filters, "u16")
+        im = dv.make_image(w, h, filters=filters, xtrans=xt, cam_mul=WB, black=BLACK, white=WHITE, wb=WB)
+        reqs.append((im, np.ascontiguousarray(raw), h, w))
+
+    def serve(server, order):
+        outs = {k: np.full((reqs[k][2], reqs[k][3], 3), np.nan, np.float32) for k in order}
+        for k in order:
+            im, raw, h, w = reqs[k]
+            server.submit(k, im, raw.ctypes.data, outs[k].ctypes.data, keep=(raw, outs[k]))
+        done = 0
+        while done < len(order):
+            for c in server.poll(timeout_ms=1000):
+                assert c["status"] == 0, c
+                assert c["t_submit"] <= c["t_admit"] <= c["t_done"]
+                done += 1
+        return outs
+
+    alone = dv.DevelopServer(0, lanes=1, hbm_budget_bytes=8 << 30)
+    solo = {k: serve(alone, [k])[k] for k in range(len(reqs))}
+    alone.close()
+    # all at once, 3 lanes, a budget that admits two or three at a time
+    mix = dv.DevelopServer(0, lanes=3, hbm_budget_bytes=64 << 20)
+    outs = serve(mix, list(range(len(reqs))) * 1)
+    _, peak = mix.footprint()
+    mix.close()
+    for k in range(len(reqs)):
+        assert np.array_equal(solo[k], outs[k]), f"request {k} differs alone vs in the mix"
+    assert peak > 0
+    # the batch path gives the same bytes too
+    S = _Set(acc, [(kind, h, w, f, "u16") for (kind, h, w, f, _) in spec])
+    b = dv.BatchDeveloper(0, 2)
+    b.run(S.images(), CHAIN)
+    b.synchronize()
+    for k, g in enumerate(S.results()):
+        assert np.array_equal(g, solo[k]), f"request {k}: batch and served results differ"
+    b.close()
