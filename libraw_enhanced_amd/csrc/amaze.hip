@@ -28,7 +28,7 @@
 namespace {
 
 constexpr int TS = 160, TSH = 80, N2 = TS * TS;
-constexpr int BLOCK = 256;
+constexpr int BLOCK = 1024;  // one tile per workgroup of 16 waves (measured best)
 constexpr float EPS = 1e-5f, EPSSQ = 1e-10f, ARTHRESH = 0.75f;
 
 // Arena layout in floats (reference :1222-1240).
@@ -39,6 +39,7 @@ constexpr int O_GREEN = 0, O_DSQ = N2, O_WV = 2 * N2, O_WH = 3 * N2, O_VCD = 4 *
               O_CFA = 11 * N2 + 3 * TS * TSH, O_NYQ = 12 * N2 + 3 * TS * TSH,
               O_NYQTEST = O_NYQ + TS * TSH / 4, ARENA_FLOATS = O_NYQTEST + TS * TSH;
 static_assert(ARENA_FLOATS % 64 == 0, "arena stride keeps 256-byte alignment");
+static_assert(2 * BLOCK >= TS, "one site of a tile row per thread in the smoothing passes");
 
 struct Args {
     const float *cfa_plane;
@@ -86,12 +87,21 @@ LRE_DEV void for_sites(const Args &A, int r0, int r1, int c0, int c1, F f) {
 
 LRE_DEV unsigned fct(const Args &A, int r, int c) { return A.cf[r & 1][c & 1]; }
 
+// LDS: the tile's CFA plane, and one half-index plane for the row-sequential
+// smoothing passes
+constexpr int LDS_CFA = N2, LDS_HALF = TS * TSH;
+constexpr size_t LDS_BYTES = sizeof(float) * (LDS_CFA + LDS_HALF);
+
 __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
     constexpr int v1 = TS, v2 = 2 * TS, v3 = 3 * TS;
     constexpr int p1 = -TS + 1, p2 = -2 * TS + 2, p3 = -3 * TS + 3;
     constexpr int m1 = TS + 1, m2 = 2 * TS + 2, m3 = 3 * TS + 3;
     const int W = A.W, H = A.H;
-    float *cfa = ar + O_CFA, *green = ar + O_GREEN;
+    extern __shared__ float lds[];
+    // cfa lives in LDS; writes the tile fill sends past its end land in the
+    // arena right behind the CFA array (the Nyquist flags), as in the reference
+    float *cfa = lds, *cfa_spill = ar + O_CFA, *green = ar + O_GREEN;
+    float *sm = lds + LDS_CFA;
     float *wv = ar + O_WV, *wh = ar + O_WH, *dsq = ar + O_DSQ;
     float *vcd = ar + O_VCD, *hcd = ar + O_HCD, *vcdalt = ar + O_VCDALT, *hcdalt = ar + O_HCDALT;
     float *dgv = ar + O_DGV, *dgh = ar + O_DGH, *cdsq = ar + O_CDSQ, *hvwt = ar + O_HVWT;
@@ -106,11 +116,13 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
     const int ccmax = right > W ? W - left : cc1;
     const float scale = A.d_maxv ? 1.0f / *A.d_maxv : A.scale;
 
-    // clear the arena (clean-tile contract)
+    // clear the arena and the LDS CFA plane (clean-tile contract)
     {
         float4 *a4 = reinterpret_cast<float4 *>(ar);
         const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
         for (int t = threadIdx.x; t < ARENA_FLOATS / 4; t += BLOCK) a4[t] = z;
+        float4 *c4 = reinterpret_cast<float4 *>(cfa);
+        for (int t = threadIdx.x; t < N2 / 4; t += BLOCK) c4[t] = z;
     }
     __syncthreads();
 
@@ -125,8 +137,9 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
         const int n_in = nr * nc;
         const int n_lo = rrmax < rr1 ? 16 * nc : 0;
         const int n_le = ccmin > 0 ? 16 * nr : 0;
-        for (int t = threadIdx.x; t < n_up + n_in + n_lo + n_le; t += BLOCK) {
-            int row, col, L;
+        const int n_all = n_up + n_in + n_lo + n_le;
+        // source pixel and arena position of fill write t
+        auto locate = [&](int t, int &row, int &col, int &L) {
             if (t < n_up) {
                 const int rr = t / nc, cc = ccmin + t % nc;
                 row = clampi(32 - rr + top, 0, H - 1); col = clampi(cc + left, 0, W - 1);
@@ -144,9 +157,31 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
                 row = clampi(rr + top, 0, H - 1); col = clampi(32 - cc + left, 0, W - 1);
                 L = rr * TS + cc;
             }
-            const float v = A.cfa_plane[(size_t)row * W + col] * scale;
-            cfa[L] = v;
-            green[L] = v;
+        };
+        // FU loads in flight per thread before the stores
+        constexpr int FU = 8;
+        for (int base = threadIdx.x; base < n_all; base += BLOCK * FU) {
+            float v[FU];
+            int Ls[FU];
+#pragma unroll
+            for (int u = 0; u < FU; u++) {
+                const int t = base + u * BLOCK;
+                Ls[u] = -1;
+                if (t < n_all) {
+                    int row, col;
+                    locate(t, row, col, Ls[u]);
+                    v[u] = A.cfa_plane[(size_t)row * W + col];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < FU; u++) {
+                const int L = Ls[u];
+                if (L < 0) continue;
+                const float x = v[u] * scale;
+                if (L < N2) cfa[L] = x;
+                else cfa_spill[L] = x;
+                green[L] = x;
+            }
         }
         __syncthreads();
         if (ccmax < cc1)
@@ -154,8 +189,10 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
                 const int rr = rrmin + t / 16, k = t % 16;
                 const int row = clampi(top + rr, 0, H - 1), col = clampi(W - k - 2, 0, W - 1);
                 const float v = A.cfa_plane[(size_t)row * W + col] * scale;
-                cfa[rr * TS + ccmax + k] = v;
-                green[rr * TS + ccmax + k] = v;
+                const int L = rr * TS + ccmax + k;
+                if (L < N2) cfa[L] = v;
+                else cfa_spill[L] = v;
+                green[L] = v;
             }
     }
     __syncthreads();
@@ -208,23 +245,54 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
     });
     __syncthreads();
 
-    // stage 3 (:1376-1397): hcd scans along rows, vcd scans down columns
+    // stage 3 (:1376-1397): hcd scans along rows, vcd scans down columns.  Each
+    // scan reads its predecessor two sites back after that one was updated; the
+    // values ahead of the scan are not yet updated, so they are fetched in
+    // chunks of SK sites before the chunk is processed.
     {
+        constexpr int SK = 4;
         const int nh = 2 * max(rr1 - 8, 0), nv = 2 * max(cc1 - 8, 0);
         for (int t = threadIdx.x; t < nh + nv; t += BLOCK) {
-            if (t < nh) {
+            const bool horiz = t < nh;
+            int base, first, end, step;  // positions base + k * step, k in [first, end)
+            if (horiz) {
                 const int rr = 4 + t / 2, par = t & 1;
-                float prev = hcd[rr * TS + 2 + par];
-                for (int cc = 4 + par; cc < cc1 - 4; cc += 2) {
-                    const int i = rr * TS + cc;
-                    const float c0 = cfa[i], cur = hcd[i], nxt = hcd[i + 2];
-                    const float a0 = hcdalt[i - 2], a1 = hcdalt[i], a2 = hcdalt[i + 2];
+                base = rr * TS; step = 1; first = 4 + par; end = cc1 - 4;
+            } else {
+                const int u = t - nh, cc = 4 + u / 2, par = u & 1;
+                base = cc; step = TS; first = 4 + par; end = rr1 - 4;
+            }
+            float *cd = horiz ? hcd : vcd;
+            const float *alt = horiz ? hcdalt : vcdalt;
+            const int s1 = step, s2 = 2 * step;
+            float prev = cd[base + (first - 2) * step];
+            float a0 = alt[base + (first - 2) * step];
+            float cur = 0.f, a1 = 0.f;
+            if (first < end) {
+                cur = cd[base + first * step];
+                a1 = alt[base + first * step];
+            }
+            for (int p = first; p < end; p += 2 * SK) {
+                float nx[SK], al[SK];
+#pragma unroll
+                for (int u = 0; u < SK; u++) {
+                    const int i = base + (p + 2 * u) * step + s2;
+                    nx[u] = cd[i];
+                    al[u] = alt[i];
+                }
+#pragma unroll
+                for (int u = 0; u < SK; u++) {
+                    const int pos = p + 2 * u;
+                    if (pos >= end) break;
+                    const int i = base + pos * step;
+                    const float c0 = cfa[i], nxt = nx[u], a2 = al[u];
                     float hv = 3.f * (lre::sq(prev) + lre::sq(cur) + lre::sq(nxt)) - lre::sq(prev + cur + nxt);
                     float hav = 3.f * (lre::sq(a0) + lre::sq(a1) + lre::sq(a2)) - lre::sq(a0 + a1 + a2);
                     float h = hav < hv ? a1 : cur;
                     float gh;
+                    const int rr = horiz ? base / TS : pos, cc = horiz ? pos : base;
                     const bool is_g = fct(A, rr, cc) & 1;
-                    const float lf = cfa[i - 1], rt = cfa[i + 1];
+                    const float lf = cfa[i - s1], rt = cfa[i + s1];
                     if (is_g) {
                         gh = -h + c0;
                         if (h > 0) {
@@ -245,44 +313,9 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
                         }
                     }
                     if (gh > A.clip_pt) h = (is_g ? -1.f : 1.f) * (lre::med3(gh, lf, rt) - c0);
-                    hcd[i] = h;
+                    cd[i] = h;
                     prev = h;
-                }
-            } else {
-                const int u = t - nh, cc = 4 + u / 2, par = u & 1;
-                float prev = vcd[(2 + par) * TS + cc];
-                for (int rr = 4 + par; rr < rr1 - 4; rr += 2) {
-                    const int i = rr * TS + cc;
-                    const float c0 = cfa[i], cur = vcd[i], nxt = vcd[i + v2];
-                    const float a0 = vcdalt[i - v2], a1 = vcdalt[i], a2 = vcdalt[i + v2];
-                    float vv = 3.f * (lre::sq(prev) + lre::sq(cur) + lre::sq(nxt)) - lre::sq(prev + cur + nxt);
-                    float vav = 3.f * (lre::sq(a0) + lre::sq(a1) + lre::sq(a2)) - lre::sq(a0 + a1 + a2);
-                    float v = vav < vv ? a1 : cur;
-                    float gv;
-                    const bool is_g = fct(A, rr, cc) & 1;
-                    const float up = cfa[i - v1], dn = cfa[i + v1];
-                    if (is_g) {
-                        gv = -v + c0;
-                        if (v > 0) {
-                            if (3.f * v > (gv + c0)) v = -lre::med3(gv, up, dn) + c0;
-                            else {
-                                float w2 = 1.f - 3.f * v / (EPS + gv + c0);
-                                v = w2 * v + (1.f - w2) * (-lre::med3(gv, up, dn) + c0);
-                            }
-                        }
-                    } else {
-                        gv = v + c0;
-                        if (v < 0) {
-                            if (3.f * v < -(gv + c0)) v = lre::med3(gv, up, dn) - c0;
-                            else {
-                                float w2 = 1.f + 3.f * v / (EPS + gv + c0);
-                                v = w2 * v + (1.f - w2) * (lre::med3(gv, up, dn) - c0);
-                            }
-                        }
-                    }
-                    if (gv > A.clip_pt) v = (is_g ? -1.f : 1.f) * (lre::med3(gv, up, dn) - c0);
-                    vcd[i] = v;
-                    prev = v;
+                    a0 = a1; a1 = a2; cur = nxt;
                 }
             }
         }
@@ -376,18 +409,24 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
     }
     float *dg0 = vcdalt;        // Dgrb0 over vcdalt (half-index)
     float *d2 = ar + O_Q1M;     // Dgrb2 {h,v} pairs over Dgrbsq1m/Dgrbsq1p
-    // hvwt smoothing reads updated upper-diagonal neighbours: row by row
+    // hvwt smoothing reads updated upper-diagonal neighbours: row by row, on a
+    // copy of the half-index plane in LDS (rows 7 .. rr1-8), copied back after
+    // (a row has at most 76 sites, fewer than the workgroup's threads).
+    for (int k = 7 * TSH + threadIdx.x; k < (rr1 - 7) * TSH; k += BLOCK) sm[k] = hvwt[k];
+    __syncthreads();
     for (int rr = 8; rr < rr1 - 8; rr++) {
         const int c0 = 8 + (int)(fct(A, rr, 2) & 1);
-        for (int cc = c0 + 2 * (int)threadIdx.x; cc < cc1 - 8; cc += 2 * BLOCK) {
+        const int cc = c0 + 2 * (int)threadIdx.x;
+        if (cc < cc1 - 8) {
             int i = rr * TS + cc, k = i >> 1;
-            float alt = 0.25f * (hvwt[(i - m1) >> 1] + hvwt[(i + p1) >> 1] + hvwt[(i - p1) >> 1] + hvwt[(i + m1) >> 1]);
-            float hk = hvwt[k];
+            float alt = 0.25f * (sm[(i - m1) >> 1] + sm[(i + p1) >> 1] + sm[(i - p1) >> 1] + sm[(i + m1) >> 1]);
+            float hk = sm[k];
             hk = fabsf(0.5f - hk) < fabsf(0.5f - alt) ? alt : hk;
-            hvwt[k] = hk;
+            sm[k] = hk;
         }
         __syncthreads();
     }
+    for (int k = 8 * TSH + threadIdx.x; k < (rr1 - 8) * TSH; k += BLOCK) hvwt[k] = sm[k];
     for_sites(A, 8, rr1 - 8, 8, cc1 - 8, [&](int rr, int cc) {
         int i = rr * TS + cc, k = i >> 1;
         float hk = hvwt[k];
@@ -491,18 +530,29 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
 
     // stage 9 (:1535-1540): pmwt smoothing reads updated upper-diagonal neighbours
     float *rbint = dsq;
+    // same recurrence for pmwt; rbint of a site needs only that site's final
+    // pmwt, so it is computed in a parallel pass afterwards
+    for (int k = 9 * TSH + threadIdx.x; k < (rr1 - 9) * TSH; k += BLOCK) sm[k] = pmwt[k];
+    __syncthreads();
     for (int rr = 10; rr < rr1 - 10; rr++) {
         const int c0 = 10 + (int)(fct(A, rr, 2) & 1);
-        for (int cc = c0 + 2 * (int)threadIdx.x; cc < cc1 - 10; cc += 2 * BLOCK) {
+        const int cc = c0 + 2 * (int)threadIdx.x;
+        if (cc < cc1 - 10) {
             int i = rr * TS + cc, k = i >> 1;
-            float alt = 0.25f * (pmwt[(i - m1) >> 1] + pmwt[(i + p1) >> 1] + pmwt[(i - p1) >> 1] + pmwt[(i + m1) >> 1]);
-            float pk = pmwt[k];
+            float alt = 0.25f * (sm[(i - m1) >> 1] + sm[(i + p1) >> 1] + sm[(i - p1) >> 1] + sm[(i + m1) >> 1]);
+            float pk = sm[k];
             if (fabsf(0.5f - pk) < fabsf(0.5f - alt)) pk = alt;
-            pmwt[k] = pk;
-            rbint[k] = half(cfa[i] + rbm[k] * (1.f - pk) + rbp[k] * pk);
+            sm[k] = pk;
         }
         __syncthreads();
     }
+    for_sites(A, 10, rr1 - 10, 10, cc1 - 10, [&](int rr, int cc) {
+        const int i = rr * TS + cc, k = i >> 1;
+        const float pk = sm[k];
+        rbint[k] = half(cfa[i] + rbm[k] * (1.f - pk) + rbp[k] * pk);
+    });
+    for (int k = 10 * TSH + threadIdx.x; k < (rr1 - 10) * TSH; k += BLOCK) pmwt[k] = sm[k];
+    __syncthreads();
     // (:1541-1566)
     for_sites(A, 12, rr1 - 12, 12, cc1 - 12, [&](int rr, int cc) {
         int i = rr * TS + cc, k = i >> 1;
@@ -677,9 +727,22 @@ int tiles_for(int n) {
 
 }  // namespace
 
+// raise the dynamic LDS limit of both kernels once per process
+static hipError_t amaze_lds_attr() {
+    static const hipError_t rc = [] {
+        hipError_t e = hipFuncSetAttribute((const void *)amaze_kernel,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_BYTES);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute((const void *)amaze_multi_kernel,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_BYTES);
+        return e;
+    }();
+    return rc;
+}
+
 static int amaze_slots(lre_ctx *ctx, int w, int h) {
     const int ntiles = tiles_for(w) * tiles_for(h);
-    const int cap = (ctx->num_cus > 0 ? ctx->num_cus : 256) * 8;
+    const int cap = (ctx->num_cus > 0 ? ctx->num_cus : 256) * 2;
     return ntiles < cap ? ntiles : cap;
 }
 
@@ -721,7 +784,8 @@ int lre_launch_amaze(lre_ctx *ctx, const float *d_cfa, float *d_rgb, int w, int 
     amaze_args(A, d_cfa, d_rgb, w, h, filters, cam_mul, maximum_value, d_maxv);
     A.arenas = static_cast<float *>(d_scratch);
     const int slots = amaze_slots(ctx, w, h);
-    hipLaunchKernelGGL(amaze_kernel, dim3(slots), dim3(BLOCK), 0, s, A);
+    LRE_HIP_CHECK(amaze_lds_attr());
+    hipLaunchKernelGGL(amaze_kernel, dim3(slots), dim3(BLOCK), LDS_BYTES, s, A);
     LRE_HIP_CHECK(hipGetLastError());
     return lre_launch_border_interpolate(nullptr, d_rgb, w, h, filters, 4, s);
 }
@@ -752,7 +816,8 @@ int lre_launch_amaze_multi(const lre_demosaic_job *jobs, int n, const void *d_ta
     if (n == 0) return LRE_OK;
     if (total_tiles > 0) {
         const int grid = slots < total_tiles ? slots : total_tiles;
-        hipLaunchKernelGGL(amaze_multi_kernel, dim3(grid), dim3(BLOCK), 0, s,
+        LRE_HIP_CHECK(amaze_lds_attr());
+        hipLaunchKernelGGL(amaze_multi_kernel, dim3(grid), dim3(BLOCK), LDS_BYTES, s,
                            static_cast<const Args *>(d_table), n, total_tiles,
                            static_cast<float *>(d_scratch));
         LRE_HIP_CHECK(hipGetLastError());

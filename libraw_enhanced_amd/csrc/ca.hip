@@ -86,17 +86,21 @@ __global__ __launch_bounds__(BLOCK) void axial_ab(AxialArgs A) {
     const int x0 = blockIdx.x * AX_TX, y0 = blockIdx.y * AX_TY;
     const int W = A.W, H = A.H;
     const float inv_norm = 1.f / mx(__uint_as_float(A.stats[0]), 1e-6f);
-    for (int t = threadIdx.x; t < RX * RY; t += BLOCK) {
-        const int y = y0 - r - 1 + t / RX, x = x0 - r - 1 + t % RX;
-        float rv = 0.f, gv = 0.f, bv = 0.f;
-        if (x >= 0 && x < W && y >= 0 && y < H) {
-            const float *p = A.src + ((size_t)y * W + x) * 3;
-            rv = p[0] * inv_norm;
-            gv = p[1] * inv_norm;
-            bv = p[2] * inv_norm;
-        }
-        sR[t] = rv; sG[t] = gv; sB[t] = bv;
-    }
+    lre::stage2d<BLOCK>(
+        RY, RX,
+        [&](int ry, int rx) {
+            const int y = y0 - r - 1 + ry, x = x0 - r - 1 + rx;
+            float3 v = make_float3(0.f, 0.f, 0.f);
+            if (x >= 0 && x < W && y >= 0 && y < H) {
+                const float *p = A.src + ((size_t)y * W + x) * 3;
+                v = make_float3(p[0], p[1], p[2]);
+            }
+            return v;
+        },
+        [&](int ry, int rx, float3 v) {
+            const int t = ry * RX + rx;
+            sR[t] = v.x * inv_norm; sG[t] = v.y * inv_norm; sB[t] = v.z * inv_norm;
+        });
     __syncthreads();
     // the 4 outputs of this thread
     constexpr int PER = AX_TX * AX_TY / BLOCK;
@@ -186,10 +190,13 @@ __global__ __launch_bounds__(BLOCK) void axial_apply(AxialArgs A) {
     const float *planes[4] = {A.aR, A.bR, A.aB, A.bB};
     for (int qi = 0; qi < 4; qi++) {
         const float *pl = planes[qi];
-        for (int t = threadIdx.x; t < RY * RX; t += BLOCK) {
-            const int y = y0 - r + t / RX, x = x0 - r - 1 + t % RX;
-            q[t] = (x >= 0 && x < W && y >= 0 && y < H) ? pl[(size_t)y * W + x] : 0.f;
-        }
+        lre::stage2d<BLOCK>(
+            RY, RX,
+            [&](int ry, int rx) {
+                const int y = y0 - r + ry, x = x0 - r - 1 + rx;
+                return (x >= 0 && x < W && y >= 0 && y < H) ? pl[(size_t)y * W + x] : 0.f;
+            },
+            [&](int ry, int rx, float v) { q[ry * RX + rx] = v; });
         __syncthreads();
         hsum(q, RX, RY, r, hs);
         __syncthreads();

@@ -181,6 +181,94 @@ __global__ __launch_bounds__(BLOCK) void blur_v(BlurSet S, Taps taps, int W, int
     }
 }
 
+// Both passes in one kernel for ks <= FB_KSMAX: a workgroup stages the
+// (64 + 2 ks)^2 input window of a 64 x 64 output tile in LDS (coordinates
+// clamped at the frame), runs the horizontal taps over every staged row (the
+// rows of the separable form's intermediate image at the clamped row
+// indices), then the vertical taps.  Each thread accumulates FB_RB outputs
+// along the filter direction; every output sums its taps in the order
+// k = 0 .. 2 ks, as the separable kernels do.
+constexpr int FB_T = 64, FB_KSMAX = 16, FB_RB = 4;
+static_assert(FB_KSMAX <= (MAX_TAPS - 1) / 2, "fused blur radius within the tap table");
+
+template <int KS>
+__global__ __launch_bounds__(BLOCK) void blur2d(BlurSet S, Taps taps, int W, int H,
+                                                const unsigned *stats) {
+    extern __shared__ float fb[];
+    constexpr int ks = KS, nt = 2 * KS + 1;
+    constexpr int IW = FB_T + 2 * ks, IH = FB_T + 2 * ks;
+    // odd row pitches: lanes that walk down rows hit distinct LDS banks
+    constexpr int PW = IW | 1;
+    constexpr int PH = FB_T + 1;
+    float *win = fb;              // IH x IW input window (pitch PW)
+    float *hor = fb + IH * PW;    // IH x FB_T horizontal result (pitch PH)
+    float kt[nt];                 // taps in registers (the loops below unroll fully)
+#pragma unroll
+    for (int t = 0; t < nt; t++) kt[t] = taps.v[t];
+    const float inv_edge = 1.f / mx(__uint_as_float(stats[1]), 1e-6f);
+    const int x0 = blockIdx.x * FB_T, y0 = blockIdx.y * FB_T;
+    for (int p = 0; p < S.count; p++) {
+        const float *in = S.in[p];
+        const bool scaled = S.scale_by_edge[p] != 0;
+        __syncthreads();
+        lre::stage2d<BLOCK>(
+            IH, IW,
+            [&](int r, int c) {
+                int yy = y0 - ks + r, xx = x0 - ks + c;
+                yy = yy < 0 ? 0 : (yy > H - 1 ? H - 1 : yy);
+                xx = xx < 0 ? 0 : (xx > W - 1 ? W - 1 : xx);
+                return in[(size_t)yy * W + xx];
+            },
+            [&](int r, int c, float v) { win[r * PW + c] = scaled ? v * inv_edge : v; });
+        __syncthreads();
+        for (int t = threadIdx.x; t < IH * (FB_T / FB_RB); t += BLOCK) {
+            const int r = t % IH, c = (t / IH) * FB_RB;
+            const float *row = win + r * PW + c;
+            float acc[FB_RB], w[FB_RB];
+#pragma unroll
+            for (int u = 0; u < FB_RB; u++) { acc[u] = 0.f; w[u] = row[u]; }
+#pragma unroll
+            for (int k = 0; k < nt; k++) {
+                const float tk = kt[k];
+#pragma unroll
+                for (int u = 0; u < FB_RB; u++) acc[u] += tk * w[u];
+                if (k + 1 == nt) break;
+#pragma unroll
+                for (int u = 0; u < FB_RB - 1; u++) w[u] = w[u + 1];
+                w[FB_RB - 1] = row[k + FB_RB];
+            }
+#pragma unroll
+            for (int u = 0; u < FB_RB; u++) hor[r * PH + c + u] = acc[u];
+        }
+        __syncthreads();
+        float *out = S.out[p];
+        for (int t = threadIdx.x; t < (FB_T / FB_RB) * FB_T; t += BLOCK) {
+            const int c = t % FB_T, r = (t / FB_T) * FB_RB;
+            const float *col = hor + r * PH + c;
+            float acc[FB_RB], w[FB_RB];
+#pragma unroll
+            for (int u = 0; u < FB_RB; u++) { acc[u] = 0.f; w[u] = col[u * PH]; }
+#pragma unroll
+            for (int k = 0; k < nt; k++) {
+                const float tk = kt[k];
+#pragma unroll
+                for (int u = 0; u < FB_RB; u++) acc[u] += tk * w[u];
+                if (k + 1 == nt) break;
+#pragma unroll
+                for (int u = 0; u < FB_RB - 1; u++) w[u] = w[u + 1];
+                w[FB_RB - 1] = col[(k + FB_RB) * PH];
+            }
+            const int x = x0 + c;
+            if (x < W)
+#pragma unroll
+                for (int u = 0; u < FB_RB; u++) {
+                    const int y = y0 + r + u;
+                    if (y < H) out[(size_t)y * W + x] = acc[u];
+                }
+        }
+    }
+}
+
 struct Planes {
     const float *src;
     float *dst;
@@ -234,17 +322,21 @@ constexpr int LL_T = 32, LL_HALO = 14, LL_R = LL_T + 2 * LL_HALO;
 __global__ __launch_bounds__(BLOCK) void line_link(Planes P, int W, int H) {
     __shared__ float s_ls[LL_R * LL_R], s_fs[LL_R * LL_R];
     const int x0 = blockIdx.x * LL_T, y0 = blockIdx.y * LL_T;
-    for (int t = threadIdx.x; t < LL_R * LL_R; t += BLOCK) {
-        int y = y0 - LL_HALO + t / LL_R, x = x0 - LL_HALO + t % LL_R;
-        x = x < 0 ? 0 : (x > W - 1 ? W - 1 : x);
-        y = y < 0 ? 0 : (y > H - 1 ? H - 1 : y);
-        const size_t i = (size_t)y * W + x;
-        const float f = P.fs[i];
-        s_fs[t] = f;
-        s_ls[t] = mx(f, P.bfs[i] * 0.55f);
-    }
+    lre::stage2d<BLOCK>(
+        LL_R, LL_R,
+        [&](int r, int c) {
+            int y = y0 - LL_HALO + r, x = x0 - LL_HALO + c;
+            x = x < 0 ? 0 : (x > W - 1 ? W - 1 : x);
+            y = y < 0 ? 0 : (y > H - 1 ? H - 1 : y);
+            const size_t i = (size_t)y * W + x;
+            return make_float2(P.fs[i], P.bfs[i]);
+        },
+        [&](int r, int c, float2 v) {
+            s_fs[r * LL_R + c] = v.x;
+            s_ls[r * LL_R + c] = mx(v.x, v.y * 0.55f);
+        });
     __syncthreads();
-    const int dtx[8] = {1, 0, 1, 1, 2, 1, 2, 1}, dty[8] = {0, 1, 1, -1, 1, 2, -1, -2};
+    // direction d: (tx, ty) = (1,0) (0,1) (1,1) (1,-1) (2,1) (1,2) (2,-1) (1,-2)
     for (int o = threadIdx.x; o < LL_T * LL_T; o += BLOCK) {
         const int ly = o / LL_T, lx = o % LL_T;
         const int x = x0 + lx, y = y0 + ly;
@@ -253,8 +345,10 @@ __global__ __launch_bounds__(BLOCK) void line_link(Planes P, int W, int H) {
         float best = 0.f;
 #pragma unroll
         for (int d = 0; d < 8; d++) {
-            const int step = dty[d] * LL_R + dtx[d];
-            const int nstep = dtx[d] * LL_R - dty[d];  // (nx, ny) = (-ty, tx)
+            const int tx = d == 1 ? 0 : ((d == 4 || d == 6) ? 2 : 1);
+            const int ty = d == 0 ? 0 : (d == 1 || d == 2 || d == 4) ? 1 : (d == 3 || d == 6) ? -1 : (d == 5 ? 2 : -2);
+            const int step = ty * LL_R + tx;
+            const int nstep = tx * LL_R - ty;  // (nx, ny) = (-ty, tx)
             float along = 0.f;
             const int max_step = d < 4 ? 14 : 6;
             for (int s = 1; s <= max_step; s++) {
@@ -588,6 +682,22 @@ int lre_launch_defringe(const float *d_in, float *d_out, int w, int h, float rad
     LRE_HIP_CHECK(hipGetLastError());
 
     auto blur = [&](BlurSet S) -> int {
+        if (ks <= FB_KSMAX) {
+            const int side = FB_T + 2 * ks;
+            const size_t lds = sizeof(float) * ((size_t)side * (side | 1) + (size_t)side * (FB_T + 1));
+            dim3 gb((w + FB_T - 1) / FB_T, (h + FB_T - 1) / FB_T);
+            switch (ks) {
+#define LRE_BLUR2D(K) \
+    case K: hipLaunchKernelGGL(blur2d<K>, gb, dim3(BLOCK), lds, s, S, taps, w, h, stats); break;
+                LRE_BLUR2D(1) LRE_BLUR2D(2) LRE_BLUR2D(3) LRE_BLUR2D(4) LRE_BLUR2D(5) LRE_BLUR2D(6)
+                LRE_BLUR2D(7) LRE_BLUR2D(8) LRE_BLUR2D(9) LRE_BLUR2D(10) LRE_BLUR2D(11)
+                LRE_BLUR2D(12) LRE_BLUR2D(13) LRE_BLUR2D(14) LRE_BLUR2D(15) LRE_BLUR2D(16)
+#undef LRE_BLUR2D
+            default: break;
+            }
+            LRE_HIP_CHECK(hipGetLastError());
+            return LRE_OK;
+        }
         dim3 gh((w + HB_W - 1) / HB_W, h);
         hipLaunchKernelGGL(blur_h, gh, dim3(BLOCK), 0, s, S, taps, w, h, ks, stats);
         dim3 gv((w + VB_W - 1) / VB_W, (h + VB_H - 1) / VB_H);

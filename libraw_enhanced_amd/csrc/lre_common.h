@@ -35,6 +35,46 @@ LRE_DEV float med3(float a, float b, float c) {
     return b;
 }
 
+// q = t / d, r = t % d for 0 <= t < 2^24, 0 < d < 2^12, from a float estimate
+// of the quotient (off by at most one) and one correction step.
+LRE_DEV void divmod(int t, int d, float inv_d, int &q, int &r) {
+    q = (int)((float)t * inv_d);
+    r = t - q * d;
+    if (r < 0) { q--; r += d; }
+    else if (r >= d) { q++; r -= d; }
+}
+
+// Stages a rows x cols window into LDS: element (r, c) = load(r, c), written by
+// store(r, c, value).  Each thread issues U independent global loads before
+// its first LDS store, so their latencies overlap.
+template <int NT, int U = 8, class Load, class Store>
+LRE_DEV void stage2d(int rows, int cols, Load load, Store store) {
+    const int n = rows * cols;
+    const float inv = 1.0f / (float)cols;
+    using V = decltype(load(0, 0));
+    for (int base = (int)threadIdx.x; base < n; base += NT * U) {
+        V v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = base + u * NT;
+            if (t < n) {
+                int r, c;
+                divmod(t, cols, inv, r, c);
+                v[u] = load(r, c);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = base + u * NT;
+            if (t < n) {
+                int r, c;
+                divmod(t, cols, inv, r, c);
+                store(r, c, v[u]);
+            }
+        }
+    }
+}
+
 LRE_DEV float aces(float x) {
     const float a = 2.51f, b = 0.03f, c = 2.43f, d = 0.59f, e = 0.14f;
     return (x * (a * x + b)) / (x * (c * x + d) + e);

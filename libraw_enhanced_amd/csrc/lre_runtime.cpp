@@ -212,8 +212,8 @@ struct lre_batch {
     int device = 0;
     int num_cus = 256;
     int lanes = 2;
-    int amaze_per_cu = 8;
-    int xtrans_per_cu = 6;
+    int amaze_per_cu = 2;
+    int xtrans_per_cu = 4;
     Lane lane[MAX_LANES];
     DevBuf amaze_arena, xtrans_arena, ingest_ws, d_tab_a, d_tab_x;
     HostBuf h_tab_a, h_tab_x;
@@ -599,8 +599,8 @@ int lre_hip_batch_autotune(lre_batch *b, const lre_image *imgs, int n, const int
     struct Cand { int lanes, a, x; };
     std::vector<Cand> cands;
     for (int lanes : {1, 2, 3, 4})
-        for (int a : {4, 6, 8, 12})
-            cands.push_back({lanes, a, a <= 8 ? a : 8});
+        for (int a : {1, 2, 4})
+            cands.push_back({lanes, a, 2 * a});
     hipEvent_t e0, e1;
     LRE_HIP_CHECK(hipEventCreate(&e0));
     LRE_HIP_CHECK(hipEventCreate(&e1));
@@ -685,7 +685,7 @@ struct lre_server {
     size_t budget = 0;
     std::vector<int> stages;
     lre_chain_params params{};
-    int demosaic_per_cu = 4;  // persistent workgroups per CU of one request's tile queue
+    int demosaic_per_cu = 2;  // persistent workgroups per CU of one request's tile queue
     Clock::time_point t0;
 
     std::mutex mu;

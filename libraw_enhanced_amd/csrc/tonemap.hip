@@ -33,61 +33,106 @@ __global__ void aces_kernel(const float4 *__restrict__ in, float4 *__restrict__ 
 constexpr int DR = 12;                  // radius
 constexpr int DTX = 64, DTY = 32;       // output tile
 constexpr int DIX = DTX + 2 * DR, DIY = DTY + 2 * DR;
+constexpr int DIXP = DIX + 1, DTXP = DTX + 1;  // odd LDS row pitches
 constexpr int DBLOCK = 256;
 
 LRE_DEV int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-// Box-filter two planes held in LDS (in0/in1: DIY x DIX, zero outside the
-// image) into out0/out1 (DTY x DTX, in LDS; may alias in0/in1, which are dead
-// after the horizontal pass).  hs0/hs1: DIY x DTX scratch.
-LRE_DEV void box2(const float *in0, const float *in1, float *hs0, float *hs1, float *out0,
-                  float *out1, int x0, int y0, int W, int H) {
-    for (int t = threadIdx.x; t < DIY * DTX; t += DBLOCK) {
-        const int ry = t / DTX, cx = t % DTX;
-        const int x = x0 + cx;
-        const float cnt = (float)(min(W - 1, x + DR) - max(0, x - DR) + 1);
-        float s0 = 0.f, s1 = 0.f;
-        const float *r0 = in0 + ry * DIX + cx, *r1 = in1 + ry * DIX + cx;
+// Box means over LDS windows.  hpass: horizontal window sums of the DIY x DIX
+// window `in` (pitch DIXP, zero outside the image) divided by the clipped tap
+// count, for the DTX output columns, into hs0 (and, with SQ, the sums of the
+// squared values into hs1); pitch DTXP.  vpass: the vertical sums of hs0/hs1
+// into out0/out1 (DTY x DTX).  Each thread sums RB neighbouring outputs along
+// the filter direction with a sliding window; every output adds its 25 taps in
+// order.  Lanes walk down rows; the odd pitches put them in distinct LDS banks.
+constexpr int RB = 4, NT = 2 * DR + 1;
+
+template <bool SQ>
+LRE_DEV void hpass(const float *in, float *hs0, float *hs1, int x0, int W) {
+    for (int t = threadIdx.x; t < DIY * (DTX / RB); t += DBLOCK) {
+        const int ry = t % DIY, cx = (t / DIY) * RB;
+        const float *r0 = in + ry * DIXP + cx;
+        float s0[RB], s1[RB], w[RB];
 #pragma unroll
-        for (int k = 0; k <= 2 * DR; k++) { s0 += r0[k]; s1 += r1[k]; }
-        hs0[t] = s0 / cnt;
-        hs1[t] = s1 / cnt;
-    }
-    __syncthreads();
-    for (int t = threadIdx.x; t < DTY * DTX; t += DBLOCK) {
-        const int oy = t / DTX, cx = t % DTX;
-        const int y = y0 + oy;
-        const float cnt = (float)(min(H - 1, y + DR) - max(0, y - DR) + 1);
-        float s0 = 0.f, s1 = 0.f;
+        for (int u = 0; u < RB; u++) { s0[u] = s1[u] = 0.f; w[u] = r0[u]; }
 #pragma unroll
-        for (int k = 0; k <= 2 * DR; k++) { s0 += hs0[(oy + k) * DTX + cx]; s1 += hs1[(oy + k) * DTX + cx]; }
-        out0[t] = s0 / cnt;
-        out1[t] = s1 / cnt;
+        for (int k = 0; k < NT; k++) {
+#pragma unroll
+            for (int u = 0; u < RB; u++) {
+                s0[u] += w[u];
+                if (SQ) s1[u] += w[u] * w[u];
+            }
+            if (k + 1 < NT) {
+#pragma unroll
+                for (int u = 0; u < RB - 1; u++) w[u] = w[u + 1];
+                w[RB - 1] = r0[k + RB];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < RB; u++) {
+            const int x = x0 + cx + u;
+            const float cnt = (float)(min(W - 1, x + DR) - max(0, x - DR) + 1);
+            hs0[ry * DTXP + cx + u] = s0[u] / cnt;
+            if (SQ) hs1[ry * DTXP + cx + u] = s1[u] / cnt;
+        }
     }
-    __syncthreads();
 }
+
+LRE_DEV void vpass(const float *hs0, const float *hs1, float *out0, float *out1, int y0, int H) {
+    for (int t = threadIdx.x; t < (DTY / RB) * DTX; t += DBLOCK) {
+        const int cx = t % DTX, oy = (t / DTX) * RB;
+        const float *c0 = hs0 + oy * DTXP + cx, *c1 = hs1 + oy * DTXP + cx;
+        float s0[RB], s1[RB], w0[RB], w1[RB];
+#pragma unroll
+        for (int u = 0; u < RB; u++) { s0[u] = s1[u] = 0.f; w0[u] = c0[u * DTXP]; w1[u] = c1[u * DTXP]; }
+#pragma unroll
+        for (int k = 0; k < NT; k++) {
+#pragma unroll
+            for (int u = 0; u < RB; u++) { s0[u] += w0[u]; s1[u] += w1[u]; }
+            if (k + 1 < NT) {
+#pragma unroll
+                for (int u = 0; u < RB - 1; u++) { w0[u] = w0[u + 1]; w1[u] = w1[u + 1]; }
+                w0[RB - 1] = c0[(k + RB) * DTXP];
+                w1[RB - 1] = c1[(k + RB) * DTXP];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < RB; u++) {
+            const int y = y0 + oy + u;
+            const float cnt = (float)(min(H - 1, y + DR) - max(0, y - DR) + 1);
+            out0[(oy + u) * DTX + cx] = s0[u] / cnt;
+            out1[(oy + u) * DTX + cx] = s1[u] / cnt;
+        }
+    }
+}
+
+static_assert(2 * DTY * DTX <= DIY * DIXP, "box outputs fit in the input window");
 
 // pass 1: guide = max(0,R,G,B); a = var/(var+eps), b = mean(1-a) -> planes
 __global__ __launch_bounds__(DBLOCK) void dtm_ab_kernel(const float *__restrict__ rgb,
                                                         float *__restrict__ pa,
                                                         float *__restrict__ pb, int W, int H) {
-    __shared__ float in0[DIY * DIX], in1[DIY * DIX];
-    __shared__ float hs0[DIY * DTX], hs1[DIY * DTX];
-    float *o0 = in0, *o1 = in1;
+    __shared__ float win[DIY * DIXP];
+    __shared__ float hs0[DIY * DTXP], hs1[DIY * DTXP];
+    float *o0 = win, *o1 = win + DTY * DTX;  // the window is dead after hpass
     const int x0 = blockIdx.x * DTX, y0 = blockIdx.y * DTY;
-    for (int t = threadIdx.x; t < DIY * DIX; t += DBLOCK) {
-        const int y = y0 - DR + t / DIX, x = x0 - DR + t % DIX;
-        float g = 0.f, g2 = 0.f;
-        if (x >= 0 && x < W && y >= 0 && y < H) {
-            const float *p = rgb + ((size_t)y * W + x) * 3;
-            g = lre::max_std(0.0f, lre::max_std(p[0], lre::max_std(p[1], p[2])));
-            g2 = g * g;
-        }
-        in0[t] = g;
-        in1[t] = g2;
-    }
+    lre::stage2d<DBLOCK>(
+        DIY, DIX,
+        [&](int r, int c) {
+            const int y = y0 - DR + r, x = x0 - DR + c;
+            float g = 0.f;
+            if (x >= 0 && x < W && y >= 0 && y < H) {
+                const float *p = rgb + ((size_t)y * W + x) * 3;
+                g = lre::max_std(0.0f, lre::max_std(p[0], lre::max_std(p[1], p[2])));
+            }
+            return g;
+        },
+        [&](int r, int c, float g) { win[r * DIXP + c] = g; });
     __syncthreads();
-    box2(in0, in1, hs0, hs1, o0, o1, x0, y0, W, H);
+    hpass<true>(win, hs0, hs1, x0, W);  // sums of g and of g * g
+    __syncthreads();
+    vpass(hs0, hs1, o0, o1, y0, H);
+    __syncthreads();
     for (int t = threadIdx.x; t < DTY * DTX; t += DBLOCK) {
         const int x = x0 + t % DTX, y = y0 + t / DTX;
         if (x >= W || y >= H) continue;
@@ -104,18 +149,26 @@ __global__ __launch_bounds__(DBLOCK) void dtm_apply_kernel(const float *__restri
                                                            const float *__restrict__ pb,
                                                            const float *in, float *out, int W,
                                                            int H) {
-    __shared__ float in0[DIY * DIX], in1[DIY * DIX];
-    __shared__ float hs0[DIY * DTX], hs1[DIY * DTX];
-    float *o0 = in0, *o1 = in1;
+    __shared__ float win[DIY * DIXP];
+    __shared__ float hs0[DIY * DTXP], hs1[DIY * DTXP];
+    float *o0 = win, *o1 = win + DTY * DTX;
     const int x0 = blockIdx.x * DTX, y0 = blockIdx.y * DTY;
-    for (int t = threadIdx.x; t < DIY * DIX; t += DBLOCK) {
-        const int y = y0 - DR + t / DIX, x = x0 - DR + t % DIX;
-        const bool ok = x >= 0 && x < W && y >= 0 && y < H;
-        in0[t] = ok ? pa[(size_t)y * W + x] : 0.f;
-        in1[t] = ok ? pb[(size_t)y * W + x] : 0.f;
+    // the a and b windows pass through the same LDS buffer in turn
+    for (int pl = 0; pl < 2; pl++) {
+        const float *src = pl ? pb : pa;
+        lre::stage2d<DBLOCK>(
+            DIY, DIX,
+            [&](int r, int c) {
+                const int y = y0 - DR + r, x = x0 - DR + c;
+                return (x >= 0 && x < W && y >= 0 && y < H) ? src[(size_t)y * W + x] : 0.f;
+            },
+            [&](int r, int c, float v) { win[r * DIXP + c] = v; });
+        __syncthreads();
+        hpass<false>(win, pl ? hs1 : hs0, nullptr, x0, W);
+        __syncthreads();
     }
+    vpass(hs0, hs1, o0, o1, y0, H);
     __syncthreads();
-    box2(in0, in1, hs0, hs1, o0, o1, x0, y0, W, H);
     for (int t = threadIdx.x; t < DTY * DTX; t += DBLOCK) {
         const int x = x0 + t % DTX, y = y0 + t / DTX;
         if (x >= W || y >= H) continue;

@@ -21,7 +21,7 @@
 namespace {
 
 constexpr int TS = 114;
-constexpr int BLOCK = 256;
+constexpr int BLOCK = 1024;  // one tile per workgroup of 16 waves (measured best)
 constexpr int LW = TS - 8, DW = TS - 10;
 constexpr size_t DIRSZ = (size_t)TS * TS * 3;  // floats per direction buffer
 
@@ -495,7 +495,7 @@ void setup(XArgs &A, const signed char *xt36) {
 
 static int xtrans_slots(lre_ctx *ctx, int w, int h) {
     const int ntiles = tiles_for(w) * tiles_for(h);
-    const int cap = (ctx->num_cus > 0 ? ctx->num_cus : 256) * 6;
+    const int cap = (ctx->num_cus > 0 ? ctx->num_cus : 256) * 4;
     return ntiles < cap ? (ntiles > 0 ? ntiles : 1) : cap;
 }
 

@@ -178,7 +178,7 @@ class BatchDeveloper:
     def replay(self):
         _check(lib().lre_hip_batch_replay(self._h))
 
-    def set_config(self, lanes, amaze_slots_per_cu=8, xtrans_slots_per_cu=6):
+    def set_config(self, lanes, amaze_slots_per_cu=2, xtrans_slots_per_cu=4):
         _check(lib().lre_hip_batch_set_config(self._h, lanes, amaze_slots_per_cu, xtrans_slots_per_cu))
 
     def config(self):

@@ -192,7 +192,7 @@ def test_batch_autotune_and_config(acc):
     b = dv.BatchDeveloper(0, 2)
     ms = b.autotune(S.images(), CHAIN)
     cfg = b.config()
-    assert ms > 0 and 1 <= cfg["lanes"] <= 4 and cfg["amaze_slots_per_cu"] in (4, 6, 8, 12)
+    assert ms > 0 and 1 <= cfg["lanes"] <= 4 and cfg["amaze_slots_per_cu"] in (1, 2, 4)
     from libraw_enhanced_amd.accelerator import LibRawInvalidArgument
     with pytest.raises(LibRawInvalidArgument):
         b.set_config(0, 8, 6)
