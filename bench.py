@@ -122,7 +122,8 @@ def main():
     ap.add_argument("--width", type=int, default=6000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--workload", default="chain24", choices=["chain24", "xtrans26", "batch64", "serve"])
+    ap.add_argument("--workload", default="chain24",
+                    choices=["chain24", "xtrans26", "batch64", "serve", "copy"])
     ap.add_argument("--rates", default="4,8,16", help="serve: arrival rates (requests/s)")
     ap.add_argument("--duration", type=float, default=20.0, help="serve: seconds of arrivals per rate")
     ap.add_argument("--lanes", type=int, default=3, help="serve: requests in flight")
@@ -132,7 +133,8 @@ def main():
               "so N GPUs would each run this same single-GPU workload", file=sys.stderr)
         sys.exit(2)
     if args.workload != "chain24":
-        run = {"xtrans26": bench_xtrans26, "batch64": bench_batch64, "serve": bench_serve}[args.workload]
+        run = {"xtrans26": bench_xtrans26, "batch64": bench_batch64, "serve": bench_serve,
+               "copy": bench_copy}[args.workload]
         print(json.dumps(run(args)), flush=True)
         return
 
@@ -317,6 +319,30 @@ def bench_xtrans26(args):
                                                    threads, "X-Trans")
     acc.close()
     return line
+
+
+def bench_copy(args):
+    """Device-to-device copies of a known size: the reference point that scales
+    the FETCH_SIZE / WRITE_SIZE counters into bytes."""
+    from libraw_enhanced_amd.accelerator import HIPAccelerator, native
+    nbytes = 1 << 30
+    acc = HIPAccelerator(0)
+    a, b = acc.dev_alloc(nbytes), acc.dev_alloc(nbytes)
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpyAsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int,
+                                   ctypes.c_void_p]
+    stream = native().lre_hip_stream(acc.handle)
+    for _ in range(max(1, args.warmup)):
+        assert hip.hipMemcpyAsync(b.ptr, a.ptr, nbytes, 3, stream) == 0  # device to device
+    acc.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        assert hip.hipMemcpyAsync(b.ptr, a.ptr, nbytes, 3, stream) == 0
+    acc.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    acc.close()
+    return {"metric": "device-to-device copy", "value": round(2 * nbytes / dt / 1e9, 1), "unit": "GB/s",
+            "bytes_per_copy": nbytes, "steps": args.steps, "ms_per_copy": round(dt * 1e3, 3)}
 
 
 # (count, height, width, filters) of the 64-image batch

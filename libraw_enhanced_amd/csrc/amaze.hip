@@ -116,13 +116,24 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
     const int ccmax = right > W ? W - left : cc1;
     const float scale = A.d_maxv ? 1.0f / *A.d_maxv : A.scale;
 
-    // clear the arena and the LDS CFA plane (clean-tile contract)
+    // Clean-tile contract: every cell whose pre-tile value can reach the output
+    // reads as zero.  Those are the CFA plane (the fill leaves the corners
+    // unwritten) and the four colour-difference planes outside
+    // [4, rr1-4) x [4, cc1-4), which stage 2 fills; the Nyquist flags are
+    // cleared after stage 5 as in the reference.  The other planes are written
+    // before they are read, or are read only for cells that do not reach the
+    // output (checked against zeroed scratch with poisoned scratch on the CPU).
     {
-        float4 *a4 = reinterpret_cast<float4 *>(ar);
-        const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
-        for (int t = threadIdx.x; t < ARENA_FLOATS / 4; t += BLOCK) a4[t] = z;
         float4 *c4 = reinterpret_cast<float4 *>(cfa);
+        const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
         for (int t = threadIdx.x; t < N2 / 4; t += BLOCK) c4[t] = z;
+        float *cd[4] = {ar + O_VCD, ar + O_HCD, ar + O_VCDALT, ar + O_HCDALT};
+        for (int t = threadIdx.x; t < N2; t += BLOCK) {
+            const int rr = t / TS, cc = t - rr * TS;
+            if (rr < 4 || rr >= rr1 - 4 || cc < 4 || cc >= cc1 - 4)
+#pragma unroll
+                for (int k = 0; k < 4; k++) cd[k][t] = 0.f;
+        }
     }
     __syncthreads();
 
@@ -363,6 +374,10 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
              GGRAD[4] * (g[i - v2 - 1] + g[i - v2 + 1] + g[i - TS - 2] + g[i - TS + 2] + g[i + TS - 2] + g[i + TS + 2] + g[i + v2 - 1] + g[i + v2 + 1]) +
              GGRAD[5] * (g[i - m2] + g[i + p2] + g[i - p2] + g[i + m2]));
     });
+    __syncthreads();
+    // Nyquist flags start at zero (:1432); the clear also wipes what the tile
+    // fill wrote past the end of the CFA array
+    for (int t = threadIdx.x; t < TS * TSH / 4; t += BLOCK) reinterpret_cast<unsigned *>(nyq)[t] = 0u;
     __syncthreads();
     int local_nyq = 0;
     for_sites(A, 6, rr1 - 6, 6, cc1 - 6, [&](int rr, int cc) {

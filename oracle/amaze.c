@@ -97,6 +97,7 @@ typedef struct {
     unsigned cf[2][2];
     int ex, ey;
     float clip_pt, clip_pt8, scale;
+    int minimal_clear; /* clear only the cells a tile may read before writing */
 } amaze_t;
 
 static inline unsigned fct(const amaze_t *P, int r, int c) { return P->cf[r & 1][c & 1]; }
@@ -120,6 +121,23 @@ static void amaze_tile(const amaze_t *P, arena_t *A, int top, int left) {
     const int rrmin = top < 0 ? 16 : 0, ccmin = left < 0 ? 16 : 0;
     const int rrmax = bottom > H ? H - top : rr1;
     const int ccmax = right > W ? W - left : cc1;
+
+    if (P->minimal_clear) {
+        /* The cells of a tile whose value before this tile's writes reaches its
+         * output: the CFA plane (the fill leaves the corners unwritten) and
+         * the colour-difference planes outside [4, rr1-4) x [4, cc1-4) (the
+         * stage-3 scans start from, and stage 4-5 reads, cells next to that
+         * range; the Dgrb0/Dgrb1 overlays read the planes' first half).  The
+         * Nyquist flags are cleared after stage 5 as in the reference.  The
+         * unfilled green corners and the Dgrb2 pairs outside their written
+         * range are read only for sites whose results never reach the output. */
+        memset(A->cfa, 0, sizeof(float) * N2);
+        float *cd[4] = {A->vcd, A->hcd, A->vcdalt, A->hcdalt};
+        for (int k = 0; k < 4; k++)
+            for (int rr = 0; rr < TS; rr++)
+                for (int cc = 0; cc < TS; cc++)
+                    if (rr < 4 || rr >= rr1 - 4 || cc < 4 || cc >= cc1 - 4) cd[k][rr * TS + cc] = 0.f;
+    }
 
     /* tile fill with reflect-101 borders; corners stay as they were (:1326) */
     if (rrmin > 0)
@@ -587,6 +605,7 @@ int oracle_amaze(const float *raw3, float *rgb3, int w, int h, uint32_t filters,
     P.clip_pt = 1.0f / gain;
     P.clip_pt8 = 0.8f / gain;
     P.scale = 1.0f / maximum_value;
+    P.minimal_clear = tile_mode == ORACLE_TILES_POISON_NAN || tile_mode == ORACLE_TILES_POISON_BIG;
     for (int r = 0; r < 2; r++)
         for (int c = 0; c < 2; c++) P.cf[r][c] = bayer3(filters, (unsigned)r, (unsigned)c);
     if (P.cf[0][0] == 1) {
@@ -613,7 +632,14 @@ int oracle_amaze(const float *raw3, float *rgb3, int w, int h, uint32_t filters,
             arena_init(&A, mem);
 #pragma omp for schedule(dynamic)
             for (int t = 0; t < ntr * ntc; t++) {
-                memset(mem, 0, arena_bytes);
+                if (P.minimal_clear) {
+                    /* stale scratch: NaN, or a large finite value */
+                    const uint32_t pat = tile_mode == ORACLE_TILES_POISON_NAN ? 0x7fc0a5a5u : 0x4b4b4b4bu;
+                    uint32_t *u = (uint32_t *)mem;
+                    for (size_t k = 0; k < arena_bytes / 4; k++) u[k] = pat;
+                } else {
+                    memset(mem, 0, arena_bytes);
+                }
                 amaze_tile(&P, &A, -16 + (t / ntc) * (TS - 32), -16 + (t % ntc) * (TS - 32));
             }
             free(mem);

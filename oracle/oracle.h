@@ -21,8 +21,14 @@ extern "C" {
  *  ORACLE_TILES_SEQUENTIAL: one scratch buffer, zeroed once, reused by the tiles
  *                           in raster order — what the reference does with one
  *                           OpenMP thread (its per-thread buffer is reused and
- *                           some reads see the previous tile's values). */
-enum { ORACLE_TILES_CLEAN = 0, ORACLE_TILES_SEQUENTIAL = 1 };
+ *                           some reads see the previous tile's values).
+ *  ORACLE_TILES_POISON_NAN / _BIG (AMaZE only): every tile starts from scratch
+ *                           filled with NaN / a large value, and only the cells
+ *                           the HIP kernel clears are zeroed; equal to CLEAN
+ *                           exactly when that clear set covers every cell a
+ *                           tile reads before writing. */
+enum { ORACLE_TILES_CLEAN = 0, ORACLE_TILES_SEQUENTIAL = 1, ORACLE_TILES_POISON_NAN = 2,
+       ORACLE_TILES_POISON_BIG = 3 };
 
 int oracle_amaze(const float *raw3, float *rgb3, int w, int h, uint32_t filters,
                  const float *cam_mul4, float maximum_value, int tile_mode);

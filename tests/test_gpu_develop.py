@@ -29,7 +29,7 @@ def _bayer(h, w, seed, filters=RGGB):
     return scene.cfa_to_rgb3(cfa, filters)
 
 
-@pytest.mark.parametrize("h,w", [(64, 80), (192, 256), (333, 517), (1024, 1536), (1000, 1500)])
+@pytest.mark.parametrize("h,w", [(64, 80), (192, 256), (333, 517), (1024, 1536), (1000, 1500), (130, 700), (258, 141)])
 def test_amaze_bit_exact(acc, h, w):
     print(f"amaze {h}x{w}", flush=True)
     raw3 = _bayer(h, w, scene.BASE_SEED + h)
