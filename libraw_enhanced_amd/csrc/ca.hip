@@ -60,16 +60,61 @@ struct AxialArgs {
     const unsigned *stats;
 };
 
-// Horizontal FP64 window sums of an LDS float region (rows ry in [0,RY),
-// region column 0 = image column x0 - r - 1) into hs[RY][AX_TX], for output
-// columns x0 .. x0+AX_TX-1; region values outside the image are 0.
-LRE_DEV void hsum(const float *reg, int RX, int RY, int r, double *hs) {
-    for (int t = threadIdx.x; t < RY * AX_TX; t += BLOCK) {
-        const int ry = t / AX_TX, cx = t % AX_TX;
-        const float *row = reg + ry * RX + cx + 1;  // region col cx+1 = image col x0+cx-r
-        double s = 0.0;
-        for (int k = 0; k <= 2 * r; k++) s += (double)row[k];
-        hs[t] = s;
+// Horizontal FP64 window sums of an LDS float region (rows ry in [0,RY), pitch
+// RXP, region column 0 = image column x0 - r - 1) into hs[RY][AX_HP], for output
+// columns x0 .. x0+AX_TX-1; region values outside the image are 0.  Each thread
+// sums 4 neighbouring outputs of one row with a sliding window; lanes walk down
+// rows (odd pitches: distinct LDS banks).  Every output adds its taps in order.
+constexpr int AX_HP = 64 + 1;  // hs pitch in doubles
+constexpr int AX_RB = 4;
+
+LRE_DEV void hsum(const float *reg, int RXP, int RY, int r, double *hs) {
+    const int nt = 2 * r + 1;
+    for (int t = threadIdx.x; t < RY * (AX_TX / AX_RB); t += BLOCK) {
+        const int ry = t % RY, cx = (t / RY) * AX_RB;
+        const float *row = reg + ry * RXP + cx + 1;  // region col cx+1 = image col x0+cx-r
+        double s[AX_RB];
+        float w[AX_RB];
+#pragma unroll
+        for (int u = 0; u < AX_RB; u++) { s[u] = 0.0; w[u] = row[u]; }
+        for (int k = 0; k < nt; k++) {
+#pragma unroll
+            for (int u = 0; u < AX_RB; u++) s[u] += (double)w[u];
+            if (k + 1 == nt) break;
+#pragma unroll
+            for (int u = 0; u < AX_RB - 1; u++) w[u] = w[u + 1];
+            w[AX_RB - 1] = row[k + AX_RB];
+        }
+#pragma unroll
+        for (int u = 0; u < AX_RB; u++) hs[ry * AX_HP + cx + u] = s[u];
+    }
+}
+
+// Vertical sums of hs for output rows oy .. oy+3 of column cx (each in order),
+// divided by the true tap count of the clipped window.
+LRE_DEV void vmean(const double *hs, int r, int cx, int oy, int x0, int y0, int W, int H,
+                   double out[AX_RB]) {
+    const int nt = 2 * r + 1;
+    const double *col = hs + oy * AX_HP + cx;
+    double s[AX_RB], w[AX_RB];
+#pragma unroll
+    for (int u = 0; u < AX_RB; u++) { s[u] = 0.0; w[u] = col[u * AX_HP]; }
+    for (int j = 0; j < nt; j++) {
+#pragma unroll
+        for (int u = 0; u < AX_RB; u++) s[u] += w[u];
+        if (j + 1 == nt) break;
+#pragma unroll
+        for (int u = 0; u < AX_RB - 1; u++) w[u] = w[u + 1];
+        w[AX_RB - 1] = col[(j + AX_RB) * AX_HP];
+    }
+    const int x = x0 + cx;
+    const int xl = max(0, x - r), xr = min(W - 1, x + r);
+#pragma unroll
+    for (int u = 0; u < AX_RB; u++) {
+        const int y = y0 + oy + u;
+        const int yl = max(0, y - r), yb = min(H - 1, y + r);
+        const int area = (xr - xl + 1) * (yb - yl + 1);
+        out[u] = area > 0 ? s[u] / (double)area : 0.0;
     }
 }
 
@@ -78,11 +123,12 @@ __global__ __launch_bounds__(BLOCK) void axial_ab(AxialArgs A) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int r = A.r;
     const int RX = AX_TX + 2 * r + 2, RY = AX_TY + 2 * r + 2;  // one extra ring for the gradient
+    const int RXP = RX | 1;                              // odd LDS pitch
     float *sR = reinterpret_cast<float *>(smem);
-    float *sG = sR + RX * RY;
-    float *sB = sG + RX * RY;
-    float *q = sB + RX * RY;                             // derived quantity, (RY-2) x RX
-    double *hs = reinterpret_cast<double *>(q + RX * RY);  // (RY-2) x AX_TX
+    float *sG = sR + RXP * RY;
+    float *sB = sG + RXP * RY;
+    float *q = sB + RXP * RY;                            // derived quantity, (RY-2) x RX
+    double *hs = reinterpret_cast<double *>(q + RXP * RY);  // (RY-2) x AX_HP
     const int x0 = blockIdx.x * AX_TX, y0 = blockIdx.y * AX_TY;
     const int W = A.W, H = A.H;
     const float inv_norm = 1.f / mx(__uint_as_float(A.stats[0]), 1e-6f);
@@ -98,19 +144,21 @@ __global__ __launch_bounds__(BLOCK) void axial_ab(AxialArgs A) {
             return v;
         },
         [&](int ry, int rx, float3 v) {
-            const int t = ry * RX + rx;
+            const int t = ry * RXP + rx;
             sR[t] = v.x * inv_norm; sG[t] = v.y * inv_norm; sB[t] = v.z * inv_norm;
         });
     __syncthreads();
-    // the 4 outputs of this thread
-    constexpr int PER = AX_TX * AX_TY / BLOCK;
+    // this thread's outputs: rows oy .. oy+3 of column cx
+    constexpr int PER = AX_RB;
+    static_assert(AX_TX * AX_TY == BLOCK * AX_RB, "one 4-row strip per thread");
+    const int cx = threadIdx.x % AX_TX, oy = (threadIdx.x / AX_TX) * AX_RB;
     double mI[PER], mII[PER], mR[PER], mIR[PER], mB[PER], mIB[PER], mE[PER];
     const int RYi = RY - 2;  // rows of the window region (one ring dropped)
     for (int qi = 0; qi < 7; qi++) {
         // derived quantity over the window region rows [1, RY-1), all columns
         for (int t = threadIdx.x; t < RYi * RX; t += BLOCK) {
             const int ry = t / RX + 1, rx = t % RX;
-            const int c = ry * RX + rx;
+            const int c = ry * RXP + rx;
             float v;
             switch (qi) {
                 case 0: v = sG[c]; break;
@@ -124,41 +172,34 @@ __global__ __launch_bounds__(BLOCK) void axial_ab(AxialArgs A) {
                     v = 0.f;
                     if (x >= 1 && x < W - 1 && y >= 1 && y < H - 1) {
                         const float gx = sG[c + 1] - sG[c - 1];
-                        const float gy = sG[c + RX] - sG[c - RX];
+                        const float gy = sG[c + RXP] - sG[c - RXP];
                         v = sqrtf(gx * gx + gy * gy);
                     }
                 }
             }
-            q[t] = v;
+            q[(ry - 1) * RXP + rx] = v;
         }
         __syncthreads();
-        hsum(q, RX, RYi, r, hs);
+        hsum(q, RXP, RYi, r, hs);
         __syncthreads();
+        double m[PER];
+        vmean(hs, r, cx, oy, x0, y0, W, H, m);
+#pragma unroll
         for (int k = 0; k < PER; k++) {
-            const int o = threadIdx.x + k * BLOCK;
-            const int oy = o / AX_TX, cx = o % AX_TX;
-            double s = 0.0;
-            for (int j = 0; j <= 2 * r; j++) s += hs[(oy + j) * AX_TX + cx];
-            const int x = x0 + cx, y = y0 + oy;
-            const int xl = max(0, x - r), yl = max(0, y - r);
-            const int xr = min(W - 1, x + r), yb = min(H - 1, y + r);
-            const int area = (xr - xl + 1) * (yb - yl + 1);
-            const double m = area > 0 ? s / (double)area : 0.0;
             switch (qi) {
-                case 0: mI[k] = m; break;
-                case 1: mII[k] = m; break;
-                case 2: mR[k] = m; break;
-                case 3: mIR[k] = m; break;
-                case 4: mB[k] = m; break;
-                case 5: mIB[k] = m; break;
-                default: mE[k] = m;
+                case 0: mI[k] = m[k]; break;
+                case 1: mII[k] = m[k]; break;
+                case 2: mR[k] = m[k]; break;
+                case 3: mIR[k] = m[k]; break;
+                case 4: mB[k] = m[k]; break;
+                case 5: mIB[k] = m[k]; break;
+                default: mE[k] = m[k];
             }
         }
         __syncthreads();
     }
     for (int k = 0; k < PER; k++) {
-        const int o = threadIdx.x + k * BLOCK;
-        const int x = x0 + o % AX_TX, y = y0 + o / AX_TX;
+        const int x = x0 + cx, y = y0 + oy + k;
         if (x >= W || y >= H) continue;
         const size_t i = (size_t)y * W + x;
         const double var = mII[k] - mI[k] * mI[k];
@@ -179,13 +220,15 @@ __global__ __launch_bounds__(BLOCK) void axial_apply(AxialArgs A) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int r = A.r;
     const int RX = AX_TX + 2 * r + 2, RY = AX_TY + 2 * r;
+    const int RXP = RX | 1;
     float *q = reinterpret_cast<float *>(smem);
-    double *hs = reinterpret_cast<double *>(q + RX * RY);
+    double *hs = reinterpret_cast<double *>(q + RXP * RY);
     const int x0 = blockIdx.x * AX_TX, y0 = blockIdx.y * AX_TY;
     const int W = A.W, H = A.H;
     const float norm_ref = mx(__uint_as_float(A.stats[0]), 1e-6f);
     const float inv_norm = 1.f / norm_ref;
-    constexpr int PER = AX_TX * AX_TY / BLOCK;
+    constexpr int PER = AX_RB;
+    const int cx = threadIdx.x % AX_TX, oy = (threadIdx.x / AX_TX) * AX_RB;
     double m[4][PER];
     const float *planes[4] = {A.aR, A.bR, A.aB, A.bB};
     for (int qi = 0; qi < 4; qi++) {
@@ -196,26 +239,15 @@ __global__ __launch_bounds__(BLOCK) void axial_apply(AxialArgs A) {
                 const int y = y0 - r + ry, x = x0 - r - 1 + rx;
                 return (x >= 0 && x < W && y >= 0 && y < H) ? pl[(size_t)y * W + x] : 0.f;
             },
-            [&](int ry, int rx, float v) { q[ry * RX + rx] = v; });
+            [&](int ry, int rx, float v) { q[ry * RXP + rx] = v; });
         __syncthreads();
-        hsum(q, RX, RY, r, hs);
+        hsum(q, RXP, RY, r, hs);
         __syncthreads();
-        for (int k = 0; k < PER; k++) {
-            const int o = threadIdx.x + k * BLOCK;
-            const int oy = o / AX_TX, cx = o % AX_TX;
-            double s = 0.0;
-            for (int j = 0; j <= 2 * r; j++) s += hs[(oy + j) * AX_TX + cx];
-            const int x = x0 + cx, y = y0 + oy;
-            const int xl = max(0, x - r), yl = max(0, y - r);
-            const int xr = min(W - 1, x + r), yb = min(H - 1, y + r);
-            const int area = (xr - xl + 1) * (yb - yl + 1);
-            m[qi][k] = area > 0 ? s / (double)area : 0.0;
-        }
+        vmean(hs, r, cx, oy, x0, y0, W, H, m[qi]);
         __syncthreads();
     }
     for (int k = 0; k < PER; k++) {
-        const int o = threadIdx.x + k * BLOCK;
-        const int x = x0 + o % AX_TX, y = y0 + o / AX_TX;
+        const int x = x0 + cx, y = y0 + oy + k;
         if (x >= W || y >= H) continue;
         const size_t i = (size_t)y * W + x;
         const float s0 = A.src[3 * i], s1 = A.src[3 * i + 1], s2 = A.src[3 * i + 2];
@@ -499,7 +531,7 @@ int lre_launch_axial_ca(const float *d_in, float *d_out, int w, int h, int radiu
     // raise the dynamic LDS limit once per process (thread-safe static initialisation)
     static const hipError_t attr_rc = [] {
         const int RX = AX_TX + 2 * AX_RMAX + 2, RY = AX_TY + 2 * AX_RMAX + 2;
-        const int lds_max = (int)(sizeof(float) * RX * RY * 4 + sizeof(double) * RY * AX_TX);
+        const int lds_max = (int)(sizeof(float) * (RX | 1) * RY * 4 + sizeof(double) * RY * AX_HP);
         hipError_t e = hipFuncSetAttribute((const void *)axial_ab, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
         if (e == hipSuccess)
             e = hipFuncSetAttribute((const void *)axial_apply, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
@@ -508,12 +540,12 @@ int lre_launch_axial_ca(const float *d_in, float *d_out, int w, int h, int radiu
     LRE_HIP_CHECK(attr_rc);
     {
         const int RX = AX_TX + 2 * radius + 2, RY = AX_TY + 2 * radius + 2;
-        const size_t lds = sizeof(float) * RX * RY * 4 + sizeof(double) * (RY - 2) * AX_TX;
+        const size_t lds = sizeof(float) * (RX | 1) * RY * 4 + sizeof(double) * (RY - 2) * AX_HP;
         hipLaunchKernelGGL(axial_ab, grid, dim3(BLOCK), lds, s, A);
     }
     {
         const int RX = AX_TX + 2 * radius + 2, RY = AX_TY + 2 * radius;
-        const size_t lds = align256(sizeof(float) * RX * RY) + sizeof(double) * RY * AX_TX;
+        const size_t lds = sizeof(float) * (RX | 1) * RY + sizeof(double) * RY * AX_HP;
         hipLaunchKernelGGL(axial_apply, grid, dim3(BLOCK), lds, s, A);
     }
     LRE_HIP_CHECK(hipGetLastError());
