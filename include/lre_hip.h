@@ -167,6 +167,36 @@ int lre_hip_dev_rgb3_to_cfa(lre_ctx *ctx, const float *d_raw_rgb, float *d_cfa, 
                             int height, uint32_t filters, const signed char xtrans[36],
                             void *stream);
 
+/* ---- output epilogue ---------------------------------------------------------- */
+
+/* Accelerator::convert_color_space (core/accelerator.h:310-313,
+ * core/cpu_accelerator.cpp:2874-2897): out = T[:, :3] * rgb + T[:, 3], with T the
+ * row-major 3x4 transform (12 floats). */
+int lre_hip_convert_color_space(lre_ctx *ctx, const float *in_rgb, float *out_rgb, int width,
+                                int height, const float transform[12]);
+/* Accelerator::gamma_correct (core/accelerator.h:315-319,
+ * core/cpu_accelerator.cpp:2903-2976): per channel, by output colour space
+ * (0 raw, 1 sRGB, 2 Adobe, 3 wide gamut, 4 ProPhoto, 5 XYZ, 6 ACEScg, 7 P3-D65,
+ * 8 Rec.2020; core/metal/constants.h:28-38). */
+int lre_hip_gamma_correct(lre_ctx *ctx, const float *in_rgb, float *out_rgb, int width,
+                          int height, float gamma_power, float gamma_slope,
+                          int output_color_space);
+/* The 8/16-bit output of the processed image (core/python_bindings.cpp:235-257):
+ * clamp(v, 0, 1) * 255 (65535) + 0.5, truncated.  out holds count uint8 or uint16. */
+int lre_hip_quantize(lre_ctx *ctx, const float *in, void *out, size_t count, int bits);
+int lre_hip_dev_convert_color_space(lre_ctx *ctx, const float *d_in, float *d_out, int width,
+                                    int height, const float transform[12], void *stream);
+int lre_hip_dev_gamma_correct(lre_ctx *ctx, const float *d_in, float *d_out, int width,
+                              int height, float gamma_power, float gamma_slope,
+                              int output_color_space, void *stream);
+/* All three in one pass over resident RGB (the order of the develop pipeline,
+ * core/libraw_wrapper.cpp:1829-1842, then the output conversion): transform may be
+ * NULL, output_color_space < 0 skips the gamma, bits is 8, 16 or 32 (float out).
+ * d_out may equal d_rgb when bits is 32. */
+int lre_hip_dev_epilogue(lre_ctx *ctx, const float *d_rgb, void *d_out, int width, int height,
+                         const float transform[12], float gamma_power, float gamma_slope,
+                         int output_color_space, int bits, void *stream);
+
 /* ---- many images: batched develop and the request loop --------------------
  * The reference develops one image per postprocess() call
  * (core/libraw_wrapper.cpp:1477-1656) and has no batch or serving layer; these
@@ -181,7 +211,9 @@ enum {
     LRE_STAGE_AXIAL_CA = 3,
     LRE_STAGE_DEFRINGE = 4,
     LRE_STAGE_DETAIL_TM = 5,
-    LRE_STAGE_TONE_MAPPING = 6
+    LRE_STAGE_TONE_MAPPING = 6,
+    LRE_STAGE_COLOR_SPACE = 7,
+    LRE_STAGE_GAMMA = 8
 };
 
 /* Parameters of the post-demosaic stages. */
@@ -195,9 +227,12 @@ typedef struct lre_chain_params {
     float lateral_max_shift, lateral_min_confidence;
     int lateral_levels;
     float tone_after_scale;
+    float color_transform[12];   /* row-major 3x4 (identity by default) */
+    float gamma_power, gamma_slope;
+    int output_color_space;
 } lre_chain_params;
 
-/* The reference's defaults (core/accelerator.h:344-367, :322-323). */
+/* The reference's defaults (core/accelerator.h:344-367, :322-323, :315-319). */
 void lre_chain_params_default(lre_chain_params *p);
 
 /* One image.  Input is either a float CFA plane (d_cfa, with maximum_value as

@@ -85,6 +85,16 @@ _SIGS = {
     "lre_hip_dev_detail_tonemap": ([_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp], ctypes.c_int),
     "lre_hip_dev_rgb3_to_cfa": ([_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, _i8p,
                                  _vp], ctypes.c_int),
+    "lre_hip_convert_color_space": ([_vp, _f32p, _f32p, ctypes.c_int, ctypes.c_int, _f32p], ctypes.c_int),
+    "lre_hip_gamma_correct": ([_vp, _f32p, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                               ctypes.c_float, ctypes.c_int], ctypes.c_int),
+    "lre_hip_quantize": ([_vp, _f32p, _vp, ctypes.c_size_t, ctypes.c_int], ctypes.c_int),
+    "lre_hip_dev_convert_color_space": ([_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _f32p, _vp],
+                                        ctypes.c_int),
+    "lre_hip_dev_gamma_correct": ([_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                   ctypes.c_float, ctypes.c_int, _vp], ctypes.c_int),
+    "lre_hip_dev_epilogue": ([_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _f32p, ctypes.c_float,
+                              ctypes.c_float, ctypes.c_int, ctypes.c_int, _vp], ctypes.c_int),
 }
 
 _lock = threading.Lock()
@@ -212,6 +222,31 @@ class HIPAccelerator:
         _check(native().lre_hip_defringe(self._h, _p(src), _p(out), w, h, radius, edge_threshold,
                                          chroma_threshold, strength, int(defringe_green),
                                          green_strength_scale))
+        return out
+
+    def convert_color_space(self, rgb_input, transform):
+        """transform: 3x4 (row-major) camera-to-output matrix (core/accelerator.h:310-313)."""
+        src = _rgb(rgb_input)
+        h, w, _ = src.shape
+        t = np.ascontiguousarray(np.asarray(transform, np.float32).reshape(12))
+        out = np.empty_like(src)
+        _check(native().lre_hip_convert_color_space(self._h, _p(src), _p(out), w, h, _p(t)))
+        return out
+
+    def gamma_correct(self, rgb_input, gamma_power: float = 0.0, gamma_slope: float = 0.0,
+                      output_color_space: int = 1):
+        src = _rgb(rgb_input)
+        h, w, _ = src.shape
+        out = np.empty_like(src)
+        _check(native().lre_hip_gamma_correct(self._h, _p(src), _p(out), w, h, gamma_power,
+                                              gamma_slope, output_color_space))
+        return out
+
+    def quantize(self, rgb, bits: int = 8):
+        """The 8/16-bit output array of the processed image (core/python_bindings.cpp:235-257)."""
+        src = np.ascontiguousarray(rgb, np.float32)
+        out = np.empty(src.shape, np.uint8 if bits == 8 else np.uint16)
+        _check(native().lre_hip_quantize(self._h, _p(src), out.ctypes.data_as(_vp), src.size, bits))
         return out
 
     def ca_axial_cleanup(self, rgb_input, radius: int = 6, epsilon: float = 1e-4,

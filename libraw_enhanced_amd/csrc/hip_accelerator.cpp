@@ -96,6 +96,22 @@ bool HIPAccelerator::ca_register_lateral(const ImageBufferFloat &in, ImageBuffer
                               max_iterations, max_shift, min_confidence, pyramid_levels) == LRE_OK;
 }
 
+bool HIPAccelerator::convert_color_space(const ImageBufferFloat &in, ImageBufferFloat &out,
+                                         const float transform[3][4]) {
+    if (!ctx_ || !same_size(in, out) || !transform) return false;
+    float t[12];
+    for (int i = 0; i < 12; i++) t[i] = transform[i / 4][i % 4];
+    return lre_hip_convert_color_space(ctx_, fp(in), fp(out), (int)in.width, (int)in.height, t) ==
+           LRE_OK;
+}
+
+bool HIPAccelerator::gamma_correct(const ImageBufferFloat &in, ImageBufferFloat &out,
+                                   float gamma_power, float gamma_slope, int output_color_space) {
+    if (!ctx_ || !same_size(in, out)) return false;
+    return lre_hip_gamma_correct(ctx_, fp(in), fp(out), (int)in.width, (int)in.height, gamma_power,
+                                 gamma_slope, output_color_space) == LRE_OK;
+}
+
 bool HIPAccelerator::ca_axial_cleanup(const ImageBufferFloat &in, ImageBufferFloat &out,
                                       int radius, float epsilon, float strength) {
     if (!ctx_ || !same_size(in, out)) return false;

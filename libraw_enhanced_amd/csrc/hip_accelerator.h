@@ -49,6 +49,11 @@ public:
     bool ca_register_lateral(const ImageBufferFloat &rgb_input, ImageBufferFloat &rgb_output,
                              int cell_size = 96, int max_iterations = 3, float max_shift = 6.0f,
                              float min_confidence = 0.02f, int pyramid_levels = 3);
+    bool convert_color_space(const ImageBufferFloat &rgb_input, ImageBufferFloat &rgb_output,
+                             const float transform[3][4]);
+    bool gamma_correct(const ImageBufferFloat &rgb_input, ImageBufferFloat &rgb_output,
+                       float gamma_power = 0.f, float gamma_slope = 0.f,
+                       int output_color_space = 1);
     bool ca_axial_cleanup(const ImageBufferFloat &rgb_input, ImageBufferFloat &rgb_output,
                           int radius = 6, float epsilon = 1e-4f, float strength = 0.3f);
 

@@ -415,6 +415,72 @@ int lre_hip_lateral_ca(lre_ctx *ctx, const float *in_rgb, float *out_rgb, int w,
                           });
 }
 
+int lre_hip_convert_color_space(lre_ctx *ctx, const float *in_rgb, float *out_rgb, int w, int h,
+                                const float transform[12]) {
+    if (!transform) return set_arg_error("null transform");
+    return rgb_stage_host(ctx, in_rgb, out_rgb, w, h, 0, [&](const float *di, float *dout, void *) {
+        return lre_launch_epilogue(di, dout, (size_t)w * h, transform, 0.f, 0.f, -1, 32, ctx->stream);
+    });
+}
+
+int lre_hip_gamma_correct(lre_ctx *ctx, const float *in_rgb, float *out_rgb, int w, int h,
+                          float gamma_power, float gamma_slope, int output_color_space) {
+    if (output_color_space < 0) return set_arg_error("output colour space");
+    return rgb_stage_host(ctx, in_rgb, out_rgb, w, h, 0, [&](const float *di, float *dout, void *) {
+        return lre_launch_epilogue(di, dout, (size_t)w * h, nullptr, gamma_power, gamma_slope,
+                                   output_color_space, 32, ctx->stream);
+    });
+}
+
+int lre_hip_quantize(lre_ctx *ctx, const float *in, void *out, size_t count, int bits) {
+    if (!ctx || !in || !out) return set_arg_error("null pointer");
+    if (bits != 8 && bits != 16) return set_arg_error("bits must be 8 or 16");
+    if (count % 3) return set_arg_error("count must be a multiple of 3 (RGB samples)");
+    LRE_HIP_CHECK(hipSetDevice(ctx->device));
+    const size_t ob = count * (size_t)(bits / 8);
+    int rc = lre_ensure_scratch(ctx, aligned(count * 4) + aligned(ob));
+    if (rc) return rc;
+    Carve c(ctx->scratch);
+    float *d_in = c.take<float>(count);
+    void *d_out = c.take<char>(ob);
+    if ((rc = copy_in(ctx, d_in, in, count * 4))) return rc;
+    if ((rc = lre_launch_epilogue(d_in, d_out, count / 3, nullptr, 0.f, 0.f, -1, bits, ctx->stream)))
+        return rc;
+    return copy_out(ctx, out, d_out, ob);
+}
+
+int lre_hip_dev_convert_color_space(lre_ctx *ctx, const float *d_in, float *d_out, int w, int h,
+                                    const float transform[12], void *stream) {
+    if (!ctx || !d_in || !d_out || !transform) return set_arg_error("null pointer");
+    if (!valid_dims(w, h)) return set_arg_error("image size");
+    LRE_HIP_CHECK(hipSetDevice(ctx->device));
+    return lre_launch_epilogue(d_in, d_out, (size_t)w * h, transform, 0.f, 0.f, -1, 32,
+                               lre_pick_stream(ctx, stream));
+}
+
+int lre_hip_dev_gamma_correct(lre_ctx *ctx, const float *d_in, float *d_out, int w, int h,
+                              float gamma_power, float gamma_slope, int output_color_space,
+                              void *stream) {
+    if (!ctx || !d_in || !d_out) return set_arg_error("null pointer");
+    if (!valid_dims(w, h)) return set_arg_error("image size");
+    if (output_color_space < 0) return set_arg_error("output colour space");
+    LRE_HIP_CHECK(hipSetDevice(ctx->device));
+    return lre_launch_epilogue(d_in, d_out, (size_t)w * h, nullptr, gamma_power, gamma_slope,
+                               output_color_space, 32, lre_pick_stream(ctx, stream));
+}
+
+int lre_hip_dev_epilogue(lre_ctx *ctx, const float *d_rgb, void *d_out, int w, int h,
+                         const float transform[12], float gamma_power, float gamma_slope,
+                         int output_color_space, int bits, void *stream) {
+    if (!ctx || !d_rgb || !d_out) return set_arg_error("null pointer");
+    if (!valid_dims(w, h)) return set_arg_error("image size");
+    if (bits != 32 && d_out == static_cast<const void *>(d_rgb))
+        return set_arg_error("a quantized output cannot overwrite its input");
+    LRE_HIP_CHECK(hipSetDevice(ctx->device));
+    return lre_launch_epilogue(d_rgb, d_out, (size_t)w * h, transform, gamma_power, gamma_slope,
+                               output_color_space, bits, lre_pick_stream(ctx, stream));
+}
+
 int lre_hip_black_level(lre_ctx *ctx, uint16_t *img4, int w, int h, const int cblack[4],
                         const int *pos_table, int pos_rows, int pos_cols, int black, int maximum) {
     if (!ctx || !img4 || !cblack) return set_arg_error("null pointer");

@@ -69,6 +69,11 @@ int lre_xtrans_multi_fill(const lre_demosaic_job *jobs, int n, void *h_table);
 // h_table: the filled host table (read for the per-image border launches)
 int lre_launch_xtrans_multi(const lre_demosaic_job *jobs, int n, const void *h_table, void *d_table,
                             int total_tiles, int slots, void *d_scratch, hipStream_t s);
+// Output epilogue: optional 3x4 matrix (transform12 NULL skips it), optional gamma
+// (output_color_space < 0 skips it), then float (bits 32) or quantized (8, 16) store.
+int lre_launch_epilogue(const float *d_in, void *d_out, size_t npx, const float *transform12,
+                        float gamma_power, float gamma_slope, int output_color_space, int bits,
+                        hipStream_t s);
 // adjust_maximum on the device: d_maxv = rule(data_max, maximum, threshold)
 int lre_launch_adjust_maximum(const float *d_data_max, float maximum, float threshold,
                               float *d_maxv, hipStream_t s);

@@ -21,7 +21,7 @@ constexpr int MAX_LANES = 8;
 
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-bool valid_stage(int s) { return s >= LRE_STAGE_DEMOSAIC && s <= LRE_STAGE_TONE_MAPPING; }
+bool valid_stage(int s) { return s >= LRE_STAGE_DEMOSAIC && s <= LRE_STAGE_GAMMA; }
 
 int check_stages(const int *stages, int ns) {
     if (ns < 0 || (ns > 0 && !stages)) {
@@ -102,6 +102,11 @@ int run_post_stage(int stage, const lre_image &im, const lre_chain_params &p, vo
         return lre_launch_detail_tonemap(rgb, rgb, w, h, scratch, s);
     case LRE_STAGE_TONE_MAPPING:
         return lre_launch_tone_mapping(rgb, rgb, (size_t)w * h * 3, p.tone_after_scale, s);
+    case LRE_STAGE_COLOR_SPACE:
+        return lre_launch_epilogue(rgb, rgb, (size_t)w * h, p.color_transform, 0.f, 0.f, -1, 32, s);
+    case LRE_STAGE_GAMMA:
+        return lre_launch_epilogue(rgb, rgb, (size_t)w * h, nullptr, p.gamma_power, p.gamma_slope,
+                                   p.output_color_space, 32, s);
     default:
         return LRE_OK;
     }
@@ -451,6 +456,10 @@ void lre_chain_params_default(lre_chain_params *p) {
     p->lateral_min_confidence = 0.02f;
     p->lateral_levels = 3;
     p->tone_after_scale = 1.0f;
+    for (int k = 0; k < 12; k++) p->color_transform[k] = (k % 5 == 0) ? 1.0f : 0.0f;
+    p->gamma_power = 0.0f;  // ProcessingParams defaults (core/accelerator.h:106, :123-124)
+    p->gamma_slope = 0.0f;
+    p->output_color_space = 1;
 }
 
 int lre_hip_batch_create(lre_batch **out, int device, int lanes) {

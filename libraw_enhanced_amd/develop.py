@@ -28,10 +28,13 @@ STAGE_AXIAL_CA = 3
 STAGE_DEFRINGE = 4
 STAGE_DETAIL_TM = 5
 STAGE_TONE_MAPPING = 6
+STAGE_COLOR_SPACE = 7
+STAGE_GAMMA = 8
 
 STAGE_NAMES = {
     "demosaic": STAGE_DEMOSAIC, "lateral_ca": STAGE_LATERAL_CA, "axial_ca": STAGE_AXIAL_CA,
     "defringe": STAGE_DEFRINGE, "detail_tm": STAGE_DETAIL_TM, "tone_mapping": STAGE_TONE_MAPPING,
+    "color_space": STAGE_COLOR_SPACE, "gamma": STAGE_GAMMA,
 }
 
 # demosaic -> axial CA -> defringe -> detail tone map
@@ -47,6 +50,8 @@ class ChainParams(ctypes.Structure):
         ("lateral_cell", _i), ("lateral_iterations", _i), ("lateral_max_shift", _f),
         ("lateral_min_confidence", _f), ("lateral_levels", _i),
         ("tone_after_scale", _f),
+        ("color_transform", _f * 12), ("gamma_power", _f), ("gamma_slope", _f),
+        ("output_color_space", _i),
     ]
 
     @classmethod

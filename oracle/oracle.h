@@ -9,6 +9,7 @@
  */
 #ifndef LRE_ORACLE_H
 #define LRE_ORACLE_H
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -53,6 +54,13 @@ int oracle_ingest_u16(const uint16_t *raw, float *cfa, int w, int h, uint32_t fi
 float oracle_adjust_maximum(float data_max, float maximum, float threshold);
 int oracle_white_balance(const uint16_t *img4, float *rgb3, int w, int h, const float *wb4,
                          uint32_t filters, const signed char *xtrans36);
+
+/* output epilogue (epilogue.c) */
+int oracle_convert_color_space(const float *in3, float *out3, int w, int h, const float *transform12);
+float oracle_gamma_value(float v, float gamma_power, float gamma_slope, int output_color_space);
+int oracle_gamma_correct(const float *in3, float *out3, int w, int h, float gamma_power,
+                         float gamma_slope, int output_color_space);
+int oracle_quantize(const float *in, void *out, size_t n, int bits);
 
 #ifdef __cplusplus
 }

@@ -117,3 +117,22 @@ int ref_axial_ca(const float *in3, float *out3, int w, int h, int radius, float 
 }
 
 }  // extern "C"
+
+extern "C" {
+
+// core/cpu_accelerator.cpp:2874 convert_color_space
+int ref_convert_color_space(const float *in3, float *out3, int w, int h, const float *transform12) {
+    float t[3][4];
+    for (int i = 0; i < 12; i++) t[i / 4][i % 4] = transform12[i];
+    ImageBufferFloat in = view(in3, w, h), out = view(out3, w, h);
+    return cpu().convert_color_space(in, out, t) ? 0 : -1;
+}
+
+// core/cpu_accelerator.cpp:2903 gamma_correct
+int ref_gamma_correct(const float *in3, float *out3, int w, int h, float gamma_power,
+                      float gamma_slope, int output_color_space) {
+    ImageBufferFloat in = view(in3, w, h), out = view(out3, w, h);
+    return cpu().gamma_correct(in, out, gamma_power, gamma_slope, output_color_space) ? 0 : -1;
+}
+
+}  // extern "C"

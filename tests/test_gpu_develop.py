@@ -215,3 +215,48 @@ def test_ingest_u16_bit_exact(acc, filters, xtrans):
     acc.synchronize()
     assert np.array_equal(d_cfa.download((h, w)), want)
     assert float(d_max.download((1,))[0]) == want_max
+
+
+# ---------------- output epilogue ----------------
+from test_epilogue_cpu import MATRICES, _img, oracle_color, oracle_gamma, oracle_quantize  # noqa: E402
+
+
+def test_color_space_bit_exact(acc):
+    img = _img(120, 170)
+    for t in MATRICES:
+        assert np.array_equal(acc.convert_color_space(img, t), oracle_color(img, t))
+
+
+@pytest.mark.parametrize("cs,gp,gs", [(0, 2.222, 4.5), (1, 0, 0), (2, 0, 0), (4, 0, 0), (7, 0, 0), (8, 0, 0)])
+def test_gamma_tolerance(acc, cs, gp, gs):
+    img = _img(120, 170, seed=cs)
+    got = acc.gamma_correct(img, gp, gs, cs)
+    want = oracle_gamma(img, gp, gs, cs)
+    # powf is evaluated in double on the GPU (correctly rounded); the C library's
+    # powf may differ by one unit in the last place
+    ulp = np.abs(got.view(np.int32).astype(np.int64) - want.view(np.int32).astype(np.int64))
+    print(f"gamma cs={cs}: {int(np.count_nonzero(ulp))} of {ulp.size} values differ, max {int(ulp.max())} ulp")
+    assert int(ulp.max()) <= 1
+    assert float(np.abs(got - want).max()) <= 2e-4
+
+
+def test_quantize_and_fused_epilogue(acc):
+    import ctypes as ct
+    from libraw_enhanced_amd.accelerator import native
+    img = _img(97, 131)
+    h, w, _ = img.shape
+    t = np.ascontiguousarray(MATRICES[1]).reshape(12)
+    for bits in (8, 16):
+        assert np.array_equal(acc.quantize(img, bits), oracle_quantize(img, bits))
+    # fused (matrix -> gamma -> quantize) equals the separate GPU stages
+    staged = acc.gamma_correct(acc.convert_color_space(img, MATRICES[1]), 0, 0, 1)
+    d_in = acc.dev_from_host(img)
+    for bits, dt in ((32, np.float32), (8, np.uint8), (16, np.uint16)):
+        d_out = acc.dev_alloc(h * w * 3 * np.dtype(dt).itemsize)
+        rc = native().lre_hip_dev_epilogue(acc.handle, d_in.ptr, d_out.ptr, w, h,
+                                          t.ctypes.data_as(ct.POINTER(ct.c_float)), 0.0, 0.0, 1, bits, None)
+        assert rc == 0
+        acc.synchronize()
+        got = d_out.download((h, w, 3), dt)
+        want = staged if bits == 32 else acc.quantize(staged, bits)
+        assert np.array_equal(got, want), bits
