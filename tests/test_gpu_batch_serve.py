@@ -122,6 +122,13 @@ def _single(acc, it, stages, p):
             rc = lib.lre_hip_dev_detail_tonemap(ctx, d_rgb.ptr, d_rgb.ptr, w, h, None)
         elif s == "tone_mapping":
             rc = lib.lre_hip_dev_tone_mapping(ctx, d_rgb.ptr, d_rgb.ptr, w, h, p.tone_after_scale, None)
+        elif s == "color_space":
+            t = np.ctypeslib.as_array(p.color_transform).astype(np.float32)
+            rc = lib.lre_hip_dev_convert_color_space(ctx, d_rgb.ptr, d_rgb.ptr, w, h,
+                                                     t.ctypes.data_as(ct.POINTER(ct.c_float)), None)
+        elif s == "gamma":
+            rc = lib.lre_hip_dev_gamma_correct(ctx, d_rgb.ptr, d_rgb.ptr, w, h, p.gamma_power,
+                                               p.gamma_slope, p.output_color_space, None)
         assert rc == 0, s
     acc.synchronize()
     return d_rgb.download((h, w, 3))
@@ -149,10 +156,12 @@ def test_batch_matches_per_image(acc, lanes):
 
 def test_batch_other_stage_lists(acc):
     from libraw_enhanced_amd import develop as dv
-    p = dv.ChainParams.defaults(defringe_green=1, axial_radius=3)
+    p = dv.ChainParams.defaults(defringe_green=1, axial_radius=3, output_color_space=2)
+    p.color_transform[:] = [1.62, -0.49, -0.13, 0.0, -0.18, 1.35, -0.17, 0.0, 0.02, -0.31, 1.29, 0.0]
     spec = _spec()[:3]
     for stages in [("demosaic",), ("demosaic", "lateral_ca", "tone_mapping"),
-                   ("demosaic", "defringe", "axial_ca", "detail_tm")]:
+                   ("demosaic", "defringe", "axial_ca", "detail_tm"),
+                   ("demosaic", "detail_tm", "color_space", "gamma")]:
         print(f"stages {stages}", flush=True)
         S = _Set(acc, spec)
         b = dv.BatchDeveloper(0, 2)

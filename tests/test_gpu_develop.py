@@ -233,11 +233,14 @@ def test_gamma_tolerance(acc, cs, gp, gs):
     got = acc.gamma_correct(img, gp, gs, cs)
     want = oracle_gamma(img, gp, gs, cs)
     # powf is evaluated in double on the GPU (correctly rounded); the C library's
-    # powf may differ by one unit in the last place
+    # powf may differ by one unit in the last place, which the sRGB / Rec.2020
+    # offsets (a * x - (a - 1)) can widen to a few units near zero
     ulp = np.abs(got.view(np.int32).astype(np.int64) - want.view(np.int32).astype(np.int64))
-    print(f"gamma cs={cs}: {int(np.count_nonzero(ulp))} of {ulp.size} values differ, max {int(ulp.max())} ulp")
-    assert int(ulp.max()) <= 1
-    assert float(np.abs(got - want).max()) <= 2e-4
+    err = float(np.abs(got - want).max())
+    print(f"gamma cs={cs}: {int(np.count_nonzero(ulp))} of {ulp.size} values differ, "
+          f"max {int(ulp.max())} ulp, max-abs {err:.3g}")
+    assert int(ulp.max()) <= 4 and np.count_nonzero(ulp) <= ulp.size // 100
+    assert err <= 1e-6
 
 
 def test_quantize_and_fused_epilogue(acc):
