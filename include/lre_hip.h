@@ -161,6 +161,21 @@ int lre_hip_dev_ingest_u16(lre_ctx *ctx, const uint16_t *d_raw, float *d_cfa, in
                            int height, uint32_t filters, const signed char xtrans[36],
                            const int cblack[4], int black, int maximum, const float wb[4],
                            float *d_data_max, void *stream);
+/* apply_green_matching (core/libraw_wrapper.cpp:496-585) on the uint16 x 4 image
+ * after black subtraction: flat, unsaturated channel-3 (G2) sites are scaled by
+ * the ratio of their channel-1 and channel-3 neighbourhood means.  No-op for
+ * X-Trans (filters == 9).  The host form works in place. */
+int lre_hip_green_matching(lre_ctx *ctx, uint16_t *img4, int width, int height, uint32_t filters,
+                           int maximum);
+/* Device form: reads d_src, writes d_dst (which must differ from d_src). */
+int lre_hip_dev_green_matching(lre_ctx *ctx, const uint16_t *d_src, uint16_t *d_dst, int width,
+                               int height, uint32_t filters, int maximum, void *stream);
+/* remove_zeroes (core/libraw_wrapper.cpp:150-212), in place: in raster order, a
+ * zero at its native channel becomes the integer mean of the non-zero samples of
+ * that channel in its 5x5 window (earlier replacements included). */
+int lre_hip_remove_zeroes(lre_ctx *ctx, uint16_t *img4, int width, int height, uint32_t filters);
+int lre_hip_dev_remove_zeroes(lre_ctx *ctx, uint16_t *d_img4, int width, int height,
+                              uint32_t filters, void *stream);
 /* Demosaic input planes: extract the native channel of the reference's 3-channel
  * layout into a CFA plane. */
 int lre_hip_dev_rgb3_to_cfa(lre_ctx *ctx, const float *d_raw_rgb, float *d_cfa, int width,

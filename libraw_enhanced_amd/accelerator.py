@@ -89,6 +89,13 @@ _SIGS = {
     "lre_hip_gamma_correct": ([_vp, _f32p, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_float,
                                ctypes.c_float, ctypes.c_int], ctypes.c_int),
     "lre_hip_quantize": ([_vp, _f32p, _vp, ctypes.c_size_t, ctypes.c_int], ctypes.c_int),
+    "lre_hip_green_matching": ([_vp, _u16p, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_int],
+                               ctypes.c_int),
+    "lre_hip_dev_green_matching": ([_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
+                                    ctypes.c_int, _vp], ctypes.c_int),
+    "lre_hip_remove_zeroes": ([_vp, _u16p, ctypes.c_int, ctypes.c_int, ctypes.c_uint32], ctypes.c_int),
+    "lre_hip_dev_remove_zeroes": ([_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, _vp],
+                                  ctypes.c_int),
     "lre_hip_dev_convert_color_space": ([_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _f32p, _vp],
                                         ctypes.c_int),
     "lre_hip_dev_gamma_correct": ([_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_float,
@@ -283,6 +290,26 @@ class HIPAccelerator:
             pr = pc = 0
             ptr = None
         _check(native().lre_hip_black_level(self._h, _p(img4, _u16p), w, h, cb, ptr, pr, pc, black, maximum))
+        return img4
+
+    def apply_green_matching(self, img4, filters: int, maximum: int):
+        """In place on a (H, W, 4) uint16 image (core/libraw_wrapper.cpp:496-585); returns it."""
+        img4 = self._img4(img4)
+        h, w, _ = img4.shape
+        _check(native().lre_hip_green_matching(self._h, _p(img4, _u16p), w, h, filters, maximum))
+        return img4
+
+    def remove_zeroes(self, img4, filters: int):
+        """In place on a (H, W, 4) uint16 image (core/libraw_wrapper.cpp:150-212); returns it."""
+        img4 = self._img4(img4)
+        h, w, _ = img4.shape
+        _check(native().lre_hip_remove_zeroes(self._h, _p(img4, _u16p), w, h, filters))
+        return img4
+
+    @staticmethod
+    def _img4(img4):
+        if img4.dtype != np.uint16 or img4.ndim != 3 or img4.shape[2] != 4 or not img4.flags.c_contiguous:
+            raise LibRawInvalidArgument("image must be a C-contiguous (H, W, 4) uint16 array")
         return img4
 
     def apply_white_balance(self, img4, wb, filters: int = RGGB, xtrans=None, rgb=None):

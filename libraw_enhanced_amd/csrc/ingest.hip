@@ -5,7 +5,10 @@
 //    balance into a float CFA plane (core/cpu_accelerator.cpp:2813-2869), and
 //    the data maximum that adjust_maximum compares against the white level
 //    (core/libraw_wrapper.cpp:1037-1077), reduced on the device.
-// Integer arithmetic and one float multiply per sample: bit-exact.
+//  * green matching (core/libraw_wrapper.cpp:496-585) and zero removal
+//    (:150-212) on the uint16 x 4 image.
+// Integer arithmetic, one float multiply per sample, and the reference's FP64
+// green-matching ratios: bit-exact.
 #include "lre_common.h"
 #include "lre_internal.h"
 
@@ -109,7 +112,168 @@ __global__ void adjust_maximum_kernel(const float *d_data_max, float maximum, fl
     *d_maxv = m;
 }
 
+// ---- green matching (:496-585): reads src, writes the channel-3 samples it
+// corrects into dst (a copy of src) ----
+__global__ __launch_bounds__(BLOCK) void green_match_kernel(const ushort4 *__restrict__ src,
+                                                           ushort4 *__restrict__ dst, int W, int H,
+                                                           int maximum) {
+    const int nj = (H - 3 - 3 + 1) / 2, ni = (W - 3 - 2 + 1) / 2;  // j = 3,5,.. < H-3; i = 2,4,.. < W-3
+    if (nj <= 0 || ni <= 0) return;
+    const size_t n = (size_t)nj * ni;
+    const float thr = 0.01f;
+    for (size_t t = (size_t)blockIdx.x * BLOCK + threadIdx.x; t < n; t += (size_t)gridDim.x * BLOCK) {
+        const int j = 3 + 2 * (int)(t / ni), i = 2 + 2 * (int)(t % ni);
+        const int o1_1 = src[(size_t)(j - 1) * W + i - 1].y, o1_2 = src[(size_t)(j - 1) * W + i + 1].y;
+        const int o1_3 = src[(size_t)(j + 1) * W + i - 1].y, o1_4 = src[(size_t)(j + 1) * W + i + 1].y;
+        const int o2_1 = src[(size_t)(j - 2) * W + i].w, o2_2 = src[(size_t)(j + 2) * W + i].w;
+        const int o2_3 = src[(size_t)j * W + i - 2].w, o2_4 = src[(size_t)j * W + i + 2].w;
+        const double m1 = (o1_1 + o1_2 + o1_3 + o1_4) / 4.0;
+        const double m2 = (o2_1 + o2_2 + o2_3 + o2_4) / 4.0;
+        const double c1 = (abs(o1_1 - o1_2) + abs(o1_1 - o1_3) + abs(o1_1 - o1_4) + abs(o1_2 - o1_3) +
+                           abs(o1_3 - o1_4) + abs(o1_2 - o1_4)) / 6.0;
+        const double c2 = (abs(o2_1 - o2_2) + abs(o2_1 - o2_3) + abs(o2_1 - o2_4) + abs(o2_2 - o2_3) +
+                           abs(o2_3 - o2_4) + abs(o2_2 - o2_4)) / 6.0;
+        const int g2 = src[(size_t)j * W + i].w;
+        if ((g2 < maximum * 0.95) && (c1 < maximum * thr) && (c2 < maximum * thr) && (m2 > 0.1)) {
+            const float correction = (float)(g2 * m1 / m2);
+            dst[(size_t)j * W + i].w =
+                correction > maximum ? (unsigned short)maximum : (unsigned short)correction;
+        }
+    }
+}
+
+// ---- zero removal (:150-212).  The reference replaces zeros in raster order,
+// so a zero can see the new value of an earlier zero of its channel in its 5x5
+// window.  Zeros with no such earlier zero ("independent") are replaced in
+// parallel: their windows hold no other independent zero.  The rest are
+// replaced afterwards by one wavefront in raster order. ----
+LRE_DEV unsigned chan(uint32_t filters, int r, int c) { return lre::bayer_native(filters, (unsigned)r, (unsigned)c); }
+LRE_DEV unsigned short sample(const unsigned short *img4, int W, int r, int c, unsigned ch) {
+    return img4[((size_t)r * W + c) * 4 + ch];
+}
+
+__global__ __launch_bounds__(BLOCK) void zero_classify_kernel(const unsigned short *__restrict__ img4,
+                                                             int W, int H, uint32_t filters,
+                                                             unsigned char *__restrict__ dep,
+                                                             unsigned *__restrict__ rowflag) {
+    const size_t n = (size_t)W * H;
+    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK) {
+        const int row = (int)(i / W), col = (int)(i % W);
+        const unsigned ch = chan(filters, row, col);
+        unsigned char d = 0;
+        if (sample(img4, W, row, col, ch) == 0) {
+            d = 1;  // independent zero
+            for (int r = row - 2; r <= row && d == 1; r++)
+                for (int c = col - 2; c <= col + 2; c++) {
+                    if (r == row && c >= col) break;
+                    if (r < 0 || c < 0 || c >= W) continue;
+                    if (chan(filters, r, c) == ch && sample(img4, W, r, c, ch) == 0) { d = 2; break; }
+                }
+            if (d == 2) rowflag[row] = 1u;
+        }
+        dep[i] = d;
+    }
+}
+
+LRE_DEV unsigned short window_mean(const unsigned short *img4, int W, int H, uint32_t filters,
+                                   int row, int col, unsigned ch, bool &any) {
+    unsigned sum = 0, cnt = 0;
+    for (int r = row - 2; r <= row + 2; r++)
+        for (int c = col - 2; c <= col + 2; c++) {
+            if (r < 0 || r >= H || c < 0 || c >= W) continue;
+            const unsigned nch = chan(filters, r, c);
+            const unsigned short v = sample(img4, W, r, c, nch);
+            if (nch == ch && v != 0) { sum += v; cnt++; }
+        }
+    any = cnt > 0;
+    return any ? (unsigned short)(sum / cnt) : 0;
+}
+
+__global__ __launch_bounds__(BLOCK) void zero_fill_independent(unsigned short *img4, int W, int H,
+                                                              uint32_t filters,
+                                                              const unsigned char *__restrict__ dep) {
+    const size_t n = (size_t)W * H;
+    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK) {
+        if (dep[i] != 1) continue;
+        const int row = (int)(i / W), col = (int)(i % W);
+        const unsigned ch = chan(filters, row, col);
+        bool any;
+        const unsigned short v = window_mean(img4, W, H, filters, row, col, ch, any);
+        if (any) img4[i * 4 + ch] = v;
+    }
+}
+
+// one wavefront: rows in order; within a flagged row, 64 columns at a time,
+// the flagged zeros of the chunk one after another (lowest column first)
+__global__ __launch_bounds__(64) void zero_fill_dependent(unsigned short *img4, int W, int H,
+                                                         uint32_t filters,
+                                                         const unsigned char *__restrict__ dep,
+                                                         const unsigned *__restrict__ rowflag) {
+    const int lane = threadIdx.x;
+    for (int row = 0; row < H; row++) {
+        if (!rowflag[row]) continue;
+        for (int c0 = 0; c0 < W; c0 += 64) {
+            const int c = c0 + lane;
+            const bool mine = c < W && dep[(size_t)row * W + c] == 2;
+            unsigned long long mask = __ballot(mine);
+            while (mask) {
+                const int k = __ffsll((long long)mask) - 1;
+                mask &= mask - 1;
+                const int col = c0 + k;
+                if (lane == 0) {
+                    const unsigned ch = chan(filters, row, col);
+                    bool any;
+                    const unsigned short v = window_mean(img4, W, H, filters, row, col, ch, any);
+                    if (any) img4[((size_t)row * W + col) * 4 + ch] = v;
+                }
+                // the next zero of the chunk reads this one's value
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+    }
+}
+
+__global__ void zero_bytes(unsigned *p, size_t n4) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
 }  // namespace
+
+int lre_launch_green_matching(const uint16_t *d_src, uint16_t *d_dst, int w, int h, uint32_t filters,
+                              int maximum, hipStream_t s) {
+    if (d_src != d_dst)
+        LRE_HIP_CHECK(hipMemcpyAsync(d_dst, d_src, (size_t)w * h * 8, hipMemcpyDeviceToDevice, s));
+    if (filters == 9) return LRE_OK;  // Bayer only (:499-503)
+    if (d_src == d_dst) {
+        lre_set_error("green matching reads the unmodified image: source and destination must differ");
+        return LRE_ERR_ARG;
+    }
+    hipLaunchKernelGGL(green_match_kernel, dim3(grid_for((size_t)w * h / 4)), dim3(BLOCK), 0, s,
+                       reinterpret_cast<const ushort4 *>(d_src), reinterpret_cast<ushort4 *>(d_dst), w,
+                       h, maximum);
+    LRE_HIP_CHECK(hipGetLastError());
+    return LRE_OK;
+}
+
+size_t lre_remove_zeroes_scratch_bytes(int w, int h) {
+    return (((size_t)w * h + 255) & ~(size_t)255) + (((size_t)h * 4 + 255) & ~(size_t)255);
+}
+
+int lre_launch_remove_zeroes(uint16_t *d_img4, int w, int h, uint32_t filters, void *d_scratch,
+                             hipStream_t s) {
+    unsigned char *dep = static_cast<unsigned char *>(d_scratch);
+    unsigned *rowflag = reinterpret_cast<unsigned *>(dep + (((size_t)w * h + 255) & ~(size_t)255));
+    const size_t nflag = ((size_t)h * 4 + 255) / 4;
+    hipLaunchKernelGGL(zero_bytes, dim3(grid_for(nflag)), dim3(BLOCK), 0, s, rowflag, nflag);
+    const int g = grid_for((size_t)w * h);
+    hipLaunchKernelGGL(zero_classify_kernel, dim3(g), dim3(BLOCK), 0, s, d_img4, w, h, filters, dep, rowflag);
+    hipLaunchKernelGGL(zero_fill_independent, dim3(g), dim3(BLOCK), 0, s, d_img4, w, h, filters, dep);
+    hipLaunchKernelGGL(zero_fill_dependent, dim3(1), dim3(64), 0, s, d_img4, w, h, filters, dep, rowflag);
+    LRE_HIP_CHECK(hipGetLastError());
+    return LRE_OK;
+}
 
 int lre_launch_adjust_maximum(const float *d_data_max, float maximum, float threshold,
                               float *d_maxv, hipStream_t s) {

@@ -481,6 +481,56 @@ int lre_hip_dev_epilogue(lre_ctx *ctx, const float *d_rgb, void *d_out, int w, i
                                output_color_space, bits, lre_pick_stream(ctx, stream));
 }
 
+int lre_hip_green_matching(lre_ctx *ctx, uint16_t *img4, int w, int h, uint32_t filters,
+                           int maximum) {
+    if (!ctx || !img4) return set_arg_error("null pointer");
+    if (!valid_dims(w, h)) return set_arg_error("image size");
+    if (filters == 9) return LRE_OK;
+    LRE_HIP_CHECK(hipSetDevice(ctx->device));
+    const size_t n = (size_t)w * h;
+    int rc = lre_ensure_scratch(ctx, aligned(n * 8) * 2);
+    if (rc) return rc;
+    Carve c(ctx->scratch);
+    uint16_t *d_src = c.take<uint16_t>(n * 4), *d_dst = c.take<uint16_t>(n * 4);
+    if ((rc = copy_in(ctx, d_src, img4, n * 8))) return rc;
+    if ((rc = lre_launch_green_matching(d_src, d_dst, w, h, filters, maximum, ctx->stream))) return rc;
+    return copy_out(ctx, img4, d_dst, n * 8);
+}
+
+int lre_hip_dev_green_matching(lre_ctx *ctx, const uint16_t *d_src, uint16_t *d_dst, int w, int h,
+                               uint32_t filters, int maximum, void *stream) {
+    if (!ctx || !d_src || !d_dst) return set_arg_error("null pointer");
+    if (!valid_dims(w, h)) return set_arg_error("image size");
+    if (d_src == d_dst && filters != 9) return set_arg_error("source and destination must differ");
+    LRE_HIP_CHECK(hipSetDevice(ctx->device));
+    return lre_launch_green_matching(d_src, d_dst, w, h, filters, maximum, lre_pick_stream(ctx, stream));
+}
+
+int lre_hip_remove_zeroes(lre_ctx *ctx, uint16_t *img4, int w, int h, uint32_t filters) {
+    if (!ctx || !img4) return set_arg_error("null pointer");
+    if (!valid_dims(w, h)) return set_arg_error("image size");
+    LRE_HIP_CHECK(hipSetDevice(ctx->device));
+    const size_t n = (size_t)w * h;
+    int rc = lre_ensure_scratch(ctx, aligned(n * 8) + lre_remove_zeroes_scratch_bytes(w, h));
+    if (rc) return rc;
+    Carve c(ctx->scratch);
+    uint16_t *d = c.take<uint16_t>(n * 4);
+    void *ws = static_cast<char *>(ctx->scratch) + c.off;
+    if ((rc = copy_in(ctx, d, img4, n * 8))) return rc;
+    if ((rc = lre_launch_remove_zeroes(d, w, h, filters, ws, ctx->stream))) return rc;
+    return copy_out(ctx, img4, d, n * 8);
+}
+
+int lre_hip_dev_remove_zeroes(lre_ctx *ctx, uint16_t *d_img4, int w, int h, uint32_t filters,
+                              void *stream) {
+    if (!ctx || !d_img4) return set_arg_error("null pointer");
+    if (!valid_dims(w, h)) return set_arg_error("image size");
+    LRE_HIP_CHECK(hipSetDevice(ctx->device));
+    int rc = lre_ensure_scratch(ctx, lre_remove_zeroes_scratch_bytes(w, h));
+    if (rc) return rc;
+    return lre_launch_remove_zeroes(d_img4, w, h, filters, ctx->scratch, lre_pick_stream(ctx, stream));
+}
+
 int lre_hip_black_level(lre_ctx *ctx, uint16_t *img4, int w, int h, const int cblack[4],
                         const int *pos_table, int pos_rows, int pos_cols, int black, int maximum) {
     if (!ctx || !img4 || !cblack) return set_arg_error("null pointer");

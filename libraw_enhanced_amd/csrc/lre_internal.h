@@ -74,6 +74,12 @@ int lre_launch_xtrans_multi(const lre_demosaic_job *jobs, int n, const void *h_t
 int lre_launch_epilogue(const float *d_in, void *d_out, size_t npx, const float *transform12,
                         float gamma_power, float gamma_slope, int output_color_space, int bits,
                         hipStream_t s);
+// Green matching (d_src != d_dst unless X-Trans) and zero removal on uint16 x 4 images.
+int lre_launch_green_matching(const uint16_t *d_src, uint16_t *d_dst, int w, int h, uint32_t filters,
+                              int maximum, hipStream_t s);
+size_t lre_remove_zeroes_scratch_bytes(int w, int h);
+int lre_launch_remove_zeroes(uint16_t *d_img4, int w, int h, uint32_t filters, void *d_scratch,
+                             hipStream_t s);
 // adjust_maximum on the device: d_maxv = rule(data_max, maximum, threshold)
 int lre_launch_adjust_maximum(const float *d_data_max, float maximum, float threshold,
                               float *d_maxv, hipStream_t s);

@@ -14,8 +14,11 @@
  *                       (core/libraw_wrapper.cpp:1010-1089), which reads each pixel's
  *                       channel through fcol_bayer even for X-Trans (:1051)
  *  oracle_adjust_maximum  the maximum rule of :1070-1077
+ *  oracle_green_matching  apply_green_matching, :496-585
+ *  oracle_remove_zeroes   remove_zeroes, :150-212
  */
 #include <stdint.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include "oracle.h"
@@ -83,4 +86,68 @@ float oracle_adjust_maximum(float data_max, float maximum, float threshold) {
     float thr = threshold > 0.99999f ? 0.75f : threshold;
     if (data_max > 0 && data_max < maximum && data_max > maximum * thr) return data_max;
     return maximum;
+}
+
+/* apply_green_matching, core/libraw_wrapper.cpp:496-585: for Bayer data, every
+ * site (j, i), j = 3, 5, .. < h-3, i = 2, 4, .. < w-3, whose channel-3 sample is
+ * below 0.95 maximum and whose four diagonal channel-1 and four axial channel-3
+ * neighbours are flat (mean absolute pair difference below 0.01 maximum) gets
+ * its channel-3 sample scaled by mean(ch1) / mean(ch3); neighbours are read
+ * from the unmodified image. */
+int oracle_green_matching(uint16_t *img4, int w, int h, uint32_t filters, int maximum) {
+    if (filters == 9) return 0;
+    const size_t n = (size_t)w * h * 4;
+    uint16_t *img = (uint16_t *)malloc(n * sizeof(uint16_t));
+    if (!img) return -1;
+    memcpy(img, img4, n * sizeof(uint16_t));
+    const float thr = 0.01f;
+#define IMG(r, c, ch) img[((size_t)(r) * w + (c)) * 4 + (ch)]
+    for (int j = 3; j < h - 3; j += 2)
+        for (int i = 2; i < w - 3; i += 2) {
+            const int o1_1 = IMG(j - 1, i - 1, 1), o1_2 = IMG(j - 1, i + 1, 1);
+            const int o1_3 = IMG(j + 1, i - 1, 1), o1_4 = IMG(j + 1, i + 1, 1);
+            const int o2_1 = IMG(j - 2, i, 3), o2_2 = IMG(j + 2, i, 3);
+            const int o2_3 = IMG(j, i - 2, 3), o2_4 = IMG(j, i + 2, 3);
+            const double m1 = (o1_1 + o1_2 + o1_3 + o1_4) / 4.0;
+            const double m2 = (o2_1 + o2_2 + o2_3 + o2_4) / 4.0;
+            const double c1 = (abs(o1_1 - o1_2) + abs(o1_1 - o1_3) + abs(o1_1 - o1_4) +
+                               abs(o1_2 - o1_3) + abs(o1_3 - o1_4) + abs(o1_2 - o1_4)) / 6.0;
+            const double c2 = (abs(o2_1 - o2_2) + abs(o2_1 - o2_3) + abs(o2_1 - o2_4) +
+                               abs(o2_2 - o2_3) + abs(o2_3 - o2_4) + abs(o2_2 - o2_4)) / 6.0;
+            if ((IMG(j, i, 3) < maximum * 0.95) && (c1 < maximum * thr) && (c2 < maximum * thr) &&
+                (m2 > 0.1)) {
+                const float correction = (float)(img4[((size_t)j * w + i) * 4 + 3] * m1 / m2);
+                img4[((size_t)j * w + i) * 4 + 3] =
+                    correction > maximum ? (uint16_t)maximum : (uint16_t)correction;
+            }
+        }
+#undef IMG
+    free(img);
+    return 0;
+}
+
+/* remove_zeroes, core/libraw_wrapper.cpp:150-212: in raster order, a zero
+ * sample at its native channel becomes the integer mean of the non-zero
+ * samples of the same native channel in its 5x5 window (clipped at the frame);
+ * samples replaced earlier count with their new value. */
+int oracle_remove_zeroes(uint16_t *img4, int w, int h, uint32_t filters) {
+    for (int row = 0; row < h; row++)
+        for (int col = 0; col < w; col++) {
+            const unsigned ch = bayer_native(filters, row, col);
+            uint16_t *cur = &img4[((size_t)row * w + col) * 4 + ch];
+            if (*cur != 0) continue;
+            unsigned sum = 0, cnt = 0;
+            for (int r = row - 2; r <= row + 2; r++)
+                for (int c = col - 2; c <= col + 2; c++) {
+                    if (r < 0 || r >= h || c < 0 || c >= w) continue;
+                    const unsigned nch = bayer_native(filters, r, c);
+                    const uint16_t v = img4[((size_t)r * w + c) * 4 + nch];
+                    if (nch == ch && v != 0) {
+                        sum += v;
+                        cnt++;
+                    }
+                }
+            if (cnt > 0) *cur = (uint16_t)(sum / cnt);
+        }
+    return 0;
 }

@@ -55,6 +55,9 @@ float oracle_adjust_maximum(float data_max, float maximum, float threshold);
 int oracle_white_balance(const uint16_t *img4, float *rgb3, int w, int h, const float *wb4,
                          uint32_t filters, const signed char *xtrans36);
 
+int oracle_green_matching(uint16_t *img4, int w, int h, uint32_t filters, int maximum);
+int oracle_remove_zeroes(uint16_t *img4, int w, int h, uint32_t filters);
+
 /* output epilogue (epilogue.c) */
 int oracle_convert_color_space(const float *in3, float *out3, int w, int h, const float *transform12);
 float oracle_gamma_value(float v, float gamma_power, float gamma_slope, int output_color_space);

@@ -263,3 +263,47 @@ def test_quantize_and_fused_epilogue(acc):
         got = d_out.download((h, w, 3), dt)
         want = staged if bits == 32 else acc.quantize(staged, bits)
         assert np.array_equal(got, want), bits
+
+
+# ---------------- green matching, zero removal ----------------
+from test_ingest_cpu import (FOUR_COLOR, _native_channels, oracle_green_matching,  # noqa: E402
+                             oracle_remove_zeroes)
+
+
+def _img4_from(vals, filters):
+    h, w = vals.shape
+    ch = _native_channels(h, w, filters)
+    img = np.zeros((h, w, 4), np.uint16)
+    for k in range(4):
+        img[..., k][ch == k] = vals[ch == k]
+    return img
+
+
+def test_green_matching_bit_exact(acc):
+    rng = np.random.Generator(np.random.PCG64(11))
+    for h, w in ((64, 90), (301, 257)):
+        base = 1000 + rng.integers(-12, 13, (h, w))          # flat with a little noise
+        base[h // 3: h // 2, w // 4: w // 2] = rng.integers(200, 4000, (h // 2 - h // 3, w // 2 - w // 4))
+        base[5, 6] = 4090                                     # saturated region
+        img = _img4_from(base.astype(np.uint16), FOUR_COLOR)
+        want = oracle_green_matching(img, FOUR_COLOR, 4095)
+        changed = int(np.count_nonzero(want != img))
+        got = acc.apply_green_matching(img.copy(), FOUR_COLOR, 4095)
+        print(f"green matching {h}x{w}: {changed} samples corrected")
+        assert changed > 0 and np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("density", [0.002, 0.05, 0.6])
+def test_remove_zeroes_bit_exact(acc, density):
+    rng = np.random.Generator(np.random.PCG64(int(density * 1000)))
+    h, w = 203, 311
+    vals = rng.integers(1, 16000, (h, w)).astype(np.uint16)
+    vals[rng.random((h, w)) < density] = 0                 # dead pixels, clusters, mostly dead
+    vals[100:110, 50:70] = 0                               # a block: long raster-order chains
+    for filters in (RGGB, FOUR_COLOR):
+        img = _img4_from(vals, filters)
+        want = oracle_remove_zeroes(img, filters)
+        got = acc.remove_zeroes(img.copy(), filters)
+        print(f"remove_zeroes density {density} filters {filters:#x}: "
+              f"{int(np.count_nonzero(want != img))} samples replaced")
+        assert np.array_equal(got, want)

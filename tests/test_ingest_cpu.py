@@ -79,3 +79,91 @@ def test_ingest_matches_reference_white_balance(filters, xtrans):
     # data maximum over each pixel's fcol_bayer channel (core/libraw_wrapper.cpp:1051)
     fc = scene.fcol_bayer(filters, h, w)
     assert dmax == float(np.take_along_axis(want3, fc[..., None].astype(np.int64), axis=2).max())
+
+
+FOUR_COLOR = 0xB4B4B4B4   # G2 reported as channel 3 at (odd row, even col)
+
+
+def oracle_green_matching(img4, filters, maximum):
+    img = np.ascontiguousarray(img4).copy()
+    h, w, _ = img.shape
+    assert oracle().oracle_green_matching(p(img, U16P), w, h, ctypes.c_uint32(filters), maximum) == 0
+    return img
+
+
+def oracle_remove_zeroes(img4, filters):
+    img = np.ascontiguousarray(img4).copy()
+    h, w, _ = img.shape
+    assert oracle().oracle_remove_zeroes(p(img, U16P), w, h, ctypes.c_uint32(filters)) == 0
+    return img
+
+
+def _native_channels(h, w, filters):
+    r = np.arange(h)[:, None]
+    c = np.arange(w)[None, :]
+    return (filters >> ((((r << 1) & 14) | (c & 1)) << 1)) & 3
+
+
+def _flat_four_color(h, w, g1=1000, g2=900, other=500):
+    ch = _native_channels(h, w, FOUR_COLOR)
+    img = np.zeros((h, w, 4), np.uint16)
+    vals = {0: other, 1: g1, 2: other, 3: g2}
+    for k, v in vals.items():
+        img[..., k][ch == k] = v
+    return img, ch
+
+
+def test_green_matching_known_answers():
+    img, ch = _flat_four_color(12, 14)
+    assert ch[1, 0] == 3 and ch[0, 1] == 1
+    out = oracle_green_matching(img, FOUR_COLOR, 4095)
+    # flat neighbourhoods: G2 sites j = 3,5,7 (< h-3), i = 2..10 (< w-3) scaled by 1000/900
+    for j in range(1, 12, 2):
+        for i in range(0, 14, 2):
+            want = 1000 if (3 <= j < 9 and 2 <= i < 11) else 900
+            assert out[j, i, 3] == want, (j, i)
+    assert np.array_equal(np.delete(out, 3, axis=2), np.delete(img, 3, axis=2))
+    # saturated G2 (>= 0.95 maximum) and busy neighbourhoods are left alone
+    sat = img.copy()
+    sat[5, 4, 3] = 3900
+    assert oracle_green_matching(sat, FOUR_COLOR, 4095)[5, 4, 3] == 3900
+    busy = img.copy()
+    busy[4, 5, 1] = 1500                                   # a diagonal G1 of (5,4) and (3,4)...
+    got = oracle_green_matching(busy, FOUR_COLOR, 4095)
+    assert got[5, 4, 3] == 900 and got[3, 6, 3] == 900     # c1 = (500*3)/6 = 250 > 40.95
+    # X-Trans and the plain RGGB word (no channel 3) change nothing
+    assert np.array_equal(oracle_green_matching(img, 9, 4095), img)
+    rggb = img.copy(); rggb[..., 3] = 0
+    assert np.array_equal(oracle_green_matching(rggb, RGGB, 4095), rggb)
+
+
+def test_remove_zeroes_raster_order():
+    h, w = 9, 11
+    ch = _native_channels(h, w, RGGB)
+    rng = np.random.Generator(np.random.PCG64(3))
+    img = np.zeros((h, w, 4), np.uint16)
+    vals = rng.integers(100, 4000, (h, w)).astype(np.uint16)
+    for k in range(4):
+        img[..., k][ch == k] = vals[ch == k]
+    # bright R samples left of column 3 lift the first zero's mean; two R zeros
+    # two columns apart: the second sees the first one's new value
+    img[:, :3, 0][ch[:, :3] == 0] = 4000
+    img[4, 4, 0] = 0
+    img[4, 6, 0] = 0
+    out = oracle_remove_zeroes(img, RGGB)
+
+    def mean_of(src, r0, c0):
+        vs = [int(src[r, c, 0]) for r in range(r0 - 2, r0 + 3) for c in range(c0 - 2, c0 + 3)
+              if 0 <= r < h and 0 <= c < w and ch[r, c] == 0 and src[r, c, 0] != 0]
+        return sum(vs) // len(vs)
+
+    first = mean_of(img, 4, 4)
+    assert out[4, 4, 0] == first
+    step = img.copy(); step[4, 4, 0] = first
+    assert out[4, 6, 0] == mean_of(step, 4, 6)
+    assert mean_of(step, 4, 6) != mean_of(img, 4, 6)       # order matters here
+    mask = np.ones_like(img, bool); mask[4, 4, 0] = mask[4, 6, 0] = False
+    assert np.array_equal(out[mask], img[mask])
+    # a zero whose window holds no non-zero sample of its channel stays zero
+    lone = np.zeros((3, 3, 4), np.uint16)
+    assert np.array_equal(oracle_remove_zeroes(lone, RGGB), lone)
