@@ -176,6 +176,21 @@ int lre_hip_dev_green_matching(lre_ctx *ctx, const uint16_t *d_src, uint16_t *d_
 int lre_hip_remove_zeroes(lre_ctx *ctx, uint16_t *img4, int width, int height, uint32_t filters);
 int lre_hip_dev_remove_zeroes(lre_ctx *ctx, uint16_t *d_img4, int width, int height,
                               uint32_t filters, void *stream);
+/* The automatic white-balance multipliers of scale_colors
+ * (core/libraw_wrapper.cpp:621-663): the mean of every native channel over the
+ * image (channel 3 folded into 1), pre_mul = (G/R, 1, G/B, 1). */
+int lre_hip_auto_wb(lre_ctx *ctx, const uint16_t *img4, int width, int height, uint32_t filters,
+                    const signed char xtrans[36], float pre_mul[4]);
+/* Device form: d_pre_mul receives the 4 multipliers. */
+int lre_hip_dev_auto_wb(lre_ctx *ctx, const uint16_t *d_img4, int width, int height,
+                        uint32_t filters, const signed char xtrans[36], float *d_pre_mul,
+                        void *stream);
+/* The legacy radial chromatic-aberration correction of scale_colors
+ * (core/libraw_wrapper.cpp:833-880), in place: channels 0 and 2 resampled
+ * bilinearly about the image centre by aber[0] and aber[2] (1 = unchanged). */
+int lre_hip_scale_aber(lre_ctx *ctx, uint16_t *img4, int width, int height, const float aber[4]);
+int lre_hip_dev_scale_aber(lre_ctx *ctx, uint16_t *d_img4, int width, int height,
+                           const float aber[4], void *stream);
 /* Demosaic input planes: extract the native channel of the reference's 3-channel
  * layout into a CFA plane. */
 int lre_hip_dev_rgb3_to_cfa(lre_ctx *ctx, const float *d_raw_rgb, float *d_cfa, int width,

@@ -94,6 +94,11 @@ _SIGS = {
     "lre_hip_dev_green_matching": ([_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
                                     ctypes.c_int, _vp], ctypes.c_int),
     "lre_hip_remove_zeroes": ([_vp, _u16p, ctypes.c_int, ctypes.c_int, ctypes.c_uint32], ctypes.c_int),
+    "lre_hip_auto_wb": ([_vp, _u16p, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, _i8p, _f32p], ctypes.c_int),
+    "lre_hip_dev_auto_wb": ([_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, _i8p, _vp, _vp],
+                            ctypes.c_int),
+    "lre_hip_scale_aber": ([_vp, _u16p, ctypes.c_int, ctypes.c_int, _f32p], ctypes.c_int),
+    "lre_hip_dev_scale_aber": ([_vp, _vp, ctypes.c_int, ctypes.c_int, _f32p, _vp], ctypes.c_int),
     "lre_hip_dev_remove_zeroes": ([_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, _vp],
                                   ctypes.c_int),
     "lre_hip_dev_convert_color_space": ([_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _f32p, _vp],
@@ -304,6 +309,24 @@ class HIPAccelerator:
         img4 = self._img4(img4)
         h, w, _ = img4.shape
         _check(native().lre_hip_remove_zeroes(self._h, _p(img4, _u16p), w, h, filters))
+        return img4
+
+    def auto_wb(self, img4, filters: int, xtrans=None):
+        """Automatic pre_mul of scale_colors (core/libraw_wrapper.cpp:621-663)."""
+        img4 = self._img4(img4)
+        h, w, _ = img4.shape
+        xt = None if xtrans is None else np.ascontiguousarray(np.asarray(xtrans).reshape(36), np.int8)
+        pm = np.zeros(4, np.float32)
+        _check(native().lre_hip_auto_wb(self._h, _p(img4, _u16p), w, h, filters,
+                                        None if xt is None else _p(xt, _i8p), _p(pm)))
+        return pm
+
+    def scale_aber(self, img4, aber):
+        """Legacy radial aberration resampling of scale_colors (:833-880), in place; returns img4."""
+        img4 = self._img4(img4)
+        h, w, _ = img4.shape
+        ab = np.ascontiguousarray(aber, np.float32)
+        _check(native().lre_hip_scale_aber(self._h, _p(img4, _u16p), w, h, _p(ab)))
         return img4
 
     @staticmethod

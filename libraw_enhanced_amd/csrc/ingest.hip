@@ -234,6 +234,94 @@ __global__ __launch_bounds__(64) void zero_fill_dependent(unsigned short *img4, 
     }
 }
 
+// ---- automatic white balance of scale_colors (:621-663): exact integer sums of
+// every native channel; the reference's double sums of integers are exact too ----
+__global__ __launch_bounds__(BLOCK) void awb_sum_kernel(const unsigned short *__restrict__ img4, int W,
+                                                       int H, uint32_t filters, XT xt,
+                                                       unsigned long long *acc) {
+    unsigned long long s[4] = {0, 0, 0, 0}, k[4] = {0, 0, 0, 0};
+    const size_t n = (size_t)W * H;
+    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK) {
+        const int row = (int)(i / W), col = (int)(i % W);
+        const unsigned cc = filters == 9 ? (unsigned)xt.v[((row + 6) % 6) * 6 + (col + 6) % 6]
+                                         : lre::bayer_native(filters, (unsigned)row, (unsigned)col);
+        const unsigned v = img4[i * 4 + cc];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            s[c] += (cc == (unsigned)c) ? v : 0u;
+            k[c] += (cc == (unsigned)c) ? 1u : 0u;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+        for (int o = 32; o > 0; o >>= 1) {
+            s[c] += __shfl_xor(s[c], o, 64);
+            k[c] += __shfl_xor(k[c], o, 64);
+        }
+    if ((threadIdx.x & 63) == 0)
+        for (int c = 0; c < 4; c++) {
+            atomicAdd(&acc[c], s[c]);
+            atomicAdd(&acc[4 + c], k[c]);
+        }
+}
+
+__global__ void awb_finish_kernel(const unsigned long long *acc, float *pre_mul) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double norm[4], count[4];
+    for (int c = 0; c < 4; c++) {
+        norm[c] = (double)acc[c];
+        count[c] = (double)acc[4 + c];
+    }
+    norm[1] += norm[3];
+    count[1] += count[3];
+    for (int c = 0; c < 3; c++) norm[c] = norm[c] / count[c];  // the p-norm with p = 1
+    pre_mul[0] = (float)(norm[1] / norm[0]);
+    pre_mul[1] = 1.0f;
+    pre_mul[2] = (float)(norm[1] / norm[2]);
+    pre_mul[3] = 1.0f;
+}
+
+__global__ void zero_u64(unsigned long long *p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
+// ---- legacy radial chromatic aberration of scale_colors (:833-880) ----
+// float -> unsigned as the reference's x86-64 build converts it: truncation to
+// a 64-bit integer, low 32 bits kept (negative sources become huge, skipped)
+LRE_DEV unsigned f2u_x86(float f) { return (unsigned)(long long)f; }
+
+__global__ __launch_bounds__(BLOCK) void aber_copy_kernel(const unsigned short *__restrict__ img4,
+                                                         unsigned short *__restrict__ p0,
+                                                         unsigned short *__restrict__ p2, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK) {
+        p0[i] = img4[i * 4];
+        p2[i] = img4[i * 4 + 2];
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void aber_warp_kernel(unsigned short *img4, const unsigned short *plane,
+                                                         int c, int W, int H, float scale) {
+    const size_t n = (size_t)W * H;
+    const float cy = (float)H * 0.5f, cx = (float)W * 0.5f;
+    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK) {
+        const int row = (int)(i / W), col = (int)(i % W);
+        const float sc = ((float)col - cx) * scale + cx;
+        const float sr = ((float)row - cy) * scale + cy;
+        const unsigned ur = f2u_x86(sr);
+        if (ur >= (unsigned)H - 1) continue;
+        const float fr = sr - (float)ur;
+        const unsigned uc = f2u_x86(sc);
+        if (uc >= (unsigned)W - 1) continue;
+        const float fc = sc - (float)uc;
+        const unsigned short *p = plane + (size_t)ur * W + uc;
+        const float v = ((float)p[0] * (1.0f - fc) + (float)p[1] * fc) * (1.0f - fr) +
+                        ((float)p[W] * (1.0f - fc) + (float)p[W + 1] * fc) * fr;
+        const float lo = (0.0f < v) ? v : 0.0f;
+        const float cl = (lo < 65535.0f) ? lo : 65535.0f;
+        img4[i * 4 + c] = (unsigned short)cl;
+    }
+}
+
 __global__ void zero_bytes(unsigned *p, size_t n4) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x)
         p[i] = 0u;
@@ -271,6 +359,42 @@ int lre_launch_remove_zeroes(uint16_t *d_img4, int w, int h, uint32_t filters, v
     hipLaunchKernelGGL(zero_classify_kernel, dim3(g), dim3(BLOCK), 0, s, d_img4, w, h, filters, dep, rowflag);
     hipLaunchKernelGGL(zero_fill_independent, dim3(g), dim3(BLOCK), 0, s, d_img4, w, h, filters, dep);
     hipLaunchKernelGGL(zero_fill_dependent, dim3(1), dim3(64), 0, s, d_img4, w, h, filters, dep, rowflag);
+    LRE_HIP_CHECK(hipGetLastError());
+    return LRE_OK;
+}
+
+int lre_launch_auto_wb(const uint16_t *d_img4, int w, int h, uint32_t filters,
+                       const signed char *xtrans, float *d_pre_mul, void *d_scratch, hipStream_t s) {
+    XT xt{};
+    if (xtrans)
+        for (int i = 0; i < 36; i++) xt.v[i] = xtrans[i];
+    unsigned long long *acc = static_cast<unsigned long long *>(d_scratch);
+    hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, s, acc, 8);
+    hipLaunchKernelGGL(awb_sum_kernel, dim3(grid_for((size_t)w * h)), dim3(BLOCK), 0, s,
+                       reinterpret_cast<const unsigned short *>(d_img4), w, h, filters, xt, acc);
+    hipLaunchKernelGGL(awb_finish_kernel, dim3(1), dim3(64), 0, s, acc, d_pre_mul);
+    LRE_HIP_CHECK(hipGetLastError());
+    return LRE_OK;
+}
+
+size_t lre_scale_aber_scratch_bytes(int w, int h) {
+    return 2 * (((size_t)w * h * 2 + 255) & ~(size_t)255);
+}
+
+int lre_launch_scale_aber(uint16_t *d_img4, int w, int h, const float aber[4], void *d_scratch,
+                          hipStream_t s) {
+    if (w <= 1 || h <= 1) return LRE_OK;
+    const size_t n = (size_t)w * h;
+    unsigned short *p0 = static_cast<unsigned short *>(d_scratch);
+    unsigned short *p2 = p0 + (((n * 2 + 255) & ~(size_t)255) / 2);
+    hipLaunchKernelGGL(aber_copy_kernel, dim3(grid_for(n)), dim3(BLOCK), 0, s,
+                       reinterpret_cast<const unsigned short *>(d_img4), p0, p2, n);
+    for (int c = 0; c < 4; c += 2) {
+        const float scale = aber[c];
+        if (fabsf(scale - 1.0f) < 1e-6f || scale <= 0.0f) continue;
+        hipLaunchKernelGGL(aber_warp_kernel, dim3(grid_for(n)), dim3(BLOCK), 0, s,
+                           reinterpret_cast<unsigned short *>(d_img4), c == 0 ? p0 : p2, c, w, h, scale);
+    }
     LRE_HIP_CHECK(hipGetLastError());
     return LRE_OK;
 }

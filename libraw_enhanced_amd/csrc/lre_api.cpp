@@ -531,6 +531,59 @@ int lre_hip_dev_remove_zeroes(lre_ctx *ctx, uint16_t *d_img4, int w, int h, uint
     return lre_launch_remove_zeroes(d_img4, w, h, filters, ctx->scratch, lre_pick_stream(ctx, stream));
 }
 
+int lre_hip_auto_wb(lre_ctx *ctx, const uint16_t *img4, int w, int h, uint32_t filters,
+                    const signed char xtrans[36], float pre_mul[4]) {
+    if (!ctx || !img4 || !pre_mul || (filters == 9 && !xtrans)) return set_arg_error("null pointer");
+    if (!valid_dims(w, h)) return set_arg_error("image size");
+    LRE_HIP_CHECK(hipSetDevice(ctx->device));
+    const size_t n = (size_t)w * h;
+    int rc = lre_ensure_scratch(ctx, aligned(n * 8) + 256 + 256);
+    if (rc) return rc;
+    Carve c(ctx->scratch);
+    uint16_t *d = c.take<uint16_t>(n * 4);
+    float *d_pm = c.take<float>(4);
+    void *ws = c.take<char>(64);
+    if ((rc = copy_in(ctx, d, img4, n * 8))) return rc;
+    if ((rc = lre_launch_auto_wb(d, w, h, filters, xtrans, d_pm, ws, ctx->stream))) return rc;
+    return copy_out(ctx, pre_mul, d_pm, 16);
+}
+
+int lre_hip_dev_auto_wb(lre_ctx *ctx, const uint16_t *d_img4, int w, int h, uint32_t filters,
+                        const signed char xtrans[36], float *d_pre_mul, void *stream) {
+    if (!ctx || !d_img4 || !d_pre_mul || (filters == 9 && !xtrans)) return set_arg_error("null pointer");
+    if (!valid_dims(w, h)) return set_arg_error("image size");
+    LRE_HIP_CHECK(hipSetDevice(ctx->device));
+    int rc = lre_ensure_scratch(ctx, 256);
+    if (rc) return rc;
+    return lre_launch_auto_wb(d_img4, w, h, filters, xtrans, d_pre_mul, ctx->scratch,
+                              lre_pick_stream(ctx, stream));
+}
+
+int lre_hip_scale_aber(lre_ctx *ctx, uint16_t *img4, int w, int h, const float aber[4]) {
+    if (!ctx || !img4 || !aber) return set_arg_error("null pointer");
+    if (!valid_dims(w, h)) return set_arg_error("image size");
+    LRE_HIP_CHECK(hipSetDevice(ctx->device));
+    const size_t n = (size_t)w * h;
+    int rc = lre_ensure_scratch(ctx, aligned(n * 8) + lre_scale_aber_scratch_bytes(w, h));
+    if (rc) return rc;
+    Carve c(ctx->scratch);
+    uint16_t *d = c.take<uint16_t>(n * 4);
+    void *ws = static_cast<char *>(ctx->scratch) + c.off;
+    if ((rc = copy_in(ctx, d, img4, n * 8))) return rc;
+    if ((rc = lre_launch_scale_aber(d, w, h, aber, ws, ctx->stream))) return rc;
+    return copy_out(ctx, img4, d, n * 8);
+}
+
+int lre_hip_dev_scale_aber(lre_ctx *ctx, uint16_t *d_img4, int w, int h, const float aber[4],
+                           void *stream) {
+    if (!ctx || !d_img4 || !aber) return set_arg_error("null pointer");
+    if (!valid_dims(w, h)) return set_arg_error("image size");
+    LRE_HIP_CHECK(hipSetDevice(ctx->device));
+    int rc = lre_ensure_scratch(ctx, lre_scale_aber_scratch_bytes(w, h));
+    if (rc) return rc;
+    return lre_launch_scale_aber(d_img4, w, h, aber, ctx->scratch, lre_pick_stream(ctx, stream));
+}
+
 int lre_hip_black_level(lre_ctx *ctx, uint16_t *img4, int w, int h, const int cblack[4],
                         const int *pos_table, int pos_rows, int pos_cols, int black, int maximum) {
     if (!ctx || !img4 || !cblack) return set_arg_error("null pointer");

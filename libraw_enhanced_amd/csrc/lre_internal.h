@@ -80,6 +80,12 @@ int lre_launch_green_matching(const uint16_t *d_src, uint16_t *d_dst, int w, int
 size_t lre_remove_zeroes_scratch_bytes(int w, int h);
 int lre_launch_remove_zeroes(uint16_t *d_img4, int w, int h, uint32_t filters, void *d_scratch,
                              hipStream_t s);
+// Automatic white balance (d_scratch: 64 bytes) and legacy radial aberration.
+int lre_launch_auto_wb(const uint16_t *d_img4, int w, int h, uint32_t filters,
+                       const signed char *xtrans, float *d_pre_mul, void *d_scratch, hipStream_t s);
+size_t lre_scale_aber_scratch_bytes(int w, int h);
+int lre_launch_scale_aber(uint16_t *d_img4, int w, int h, const float aber[4], void *d_scratch,
+                          hipStream_t s);
 // adjust_maximum on the device: d_maxv = rule(data_max, maximum, threshold)
 int lre_launch_adjust_maximum(const float *d_data_max, float maximum, float threshold,
                               float *d_maxv, hipStream_t s);

@@ -16,7 +16,10 @@
  *  oracle_adjust_maximum  the maximum rule of :1070-1077
  *  oracle_green_matching  apply_green_matching, :496-585
  *  oracle_remove_zeroes   remove_zeroes, :150-212
+ *  oracle_auto_wb         the automatic multipliers of scale_colors, :621-663
+ *  oracle_scale_aber      the legacy radial aberration resampling of scale_colors, :833-880
  */
+#include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -149,5 +152,70 @@ int oracle_remove_zeroes(uint16_t *img4, int w, int h, uint32_t filters) {
                 }
             if (cnt > 0) *cur = (uint16_t)(sum / cnt);
         }
+    return 0;
+}
+
+/* Automatic white balance of scale_colors, core/libraw_wrapper.cpp:621-663
+ * (p = 1): the mean of every native channel (3 folded into 1) over the whole
+ * image; pre_mul = (mean G / mean R, 1, mean G / mean B, 1).  Sums of integers
+ * in double are exact, so their order does not matter. */
+int oracle_auto_wb(const uint16_t *img4, int w, int h, uint32_t filters, const signed char *xtrans36,
+                   float *pre_mul4) {
+    double norm[4] = {0, 0, 0, 0}, count[4] = {0, 0, 0, 0};
+    for (int row = 0; row < h; row++)
+        for (int col = 0; col < w; col++) {
+            const unsigned cc = filters == 9 ? (unsigned)xtrans36[((row + 6) % 6) * 6 + (col + 6) % 6]
+                                             : bayer_native(filters, row, col);
+            norm[cc] += pow(img4[((size_t)row * w + col) * 4 + cc], 1.0);
+            count[cc] += 1;
+        }
+    norm[1] += norm[3];
+    count[1] += count[3];
+    for (int c = 0; c < 3; c++) norm[c] = pow(norm[c] / count[c], 1.0 / 1.0);
+    pre_mul4[0] = (float)(norm[1] / norm[0]);
+    pre_mul4[1] = 1.0f;
+    pre_mul4[2] = (float)(norm[1] / norm[2]);
+    pre_mul4[3] = 1.0f;
+    return 0;
+}
+
+/* float -> unsigned as the reference's x86-64 build converts it: the value is
+ * truncated to a 64-bit integer and its low 32 bits are kept, so sources left
+ * of / above the frame by a pixel or more become huge and are skipped. */
+static unsigned f2u_x86(float f) { return (unsigned)(long long)f; }
+
+/* Legacy radial chromatic aberration of scale_colors, :833-880: channels 0 and
+ * 2 resampled bilinearly at (x - cx) * aber[c] + cx, (y - cy) * aber[c] + cy
+ * from a copy of the channel; samples whose source cell leaves the frame keep
+ * their value. */
+int oracle_scale_aber(uint16_t *img4, int w, int h, const float *aber4) {
+    if (w <= 1 || h <= 1) return 0;
+    const size_t size = (size_t)w * h;
+    uint16_t *plane = (uint16_t *)malloc(size * sizeof(uint16_t));
+    if (!plane) return -1;
+    const float cy = (float)h * 0.5f, cx = (float)w * 0.5f;
+    for (int c = 0; c < 4; c += 2) {
+        const float scale = aber4[c];
+        if (fabsf(scale - 1.0f) < 1e-6f || scale <= 0.0f) continue;
+        for (size_t i = 0; i < size; i++) plane[i] = img4[i * 4 + c];
+        for (int row = 0; row < h; row++)
+            for (int col = 0; col < w; col++) {
+                const float sc = ((float)col - cx) * scale + cx;
+                const float sr = ((float)row - cy) * scale + cy;
+                const unsigned ur = f2u_x86(sr);
+                if (ur >= (unsigned)h - 1) continue;
+                const float fr = sr - (float)ur;
+                const unsigned uc = f2u_x86(sc);
+                if (uc >= (unsigned)w - 1) continue;
+                const float fc = sc - (float)uc;
+                const uint16_t *p = plane + (size_t)ur * w + uc;
+                const float v = (p[0] * (1.0f - fc) + p[1] * fc) * (1.0f - fr) +
+                                (p[w] * (1.0f - fc) + p[w + 1] * fc) * fr;
+                const float lo = (0.0f < v) ? v : 0.0f;            /* std::max(0.0f, v) */
+                const float cl = (lo < 65535.0f) ? lo : 65535.0f;   /* std::min(65535.0f, lo) */
+                img4[((size_t)row * w + col) * 4 + c] = (uint16_t)cl;
+            }
+    }
+    free(plane);
     return 0;
 }

@@ -57,6 +57,9 @@ int oracle_white_balance(const uint16_t *img4, float *rgb3, int w, int h, const 
 
 int oracle_green_matching(uint16_t *img4, int w, int h, uint32_t filters, int maximum);
 int oracle_remove_zeroes(uint16_t *img4, int w, int h, uint32_t filters);
+int oracle_auto_wb(const uint16_t *img4, int w, int h, uint32_t filters, const signed char *xtrans36,
+                   float *pre_mul4);
+int oracle_scale_aber(uint16_t *img4, int w, int h, const float *aber4);
 
 /* output epilogue (epilogue.c) */
 int oracle_convert_color_space(const float *in3, float *out3, int w, int h, const float *transform12);

@@ -307,3 +307,32 @@ def test_remove_zeroes_bit_exact(acc, density):
         print(f"remove_zeroes density {density} filters {filters:#x}: "
               f"{int(np.count_nonzero(want != img))} samples replaced")
         assert np.array_equal(got, want)
+
+
+from test_ingest_cpu import oracle_auto_wb, oracle_scale_aber  # noqa: E402
+
+
+def test_auto_wb_bit_exact(acc):
+    rng = np.random.Generator(np.random.PCG64(21))
+    for filters, xt in ((RGGB, None), (FOUR_COLOR, None), (9, scene.XTRANS)):
+        vals = rng.integers(0, 16384, (517, 333)).astype(np.uint16)
+        if xt is None:
+            img = _img4_from(vals, filters)
+        else:
+            cx = np.asarray(xt)[np.arange(517)[:, None] % 6, np.arange(333)[None, :] % 6]
+            img = np.zeros((517, 333, 4), np.uint16)
+            for k in range(3):
+                img[..., k][cx == k] = vals[cx == k]
+        got = acc.auto_wb(img, filters, xt)
+        want = oracle_auto_wb(img, filters, xt)
+        assert np.array_equal(got, want), (filters, got, want)
+
+
+@pytest.mark.parametrize("aber", [(1.0008, 1, 0.9993, 1), (1.05, 1, 0.95, 1), (0.9, 1, 1.0, 1)])
+def test_scale_aber_bit_exact(acc, aber):
+    rng = np.random.Generator(np.random.PCG64(5))
+    img = rng.integers(0, 65536, (301, 411, 4)).astype(np.uint16)
+    want = oracle_scale_aber(img, aber)
+    got = acc.scale_aber(img.copy(), aber)
+    assert np.array_equal(got, want)
+    assert np.count_nonzero(want != img) > 0

@@ -167,3 +167,67 @@ def test_remove_zeroes_raster_order():
     # a zero whose window holds no non-zero sample of its channel stays zero
     lone = np.zeros((3, 3, 4), np.uint16)
     assert np.array_equal(oracle_remove_zeroes(lone, RGGB), lone)
+
+
+def oracle_auto_wb(img4, filters, xtrans=None):
+    img = np.ascontiguousarray(img4)
+    h, w, _ = img.shape
+    pm = np.zeros(4, np.float32)
+    xt = None if xtrans is None else np.ascontiguousarray(np.asarray(xtrans).reshape(36), np.int8)
+    assert oracle().oracle_auto_wb(p(img, U16P), w, h, ctypes.c_uint32(filters),
+                                   None if xt is None else p(xt, I8P), p(pm)) == 0
+    return pm
+
+
+def oracle_scale_aber(img4, aber):
+    img = np.ascontiguousarray(img4).copy()
+    h, w, _ = img.shape
+    ab = np.ascontiguousarray(aber, np.float32)
+    assert oracle().oracle_scale_aber(p(img, U16P), w, h, p(ab)) == 0
+    return img
+
+
+def test_auto_wb_known_answers():
+    h, w = 24, 30
+    ch = _native_channels(h, w, RGGB)
+    img = np.zeros((h, w, 4), np.uint16)
+    for k, v in {0: 100, 1: 200, 2: 50}.items():
+        img[..., k][ch == k] = v
+    assert oracle_auto_wb(img, RGGB).tolist() == [2.0, 1.0, 4.0, 1.0]
+    # channel 3 (four-colour G2) is folded into green
+    ch4 = _native_channels(h, w, FOUR_COLOR)
+    img4c = np.zeros((h, w, 4), np.uint16)
+    for k, v in {0: 100, 1: 300, 2: 50, 3: 100}.items():
+        img4c[..., k][ch4 == k] = v
+    assert oracle_auto_wb(img4c, FOUR_COLOR).tolist() == [2.0, 1.0, 4.0, 1.0]
+    # X-Trans uses its own pattern (20 of 36 sites green)
+    xt = scene.XTRANS
+    r = np.arange(h)[:, None] % 6
+    c = np.arange(w)[None, :] % 6
+    cx = np.asarray(xt)[r, c]
+    imgx = np.zeros((h, w, 4), np.uint16)
+    for k, v in {0: 60, 1: 180, 2: 90}.items():
+        imgx[..., k][cx == k] = v
+    assert oracle_auto_wb(imgx, 9, xt).tolist() == [3.0, 1.0, 2.0, 1.0]
+
+
+def test_scale_aber_closed_forms():
+    h, w = 40, 64
+    img = np.zeros((h, w, 4), np.uint16)
+    img[..., 0] = (np.arange(w)[None, :] * 100 + 50).astype(np.uint16)     # linear in x
+    img[..., 2] = (np.arange(h)[:, None] * 300 + 7).astype(np.uint16)      # linear in y
+    img[..., 1] = 1234
+    assert np.array_equal(oracle_scale_aber(img, (1, 1, 1, 1)), img)
+    out = oracle_scale_aber(img, (1.01, 1, 0.98, 1))
+    assert np.array_equal(out[..., 1], img[..., 1])
+    # bilinear resampling of a linear ramp is the ramp at the source position
+    cx, cy = np.float32(w * 0.5), np.float32(h * 0.5)
+    for (y, x) in [(10, 5), (20, 32), (30, 60), (39, 63), (0, 0)]:
+        sx = (np.float32(x) - cx) * np.float32(1.01) + cx
+        sy = (np.float32(y) - cy) * np.float32(1.01) + cy
+        if 0 <= sy < h - 1 and 0 <= sx < w - 1:
+            assert abs(int(out[y, x, 0]) - (float(sx) * 100 + 50)) <= 1, (y, x)
+        elif sx < -1 or sy < -1 or sx >= w - 1 or sy >= h - 1:
+            assert out[y, x, 0] == img[y, x, 0], (y, x)      # source cell outside: unchanged
+        sy2 = (np.float32(y) - cy) * np.float32(0.98) + cy
+        assert abs(int(out[y, x, 2]) - (float(sy2) * 300 + 7)) <= 1, (y, x)
