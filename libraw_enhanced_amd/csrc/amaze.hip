@@ -39,7 +39,6 @@ constexpr int O_GREEN = 0, O_DSQ = N2, O_WV = 2 * N2, O_WH = 3 * N2, O_VCD = 4 *
               O_CFA = 11 * N2 + 3 * TS * TSH, O_NYQ = 12 * N2 + 3 * TS * TSH,
               O_NYQTEST = O_NYQ + TS * TSH / 4, ARENA_FLOATS = O_NYQTEST + TS * TSH;
 static_assert(ARENA_FLOATS % 64 == 0, "arena stride keeps 256-byte alignment");
-static_assert(2 * BLOCK >= TS, "one site of a tile row per thread in the smoothing passes");
 
 struct Args {
     const float *cfa_plane;
@@ -86,6 +85,81 @@ LRE_DEV void for_sites(const Args &A, int r0, int r1, int c0, int c1, F f) {
 }
 
 LRE_DEV unsigned fct(const Args &A, int r, int c) { return A.cf[r & 1][c & 1]; }
+
+// Lane exchange inside one wavefront: lane l receives lane l-1's value
+// (wave_shr:1) or lane l+1's (wave_shl:1); the lane without a source gets 0.
+LRE_DEV float from_lane_below(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xF, 0xF, false));
+}
+LRE_DEV float from_lane_above(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xF, 0xF, false));
+}
+
+// Row-sequential smoothing of a half-index weight plane held in LDS (hvwt
+// :1446-1448, pmwt :1537-1539): for the sites of rows [b, rr1-b) and columns
+// [b, cc1-b), in raster order, a weight is replaced by the mean of its four
+// diagonal neighbours (upper-left, upper-right, lower-left, lower-right, summed
+// in that order) when that mean lies further from 0.5; the upper neighbours
+// are already replaced.
+//
+// In half-index columns h = cc >> 1 a site of row rr sits at 2h + parity(rr)
+// and its diagonal neighbours at h-1, h (even parity) or h, h+1 (odd parity)
+// of rows rr-1 and rr+1.  One wavefront runs the recurrence: lane l owns the
+// half-columns h0 + 2l and h0 + 2l + 1 (h0 = b/2 - 2) and keeps the row above
+// in registers, taking the neighbouring lanes' values by DPP; the lower row's
+// original values are read from LDS one row ahead.  Every lane stores both of
+// its cells back each row: a cell that is not a site of the range keeps its
+// value, and lanes past the row end all address its last cell, which is never
+// a site.  Sites of one row do not depend on each other.
+LRE_DEV void smooth_rows(const Args &A, float *sm, int b, int rr1, int cc1) {
+    if (threadIdx.x < 64) {
+        const int h = b / 2 - 2 + 2 * (int)threadIdx.x, h1 = h + 1;
+        const int last = rr1 - b;  // last row read (lower neighbours of the last row)
+        const int om = min(h - 1, TSH - 1), o0 = min(h, TSH - 1), o1 = min(h1, TSH - 1), o2 = min(h + 2, TSH - 1);
+        const int pe = (int)(A.cf[0][0] & 1), po = (int)(A.cf[1][0] & 1);
+        const bool in0 = h >= b / 2, in1 = h1 >= b / 2;
+        const bool ok0e = in0 && 2 * h + pe < cc1 - b, ok1e = in1 && 2 * h1 + pe < cc1 - b;
+        const bool ok0o = in0 && 2 * h + po < cc1 - b, ok1o = in1 && 2 * h1 + po < cc1 - b;
+        const float *row = sm + (b - 1) * TSH;
+        float s0 = row[o0], s1 = row[o1];  // row above, final values
+        row += TSH;
+        float x0 = row[o0], x1 = row[o1];  // this row, original values
+        row += TSH;
+        float lm = row[om], l0 = row[o0], l1 = row[o1], l2 = row[o2];  // row below
+        // one row; rows alternate parity, starting with an even row (b is even)
+        auto step = [&](int rr, int par, bool ok0, bool ok1) {
+            const float *nx = sm + min(rr + 2, last) * TSH;  // row after next
+            const float nm = nx[om], n0 = nx[o0], n1 = nx[o1], n2 = nx[o2];
+            const float mid = s0 + s1 + l0 + l1;
+            float a0, a1;
+            if (par == 0) {
+                a0 = from_lane_below(s1) + s0 + lm + l0;
+                a1 = mid;
+            } else {
+                a0 = mid;
+                a1 = s1 + from_lane_above(s0) + l1 + l2;
+            }
+            a0 = 0.25f * a0;
+            a1 = 0.25f * a1;
+            const bool t0 = ok0 & (fabsf(0.5f - x0) < fabsf(0.5f - a0));
+            const bool t1 = ok1 & (fabsf(0.5f - x1) < fabsf(0.5f - a1));
+            const float y0 = t0 ? a0 : x0, y1 = t1 ? a1 : x1;
+            float *cur = sm + rr * TSH;
+            cur[o0] = y0;
+            cur[o1] = y1;
+            s0 = y0; s1 = y1;
+            x0 = l0; x1 = l1;
+            lm = nm; l0 = n0; l1 = n1; l2 = n2;
+        };
+        int rr = b;
+        for (; rr + 1 < last; rr += 2) {
+            step(rr, pe, ok0e, ok1e);
+            step(rr + 1, po, ok0o, ok1o);
+        }
+        if (rr < last) step(rr, pe, ok0e, ok1e);
+    }
+    __syncthreads();
+}
 
 // LDS: the tile's CFA plane, and one half-index plane for the row-sequential
 // smoothing passes
@@ -362,7 +436,9 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
         float hvar1 = EPSSQ + hw * hr2 + (1.f - hw) * hl2;
         float varwt = hvar / (vvar + hvar);
         float diffwt = hvar1 / (vvar1 + hvar1);
-        hvwt[i >> 1] = ((0.5f - varwt) * (0.5f - diffwt) > 0.f && fabsf(0.5f - diffwt) < fabsf(0.5f - varwt)) ? varwt : diffwt;
+        const float w = ((0.5f - varwt) * (0.5f - diffwt) > 0.f && fabsf(0.5f - diffwt) < fabsf(0.5f - varwt)) ? varwt : diffwt;
+        hvwt[i >> 1] = w;
+        sm[i >> 1] = w;  // LDS copy for the smoothing pass
         const float *d = cdsq, *g = dsq;
         nyqtest[i >> 1] =
             (GODD[0] * d[i] + GODD[1] * (d[i - m1] + d[i + p1] + d[i - p1] + d[i + m1]) +
@@ -418,33 +494,21 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
             sv = scfa - half(sv);
             area = half(area);
             float hv2 = EPSSQ + fabsf(area * sqh - sh * sh), vv2 = EPSSQ + fabsf(area * sqv - sv * sv);
-            hvwt[i >> 1] = hv2 / (vv2 + hv2);
+            const float w = hv2 / (vv2 + hv2);
+            hvwt[i >> 1] = w;
+            sm[i >> 1] = w;
         });
         __syncthreads();
     }
     float *dg0 = vcdalt;        // Dgrb0 over vcdalt (half-index)
     float *d2 = ar + O_Q1M;     // Dgrb2 {h,v} pairs over Dgrbsq1m/Dgrbsq1p
-    // hvwt smoothing reads updated upper-diagonal neighbours: row by row, on a
-    // copy of the half-index plane in LDS (rows 7 .. rr1-8), copied back after
-    // (a row has at most 76 sites, fewer than the workgroup's threads).
-    for (int k = 7 * TSH + threadIdx.x; k < (rr1 - 7) * TSH; k += BLOCK) sm[k] = hvwt[k];
-    __syncthreads();
-    for (int rr = 8; rr < rr1 - 8; rr++) {
-        const int c0 = 8 + (int)(fct(A, rr, 2) & 1);
-        const int cc = c0 + 2 * (int)threadIdx.x;
-        if (cc < cc1 - 8) {
-            int i = rr * TS + cc, k = i >> 1;
-            float alt = 0.25f * (sm[(i - m1) >> 1] + sm[(i + p1) >> 1] + sm[(i - p1) >> 1] + sm[(i + m1) >> 1]);
-            float hk = sm[k];
-            hk = fabsf(0.5f - hk) < fabsf(0.5f - alt) ? alt : hk;
-            sm[k] = hk;
-        }
-        __syncthreads();
-    }
-    for (int k = 8 * TSH + threadIdx.x; k < (rr1 - 8) * TSH; k += BLOCK) hvwt[k] = sm[k];
+    // hvwt smoothing (:1446-1448) on the LDS copy that stages 4-6 wrote beside
+    // hvwt; each smoothed site's weight is stored back by the pass below
+    smooth_rows(A, sm, 8, rr1, cc1);
     for_sites(A, 8, rr1 - 8, 8, cc1 - 8, [&](int rr, int cc) {
         int i = rr * TS + cc, k = i >> 1;
-        float hk = hvwt[k];
+        const float hk = sm[k];
+        hvwt[k] = hk;
         float d0 = hk * vcd[i] + (1.f - hk) * hcd[i];
         dg0[k] = d0;
         float g = cfa[i] + d0;
@@ -539,7 +603,9 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
                               GEVEN[1] * (QS(q1p, -v2 - 1) + QS(q1p, -v2 + 1) + QS(q1p, -2 - v1) + QS(q1p, 2 - v1) +
                                           QS(q1p, -2 + v1) + QS(q1p, 2 + v1) + QS(q1p, v2 - 1) + QS(q1p, v2 + 1)));
 #undef QS
-        pmwt[k] = varm / (varp + varm);
+        const float w = varm / (varp + varm);
+        pmwt[k] = w;
+        sm[k] = w;  // LDS copy for the smoothing pass
     });
     __syncthreads();
 
@@ -547,26 +613,13 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
     float *rbint = dsq;
     // same recurrence for pmwt; rbint of a site needs only that site's final
     // pmwt, so it is computed in a parallel pass afterwards
-    for (int k = 9 * TSH + threadIdx.x; k < (rr1 - 9) * TSH; k += BLOCK) sm[k] = pmwt[k];
-    __syncthreads();
-    for (int rr = 10; rr < rr1 - 10; rr++) {
-        const int c0 = 10 + (int)(fct(A, rr, 2) & 1);
-        const int cc = c0 + 2 * (int)threadIdx.x;
-        if (cc < cc1 - 10) {
-            int i = rr * TS + cc, k = i >> 1;
-            float alt = 0.25f * (sm[(i - m1) >> 1] + sm[(i + p1) >> 1] + sm[(i - p1) >> 1] + sm[(i + m1) >> 1]);
-            float pk = sm[k];
-            if (fabsf(0.5f - pk) < fabsf(0.5f - alt)) pk = alt;
-            sm[k] = pk;
-        }
-        __syncthreads();
-    }
+    smooth_rows(A, sm, 10, rr1, cc1);
     for_sites(A, 10, rr1 - 10, 10, cc1 - 10, [&](int rr, int cc) {
         const int i = rr * TS + cc, k = i >> 1;
         const float pk = sm[k];
+        pmwt[k] = pk;
         rbint[k] = half(cfa[i] + rbm[k] * (1.f - pk) + rbp[k] * pk);
     });
-    for (int k = 10 * TSH + threadIdx.x; k < (rr1 - 10) * TSH; k += BLOCK) pmwt[k] = sm[k];
     __syncthreads();
     // (:1541-1566)
     for_sites(A, 12, rr1 - 12, 12, cc1 - 12, [&](int rr, int cc) {
