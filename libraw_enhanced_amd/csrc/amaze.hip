@@ -86,6 +86,41 @@ LRE_DEV void for_sites(const Args &A, int r0, int r1, int c0, int c1, F f) {
 
 LRE_DEV unsigned fct(const Args &A, int r, int c) { return A.cf[r & 1][c & 1]; }
 
+// for_rect / for_sites with the work split into compute(rr, cc) -> R and
+// store(rr, cc, R), one pixel or site per iteration.
+template <class C, class S>
+LRE_DEV void for_rect_split(int r0, int r1, int c0, int c1, C compute, S store) {
+    for_rect(r0, r1, c0, c1, [&](int rr, int cc) { store(rr, cc, compute(rr, cc)); });
+}
+template <class C, class S>
+LRE_DEV void for_sites_split(const Args &A, int r0, int r1, int c0, int c1, C compute, S store) {
+    for_sites(A, r0, r1, c0, c1, [&](int rr, int cc) { store(rr, cc, compute(rr, cc)); });
+}
+
+// The same sites as for_sites, two per thread: (rr, cc) and (rr + 2, cc) share
+// a column and a colour.  compute(rr, cc) only loads and computes; both results
+// are stored after both computations, so the loads of the two sites overlap
+// and the loads they have in common are issued once.  Rows go in groups of
+// four: pairs (r, r+2) and (r+1, r+3).
+template <class C, class S>
+LRE_DEV void for_site_pairs(const Args &A, int r0, int r1, int c0, int c1, C compute, S store) {
+    const int ns = (c1 - c0 + 1) / 2, nr = r1 - r0;
+    if (ns <= 0 || nr <= 0) return;
+    const int n = ((nr + 3) / 4) * 2 * ns;
+    for (int t = threadIdx.x; t < n; t += BLOCK) {
+        const int g = t / (2 * ns), rem = t - g * 2 * ns;
+        const int p = rem >= ns ? 1 : 0, j = rem - p * ns;
+        const int ra = r0 + 4 * g + p, rb = ra + 2;
+        const int cc = c0 + (int)(A.cf[ra & 1][0] & 1) + 2 * j;
+        if (ra >= r1 || cc >= c1) continue;
+        const bool vb = rb < r1;
+        const auto Ra = compute(ra, cc);
+        const auto Rb = compute(vb ? rb : ra, cc);  // a row past the range repeats the first site
+        store(ra, cc, Ra);
+        if (vb) store(rb, cc, Rb);
+    }
+}
+
 // Lane exchange inside one wavefront: lane l receives lane l-1's value
 // (wave_shr:1) or lane l+1's (wave_shl:1); the lane without a source gets 0.
 LRE_DEV float from_lane_below(float v) {
@@ -413,8 +448,8 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
     __syncthreads();
 
     // stages 4-5 (:1400-1428)
-    for_sites(A, 6, rr1 - 6, 6, cc1 - 6, [&](int rr, int cc) {
-        int i = rr * TS + cc;
+    for_site_pairs(A, 6, rr1 - 6, 6, cc1 - 6, [&](int rr, int cc) {
+        const int i = rr * TS + cc;
         float uave = vcd[i] + vcd[i - v1] + vcd[i - v2] + vcd[i - v3];
         float dave = vcd[i] + vcd[i + v1] + vcd[i + v2] + vcd[i + v3];
         float lave = hcd[i] + hcd[i - 1] + hcd[i - 2] + hcd[i - 3];
@@ -436,11 +471,16 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
         float hvar1 = EPSSQ + hw * hr2 + (1.f - hw) * hl2;
         float varwt = hvar / (vvar + hvar);
         float diffwt = hvar1 / (vvar1 + hvar1);
-        const float w = ((0.5f - varwt) * (0.5f - diffwt) > 0.f && fabsf(0.5f - diffwt) < fabsf(0.5f - varwt)) ? varwt : diffwt;
-        hvwt[i >> 1] = w;
-        sm[i >> 1] = w;  // LDS copy for the smoothing pass
+        return ((0.5f - varwt) * (0.5f - diffwt) > 0.f && fabsf(0.5f - diffwt) < fabsf(0.5f - varwt)) ? varwt : diffwt;
+    }, [&](int rr, int cc, float w) {
+        const int k = (rr * TS + cc) >> 1;
+        hvwt[k] = w;
+        sm[k] = w;  // LDS copy for the smoothing pass
+    });
+    for_site_pairs(A, 6, rr1 - 6, 6, cc1 - 6, [&](int rr, int cc) {
+        const int i = rr * TS + cc;
         const float *d = cdsq, *g = dsq;
-        nyqtest[i >> 1] =
+        return
             (GODD[0] * d[i] + GODD[1] * (d[i - m1] + d[i + p1] + d[i - p1] + d[i + m1]) +
              GODD[2] * (d[i - v2] + d[i - 2] + d[i + 2] + d[i + v2]) +
              GODD[3] * (d[i - m2] + d[i + p2] + d[i - p2] + d[i + m2])) -
@@ -449,7 +489,7 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
              GGRAD[3] * (g[i - v2] + g[i - 2] + g[i + 2] + g[i + v2]) +
              GGRAD[4] * (g[i - v2 - 1] + g[i - v2 + 1] + g[i - TS - 2] + g[i - TS + 2] + g[i + TS - 2] + g[i + TS + 2] + g[i + v2 - 1] + g[i + v2 + 1]) +
              GGRAD[5] * (g[i - m2] + g[i + p2] + g[i - p2] + g[i + m2]));
-    });
+    }, [&](int rr, int cc, float nt) { nyqtest[(rr * TS + cc) >> 1] = nt; });
     __syncthreads();
     // Nyquist flags start at zero (:1432); the clear also wipes what the tile
     // fill wrote past the end of the CFA array
@@ -505,16 +545,26 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
     // hvwt smoothing (:1446-1448) on the LDS copy that stages 4-6 wrote beside
     // hvwt; each smoothed site's weight is stored back by the pass below
     smooth_rows(A, sm, 8, rr1, cc1);
-    for_sites(A, 8, rr1 - 8, 8, cc1 - 8, [&](int rr, int cc) {
-        int i = rr * TS + cc, k = i >> 1;
-        const float hk = sm[k];
-        hvwt[k] = hk;
-        float d0 = hk * vcd[i] + (1.f - hk) * hcd[i];
-        dg0[k] = d0;
-        float g = cfa[i] + d0;
-        green[i] = g;
-        d2[2 * k] = nyq2[k] ? lre::sq(g - half(green[i - 1] + green[i + 1])) : 0.f;
-        d2[2 * k + 1] = nyq2[k] ? lre::sq(g - half(green[i - v1] + green[i + v1])) : 0.f;
+    struct SG {
+        float hk, d0, g, dh, dv;
+    };
+    for_site_pairs(A, 8, rr1 - 8, 8, cc1 - 8, [&](int rr, int cc) {
+        const int i = rr * TS + cc, k = i >> 1;
+        SG R;
+        R.hk = sm[k];
+        R.d0 = R.hk * vcd[i] + (1.f - R.hk) * hcd[i];
+        R.g = cfa[i] + R.d0;
+        const bool q = nyq2[k] != 0;
+        R.dh = q ? lre::sq(R.g - half(green[i - 1] + green[i + 1])) : 0.f;
+        R.dv = q ? lre::sq(R.g - half(green[i - v1] + green[i + v1])) : 0.f;
+        return R;
+    }, [&](int rr, int cc, const SG &R) {
+        const int i = rr * TS + cc, k = i >> 1;
+        hvwt[k] = R.hk;
+        dg0[k] = R.d0;
+        green[i] = R.g;
+        d2[2 * k] = R.dh;
+        d2[2 * k + 1] = R.dv;
     });
     __syncthreads();
     if (any_nyq) {
@@ -560,7 +610,7 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
     // stage 8 (:1505-1532)
     float *rbm = vcd, *rbp = hcdalt, *pmwt = wh;
     for_sites(A, 8, rr1 - 8, 8, cc1 - 8, [&](int rr, int cc) {
-        int i = rr * TS + cc, k = i >> 1;
+        const int i = rr * TS + cc, k = i >> 1;
         float c0 = cfa[i];
         float rse = twice(cfa[i + m1]) / (EPS + c0 + cfa[i + m2]);
         float rnw = twice(cfa[i - m1]) / (EPS + c0 + cfa[i - m2]);
@@ -622,9 +672,14 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
     });
     __syncthreads();
     // (:1541-1566)
-    for_sites(A, 12, rr1 - 12, 12, cc1 - 12, [&](int rr, int cc) {
-        int i = rr * TS + cc, k = i >> 1;
-        if (fabsf(0.5f - pmwt[k]) >= fabsf(0.5f - hvwt[k])) return;
+    struct SR {
+        bool on;
+        float g, d0;
+    };
+    for_sites_split(A, 12, rr1 - 12, 12, cc1 - 12, [&](int rr, int cc) {
+        const int i = rr * TS + cc, k = i >> 1;
+        SR R{false, 0.f, 0.f};
+        if (fabsf(0.5f - pmwt[k]) >= fabsf(0.5f - hvwt[k])) return R;
         float rb = rbint[k];
         float ru = twice(cfa[i - v1]) / (EPS + rb + rbint[k - v1]);
         float rd = twice(cfa[i + v1]) / (EPS + rb + rbint[k + v1]);
@@ -655,8 +710,15 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
         if (gv > A.clip_pt) gv = lre::med3(gv, up, dn);
         float hk = hvwt[k];
         float g = gh * (1.f - hk) + gv * hk;
-        green[i] = g;
-        dg0[k] = g - cfa[i];
+        R.on = true;
+        R.g = g;
+        R.d0 = g - cfa[i];
+        return R;
+    }, [&](int rr, int cc, const SR &R) {
+        if (!R.on) return;
+        const int i = rr * TS + cc;
+        green[i] = R.g;
+        dg0[i >> 1] = R.d0;
     });
     __syncthreads();
 
@@ -696,10 +758,12 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
     __syncthreads();
 
     // stage 10 (:1597-1625)
-    for_rect(16, rr1 - 16, 16, cc1 - 16, [&](int rr, int cc) {
-        int row = rr + top, col = cc + left;
-        if (row < 0 || row >= H || col < 0 || col >= W) return;
-        int t = rr * TS + cc;
+    struct SO {
+        float r, g, b;
+    };
+    for_rect_split(16, rr1 - 16, 16, cc1 - 16, [&](int rr, int cc) {
+        const int row = rr + top, col = cc + left;
+        const int t = rr * TS + cc;
         float r, g, b;
         if (fct(A, row, col) == 1) {
             g = cfa[t];
@@ -714,10 +778,14 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
             r = g - dg0[t >> 1];
             b = g - dg1[t >> 1];
         }
+        return SO{r, g, b};
+    }, [&](int rr, int cc, const SO &R) {
+        const int row = rr + top, col = cc + left;
+        if (row < 0 || row >= H || col < 0 || col >= W) return;
         float *o = A.rgb + ((size_t)row * W + col) * 3;
-        o[0] = (0.f < r) ? r : 0.f;
-        o[1] = (0.f < g) ? g : 0.f;
-        o[2] = (0.f < b) ? b : 0.f;
+        o[0] = (0.f < R.r) ? R.r : 0.f;
+        o[1] = (0.f < R.g) ? R.g : 0.f;
+        o[2] = (0.f < R.b) ? R.b : 0.f;
     });
     __syncthreads();
 }
