@@ -13,6 +13,13 @@
 // the last write wins.  The homogeneity map is cleared per tile, so rows the
 // tile does not write read as zero (the reference's per-thread buffers keep
 // the previous tile's values there).
+//
+// Every interpolation step reads and writes one direction buffer at a time,
+// and buffers 4-7 start as copies of 0-3 after the first pass.  So the tile
+// runs direction by direction: buffer j is built in LDS (156 KB), goes through
+// the first pass, is stored as buffer j, then goes through the second and
+// third passes and is stored as buffer j+4; the derivative maps of both are
+// computed from LDS before each store.
 #include <cfloat>
 
 #include "lre_common.h"
@@ -78,8 +85,14 @@ LRE_DEV int walk_col(int ls, int coff0, int k) {
     return ls + 3 * (k >> 1) + ((k & 1) ? coff0 : 0);
 }
 
+// LDS: one direction buffer, [row][col][rgb] like a buffer of the arena
+constexpr size_t LDS_BYTES = DIRSZ * sizeof(float);
+// bound on the number of sites a column walk visits in one tile row
+constexpr int KW = (2 * TS) / 3 + 2;
+
 __device__ void xtrans_tile(const XArgs &A, char *ar, int top, int left) {
     const int W = A.W, H = A.H;
+    extern __shared__ float lds[];
     float *rgb0 = reinterpret_cast<float *>(ar + A_RGB);
     float *drv = reinterpret_cast<float *>(ar + A_DRV);
     MM *gm = reinterpret_cast<MM *>(ar + A_GM);
@@ -89,6 +102,7 @@ __device__ void xtrans_tile(const XArgs &A, char *ar, int top, int left) {
     const int mrow0 = min(top + TS, H - 3), mcol0 = min(left + TS, W - 3);
     const int nr = mrow0 - top, nc = mcol0 - left;
 #define RGB(base, d, r, c) ((base) + ((size_t)(d) * DIRSZ + ((size_t)(r) * TS + (c)) * 3))
+#define PX(r, c) (lds + ((r) * TS + (c)) * 3)
 #define GM(r, c) gm[(size_t)(r) * (TS / 2) + (c)]
 
     // clear the homogeneity map (rows the tile leaves unwritten read as zero)
@@ -125,178 +139,167 @@ __device__ void xtrans_tile(const XArgs &A, char *ar, int top, int left) {
             if (col < mcol0) GM(row - top, (col - left) >> 1) = mmhex(col, hx);
         }
     }
-    // direction buffers 0-3: the raw sample at its colour, zero elsewhere; the
-    // reference copies buffer 0 into 1-3 one float short, so that last float is 0
-    for (int t = threadIdx.x; t < TS * TS; t += BLOCK) {
-        const int r = t / TS, c = t % TS;
-        float v[3] = {0.f, 0.f, 0.f};
-        if (r < nr && c < nc) v[fcolx(A, r + top, c + left)] = rawv(A, r + top, c + left);
-        for (int d = 0; d < 4; d++) {
-            float *p = RGB(rgb0, d, r, c);
-            p[0] = v[0];
-            p[1] = v[1];
-            p[2] = (d > 0 && t == TS * TS - 1) ? 0.f : v[2];
-        }
-    }
     __syncthreads();
 
-    // green along four directions (:2075-2128)
-    for (int t = threadIdx.x; t < nr * TS; t += BLOCK) {
-        const int row = top + t / TS, k = t % TS;
-        const int ls = first_nongreen(A, row, left, mcol0);
-        const int coff0 = A.rshift[row % 3] == 1 ? 3 : 1 + (fcolx(A, row, ls + 1) & 1);
-        const int col = walk_col(ls, coff0, k);
-        if (col >= mcol0) continue;
-        const short *hx = coff0 == 3 ? A.hex[0][row % 3][ls % 3]
-                                     : A.hex[0][row % 3][((k & 1) ? ls + coff0 : ls) % 3];
-        const int swap = coff0 == 3 ? 0 : 1;
-        float color[4];
-        color[0] = 0.6796875f * (rawhex(A, row, col, hx[1]) + rawhex(A, row, col, hx[0])) -
-                   0.1796875f * (rawhex(A, row, col, 2 * hx[1]) + rawhex(A, row, col, 2 * hx[0]));
-        color[1] = 0.87109375f * rawhex(A, row, col, hx[3]) + rawhex(A, row, col, hx[2]) * 0.12890625f +
-                   0.359375f * (rawhex(A, row, col, 0) - rawhex(A, row, col, -hx[2]));
-        for (int c = 0; c < 2; c++)
-            color[2 + c] = 0.640625f * rawhex(A, row, col, hx[4 + c]) + 0.359375f * rawhex(A, row, col, -2 * hx[4 + c]) +
-                           0.12890625f * (2.f * rawhex(A, row, col, 0) - rawhex(A, row, col, 3 * hx[4 + c]) -
-                                          rawhex(A, row, col, -3 * hx[4 + c]));
-        const MM m = GM(row - top, (col - left) >> 1);
-        for (int c = 0; c < 4; c++) RGB(rgb0, c ^ swap, row - top, col - left)[1] = limf(color[c], m.min, m.max);
-    }
-    __syncthreads();
+    // rows of the R/B steps for 2x2 green blocks: those with (row - sgrow) % 3 != 0
+    int topstart = top + 2;
+    for (; topstart < mrow0 - 2; topstart++)
+        if ((topstart - A.sgrow) % 3) break;
+    int ls22 = left + 2;
+    for (; ls22 < mcol0 - 2; ls22++)
+        if ((ls22 - A.sgcol) % 3) break;
+    const int coff22 = 2 - (fcolx(A, topstart, ls22 + 1) & 1);
+    const int adj22 = ((topstart - A.sgrow) % 3 + 3) % 3 == 2 ? 1 : 0;  // q-th valid row: topstart + (3q + adj) / 2
+    int nrow22 = 0;
+    while (topstart + (3 * nrow22 + adj22) / 2 < mrow0 - 2) nrow22++;
 
-    float *rgb = rgb0;
-    for (int pass = 0; pass < 3; pass++) {
-        if (pass == 1) {
-            // buffers 0-3 copied into 4-7 one float short; later passes refine 4-7
-            float *dst = rgb0 + 4 * DIRSZ;
-            const int n4 = (int)(4 * DIRSZ);
-            for (int t = threadIdx.x; t < n4; t += BLOCK) dst[t] = (t == n4 - 1) ? 0.f : rgb0[t];
-            __syncthreads();
-            rgb = rgb0 + 4 * DIRSZ;
+    // green along four directions (:2075-2128), the value that lands in buffer j
+    auto green4 = [&](int j) {
+        for (int t = threadIdx.x; t < nr * KW; t += BLOCK) {
+            const int row = top + t / KW, k = t % KW;
+            const int ls = first_nongreen(A, row, left, mcol0);
+            const int coff0 = A.rshift[row % 3] == 1 ? 3 : 1 + (fcolx(A, row, ls + 1) & 1);
+            const int col = walk_col(ls, coff0, k);
+            if (col >= mcol0) continue;
+            const short *hx = coff0 == 3 ? A.hex[0][row % 3][ls % 3]
+                                         : A.hex[0][row % 3][((k & 1) ? ls + coff0 : ls) % 3];
+            const int c = j ^ (coff0 == 3 ? 0 : 1);
+            float color;
+            if (c == 0)
+                color = 0.6796875f * (rawhex(A, row, col, hx[1]) + rawhex(A, row, col, hx[0])) -
+                        0.1796875f * (rawhex(A, row, col, 2 * hx[1]) + rawhex(A, row, col, 2 * hx[0]));
+            else if (c == 1)
+                color = 0.87109375f * rawhex(A, row, col, hx[3]) + rawhex(A, row, col, hx[2]) * 0.12890625f +
+                        0.359375f * (rawhex(A, row, col, 0) - rawhex(A, row, col, -hx[2]));
+            else
+                color = 0.640625f * rawhex(A, row, col, hx[2 + c]) + 0.359375f * rawhex(A, row, col, -2 * hx[2 + c]) +
+                        0.12890625f * (2.f * rawhex(A, row, col, 0) - rawhex(A, row, col, 3 * hx[2 + c]) -
+                                       rawhex(A, row, col, -3 * hx[2 + c]));
+            const MM m = GM(row - top, (col - left) >> 1);
+            PX(row - top, col - left)[1] = limf(color, m.min, m.max);
         }
-        if (pass) {  // green from closer pixels (:2141-2182)
-            const int r0 = top + 2, r1 = mrow0 - 2;
-            for (int t = threadIdx.x; t < max(r1 - r0, 0) * TS; t += BLOCK) {
-                const int row = r0 + t / TS, k = t % TS;
-                const int ls = first_nongreen(A, row, left + 2, mcol0 - 2);
-                const int coff0 = A.rshift[row % 3] == 1 ? 3 : 1 + (fcolx(A, row, ls + 1) & 1);
-                const int col = walk_col(ls, coff0, k);
-                if (col >= mcol0 - 2) continue;
-                const int f = fcolx(A, row, col);
-                const short *hx = coff0 == 3 ? A.hex[1][row % 3][ls % 3]
-                                             : A.hex[1][row % 3][((k & 1) ? ls + coff0 : ls) % 3];
-                const MM m = GM(row - top, (col - left) >> 1);
-                for (int d = 3; d < 6; d++) {
-                    const int dd = coff0 == 3 ? d - 2 : (d - 2) ^ 1;
-                    float *rix = RGB(rgb, dd, row - top, col - left);
-                    const int o1 = 3 * hx[d], o2 = 3 * (-2 * hx[d]);
-                    const float val = 0.33333333f * (rix[o2 + 1] + 2 * (rix[o1 + 1] - rix[o1 + f]) - rix[o2 + f]) + rix[f];
-                    rix[1] = limf(val, m.min, m.max);
-                }
-            }
-            __syncthreads();
-        }
-        // R and B at solitary green sites (:2184-2222)
-        {
-            const int rstart = (top - A.sgrow + 4) / 3 * 3 + A.sgrow;
-            const int cstart = (left - A.sgcol + 4) / 3 * 3 + A.sgcol;
-            const int nrs = mrow0 - 2 > rstart ? (mrow0 - 2 - rstart + 2) / 3 : 0;
-            const int ncs = mcol0 - 2 > cstart ? (mcol0 - 2 - cstart + 2) / 3 : 0;
-            for (int t = threadIdx.x; t < nrs * ncs; t += BLOCK) {
-                const int row = rstart + 3 * (t / ncs), col = cstart + 3 * (t % ncs);
-                int h = fcolx(A, row, cstart + 1) ^ (((t % ncs) & 1) ? 2 : 0);
-                float *rix = RGB(rgb, 0, row - top, col - left);
-                float diff[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-                float color[3][6];
-                for (int i = 1, d = 0; d < 6; d++, i ^= TS ^ 1, h ^= 2) {
-                    for (int c = 0; c < 2; c++, h ^= 2) {
-                        const int o = i << c;
-                        const float g = rix[1] + rix[1] - rix[3 * o + 1] - rix[3 * -o + 1];
-                        color[h][d] = g + rix[3 * o + h] + rix[3 * -o + h];
-                        if (d > 1) diff[d] += sq(rix[3 * o + 1] - rix[3 * -o + 1] - rix[3 * o + h] + rix[3 * -o + h]) + sq(g);
-                    }
-                    if (d > 2 && (d & 1))
-                        if (diff[d - 1] < diff[d])
-                            for (int c = 0; c < 2; c++) color[c * 2][d] = color[c * 2][d - 1];
-                    if ((d & 1) || d < 2) {
-                        for (int c = 0; c < 2; c++) rix[c * 2] = 0.5f * color[c * 2][d];
-                        rix += DIRSZ;
-                    }
-                }
-            }
-        }
-        __syncthreads();
-        // R at B sites and B at R sites (:2224-2266)
-        {
-            const int r0 = top + 3, r1 = mrow0 - 3;
-            for (int t = threadIdx.x; t < max(r1 - r0, 0) * TS; t += BLOCK) {
-                const int row = r0 + t / TS, k = t % TS;
-                const int ls = first_nongreen(A, row, left + 3, mcol0 - 1);
-                int coff0 = A.rshift[row % 3] == 1 ? 3 : (fcolx(A, row, ls + 1) == 1 ? 2 : 1);
-                const int col = walk_col(ls, coff0, k);
-                if (col >= mcol0 - 3) continue;
-                const int c = ((row - A.sgrow) % 3) ? TS : 1;
-                const int h = 3 * (c ^ TS ^ 1);
-                const int f = 2 - fcolx(A, row, col);
-                float *rix = RGB(rgb, 0, row - top, col - left);
-                for (int d = 0; d < 4; d++, rix += DIRSZ) {
-                    const int i = d > 1 || ((d ^ c) & 1) ||
-                                          ((fabsf(rix[1] - rix[3 * c + 1]) + fabsf(rix[1] - rix[3 * -c + 1])) <
-                                           2.f * (fabsf(rix[1] - rix[3 * h + 1]) + fabsf(rix[1] - rix[3 * -h + 1])))
-                                      ? c : h;
-                    rix[f] = rix[1] + 0.5f * (rix[3 * i + f] + rix[3 * -i + f] - rix[3 * i + 1] - rix[3 * -i + 1]);
-                }
-            }
-        }
-        __syncthreads();
-        // R and B for 2x2 green blocks (:2268-2314)
-        {
-            int topstart = top + 2;
-            for (; topstart < mrow0 - 2; topstart++)
-                if ((topstart - A.sgrow) % 3) break;
-            int ls = left + 2;
-            for (; ls < mcol0 - 2; ls++)
-                if ((ls - A.sgcol) % 3) break;
-            const int coffstart = 2 - (fcolx(A, topstart, ls + 1) & 1);
-            const int r0 = topstart, r1 = mrow0 - 2;
-            for (int t = threadIdx.x; t < max(r1 - r0, 0) * TS; t += BLOCK) {
-                const int row = r0 + t / TS, k = t % TS;
-                if (!((row - A.sgrow) % 3)) continue;
-                const int col = walk_col(ls, coffstart, k);
-                if (col >= mcol0 - 2) continue;
-                const short *hx = A.hex[1][row % 3][((k & 1) ? ls + coffstart : ls) % 3];
-                float *rix = RGB(rgb, 0, row - top, col - left);
-                for (int d = 0; d < 8; d += 2, rix += DIRSZ) {
-                    if (hx[d] + hx[d + 1]) {
-                        const float g = 3 * rix[1] - 2 * rix[3 * hx[d] + 1] - rix[3 * hx[d + 1] + 1];
-                        for (int c = 0; c < 4; c += 2)
-                            rix[c] = (g + 2 * rix[3 * hx[d] + c] + rix[3 * hx[d + 1] + c]) * 0.33333333f;
-                    } else {
-                        const float g = 2 * rix[1] - rix[3 * hx[d] + 1] - rix[3 * hx[d + 1] + 1];
-                        for (int c = 0; c < 4; c += 2) rix[c] = (g + rix[3 * hx[d] + c] + rix[3 * hx[d + 1] + c]) * 0.5f;
-                    }
-                }
-            }
-        }
-        __syncthreads();
-    }
+    };
 
-    // YPbPr derivatives (:2346-2419), luma/chroma evaluated where needed
-    int mrow = mrow0 - top, mcol = mcol0 - left;
-    {
-        const int rows = max(mrow - 10, 0), cols = max(mcol - 10, 0);
-        for (int t = threadIdx.x; t < 8 * rows * cols; t += BLOCK) {
-            const int d = t / (rows * cols), u = t % (rows * cols);
-            const int row = 5 + u / cols, col = 5 + u % cols;
-            const int fo = (d & 3) == 0 ? 1 : (d & 3) == 1 ? TS : (d & 3) == 2 ? TS + 1 : TS - 1;  // :1878
+    // green from closer pixels (:2141-2182), passes 1 and 2: the d of 3..5
+    // whose buffer is j
+    auto pass_green = [&](int j) {
+        const int r0 = top + 2, r1 = mrow0 - 2;
+        for (int t = threadIdx.x; t < max(r1 - r0, 0) * KW; t += BLOCK) {
+            const int row = r0 + t / KW, k = t % KW;
+            const int ls = first_nongreen(A, row, left + 2, mcol0 - 2);
+            const int coff0 = A.rshift[row % 3] == 1 ? 3 : 1 + (fcolx(A, row, ls + 1) & 1);
+            const int d = coff0 == 3 ? j + 2 : (j ^ 1) + 2;
+            if (d < 3 || d > 5) continue;
+            const int col = walk_col(ls, coff0, k);
+            if (col >= mcol0 - 2) continue;
+            const int f = fcolx(A, row, col);
+            const short *hx = coff0 == 3 ? A.hex[1][row % 3][ls % 3]
+                                         : A.hex[1][row % 3][((k & 1) ? ls + coff0 : ls) % 3];
+            const MM m = GM(row - top, (col - left) >> 1);
+            float *rix = PX(row - top, col - left);
+            const int o1 = 3 * hx[d], o2 = 3 * (-2 * hx[d]);
+            const float val = 0.33333333f * (rix[o2 + 1] + 2 * (rix[o1 + 1] - rix[o1 + f]) - rix[o2 + f]) + rix[f];
+            rix[1] = limf(val, m.min, m.max);
+        }
+    };
+
+    // R and B at solitary green sites (:2184-2222).  The reference walks d = 0..5
+    // with one buffer pointer: buffer 0 takes d = 0, buffer 1 d = 1, buffer 2
+    // the better of d = 2, 3 and buffer 3 the better of d = 4, 5.
+    auto solitary = [&](int j) {
+        const int rstart = (top - A.sgrow + 4) / 3 * 3 + A.sgrow;
+        const int cstart = (left - A.sgcol + 4) / 3 * 3 + A.sgcol;
+        const int nrs = mrow0 - 2 > rstart ? (mrow0 - 2 - rstart + 2) / 3 : 0;
+        const int ncs = mcol0 - 2 > cstart ? (mcol0 - 2 - cstart + 2) / 3 : 0;
+        const int dlo = j < 2 ? j : 2 * j - 2, dhi = j < 2 ? j : 2 * j - 1;
+        for (int t = threadIdx.x; t < nrs * ncs; t += BLOCK) {
+            const int row = rstart + 3 * (t / ncs), col = cstart + 3 * (t % ncs);
+            const int h0 = fcolx(A, row, cstart + 1) ^ (((t % ncs) & 1) ? 2 : 0);
+            float *rix = PX(row - top, col - left);
+            float cr[2] = {0.f, 0.f}, cb[2] = {0.f, 0.f}, diff[2] = {0.f, 0.f};  // colours 0 and 2 of d = dlo + q
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                const int d = dlo + q;
+                if (d > dhi) break;
+                const int i = (d & 1) ? TS : 1;
+                int h = h0 ^ ((d & 1) ? 2 : 0);
+                float col[2];
+#pragma unroll
+                for (int c = 0; c < 2; c++, h ^= 2) {
+                    const int o = i << c;
+                    const float g = rix[1] + rix[1] - rix[3 * o + 1] - rix[3 * -o + 1];
+                    col[c] = g + rix[3 * o + h] + rix[3 * -o + h];
+                    if (d > 1) diff[q] += sq(rix[3 * o + 1] - rix[3 * -o + 1] - rix[3 * o + h] + rix[3 * -o + h]) + sq(g);
+                }
+                const bool h_is_0 = (h0 ^ ((d & 1) ? 2 : 0)) == 0;  // colour of the c = 0 step
+                cr[q] = h_is_0 ? col[0] : col[1];
+                cb[q] = h_is_0 ? col[1] : col[0];
+            }
+            // d = dhi, or d = dhi - 1 of a pair when its difference is smaller
+            const bool first = dhi > 2 && diff[0] < diff[1];
+            const int last = dhi - dlo;
+            rix[0] = 0.5f * (first ? cr[0] : (last ? cr[1] : cr[0]));
+            rix[2] = 0.5f * (first ? cb[0] : (last ? cb[1] : cb[0]));
+        }
+    };
+
+    // R at B sites and B at R sites (:2224-2266), buffer j of the pass
+    auto r_at_b = [&](int j) {
+        const int r0 = top + 3, r1 = mrow0 - 3;
+        for (int t = threadIdx.x; t < max(r1 - r0, 0) * KW; t += BLOCK) {
+            const int row = r0 + t / KW, k = t % KW;
+            const int ls = first_nongreen(A, row, left + 3, mcol0 - 1);
+            const int coff0 = A.rshift[row % 3] == 1 ? 3 : (fcolx(A, row, ls + 1) == 1 ? 2 : 1);
+            const int col = walk_col(ls, coff0, k);
+            if (col >= mcol0 - 3) continue;
+            const int c = ((row - A.sgrow) % 3) ? TS : 1;
+            const int h = 3 * (c ^ TS ^ 1);
+            const int f = 2 - fcolx(A, row, col);
+            float *rix = PX(row - top, col - left);
+            const int i = j > 1 || ((j ^ c) & 1) ||
+                                  ((fabsf(rix[1] - rix[3 * c + 1]) + fabsf(rix[1] - rix[3 * -c + 1])) <
+                                   2.f * (fabsf(rix[1] - rix[3 * h + 1]) + fabsf(rix[1] - rix[3 * -h + 1])))
+                              ? c : h;
+            rix[f] = rix[1] + 0.5f * (rix[3 * i + f] + rix[3 * -i + f] - rix[3 * i + 1] - rix[3 * -i + 1]);
+        }
+    };
+
+    // R and B for 2x2 green blocks (:2268-2314), buffer j of the pass (d = 2j)
+    auto two_by_two = [&](int j) {
+        const int d = 2 * j;
+        for (int t = threadIdx.x; t < nrow22 * KW; t += BLOCK) {
+            const int q = t / KW, k = t % KW;
+            const int row = topstart + (3 * q + adj22) / 2;
+            const int col = walk_col(ls22, coff22, k);
+            if (col >= mcol0 - 2) continue;
+            const short *hx = A.hex[1][row % 3][((k & 1) ? ls22 + coff22 : ls22) % 3];
+            float *rix = PX(row - top, col - left);
+            if (hx[d] + hx[d + 1]) {
+                const float g = 3 * rix[1] - 2 * rix[3 * hx[d] + 1] - rix[3 * hx[d + 1] + 1];
+                for (int c = 0; c < 4; c += 2)
+                    rix[c] = (g + 2 * rix[3 * hx[d] + c] + rix[3 * hx[d + 1] + c]) * 0.33333333f;
+            } else {
+                const float g = 2 * rix[1] - rix[3 * hx[d] + 1] - rix[3 * hx[d + 1] + 1];
+                for (int c = 0; c < 4; c += 2) rix[c] = (g + rix[3 * hx[d] + c] + rix[3 * hx[d + 1] + c]) * 0.5f;
+            }
+        }
+    };
+
+    // YPbPr derivatives (:2346-2419) of buffer d, luma/chroma evaluated where needed
+    const int mrow_d = mrow0 - top, mcol_d = mcol0 - left;
+    auto derivatives = [&](int d) {
+        const int rows = max(mrow_d - 10, 0), cols = max(mcol_d - 10, 0);
+        const int fo = (d & 3) == 0 ? 1 : (d & 3) == 1 ? TS : (d & 3) == 2 ? TS + 1 : TS - 1;  // :1878
+        // f offsets in the reference index a (TS-8)-wide plane; here rows are TS wide
+        const int dr = fo == 1 ? 0 : 1, dc = fo == 1 ? 1 : fo - TS;
+        for (int t = threadIdx.x; t < rows * cols; t += BLOCK) {
+            const int row = 5 + t / cols, col = 5 + t % cols;
             auto ypp = [&](int r, int c, float &y, float &pb, float &pr) {
-                const float *p = RGB(rgb0, d, r, c);
+                const float *p = PX(r, c);
                 y = 0.2627f * p[0] + 0.6780f * p[1] + 0.0593f * p[2];
                 pb = (p[2] - y) * 0.56433f;
                 pr = (p[0] - y) * 0.67815f;
             };
-            // f offsets in the reference index a (TS-8)-wide plane; here rows are TS wide
-            const int dr = fo == 1 ? 0 : 1, dc = fo == 1 ? 1 : fo - TS;
             float y0, u0, v0, y1, u1, v1, y2, u2, v2;
             ypp(row, col, y0, u0, v0);
             ypp(row + dr, col + dc, y1, u1, v1);
@@ -304,8 +307,56 @@ __device__ void xtrans_tile(const XArgs &A, char *ar, int top, int left) {
             drv[(size_t)d * DW * DW + (size_t)(row - 5) * DW + col - 5] =
                 sq(2 * y0 - y1 - y2) + sq(2 * u0 - u1 - u2) + sq(2 * v0 - v1 - v2);
         }
+    };
+
+    // LDS buffer -> arena buffer d
+    auto store_buffer = [&](int d) {
+        const float4 *src = reinterpret_cast<const float4 *>(lds);
+        float4 *dst = reinterpret_cast<float4 *>(RGB(rgb0, d, 0, 0));
+        for (int t = threadIdx.x; t < (int)(DIRSZ / 4); t += BLOCK) dst[t] = src[t];
+    };
+
+    for (int j = 0; j < 4; j++) {
+        // buffer j: the raw sample at its colour, zero elsewhere; the reference
+        // copies buffer 0 into 1-3 one float short, so their last float is 0
+        for (int t = threadIdx.x; t < TS * TS; t += BLOCK) {
+            const int r = t / TS, c = t % TS;
+            float v[3] = {0.f, 0.f, 0.f};
+            if (r < nr && c < nc) v[fcolx(A, r + top, c + left)] = rawv(A, r + top, c + left);
+            float *p = lds + 3 * t;
+            p[0] = v[0];
+            p[1] = v[1];
+            p[2] = (j > 0 && t == TS * TS - 1) ? 0.f : v[2];
+        }
+        __syncthreads();
+        green4(j);
+        __syncthreads();
+        for (int pass = 0; pass < 3; pass++) {
+            if (pass == 1) {
+                // buffers 0-3 are copied into 4-7 one float short: buffer 7's last float is 0
+                store_buffer(j);
+                derivatives(j);
+                __syncthreads();
+                if (j == 3 && threadIdx.x == 0) lds[DIRSZ - 1] = 0.f;
+                __syncthreads();
+            }
+            if (pass) {
+                pass_green(j);
+                __syncthreads();
+            }
+            solitary(j);
+            __syncthreads();
+            r_at_b(j);
+            __syncthreads();
+            two_by_two(j);
+            __syncthreads();
+        }
+        store_buffer(j + 4);
+        derivatives(j + 4);
+        __syncthreads();
     }
-    __syncthreads();
+
+    int mrow = mrow_d, mcol = mcol_d;
 #define DRV(d, r, c) drv[(size_t)(d) * DW * DW + (size_t)(r) * DW + (c)]
 #define HOMO(d, r, c) homo[((size_t)(d) * TS + (r)) * TS + (c)]
 #define HSUM(d, r, c) hsum[((size_t)(d) * TS + (r)) * TS + (c)]
@@ -377,6 +428,7 @@ __device__ void xtrans_tile(const XArgs &A, char *ar, int top, int left) {
 #undef HOMO
 #undef HSUM
 #undef RGB
+#undef PX
 #undef GM
 }
 
@@ -493,6 +545,19 @@ void setup(XArgs &A, const signed char *xt36) {
 
 }  // namespace
 
+// raise the dynamic LDS limit of both tile kernels once per process
+static hipError_t xtrans_lds_attr() {
+    static const hipError_t rc = [] {
+        hipError_t e = hipFuncSetAttribute((const void *)xtrans_kernel,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_BYTES);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute((const void *)xtrans_multi_kernel,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_BYTES);
+        return e;
+    }();
+    return rc;
+}
+
 static int xtrans_slots(lre_ctx *ctx, int w, int h) {
     const int ntiles = tiles_for(w) * tiles_for(h);
     const int cap = (ctx->num_cus > 0 ? ctx->num_cus : 256) * 4;
@@ -522,7 +587,8 @@ int lre_launch_xtrans3(lre_ctx *ctx, const float *d_cfa, float *d_rgb, int w, in
     A.tile_base = 0;
     A.arenas = static_cast<char *>(d_scratch);
     if (A.ntiles > 0) {
-        hipLaunchKernelGGL(xtrans_kernel, dim3(xtrans_slots(ctx, w, h)), dim3(BLOCK), 0, s, A);
+        LRE_HIP_CHECK(xtrans_lds_attr());
+        hipLaunchKernelGGL(xtrans_kernel, dim3(xtrans_slots(ctx, w, h)), dim3(BLOCK), LDS_BYTES, s, A);
         LRE_HIP_CHECK(hipGetLastError());
     }
     const size_t frame = (size_t)w * h;
@@ -575,7 +641,8 @@ int lre_launch_xtrans_multi(const lre_demosaic_job *jobs, int n, const void *h_t
             hipLaunchKernelGGL(xtrans_set_maxv, dim3(1), dim3(64), 0, s, dtab + k, jobs[k].d_maxv);
     if (total_tiles > 0) {
         const int grid = slots < total_tiles ? slots : total_tiles;
-        hipLaunchKernelGGL(xtrans_multi_kernel, dim3(grid), dim3(BLOCK), 0, s,
+        LRE_HIP_CHECK(xtrans_lds_attr());
+        hipLaunchKernelGGL(xtrans_multi_kernel, dim3(grid), dim3(BLOCK), LDS_BYTES, s,
                            static_cast<const XArgs *>(dtab), n, total_tiles,
                            static_cast<char *>(d_scratch));
         LRE_HIP_CHECK(hipGetLastError());
