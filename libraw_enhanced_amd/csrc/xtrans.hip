@@ -3,7 +3,7 @@
 //
 // Each workgroup processes one reference tile (114 x 114, step 98, first tile
 // at (3,3)) in a private arena holding the 8 direction buffers, the derivative
-// and homogeneity maps and the green bounds.  Every step of the reference is a
+// maps and the green bounds; the homogeneity maps live in LDS.  Every step of the reference is a
 // loop over one class of CFA sites whose iterations neither read nor write each
 // other's cells (checked on the X-Trans pattern), so each step runs as one
 // parallel pass over its sites with a barrier after it.  Site positions along a
@@ -36,10 +36,7 @@ constexpr size_t DIRSZ = (size_t)TS * TS * 3;  // floats per direction buffer
 constexpr size_t A_RGB = 0;
 constexpr size_t A_DRV = A_RGB + 8 * DIRSZ * 4;
 constexpr size_t A_GM = A_DRV + (size_t)8 * DW * DW * 4;
-constexpr size_t A_HOMO = A_GM + (size_t)TS * (TS / 2) * 8;
-constexpr size_t A_HSUM = A_HOMO + (size_t)8 * TS * TS;
-constexpr size_t A_HMAX = A_HSUM + (size_t)8 * TS * TS;
-constexpr size_t ARENA = ((A_HMAX + (size_t)TS * TS) + 255) & ~(size_t)255;
+constexpr size_t ARENA = ((A_GM + (size_t)TS * (TS / 2) * 8) + 255) & ~(size_t)255;
 
 struct XArgs {
     const float *cfa;  // CFA plane (one sample per site)
@@ -85,28 +82,47 @@ LRE_DEV int walk_col(int ls, int coff0, int k) {
     return ls + 3 * (k >> 1) + ((k & 1) ? coff0 : 0);
 }
 
-// LDS: one direction buffer, [row][col][rgb] like a buffer of the arena
-constexpr size_t LDS_BYTES = DIRSZ * sizeof(float);
+// LDS: one direction buffer, [row][col][rgb] like a buffer of the arena, then
+// the tile's tables: the hexagon offsets of both strides (2 x 9 x 8 shorts),
+// the 6x6 pattern, and one column walk per row for three of the steps
+constexpr size_t LDS_BUF = DIRSZ * sizeof(float);
+constexpr size_t LDS_HEX = LDS_BUF, LDS_XT = LDS_HEX + 2 * 9 * 8 * sizeof(short);
+constexpr size_t LDS_WALK = LDS_XT + 48;
+constexpr size_t LDS_BYTES = LDS_WALK + 3 * TS * sizeof(uint2);
+static_assert(LDS_BUF % 16 == 0 && LDS_WALK % 8 == 0, "table alignment");
+
+// One tile row's column walk, packed: first site ls (bits 0-15), stride
+// pattern coff (16-19: 3, or alternating coff and 3 - coff), hexagon table
+// index (row % 3) * 3 + col % 3 of even and odd steps (20-23, 24-27); and the
+// colour of step k at byte k & 3 of the second word (the colours repeat with
+// period 4 along a walk).
+LRE_DEV uint2 make_walk(const XArgs &A, int row, int c0, int lim) {
+    const int ls = first_nongreen(A, row, c0, lim);
+    const int coff = A.rshift[row % 3] == 1 ? 3 : 1 + (fcolx(A, row, ls + 1) & 1);
+    const int hx0 = (row % 3) * 3 + ls % 3, hx1 = (row % 3) * 3 + (ls + (coff == 3 ? 0 : coff)) % 3;
+    uint2 w;
+    w.x = (unsigned)ls | ((unsigned)coff << 16) | ((unsigned)hx0 << 20) | ((unsigned)hx1 << 24);
+    w.y = 0;
+    for (int q = 0; q < 4; q++) w.y |= (unsigned)fcolx(A, row, walk_col(ls, coff, q)) << (8 * q);
+    return w;
+}
 // bound on the number of sites a column walk visits in one tile row
 constexpr int KW = (2 * TS) / 3 + 2;
 
 __device__ void xtrans_tile(const XArgs &A, char *ar, int top, int left) {
     const int W = A.W, H = A.H;
     extern __shared__ float lds[];
+    short *hexl = reinterpret_cast<short *>(reinterpret_cast<char *>(lds) + LDS_HEX);
+    signed char *xtl = reinterpret_cast<signed char *>(lds) + LDS_XT;
+    uint2 *walks = reinterpret_cast<uint2 *>(reinterpret_cast<char *>(lds) + LDS_WALK);
     float *rgb0 = reinterpret_cast<float *>(ar + A_RGB);
     float *drv = reinterpret_cast<float *>(ar + A_DRV);
     MM *gm = reinterpret_cast<MM *>(ar + A_GM);
-    unsigned char *homo = reinterpret_cast<unsigned char *>(ar + A_HOMO);
-    unsigned char *hsum = reinterpret_cast<unsigned char *>(ar + A_HSUM);
-    unsigned char *hmax = reinterpret_cast<unsigned char *>(ar + A_HMAX);
     const int mrow0 = min(top + TS, H - 3), mcol0 = min(left + TS, W - 3);
     const int nr = mrow0 - top, nc = mcol0 - left;
 #define RGB(base, d, r, c) ((base) + ((size_t)(d) * DIRSZ + ((size_t)(r) * TS + (c)) * 3))
 #define PX(r, c) (lds + ((r) * TS + (c)) * 3)
 #define GM(r, c) gm[(size_t)(r) * (TS / 2) + (c)]
-
-    // clear the homogeneity map (rows the tile leaves unwritten read as zero)
-    for (int t = threadIdx.x; t < 8 * TS * TS / 4; t += BLOCK) reinterpret_cast<unsigned *>(homo)[t] = 0u;
 
     // green min/max (:1973-2059): one thread per row, the reference's walk
     for (int row = top + (int)threadIdx.x; row < mrow0; row += BLOCK) {
@@ -139,7 +155,24 @@ __device__ void xtrans_tile(const XArgs &A, char *ar, int top, int left) {
             if (col < mcol0) GM(row - top, (col - left) >> 1) = mmhex(col, hx);
         }
     }
+    // tables: walks of green4 (from left), the pass green step (from left + 2)
+    // and the R-at-B step (from left + 3)
+    if (threadIdx.x < 2 * 9 * 8) hexl[threadIdx.x] = (&A.hex[0][0][0][0])[threadIdx.x];
+    if (threadIdx.x < 36) xtl[threadIdx.x] = A.xt[threadIdx.x / 6][threadIdx.x % 6];
+    for (int t = threadIdx.x; t < 3 * TS; t += BLOCK) {
+        const int step = t / TS, row = top + t % TS;
+        if (row >= mrow0) continue;
+        walks[t] = step == 0 ? make_walk(A, row, left, mcol0)
+                 : step == 1 ? make_walk(A, row, left + 2, mcol0 - 2)
+                             : make_walk(A, row, left + 3, mcol0 - 1);
+    }
     __syncthreads();
+    auto walk_ls = [](uint2 w) { return (int)(w.x & 0xffffu); };
+    auto walk_coff = [](uint2 w) { return (int)((w.x >> 16) & 15u); };
+    auto walk_hex = [&](int set, uint2 w, int k) {
+        return hexl + (set * 9 + (int)((w.x >> ((k & 1) ? 24 : 20)) & 15u)) * 8;
+    };
+    auto walk_colour = [](uint2 w, int k) { return (int)((w.y >> (8 * (k & 3))) & 255u); };
 
     // rows of the R/B steps for 2x2 green blocks: those with (row - sgrow) % 3 != 0
     int topstart = top + 2;
@@ -157,12 +190,11 @@ __device__ void xtrans_tile(const XArgs &A, char *ar, int top, int left) {
     auto green4 = [&](int j) {
         for (int t = threadIdx.x; t < nr * KW; t += BLOCK) {
             const int row = top + t / KW, k = t % KW;
-            const int ls = first_nongreen(A, row, left, mcol0);
-            const int coff0 = A.rshift[row % 3] == 1 ? 3 : 1 + (fcolx(A, row, ls + 1) & 1);
-            const int col = walk_col(ls, coff0, k);
+            const uint2 w = walks[row - top];
+            const int coff0 = walk_coff(w);
+            const int col = walk_col(walk_ls(w), coff0, k);
             if (col >= mcol0) continue;
-            const short *hx = coff0 == 3 ? A.hex[0][row % 3][ls % 3]
-                                         : A.hex[0][row % 3][((k & 1) ? ls + coff0 : ls) % 3];
+            const short *hx = walk_hex(0, w, k);
             const int c = j ^ (coff0 == 3 ? 0 : 1);
             float color;
             if (c == 0)
@@ -186,15 +218,14 @@ __device__ void xtrans_tile(const XArgs &A, char *ar, int top, int left) {
         const int r0 = top + 2, r1 = mrow0 - 2;
         for (int t = threadIdx.x; t < max(r1 - r0, 0) * KW; t += BLOCK) {
             const int row = r0 + t / KW, k = t % KW;
-            const int ls = first_nongreen(A, row, left + 2, mcol0 - 2);
-            const int coff0 = A.rshift[row % 3] == 1 ? 3 : 1 + (fcolx(A, row, ls + 1) & 1);
+            const uint2 w = walks[TS + row - top];
+            const int coff0 = walk_coff(w);
             const int d = coff0 == 3 ? j + 2 : (j ^ 1) + 2;
             if (d < 3 || d > 5) continue;
-            const int col = walk_col(ls, coff0, k);
+            const int col = walk_col(walk_ls(w), coff0, k);
             if (col >= mcol0 - 2) continue;
-            const int f = fcolx(A, row, col);
-            const short *hx = coff0 == 3 ? A.hex[1][row % 3][ls % 3]
-                                         : A.hex[1][row % 3][((k & 1) ? ls + coff0 : ls) % 3];
+            const int f = walk_colour(w, k);
+            const short *hx = walk_hex(1, w, k);
             const MM m = GM(row - top, (col - left) >> 1);
             float *rix = PX(row - top, col - left);
             const int o1 = 3 * hx[d], o2 = 3 * (-2 * hx[d]);
@@ -248,13 +279,12 @@ __device__ void xtrans_tile(const XArgs &A, char *ar, int top, int left) {
         const int r0 = top + 3, r1 = mrow0 - 3;
         for (int t = threadIdx.x; t < max(r1 - r0, 0) * KW; t += BLOCK) {
             const int row = r0 + t / KW, k = t % KW;
-            const int ls = first_nongreen(A, row, left + 3, mcol0 - 1);
-            const int coff0 = A.rshift[row % 3] == 1 ? 3 : (fcolx(A, row, ls + 1) == 1 ? 2 : 1);
-            const int col = walk_col(ls, coff0, k);
+            const uint2 w = walks[2 * TS + row - top];
+            const int col = walk_col(walk_ls(w), walk_coff(w), k);
             if (col >= mcol0 - 3) continue;
             const int c = ((row - A.sgrow) % 3) ? TS : 1;
             const int h = 3 * (c ^ TS ^ 1);
-            const int f = 2 - fcolx(A, row, col);
+            const int f = 2 - walk_colour(w, k);
             float *rix = PX(row - top, col - left);
             const int i = j > 1 || ((j ^ c) & 1) ||
                                   ((fabsf(rix[1] - rix[3 * c + 1]) + fabsf(rix[1] - rix[3 * -c + 1])) <
@@ -272,7 +302,7 @@ __device__ void xtrans_tile(const XArgs &A, char *ar, int top, int left) {
             const int row = topstart + (3 * q + adj22) / 2;
             const int col = walk_col(ls22, coff22, k);
             if (col >= mcol0 - 2) continue;
-            const short *hx = A.hex[1][row % 3][((k & 1) ? ls22 + coff22 : ls22) % 3];
+            const short *hx = hexl + (9 + (row % 3) * 3 + ((k & 1) ? ls22 + coff22 : ls22) % 3) * 8;
             float *rix = PX(row - top, col - left);
             if (hx[d] + hx[d + 1]) {
                 const float g = 3 * rix[1] - 2 * rix[3 * hx[d] + 1] - rix[3 * hx[d + 1] + 1];
@@ -322,7 +352,7 @@ __device__ void xtrans_tile(const XArgs &A, char *ar, int top, int left) {
         for (int t = threadIdx.x; t < TS * TS; t += BLOCK) {
             const int r = t / TS, c = t % TS;
             float v[3] = {0.f, 0.f, 0.f};
-            if (r < nr && c < nc) v[fcolx(A, r + top, c + left)] = rawv(A, r + top, c + left);
+            if (r < nr && c < nc) v[xtl[((r + top) % 6) * 6 + (c + left) % 6]] = rawv(A, r + top, c + left);
             float *p = lds + 3 * t;
             p[0] = v[0];
             p[1] = v[1];
@@ -358,20 +388,47 @@ __device__ void xtrans_tile(const XArgs &A, char *ar, int top, int left) {
 
     int mrow = mrow_d, mcol = mcol_d;
 #define DRV(d, r, c) drv[(size_t)(d) * DW * DW + (size_t)(r) * DW + (c)]
-#define HOMO(d, r, c) homo[((size_t)(d) * TS + (r)) * TS + (c)]
-#define HSUM(d, r, c) hsum[((size_t)(d) * TS + (r)) * TS + (c)]
+    // homogeneity maps in LDS: directions 0-3 and 4-7 as the bytes of one word
+    // per pixel (a 5x5 sum of counts up to 9 stays below 256 in every byte);
+    // cleared per tile, so rows the tile does not write read as zero
+    unsigned *hw = reinterpret_cast<unsigned *>(lds);
+    unsigned char *hb = reinterpret_cast<unsigned char *>(lds);
+    static_assert(2 * TS * TS * sizeof(unsigned) <= LDS_BUF, "homogeneity maps fit the LDS buffer");
+    for (int t = threadIdx.x; t < 2 * TS * TS; t += BLOCK) hw[t] = 0u;
+    __syncthreads();
     {
+        // four rows per thread share the 3x3 windows of each direction
         const int rows = max(mrow - 12, 0), cols = max(mcol - 12, 0);
-        for (int t = threadIdx.x; t < rows * cols; t += BLOCK) {
-            const int row = 6 + t / cols, col = 6 + t % cols;
-            float tr = DRV(0, row - 5, col - 5) < DRV(1, row - 5, col - 5) ? DRV(0, row - 5, col - 5) : DRV(1, row - 5, col - 5);
-            for (int d = 2; d < 8; d++) tr = DRV(d, row - 5, col - 5) < tr ? DRV(d, row - 5, col - 5) : tr;
-            tr *= 8;
+        const int ng = (rows + 3) / 4;
+        for (int t = threadIdx.x; t < ng * cols; t += BLOCK) {
+            const int row0 = 6 + 4 * (t / cols), col = 6 + t % cols;
+            const int nq = min(4, 6 + rows - row0);
+            float tr[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int r = row0 + min(q, nq - 1) - 5;
+                float m = DRV(0, r, col - 5) < DRV(1, r, col - 5) ? DRV(0, r, col - 5) : DRV(1, r, col - 5);
+                for (int d = 2; d < 8; d++) m = DRV(d, r, col - 5) < m ? DRV(d, r, col - 5) : m;
+                tr[q] = m * 8;
+            }
             for (int d = 0; d < 8; d++) {
-                unsigned char s = 0;
-                for (int v = -1; v <= 1; v++)
-                    for (int h = -1; h <= 1; h++) s += (DRV(d, row + v - 5, col + h - 5) <= tr ? 1 : 0);
-                HOMO(d, row, col) = s;
+                float w[6][3];  // rows row0-1 .. row0+4, columns col-1 .. col+1
+#pragma unroll
+                for (int a = 0; a < 6; a++) {
+                    const int r = min(row0 - 6 + a, DW - 1);  // rows past the last group are not used
+#pragma unroll
+                    for (int b = 0; b < 3; b++) w[a][b] = DRV(d, r, col - 6 + b);
+                }
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    if (q >= nq) break;
+                    unsigned s = 0;
+#pragma unroll
+                    for (int a = q; a < q + 3; a++)
+#pragma unroll
+                        for (int b = 0; b < 3; b++) s += w[a][b] <= tr[q] ? 1u : 0u;
+                    hb[(((d >> 2) * TS + row0 + q) * TS + col) * 4 + (d & 3)] = (unsigned char)s;
+                }
             }
         }
     }
@@ -380,36 +437,36 @@ __device__ void xtrans_tile(const XArgs &A, char *ar, int top, int left) {
     if (W - left < TS + 4) mcol = W - left + 2;
     const int rs = top < 8 ? top : 8, cs = left < 8 ? left : 8;
     {
+        // 5x5 sums of the maps (exact integers: the running form of the
+        // reference gives the same), their maximum less an eighth, and the
+        // average of the most homogeneous directions, per pixel
         const int rows = max(mrow - 8 - rs, 0), cols = max(mcol - 8 - cs, 0);
         for (int t = threadIdx.x; t < rows * cols; t += BLOCK) {
             const int row = rs + t / cols, col = cs + t % cols;
-            unsigned char m = 0;
-            for (int d = 0; d < 8; d++) {
-                int s = 0;  // exact integer sums: the running form of the reference gives the same
-                for (int v = -2; v <= 2; v++)
-                    for (int h = -2; h <= 2; h++) s += HOMO(d, row + v, col + h);
-                const unsigned char us = (unsigned char)s;
-                HSUM(d, row, col) = us;
-                m = d == 0 ? us : (m < us ? us : m);
-            }
-            m -= m >> 3;
-            hmax[(size_t)row * TS + col] = m;
-        }
-    }
-    __syncthreads();
-    {
-        const int rows = max(mrow - 8 - rs, 0), cols = max(mcol - 8 - cs, 0);
-        for (int t = threadIdx.x; t < rows * cols; t += BLOCK) {
-            const int row = rs + t / cols, col = cs + t % cols;
+            unsigned s0 = 0, s1 = 0;
+            for (int v = -2; v <= 2; v++)
+#pragma unroll
+                for (int h = -2; h <= 2; h++) {
+                    const int k = (row + v) * TS + col + h;
+                    s0 += hw[k];
+                    s1 += hw[TS * TS + k];
+                }
             unsigned char hm[8];
-            for (int d = 0; d < 4; d++) hm[d] = HSUM(d, row, col);
+#pragma unroll
+            for (int d = 0; d < 4; d++) {
+                hm[d] = (unsigned char)(s0 >> (8 * d));
+                hm[d + 4] = (unsigned char)(s1 >> (8 * d));
+            }
+            unsigned char mx = hm[0];
+#pragma unroll
+            for (int d = 1; d < 8; d++) mx = mx < hm[d] ? hm[d] : mx;
+            mx -= mx >> 3;
+#pragma unroll
             for (int d = 4; d < 8; d++) {
-                hm[d] = HSUM(d, row, col);
                 if (hm[d - 4] < hm[d]) hm[d - 4] = 0;
                 else if (hm[d - 4] > hm[d]) hm[d] = 0;
             }
             float avg[4] = {0.f, 0.f, 0.f, 0.f};
-            const unsigned char mx = hmax[(size_t)row * TS + col];
             for (int d = 0; d < 8; d++)
                 if (hm[d] >= mx) {
                     const float *p = RGB(rgb0, d, row, col);
@@ -425,8 +482,6 @@ __device__ void xtrans_tile(const XArgs &A, char *ar, int top, int left) {
     }
     __syncthreads();
 #undef DRV
-#undef HOMO
-#undef HSUM
 #undef RGB
 #undef PX
 #undef GM
