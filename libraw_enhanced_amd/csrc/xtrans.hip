@@ -124,8 +124,14 @@ __device__ void xtrans_tile(const XArgs &A, char *ar, int top, int left) {
 #define PX(r, c) (lds + ((r) * TS + (c)) * 3)
 #define GM(r, c) gm[(size_t)(r) * (TS / 2) + (c)]
 
-    // green min/max (:1973-2059): one thread per row, the reference's walk
-    for (int row = top + (int)threadIdx.x; row < mrow0; row += BLOCK) {
+    // green min/max (:1973-2059).  The reference walks each row: on rows with
+    // one non-green site per three columns every site fills its own slot
+    // (slots are column pairs); on the other rows a first lone site may come,
+    // then each pair of adjacent sites fills both its slots with the bounds of
+    // its first site, and a last lone site may follow.  No two units of a
+    // row share a slot with different values, so the units run in parallel.
+    for (int t = threadIdx.x; t < nr * KW; t += BLOCK) {
+        const int row = top + t / KW, u = t % KW;
         const int ls = first_nongreen(A, row, left, mcol0);
         const int coff = A.rshift[row % 3] == 1 ? 3 : 1 + (fcolx(A, row, ls + 1) & 1);
         auto mmhex = [&](int col, const short *hx) {
@@ -138,22 +144,24 @@ __device__ void xtrans_tile(const XArgs &A, char *ar, int top, int left) {
             return MM{lo, hi};
         };
         if (coff == 3) {
-            const short *hx = A.hex[0][row % 3][ls % 3];
-            for (int col = ls; col < mcol0; col += 3) GM(row - top, (col - left) >> 1) = mmhex(col, hx);
-        } else {
-            int col = ls;
-            if (coff == 2) {
-                GM(row - top, (col - left) >> 1) = mmhex(col, A.hex[0][row % 3][col % 3]);
-                col += 2;
-            }
-            const short *hx = A.hex[0][row % 3][col % 3];
-            for (; col < mcol0 - 1; col += 3) {
-                const MM m = mmhex(col, hx);
-                GM(row - top, (col - left) >> 1) = m;
-                GM(row - top, (col + 1 - left) >> 1) = m;
-            }
-            if (col < mcol0) GM(row - top, (col - left) >> 1) = mmhex(col, hx);
+            const int col = ls + 3 * u;
+            if (col < mcol0) GM(row - top, (col - left) >> 1) = mmhex(col, A.hex[0][row % 3][ls % 3]);
+            continue;
         }
+        int col;
+        if (coff == 2) {
+            if (u == 0) {
+                if (ls < mcol0) GM(row - top, (ls - left) >> 1) = mmhex(ls, A.hex[0][row % 3][ls % 3]);
+                continue;
+            }
+            col = ls + 2 + 3 * (u - 1);
+        } else {
+            col = ls + 3 * u;
+        }
+        if (col >= mcol0) continue;
+        const MM m = mmhex(col, A.hex[0][row % 3][col % 3]);
+        GM(row - top, (col - left) >> 1) = m;
+        if (col < mcol0 - 1) GM(row - top, (col + 1 - left) >> 1) = m;
     }
     // tables: walks of green4 (from left), the pass green step (from left + 2)
     // and the R-at-B step (from left + 3)
