@@ -822,25 +822,10 @@ __global__ __launch_bounds__(BLOCK) void amaze_multi_kernel(const Args *__restri
 // of each non-native colour.  Reads only native channels, writes only
 // non-native ones, so it runs in place.
 __global__ void border_kernel(float *img, int w, int h, uint32_t filters, int border) {
-    const int nframe_rows = min(border, h);
-    // enumerate frame pixels: full rows at top/bottom, then the left/right strips
-    const int top_n = nframe_rows * w;
-    const int bot_rows = max(0, min(border, h - border));
-    const int bot_n = bot_rows * w;
-    const int mid_rows = max(0, h - 2 * border);
-    const int strip = min(border, w);
-    const int side_n = mid_rows * 2 * strip;
-    const int total = top_n + bot_n + side_n;
+    const int total = lre::frame_pixels(w, h, border);
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
         int row, col;
-        if (t < top_n) { row = t / w; col = t % w; }
-        else if (t < top_n + bot_n) { int u = t - top_n; row = h - bot_rows + u / w; col = u % w; }
-        else {
-            int u = t - top_n - bot_n;
-            row = border + u / (2 * strip);
-            int s = u % (2 * strip);
-            col = s < strip ? s : w - strip + (s - strip);
-        }
+        lre::frame_pixel(t, w, h, border, row, col);
         if (col >= border && col < w - border && row >= border && row < h - border) continue;
         float sum[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         for (int y = row > 0 ? row - 1 : 0; y < row + 2 && y < h; y++)

@@ -49,7 +49,8 @@ __global__ __launch_bounds__(BLOCK) void axial_max_g(const float *__restrict__ s
     }
 }
 
-constexpr int AX_TX = 64, AX_TY = 16, AX_RMAX = 16;
+constexpr int AX_TX = 64, AX_TY = 32, AX_RMAX = 16;
+constexpr int AX_BLOCK = 512;  // one 4-row strip of one column per thread
 
 struct AxialArgs {
     const float *src;
@@ -60,30 +61,32 @@ struct AxialArgs {
     const unsigned *stats;
 };
 
-// Horizontal FP64 window sums of an LDS float region (rows ry in [0,RY), pitch
-// RXP, region column 0 = image column x0 - r - 1) into hs[RY][AX_HP], for output
-// columns x0 .. x0+AX_TX-1; region values outside the image are 0.  Each thread
-// sums 4 neighbouring outputs of one row with a sliding window; lanes walk down
-// rows (odd pitches: distinct LDS banks).  Every output adds its taps in order.
+// Horizontal FP64 window sums over LDS rows [0, RY) into hs[RY][AX_HP], for
+// output columns x0 .. x0+AX_TX-1; region column 0 is image column x0 - r - 1
+// and region values outside the image are 0.  q(ry, c) gives the summed value
+// at region column c.  Each thread sums 4 neighbouring outputs of one row with a
+// sliding window; lanes walk down rows (odd pitches: distinct LDS banks).  Every
+// output adds its taps in order.
 constexpr int AX_HP = 64 + 1;  // hs pitch in doubles
 constexpr int AX_RB = 4;
 
-LRE_DEV void hsum(const float *reg, int RXP, int RY, int r, double *hs) {
+template <class Q>
+LRE_DEV void hsum(Q q, int RY, int r, double *hs) {
     const int nt = 2 * r + 1;
-    for (int t = threadIdx.x; t < RY * (AX_TX / AX_RB); t += BLOCK) {
+    for (int t = threadIdx.x; t < RY * (AX_TX / AX_RB); t += AX_BLOCK) {
         const int ry = t % RY, cx = (t / RY) * AX_RB;
-        const float *row = reg + ry * RXP + cx + 1;  // region col cx+1 = image col x0+cx-r
+        const int c0 = cx + 1;  // region col cx+1 = image col x0+cx-r
         double s[AX_RB];
         float w[AX_RB];
 #pragma unroll
-        for (int u = 0; u < AX_RB; u++) { s[u] = 0.0; w[u] = row[u]; }
+        for (int u = 0; u < AX_RB; u++) { s[u] = 0.0; w[u] = q(ry, c0 + u); }
         for (int k = 0; k < nt; k++) {
 #pragma unroll
             for (int u = 0; u < AX_RB; u++) s[u] += (double)w[u];
             if (k + 1 == nt) break;
 #pragma unroll
             for (int u = 0; u < AX_RB - 1; u++) w[u] = w[u + 1];
-            w[AX_RB - 1] = row[k + AX_RB];
+            w[AX_RB - 1] = q(ry, c0 + k + AX_RB);
         }
 #pragma unroll
         for (int u = 0; u < AX_RB; u++) hs[ry * AX_HP + cx + u] = s[u];
@@ -118,8 +121,15 @@ LRE_DEV void vmean(const double *hs, int r, int cx, int oy, int x0, int y0, int 
     }
 }
 
+static_assert(AX_TX * AX_TY == AX_BLOCK * AX_RB, "one 4-row strip per thread");
+
+size_t axial_ab_lds(int r) {
+    const int RX = AX_TX + 2 * r + 2, RY = AX_TY + 2 * r + 2;
+    return sizeof(float) * (RX | 1) * (4 * RY - 2) + sizeof(double) * (RY - 2) * AX_HP;
+}
+
 // pass 1: a, b for R and B, and the edge weight
-__global__ __launch_bounds__(BLOCK) void axial_ab(AxialArgs A) {
+__global__ __launch_bounds__(AX_BLOCK) void axial_ab(AxialArgs A) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int r = A.r;
     const int RX = AX_TX + 2 * r + 2, RY = AX_TY + 2 * r + 2;  // one extra ring for the gradient
@@ -127,12 +137,12 @@ __global__ __launch_bounds__(BLOCK) void axial_ab(AxialArgs A) {
     float *sR = reinterpret_cast<float *>(smem);
     float *sG = sR + RXP * RY;
     float *sB = sG + RXP * RY;
-    float *q = sB + RXP * RY;                            // derived quantity, (RY-2) x RX
-    double *hs = reinterpret_cast<double *>(q + RXP * RY);  // (RY-2) x AX_HP
+    float *sE = sB + RXP * RY;                           // gradient magnitude, rows 1 .. RY-2
+    double *hs = reinterpret_cast<double *>(sE + RXP * (RY - 2));  // (RY-2) x AX_HP
     const int x0 = blockIdx.x * AX_TX, y0 = blockIdx.y * AX_TY;
     const int W = A.W, H = A.H;
     const float inv_norm = 1.f / mx(__uint_as_float(A.stats[0]), 1e-6f);
-    lre::stage2d<BLOCK>(
+    lre::stage2d<AX_BLOCK>(
         RY, RX,
         [&](int ry, int rx) {
             const int y = y0 - r - 1 + ry, x = x0 - r - 1 + rx;
@@ -148,81 +158,77 @@ __global__ __launch_bounds__(BLOCK) void axial_ab(AxialArgs A) {
             sR[t] = v.x * inv_norm; sG[t] = v.y * inv_norm; sB[t] = v.z * inv_norm;
         });
     __syncthreads();
+    const int RYi = RY - 2;  // rows of the window region (the gradient ring dropped)
+    // 2-tap central-difference magnitude, 0 on the image's 1-px frame (:3876-3890)
+    for (int t = threadIdx.x; t < RYi * RX; t += AX_BLOCK) {
+        const int ry = t / RX + 1, rx = t % RX;
+        const int c = ry * RXP + rx;
+        const int y = y0 - r - 1 + ry, x = x0 - r - 1 + rx;
+        float v = 0.f;
+        if (x >= 1 && x < W - 1 && y >= 1 && y < H - 1) {
+            const float gx = sG[c + 1] - sG[c - 1];
+            const float gy = sG[c + RXP] - sG[c - RXP];
+            v = sqrtf(gx * gx + gy * gy);
+        }
+        sE[(ry - 1) * RXP + rx] = v;
+    }
+    __syncthreads();
     // this thread's outputs: rows oy .. oy+3 of column cx
     constexpr int PER = AX_RB;
-    static_assert(AX_TX * AX_TY == BLOCK * AX_RB, "one 4-row strip per thread");
     const int cx = threadIdx.x % AX_TX, oy = (threadIdx.x / AX_TX) * AX_RB;
-    double mI[PER], mII[PER], mR[PER], mIR[PER], mB[PER], mIB[PER], mE[PER];
-    const int RYi = RY - 2;  // rows of the window region (one ring dropped)
-    for (int qi = 0; qi < 7; qi++) {
-        // derived quantity over the window region rows [1, RY-1), all columns
-        for (int t = threadIdx.x; t < RYi * RX; t += BLOCK) {
-            const int ry = t / RX + 1, rx = t % RX;
-            const int c = ry * RXP + rx;
-            float v;
-            switch (qi) {
-                case 0: v = sG[c]; break;
-                case 1: v = sG[c] * sG[c]; break;
-                case 2: v = sR[c]; break;
-                case 3: v = sG[c] * sR[c]; break;
-                case 4: v = sB[c]; break;
-                case 5: v = sG[c] * sB[c]; break;
-                default: {
-                    const int y = y0 - r - 1 + ry, x = x0 - r - 1 + rx;
-                    v = 0.f;
-                    if (x >= 1 && x < W - 1 && y >= 1 && y < H - 1) {
-                        const float gx = sG[c + 1] - sG[c - 1];
-                        const float gy = sG[c + RXP] - sG[c - RXP];
-                        v = sqrtf(gx * gx + gy * gy);
-                    }
-                }
-            }
-            q[(ry - 1) * RXP + rx] = v;
-        }
-        __syncthreads();
-        hsum(q, RXP, RYi, r, hs);
-        __syncthreads();
-        double m[PER];
-        vmean(hs, r, cx, oy, x0, y0, W, H, m);
+    double m[7][PER];
 #pragma unroll
-        for (int k = 0; k < PER; k++) {
-            switch (qi) {
-                case 0: mI[k] = m[k]; break;
-                case 1: mII[k] = m[k]; break;
-                case 2: mR[k] = m[k]; break;
-                case 3: mIR[k] = m[k]; break;
-                case 4: mB[k] = m[k]; break;
-                case 5: mIB[k] = m[k]; break;
-                default: mE[k] = m[k];
-            }
-        }
+    for (int qi = 0; qi < 7; qi++) {
+        // the summed quantity at window row ry (region row ry + 1), region column c
+        hsum(
+            [&](int ry, int c) {
+                const int k = (ry + 1) * RXP + c;
+                switch (qi) {
+                    case 0: return sG[k];
+                    case 1: return sG[k] * sG[k];
+                    case 2: return sR[k];
+                    case 3: return sG[k] * sR[k];
+                    case 4: return sB[k];
+                    case 5: return sG[k] * sB[k];
+                    default: return sE[ry * RXP + c];
+                }
+            },
+            RYi, r, hs);
+        __syncthreads();
+        vmean(hs, r, cx, oy, x0, y0, W, H, m[qi]);
         __syncthreads();
     }
     for (int k = 0; k < PER; k++) {
         const int x = x0 + cx, y = y0 + oy + k;
         if (x >= W || y >= H) continue;
         const size_t i = (size_t)y * W + x;
-        const double var = mII[k] - mI[k] * mI[k];
+        const double mI = m[0][k], mII = m[1][k], mR = m[2][k], mIR = m[3][k], mB = m[4][k], mIB = m[5][k];
+        const double var = mII - mI * mI;
         const double e = (double)A.eps;
-        const double ar = (mIR[k] - mI[k] * mR[k]) / (var + e);
-        const double ab = (mIB[k] - mI[k] * mB[k]) / (var + e);
+        const double ar = (mIR - mI * mR) / (var + e);
+        const double ab = (mIB - mI * mB) / (var + e);
         A.aR[i] = (float)ar;
-        A.bR[i] = (float)(mR[k] - ar * mI[k]);
+        A.bR[i] = (float)(mR - ar * mI);
         A.aB[i] = (float)ab;
-        A.bB[i] = (float)(mB[k] - ab * mI[k]);
-        const float t = lre::clamp_std(((float)mE[k] - 0.01f) / 0.05f, 0.f, 1.f);
+        A.bB[i] = (float)(mB - ab * mI);
+        const float t = lre::clamp_std(((float)m[6][k] - 0.01f) / 0.05f, 0.f, 1.f);
         A.ew[i] = t * t * (3.f - 2.f * t);
     }
 }
 
+size_t axial_apply_lds(int r) {
+    const int RX = AX_TX + 2 * r + 2, RY = AX_TY + 2 * r;
+    return sizeof(float) * (RX | 1) * RY * 4 + sizeof(double) * RY * AX_HP;
+}
+
 // pass 2: box means of a, b; q = ma * G + mb; blend by strength * edge weight
-__global__ __launch_bounds__(BLOCK) void axial_apply(AxialArgs A) {
+__global__ __launch_bounds__(AX_BLOCK) void axial_apply(AxialArgs A) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int r = A.r;
     const int RX = AX_TX + 2 * r + 2, RY = AX_TY + 2 * r;
     const int RXP = RX | 1;
-    float *q = reinterpret_cast<float *>(smem);
-    double *hs = reinterpret_cast<double *>(q + RXP * RY);
+    float *q = reinterpret_cast<float *>(smem);             // 4 planes, RY x RXP each
+    double *hs = reinterpret_cast<double *>(q + 4 * RXP * RY);
     const int x0 = blockIdx.x * AX_TX, y0 = blockIdx.y * AX_TY;
     const int W = A.W, H = A.H;
     const float norm_ref = mx(__uint_as_float(A.stats[0]), 1e-6f);
@@ -230,18 +236,27 @@ __global__ __launch_bounds__(BLOCK) void axial_apply(AxialArgs A) {
     constexpr int PER = AX_RB;
     const int cx = threadIdx.x % AX_TX, oy = (threadIdx.x / AX_TX) * AX_RB;
     double m[4][PER];
-    const float *planes[4] = {A.aR, A.bR, A.aB, A.bB};
+    // the four planes staged together
+    lre::stage2d<AX_BLOCK>(
+        RY, RX,
+        [&](int ry, int rx) {
+            const int y = y0 - r + ry, x = x0 - r - 1 + rx;
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (x >= 0 && x < W && y >= 0 && y < H) {
+                const size_t i = (size_t)y * W + x;
+                v = make_float4(A.aR[i], A.bR[i], A.aB[i], A.bB[i]);
+            }
+            return v;
+        },
+        [&](int ry, int rx, float4 v) {
+            const int t = ry * RXP + rx;
+            q[t] = v.x; q[RXP * RY + t] = v.y; q[2 * RXP * RY + t] = v.z; q[3 * RXP * RY + t] = v.w;
+        });
+    __syncthreads();
+#pragma unroll
     for (int qi = 0; qi < 4; qi++) {
-        const float *pl = planes[qi];
-        lre::stage2d<BLOCK>(
-            RY, RX,
-            [&](int ry, int rx) {
-                const int y = y0 - r + ry, x = x0 - r - 1 + rx;
-                return (x >= 0 && x < W && y >= 0 && y < H) ? pl[(size_t)y * W + x] : 0.f;
-            },
-            [&](int ry, int rx, float v) { q[ry * RXP + rx] = v; });
-        __syncthreads();
-        hsum(q, RXP, RY, r, hs);
+        const float *pl = q + qi * RXP * RY;
+        hsum([&](int ry, int c) { return pl[ry * RXP + c]; }, RY, r, hs);
         __syncthreads();
         vmean(hs, r, cx, oy, x0, y0, W, H, m[qi]);
         __syncthreads();
@@ -530,24 +545,16 @@ int lre_launch_axial_ca(const float *d_in, float *d_out, int w, int h, int radiu
     dim3 grid((w + AX_TX - 1) / AX_TX, (h + AX_TY - 1) / AX_TY);
     // raise the dynamic LDS limit once per process (thread-safe static initialisation)
     static const hipError_t attr_rc = [] {
-        const int RX = AX_TX + 2 * AX_RMAX + 2, RY = AX_TY + 2 * AX_RMAX + 2;
-        const int lds_max = (int)(sizeof(float) * (RX | 1) * RY * 4 + sizeof(double) * RY * AX_HP);
-        hipError_t e = hipFuncSetAttribute((const void *)axial_ab, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
+        hipError_t e = hipFuncSetAttribute((const void *)axial_ab, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)axial_ab_lds(AX_RMAX));
         if (e == hipSuccess)
-            e = hipFuncSetAttribute((const void *)axial_apply, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
+            e = hipFuncSetAttribute((const void *)axial_apply, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)axial_apply_lds(AX_RMAX));
         return e;
     }();
     LRE_HIP_CHECK(attr_rc);
-    {
-        const int RX = AX_TX + 2 * radius + 2, RY = AX_TY + 2 * radius + 2;
-        const size_t lds = sizeof(float) * (RX | 1) * RY * 4 + sizeof(double) * (RY - 2) * AX_HP;
-        hipLaunchKernelGGL(axial_ab, grid, dim3(BLOCK), lds, s, A);
-    }
-    {
-        const int RX = AX_TX + 2 * radius + 2, RY = AX_TY + 2 * radius;
-        const size_t lds = sizeof(float) * (RX | 1) * RY + sizeof(double) * RY * AX_HP;
-        hipLaunchKernelGGL(axial_apply, grid, dim3(BLOCK), lds, s, A);
-    }
+    hipLaunchKernelGGL(axial_ab, grid, dim3(AX_BLOCK), axial_ab_lds(radius), s, A);
+    hipLaunchKernelGGL(axial_apply, grid, dim3(AX_BLOCK), axial_apply_lds(radius), s, A);
     LRE_HIP_CHECK(hipGetLastError());
     return LRE_OK;
 }

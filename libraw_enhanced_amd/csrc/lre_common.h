@@ -80,6 +80,34 @@ LRE_DEV float aces(float x) {
     return (x * (a * x + b)) / (x * (c * x + d) + e);
 }
 
+// Pixels of the frame of width `border` around a w x h image, enumerated as
+// the full rows at the top, then at the bottom, then the left and right strips
+// of the rows between (on images narrower than two strips a pixel may appear
+// twice).  frame_pixels gives the count, frame_pixel the t-th position.
+LRE_DEV __host__ int frame_pixels(int w, int h, int border) {
+    const int top = border < h ? border : h;
+    const int bot0 = border < h - border ? border : h - border, bot = bot0 > 0 ? bot0 : 0;
+    const int mid = h - 2 * border > 0 ? h - 2 * border : 0, strip = border < w ? border : w;
+    return (top + bot) * w + mid * 2 * strip;
+}
+LRE_DEV void frame_pixel(int t, int w, int h, int border, int &row, int &col) {
+    const int top_n = min(border, h) * w;
+    const int bot_rows = max(0, min(border, h - border)), bot_n = bot_rows * w;
+    const int strip = min(border, w);
+    if (t < top_n) {
+        row = t / w;
+        col = t % w;
+    } else if (t < top_n + bot_n) {
+        const int u = t - top_n;
+        row = h - bot_rows + u / w;
+        col = u % w;
+    } else {
+        const int u = t - top_n - bot_n, s = u % (2 * strip);
+        row = border + u / (2 * strip);
+        col = s < strip ? s : w - strip + (s - strip);
+    }
+}
+
 }  // namespace lre
 
 // Error plumbing of the C-ABI: every HIP call is checked, the message is kept

@@ -528,11 +528,12 @@ __global__ void xtrans_set_maxv(XArgs *entry, const float *d_maxv) {
 __global__ void xtrans_border(XArgs A, int border, const float *d_maxv) {
     if (d_maxv) A.maxv = *d_maxv;
     const int W = A.W, H = A.H;
-    const size_t n = (size_t)W * H;
     const float wt[3][3] = {{0.25f, 0.5f, 0.25f}, {0.5f, 0.f, 0.5f}, {0.25f, 0.5f, 0.25f}};
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const int row = (int)(i / W), col = (int)(i % W);
-        if (col >= border && col < W - border && row >= border && row < H - border) continue;
+    const int total = lre::frame_pixels(W, H, border);
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+        int row, col;
+        lre::frame_pixel(t, W, H, border, row, col);
+        const size_t i = (size_t)row * W + col;
         float sum[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         for (int y = row > 0 ? row - 1 : 0, v = row == 0 ? 0 : -1; y < row + 2 && y < H; y++, v++)
             for (int x = col > 0 ? col - 1 : 0, h = col == 0 ? 0 : -1; x < col + 2 && x < W; x++, h++) {
@@ -654,9 +655,10 @@ int lre_launch_xtrans3(lre_ctx *ctx, const float *d_cfa, float *d_rgb, int w, in
         hipLaunchKernelGGL(xtrans_kernel, dim3(xtrans_slots(ctx, w, h)), dim3(BLOCK), LDS_BYTES, s, A);
         LRE_HIP_CHECK(hipGetLastError());
     }
-    const size_t frame = (size_t)w * h;
+    const size_t frame = (size_t)lre::frame_pixels(w, h, 8);
     size_t blocks = (frame + 255) / 256;
     if (blocks > 8192) blocks = 8192;
+    if (blocks < 1) blocks = 1;
     hipLaunchKernelGGL(xtrans_border, dim3((unsigned)blocks), dim3(256), 0, s, A, 8,
                        static_cast<const float *>(nullptr));
     LRE_HIP_CHECK(hipGetLastError());
@@ -711,9 +713,10 @@ int lre_launch_xtrans_multi(const lre_demosaic_job *jobs, int n, const void *h_t
         LRE_HIP_CHECK(hipGetLastError());
     }
     for (int k = 0; k < n; k++) {
-        const size_t frame = (size_t)jobs[k].w * jobs[k].h;
+        const size_t frame = (size_t)lre::frame_pixels(jobs[k].w, jobs[k].h, 8);
         size_t blocks = (frame + 255) / 256;
         if (blocks > 8192) blocks = 8192;
+        if (blocks < 1) blocks = 1;
         hipLaunchKernelGGL(xtrans_border, dim3((unsigned)blocks), dim3(256), 0, s, tab[k], 8,
                            jobs[k].d_maxv);
         LRE_HIP_CHECK(hipGetLastError());
