@@ -317,14 +317,23 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
     }
     __syncthreads();
 
-    // stage 1 (:1331-1339)
+    // stage 1 (:1331-1339).  Its planes are functions of the LDS CFA plane:
+    // stages 2 and 4-5 recompute the values they need from LDS instead of
+    // loading them (bit-identical); the planes are still stored, because later
+    // stages read wh after pmwt overwrote part of it, and rbint over dsq.
+    auto f_dh = [&](int i) { return fabsf(cfa[i + 1] - cfa[i - 1]); };
+    auto f_dv = [&](int i) { return fabsf(cfa[i + v1] - cfa[i - v1]); };
+    auto f_wv = [&](int i) { return EPS + fabsf(cfa[i + v2] - cfa[i]) + fabsf(cfa[i] - cfa[i - v2]) + f_dv(i); };
+    auto f_wh = [&](int i) { return EPS + fabsf(cfa[i + 2] - cfa[i]) + fabsf(cfa[i] - cfa[i - 2]) + f_dh(i); };
+    auto f_dsq = [&](int i) {
+        const float dh = f_dh(i), dv = f_dv(i);
+        return lre::sq(dh) + lre::sq(dv);
+    };
     for_rect(2, rr1 - 2, 2, cc1 - 2, [&](int rr, int cc) {
         int i = rr * TS + cc;
-        float dh = fabsf(cfa[i + 1] - cfa[i - 1]);
-        float dv = fabsf(cfa[i + v1] - cfa[i - v1]);
-        wv[i] = EPS + fabsf(cfa[i + v2] - cfa[i]) + fabsf(cfa[i] - cfa[i - v2]) + dv;
-        wh[i] = EPS + fabsf(cfa[i + 2] - cfa[i]) + fabsf(cfa[i] - cfa[i - 2]) + dh;
-        dsq[i] = lre::sq(dh) + lre::sq(dv);
+        wv[i] = f_wv(i);
+        wh[i] = f_wh(i);
+        dsq[i] = f_dsq(i);
     });
     __syncthreads();
 
@@ -333,10 +342,10 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
         int i = rr * TS + cc;
         float sgn = (fct(A, rr, cc) & 1) ? -1.f : 1.f;
         float c0 = cfa[i];
-        float ru = cfa[i - v1] * (wv[i - v2] + wv[i]) / (wv[i - v2] * (EPS + c0) + wv[i] * (EPS + cfa[i - v2]));
-        float rd = cfa[i + v1] * (wv[i + v2] + wv[i]) / (wv[i + v2] * (EPS + c0) + wv[i] * (EPS + cfa[i + v2]));
-        float rl = cfa[i - 1] * (wh[i - 2] + wh[i]) / (wh[i - 2] * (EPS + c0) + wh[i] * (EPS + cfa[i - 2]));
-        float rr_ = cfa[i + 1] * (wh[i + 2] + wh[i]) / (wh[i + 2] * (EPS + c0) + wh[i] * (EPS + cfa[i + 2]));
+        float ru = cfa[i - v1] * (f_wv(i - v2) + f_wv(i)) / (f_wv(i - v2) * (EPS + c0) + f_wv(i) * (EPS + cfa[i - v2]));
+        float rd = cfa[i + v1] * (f_wv(i + v2) + f_wv(i)) / (f_wv(i + v2) * (EPS + c0) + f_wv(i) * (EPS + cfa[i + v2]));
+        float rl = cfa[i - 1] * (f_wh(i - 2) + f_wh(i)) / (f_wh(i - 2) * (EPS + c0) + f_wh(i) * (EPS + cfa[i - 2]));
+        float rr_ = cfa[i + 1] * (f_wh(i + 2) + f_wh(i)) / (f_wh(i + 2) * (EPS + c0) + f_wh(i) * (EPS + cfa[i + 2]));
         float gu_h = cfa[i - v1] + half(c0 - cfa[i - v2]);
         float gd_h = cfa[i + v1] + half(c0 - cfa[i + v2]);
         float gl_h = cfa[i - 1] + half(c0 - cfa[i - 2]);
@@ -345,8 +354,8 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
         float gd_a = fabsf(1.f - rd) < ARTHRESH ? c0 * rd : gd_h;
         float gl_a = fabsf(1.f - rl) < ARTHRESH ? c0 * rl : gl_h;
         float gr_a = fabsf(1.f - rr_) < ARTHRESH ? c0 * rr_ : gr_h;
-        float hw = wh[i - 1] / (wh[i - 1] + wh[i + 1]);
-        float vw = wv[i - v1] / (wv[i + v1] + wv[i - v1]);
+        float hw = f_wh(i - 1) / (f_wh(i - 1) + f_wh(i + 1));
+        float vw = f_wv(i - v1) / (f_wv(i + v1) + f_wv(i - v1));
         float gv_h = vw * gd_h + (1.f - vw) * gu_h;
         float gh_h = hw * gr_h + (1.f - hw) * gl_h;
         float va = sgn * (gv_h - c0), ha = sgn * (gh_h - c0);
@@ -459,8 +468,8 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
         float vd = sq(vcd[i] - dave) + sq(vcd[i + v1] - dave) + sq(vcd[i + v2] - dave) + sq(vcd[i + v3] - dave);
         float hl = sq(hcd[i] - lave) + sq(hcd[i - 1] - lave) + sq(hcd[i - 2] - lave) + sq(hcd[i - 3] - lave);
         float hr = sq(hcd[i] - rave) + sq(hcd[i + 1] - rave) + sq(hcd[i + 2] - rave) + sq(hcd[i + 3] - rave);
-        float hw = wh[i - 1] / (wh[i - 1] + wh[i + 1]);
-        float vw = wv[i - v1] / (wv[i + v1] + wv[i - v1]);
+        float hw = f_wh(i - 1) / (f_wh(i - 1) + f_wh(i + 1));
+        float vw = f_wv(i - v1) / (f_wv(i + v1) + f_wv(i - v1));
         float vvar = EPSSQ + vw * vd + (1.f - vw) * vu;
         float hvar = EPSSQ + hw * hr + (1.f - hw) * hl;
         float vu2 = dgv[i] + dgv[i - v1] + dgv[i - v2];
@@ -477,18 +486,18 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
         hvwt[k] = w;
         sm[k] = w;  // LDS copy for the smoothing pass
     });
-    for_site_pairs(A, 6, rr1 - 6, 6, cc1 - 6, [&](int rr, int cc) {
+    for_sites_split(A, 6, rr1 - 6, 6, cc1 - 6, [&](int rr, int cc) {
         const int i = rr * TS + cc;
-        const float *d = cdsq, *g = dsq;
+        const float *d = cdsq;
         return
             (GODD[0] * d[i] + GODD[1] * (d[i - m1] + d[i + p1] + d[i - p1] + d[i + m1]) +
              GODD[2] * (d[i - v2] + d[i - 2] + d[i + 2] + d[i + v2]) +
              GODD[3] * (d[i - m2] + d[i + p2] + d[i - p2] + d[i + m2])) -
-            (GGRAD[0] * g[i] + GGRAD[1] * (g[i - v1] + g[i + 1] + g[i - 1] + g[i + v1]) +
-             GGRAD[2] * (g[i - m1] + g[i + p1] + g[i - p1] + g[i + m1]) +
-             GGRAD[3] * (g[i - v2] + g[i - 2] + g[i + 2] + g[i + v2]) +
-             GGRAD[4] * (g[i - v2 - 1] + g[i - v2 + 1] + g[i - TS - 2] + g[i - TS + 2] + g[i + TS - 2] + g[i + TS + 2] + g[i + v2 - 1] + g[i + v2 + 1]) +
-             GGRAD[5] * (g[i - m2] + g[i + p2] + g[i - p2] + g[i + m2]));
+            (GGRAD[0] * f_dsq(i) + GGRAD[1] * (f_dsq(i - v1) + f_dsq(i + 1) + f_dsq(i - 1) + f_dsq(i + v1)) +
+             GGRAD[2] * (f_dsq(i - m1) + f_dsq(i + p1) + f_dsq(i - p1) + f_dsq(i + m1)) +
+             GGRAD[3] * (f_dsq(i - v2) + f_dsq(i - 2) + f_dsq(i + 2) + f_dsq(i + v2)) +
+             GGRAD[4] * (f_dsq(i - v2 - 1) + f_dsq(i - v2 + 1) + f_dsq(i - TS - 2) + f_dsq(i - TS + 2) + f_dsq(i + TS - 2) + f_dsq(i + TS + 2) + f_dsq(i + v2 - 1) + f_dsq(i + v2 + 1)) +
+             GGRAD[5] * (f_dsq(i - m2) + f_dsq(i + p2) + f_dsq(i - p2) + f_dsq(i + m2)));
     }, [&](int rr, int cc, float nt) { nyqtest[(rr * TS + cc) >> 1] = nt; });
     __syncthreads();
     // Nyquist flags start at zero (:1432); the clear also wipes what the tile
