@@ -616,7 +616,15 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
     });
     __syncthreads();
 
-    // stage 8 (:1505-1532)
+    // stage 8 (:1505-1532).  The stage-7 cells it reads are functions of the
+    // LDS CFA plane and lie inside stage 7's range: the diagonal gradients of a
+    // site s, and the q1 sums at a green n (the diagonal neighbours of a site
+    // are sites, its axial and knight neighbours greens).  They are recomputed
+    // here rather than loaded (bit-identical).
+    auto f_delp = [&](int s) { return fabsf(cfa[s + p1] - cfa[s - p1]); };
+    auto f_delm = [&](int s) { return fabsf(cfa[s + m1] - cfa[s - m1]); };
+    auto f_q1p = [&](int n) { return lre::sq(cfa[n] - cfa[n - p1]) + lre::sq(cfa[n] - cfa[n + p1]); };
+    auto f_q1m = [&](int n) { return lre::sq(cfa[n] - cfa[n - m1]) + lre::sq(cfa[n] - cfa[n + m1]); };
     float *rbm = vcd, *rbp = hcdalt, *pmwt = wh;
     for_sites(A, 8, rr1 - 8, 8, cc1 - 8, [&](int rr, int cc) {
         const int i = rr * TS + cc, k = i >> 1;
@@ -629,10 +637,10 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
         float bnw = fabsf(1.f - rnw) < ARTHRESH ? c0 * rnw : (cfa[i - m1]) + half(c0 - cfa[i - m2]);
         float bne = fabsf(1.f - rne) < ARTHRESH ? c0 * rne : (cfa[i + p1]) + half(c0 - cfa[i + p2]);
         float bsw = fabsf(1.f - rsw) < ARTHRESH ? c0 * rsw : (cfa[i - p1]) + half(c0 - cfa[i - p2]);
-        float wse = EPS + delm[k] + delm[(i + m1) >> 1] + delm[(i + m2) >> 1];
-        float wnw = EPS + delm[k] + delm[(i - m1) >> 1] + delm[(i - m2) >> 1];
-        float wne = EPS + delp[k] + delp[(i + p1) >> 1] + delp[(i + p2) >> 1];
-        float wsw = EPS + delp[k] + delp[(i - p1) >> 1] + delp[(i - p2) >> 1];
+        float wse = EPS + f_delm(i) + f_delm(i + m1) + f_delm(i + m2);
+        float wnw = EPS + f_delm(i) + f_delm(i - m1) + f_delm(i - m2);
+        float wne = EPS + f_delp(i) + f_delp(i + p1) + f_delp(i + p2);
+        float wsw = EPS + f_delp(i) + f_delp(i - p1) + f_delp(i - p2);
         float m = (wse * bnw + wnw * bse) / (wse + wnw);
         float p = (wne * bsw + wsw * bne) / (wne + wsw);
         const float cp = cfa[i - p1], cp2 = cfa[i + p1], cm = cfa[i - m1], cm2 = cfa[i + m1];
@@ -654,13 +662,13 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
         if (m > A.clip_pt) m = lre::med3(m, cm, cm2);
         rbp[k] = p;
         rbm[k] = m;
-#define QS(a, o) a[(i + (o)) >> 1]
-        float varm = EPSSQ + (GEVEN[0] * (QS(q1m, -v1) + QS(q1m, -1) + QS(q1m, 1) + QS(q1m, v1)) +
-                              GEVEN[1] * (QS(q1m, -v2 - 1) + QS(q1m, -v2 + 1) + QS(q1m, -2 - v1) + QS(q1m, 2 - v1) +
-                                          QS(q1m, -2 + v1) + QS(q1m, 2 + v1) + QS(q1m, v2 - 1) + QS(q1m, v2 + 1)));
-        float varp = EPSSQ + (GEVEN[0] * (QS(q1p, -v1) + QS(q1p, -1) + QS(q1p, 1) + QS(q1p, v1)) +
-                              GEVEN[1] * (QS(q1p, -v2 - 1) + QS(q1p, -v2 + 1) + QS(q1p, -2 - v1) + QS(q1p, 2 - v1) +
-                                          QS(q1p, -2 + v1) + QS(q1p, 2 + v1) + QS(q1p, v2 - 1) + QS(q1p, v2 + 1)));
+#define QS(f, o) f(i + (o))
+        float varm = EPSSQ + (GEVEN[0] * (QS(f_q1m, -v1) + QS(f_q1m, -1) + QS(f_q1m, 1) + QS(f_q1m, v1)) +
+                              GEVEN[1] * (QS(f_q1m, -v2 - 1) + QS(f_q1m, -v2 + 1) + QS(f_q1m, -2 - v1) + QS(f_q1m, 2 - v1) +
+                                          QS(f_q1m, -2 + v1) + QS(f_q1m, 2 + v1) + QS(f_q1m, v2 - 1) + QS(f_q1m, v2 + 1)));
+        float varp = EPSSQ + (GEVEN[0] * (QS(f_q1p, -v1) + QS(f_q1p, -1) + QS(f_q1p, 1) + QS(f_q1p, v1)) +
+                              GEVEN[1] * (QS(f_q1p, -v2 - 1) + QS(f_q1p, -v2 + 1) + QS(f_q1p, -2 - v1) + QS(f_q1p, 2 - v1) +
+                                          QS(f_q1p, -2 + v1) + QS(f_q1p, 2 + v1) + QS(f_q1p, v2 - 1) + QS(f_q1p, v2 + 1)));
 #undef QS
         const float w = varm / (varp + varm);
         pmwt[k] = w;
