@@ -199,7 +199,11 @@ LRE_DEV void smooth_rows(const Args &A, float *sm, int b, int rr1, int cc1) {
 // LDS: the tile's CFA plane, and one half-index plane for the row-sequential
 // smoothing passes
 constexpr int LDS_CFA = N2, LDS_HALF = TS * TSH;
-constexpr size_t LDS_BYTES = sizeof(float) * (LDS_CFA + LDS_HALF);
+// and the two Nyquist flag planes as bit masks (the byte planes of the arena
+// are still written, the masks serve the reads)
+constexpr int NYQ_WORDS = TS * TSH / 32;
+constexpr size_t LDS_BYTES = sizeof(float) * (LDS_CFA + LDS_HALF) + 2 * sizeof(unsigned) * NYQ_WORDS;
+static_assert((TS * TSH) % 32 == 0, "whole mask words");
 
 __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
     constexpr int v1 = TS, v2 = 2 * TS, v3 = 3 * TS;
@@ -211,6 +215,8 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
     // arena right behind the CFA array (the Nyquist flags), as in the reference
     float *cfa = lds, *cfa_spill = ar + O_CFA, *green = ar + O_GREEN;
     float *sm = lds + LDS_CFA;
+    unsigned *nqm = reinterpret_cast<unsigned *>(sm + LDS_HALF), *nqm2 = nqm + NYQ_WORDS;
+    auto nbit = [](const unsigned *m, int k) { return (m[k >> 5] >> (k & 31)) & 1u; };
     float *wv = ar + O_WV, *wh = ar + O_WH, *dsq = ar + O_DSQ;
     float *vcd = ar + O_VCD, *hcd = ar + O_HCD, *vcdalt = ar + O_VCDALT, *hcdalt = ar + O_HCDALT;
     float *dgv = ar + O_DGV, *dgh = ar + O_DGH, *cdsq = ar + O_CDSQ, *hvwt = ar + O_HVWT;
@@ -503,11 +509,16 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
     // Nyquist flags start at zero (:1432); the clear also wipes what the tile
     // fill wrote past the end of the CFA array
     for (int t = threadIdx.x; t < TS * TSH / 4; t += BLOCK) reinterpret_cast<unsigned *>(nyq)[t] = 0u;
+    for (int t = threadIdx.x; t < 2 * NYQ_WORDS; t += BLOCK) nqm[t] = 0u;
     __syncthreads();
     int local_nyq = 0;
     for_sites(A, 6, rr1 - 6, 6, cc1 - 6, [&](int rr, int cc) {
-        int i = rr * TS + cc;
-        if (nyqtest[i >> 1] > 0.f) { nyq[i >> 1] = 1; local_nyq = 1; }
+        const int k = (rr * TS + cc) >> 1;
+        if (nyqtest[k] > 0.f) {
+            nyq[k] = 1;
+            atomicOr(&nqm[k >> 5], 1u << (k & 31));
+            local_nyq = 1;
+        }
     });
     const int any_nyq = __syncthreads_or(local_nyq);
 
@@ -517,19 +528,23 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
         __syncthreads();
         for_sites(A, 8, rr1 - 8, 8, cc1 - 8, [&](int rr, int cc) {
             int i = rr * TS + cc;
-            unsigned s = nyq[(i - v2) >> 1] + nyq[(i - m1) >> 1] + nyq[(i + p1) >> 1] + nyq[(i - 2) >> 1] +
-                         nyq[(i + 2) >> 1] + nyq[(i - p1) >> 1] + nyq[(i + m1) >> 1] + nyq[(i + v2) >> 1];
-            nyq2[i >> 1] = s > 4 ? 1 : (s < 4 ? 0 : nyq[i >> 1]);
+            const unsigned s = nbit(nqm, (i - v2) >> 1) + nbit(nqm, (i - m1) >> 1) + nbit(nqm, (i + p1) >> 1) +
+                               nbit(nqm, (i - 2) >> 1) + nbit(nqm, (i + 2) >> 1) + nbit(nqm, (i - p1) >> 1) +
+                               nbit(nqm, (i + m1) >> 1) + nbit(nqm, (i + v2) >> 1);
+            const unsigned f = s > 4 ? 1u : (s < 4 ? 0u : nbit(nqm, i >> 1));
+            const int k = i >> 1;
+            nyq2[k] = (unsigned char)f;
+            if (f) atomicOr(&nqm2[k >> 5], 1u << (k & 31));
         });
         __syncthreads();
         for_sites(A, 8, rr1 - 8, 8, cc1 - 8, [&](int rr, int cc) {
             int i = rr * TS + cc;
-            if (!nyq2[i >> 1]) return;
+            if (!nbit(nqm2, i >> 1)) return;
             float scfa = 0, sh = 0, sv = 0, sqh = 0, sqv = 0, area = 0;
             for (int a = -6; a < 7; a += 2)
                 for (int b = -6; b < 7; b += 2) {
                     int j = i + a * TS + b;
-                    if (j >= 0 && j < N2 && nyq2[j >> 1]) {
+                    if (j >= 0 && j < N2 && nbit(nqm2, j >> 1)) {
                         float t = cfa[j];
                         scfa += t;
                         sh += (cfa[j - 1] + cfa[j + 1]);
@@ -563,7 +578,7 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
         R.hk = sm[k];
         R.d0 = R.hk * vcd[i] + (1.f - R.hk) * hcd[i];
         R.g = cfa[i] + R.d0;
-        const bool q = nyq2[k] != 0;
+        const bool q = nbit(nqm2, k) != 0;
         R.dh = q ? lre::sq(R.g - half(green[i - 1] + green[i + 1])) : 0.f;
         R.dv = q ? lre::sq(R.g - half(green[i - v1] + green[i + v1])) : 0.f;
         return R;
@@ -579,7 +594,7 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
     if (any_nyq) {
         for_sites(A, 8, rr1 - 8, 8, cc1 - 8, [&](int rr, int cc) {
             int i = rr * TS + cc, k = i >> 1;
-            if (!nyq2[k]) return;
+            if (!nbit(nqm2, k)) return;
 #define D2H(o) d2[2 * ((i + (o)) >> 1)]
 #define D2V(o) d2[2 * ((i + (o)) >> 1) + 1]
             float gvh = EPSSQ + (GQUINC[0] * D2H(0) + GQUINC[1] * (D2H(-m1) + D2H(p1) + D2H(-p1) + D2H(m1)) +
