@@ -711,7 +711,7 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
     for_sites_split(A, 12, rr1 - 12, 12, cc1 - 12, [&](int rr, int cc) {
         const int i = rr * TS + cc, k = i >> 1;
         SR R{false, 0.f, 0.f};
-        if (fabsf(0.5f - pmwt[k]) >= fabsf(0.5f - hvwt[k])) return R;
+        if (fabsf(0.5f - sm[k]) >= fabsf(0.5f - hvwt[k])) return R;  // sm holds the smoothed pmwt
         float rb = rbint[k];
         float ru = twice(cfa[i - v1]) / (EPS + rb + rbint[k - v1]);
         float rd = twice(cfa[i + v1]) / (EPS + rb + rbint[k + v1]);
@@ -754,28 +754,42 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
     });
     __syncthreads();
 
-    // second chroma difference (:1571-1595)
+    // second chroma difference (:1571-1595).  Rows r0 + 2m (the B rows) move
+    // their Dgrb0 into Dgrb1 and clear it.  The values before the move (Dgrb0 of
+    // every site: call it X) are also copied into the LDS half plane for rows
+    // 11 .. rr1-12, which covers every read below: the pass reads only X, and
+    // stage 10 reads X there and the pass's output from the arena.
     float *dg1 = hcd;  // Dgrb1 over hcd (half-index)
+    const int r0b = 13 - A.ey;
     {
-        const int r0 = 13 - A.ey;
-        const int nr = rr1 - 12 - r0 > 0 ? (rr1 - 12 - r0 + 1) / 2 : 0;
-        const int k0 = (13 - A.ex) >> 1;  // offsets within a row are row-independent (TS even)
-        const int nk = ((cc1 - 12) >> 1) - k0;
-        if (nk > 0)
-            for (int t = threadIdx.x; t < nr * nk; t += BLOCK) {
-                int rr = r0 + 2 * (t / nk);
-                int k = ((rr * TS + 13 - A.ex) >> 1) + t % nk;
-                if (k < ((rr * TS + cc1 - 12) >> 1)) {
-                    dg1[k] = dg0[k];
+        const int nr = max(rr1 - 22, 0), n = nr * TSH;
+        constexpr int U = 2;
+        for (int t0 = threadIdx.x; t0 < n; t0 += U * BLOCK) {
+            float v[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int t = t0 + u * BLOCK;
+                v[u] = t < n ? dg0[11 * TSH + t] : 0.f;
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int t = t0 + u * BLOCK;
+                if (t >= n) break;
+                const int rr = 11 + t / TSH, k = 11 * TSH + t;
+                sm[k] = v[u];
+                if (rr >= r0b && ((rr - r0b) & 1) == 0 && rr < rr1 - 12 && k >= ((rr * TS + 13 - A.ex) >> 1) &&
+                    k < ((rr * TS + cc1 - 12) >> 1)) {
+                    dg1[k] = v[u];
                     dg0[k] = 0;
                 }
             }
+        }
     }
     __syncthreads();
     for_sites(A, 14, rr1 - 14, 14, cc1 - 14, [&](int rr, int cc) {
         int i = rr * TS + cc;
         float *D = (1 - (int)fct(A, rr, cc) / 2) ? dg1 : dg0;
-#define DD(o) D[(i + (o)) >> 1]
+#define DD(o) sm[(i + (o)) >> 1]
         float wnw = 1.f / (EPS + fabsf(DD(-m1) - DD(m1)) + fabsf(DD(-m1) - DD(-m3)) + fabsf(DD(m1) - DD(-m3)));
         float wne = 1.f / (EPS + fabsf(DD(p1) - DD(-p1)) + fabsf(DD(p1) - DD(p3)) + fabsf(DD(-p1) - DD(p3)));
         float wsw = 1.f / (EPS + fabsf(DD(-p1) - DD(p1)) + fabsf(DD(-p1) - DD(m3)) + fabsf(DD(p1) - DD(-p3)));
@@ -797,18 +811,23 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
         const int row = rr + top, col = cc + left;
         const int t = rr * TS + cc;
         float r, g, b;
+        // Dgrb0 / Dgrb1 of the site cell k in tile row q: on B rows X is Dgrb1
+        // and the pass's output Dgrb0, on R rows the other way round
+        auto d0 = [&](int q, int k) { return ((q - r0b) & 1) == 0 ? dg0[k] : sm[k]; };
+        auto d1 = [&](int q, int k) { return ((q - r0b) & 1) == 0 ? sm[k] : dg1[k]; };
         if (fct(A, row, col) == 1) {
             g = cfa[t];
             float hu = hvwt[(t - v1) >> 1], hr = hvwt[(t + 1) >> 1], hl = hvwt[(t - 1) >> 1], hd = hvwt[(t + v1) >> 1];
             float inv = 1.0f / (hu + 2.f - hr - hl + hd);
-            float rd = (hu * dg0[(t - v1) >> 1] + (1.f - hr) * dg0[(t + 1) >> 1] + (1.f - hl) * dg0[(t - 1) >> 1] + hd * dg0[(t + v1) >> 1]) * inv;
-            float bd = (hu * dg1[(t - v1) >> 1] + (1.f - hr) * dg1[(t + 1) >> 1] + (1.f - hl) * dg1[(t - 1) >> 1] + hd * dg1[(t + v1) >> 1]) * inv;
+            const int ku = (t - v1) >> 1, kr = (t + 1) >> 1, kl = (t - 1) >> 1, kd = (t + v1) >> 1;
+            float rd = (hu * d0(rr - 1, ku) + (1.f - hr) * d0(rr, kr) + (1.f - hl) * d0(rr, kl) + hd * d0(rr + 1, kd)) * inv;
+            float bd = (hu * d1(rr - 1, ku) + (1.f - hr) * d1(rr, kr) + (1.f - hl) * d1(rr, kl) + hd * d1(rr + 1, kd)) * inv;
             r = g - rd;
             b = g - bd;
         } else {
             g = green[t];
-            r = g - dg0[t >> 1];
-            b = g - dg1[t >> 1];
+            r = g - d0(rr, t >> 1);
+            b = g - d1(rr, t >> 1);
         }
         return SO{r, g, b};
     }, [&](int rr, int cc, const SO &R) {
