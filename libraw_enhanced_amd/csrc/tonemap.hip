@@ -26,23 +26,24 @@ __global__ void aces_kernel(const float4 *__restrict__ in, float4 *__restrict__ 
 // ---- detail-preserving tone map, core/libraw_wrapper.cpp:1398-1471 ---------
 // Box means of radius 12 with the window clipped at the frame and divided by
 // the true tap count, rows first then columns (core/libraw_wrapper.cpp:45-116).
-// Each workgroup stages a (64+24) x (32+24) input window in LDS, sums 25 taps
-// horizontally into LDS, then 25 taps vertically.  The reference's running
-// sums are evaluated in a different order; the results agree to ~1e-6
-// relative.
+// Each workgroup stages a (64+24) x (64+24) input window in LDS, sums 25 taps
+// horizontally into LDS, then 25 taps vertically, finishing each output in
+// the thread that sums it.  The reference's running sums are evaluated in a
+// different order; the results agree to ~1e-6 relative.
 constexpr int DR = 12;                  // radius
-constexpr int DTX = 64, DTY = 32;       // output tile
+constexpr int DTX = 64, DTY = 64;       // output tile
 constexpr int DIX = DTX + 2 * DR, DIY = DTY + 2 * DR;
 constexpr int DIXP = DIX + 1, DTXP = DTX + 1;  // odd LDS row pitches
-constexpr int DBLOCK = 256;
+constexpr int DBLOCK = 512;
 
 LRE_DEV int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 // Box means over LDS windows.  hpass: horizontal window sums of the DIY x DIX
 // window `in` (pitch DIXP, zero outside the image) divided by the clipped tap
 // count, for the DTX output columns, into hs0 (and, with SQ, the sums of the
-// squared values into hs1); pitch DTXP.  vpass: the vertical sums of hs0/hs1
-// into out0/out1 (DTY x DTX).  Each thread sums RB neighbouring outputs along
+// squared values into hs1); pitch DTXP.  vpass: the vertical sums of hs0/hs1,
+// handed to f(row in tile, column in tile, mean0, mean1).  Each thread sums RB
+// neighbouring outputs along
 // the filter direction with a sliding window; every output adds its 25 taps in
 // order.  Lanes walk down rows; the odd pitches put them in distinct LDS banks.
 constexpr int RB = 4, NT = 2 * DR + 1;
@@ -78,7 +79,8 @@ LRE_DEV void hpass(const float *in, float *hs0, float *hs1, int x0, int W) {
     }
 }
 
-LRE_DEV void vpass(const float *hs0, const float *hs1, float *out0, float *out1, int y0, int H) {
+template <class F>
+LRE_DEV void vpass(const float *hs0, const float *hs1, int y0, int H, F f) {
     for (int t = threadIdx.x; t < (DTY / RB) * DTX; t += DBLOCK) {
         const int cx = t % DTX, oy = (t / DTX) * RB;
         const float *c0 = hs0 + oy * DTXP + cx, *c1 = hs1 + oy * DTXP + cx;
@@ -100,21 +102,18 @@ LRE_DEV void vpass(const float *hs0, const float *hs1, float *out0, float *out1,
         for (int u = 0; u < RB; u++) {
             const int y = y0 + oy + u;
             const float cnt = (float)(min(H - 1, y + DR) - max(0, y - DR) + 1);
-            out0[(oy + u) * DTX + cx] = s0[u] / cnt;
-            out1[(oy + u) * DTX + cx] = s1[u] / cnt;
+            f(oy + u, cx, s0[u] / cnt, s1[u] / cnt);
         }
     }
 }
 
-static_assert(2 * DTY * DTX <= DIY * DIXP, "box outputs fit in the input window");
 
 // pass 1: guide = max(0,R,G,B); a = var/(var+eps), b = mean(1-a) -> planes
 __global__ __launch_bounds__(DBLOCK) void dtm_ab_kernel(const float *__restrict__ rgb,
                                                         float *__restrict__ pa,
                                                         float *__restrict__ pb, int W, int H) {
-    __shared__ float win[DIY * DIXP];
-    __shared__ float hs0[DIY * DTXP], hs1[DIY * DTXP];
-    float *o0 = win, *o1 = win + DTY * DTX;  // the window is dead after hpass
+    extern __shared__ float dsm[];
+    float *win = dsm, *hs0 = win + DIY * DIXP, *hs1 = hs0 + DIY * DTXP;
     const int x0 = blockIdx.x * DTX, y0 = blockIdx.y * DTY;
     lre::stage2d<DBLOCK>(
         DIY, DIX,
@@ -131,17 +130,14 @@ __global__ __launch_bounds__(DBLOCK) void dtm_ab_kernel(const float *__restrict_
     __syncthreads();
     hpass<true>(win, hs0, hs1, x0, W);  // sums of g and of g * g
     __syncthreads();
-    vpass(hs0, hs1, o0, o1, y0, H);
-    __syncthreads();
-    for (int t = threadIdx.x; t < DTY * DTX; t += DBLOCK) {
-        const int x = x0 + t % DTX, y = y0 + t / DTX;
-        if (x >= W || y >= H) continue;
-        const float mean = o0[t];
-        const float var = lre::max_std(0.0f, o1[t] - mean * mean);
+    vpass(hs0, hs1, y0, H, [&](int ty, int tx, float mean, float msq) {
+        const int x = x0 + tx, y = y0 + ty;
+        if (x >= W || y >= H) return;
+        const float var = lre::max_std(0.0f, msq - mean * mean);
         const float a = var / (var + 0.0025f);
         pa[(size_t)y * W + x] = a;
         pb[(size_t)y * W + x] = mean * (1.0f - a);
-    }
+    });
 }
 
 // pass 2: box(a), box(b); base = max(1e-6, A*guide + B); RGB *= 1 + (gain-1) w
@@ -149,9 +145,8 @@ __global__ __launch_bounds__(DBLOCK) void dtm_apply_kernel(const float *__restri
                                                            const float *__restrict__ pb,
                                                            const float *in, float *out, int W,
                                                            int H) {
-    __shared__ float win[DIY * DIXP];
-    __shared__ float hs0[DIY * DTXP], hs1[DIY * DTXP];
-    float *o0 = win, *o1 = win + DTY * DTX;
+    extern __shared__ float dsm[];
+    float *win = dsm, *hs0 = win + DIY * DIXP, *hs1 = hs0 + DIY * DTXP;
     const int x0 = blockIdx.x * DTX, y0 = blockIdx.y * DTY;
     // the a and b windows pass through the same LDS buffer in turn
     for (int pl = 0; pl < 2; pl++) {
@@ -167,15 +162,13 @@ __global__ __launch_bounds__(DBLOCK) void dtm_apply_kernel(const float *__restri
         hpass<false>(win, pl ? hs1 : hs0, nullptr, x0, W);
         __syncthreads();
     }
-    vpass(hs0, hs1, o0, o1, y0, H);
-    __syncthreads();
-    for (int t = threadIdx.x; t < DTY * DTX; t += DBLOCK) {
-        const int x = x0 + t % DTX, y = y0 + t / DTX;
-        if (x >= W || y >= H) continue;
+    vpass(hs0, hs1, y0, H, [&](int ty, int tx, float ma, float mb) {
+        const int x = x0 + tx, y = y0 + ty;
+        if (x >= W || y >= H) return;
         const size_t i = ((size_t)y * W + x) * 3;
         float r = in[i], g = in[i + 1], b = in[i + 2];
         const float guide = lre::max_std(0.0f, lre::max_std(r, lre::max_std(g, b)));
-        const float base = lre::max_std(1e-6f, o0[t] * guide + o1[t]);
+        const float base = lre::max_std(1e-6f, ma * guide + mb);
         float gain = lre::clamp_std(lre::aces(base) / base, 0.0f, 1.0f);
         float w = lre::clamp_std((base - 0.55f) / (0.90f - 0.55f), 0.0f, 1.0f);
         w = w * w * (3.0f - 2.0f * w);
@@ -188,7 +181,7 @@ __global__ __launch_bounds__(DBLOCK) void dtm_apply_kernel(const float *__restri
         out[i] = r;
         out[i + 1] = g;
         out[i + 2] = b;
-    }
+    });
 }
 
 // ---- white balance, core/cpu_accelerator.cpp:2813-2869 ---------------------
@@ -223,6 +216,21 @@ __global__ void rgb3_to_cfa_kernel(const float *__restrict__ rgb3, float *__rest
     }
 }
 
+constexpr size_t DTM_LDS = sizeof(float) * (DIY * DIXP + 2 * DIY * DTXP);
+
+// raise the dynamic LDS limit of both box-filter kernels once per process
+hipError_t dtm_lds_attr() {
+    static const hipError_t rc = [] {
+        hipError_t e = hipFuncSetAttribute((const void *)dtm_ab_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)DTM_LDS);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute((const void *)dtm_apply_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)DTM_LDS);
+        return e;
+    }();
+    return rc;
+}
+
 int grid_for(size_t n, int block) {
     size_t b = (n + block - 1) / block;
     if (b > 65536) b = 65536;
@@ -249,9 +257,10 @@ int lre_launch_detail_tonemap(const float *d_in, float *d_out, int w, int h, voi
     float *pa = static_cast<float *>(d_scratch);
     float *pb = pa + (size_t)w * h;
     dim3 grid((w + DTX - 1) / DTX, (h + DTY - 1) / DTY);
-    hipLaunchKernelGGL(dtm_ab_kernel, grid, dim3(DBLOCK), 0, s, d_in, pa, pb, w, h);
+    LRE_HIP_CHECK(dtm_lds_attr());
+    hipLaunchKernelGGL(dtm_ab_kernel, grid, dim3(DBLOCK), DTM_LDS, s, d_in, pa, pb, w, h);
     LRE_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(dtm_apply_kernel, grid, dim3(DBLOCK), 0, s, pa, pb, d_in, d_out, w, h);
+    hipLaunchKernelGGL(dtm_apply_kernel, grid, dim3(DBLOCK), DTM_LDS, s, pa, pb, d_in, d_out, w, h);
     LRE_HIP_CHECK(hipGetLastError());
     return LRE_OK;
 }
