@@ -33,11 +33,7 @@ __global__ void zero_f32(float *p, size_t n) {
 // max(1e-6, max G) folded into stats[0] (:3823-3830)
 __global__ __launch_bounds__(BLOCK) void axial_max_g(const float *__restrict__ src, size_t n,
                                                      unsigned *stats) {
-    float m = 1e-6f;
-    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK) {
-        const float g = src[3 * i + 1];
-        if (g > m) m = g;
-    }
+    float m = lre::green_max(src, n, 1e-6f, [](float v) { return v; });
     for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
     __shared__ float red[BLOCK / 64];
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;

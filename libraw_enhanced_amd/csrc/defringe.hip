@@ -55,11 +55,7 @@ __global__ void init_stats(unsigned *stats) {
 // max over pixels of max(0, G)  (:3961-3969)
 __global__ __launch_bounds__(BLOCK) void max_green(const float *__restrict__ src, size_t n,
                                                    unsigned *stats) {
-    float m = 0.f;
-    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK) {
-        float g = mx(0.f, src[3 * i + 1]);
-        if (g > m) m = g;
-    }
+    const float m = lre::green_max(src, n, 0.f, [](float v) { return mx(0.f, v); });
     block_max_to(m, &stats[0]);
 }
 
@@ -188,30 +184,31 @@ __global__ __launch_bounds__(BLOCK) void blur_v(BlurSet S, Taps taps, int W, int
 // indices), then the vertical taps.  Each thread accumulates FB_RB outputs
 // along the filter direction; every output sums its taps in the order
 // k = 0 .. 2 ks, as the separable kernels do.
-constexpr int FB_T = 64, FB_KSMAX = 16, FB_RB = 4;
+constexpr int FB_TX = 64, FB_TY = 96, FB_KSMAX = 16, FB_RB = 4;
+constexpr int FB_BLOCK = 512;
 static_assert(FB_KSMAX <= (MAX_TAPS - 1) / 2, "fused blur radius within the tap table");
 
 template <int KS>
-__global__ __launch_bounds__(BLOCK) void blur2d(BlurSet S, Taps taps, int W, int H,
+__global__ __launch_bounds__(FB_BLOCK) void blur2d(BlurSet S, Taps taps, int W, int H,
                                                 const unsigned *stats) {
     extern __shared__ float fb[];
     constexpr int ks = KS, nt = 2 * KS + 1;
-    constexpr int IW = FB_T + 2 * ks, IH = FB_T + 2 * ks;
+    constexpr int IW = FB_TX + 2 * ks, IH = FB_TY + 2 * ks;
     // odd row pitches: lanes that walk down rows hit distinct LDS banks
     constexpr int PW = IW | 1;
-    constexpr int PH = FB_T + 1;
+    constexpr int PH = FB_TX + 1;
     float *win = fb;              // IH x IW input window (pitch PW)
-    float *hor = fb + IH * PW;    // IH x FB_T horizontal result (pitch PH)
+    float *hor = fb + IH * PW;    // IH x FB_TX horizontal result (pitch PH)
     float kt[nt];                 // taps in registers (the loops below unroll fully)
 #pragma unroll
     for (int t = 0; t < nt; t++) kt[t] = taps.v[t];
     const float inv_edge = 1.f / mx(__uint_as_float(stats[1]), 1e-6f);
-    const int x0 = blockIdx.x * FB_T, y0 = blockIdx.y * FB_T;
+    const int x0 = blockIdx.x * FB_TX, y0 = blockIdx.y * FB_TY;
     for (int p = 0; p < S.count; p++) {
         const float *in = S.in[p];
         const bool scaled = S.scale_by_edge[p] != 0;
         __syncthreads();
-        lre::stage2d<BLOCK>(
+        lre::stage2d<FB_BLOCK>(
             IH, IW,
             [&](int r, int c) {
                 int yy = y0 - ks + r, xx = x0 - ks + c;
@@ -221,7 +218,7 @@ __global__ __launch_bounds__(BLOCK) void blur2d(BlurSet S, Taps taps, int W, int
             },
             [&](int r, int c, float v) { win[r * PW + c] = scaled ? v * inv_edge : v; });
         __syncthreads();
-        for (int t = threadIdx.x; t < IH * (FB_T / FB_RB); t += BLOCK) {
+        for (int t = threadIdx.x; t < IH * (FB_TX / FB_RB); t += FB_BLOCK) {
             const int r = t % IH, c = (t / IH) * FB_RB;
             const float *row = win + r * PW + c;
             float acc[FB_RB], w[FB_RB];
@@ -242,8 +239,8 @@ __global__ __launch_bounds__(BLOCK) void blur2d(BlurSet S, Taps taps, int W, int
         }
         __syncthreads();
         float *out = S.out[p];
-        for (int t = threadIdx.x; t < (FB_T / FB_RB) * FB_T; t += BLOCK) {
-            const int c = t % FB_T, r = (t / FB_T) * FB_RB;
+        for (int t = threadIdx.x; t < (FB_TY / FB_RB) * FB_TX; t += FB_BLOCK) {
+            const int c = t % FB_TX, r = (t / FB_TX) * FB_RB;
             const float *col = hor + r * PH + c;
             float acc[FB_RB], w[FB_RB];
 #pragma unroll
@@ -267,6 +264,15 @@ __global__ __launch_bounds__(BLOCK) void blur2d(BlurSet S, Taps taps, int W, int
                 }
         }
     }
+}
+
+// the LDS window of blur2d<KS> exceeds 64 KiB: raise the limit once per radius
+template <int KS>
+hipError_t blur2d_lds_attr() {
+    static const hipError_t rc = hipFuncSetAttribute(
+        (const void *)blur2d<KS>, hipFuncAttributeMaxDynamicSharedMemorySize,
+        (int)(sizeof(float) * ((FB_TY + 2 * KS) * ((FB_TX + 2 * KS) | 1) + (FB_TY + 2 * KS) * (FB_TX + 1))));
+    return rc;
 }
 
 struct Planes {
@@ -683,12 +689,13 @@ int lre_launch_defringe(const float *d_in, float *d_out, int w, int h, float rad
 
     auto blur = [&](BlurSet S) -> int {
         if (ks <= FB_KSMAX) {
-            const int side = FB_T + 2 * ks;
-            const size_t lds = sizeof(float) * ((size_t)side * (side | 1) + (size_t)side * (FB_T + 1));
-            dim3 gb((w + FB_T - 1) / FB_T, (h + FB_T - 1) / FB_T);
+            const size_t lds = sizeof(float) * ((size_t)(FB_TY + 2 * ks) * ((FB_TX + 2 * ks) | 1) +
+                                                (size_t)(FB_TY + 2 * ks) * (FB_TX + 1));
+            dim3 gb((w + FB_TX - 1) / FB_TX, (h + FB_TY - 1) / FB_TY);
             switch (ks) {
 #define LRE_BLUR2D(K) \
-    case K: hipLaunchKernelGGL(blur2d<K>, gb, dim3(BLOCK), lds, s, S, taps, w, h, stats); break;
+    case K: LRE_HIP_CHECK(blur2d_lds_attr<K>()); \
+            hipLaunchKernelGGL(blur2d<K>, gb, dim3(FB_BLOCK), lds, s, S, taps, w, h, stats); break;
                 LRE_BLUR2D(1) LRE_BLUR2D(2) LRE_BLUR2D(3) LRE_BLUR2D(4) LRE_BLUR2D(5) LRE_BLUR2D(6)
                 LRE_BLUR2D(7) LRE_BLUR2D(8) LRE_BLUR2D(9) LRE_BLUR2D(10) LRE_BLUR2D(11)
                 LRE_BLUR2D(12) LRE_BLUR2D(13) LRE_BLUR2D(14) LRE_BLUR2D(15) LRE_BLUR2D(16)

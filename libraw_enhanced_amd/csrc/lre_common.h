@@ -80,6 +80,36 @@ LRE_DEV float aces(float x) {
     return (x * (a * x + b)) / (x * (c * x + d) + e);
 }
 
+// Grid-stride maximum over the green samples of an interleaved RGB image:
+// m = max(m, g(G)) for every pixel, g applied per sample.  Aligned images are
+// read as three float4 per four pixels, two groups per iteration.
+template <class G>
+LRE_DEV float green_max(const float *src, size_t n, float m, G g) {
+    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (size_t)gridDim.x * blockDim.x;
+    size_t done = 0;
+    if ((reinterpret_cast<uintptr_t>(src) & 15) == 0) {
+        const float4 *s4 = reinterpret_cast<const float4 *>(src);
+        const size_t n4 = n / 4;  // groups of four pixels
+        for (size_t q = tid; q < n4; q += 2 * stride) {
+            const size_t q2 = q + stride < n4 ? q + stride : q;
+            const float4 a = s4[3 * q], b = s4[3 * q + 1], c = s4[3 * q + 2];
+            const float4 d = s4[3 * q2], e = s4[3 * q2 + 1], f = s4[3 * q2 + 2];
+            const float v[8] = {a.y, b.x, b.w, c.z, d.y, e.x, e.w, f.z};  // G of pixels 0-3 of both groups
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const float x = g(v[k]);
+                m = x > m ? x : m;
+            }
+        }
+        done = n4 * 4;
+    }
+    for (size_t i = done + tid; i < n; i += stride) {
+        const float x = g(src[3 * i + 1]);
+        m = x > m ? x : m;
+    }
+    return m;
+}
+
 // Pixels of the frame of width `border` around a w x h image, enumerated as
 // the full rows at the top, then at the bottom, then the left and right strips
 // of the rows between (on images narrower than two strips a pixel may appear
