@@ -456,13 +456,31 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
         }
     }
     __syncthreads();
+    // cddiffsq; the Nyquist test reads it at sites only, so the sites' values
+    // also go to the LDS half plane (free until stage 4-5 fills it with hvwt)
     for_rect(4, rr1 - 4, 4, cc1 - 4, [&](int rr, int cc) {
         int i = rr * TS + cc;
-        cdsq[i] = lre::sq(vcd[i] - hcd[i]);
+        const float v = lre::sq(vcd[i] - hcd[i]);
+        cdsq[i] = v;
+        if ((fct(A, rr, cc) & 1) == 0) sm[i >> 1] = v;
     });
     __syncthreads();
 
     // stages 4-5 (:1400-1428)
+    for_sites_split(A, 6, rr1 - 6, 6, cc1 - 6, [&](int rr, int cc) {
+        const int i = rr * TS + cc;
+        const float *d = sm;  // cddiffsq at sites, half-index
+        return
+            (GODD[0] * d[(i) >> 1] + GODD[1] * (d[(i - m1) >> 1] + d[(i + p1) >> 1] + d[(i - p1) >> 1] + d[(i + m1) >> 1]) +
+             GODD[2] * (d[(i - v2) >> 1] + d[(i - 2) >> 1] + d[(i + 2) >> 1] + d[(i + v2) >> 1]) +
+             GODD[3] * (d[(i - m2) >> 1] + d[(i + p2) >> 1] + d[(i - p2) >> 1] + d[(i + m2) >> 1])) -
+            (GGRAD[0] * f_dsq(i) + GGRAD[1] * (f_dsq(i - v1) + f_dsq(i + 1) + f_dsq(i - 1) + f_dsq(i + v1)) +
+             GGRAD[2] * (f_dsq(i - m1) + f_dsq(i + p1) + f_dsq(i - p1) + f_dsq(i + m1)) +
+             GGRAD[3] * (f_dsq(i - v2) + f_dsq(i - 2) + f_dsq(i + 2) + f_dsq(i + v2)) +
+             GGRAD[4] * (f_dsq(i - v2 - 1) + f_dsq(i - v2 + 1) + f_dsq(i - TS - 2) + f_dsq(i - TS + 2) + f_dsq(i + TS - 2) + f_dsq(i + TS + 2) + f_dsq(i + v2 - 1) + f_dsq(i + v2 + 1)) +
+             GGRAD[5] * (f_dsq(i - m2) + f_dsq(i + p2) + f_dsq(i - p2) + f_dsq(i + m2)));
+    }, [&](int rr, int cc, float nt) { nyqtest[(rr * TS + cc) >> 1] = nt; });
+    __syncthreads();  // the hvwt pass below overwrites the LDS half plane
     for_site_pairs(A, 6, rr1 - 6, 6, cc1 - 6, [&](int rr, int cc) {
         const int i = rr * TS + cc;
         float uave = vcd[i] + vcd[i - v1] + vcd[i - v2] + vcd[i - v3];
@@ -492,19 +510,6 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
         hvwt[k] = w;
         sm[k] = w;  // LDS copy for the smoothing pass
     });
-    for_sites_split(A, 6, rr1 - 6, 6, cc1 - 6, [&](int rr, int cc) {
-        const int i = rr * TS + cc;
-        const float *d = cdsq;
-        return
-            (GODD[0] * d[i] + GODD[1] * (d[i - m1] + d[i + p1] + d[i - p1] + d[i + m1]) +
-             GODD[2] * (d[i - v2] + d[i - 2] + d[i + 2] + d[i + v2]) +
-             GODD[3] * (d[i - m2] + d[i + p2] + d[i - p2] + d[i + m2])) -
-            (GGRAD[0] * f_dsq(i) + GGRAD[1] * (f_dsq(i - v1) + f_dsq(i + 1) + f_dsq(i - 1) + f_dsq(i + v1)) +
-             GGRAD[2] * (f_dsq(i - m1) + f_dsq(i + p1) + f_dsq(i - p1) + f_dsq(i + m1)) +
-             GGRAD[3] * (f_dsq(i - v2) + f_dsq(i - 2) + f_dsq(i + 2) + f_dsq(i + v2)) +
-             GGRAD[4] * (f_dsq(i - v2 - 1) + f_dsq(i - v2 + 1) + f_dsq(i - TS - 2) + f_dsq(i - TS + 2) + f_dsq(i + TS - 2) + f_dsq(i + TS + 2) + f_dsq(i + v2 - 1) + f_dsq(i + v2 + 1)) +
-             GGRAD[5] * (f_dsq(i - m2) + f_dsq(i + p2) + f_dsq(i - p2) + f_dsq(i + m2)));
-    }, [&](int rr, int cc, float nt) { nyqtest[(rr * TS + cc) >> 1] = nt; });
     __syncthreads();
     // Nyquist flags start at zero (:1432); the clear also wipes what the tile
     // fill wrote past the end of the CFA array
