@@ -217,7 +217,7 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
     float *sm = lds + LDS_CFA;
     unsigned *nqm = reinterpret_cast<unsigned *>(sm + LDS_HALF), *nqm2 = nqm + NYQ_WORDS;
     auto nbit = [](const unsigned *m, int k) { return (m[k >> 5] >> (k & 31)) & 1u; };
-    float *wv = ar + O_WV, *wh = ar + O_WH, *dsq = ar + O_DSQ;
+    float *wh = ar + O_WH, *dsq = ar + O_DSQ;
     float *vcd = ar + O_VCD, *hcd = ar + O_HCD, *vcdalt = ar + O_VCDALT, *hcdalt = ar + O_HCDALT;
     float *dgv = ar + O_DGV, *dgh = ar + O_DGH, *cdsq = ar + O_CDSQ, *hvwt = ar + O_HVWT;
     float *nyqtest = ar + O_NYQTEST;
@@ -324,9 +324,10 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
     __syncthreads();
 
     // stage 1 (:1331-1339).  Its planes are functions of the LDS CFA plane:
-    // stages 2 and 4-5 recompute the values they need from LDS instead of
-    // loading them (bit-identical); the planes are still stored, because later
-    // stages read wh after pmwt overwrote part of it, and rbint over dsq.
+    // every reader recomputes the values from LDS (bit-identical) except the
+    // R/B refinement, which reads dirwts1 after pmwt (stored over its first
+    // half) has replaced part of it; so only wh is stored.  The dsq cells the
+    // refinement reads as rbint are all rewritten before.
     auto f_dh = [&](int i) { return fabsf(cfa[i + 1] - cfa[i - 1]); };
     auto f_dv = [&](int i) { return fabsf(cfa[i + v1] - cfa[i - v1]); };
     auto f_wv = [&](int i) { return EPS + fabsf(cfa[i + v2] - cfa[i]) + fabsf(cfa[i] - cfa[i - v2]) + f_dv(i); };
@@ -337,9 +338,7 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
     };
     for_rect(2, rr1 - 2, 2, cc1 - 2, [&](int rr, int cc) {
         int i = rr * TS + cc;
-        wv[i] = f_wv(i);
         wh[i] = f_wh(i);
-        dsq[i] = f_dsq(i);
     });
     __syncthreads();
 
@@ -617,30 +616,15 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
         __syncthreads();
     }
 
-    // stage 7 (:1484-1502)
-    float *delp = cdsq, *delm = cdsq + TS * TSH, *q1p = ar + O_Q1P, *q1m = ar + O_Q1M;
-    for_rect(6, rr1 - 6, 0, (cc1 - 6 - 6 + 1) / 2, [&](int rr, int j) {
-        int cc = 6 + 2 * j;
-        int i = rr * TS + cc;
-        if ((fct(A, rr, 2) & 1) == 0) {
-            delp[i >> 1] = fabsf(cfa[i + p1] - cfa[i - p1]);
-            delm[i >> 1] = fabsf(cfa[i + m1] - cfa[i - m1]);
-            q1p[i >> 1] = lre::sq(cfa[i + 1] - cfa[i + 1 - p1]) + lre::sq(cfa[i + 1] - cfa[i + 1 + p1]);
-            q1m[i >> 1] = lre::sq(cfa[i + 1] - cfa[i + 1 - m1]) + lre::sq(cfa[i + 1] - cfa[i + 1 + m1]);
-        } else {
-            q1p[i >> 1] = lre::sq(cfa[i] - cfa[i - p1]) + lre::sq(cfa[i] - cfa[i + p1]);
-            q1m[i >> 1] = lre::sq(cfa[i] - cfa[i - m1]) + lre::sq(cfa[i] - cfa[i + m1]);
-            delp[i >> 1] = fabsf(cfa[i + 1 + p1] - cfa[i + 1 - p1]);
-            delm[i >> 1] = fabsf(cfa[i + 1 + m1] - cfa[i + 1 - m1]);
-        }
-    });
-    __syncthreads();
+    // stage 7 (:1484-1502) computes the diagonal gradients and q1 sums that
+    // stage 8 reads; stage 8 recomputes them from LDS, and no later stage reads
+    // those cells, so the stage has no pass of its own.
 
     // stage 8 (:1505-1532).  The stage-7 cells it reads are functions of the
     // LDS CFA plane and lie inside stage 7's range: the diagonal gradients of a
     // site s, and the q1 sums at a green n (the diagonal neighbours of a site
-    // are sites, its axial and knight neighbours greens).  They are recomputed
-    // here rather than loaded (bit-identical).
+    // are sites, its axial and knight neighbours greens).  They are computed
+    // here (bit-identical to the stored cells).
     auto f_delp = [&](int s) { return fabsf(cfa[s + p1] - cfa[s - p1]); };
     auto f_delm = [&](int s) { return fabsf(cfa[s + m1] - cfa[s - m1]); };
     auto f_q1p = [&](int n) { return lre::sq(cfa[n] - cfa[n - p1]) + lre::sq(cfa[n] - cfa[n + p1]); };
@@ -726,7 +710,7 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
         float gd = fabsf(1.f - rd) < ARTHRESH ? rb * rd : cfa[i + v1] + half(rb - rbint[k + v1]);
         float gl = fabsf(1.f - rl) < ARTHRESH ? rb * rl : cfa[i - 1] + half(rb - rbint[k - 1]);
         float gr = fabsf(1.f - rr_) < ARTHRESH ? rb * rr_ : cfa[i + 1] + half(rb - rbint[k + 1]);
-        float gv = (wv[i - v1] * gd + wv[i + v1] * gu) / (wv[i + v1] + wv[i - v1]);
+        float gv = (f_wv(i - v1) * gd + f_wv(i + v1) * gu) / (f_wv(i + v1) + f_wv(i - v1));
         float gh = (wh[i - 1] * gr + wh[i + 1] * gl) / (wh[i - 1] + wh[i + 1]);
         const float up = cfa[i - v1], dn = cfa[i + v1], lf = cfa[i - 1], rt = cfa[i + 1];
         if (gv < rb) {
