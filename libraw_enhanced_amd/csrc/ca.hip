@@ -60,9 +60,11 @@ struct AxialArgs {
 // Horizontal FP64 window sums over LDS rows [0, RY) into hs[RY][AX_HP], for
 // output columns x0 .. x0+AX_TX-1; region column 0 is image column x0 - r - 1
 // and region values outside the image are 0.  q(ry, c) gives the summed value
-// at region column c.  Each thread sums 4 neighbouring outputs of one row with a
-// sliding window; lanes walk down rows (odd pitches: distinct LDS banks).  Every
-// output adds its taps in order.
+// at region column c.  Each thread sums 4 neighbouring outputs of one row: the
+// first window in full, the next ones by adding the entering and subtracting
+// the leaving tap (FP64, so the result stays well inside the tolerance of the
+// reference's FP64 summed-area tables); lanes walk down rows (odd pitches:
+// distinct LDS banks).
 constexpr int AX_HP = 64 + 1;  // hs pitch in doubles
 constexpr int AX_RB = 4;
 
@@ -72,39 +74,32 @@ LRE_DEV void hsum(Q q, int RY, int r, double *hs) {
     for (int t = threadIdx.x; t < RY * (AX_TX / AX_RB); t += AX_BLOCK) {
         const int ry = t % RY, cx = (t / RY) * AX_RB;
         const int c0 = cx + 1;  // region col cx+1 = image col x0+cx-r
-        double s[AX_RB];
-        float w[AX_RB];
+        double s = 0.0;
+        for (int k = 0; k < nt; k++) s += (double)q(ry, c0 + k);
+        hs[ry * AX_HP + cx] = s;
 #pragma unroll
-        for (int u = 0; u < AX_RB; u++) { s[u] = 0.0; w[u] = q(ry, c0 + u); }
-        for (int k = 0; k < nt; k++) {
-#pragma unroll
-            for (int u = 0; u < AX_RB; u++) s[u] += (double)w[u];
-            if (k + 1 == nt) break;
-#pragma unroll
-            for (int u = 0; u < AX_RB - 1; u++) w[u] = w[u + 1];
-            w[AX_RB - 1] = q(ry, c0 + k + AX_RB);
+        for (int u = 1; u < AX_RB; u++) {
+            s += (double)q(ry, c0 + u - 1 + nt) - (double)q(ry, c0 + u - 1);
+            hs[ry * AX_HP + cx + u] = s;
         }
-#pragma unroll
-        for (int u = 0; u < AX_RB; u++) hs[ry * AX_HP + cx + u] = s[u];
     }
 }
 
-// Vertical sums of hs for output rows oy .. oy+3 of column cx (each in order),
-// divided by the true tap count of the clipped window.
+// Vertical sums of hs for output rows oy .. oy+3 of column cx (a full window,
+// then entering minus leaving taps), divided by the true tap count of the
+// clipped window.
 LRE_DEV void vmean(const double *hs, int r, int cx, int oy, int x0, int y0, int W, int H,
                    double out[AX_RB]) {
     const int nt = 2 * r + 1;
     const double *col = hs + oy * AX_HP + cx;
-    double s[AX_RB], w[AX_RB];
+    double s[AX_RB];
+    double acc = 0.0;
+    for (int j = 0; j < nt; j++) acc += col[j * AX_HP];
+    s[0] = acc;
 #pragma unroll
-    for (int u = 0; u < AX_RB; u++) { s[u] = 0.0; w[u] = col[u * AX_HP]; }
-    for (int j = 0; j < nt; j++) {
-#pragma unroll
-        for (int u = 0; u < AX_RB; u++) s[u] += w[u];
-        if (j + 1 == nt) break;
-#pragma unroll
-        for (int u = 0; u < AX_RB - 1; u++) w[u] = w[u + 1];
-        w[AX_RB - 1] = col[(j + AX_RB) * AX_HP];
+    for (int u = 1; u < AX_RB; u++) {
+        acc += col[(u - 1 + nt) * AX_HP] - col[(u - 1) * AX_HP];
+        s[u] = acc;
     }
     const int x = x0 + cx;
     const int xl = max(0, x - r), xr = min(W - 1, x + r);
