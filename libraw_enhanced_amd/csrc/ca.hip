@@ -41,7 +41,9 @@ __global__ __launch_bounds__(BLOCK) void axial_max_g(const float *__restrict__ s
     if (threadIdx.x == 0) {
         float v = red[0];
         for (int k = 1; k < BLOCK / 64; k++) v = fmaxf(v, red[k]);
-        atomicMax(stats, __float_as_uint(v));
+        // the maximum only grows: skip the contended atomic below a stored value
+        const unsigned u = __float_as_uint(v);
+        if (u > __hip_atomic_load(stats, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(stats, u);
     }
 }
 

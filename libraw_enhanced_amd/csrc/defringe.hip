@@ -44,7 +44,10 @@ LRE_DEV void block_max_to(float v, unsigned *dst) {
     if (threadIdx.x == 0) {
         float m = red[0];
         for (int k = 1; k < BLOCK / 64; k++) m = fmaxf(m, red[k]);
-        atomicMax(dst, __float_as_uint(m));
+        // the maximum only grows, so a block below a value already stored
+        // (even a stale one) leaves it unchanged: skip the contended atomic
+        const unsigned u = __float_as_uint(m);
+        if (u > __hip_atomic_load(dst, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(dst, u);
     }
 }
 

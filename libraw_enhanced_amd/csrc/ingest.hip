@@ -76,7 +76,9 @@ __global__ __launch_bounds__(BLOCK) void ingest_kernel(const unsigned short *__r
     if (threadIdx.x == 0 && dmax) {
         float v = red[0];
         for (int k = 1; k < BLOCK / 64; k++) v = fmaxf(v, red[k]);
-        atomicMax(dmax, __float_as_uint(v));
+        // the maximum only grows: skip the contended atomic below a stored value
+        const unsigned u = __float_as_uint(v);
+        if (u > __hip_atomic_load(dmax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(dmax, u);
     }
 }
 
