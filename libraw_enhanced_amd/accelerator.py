@@ -201,7 +201,9 @@ class HIPAccelerator:
     def demosaic_xtrans_3pass(self, raw_rgb, xtrans, maximum_value: float = 1.0):
         raw = _rgb(raw_rgb, "raw_buffer")
         h, w, _ = raw.shape
-        out = np.empty_like(raw)
+        # a new zeroed buffer, as the reference's pipeline allocates: frames
+        # with a side of 22 px or less have pixels no tile writes
+        out = np.zeros_like(raw)
         xt = np.ascontiguousarray(np.asarray(xtrans).reshape(-1), np.int8)
         _check(native().lre_hip_xtrans3(self._h, _p(raw), _p(out), w, h, _p(xt, _i8p), maximum_value))
         return out

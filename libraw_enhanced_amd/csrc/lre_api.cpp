@@ -339,7 +339,10 @@ int lre_hip_xtrans3(lre_ctx *ctx, const float *raw_rgb, float *rgb, int w, int h
     void *d_ws = static_cast<char *>(ctx->scratch) + c.off;
     if ((rc = copy_in(ctx, d_in, raw_rgb, n * 12))) return rc;
     if ((rc = lre_launch_rgb3_to_cfa(d_in, d_cfa, w, h, 9, xtrans, ctx->stream))) return rc;
-    LRE_HIP_CHECK(hipMemcpyAsync(d_out, d_in, n * 12, hipMemcpyDeviceToDevice, ctx->stream));
+    // The reference writes the 8-px frame and whatever its tiles cover; with no
+    // tile (a side of 22 px or less) the pixels inside the frame keep the
+    // caller's values, so those go to the device first.
+    if (lre_xtrans_tiles(w, h) == 0 && (rc = copy_in(ctx, d_out, rgb, n * 12))) return rc;
     if ((rc = lre_launch_xtrans3(ctx, d_cfa, d_out, w, h, xtrans, maximum_value, d_ws, ctx->stream)))
         return rc;
     return copy_out(ctx, rgb, d_out, n * 12);

@@ -336,3 +336,41 @@ def test_scale_aber_bit_exact(acc, aber):
     got = acc.scale_aber(img.copy(), aber)
     assert np.array_equal(got, want)
     assert np.count_nonzero(want != img) > 0
+
+
+# ---------------- tiny and ragged frames ----------------
+# Frames smaller than a tile, one pixel wide or tall, and odd sizes: the tile
+# walks, the border passes and the clamped window loads must still match.
+
+@pytest.mark.parametrize("h,w", [(1, 1), (2, 3), (5, 7), (17, 9), (31, 33), (1, 200), (200, 2)])
+def test_amaze_tiny_frames(acc, h, w):
+    print(f"amaze tiny {h}x{w}", flush=True)
+    raw3 = _bayer(h, w, scene.BASE_SEED + 3 * h + w)
+    got = acc.demosaic_bayer_amaze(raw3, RGGB, (1, 1, 1, 1), 1.0)
+    want = oracle_amaze(raw3)
+    assert np.array_equal(got, want), f"max-abs {np.abs(got - want).max()}"
+
+
+@pytest.mark.parametrize("h,w", [(1, 1), (6, 6), (21, 30), (23, 47), (40, 200), (200, 23)])
+def test_xtrans_tiny_frames(acc, h, w):
+    print(f"xtrans tiny {h}x{w}", flush=True)
+    raw3 = _xtrans_raw3(h, w, scene.BASE_SEED + 5 * h + w)
+    got = acc.demosaic_xtrans_3pass(raw3, scene.XTRANS, 1.0)
+    want = _oracle_xtrans(raw3)
+    assert np.array_equal(got, want), f"max-abs {np.abs(got - want).max()}"
+
+
+@pytest.mark.parametrize("h,w", [(1, 1), (3, 5), (1, 64), (40, 3), (33, 47)])
+def test_rgb_stages_tiny_frames(acc, h, w):
+    print(f"rgb stages tiny {h}x{w}", flush=True)
+    img = _ca_scene(h, w)
+    checks = [
+        (_gpu_defringe(acc, img, False), _oracle_defringe(img, False)),
+        (_gpu_defringe(acc, img, True), _oracle_defringe(img, True)),
+        (acc.ca_axial_cleanup(img, 6, 1e-4, 0.3), _axial(oracle(), "oracle_axial_ca", img)),
+        (acc.ca_register_lateral(img, 96, 3, 6.0, 0.02, 3), _lateral(oracle(), "oracle_lateral_ca", img)),
+        (acc.apply_detail_preserving_tonemap(img), oracle_detail_tonemap(img)),
+    ]
+    for k, (got, want) in enumerate(checks):
+        err = float(np.abs(got - want).max())
+        assert err <= 2e-4, (k, err)
