@@ -778,6 +778,9 @@ int serve_one(lre_server *S, Lane &L, const lre_request &rq, char *dev, const Fo
         if (im.filters == 9) {
             const int t = lre_xtrans_multi_fill(&j, 1, h_tab);
             if (t < 0) return t;
+            // no tile on a frame with a side of 22 px or less: the pixels inside
+            // the 8-px frame stay as in a new (zeroed) output buffer
+            if (t == 0) LRE_HIP_CHECK(hipMemsetAsync(d_rgb, 0, px * 12, s));
             LRE_HIP_CHECK(hipMemcpyAsync(d_tab, h_tab, lre_xtrans_multi_table_bytes(1), hipMemcpyHostToDevice, s));
             if ((rc = lre_launch_xtrans_multi(&j, 1, h_tab, d_tab, t, slots, d_arena, s))) return rc;
         } else {
