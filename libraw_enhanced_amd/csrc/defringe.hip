@@ -30,6 +30,77 @@ LRE_DEV float sstep(float e0, float e1, float x) {  // :3322-3325
     float t = lre::clamp_std((x - e0) / (e1 - e0), 0.0f, 1.0f);
     return t * t * (3.0f - 2.0f * t);
 }
+
+// Smoothstep edges known before the launch: the ramp starts at e0 and is d
+// wide; y is 1/d rounded to nearest.
+struct Edge {
+    float e0, d, y;
+};
+
+// n / d correctly rounded, from y = RN(1/d): the two residual corrections of
+// the hardware division expansion (q = n y; r = n - d q; q += r y; twice)
+// without its operand scaling, which is needed only near the ends of the
+// exponent range.  Callers keep 2^-20 <= d <= 2^100 and use the quotient for
+// 0 < n < d only; in that range the sequence is the IEEE quotient (checked
+// against division for every numerator of two binades and 20000 divisors,
+// and for every numerator in [d 2^-76, 2d] of the divisors used here).
+LRE_DEV float quot(float n, float d, float y) {
+    float q = n * y;
+    float r = __builtin_fmaf(-d, q, n);
+    q = __builtin_fmaf(r, y, q);
+    r = __builtin_fmaf(-d, q, n);
+    return __builtin_fmaf(r, y, q);
+}
+
+// smoothstep over a precomputed Edge.  Outside (0, d) the clamp decides
+// (n >= d -> 1, n <= 0 -> 0, NaN stays NaN), as clamp(n / d, 0, 1) does; a
+// t below 2^-76 gives 0 either way, since t * t underflows.
+template <bool FAST>
+LRE_DEV float sstep(const Edge &e, float x) {
+    const float n = x - e.e0;
+    float t;
+    if (FAST) {
+        t = quot(n, e.d, e.y);
+        t = n >= e.d ? 1.0f : t;
+        t = n <= 0.0f ? 0.0f : t;
+    } else {
+        t = lre::clamp_std(n / e.d, 0.0f, 1.0f);
+    }
+    return t * t * (3.0f - 2.0f * t);
+}
+
+// smoothstep with literal edges: d and 1/d fold at compile time
+LRE_DEV float sstepc(float e0, float e1, float x) {
+    const Edge e{e0, e1 - e0, 1.0f / (e1 - e0)};
+    return sstep<true>(e, x);
+}
+
+// Edges that depend on the call's parameters (isens, edge and chroma
+// thresholds, mask cutoff), formed on the host with the reference's float
+// products in its order.
+enum Gate {
+    G_EDGE,          // et 0.35 i .. et 1.50 i, on the blurred normalised edge
+    G_EDGE_RATIO,    // 0.55 i .. 1.35 i, on the blurred raw edge
+    G_EDGE_LOW,      // et 0.10 i .. et 0.80 i
+    G_RATIO_LOW,     // 0.20 i .. 0.90 i
+    G_BRIGHT,        // 0.24 i .. 0.52 i
+    G_CHROMA,        // ct 0.45 i .. ct 1.60 i
+    G_BAL_HUE,       // 0.06 i .. 0.22 i
+    G_DIR,           // 0.035 i .. 0.16 i
+    G_GREEN_SOFT,    // 0.05 i .. 0.20 i
+    G_SOLO,          // ct 0.22 i .. ct 0.95 i
+    G_SOLO_LOCAL,    // ct 0.12 i .. ct 0.65 i
+    G_LOCAL,         // ct 0.10 i .. ct 0.55 i
+    G_ABS,           // ct 0.35 i .. ct 1.20 i
+    G_LOW_CONTRAST,  // ct 1.20 i .. ct 2.40 i
+    G_PEAK,          // ct 0.80 i .. ct 2.50 i
+    G_HALO,          // ct 0.55 i .. ct 1.65 i
+    G_MASK,          // mask_cutoff .. 5 mask_cutoff
+    G_COUNT
+};
+struct Gates {
+    Edge g[G_COUNT];
+};
 LRE_DEV float mx(float a, float b) { return lre::max_std(a, b); }
 LRE_DEV float mn(float a, float b) { return lre::min_std(a, b); }
 LRE_DEV float expf_cr(float x) { return (float)exp((double)x); }
@@ -285,9 +356,10 @@ struct Planes {
 };
 
 // fringe support seed (:4162-4248)
-__global__ __launch_bounds__(BLOCK) void support(Planes P, Scal C, const unsigned *stats) {
+template <bool FAST>
+__global__ __launch_bounds__(BLOCK) void support(Planes P, Scal C, Gates G) {
     const size_t n = (size_t)C.W * C.H;
-    const float isens = C.isens, et = C.edge_thr, ct = C.chroma_thr;
+    const auto gate = [&](Gate k, float x) { return sstep<FAST>(G.g[k], x); };
     for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK) {
         const float rg = P.lrg[i], bg = P.lbg[i];
         const float dr = rg - P.brg[i], db = bg - P.bbg[i];
@@ -295,28 +367,28 @@ __global__ __launch_bounds__(BLOCK) void support(Planes P, Scal C, const unsigne
         const float purple_excess = mn(mx(0.f, dr), mx(0.f, db));
         const float denom = mx(P.bg[i], 0.05f);
         const float g_over_min = mx(0.f, g0 - mn(r0, b0)) / denom;
-        const float green_dom = sstep(0.010f, 0.060f, g_over_min);
+        const float green_dom = sstepc(0.010f, 0.060f, g_over_min);
         const float green_excess = mx(mx(0.f, -dr), mx(0.f, -db)) * green_dom;
         const bool is_purple = purple_excess >= green_excess;
         const float excess = is_purple ? purple_excess : green_excess;
         const float min_dir = is_purple ? mx(0.f, mn(rg, bg)) : mx(0.f, mn(-rg, -bg));
         const float max_dir = is_purple ? mx(1e-4f, mx(rg, bg)) : mx(1e-4f, mx(-rg, -bg));
         const float balance = min_dir / max_dir;
-        const float c_edge = sstep(et * 0.35f * isens, et * 1.50f * isens, P.be[i]) *
-                             sstep(0.55f * isens, 1.35f * isens, P.ber[i]);
-        const float c_bright = sstep(0.24f * isens, 0.52f * isens, P.bg[i]);
-        const float c_chroma = sstep(ct * 0.45f * isens, ct * 1.60f * isens, excess);
+        const float c_edge = gate(G_EDGE, P.be[i]) *
+                             gate(G_EDGE_RATIO, P.ber[i]);
+        const float c_bright = gate(G_BRIGHT, P.bg[i]);
+        const float c_chroma = gate(G_CHROMA, excess);
         const float red_strength = mx(0.f, r0 - mx(g0, b0)) / mx(r0, 1e-4f);
         const float rb_balance = b0 / mx(r0, 1e-4f);
         const float blue_lift = mx(0.f, b0 - g0) / mx(r0 - g0, 1e-4f);
-        const float red_purple = mx(sstep(0.55f, 0.78f, rb_balance), sstep(0.16f, 0.34f, blue_lift));
-        const float rp_signal = sstep(0.50f, 0.72f, rb_balance) * sstep(0.24f, 0.42f, blue_lift);
-        const float balanced_hue = sstep(0.06f * isens, 0.22f * isens, min_dir) * sstep(0.35f, 0.75f, balance);
-        const float rp_hue = is_purple ? mn(1.f, rp_signal * 1.50f) * sstep(0.035f * isens, 0.16f * isens, min_dir) *
-                                             sstep(0.18f, 0.58f, balance)
+        const float red_purple = mx(sstepc(0.55f, 0.78f, rb_balance), sstepc(0.16f, 0.34f, blue_lift));
+        const float rp_signal = sstepc(0.50f, 0.72f, rb_balance) * sstepc(0.24f, 0.42f, blue_lift);
+        const float balanced_hue = gate(G_BAL_HUE, min_dir) * sstepc(0.35f, 0.75f, balance);
+        const float rp_hue = is_purple ? mn(1.f, rp_signal * 1.50f) * gate(G_DIR, min_dir) *
+                                             sstepc(0.18f, 0.58f, balance)
                                        : 0.f;
         const float c_hue = mx(balanced_hue, rp_hue);
-        const float red_guard = 1.f - sstep(0.08f, 0.25f, red_strength) * (1.f - red_purple);
+        const float red_guard = 1.f - sstepc(0.08f, 0.25f, red_strength) * (1.f - red_purple);
         const float green_enable = (C.enable_green || is_purple) ? 1.f : 0.f;
         P.fs[i] = c_edge * c_bright * c_hue * c_chroma * red_guard * green_enable;
     }
@@ -373,9 +445,10 @@ __global__ __launch_bounds__(BLOCK) void line_link(Planes P, int W, int H) {
 }
 
 // solo seeds (:4325-4376)
-__global__ __launch_bounds__(BLOCK) void solo_seed(Planes P, Scal C) {
+template <bool FAST>
+__global__ __launch_bounds__(BLOCK) void solo_seed(Planes P, Scal C, Gates G) {
     const size_t n = (size_t)C.W * C.H;
-    const float isens = C.isens, et = C.edge_thr, ct = C.chroma_thr;
+    const auto gate = [&](Gate k, float x) { return sstep<FAST>(G.g[k], x); };
     for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK) {
         const float s0 = P.src[3 * i], s1 = P.src[3 * i + 1], s2 = P.src[3 * i + 2];
         const float g = mx(s1, 0.f);
@@ -383,25 +456,27 @@ __global__ __launch_bounds__(BLOCK) void solo_seed(Planes P, Scal C) {
         const float tb = g * expf_cr(P.bbg[i]);
         const float solo_r = mx(0.f, s0 - tr) / mx(P.bg[i], 0.05f);
         const float solo_b = mx(0.f, s2 - tb) / mx(P.bg[i], 0.05f);
-        const float linked = sstep(0.004f, 0.045f, mx(P.bfs[i], P.lfs[i] * 0.90f));
-        const float c_edge = sstep(et * 0.35f * isens, et * 1.50f * isens, P.be[i]) *
-                             sstep(0.55f * isens, 1.35f * isens, P.ber[i]);
-        const float c_bright = sstep(0.24f * isens, 0.52f * isens, P.bg[i]);
+        const float linked = sstepc(0.004f, 0.045f, mx(P.bfs[i], P.lfs[i] * 0.90f));
+        const float c_edge = gate(G_EDGE, P.be[i]) *
+                             gate(G_EDGE_RATIO, P.ber[i]);
+        const float c_bright = gate(G_BRIGHT, P.bg[i]);
         const float r0 = mx(s0, 0.f), g0 = mx(s1, 0.f), b0 = mx(s2, 0.f);
         const float red_strength = mx(0.f, r0 - mx(g0, b0)) / mx(r0, 1e-4f);
-        const float red_guard = 1.f - 0.90f * sstep(0.12f, 0.32f, red_strength);
+        const float red_guard = 1.f - 0.90f * sstepc(0.12f, 0.32f, red_strength);
         P.sr[i] = C.solo_enable * c_edge * c_bright * linked * red_guard *
-                  sstep(ct * 0.22f * isens, ct * 0.95f * isens, solo_r);
-        P.sb[i] = C.solo_enable * c_edge * c_bright * sstep(ct * 0.22f * isens, ct * 0.95f * isens, solo_b);
+                  gate(G_SOLO, solo_r);
+        P.sb[i] = C.solo_enable * c_edge * c_bright * gate(G_SOLO, solo_b);
     }
 }
 
 // main correction and solo cleanup (:4377-4987)
-__global__ __launch_bounds__(BLOCK) void correct(Planes P, Scal C, const unsigned *stats) {
+template <bool FAST>
+__global__ __launch_bounds__(BLOCK) void correct(Planes P, Scal C, Gates G, const unsigned *stats) {
     const size_t n = (size_t)C.W * C.H;
-    const float isens = C.isens, et = C.edge_thr, ct = C.chroma_thr, strength = C.strength;
+    const float strength = C.strength;
+    const auto gate = [&](Gate k, float x) { return sstep<FAST>(G.g[k], x); };
     const bool eg = C.enable_green != 0;
-    const float scene_gate = sstep(0.55f, 0.75f, max_guide_of(stats));
+    const float scene_gate = sstepc(0.55f, 0.75f, max_guide_of(stats));
     const float mask_cutoff = C.mask_cutoff;
     for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK) {
         const float s0 = P.src[3 * i], s1 = P.src[3 * i + 1], s2 = P.src[3 * i + 2];
@@ -413,28 +488,28 @@ __global__ __launch_bounds__(BLOCK) void correct(Planes P, Scal C, const unsigne
         const float purple_excess = mn(mx(0.f, dr), mx(0.f, db));
         const float denom = mx(bguide, 0.05f);
         const float g_over_min = mx(0.f, g0 - mn(r0, b0)) / denom;
-        const float green_dom = sstep(0.010f, 0.060f, g_over_min);
+        const float green_dom = sstepc(0.010f, 0.060f, g_over_min);
         const float green_excess = mx(mx(0.f, -dr), mx(0.f, -db)) * green_dom;
         const bool is_purple = purple_excess >= green_excess;
         const float excess = is_purple ? purple_excess : green_excess;
         const float min_dir = is_purple ? mx(0.f, mn(rg, bg)) : mx(0.f, mn(-rg, -bg));
         const float max_dir = is_purple ? mx(1e-4f, mx(rg, bg)) : mx(1e-4f, mx(-rg, -bg));
         const float balance = min_dir / max_dir;
-        const float c_edge_core = sstep(et * 0.35f * isens, et * 1.50f * isens, P.be[i]) *
-                                  sstep(0.55f * isens, 1.35f * isens, P.ber[i]);
-        const float c_edge_low = sstep(et * 0.10f * isens, et * 0.80f * isens, P.be[i]) *
-                                 sstep(0.20f * isens, 0.90f * isens, P.ber[i]);
-        const float linked = sstep(0.004f, 0.045f, mx(bfs, lfs * 0.90f));
-        const float c_bright = sstep(0.24f * isens, 0.52f * isens, bguide);
-        const float c_chroma = sstep(ct * 0.45f * isens, ct * 1.60f * isens, excess);
+        const float c_edge_core = gate(G_EDGE, P.be[i]) *
+                                  gate(G_EDGE_RATIO, P.ber[i]);
+        const float c_edge_low = gate(G_EDGE_LOW, P.be[i]) *
+                                 gate(G_RATIO_LOW, P.ber[i]);
+        const float linked = sstepc(0.004f, 0.045f, mx(bfs, lfs * 0.90f));
+        const float c_bright = gate(G_BRIGHT, bguide);
+        const float c_chroma = gate(G_CHROMA, excess);
         const float rb_balance_hue = b0 / mx(r0, 1e-4f);
         const float blue_lift_hue = mx(0.f, b0 - g0) / mx(r0 - g0, 1e-4f);
-        const float rp_signal = sstep(0.50f, 0.72f, rb_balance_hue) * sstep(0.24f, 0.42f, blue_lift_hue);
-        const float balanced_hue = sstep(0.06f * isens, 0.22f * isens, min_dir) * sstep(0.35f, 0.75f, balance);
-        const float rp_gate = is_purple ? mn(1.f, rp_signal * 1.50f) * sstep(0.035f * isens, 0.16f * isens, min_dir) *
-                                              sstep(0.18f, 0.58f, balance)
+        const float rp_signal = sstepc(0.50f, 0.72f, rb_balance_hue) * sstepc(0.24f, 0.42f, blue_lift_hue);
+        const float balanced_hue = gate(G_BAL_HUE, min_dir) * sstepc(0.35f, 0.75f, balance);
+        const float rp_gate = is_purple ? mn(1.f, rp_signal * 1.50f) * gate(G_DIR, min_dir) *
+                                              sstepc(0.18f, 0.58f, balance)
                                         : 0.f;
-        const float green_unbal = (eg && !is_purple) ? sstep(0.035f * isens, 0.16f * isens, max_dir) : 0.f;
+        const float green_unbal = (eg && !is_purple) ? gate(G_DIR, max_dir) : 0.f;
         const float c_hue = mx(balanced_hue, mx(rp_gate, green_unbal));
         const float purple_abs = mn(mx(0.f, r0 - g0), mx(0.f, b0 - g0)) / mx(bguide, 0.05f);
         const float green_abs_base = mx(0.f, g0 - rb_avg) / mx(bguide, 0.05f);
@@ -442,9 +517,9 @@ __global__ __launch_bounds__(BLOCK) void correct(Planes P, Scal C, const unsigne
         const float green_abs = mx(green_abs_base, 0.55f * green_abs_alt);
         const bool abs_is_purple = purple_abs >= green_abs;
         const float abs_excess = abs_is_purple ? purple_abs : green_abs;
-        const float c_abs = sstep(ct * 0.35f * isens, ct * 1.20f * isens, abs_excess);
-        const float low_contrast = sstep(ct * 1.20f * isens, ct * 2.40f * isens, abs_excess);
-        const float green_soft = (eg && !is_purple) ? c_edge_low * sstep(0.05f * isens, 0.20f * isens, max_dir) : 0.f;
+        const float c_abs = gate(G_ABS, abs_excess);
+        const float low_contrast = gate(G_LOW_CONTRAST, abs_excess);
+        const float green_soft = (eg && !is_purple) ? c_edge_low * gate(G_GREEN_SOFT, max_dir) : 0.f;
         const float c_edge = mx(c_edge_core, mx(c_edge_low * linked * 1.45f, mx(c_edge_low * low_contrast * 0.35f, green_soft)));
         const float ratio_mask = c_edge * c_bright * c_hue * c_chroma;
         const float direct_mask = c_edge * c_bright * c_hue * c_abs;
@@ -452,35 +527,35 @@ __global__ __launch_bounds__(BLOCK) void correct(Planes P, Scal C, const unsigne
         float mask = mx(ratio_mask, direct_mask) * scene_gate;
         const bool correct_purple = use_direct ? abs_is_purple : is_purple;
         if (correct_purple) {
-            mask *= sstep(ct * 0.10f * isens, ct * 0.55f * isens, purple_abs);
+            mask *= gate(G_LOCAL, purple_abs);
         } else {
-            const float local_gate = sstep(ct * 0.10f * isens, ct * 0.55f * isens, green_abs);
+            const float local_gate = gate(G_LOCAL, green_abs);
             if (eg) {
-                const float peak_chroma = sstep(ct * 0.80f * isens, ct * 2.50f * isens, green_abs);
-                const float peak_nb = sstep(0.005f, 0.030f, bfs);
+                const float peak_chroma = gate(G_PEAK, green_abs);
+                const float peak_nb = sstepc(0.005f, 0.030f, bfs);
                 mask = mx(mask, peak_chroma * peak_nb * c_bright * c_hue * scene_gate);
             }
-            const float hl_protect = 1.f - 0.85f * sstep(0.62f, 0.88f, bguide);
+            const float hl_protect = 1.f - 0.85f * sstepc(0.62f, 0.88f, bguide);
             const float rb_to_green = rb_avg / mx(g0, 1e-4f);
-            const float natural_hl = sstep(0.22f, 0.46f, rb_to_green) * sstep(0.30f, 0.55f, g0);
+            const float natural_hl = sstepc(0.22f, 0.46f, rb_to_green) * sstepc(0.30f, 0.55f, g0);
             const float nat_suppress = 1.f - 0.99f * natural_hl;
             const float nat_suppress_sq = nat_suppress * nat_suppress;
-            const float halo = sstep(ct * 0.55f * isens, ct * 1.65f * isens, green_abs);
+            const float halo = gate(G_HALO, green_abs);
             const float green_gain = (eg ? 0.60f : 0.35f) + (eg ? 1.10f : 0.95f) * halo;
-            const float green_nb = sstep(0.02f, 0.10f, mx(0.f, -mn(brg, bbg)));
+            const float green_nb = sstepc(0.02f, 0.10f, mx(0.f, -mn(brg, bbg)));
             const float smax = mx(r0, mx(g0, b0)), smin = mn(r0, mn(g0, b0));
             const float ssat = (smax - smin) / mx(smax, 1e-4f);
-            const float whiteish = 1.f - sstep(0.05f, 0.13f, ssat);
-            const float foliage_guard = green_nb * whiteish * sstep(0.35f, 0.75f, bguide) * (1.f - halo);
-            const float strength_release = eg ? sstep(2.0f, 6.0f, strength) : 0.f;
-            const float line_evidence = sstep(0.004f, 0.030f, lfs) * sstep(0.20f, 0.55f, halo) * sstep(0.05f, 0.15f, ssat);
+            const float whiteish = 1.f - sstepc(0.05f, 0.13f, ssat);
+            const float foliage_guard = green_nb * whiteish * sstepc(0.35f, 0.75f, bguide) * (1.f - halo);
+            const float strength_release = eg ? sstepc(2.0f, 6.0f, strength) : 0.f;
+            const float line_evidence = sstepc(0.004f, 0.030f, lfs) * sstepc(0.20f, 0.55f, halo) * sstepc(0.05f, 0.15f, ssat);
             const float release = strength_release * line_evidence * (1.f - natural_hl);
             const float rel_hl = hl_protect + (1.f - hl_protect) * release;
             const float rel_nat = nat_suppress_sq + (1.f - nat_suppress_sq) * release;
             const float foliage_term = 1.f - 0.999f * foliage_guard;
             const float rel_foliage = foliage_term + (1.f - foliage_term) * release;
             mask *= local_gate * green_gain * rel_hl * rel_nat * rel_foliage;
-            const float kill_evidence = sstep(0.04f, 0.15f, lfs);
+            const float kill_evidence = sstepc(0.04f, 0.15f, lfs);
             const float natural_blob = sqrtf(mx(0.f, natural_hl)) * (1.f - kill_evidence) * green_nb;
             mask *= 1.f - 0.999f * natural_blob;
             if (!eg) mask = 0.f;
@@ -495,43 +570,43 @@ __global__ __launch_bounds__(BLOCK) void correct(Planes P, Scal C, const unsigne
             const float red_strength = mx(0.f, r0 - mx(g0, b0)) / mx(r0, 1e-4f);
             const float rb_balance = b0 / mx(r0, 1e-4f);
             const float blue_lift = mx(0.f, b0 - g0) / mx(r0 - g0, 1e-4f);
-            const float red_purple = mx(sstep(0.55f, 0.78f, rb_balance), sstep(0.16f, 0.34f, blue_lift));
-            const float rsg = sstep(0.08f, 0.25f, red_strength);
+            const float red_purple = mx(sstepc(0.55f, 0.78f, rb_balance), sstepc(0.16f, 0.34f, blue_lift));
+            const float rsg = sstepc(0.08f, 0.25f, red_strength);
             float rp_context = 0.f, ctx_rp = 0.f, red_magenta = 0.f;
             if (red_purple > 0.50f) {
                 rp_context = linked;
                 ctx_rp = red_purple * rp_context;
                 if (rp_context < 0.20f && rsg < 0.50f && red_purple > 0.50f)
-                    red_magenta = sstep(0.02f, 0.12f, purple_abs) * (1.f - rp_context);
+                    red_magenta = sstepc(0.02f, 0.12f, purple_abs) * (1.f - rp_context);
             }
             float ngs = 0.f;
             if (strength > 3.f && rp_context > 0.80f && lfs > 0.012f && rsg < 0.98f && blue_lift > 1.10f &&
                 rb_balance > 0.82f && purple_abs > 0.015f) {
-                const float ng_edge = sstep(1.10f, 1.80f, blue_lift) * sstep(0.82f, 1.05f, rb_balance) * (1.f - rsg);
-                ngs = ng_edge * sstep(0.015f, 0.08f, purple_abs) * sstep(3.f, 8.f, strength);
+                const float ng_edge = sstepc(1.10f, 1.80f, blue_lift) * sstepc(0.82f, 1.05f, rb_balance) * (1.f - rsg);
+                ngs = ng_edge * sstepc(0.015f, 0.08f, purple_abs) * sstepc(3.f, 8.f, strength);
             }
             const float red_object = mx(rsg, red_magenta);
             const float nwg_base = 1.f - red_object * (1.f - ctx_rp);
             const float nwg_sq = nwg_base * nwg_base * nwg_base * nwg_base;
             const float nwg = nwg_sq * nwg_sq;
-            const float line_relief = sstep(0.012f, 0.06f, lfs) * sstep(0.10f, 0.24f, blue_lift);
-            const float rog = sstep(0.55f, 0.90f, red_object);
+            const float line_relief = sstepc(0.012f, 0.06f, lfs) * sstepc(0.10f, 0.24f, blue_lift);
+            const float rog = sstepc(0.55f, 0.90f, red_object);
             const float linked_override = linked * 1.05f * (1.f - 0.85f * rog);
             const float line_override = line_relief * (1.f - 0.85f * rog);
             const bool force_warm = (rog > 0.75f) && (linked < 0.65f) && (line_relief < 0.45f);
             const float warm_protect = correct_purple ? (force_warm ? nwg : mx(nwg, mx(linked_override, line_override))) : 1.f;
-            const float red_wo_support = sstep(0.22f, 0.45f, red_strength) * (1.f - sstep(0.02f, 0.12f, bfs));
+            const float red_wo_support = sstepc(0.22f, 0.45f, red_strength) * (1.f - sstepc(0.02f, 0.12f, bfs));
             const float protect = correct_purple ? red_wo_support : 0.f;
-            const float smooth_mask = sstep(mask_cutoff, mask_cutoff * 5.0f, mask);
+            const float smooth_mask = gate(G_MASK, mask);
             const float smax = mx(r0, mx(g0, b0)), smin = mn(r0, mn(g0, b0));
             const float ssat = (smax - smin) / mx(smax, 1e-4f);
-            const float ng_base = sstep(0.008f, 0.030f, ssat);
-            const float neutral_guard = 1.f - (1.f - ng_base) * sstep(0.35f, 0.80f, red_object);
+            const float ng_base = sstepc(0.008f, 0.030f, ssat);
+            const float neutral_guard = 1.f - (1.f - ng_base) * sstepc(0.35f, 0.80f, red_object);
             float amount = lre::clamp_std(smooth_mask * strength * 1.90f * (1.f - protect) * warm_protect *
                                               neutral_guard * C.threshold_soften,
                                           0.f, 1.f);
-            const float warm_edge = sstep(0.006f, 0.035f, purple_abs) * sstep(0.25f, 0.65f, red_purple) *
-                                    sstep(0.15f, 0.65f, rp_context) * (1.f - 0.55f * rsg) * sstep(3.f, 8.f, strength);
+            const float warm_edge = sstepc(0.006f, 0.035f, purple_abs) * sstepc(0.25f, 0.65f, red_purple) *
+                                    sstepc(0.15f, 0.65f, rp_context) * (1.f - 0.55f * rsg) * sstepc(3.f, 8.f, strength);
             const float safe_surface = mx(ngs, warm_edge);
             if (correct_purple && safe_surface > 1e-4f) amount = mn(amount, 1.f - 0.99f * safe_surface);
             if (amount <= 1e-4f) {
@@ -592,12 +667,12 @@ __global__ __launch_bounds__(BLOCK) void correct(Planes P, Scal C, const unsigne
             const float target_b = g * e_bg;
             const float solo_r = mx(0.f, q0 - target_r) / mx(bguide, 0.05f);
             const float solo_b = mx(0.f, q2 - target_b) / mx(bguide, 0.05f);
-            const float local_r = sstep(ct * 0.12f * isens, ct * 0.65f * isens, solo_r);
-            const float local_b = sstep(ct * 0.12f * isens, ct * 0.65f * isens, solo_b);
+            const float local_r = gate(G_SOLO_LOCAL, solo_r);
+            const float local_b = gate(G_SOLO_LOCAL, solo_b);
             const float qr0 = mx(q0, 0.f), qb0 = mx(q2, 0.f);
             const float gom = mx(0.f, g - mn(qr0, qb0)) / mx(bguide, 0.05f);
-            const float gdom = sstep(0.010f, 0.060f, gom);
-            const float purpleish = sstep(0.010f, 0.060f, mx(mx(0.f, qr0 - g), mx(0.f, qb0 - g)) / mx(bguide, 0.05f));
+            const float gdom = sstepc(0.010f, 0.060f, gom);
+            const float purpleish = sstepc(0.010f, 0.060f, mx(mx(0.f, qr0 - g), mx(0.f, qb0 - g)) / mx(bguide, 0.05f));
             const float allow = 1.f - gdom * (1.f - purpleish);
             const float mr = P.srb[i] * local_r * allow;
             const float mb = P.sbb[i] * local_b * allow;
@@ -682,6 +757,35 @@ int lre_launch_defringe(const float *d_in, float *d_out, int w, int h, float rad
         u = u < 0.f ? 0.f : (1.f < u ? 1.f : u);
         C.threshold_soften = 1.f - 0.86f * (u * u * (3.0f - 2.0f * u));
     }
+    // parameter-dependent smoothstep edges; the quotient shortcut needs every
+    // ramp width inside [2^-20, 2^100] (else the kernels divide)
+    Gates G;
+    bool fast = true;
+    {
+        const float i = C.isens, et = edge_threshold, ct = chroma_threshold, mc = C.mask_cutoff;
+        auto set = [&](Gate k, float e0, float e1) {
+            const float d = e1 - e0;
+            G.g[k] = Edge{e0, d, 1.0f / d};
+            if (!(d >= 0x1p-20f && d <= 0x1p100f)) fast = false;
+        };
+        set(G_EDGE, et * 0.35f * i, et * 1.50f * i);
+        set(G_EDGE_RATIO, 0.55f * i, 1.35f * i);
+        set(G_EDGE_LOW, et * 0.10f * i, et * 0.80f * i);
+        set(G_RATIO_LOW, 0.20f * i, 0.90f * i);
+        set(G_BRIGHT, 0.24f * i, 0.52f * i);
+        set(G_CHROMA, ct * 0.45f * i, ct * 1.60f * i);
+        set(G_BAL_HUE, 0.06f * i, 0.22f * i);
+        set(G_DIR, 0.035f * i, 0.16f * i);
+        set(G_GREEN_SOFT, 0.05f * i, 0.20f * i);
+        set(G_SOLO, ct * 0.22f * i, ct * 0.95f * i);
+        set(G_SOLO_LOCAL, ct * 0.12f * i, ct * 0.65f * i);
+        set(G_LOCAL, ct * 0.10f * i, ct * 0.55f * i);
+        set(G_ABS, ct * 0.35f * i, ct * 1.20f * i);
+        set(G_LOW_CONTRAST, ct * 1.20f * i, ct * 2.40f * i);
+        set(G_PEAK, ct * 0.80f * i, ct * 2.50f * i);
+        set(G_HALO, ct * 0.55f * i, ct * 1.65f * i);
+        set(G_MASK, mc, mc * 5.0f);
+    }
 
     const int g = grid_for(n);
     hipLaunchKernelGGL(init_stats, dim3(1), dim3(64), 0, s, stats);
@@ -729,18 +833,21 @@ int lre_launch_defringe(const float *d_in, float *d_out, int w, int h, float rad
     P.src = d_in; P.dst = d_out;
     P.lrg = lrg; P.lbg = lbg; P.bg = bg; P.brg = brg; P.bbg = bbg; P.be = be; P.ber = ber;
     P.fs = fs; P.bfs = bfs; P.lfs = lfs; P.sr = sr; P.sb = sb; P.srb = srb; P.sbb = sbb;
-    hipLaunchKernelGGL(support, dim3(g), dim3(BLOCK), 0, s, P, C, stats);
+    if (fast) hipLaunchKernelGGL(support<true>, dim3(g), dim3(BLOCK), 0, s, P, C, G);
+    else hipLaunchKernelGGL(support<false>, dim3(g), dim3(BLOCK), 0, s, P, C, G);
     BlurSet S1{};
     S1.count = 1; S1.in[0] = fs; S1.tmp[0] = t2; S1.out[0] = bfs;
     if ((rc = blur(S1))) return rc;
     hipLaunchKernelGGL(line_link, dim3((w + LL_T - 1) / LL_T, (h + LL_T - 1) / LL_T), dim3(BLOCK), 0, s, P, w, h);
-    hipLaunchKernelGGL(solo_seed, dim3(g), dim3(BLOCK), 0, s, P, C);
+    if (fast) hipLaunchKernelGGL(solo_seed<true>, dim3(g), dim3(BLOCK), 0, s, P, C, G);
+    else hipLaunchKernelGGL(solo_seed<false>, dim3(g), dim3(BLOCK), 0, s, P, C, G);
     BlurSet S2{};
     S2.count = 2;
     S2.in[0] = sr; S2.tmp[0] = t3; S2.out[0] = srb;
     S2.in[1] = sb; S2.tmp[1] = t4; S2.out[1] = sbb;
     if ((rc = blur(S2))) return rc;
-    hipLaunchKernelGGL(correct, dim3(g), dim3(BLOCK), 0, s, P, C, stats);
+    if (fast) hipLaunchKernelGGL(correct<true>, dim3(g), dim3(BLOCK), 0, s, P, C, G, stats);
+    else hipLaunchKernelGGL(correct<false>, dim3(g), dim3(BLOCK), 0, s, P, C, G, stats);
     LRE_HIP_CHECK(hipGetLastError());
     return LRE_OK;
 }

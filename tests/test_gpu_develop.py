@@ -103,7 +103,9 @@ def _gpu_defringe(acc, img, inplace, radius=10.0, edge=0.1, chroma=0.15, strengt
 
 
 @pytest.mark.parametrize("inplace", [False, True])
-@pytest.mark.parametrize("kw", [dict(), dict(green=True), dict(radius=6.0, strength=2.0, green=True, gscale=0.5)])
+# edge=1e-7 narrows a smoothstep ramp below 2^-20, which selects the kernels that divide
+@pytest.mark.parametrize("kw", [dict(), dict(green=True), dict(radius=6.0, strength=2.0, green=True, gscale=0.5),
+                                dict(edge=1e-7, green=True)])
 def test_defringe_tolerance(acc, inplace, kw):
     h, w = 480, 720
     img = oracle_amaze(scene.cfa_to_rgb3(scene.bayer_scene(h, w, scene.BASE_SEED + 21))) * 1.8
