@@ -19,8 +19,8 @@
 //     of its input (the reference's per-thread arena carries values from the
 //     previous tile of the same OpenMP thread into a few unfilled cells).
 //
-// The kernel is persistent: a grid of `slots` workgroups walks the tile list,
-// each owning one arena (1.41 MiB).  All arithmetic is FP32 with the operand
+// The kernel is persistent: a grid of `slots` workgroups takes tiles from a
+// queue (one atomic counter), each workgroup owning one arena (1.41 MiB).  All arithmetic is FP32 with the operand
 // order of the reference (built with -ffp-contract=off).
 #include "lre_common.h"
 #include "lre_internal.h"
@@ -210,7 +210,7 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
     constexpr int p1 = -TS + 1, p2 = -2 * TS + 2, p3 = -3 * TS + 3;
     constexpr int m1 = TS + 1, m2 = 2 * TS + 2, m3 = 3 * TS + 3;
     const int W = A.W, H = A.H;
-    extern __shared__ float lds[];
+    extern __shared__ __align__(16) float lds[];  // 16-byte aligned behind the static LDS
     // cfa lives in LDS; writes the tile fill sends past its end land in the
     // arena right behind the CFA array (the Nyquist flags), as in the reference
     float *cfa = lds, *cfa_spill = ar + O_CFA, *green = ar + O_GREEN;
@@ -830,21 +830,29 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
     __syncthreads();
 }
 
+// The tile queue's counter sits behind the arenas and the spare one.
+LRE_DEV unsigned *queue_of(float *arenas, int slots) {
+    return reinterpret_cast<unsigned *>(arenas + (size_t)(slots + 1) * ARENA_FLOATS);
+}
+
+__global__ void queue_reset(float *arenas, int slots) {
+    if (threadIdx.x == 0) *queue_of(arenas, slots) = 0u;
+}
+
 __global__ __launch_bounds__(BLOCK) void amaze_kernel(Args A) {
     float *ar = A.arenas + (size_t)blockIdx.x * ARENA_FLOATS;
-    for (int t = blockIdx.x; t < A.ntiles; t += gridDim.x) {
+    lre::tile_queue(queue_of(A.arenas, gridDim.x), A.ntiles, [&](int t) {
         const int tr = t / A.ntc, tc = t % A.ntc;
         amaze_tile(A, ar, -16 + tr * (TS - 32), -16 + tc * (TS - 32));
-    }
+    });
 }
 
 // One tile queue over several images: tile t belongs to the last image whose
-// tile_base is <= t.  Every workgroup walks the queue with a fixed stride and
-// ends when it passes total_tiles.
+// tile_base is <= t.
 __global__ __launch_bounds__(BLOCK) void amaze_multi_kernel(const Args *__restrict__ tab, int nimg,
-                                                            int total_tiles, float *arenas) {
+                                                            int total_tiles, int slots, float *arenas) {
     float *ar = arenas + (size_t)blockIdx.x * ARENA_FLOATS;
-    for (int t = blockIdx.x; t < total_tiles; t += gridDim.x) {
+    lre::tile_queue(queue_of(arenas, slots), total_tiles, [&](int t) {
         int lo = 0, hi = nimg - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
@@ -855,7 +863,7 @@ __global__ __launch_bounds__(BLOCK) void amaze_multi_kernel(const Args *__restri
         const int lt = t - A.tile_base;
         const int tr = lt / A.ntc, tc = lt % A.ntc;
         amaze_tile(A, ar, -16 + tr * (TS - 32), -16 + tc * (TS - 32));
-    }
+    });
 }
 
 // LibRaw-style border interpolation (:2984-3028): frame pixels get the 3x3 mean
@@ -901,16 +909,20 @@ static hipError_t amaze_lds_attr() {
     return rc;
 }
 
+// One workgroup fits a CU (LDS), and the tile queue balances the tiles, so
+// the grid is one workgroup per CU.
 static int amaze_slots(lre_ctx *ctx, int w, int h) {
     const int ntiles = tiles_for(w) * tiles_for(h);
-    const int cap = (ctx->num_cus > 0 ? ctx->num_cus : 256) * 2;
-    return ntiles < cap ? ntiles : cap;
+    const int cap = ctx->num_cus > 0 ? ctx->num_cus : 256;
+    return ntiles < cap ? (ntiles > 0 ? ntiles : 1) : cap;
 }
 
-size_t lre_amaze_scratch_bytes(lre_ctx *ctx, int w, int h) {
-    // one spare arena of padding behind the last slot
-    return (size_t)(amaze_slots(ctx, w, h) + 1) * ARENA_FLOATS * sizeof(float);
+// the slots' arenas, one spare arena of padding behind the last, the queue counter
+static size_t arenas_bytes(int slots) {
+    return (size_t)(slots + 1) * ARENA_FLOATS * sizeof(float) + 256;
 }
+
+size_t lre_amaze_scratch_bytes(lre_ctx *ctx, int w, int h) { return arenas_bytes(amaze_slots(ctx, w, h)); }
 
 static void amaze_args(Args &A, const float *d_cfa, float *d_rgb, int w, int h, uint32_t filters,
                        const float cam_mul[4], float maximum_value, const float *d_maxv) {
@@ -946,6 +958,7 @@ int lre_launch_amaze(lre_ctx *ctx, const float *d_cfa, float *d_rgb, int w, int 
     A.arenas = static_cast<float *>(d_scratch);
     const int slots = amaze_slots(ctx, w, h);
     LRE_HIP_CHECK(amaze_lds_attr());
+    hipLaunchKernelGGL(queue_reset, dim3(1), dim3(64), 0, s, A.arenas, slots);
     hipLaunchKernelGGL(amaze_kernel, dim3(slots), dim3(BLOCK), LDS_BYTES, s, A);
     LRE_HIP_CHECK(hipGetLastError());
     return lre_launch_border_interpolate(nullptr, d_rgb, w, h, filters, 4, s);
@@ -953,9 +966,7 @@ int lre_launch_amaze(lre_ctx *ctx, const float *d_cfa, float *d_rgb, int w, int 
 
 size_t lre_amaze_multi_table_bytes(int n) { return sizeof(Args) * (size_t)n; }
 
-size_t lre_amaze_multi_scratch_bytes(int slots) {
-    return (size_t)(slots + 1) * ARENA_FLOATS * sizeof(float);
-}
+size_t lre_amaze_multi_scratch_bytes(int slots) { return arenas_bytes(slots); }
 
 int lre_amaze_tiles(int w, int h) { return tiles_for(w) * tiles_for(h); }
 
@@ -978,8 +989,9 @@ int lre_launch_amaze_multi(const lre_demosaic_job *jobs, int n, const void *d_ta
     if (total_tiles > 0) {
         const int grid = slots < total_tiles ? slots : total_tiles;
         LRE_HIP_CHECK(amaze_lds_attr());
+        hipLaunchKernelGGL(queue_reset, dim3(1), dim3(64), 0, s, static_cast<float *>(d_scratch), slots);
         hipLaunchKernelGGL(amaze_multi_kernel, dim3(grid), dim3(BLOCK), LDS_BYTES, s,
-                           static_cast<const Args *>(d_table), n, total_tiles,
+                           static_cast<const Args *>(d_table), n, total_tiles, slots,
                            static_cast<float *>(d_scratch));
         LRE_HIP_CHECK(hipGetLastError());
     }

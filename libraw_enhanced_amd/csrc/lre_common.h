@@ -138,6 +138,24 @@ LRE_DEV void frame_pixel(int t, int w, int h, int border, int &row, int &col) {
     }
 }
 
+// Tile queue of a persistent grid: thread 0 takes the next tile with one
+// atomic add on *counter (zeroed before the launch) and the workgroup reads it
+// from LDS, so a workgroup that finishes early takes more tiles.  Each
+// workgroup leaves once the counter has passed n; workgroups never wait for
+// each other, so the grid may exceed what is resident at once.
+template <class F>
+LRE_DEV void tile_queue(unsigned *counter, int n, F f) {
+    __shared__ int next;
+    for (;;) {
+        if (threadIdx.x == 0) next = (int)atomicAdd(counter, 1u);
+        __syncthreads();
+        const int t = next;
+        __syncthreads();  // every wave has read `next` before thread 0 replaces it
+        if (t >= n) break;
+        f(t);
+    }
+}
+
 }  // namespace lre
 
 // Error plumbing of the C-ABI: every HIP call is checked, the message is kept

@@ -111,7 +111,7 @@ constexpr int KW = (2 * TS) / 3 + 2;
 
 __device__ void xtrans_tile(const XArgs &A, char *ar, int top, int left) {
     const int W = A.W, H = A.H;
-    extern __shared__ float lds[];
+    extern __shared__ __align__(16) float lds[];  // 16-byte aligned behind the static LDS
     short *hexl = reinterpret_cast<short *>(reinterpret_cast<char *>(lds) + LDS_HEX);
     signed char *xtl = reinterpret_cast<signed char *>(lds) + LDS_XT;
     uint2 *walks = reinterpret_cast<uint2 *>(reinterpret_cast<char *>(lds) + LDS_WALK);
@@ -495,18 +495,28 @@ __device__ void xtrans_tile(const XArgs &A, char *ar, int top, int left) {
 #undef GM
 }
 
+// The tile queue's counter sits behind the arenas and the spare one.
+LRE_DEV unsigned *queue_of(char *arenas, int slots) {
+    return reinterpret_cast<unsigned *>(arenas + (size_t)(slots + 1) * ARENA);
+}
+
+__global__ void queue_reset(char *arenas, int slots) {
+    if (threadIdx.x == 0) *queue_of(arenas, slots) = 0u;
+}
+
 __global__ __launch_bounds__(BLOCK) void xtrans_kernel(XArgs A) {
     char *ar = A.arenas + (size_t)blockIdx.x * ARENA;
-    for (int t = blockIdx.x; t < A.ntiles; t += gridDim.x)
+    lre::tile_queue(queue_of(A.arenas, gridDim.x), A.ntiles, [&](int t) {
         xtrans_tile(A, ar, 3 + (t / A.ntc) * (TS - 16), 3 + (t % A.ntc) * (TS - 16));
+    });
 }
 
 // One tile queue over several images (table in device memory): tile t belongs
 // to the last image whose tile_base is <= t.
 __global__ __launch_bounds__(BLOCK) void xtrans_multi_kernel(const XArgs *__restrict__ tab, int nimg,
-                                                             int total_tiles, char *arenas) {
+                                                             int total_tiles, int slots, char *arenas) {
     char *ar = arenas + (size_t)blockIdx.x * ARENA;
-    for (int t = blockIdx.x; t < total_tiles; t += gridDim.x) {
+    lre::tile_queue(queue_of(arenas, slots), total_tiles, [&](int t) {
         int lo = 0, hi = nimg - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
@@ -516,7 +526,7 @@ __global__ __launch_bounds__(BLOCK) void xtrans_multi_kernel(const XArgs *__rest
         const XArgs &A = tab[lo];
         const int lt = t - A.tile_base;
         xtrans_tile(A, ar, 3 + (lt / A.ntc) * (TS - 16), 3 + (lt % A.ntc) * (TS - 16));
-    }
+    });
 }
 
 // copies a demosaic maximum produced on the device into an image's table entry
@@ -622,15 +632,18 @@ static hipError_t xtrans_lds_attr() {
     return rc;
 }
 
+// One workgroup fits a CU (LDS), and the tile queue balances the tiles, so
+// the grid is one workgroup per CU.
 static int xtrans_slots(lre_ctx *ctx, int w, int h) {
     const int ntiles = tiles_for(w) * tiles_for(h);
-    const int cap = (ctx->num_cus > 0 ? ctx->num_cus : 256) * 4;
+    const int cap = ctx->num_cus > 0 ? ctx->num_cus : 256;
     return ntiles < cap ? (ntiles > 0 ? ntiles : 1) : cap;
 }
 
-size_t lre_xtrans_scratch_bytes(lre_ctx *ctx, int w, int h) {
-    return (size_t)(xtrans_slots(ctx, w, h) + 1) * ARENA;
-}
+// the slots' arenas, one spare arena behind the last, the queue counter
+static size_t arenas_bytes(int slots) { return (size_t)(slots + 1) * ARENA + 256; }
+
+size_t lre_xtrans_scratch_bytes(lre_ctx *ctx, int w, int h) { return arenas_bytes(xtrans_slots(ctx, w, h)); }
 
 int lre_launch_xtrans3(lre_ctx *ctx, const float *d_cfa, float *d_rgb, int w, int h,
                        const signed char xtrans36[36], float maximum_value, void *d_scratch,
@@ -652,7 +665,9 @@ int lre_launch_xtrans3(lre_ctx *ctx, const float *d_cfa, float *d_rgb, int w, in
     A.arenas = static_cast<char *>(d_scratch);
     if (A.ntiles > 0) {
         LRE_HIP_CHECK(xtrans_lds_attr());
-        hipLaunchKernelGGL(xtrans_kernel, dim3(xtrans_slots(ctx, w, h)), dim3(BLOCK), LDS_BYTES, s, A);
+        const int slots = xtrans_slots(ctx, w, h);
+        hipLaunchKernelGGL(queue_reset, dim3(1), dim3(64), 0, s, A.arenas, slots);
+        hipLaunchKernelGGL(xtrans_kernel, dim3(slots), dim3(BLOCK), LDS_BYTES, s, A);
         LRE_HIP_CHECK(hipGetLastError());
     }
     const size_t frame = (size_t)lre::frame_pixels(w, h, 8);
@@ -667,7 +682,7 @@ int lre_launch_xtrans3(lre_ctx *ctx, const float *d_cfa, float *d_rgb, int w, in
 
 size_t lre_xtrans_multi_table_bytes(int n) { return sizeof(XArgs) * (size_t)n; }
 
-size_t lre_xtrans_multi_scratch_bytes(int slots) { return (size_t)(slots + 1) * ARENA; }
+size_t lre_xtrans_multi_scratch_bytes(int slots) { return arenas_bytes(slots); }
 
 int lre_xtrans_tiles(int w, int h) { return tiles_for(w) * tiles_for(h); }
 
@@ -707,8 +722,9 @@ int lre_launch_xtrans_multi(const lre_demosaic_job *jobs, int n, const void *h_t
     if (total_tiles > 0) {
         const int grid = slots < total_tiles ? slots : total_tiles;
         LRE_HIP_CHECK(xtrans_lds_attr());
+        hipLaunchKernelGGL(queue_reset, dim3(1), dim3(64), 0, s, static_cast<char *>(d_scratch), slots);
         hipLaunchKernelGGL(xtrans_multi_kernel, dim3(grid), dim3(BLOCK), LDS_BYTES, s,
-                           static_cast<const XArgs *>(dtab), n, total_tiles,
+                           static_cast<const XArgs *>(dtab), n, total_tiles, slots,
                            static_cast<char *>(d_scratch));
         LRE_HIP_CHECK(hipGetLastError());
     }
