@@ -72,16 +72,21 @@ LRE_DEV void for_rect(int r0, int r1, int c0, int c1, F f) {
     for (int t = threadIdx.x; t < nr * nc; t += BLOCK) f(r0 + t / nc, c0 + t % nc);
 }
 
-// Non-green ("site") positions of rows [r0,r1): cc = c0 + parity(row) + 2j < c1.
+// Non-green ("site") positions of rows [r0,r1): cc = c0 + parity(row) + 2j < c1,
+// spread over the threads [tid0, tid0 + nthr) of the workgroup.
 template <class F>
-LRE_DEV void for_sites(const Args &A, int r0, int r1, int c0, int c1, F f) {
+LRE_DEV void for_sites_on(const Args &A, int r0, int r1, int c0, int c1, int tid0, int nthr, F f) {
     const int ns = (c1 - c0 + 1) / 2, nr = r1 - r0;
     if (ns <= 0 || nr <= 0) return;
-    for (int t = threadIdx.x; t < nr * ns; t += BLOCK) {
+    for (int t = (int)threadIdx.x - tid0; t < nr * ns; t += nthr) {
         int rr = r0 + t / ns;
         int cc = c0 + (int)(A.cf[rr & 1][0] & 1) + 2 * (t % ns);
         if (cc < c1) f(rr, cc);
     }
+}
+template <class F>
+LRE_DEV void for_sites(const Args &A, int r0, int r1, int c0, int c1, F f) {
+    for_sites_on(A, r0, r1, c0, c1, 0, BLOCK, f);
 }
 
 LRE_DEV unsigned fct(const Args &A, int r, int c) { return A.cf[r & 1][c & 1]; }
@@ -383,9 +388,27 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
     // scan reads its predecessor two sites back after that one was updated; the
     // values ahead of the scan are not yet updated, so they are fetched in
     // chunks of SK sites before the chunk is processed.
+    // The waves without a scan meanwhile take the Gaussian-gradient sum of the
+    // Nyquist test (:1403-1409, the subtrahend), a function of the LDS CFA
+    // plane only, and store it in the nyqtest cells; stage 4-5 subtracts it.
+    auto ggrad_sum = [&](int i) {
+        return GGRAD[0] * f_dsq(i) + GGRAD[1] * (f_dsq(i - v1) + f_dsq(i + 1) + f_dsq(i - 1) + f_dsq(i + v1)) +
+               GGRAD[2] * (f_dsq(i - m1) + f_dsq(i + p1) + f_dsq(i - p1) + f_dsq(i + m1)) +
+               GGRAD[3] * (f_dsq(i - v2) + f_dsq(i - 2) + f_dsq(i + 2) + f_dsq(i + v2)) +
+               GGRAD[4] * (f_dsq(i - v2 - 1) + f_dsq(i - v2 + 1) + f_dsq(i - TS - 2) + f_dsq(i - TS + 2) +
+                           f_dsq(i + TS - 2) + f_dsq(i + TS + 2) + f_dsq(i + v2 - 1) + f_dsq(i + v2 + 1)) +
+               GGRAD[5] * (f_dsq(i - m2) + f_dsq(i + p2) + f_dsq(i - p2) + f_dsq(i + m2));
+    };
     {
         constexpr int SK = 4;
         const int nh = 2 * max(rr1 - 8, 0), nv = 2 * max(cc1 - 8, 0);
+        const int nscan = ((nh + nv + 63) / 64) * 64;  // whole waves scan (at most 640 threads)
+        static_assert(2 * 2 * (TS - 8) <= BLOCK - 5 * 64, "scans leave waves for the gradient sums");
+        if ((int)threadIdx.x >= nscan)
+            for_sites_on(A, 6, rr1 - 6, 6, cc1 - 6, nscan, BLOCK - nscan, [&](int rr, int cc) {
+                const int i = rr * TS + cc;
+                nyqtest[i >> 1] = ggrad_sum(i);
+            });
         for (int t = threadIdx.x; t < nh + nv; t += BLOCK) {
             const bool horiz = t < nh;
             int base, first, end, step;  // positions base + k * step, k in [first, end)
@@ -465,7 +488,7 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
     });
     __syncthreads();
 
-    // stages 4-5 (:1400-1428)
+    // stages 4-5 (:1400-1428); the gradient sums came with stage 3
     for_sites_split(A, 6, rr1 - 6, 6, cc1 - 6, [&](int rr, int cc) {
         const int i = rr * TS + cc;
         const float *d = sm;  // cddiffsq at sites, half-index
@@ -473,11 +496,7 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
             (GODD[0] * d[(i) >> 1] + GODD[1] * (d[(i - m1) >> 1] + d[(i + p1) >> 1] + d[(i - p1) >> 1] + d[(i + m1) >> 1]) +
              GODD[2] * (d[(i - v2) >> 1] + d[(i - 2) >> 1] + d[(i + 2) >> 1] + d[(i + v2) >> 1]) +
              GODD[3] * (d[(i - m2) >> 1] + d[(i + p2) >> 1] + d[(i - p2) >> 1] + d[(i + m2) >> 1])) -
-            (GGRAD[0] * f_dsq(i) + GGRAD[1] * (f_dsq(i - v1) + f_dsq(i + 1) + f_dsq(i - 1) + f_dsq(i + v1)) +
-             GGRAD[2] * (f_dsq(i - m1) + f_dsq(i + p1) + f_dsq(i - p1) + f_dsq(i + m1)) +
-             GGRAD[3] * (f_dsq(i - v2) + f_dsq(i - 2) + f_dsq(i + 2) + f_dsq(i + v2)) +
-             GGRAD[4] * (f_dsq(i - v2 - 1) + f_dsq(i - v2 + 1) + f_dsq(i - TS - 2) + f_dsq(i - TS + 2) + f_dsq(i + TS - 2) + f_dsq(i + TS + 2) + f_dsq(i + v2 - 1) + f_dsq(i + v2 + 1)) +
-             GGRAD[5] * (f_dsq(i - m2) + f_dsq(i + p2) + f_dsq(i - p2) + f_dsq(i + m2)));
+            nyqtest[i >> 1];
     }, [&](int rr, int cc, float nt) { nyqtest[(rr * TS + cc) >> 1] = nt; });
     __syncthreads();  // the hvwt pass below overwrites the LDS half plane
     for_site_pairs(A, 6, rr1 - 6, 6, cc1 - 6, [&](int rr, int cc) {
