@@ -151,8 +151,14 @@ LRE_DEV float from_lane_above(float v) {
 // its cells back each row: a cell that is not a site of the range keeps its
 // value, and lanes past the row end all address its last cell, which is never
 // a site.  Sites of one row do not depend on each other.
-LRE_DEV void smooth_rows(const Args &A, float *sm, int b, int rr1, int cc1) {
+//
+// The other waves meanwhile run `others` (work that neither reads nor writes
+// the half plane); the smoothing wave raises its issue priority so that its
+// serial chain keeps its pace beside them.
+template <class F>
+LRE_DEV void smooth_rows(const Args &A, float *sm, int b, int rr1, int cc1, F others) {
     if (threadIdx.x < 64) {
+        __builtin_amdgcn_s_setprio(2);
         const int h = b / 2 - 2 + 2 * (int)threadIdx.x, h1 = h + 1;
         const int last = rr1 - b;  // last row read (lower neighbours of the last row)
         const int om = min(h - 1, TSH - 1), o0 = min(h, TSH - 1), o1 = min(h1, TSH - 1), o2 = min(h + 2, TSH - 1);
@@ -197,6 +203,9 @@ LRE_DEV void smooth_rows(const Args &A, float *sm, int b, int rr1, int cc1) {
             step(rr + 1, po, ok0o, ok1o);
         }
         if (rr < last) step(rr, pe, ok0e, ok1e);
+        __builtin_amdgcn_s_setprio(0);
+    } else {
+        others();
     }
     __syncthreads();
 }
@@ -589,52 +598,6 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
     }
     float *dg0 = vcdalt;        // Dgrb0 over vcdalt (half-index)
     float *d2 = ar + O_Q1M;     // Dgrb2 {h,v} pairs over Dgrbsq1m/Dgrbsq1p
-    // hvwt smoothing (:1446-1448) on the LDS copy that stages 4-6 wrote beside
-    // hvwt; each smoothed site's weight is stored back by the pass below
-    smooth_rows(A, sm, 8, rr1, cc1);
-    struct SG {
-        float hk, d0, g, dh, dv;
-    };
-    for_site_pairs(A, 8, rr1 - 8, 8, cc1 - 8, [&](int rr, int cc) {
-        const int i = rr * TS + cc, k = i >> 1;
-        SG R;
-        R.hk = sm[k];
-        R.d0 = R.hk * vcd[i] + (1.f - R.hk) * hcd[i];
-        R.g = cfa[i] + R.d0;
-        const bool q = nbit(nqm2, k) != 0;
-        R.dh = q ? lre::sq(R.g - half(green[i - 1] + green[i + 1])) : 0.f;
-        R.dv = q ? lre::sq(R.g - half(green[i - v1] + green[i + v1])) : 0.f;
-        return R;
-    }, [&](int rr, int cc, const SG &R) {
-        const int i = rr * TS + cc, k = i >> 1;
-        hvwt[k] = R.hk;
-        dg0[k] = R.d0;
-        green[i] = R.g;
-        d2[2 * k] = R.dh;
-        d2[2 * k + 1] = R.dv;
-    });
-    __syncthreads();
-    if (any_nyq) {
-        for_sites(A, 8, rr1 - 8, 8, cc1 - 8, [&](int rr, int cc) {
-            int i = rr * TS + cc, k = i >> 1;
-            if (!nbit(nqm2, k)) return;
-#define D2H(o) d2[2 * ((i + (o)) >> 1)]
-#define D2V(o) d2[2 * ((i + (o)) >> 1) + 1]
-            float gvh = EPSSQ + (GQUINC[0] * D2H(0) + GQUINC[1] * (D2H(-m1) + D2H(p1) + D2H(-p1) + D2H(m1)) +
-                                 GQUINC[2] * (D2H(-v2) + D2H(-2) + D2H(2) + D2H(v2)) +
-                                 GQUINC[3] * (D2H(-m2) + D2H(p2) + D2H(-p2) + D2H(m2)));
-            float gvv = EPSSQ + (GQUINC[0] * D2V(0) + GQUINC[1] * (D2V(-m1) + D2V(p1) + D2V(-p1) + D2V(m1)) +
-                                 GQUINC[2] * (D2V(-v2) + D2V(-2) + D2V(2) + D2V(v2)) +
-                                 GQUINC[3] * (D2V(-m2) + D2V(p2) + D2V(-p2) + D2V(m2)));
-#undef D2H
-#undef D2V
-            float d0 = (hcd[i] * gvv + vcd[i] * gvh) / (gvv + gvh);
-            dg0[k] = d0;
-            green[i] = cfa[i] + d0;
-        });
-        __syncthreads();
-    }
-
     // stage 7 (:1484-1502) computes the diagonal gradients and q1 sums that
     // stage 8 reads; stage 8 recomputes them from LDS, and no later stage reads
     // those cells, so the stage has no pass of its own.
@@ -648,8 +611,12 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
     auto f_delm = [&](int s) { return fabsf(cfa[s + m1] - cfa[s - m1]); };
     auto f_q1p = [&](int n) { return lre::sq(cfa[n] - cfa[n - p1]) + lre::sq(cfa[n] - cfa[n + p1]); };
     auto f_q1m = [&](int n) { return lre::sq(cfa[n] - cfa[n - m1]) + lre::sq(cfa[n] - cfa[n + m1]); };
-    float *rbm = vcd, *rbp = hcdalt, *pmwt = wh;
-    for_sites(A, 8, rr1 - 8, 8, cc1 - 8, [&](int rr, int cc) {
+    // rbp over hcdalt and pmwt over dirwts1, as in the reference.  rbm lives in
+    // the second half of the delhvsqsum plane (rbint fills only the first)
+    // instead of over vcd: after this stage only the rbint pass reads it,
+    // while the green passes, which here run after stage 8, read vcd.
+    float *rbm = dsq + N2 / 2, *rbp = hcdalt, *pmwt = wh;
+    auto stage8 = [&](int rr, int cc) {
         const int i = rr * TS + cc, k = i >> 1;
         float c0 = cfa[i];
         float rse = twice(cfa[i + m1]) / (EPS + c0 + cfa[i + m2]);
@@ -695,15 +662,63 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
 #undef QS
         const float w = varm / (varp + varm);
         pmwt[k] = w;
-        sm[k] = w;  // LDS copy for the smoothing pass
+    };
+    // green refinement at Nyquist sites (:1449-1481)
+    auto nyq_refine = [&](int rr, int cc) {
+        int i = rr * TS + cc, k = i >> 1;
+        if (!nbit(nqm2, k)) return;
+#define D2H(o) d2[2 * ((i + (o)) >> 1)]
+#define D2V(o) d2[2 * ((i + (o)) >> 1) + 1]
+        float gvh = EPSSQ + (GQUINC[0] * D2H(0) + GQUINC[1] * (D2H(-m1) + D2H(p1) + D2H(-p1) + D2H(m1)) +
+                             GQUINC[2] * (D2H(-v2) + D2H(-2) + D2H(2) + D2H(v2)) +
+                             GQUINC[3] * (D2H(-m2) + D2H(p2) + D2H(-p2) + D2H(m2)));
+        float gvv = EPSSQ + (GQUINC[0] * D2V(0) + GQUINC[1] * (D2V(-m1) + D2V(p1) + D2V(-p1) + D2V(m1)) +
+                             GQUINC[2] * (D2V(-v2) + D2V(-2) + D2V(2) + D2V(v2)) +
+                             GQUINC[3] * (D2V(-m2) + D2V(p2) + D2V(-p2) + D2V(m2)));
+#undef D2H
+#undef D2V
+        float d0 = (hcd[i] * gvv + vcd[i] * gvh) / (gvv + gvh);
+        dg0[k] = d0;
+        green[i] = cfa[i] + d0;
+    };
+    // hvwt smoothing (:1446-1448) on the LDS copy that stages 4-6 wrote beside
+    // hvwt, while the other waves run stage 8 (the LDS CFA plane in, planes
+    // that nothing reads before the R/B interpolation out); each smoothed
+    // weight is stored back by the green pass, which also moves pmwt into the
+    // LDS plane
+    smooth_rows(A, sm, 8, rr1, cc1, [&] { for_sites_on(A, 8, rr1 - 8, 8, cc1 - 8, 64, BLOCK - 64, stage8); });
+    struct SG {
+        float hk, d0, g, dh, dv, pw;
+    };
+    for_site_pairs(A, 8, rr1 - 8, 8, cc1 - 8, [&](int rr, int cc) {
+        const int i = rr * TS + cc, k = i >> 1;
+        SG R;
+        R.hk = sm[k];
+        R.pw = pmwt[k];
+        R.d0 = R.hk * vcd[i] + (1.f - R.hk) * hcd[i];
+        R.g = cfa[i] + R.d0;
+        const bool q = nbit(nqm2, k) != 0;
+        R.dh = q ? lre::sq(R.g - half(green[i - 1] + green[i + 1])) : 0.f;
+        R.dv = q ? lre::sq(R.g - half(green[i - v1] + green[i + v1])) : 0.f;
+        return R;
+    }, [&](int rr, int cc, const SG &R) {
+        const int i = rr * TS + cc, k = i >> 1;
+        hvwt[k] = R.hk;
+        sm[k] = R.pw;  // the LDS plane now serves the pmwt smoothing
+        dg0[k] = R.d0;
+        green[i] = R.g;
+        d2[2 * k] = R.dh;
+        d2[2 * k + 1] = R.dv;
     });
     __syncthreads();
-
     // stage 9 (:1535-1540): pmwt smoothing reads updated upper-diagonal neighbours
     float *rbint = dsq;
     // same recurrence for pmwt; rbint of a site needs only that site's final
     // pmwt, so it is computed in a parallel pass afterwards
-    smooth_rows(A, sm, 10, rr1, cc1);
+    smooth_rows(A, sm, 10, rr1, cc1, [&] {
+        // the Nyquist refinement reads and writes neither the LDS plane nor the stage-8 planes
+        if (any_nyq) for_sites_on(A, 8, rr1 - 8, 8, cc1 - 8, 64, BLOCK - 64, nyq_refine);
+    });
     for_sites(A, 10, rr1 - 10, 10, cc1 - 10, [&](int rr, int cc) {
         const int i = rr * TS + cc, k = i >> 1;
         const float pk = sm[k];
