@@ -47,8 +47,31 @@ LRE_DEV void divmod(int t, int d, float inv_d, int &q, int &r) {
 // Stages a rows x cols window into LDS: element (r, c) = load(r, c), written by
 // store(r, c, value).  Each thread issues U independent global loads before
 // its first LDS store, so their latencies overlap.
+// When a row fits the workgroup, each thread keeps one column and walks the
+// rows (neighbouring threads take neighbouring columns), so the column's part
+// of the address and of any clamping is computed once per thread.
 template <int NT, int U = 8, class Load, class Store>
 LRE_DEV void stage2d(int rows, int cols, Load load, Store store) {
+    if (cols <= NT) {
+        using V = decltype(load(0, 0));
+        const int per = NT / cols;  // rows per pass
+        const int c = (int)threadIdx.x % cols, r0 = (int)threadIdx.x / cols;
+        if (r0 >= per) return;
+        for (int base = r0; base < rows; base += per * U) {
+            V v[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int r = base + u * per;
+                if (r < rows) v[u] = load(r, c);
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int r = base + u * per;
+                if (r < rows) store(r, c, v[u]);
+            }
+        }
+        return;
+    }
     const int n = rows * cols;
     const float inv = 1.0f / (float)cols;
     using V = decltype(load(0, 0));
