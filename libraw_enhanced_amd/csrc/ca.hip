@@ -166,13 +166,12 @@ __global__ __launch_bounds__(AX_BLOCK) void axial_ab(AxialArgs A) {
         sE[(ry - 1) * RXP + rx] = v;
     }
     __syncthreads();
-    // this thread's outputs: rows oy .. oy+3 of column cx
+    // this thread's outputs: rows oy .. oy+3 of column cx.  Each output is
+    // finished as soon as its means are known, so few FP64 means stay live.
     constexpr int PER = AX_RB;
     const int cx = threadIdx.x % AX_TX, oy = (threadIdx.x / AX_TX) * AX_RB;
-    double m[7][PER];
-#pragma unroll
-    for (int qi = 0; qi < 7; qi++) {
-        // the summed quantity at window row ry (region row ry + 1), region column c
+    // window means of quantity qi (region row ry + 1 holds window row ry)
+    auto means = [&](int qi, double out[PER]) {
         hsum(
             [&](int ry, int c) {
                 const int k = (ry + 1) * RXP + c;
@@ -188,23 +187,45 @@ __global__ __launch_bounds__(AX_BLOCK) void axial_ab(AxialArgs A) {
             },
             RYi, r, hs);
         __syncthreads();
-        vmean(hs, r, cx, oy, x0, y0, W, H, m[qi]);
+        vmean(hs, r, cx, oy, x0, y0, W, H, out);
         __syncthreads();
-    }
-    for (int k = 0; k < PER; k++) {
+    };
+    auto index_of = [&](int k, size_t &i) {
         const int x = x0 + cx, y = y0 + oy + k;
-        if (x >= W || y >= H) continue;
-        const size_t i = (size_t)y * W + x;
-        const double mI = m[0][k], mII = m[1][k], mR = m[2][k], mIR = m[3][k], mB = m[4][k], mIB = m[5][k];
-        const double var = mII - mI * mI;
-        const double e = (double)A.eps;
-        const double ar = (mIR - mI * mR) / (var + e);
-        const double ab = (mIB - mI * mB) / (var + e);
-        A.aR[i] = (float)ar;
-        A.bR[i] = (float)(mR - ar * mI);
-        A.aB[i] = (float)ab;
-        A.bB[i] = (float)(mB - ab * mI);
-        const float t = lre::clamp_std(((float)m[6][k] - 0.01f) / 0.05f, 0.f, 1.f);
+        i = (size_t)y * W + x;
+        return x < W && y < H;
+    };
+    double mI[PER], var[PER];
+    {
+        double mII[PER];
+        means(0, mI);
+        means(1, mII);
+#pragma unroll
+        for (int k = 0; k < PER; k++) var[k] = mII[k] - mI[k] * mI[k];
+    }
+    const double e = (double)A.eps;
+    // a = cov(I, p) / (var + eps), b = mean(p) - a mean(I) for p = R (qi 2, 3) and B (qi 4, 5)
+#pragma unroll
+    for (int ch = 0; ch < 2; ch++) {
+        double mP[PER], mIP[PER];
+        means(2 + 2 * ch, mP);
+        means(3 + 2 * ch, mIP);
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            size_t i;
+            if (!index_of(k, i)) continue;
+            const double a = (mIP[k] - mI[k] * mP[k]) / (var[k] + e);
+            (ch ? A.aB : A.aR)[i] = (float)a;
+            (ch ? A.bB : A.bR)[i] = (float)(mP[k] - a * mI[k]);
+        }
+    }
+    double mE[PER];
+    means(6, mE);
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        size_t i;
+        if (!index_of(k, i)) continue;
+        const float t = lre::clamp_std(((float)mE[k] - 0.01f) / 0.05f, 0.f, 1.f);
         A.ew[i] = t * t * (3.f - 2.f * t);
     }
 }
