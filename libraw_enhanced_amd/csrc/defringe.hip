@@ -175,6 +175,49 @@ __global__ __launch_bounds__(BLOCK) void sobel(const float *__restrict__ s, floa
     block_max_to(m, &stats[1]);
 }
 
+struct Planes {
+    const float *src;
+    float *dst;
+    float *lrg, *lbg, *bg, *brg, *bbg, *be, *ber, *fs, *bfs, *lfs, *sr, *sb, *srb, *sbb;
+};
+
+// fringe support seed of pixel i (:4162-4248); ber is the blurred raw edge
+// map at i (the caller may hold it before it is stored)
+template <bool FAST>
+LRE_DEV float support_px(const Planes &P, const Scal &C, const Gates &G, size_t i, float ber) {
+    const auto gate = [&](Gate k, float x) { return sstep<FAST>(G.g[k], x); };
+    const float rg = P.lrg[i], bg = P.lbg[i];
+    const float dr = rg - P.brg[i], db = bg - P.bbg[i];
+    const float r0 = mx(P.src[3 * i], 0.f), g0 = mx(P.src[3 * i + 1], 0.f), b0 = mx(P.src[3 * i + 2], 0.f);
+    const float purple_excess = mn(mx(0.f, dr), mx(0.f, db));
+    const float denom = mx(P.bg[i], 0.05f);
+    const float g_over_min = mx(0.f, g0 - mn(r0, b0)) / denom;
+    const float green_dom = sstepc(0.010f, 0.060f, g_over_min);
+    const float green_excess = mx(mx(0.f, -dr), mx(0.f, -db)) * green_dom;
+    const bool is_purple = purple_excess >= green_excess;
+    const float excess = is_purple ? purple_excess : green_excess;
+    const float min_dir = is_purple ? mx(0.f, mn(rg, bg)) : mx(0.f, mn(-rg, -bg));
+    const float max_dir = is_purple ? mx(1e-4f, mx(rg, bg)) : mx(1e-4f, mx(-rg, -bg));
+    const float balance = min_dir / max_dir;
+    const float c_edge = gate(G_EDGE, P.be[i]) *
+                         gate(G_EDGE_RATIO, ber);
+    const float c_bright = gate(G_BRIGHT, P.bg[i]);
+    const float c_chroma = gate(G_CHROMA, excess);
+    const float red_strength = mx(0.f, r0 - mx(g0, b0)) / mx(r0, 1e-4f);
+    const float rb_balance = b0 / mx(r0, 1e-4f);
+    const float blue_lift = mx(0.f, b0 - g0) / mx(r0 - g0, 1e-4f);
+    const float red_purple = mx(sstepc(0.55f, 0.78f, rb_balance), sstepc(0.16f, 0.34f, blue_lift));
+    const float rp_signal = sstepc(0.50f, 0.72f, rb_balance) * sstepc(0.24f, 0.42f, blue_lift);
+    const float balanced_hue = gate(G_BAL_HUE, min_dir) * sstepc(0.35f, 0.75f, balance);
+    const float rp_hue = is_purple ? mn(1.f, rp_signal * 1.50f) * gate(G_DIR, min_dir) *
+                                         sstepc(0.18f, 0.58f, balance)
+                                   : 0.f;
+    const float c_hue = mx(balanced_hue, rp_hue);
+    const float red_guard = 1.f - sstepc(0.08f, 0.25f, red_strength) * (1.f - red_purple);
+    const float green_enable = (C.enable_green || is_purple) ? 1.f : 0.f;
+    return c_edge * c_bright * c_hue * c_chroma * red_guard * green_enable;
+}
+
 // ---- separable Gaussian, clamp-to-edge, taps in the reference's order ----
 struct BlurSet {
     const float *in[5];
@@ -262,9 +305,13 @@ constexpr int FB_TX = 64, FB_TY = 96, FB_KSMAX = 16, FB_RB = 4;
 constexpr int FB_BLOCK = 512;
 static_assert(FB_KSMAX <= (MAX_TAPS - 1) / 2, "fused blur radius within the tap table");
 
-template <int KS>
+// With SUP, the last plane of the set is the blurred raw edge map and each
+// output of it also yields the fringe support seed of its pixel: the thread
+// that writes it wrote the pixel's other blurred planes in the earlier plane
+// iterations (barriers between them).
+template <int KS, bool SUP>
 __global__ __launch_bounds__(FB_BLOCK) void blur2d(BlurSet S, Taps taps, int W, int H,
-                                                const unsigned *stats) {
+                                                const unsigned *stats, Planes P, Scal C, Gates G) {
     extern __shared__ float fb[];
     constexpr int ks = KS, nt = 2 * KS + 1;
     constexpr int IW = FB_TX + 2 * ks, IH = FB_TY + 2 * ks;
@@ -330,68 +377,36 @@ __global__ __launch_bounds__(FB_BLOCK) void blur2d(BlurSet S, Taps taps, int W, 
                 w[FB_RB - 1] = col[(k + FB_RB) * PH];
             }
             const int x = x0 + c;
+            const bool seed = SUP && p == S.count - 1;
             if (x < W)
 #pragma unroll
                 for (int u = 0; u < FB_RB; u++) {
                     const int y = y0 + r + u;
-                    if (y < H) out[(size_t)y * W + x] = acc[u];
+                    if (y >= H) continue;
+                    const size_t o = (size_t)y * W + x;
+                    out[o] = acc[u];
+                    if (seed) P.fs[o] = support_px<true>(P, C, G, o, acc[u]);
                 }
         }
     }
 }
 
 // the LDS window of blur2d<KS> exceeds 64 KiB: raise the limit once per radius
-template <int KS>
+template <int KS, bool SUP>
 hipError_t blur2d_lds_attr() {
     static const hipError_t rc = hipFuncSetAttribute(
-        (const void *)blur2d<KS>, hipFuncAttributeMaxDynamicSharedMemorySize,
+        (const void *)blur2d<KS, SUP>, hipFuncAttributeMaxDynamicSharedMemorySize,
         (int)(sizeof(float) * ((FB_TY + 2 * KS) * ((FB_TX + 2 * KS) | 1) + (FB_TY + 2 * KS) * (FB_TX + 1))));
     return rc;
 }
 
-struct Planes {
-    const float *src;
-    float *dst;
-    float *lrg, *lbg, *bg, *brg, *bbg, *be, *ber, *fs, *bfs, *lfs, *sr, *sb, *srb, *sbb;
-};
-
-// fringe support seed (:4162-4248)
+// fringe support seed, as a pass of its own (the blur kernels with a
+// radius above the fused limit, and parameters that need the dividing smoothsteps)
 template <bool FAST>
 __global__ __launch_bounds__(BLOCK) void support(Planes P, Scal C, Gates G) {
     const size_t n = (size_t)C.W * C.H;
-    const auto gate = [&](Gate k, float x) { return sstep<FAST>(G.g[k], x); };
-    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK) {
-        const float rg = P.lrg[i], bg = P.lbg[i];
-        const float dr = rg - P.brg[i], db = bg - P.bbg[i];
-        const float r0 = mx(P.src[3 * i], 0.f), g0 = mx(P.src[3 * i + 1], 0.f), b0 = mx(P.src[3 * i + 2], 0.f);
-        const float purple_excess = mn(mx(0.f, dr), mx(0.f, db));
-        const float denom = mx(P.bg[i], 0.05f);
-        const float g_over_min = mx(0.f, g0 - mn(r0, b0)) / denom;
-        const float green_dom = sstepc(0.010f, 0.060f, g_over_min);
-        const float green_excess = mx(mx(0.f, -dr), mx(0.f, -db)) * green_dom;
-        const bool is_purple = purple_excess >= green_excess;
-        const float excess = is_purple ? purple_excess : green_excess;
-        const float min_dir = is_purple ? mx(0.f, mn(rg, bg)) : mx(0.f, mn(-rg, -bg));
-        const float max_dir = is_purple ? mx(1e-4f, mx(rg, bg)) : mx(1e-4f, mx(-rg, -bg));
-        const float balance = min_dir / max_dir;
-        const float c_edge = gate(G_EDGE, P.be[i]) *
-                             gate(G_EDGE_RATIO, P.ber[i]);
-        const float c_bright = gate(G_BRIGHT, P.bg[i]);
-        const float c_chroma = gate(G_CHROMA, excess);
-        const float red_strength = mx(0.f, r0 - mx(g0, b0)) / mx(r0, 1e-4f);
-        const float rb_balance = b0 / mx(r0, 1e-4f);
-        const float blue_lift = mx(0.f, b0 - g0) / mx(r0 - g0, 1e-4f);
-        const float red_purple = mx(sstepc(0.55f, 0.78f, rb_balance), sstepc(0.16f, 0.34f, blue_lift));
-        const float rp_signal = sstepc(0.50f, 0.72f, rb_balance) * sstepc(0.24f, 0.42f, blue_lift);
-        const float balanced_hue = gate(G_BAL_HUE, min_dir) * sstepc(0.35f, 0.75f, balance);
-        const float rp_hue = is_purple ? mn(1.f, rp_signal * 1.50f) * gate(G_DIR, min_dir) *
-                                             sstepc(0.18f, 0.58f, balance)
-                                       : 0.f;
-        const float c_hue = mx(balanced_hue, rp_hue);
-        const float red_guard = 1.f - sstepc(0.08f, 0.25f, red_strength) * (1.f - red_purple);
-        const float green_enable = (C.enable_green || is_purple) ? 1.f : 0.f;
-        P.fs[i] = c_edge * c_bright * c_hue * c_chroma * red_guard * green_enable;
-    }
+    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK)
+        P.fs[i] = support_px<FAST>(P, C, G, i, P.ber[i]);
 }
 
 // line-linked support (:4266-4323): ls = max(fs, 0.55 * blur(fs)) sampled with
@@ -794,15 +809,28 @@ int lre_launch_defringe(const float *d_in, float *d_out, int w, int h, float rad
     hipLaunchKernelGGL(sobel, dim3(g), dim3(BLOCK), 0, s, gs, er, w, h, stats);
     LRE_HIP_CHECK(hipGetLastError());
 
-    auto blur = [&](BlurSet S) -> int {
+    Planes P;
+    P.src = d_in; P.dst = d_out;
+    P.lrg = lrg; P.lbg = lbg; P.bg = bg; P.brg = brg; P.bbg = bbg; P.be = be; P.ber = ber;
+    P.fs = fs; P.bfs = bfs; P.lfs = lfs; P.sr = sr; P.sb = sb; P.srb = srb; P.sbb = sbb;
+    // blur of a plane set; with seed, also the support seeds (the fused form
+    // exists for the fused radii and the quotient smoothsteps only)
+    auto blur = [&](BlurSet S, bool seed) -> int {
         if (ks <= FB_KSMAX) {
             const size_t lds = sizeof(float) * ((size_t)(FB_TY + 2 * ks) * ((FB_TX + 2 * ks) | 1) +
                                                 (size_t)(FB_TY + 2 * ks) * (FB_TX + 1));
             dim3 gb((w + FB_TX - 1) / FB_TX, (h + FB_TY - 1) / FB_TY);
             switch (ks) {
 #define LRE_BLUR2D(K) \
-    case K: LRE_HIP_CHECK(blur2d_lds_attr<K>()); \
-            hipLaunchKernelGGL(blur2d<K>, gb, dim3(FB_BLOCK), lds, s, S, taps, w, h, stats); break;
+    case K: \
+        if (seed) { \
+            LRE_HIP_CHECK((blur2d_lds_attr<K, true>())); \
+            hipLaunchKernelGGL((blur2d<K, true>), gb, dim3(FB_BLOCK), lds, s, S, taps, w, h, stats, P, C, G); \
+        } else { \
+            LRE_HIP_CHECK((blur2d_lds_attr<K, false>())); \
+            hipLaunchKernelGGL((blur2d<K, false>), gb, dim3(FB_BLOCK), lds, s, S, taps, w, h, stats, P, C, G); \
+        } \
+        break;
                 LRE_BLUR2D(1) LRE_BLUR2D(2) LRE_BLUR2D(3) LRE_BLUR2D(4) LRE_BLUR2D(5) LRE_BLUR2D(6)
                 LRE_BLUR2D(7) LRE_BLUR2D(8) LRE_BLUR2D(9) LRE_BLUR2D(10) LRE_BLUR2D(11)
                 LRE_BLUR2D(12) LRE_BLUR2D(13) LRE_BLUR2D(14) LRE_BLUR2D(15) LRE_BLUR2D(16)
@@ -826,18 +854,16 @@ int lre_launch_defringe(const float *d_in, float *d_out, int w, int h, float rad
     float *outs[5] = {bg, brg, bbg, be, ber};
     for (int k = 0; k < 5; k++) { S5.in[k] = ins[k]; S5.tmp[k] = tms[k]; S5.out[k] = outs[k]; S5.scale_by_edge[k] = 0; }
     S5.scale_by_edge[3] = 1;  // blur of the normalised edge map
-    int rc = blur(S5);
+    const bool seed_in_blur = fast && ks <= FB_KSMAX;
+    int rc = blur(S5, seed_in_blur);
     if (rc) return rc;
-
-    Planes P;
-    P.src = d_in; P.dst = d_out;
-    P.lrg = lrg; P.lbg = lbg; P.bg = bg; P.brg = brg; P.bbg = bbg; P.be = be; P.ber = ber;
-    P.fs = fs; P.bfs = bfs; P.lfs = lfs; P.sr = sr; P.sb = sb; P.srb = srb; P.sbb = sbb;
-    if (fast) hipLaunchKernelGGL(support<true>, dim3(g), dim3(BLOCK), 0, s, P, C, G);
-    else hipLaunchKernelGGL(support<false>, dim3(g), dim3(BLOCK), 0, s, P, C, G);
+    if (!seed_in_blur) {
+        if (fast) hipLaunchKernelGGL(support<true>, dim3(g), dim3(BLOCK), 0, s, P, C, G);
+        else hipLaunchKernelGGL(support<false>, dim3(g), dim3(BLOCK), 0, s, P, C, G);
+    }
     BlurSet S1{};
     S1.count = 1; S1.in[0] = fs; S1.tmp[0] = t2; S1.out[0] = bfs;
-    if ((rc = blur(S1))) return rc;
+    if ((rc = blur(S1, false))) return rc;
     hipLaunchKernelGGL(line_link, dim3((w + LL_T - 1) / LL_T, (h + LL_T - 1) / LL_T), dim3(BLOCK), 0, s, P, w, h);
     if (fast) hipLaunchKernelGGL(solo_seed<true>, dim3(g), dim3(BLOCK), 0, s, P, C, G);
     else hipLaunchKernelGGL(solo_seed<false>, dim3(g), dim3(BLOCK), 0, s, P, C, G);
@@ -845,7 +871,7 @@ int lre_launch_defringe(const float *d_in, float *d_out, int w, int h, float rad
     S2.count = 2;
     S2.in[0] = sr; S2.tmp[0] = t3; S2.out[0] = srb;
     S2.in[1] = sb; S2.tmp[1] = t4; S2.out[1] = sbb;
-    if ((rc = blur(S2))) return rc;
+    if ((rc = blur(S2, false))) return rc;
     if (fast) hipLaunchKernelGGL(correct<true>, dim3(g), dim3(BLOCK), 0, s, P, C, G, stats);
     else hipLaunchKernelGGL(correct<false>, dim3(g), dim3(BLOCK), 0, s, P, C, G, stats);
     LRE_HIP_CHECK(hipGetLastError());

@@ -103,9 +103,10 @@ def _gpu_defringe(acc, img, inplace, radius=10.0, edge=0.1, chroma=0.15, strengt
 
 
 @pytest.mark.parametrize("inplace", [False, True])
-# edge=1e-7 narrows a smoothstep ramp below 2^-20, which selects the kernels that divide
+# edge=1e-7 narrows a smoothstep ramp below 2^-20, which selects the kernels that divide;
+# radius 20 (21 taps a side) takes the separable blur passes and the support pass of its own
 @pytest.mark.parametrize("kw", [dict(), dict(green=True), dict(radius=6.0, strength=2.0, green=True, gscale=0.5),
-                                dict(edge=1e-7, green=True)])
+                                dict(edge=1e-7, green=True), dict(radius=20.0, green=True)])
 def test_defringe_tolerance(acc, inplace, kw):
     h, w = 480, 720
     img = oracle_amaze(scene.cfa_to_rgb3(scene.bayer_scene(h, w, scene.BASE_SEED + 21))) * 1.8
