@@ -1,9 +1,10 @@
 // Edge-gated Gaussian defringe for gfx950 (core/cpu_accelerator.cpp:3939-5064).
 //
-// Pipeline (each box is one kernel; planes are float, H x W, in the context scratch):
+// Pipeline (planes are float, H x W, in the context scratch):
 //   max(G) ─► guide / guide_signal / log R/G / log B/G ─► Sobel |∇| + max ─►
-//   5 Gaussian blurs (horizontal + vertical pass) ─► support seed ─► blur ─►
-//   8-direction line link ─► solo seeds ─► 2 blurs ─► per-pixel correction.
+//   5 Gaussian blurs (the support seed in the last one's epilogue) ─► blur ─►
+//   8-direction line link (the solo seeds in its epilogue) ─► 2 blurs ─►
+//   per-pixel correction.
 // The two global maxima stay on the device (atomicMax on the bit pattern of a
 // non-negative float), so the whole stage is enqueued without a host sync.
 //
@@ -409,13 +410,37 @@ __global__ __launch_bounds__(BLOCK) void support(Planes P, Scal C, Gates G) {
         P.fs[i] = support_px<FAST>(P, C, G, i, P.ber[i]);
 }
 
+// solo seeds of pixel i (:4325-4376), given its line-linked support lfs
+template <bool FAST>
+LRE_DEV void solo_px(const Planes &P, const Scal &C, const Gates &G, size_t i, float lfs) {
+    const auto gate = [&](Gate k, float x) { return sstep<FAST>(G.g[k], x); };
+    const float s0 = P.src[3 * i], s1 = P.src[3 * i + 1], s2 = P.src[3 * i + 2];
+    const float g = mx(s1, 0.f);
+    const float tr = g * expf_cr(P.brg[i]);
+    const float tb = g * expf_cr(P.bbg[i]);
+    const float solo_r = mx(0.f, s0 - tr) / mx(P.bg[i], 0.05f);
+    const float solo_b = mx(0.f, s2 - tb) / mx(P.bg[i], 0.05f);
+    const float linked = sstepc(0.004f, 0.045f, mx(P.bfs[i], lfs * 0.90f));
+    const float c_edge = gate(G_EDGE, P.be[i]) *
+                         gate(G_EDGE_RATIO, P.ber[i]);
+    const float c_bright = gate(G_BRIGHT, P.bg[i]);
+    const float r0 = mx(s0, 0.f), g0 = mx(s1, 0.f), b0 = mx(s2, 0.f);
+    const float red_strength = mx(0.f, r0 - mx(g0, b0)) / mx(r0, 1e-4f);
+    const float red_guard = 1.f - 0.90f * sstepc(0.12f, 0.32f, red_strength);
+    P.sr[i] = C.solo_enable * c_edge * c_bright * linked * red_guard *
+              gate(G_SOLO, solo_r);
+    P.sb[i] = C.solo_enable * c_edge * c_bright * gate(G_SOLO, solo_b);
+}
+
 // line-linked support (:4266-4323): ls = max(fs, 0.55 * blur(fs)) sampled with
 // clamp-to-edge.  Each workgroup stages ls and fs for its 32x32 outputs plus a
 // 14-px halo in LDS (coordinates clamped at load time, which equals clamping
 // every sample).
 constexpr int LL_T = 32, LL_HALO = 14, LL_R = LL_T + 2 * LL_HALO;
 
-__global__ __launch_bounds__(BLOCK) void line_link(Planes P, int W, int H) {
+template <bool FAST>
+__global__ __launch_bounds__(BLOCK) void line_link(Planes P, Scal C, Gates G) {
+    const int W = C.W, H = C.H;
     __shared__ float s_ls[LL_R * LL_R], s_fs[LL_R * LL_R];
     const int x0 = blockIdx.x * LL_T, y0 = blockIdx.y * LL_T;
     lre::stage2d<BLOCK>(
@@ -455,32 +480,9 @@ __global__ __launch_bounds__(BLOCK) void line_link(Planes P, int W, int H) {
             const float thin = 1.f - sstep(along * 0.45f, along * 0.95f, across);
             best = mx(best, along * thin);
         }
-        P.lfs[(size_t)y * W + x] = best;
-    }
-}
-
-// solo seeds (:4325-4376)
-template <bool FAST>
-__global__ __launch_bounds__(BLOCK) void solo_seed(Planes P, Scal C, Gates G) {
-    const size_t n = (size_t)C.W * C.H;
-    const auto gate = [&](Gate k, float x) { return sstep<FAST>(G.g[k], x); };
-    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK) {
-        const float s0 = P.src[3 * i], s1 = P.src[3 * i + 1], s2 = P.src[3 * i + 2];
-        const float g = mx(s1, 0.f);
-        const float tr = g * expf_cr(P.brg[i]);
-        const float tb = g * expf_cr(P.bbg[i]);
-        const float solo_r = mx(0.f, s0 - tr) / mx(P.bg[i], 0.05f);
-        const float solo_b = mx(0.f, s2 - tb) / mx(P.bg[i], 0.05f);
-        const float linked = sstepc(0.004f, 0.045f, mx(P.bfs[i], P.lfs[i] * 0.90f));
-        const float c_edge = gate(G_EDGE, P.be[i]) *
-                             gate(G_EDGE_RATIO, P.ber[i]);
-        const float c_bright = gate(G_BRIGHT, P.bg[i]);
-        const float r0 = mx(s0, 0.f), g0 = mx(s1, 0.f), b0 = mx(s2, 0.f);
-        const float red_strength = mx(0.f, r0 - mx(g0, b0)) / mx(r0, 1e-4f);
-        const float red_guard = 1.f - 0.90f * sstepc(0.12f, 0.32f, red_strength);
-        P.sr[i] = C.solo_enable * c_edge * c_bright * linked * red_guard *
-                  gate(G_SOLO, solo_r);
-        P.sb[i] = C.solo_enable * c_edge * c_bright * gate(G_SOLO, solo_b);
+        const size_t i = (size_t)y * W + x;
+        P.lfs[i] = best;
+        solo_px<FAST>(P, C, G, i, best);  // the solo seeds need this pixel's lfs only
     }
 }
 
@@ -864,9 +866,9 @@ int lre_launch_defringe(const float *d_in, float *d_out, int w, int h, float rad
     BlurSet S1{};
     S1.count = 1; S1.in[0] = fs; S1.tmp[0] = t2; S1.out[0] = bfs;
     if ((rc = blur(S1, false))) return rc;
-    hipLaunchKernelGGL(line_link, dim3((w + LL_T - 1) / LL_T, (h + LL_T - 1) / LL_T), dim3(BLOCK), 0, s, P, w, h);
-    if (fast) hipLaunchKernelGGL(solo_seed<true>, dim3(g), dim3(BLOCK), 0, s, P, C, G);
-    else hipLaunchKernelGGL(solo_seed<false>, dim3(g), dim3(BLOCK), 0, s, P, C, G);
+    const dim3 gl_grid((w + LL_T - 1) / LL_T, (h + LL_T - 1) / LL_T);
+    if (fast) hipLaunchKernelGGL(line_link<true>, gl_grid, dim3(BLOCK), 0, s, P, C, G);
+    else hipLaunchKernelGGL(line_link<false>, gl_grid, dim3(BLOCK), 0, s, P, C, G);
     BlurSet S2{};
     S2.count = 2;
     S2.in[0] = sr; S2.tmp[0] = t3; S2.out[0] = srb;
