@@ -213,8 +213,8 @@ LRE_DEV void smooth_rows(const Args &A, float *sm, int b, int rr1, int cc1, F ot
 // LDS: the tile's CFA plane, and one half-index plane for the row-sequential
 // smoothing passes
 constexpr int LDS_CFA = N2, LDS_HALF = TS * TSH;
-// and the two Nyquist flag planes as bit masks (the byte planes of the arena
-// are still written, the masks serve the reads)
+// and the two Nyquist flag planes as bit masks (in place of the arena's byte
+// planes, which are not written)
 constexpr int NYQ_WORDS = TS * TSH / 32;
 constexpr size_t LDS_BYTES = sizeof(float) * (LDS_CFA + LDS_HALF) + 2 * sizeof(unsigned) * NYQ_WORDS;
 static_assert((TS * TSH) % 32 == 0, "whole mask words");
@@ -225,18 +225,17 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
     constexpr int m1 = TS + 1, m2 = 2 * TS + 2, m3 = 3 * TS + 3;
     const int W = A.W, H = A.H;
     extern __shared__ __align__(16) float lds[];  // 16-byte aligned behind the static LDS
-    // cfa lives in LDS; writes the tile fill sends past its end land in the
-    // arena right behind the CFA array (the Nyquist flags), as in the reference
-    float *cfa = lds, *cfa_spill = ar + O_CFA, *green = ar + O_GREEN;
+    // cfa lives in LDS.  The reference's tile fill also writes past the end of
+    // its CFA array, into the Nyquist flags, which stage 5 clears before use;
+    // those values reach no output, so those writes are dropped here.
+    float *cfa = lds, *green = ar + O_GREEN;
     float *sm = lds + LDS_CFA;
     unsigned *nqm = reinterpret_cast<unsigned *>(sm + LDS_HALF), *nqm2 = nqm + NYQ_WORDS;
     auto nbit = [](const unsigned *m, int k) { return (m[k >> 5] >> (k & 31)) & 1u; };
     float *wh = ar + O_WH, *dsq = ar + O_DSQ;
     float *vcd = ar + O_VCD, *hcd = ar + O_HCD, *vcdalt = ar + O_VCDALT, *hcdalt = ar + O_HCDALT;
-    float *dgv = ar + O_DGV, *dgh = ar + O_DGH, *cdsq = ar + O_CDSQ, *hvwt = ar + O_HVWT;
+    float *dgv = ar + O_DGV, *dgh = ar + O_DGH, *hvwt = ar + O_HVWT;
     float *nyqtest = ar + O_NYQTEST;
-    unsigned char *nyq = reinterpret_cast<unsigned char *>(ar + O_NYQ);
-    unsigned char *nyq2 = reinterpret_cast<unsigned char *>(cdsq);
 
     const int bottom = min(top + TS, H + 16), right = min(left + TS, W + 16);
     const int rr1 = bottom - top, cc1 = right - left;
@@ -319,7 +318,6 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
                 if (L < 0) continue;
                 const float x = v[u] * scale;
                 if (L < N2) cfa[L] = x;
-                else cfa_spill[L] = x;
                 green[L] = x;
             }
         }
@@ -331,7 +329,6 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
                 const float v = A.cfa_plane[(size_t)row * W + col] * scale;
                 const int L = rr * TS + ccmax + k;
                 if (L < N2) cfa[L] = v;
-                else cfa_spill[L] = v;
                 green[L] = v;
             }
     }
@@ -487,13 +484,12 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
         }
     }
     __syncthreads();
-    // cddiffsq; the Nyquist test reads it at sites only, so the sites' values
-    // also go to the LDS half plane (free until stage 4-5 fills it with hvwt)
-    for_rect(4, rr1 - 4, 4, cc1 - 4, [&](int rr, int cc) {
+    // cddiffsq; its only reader, the Nyquist test, reads it at sites, so the
+    // sites' values go to the LDS half plane (free until stage 4-5 fills it with
+    // hvwt) and the arena plane is not written
+    for_sites(A, 4, rr1 - 4, 4, cc1 - 4, [&](int rr, int cc) {
         int i = rr * TS + cc;
-        const float v = lre::sq(vcd[i] - hcd[i]);
-        cdsq[i] = v;
-        if ((fct(A, rr, cc) & 1) == 0) sm[i >> 1] = v;
+        sm[i >> 1] = lre::sq(vcd[i] - hcd[i]);
     });
     __syncthreads();
 
@@ -538,16 +534,13 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
         sm[k] = w;  // LDS copy for the smoothing pass
     });
     __syncthreads();
-    // Nyquist flags start at zero (:1432); the clear also wipes what the tile
-    // fill wrote past the end of the CFA array
-    for (int t = threadIdx.x; t < TS * TSH / 4; t += BLOCK) reinterpret_cast<unsigned *>(nyq)[t] = 0u;
+    // Nyquist flags start at zero (:1432)
     for (int t = threadIdx.x; t < 2 * NYQ_WORDS; t += BLOCK) nqm[t] = 0u;
     __syncthreads();
     int local_nyq = 0;
     for_sites(A, 6, rr1 - 6, 6, cc1 - 6, [&](int rr, int cc) {
         const int k = (rr * TS + cc) >> 1;
         if (nyqtest[k] > 0.f) {
-            nyq[k] = 1;
             atomicOr(&nqm[k >> 5], 1u << (k & 31));
             local_nyq = 1;
         }
@@ -556,8 +549,6 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
 
     // stage 6 (:1436-1481)
     if (any_nyq) {
-        for (int t = threadIdx.x; t < TS * TSH / 4; t += BLOCK) reinterpret_cast<unsigned *>(nyq2)[t] = 0u;
-        __syncthreads();
         for_sites(A, 8, rr1 - 8, 8, cc1 - 8, [&](int rr, int cc) {
             int i = rr * TS + cc;
             const unsigned s = nbit(nqm, (i - v2) >> 1) + nbit(nqm, (i - m1) >> 1) + nbit(nqm, (i + p1) >> 1) +
@@ -565,7 +556,6 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
                                nbit(nqm, (i + m1) >> 1) + nbit(nqm, (i + v2) >> 1);
             const unsigned f = s > 4 ? 1u : (s < 4 ? 0u : nbit(nqm, i >> 1));
             const int k = i >> 1;
-            nyq2[k] = (unsigned char)f;
             if (f) atomicOr(&nqm2[k >> 5], 1u << (k & 31));
         });
         __syncthreads();
