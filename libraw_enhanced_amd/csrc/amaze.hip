@@ -529,9 +529,10 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
         float diffwt = hvar1 / (vvar1 + hvar1);
         return ((0.5f - varwt) * (0.5f - diffwt) > 0.f && fabsf(0.5f - diffwt) < fabsf(0.5f - varwt)) ? varwt : diffwt;
     }, [&](int rr, int cc, float w) {
-        const int k = (rr * TS + cc) >> 1;
-        hvwt[k] = w;
-        sm[k] = w;  // LDS copy for the smoothing pass
+        // the smoothing reads the LDS copy; the green pass stores the smoothed
+        // weight of every site that is read later ([8, rr1-8)), so the arena
+        // cell is not written here
+        sm[(rr * TS + cc) >> 1] = w;
     });
     __syncthreads();
     // Nyquist flags start at zero (:1432)
@@ -580,9 +581,7 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
             sv = scfa - half(sv);
             area = half(area);
             float hv2 = EPSSQ + fabsf(area * sqh - sh * sh), vv2 = EPSSQ + fabsf(area * sqv - sv * sv);
-            const float w = hv2 / (vv2 + hv2);
-            hvwt[i >> 1] = w;
-            sm[i >> 1] = w;
+            sm[i >> 1] = hv2 / (vv2 + hv2);  // the arena cell: as in the hvwt pass
         });
         __syncthreads();
     }
@@ -697,8 +696,10 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
         sm[k] = R.pw;  // the LDS plane now serves the pmwt smoothing
         dg0[k] = R.d0;
         green[i] = R.g;
-        d2[2 * k] = R.dh;
-        d2[2 * k + 1] = R.dv;
+        if (any_nyq) {  // Dgrb2 has no reader but the Nyquist refinement
+            d2[2 * k] = R.dh;
+            d2[2 * k + 1] = R.dv;
+        }
     });
     __syncthreads();
     // stage 9 (:1535-1540): pmwt smoothing reads updated upper-diagonal neighbours
