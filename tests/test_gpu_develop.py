@@ -39,6 +39,29 @@ def test_amaze_bit_exact(acc, h, w):
     assert bad == 0, f"{bad} values differ, max-abs {np.abs(got - want).max()}"
 
 
+def test_amaze_nyquist_patterns(acc):
+    """Period-2 stripes, a checkerboard and a chirp up to 0.5 cycles/px: the
+    content that AMaZE's Nyquist test flags, so its refinement paths run."""
+    print("amaze nyquist patterns", flush=True)
+    h, w = 400, 600
+    y, x = np.mgrid[0:h, 0:w].astype(np.float32)
+    lum = np.empty((h, w), np.float32)
+    q = w // 4
+    lum[:, :q] = 0.5 + 0.4 * ((x[:, :q].astype(np.int32) & 1) * 2 - 1)
+    lum[:, q:2 * q] = 0.5 + 0.4 * ((y[:, q:2 * q].astype(np.int32) & 1) * 2 - 1)
+    lum[:, 2 * q:3 * q] = 0.5 + 0.4 * (((x + y)[:, 2 * q:3 * q].astype(np.int32) & 1) * 2 - 1)
+    xx, yy = x[:, 3 * q:] - 3 * q, y[:, 3 * q:]
+    lum[:, 3 * q:] = 0.5 + 0.4 * np.cos(np.pi * (xx * xx + yy * yy) / (2.0 * max(q, h)))
+    tint = np.stack([0.9 + 0.1 * np.sin(y / 37.0), np.ones_like(y), 0.8 + 0.2 * np.cos(x / 53.0)], axis=2)
+    rgb = np.ascontiguousarray(lum[:, :, None] * tint, np.float32)
+    cfa = scene.mosaic(rgb, scene.BASE_SEED, RGGB, None, 0.0, 0.0)
+    raw3 = scene.cfa_to_rgb3(cfa, RGGB)
+    got = acc.demosaic_bayer_amaze(raw3, RGGB, (1, 1, 1, 1), 1.0)
+    want = oracle_amaze(raw3)
+    bad = np.count_nonzero(got != want)
+    assert bad == 0, f"{bad} values differ, max-abs {np.abs(got - want).max()}"
+
+
 @pytest.mark.parametrize("filters", [BGGR, GRBG, GBRG])
 def test_amaze_patterns_and_gain(acc, filters):
     h, w = 300, 420
