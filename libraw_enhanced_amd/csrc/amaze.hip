@@ -225,9 +225,12 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
     constexpr int m1 = TS + 1, m2 = 2 * TS + 2, m3 = 3 * TS + 3;
     const int W = A.W, H = A.H;
     extern __shared__ __align__(16) float lds[];  // 16-byte aligned behind the static LDS
-    // cfa lives in LDS.  The reference's tile fill also writes past the end of
-    // its CFA array, into the Nyquist flags, which stage 5 clears before use;
-    // those values reach no output, so those writes are dropped here.
+    // cfa lives in LDS.  The reference's tile fill also writes every value into
+    // the green plane and past the end of its CFA array, into the Nyquist flags,
+    // which stage 5 clears before use.  Here the green plane is written only at
+    // the sites (by the green passes): at green sites it holds the CFA value,
+    // which its readers take from LDS.  The writes past the end reach no
+    // output and are dropped.
     float *cfa = lds, *green = ar + O_GREEN;
     float *sm = lds + LDS_CFA;
     unsigned *nqm = reinterpret_cast<unsigned *>(sm + LDS_HALF), *nqm2 = nqm + NYQ_WORDS;
@@ -316,9 +319,7 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
             for (int u = 0; u < FU; u++) {
                 const int L = Ls[u];
                 if (L < 0) continue;
-                const float x = v[u] * scale;
-                if (L < N2) cfa[L] = x;
-                green[L] = x;
+                if (L < N2) cfa[L] = v[u] * scale;
             }
         }
         __syncthreads();
@@ -329,7 +330,6 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
                 const float v = A.cfa_plane[(size_t)row * W + col] * scale;
                 const int L = rr * TS + ccmax + k;
                 if (L < N2) cfa[L] = v;
-                green[L] = v;
             }
     }
     __syncthreads();
@@ -687,8 +687,10 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
         R.d0 = R.hk * vcd[i] + (1.f - R.hk) * hcd[i];
         R.g = cfa[i] + R.d0;
         const bool q = nbit(nqm2, k) != 0;
-        R.dh = q ? lre::sq(R.g - half(green[i - 1] + green[i + 1])) : 0.f;
-        R.dv = q ? lre::sq(R.g - half(green[i - v1] + green[i + v1])) : 0.f;
+        // green at green sites is the CFA value (the reference's fill puts it
+        // there and no pass writes those cells), so it is read from LDS
+        R.dh = q ? lre::sq(R.g - half(cfa[i - 1] + cfa[i + 1])) : 0.f;
+        R.dv = q ? lre::sq(R.g - half(cfa[i - v1] + cfa[i + v1])) : 0.f;
         return R;
     }, [&](int rr, int cc, const SG &R) {
         const int i = rr * TS + cc, k = i >> 1;
