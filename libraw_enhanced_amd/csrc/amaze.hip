@@ -334,11 +334,12 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
     }
     __syncthreads();
 
-    // stage 1 (:1331-1339).  Its planes are functions of the LDS CFA plane:
-    // every reader recomputes the values from LDS (bit-identical) except the
-    // R/B refinement, which reads dirwts1 after pmwt (stored over its first
-    // half) has replaced part of it; so only wh is stored.  The dsq cells the
-    // refinement reads as rbint are all rewritten before.
+    // stage 1 (:1331-1339).  Its planes are functions of the LDS CFA plane, so
+    // every reader recomputes the values from LDS (bit-identical) and the stage
+    // has no pass.  The R/B refinement reads dirwts1 after pmwt (stored over its
+    // first half by stages 8 and 9) has replaced part of it: it takes those
+    // cells from the arena and the others from LDS.  The dsq cells the
+    // refinement reads as rbint are all written before.
     auto f_dh = [&](int i) { return fabsf(cfa[i + 1] - cfa[i - 1]); };
     auto f_dv = [&](int i) { return fabsf(cfa[i + v1] - cfa[i - v1]); };
     auto f_wv = [&](int i) { return EPS + fabsf(cfa[i + v2] - cfa[i]) + fabsf(cfa[i] - cfa[i - v2]) + f_dv(i); };
@@ -347,11 +348,6 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
         const float dh = f_dh(i), dv = f_dv(i);
         return lre::sq(dh) + lre::sq(dv);
     };
-    for_rect(2, rr1 - 2, 2, cc1 - 2, [&](int rr, int cc) {
-        int i = rr * TS + cc;
-        wh[i] = f_wh(i);
-    });
-    __syncthreads();
 
     // stage 2 (:1342-1373)
     for_rect(4, rr1 - 4, 4, cc1 - 4, [&](int rr, int cc) {
@@ -720,6 +716,17 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
     });
     __syncthreads();
     // (:1541-1566)
+    // dirwts1 of green cell j as the refinement reads it: where the cell lies in
+    // the plane's first half and its pair's site in stage 8's range, pmwt has
+    // replaced it (arena); elsewhere it holds the stage-1 value
+    auto wh_at = [&](int j) {
+        if (j < N2 / 2) {
+            const int e = 2 * j, er = e / TS, ec = e - er * TS;
+            const int sc = (fct(A, er, ec) & 1) ? ec + 1 : ec;  // the site of the pair (same row)
+            if (er >= 8 && er < rr1 - 8 && sc >= 8 && sc < cc1 - 8) return wh[j];
+        }
+        return f_wh(j);
+    };
     struct SR {
         bool on;
         float g, d0;
@@ -738,7 +745,8 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
         float gl = fabsf(1.f - rl) < ARTHRESH ? rb * rl : cfa[i - 1] + half(rb - rbint[k - 1]);
         float gr = fabsf(1.f - rr_) < ARTHRESH ? rb * rr_ : cfa[i + 1] + half(rb - rbint[k + 1]);
         float gv = (f_wv(i - v1) * gd + f_wv(i + v1) * gu) / (f_wv(i + v1) + f_wv(i - v1));
-        float gh = (wh[i - 1] * gr + wh[i + 1] * gl) / (wh[i - 1] + wh[i + 1]);
+        const float wl = wh_at(i - 1), wr = wh_at(i + 1);
+        float gh = (wl * gr + wr * gl) / (wl + wr);
         const float up = cfa[i - v1], dn = cfa[i + v1], lf = cfa[i - 1], rt = cfa[i + 1];
         if (gv < rb) {
             if (twice(gv) < rb) gv = lre::med3(gv, up, dn);
