@@ -250,14 +250,17 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
     // Clean-tile contract: every cell whose pre-tile value can reach the output
     // reads as zero.  Those are the CFA plane (the fill leaves the corners
     // unwritten) and the four colour-difference planes outside
-    // [4, rr1-4) x [4, cc1-4), which stage 2 fills; the Nyquist flags are
-    // cleared after stage 5 as in the reference.  The other planes are written
+    // [4, rr1-4) x [4, cc1-4), which stage 2 fills; the Nyquist flags (LDS
+    // masks) are cleared here, where the reference clears them after stage 5:
+    // nothing reads them before.  The other planes are written
     // before they are read, or are read only for cells that do not reach the
     // output (checked against zeroed scratch with poisoned scratch on the CPU).
     {
         float4 *c4 = reinterpret_cast<float4 *>(cfa);
         const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
         for (int t = threadIdx.x; t < N2 / 4; t += BLOCK) c4[t] = z;
+        // Nyquist flags start at zero (:1432; nothing reads them before)
+        for (int t = threadIdx.x; t < 2 * NYQ_WORDS; t += BLOCK) nqm[t] = 0u;
         float *cd[4] = {ar + O_VCD, ar + O_HCD, ar + O_VCDALT, ar + O_HCDALT};
         for (int t = threadIdx.x; t < N2; t += BLOCK) {
             const int rr = t / TS, cc = t - rr * TS;
@@ -489,7 +492,9 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
     });
     __syncthreads();
 
-    // stages 4-5 (:1400-1428); the gradient sums came with stage 3
+    // stages 4-5 (:1400-1428); the gradient sums came with stage 3.  A site's
+    // Nyquist flag (:1432-1434) is set as soon as its test value is known
+    int local_nyq = 0;
     for_sites_split(A, 6, rr1 - 6, 6, cc1 - 6, [&](int rr, int cc) {
         const int i = rr * TS + cc;
         const float *d = sm;  // cddiffsq at sites, half-index
@@ -498,8 +503,15 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
              GODD[2] * (d[(i - v2) >> 1] + d[(i - 2) >> 1] + d[(i + 2) >> 1] + d[(i + v2) >> 1]) +
              GODD[3] * (d[(i - m2) >> 1] + d[(i + p2) >> 1] + d[(i - p2) >> 1] + d[(i + m2) >> 1])) -
             nyqtest[i >> 1];
-    }, [&](int rr, int cc, float nt) { nyqtest[(rr * TS + cc) >> 1] = nt; });
-    __syncthreads();  // the hvwt pass below overwrites the LDS half plane
+    }, [&](int rr, int cc, float nt) {
+        if (nt > 0.f) {
+            const int k = (rr * TS + cc) >> 1;
+            atomicOr(&nqm[k >> 5], 1u << (k & 31));
+            local_nyq = 1;
+        }
+    });
+    // (also: the hvwt pass below overwrites the LDS half plane)
+    const int any_nyq = __syncthreads_or(local_nyq);
     for_site_pairs(A, 6, rr1 - 6, 6, cc1 - 6, [&](int rr, int cc) {
         const int i = rr * TS + cc;
         float uave = vcd[i] + vcd[i - v1] + vcd[i - v2] + vcd[i - v3];
@@ -531,18 +543,6 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
         sm[(rr * TS + cc) >> 1] = w;
     });
     __syncthreads();
-    // Nyquist flags start at zero (:1432)
-    for (int t = threadIdx.x; t < 2 * NYQ_WORDS; t += BLOCK) nqm[t] = 0u;
-    __syncthreads();
-    int local_nyq = 0;
-    for_sites(A, 6, rr1 - 6, 6, cc1 - 6, [&](int rr, int cc) {
-        const int k = (rr * TS + cc) >> 1;
-        if (nyqtest[k] > 0.f) {
-            atomicOr(&nqm[k >> 5], 1u << (k & 31));
-            local_nyq = 1;
-        }
-    });
-    const int any_nyq = __syncthreads_or(local_nyq);
 
     // stage 6 (:1436-1481)
     if (any_nyq) {
