@@ -779,14 +779,17 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
     __syncthreads();
 
     // second chroma difference (:1571-1595).  Rows r0 + 2m (the B rows) move
-    // their Dgrb0 into Dgrb1 and clear it.  The values before the move (Dgrb0 of
-    // every site: call it X) are also copied into the LDS half plane for rows
+    // their Dgrb0 into Dgrb1 and clear it; call the values before the move
+    // (Dgrb0 of every site) X.  X goes to the LDS half plane for rows
     // 11 .. rr1-12, which covers every read below: the pass reads only X, and
-    // stage 10 reads X there and the pass's output from the arena.
+    // stage 10 reads X there and the pass's output from the arena.  Every cell
+    // stage 10 reads in the arena is written by the pass, so the move itself
+    // (Dgrb1 = X, Dgrb0 = 0 on B rows) leaves nothing that is read and is not
+    // stored.
     float *dg1 = hcd;  // Dgrb1 over hcd (half-index)
     const int r0b = 13 - A.ey;
     {
-        const int nr = max(rr1 - 22, 0), n = nr * TSH;
+        const int n = max(rr1 - 22, 0) * TSH;
         constexpr int U = 2;
         for (int t0 = threadIdx.x; t0 < n; t0 += U * BLOCK) {
             float v[U];
@@ -798,14 +801,7 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const int t = t0 + u * BLOCK;
-                if (t >= n) break;
-                const int rr = 11 + t / TSH, k = 11 * TSH + t;
-                sm[k] = v[u];
-                if (rr >= r0b && ((rr - r0b) & 1) == 0 && rr < rr1 - 12 && k >= ((rr * TS + 13 - A.ex) >> 1) &&
-                    k < ((rr * TS + cc1 - 12) >> 1)) {
-                    dg1[k] = v[u];
-                    dg0[k] = 0;
-                }
+                if (t < n) sm[11 * TSH + t] = v[u];
             }
         }
     }
