@@ -347,11 +347,15 @@ __device__ void xtrans_tile(const XArgs &A, char *ar, int top, int left) {
         }
     };
 
-    // LDS buffer -> arena buffer d
+    // LDS buffer -> arena buffer d.  Only the rows the final average reads go
+    // out: [rs, re) with the bounds of the last pass below.
+    const int row_s = top < 8 ? top : 8;
+    const int row_e = max(((H - top < TS + 4) ? H - top + 2 : mrow_d) - 8, row_s);
+    const int q0 = row_s * TS * 3 / 4, q1 = (row_e * TS * 3 + 3) / 4;  // float4 range
     auto store_buffer = [&](int d) {
         const float4 *src = reinterpret_cast<const float4 *>(lds);
         float4 *dst = reinterpret_cast<float4 *>(RGB(rgb0, d, 0, 0));
-        for (int t = threadIdx.x; t < (int)(DIRSZ / 4); t += BLOCK) dst[t] = src[t];
+        for (int t = q0 + threadIdx.x; t < q1; t += BLOCK) dst[t] = src[t];
     };
 
     for (int j = 0; j < 4; j++) {
