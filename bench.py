@@ -20,6 +20,8 @@ Other workloads, one JSON line each:
   serve     Poisson arrivals of uint16 requests (40% 24 MP, 30% 45 MP, 20%
             61 MP Bayer, 10% 40 MP X-Trans; black 512, white 16383, WB
             2.1/1.0/1.5) through the request loop; p50/p99 latency and MP/s
+  micro24   micro-contrast with a highlight mask on one resident 24 MP RGB
+            image (the stage the pipeline runs for highlight_mode >= 4)
 """
 from __future__ import annotations
 
@@ -123,7 +125,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--workload", default="chain24",
-                    choices=["chain24", "xtrans26", "batch64", "serve", "copy"])
+                    choices=["chain24", "xtrans26", "batch64", "serve", "micro24", "copy"])
     ap.add_argument("--rates", default="4,8,16", help="serve: arrival rates (requests/s)")
     ap.add_argument("--duration", type=float, default=20.0, help="serve: seconds of arrivals per rate")
     ap.add_argument("--lanes", type=int, default=3, help="serve: requests in flight")
@@ -134,7 +136,7 @@ def main():
         sys.exit(2)
     if args.workload != "chain24":
         run = {"xtrans26": bench_xtrans26, "batch64": bench_batch64, "serve": bench_serve,
-               "copy": bench_copy}[args.workload]
+               "micro24": bench_micro24, "copy": bench_copy}[args.workload]
         print(json.dumps(run(args)), flush=True)
         return
 
@@ -320,6 +322,71 @@ def bench_xtrans26(args):
         raw3 = scene.cfa_to_rgb3(cfa, 0, scene.XTRANS)
         line["cpu_baseline"] = cpu_baseline_stages(raw3, 9, scene.XTRANS, ["demosaic", "lateral_ca", "axial_ca"],
                                                    threads, "X-Trans")
+    acc.close()
+    return line
+
+
+def bench_micro24(args):
+    """Micro-contrast (core/cpu_accelerator.cpp:3135-3319) on one resident 24 MP
+    RGB image with a highlight mask, as the develop pipeline calls it (strength 8,
+    target contrast 0.04).  The reference's own code for this stage needs
+    Apple's simd headers, so the CPU baseline is the restatement ("port")."""
+    from libraw_enhanced_amd import scene
+    from libraw_enhanced_amd.accelerator import HIPAccelerator, native
+    h, w = args.height, args.width
+    _log(f"generating {w}x{h} RGB scene and mask")
+    img = np.ascontiguousarray(scene.render(h, w, scene.BASE_SEED) * 1.3, np.float32)
+    t = np.clip((img[:, :, 1] - 0.95) / 0.05, 0.0, 1.0)
+    mask = np.ascontiguousarray(t * t * (3.0 - 2.0 * t), np.float32)
+    acc = HIPAccelerator(0)
+    lib = native()
+    d_in, d_mask = acc.dev_from_host(img), acc.dev_from_host(mask)
+    d_out = acc.dev_alloc(img.nbytes)
+    stream = lib.lre_hip_stream(acc.handle)
+    args_c = (ctypes.c_float(1.3), ctypes.c_float(8.0), ctypes.c_float(0.04))
+
+    def step():
+        rc = lib.lre_hip_dev_micro_contrast(acc.handle, d_in.ptr, d_out.ptr, w, h, *args_c, d_mask.ptr, stream)
+        assert rc == 0, rc
+    for _ in range(args.warmup):
+        step()
+    acc.synchronize()
+    ev = HipEvents()
+    marks = [(ev.create(), ev.create()) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for a, b in marks:
+        ev.record(a, stream)
+        step()
+        ev.record(b, stream)
+    acc.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms = float(np.mean([ev.elapsed_ms(a, b) for a, b in marks]))
+    mp = h * w / 1e6
+    alg = 28 * h * w  # RGB in 12 + mask 4 + RGB out 12 bytes per pixel
+    line = {"metric": "MP/s micro-contrast (24 MP)", "value": round(mp * args.steps / elapsed, 2), "unit": "MP/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(elapsed * 1e3 / args.steps, 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scene x 1.3, green smoothstep mask)",
+            "config": {"workload": f"enhance_micro_contrast with mask on one {w}x{h} float32 RGB image",
+                       "image_mp": round(mp, 2), "stage_ms": round(ms, 4)},
+            "roofline": {"kernel": "micro_contrast", "bound": "hbm", "achieved": round(alg / (ms * 1e-3) / 1e9, 1),
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                         "alg_bytes_per_launch": alg}}
+    if not args.no_cpu:
+        threads = _threads(args)
+        _log(f"CPU baseline (restatement) on {threads} threads")
+        os.environ["OMP_NUM_THREADS"] = str(threads)
+        olib = ctypes.CDLL(os.path.join(ROOT, "oracle", "liblre_oracle.so"))
+        f32p = ctypes.POINTER(ctypes.c_float)
+        out = np.empty_like(img)
+        t1 = time.perf_counter()
+        olib.oracle_micro_contrast(img.ctypes.data_as(f32p), out.ctypes.data_as(f32p), w, h,
+                                   ctypes.c_float(1.3), ctypes.c_float(8.0), ctypes.c_float(0.04),
+                                   mask.ctypes.data_as(f32p))
+        dt = time.perf_counter() - t1
+        line["cpu_baseline"] = {"value": round(mp / dt, 3), "unit": "MP/s", "cores": threads, "kind": "port",
+                                "sample": f"the whole {w}x{h} image ({dt:.2f} s)"}
     acc.close()
     return line
 

@@ -86,6 +86,16 @@ int lre_hip_tone_mapping(lre_ctx *ctx, const float *in_rgb, float *out_rgb, int 
 int lre_hip_detail_tonemap(lre_ctx *ctx, const float *in_rgb, float *out_rgb, int width,
                            int height);
 
+/* Accelerator::enhance_micro_contrast (core/accelerator.h:338-341,
+ * core/cpu_accelerator.cpp:3135-3319); defaults strength 8, target_contrast
+ * 0.06 (the pipeline passes 0.04 and its highlight mask).  mask: w x h floats
+ * in [0, 1] blending input and enhanced result, or NULL for the gate on green
+ * (green >= threshold enhances, else the pixel passes through).  in == out is
+ * allowed. */
+int lre_hip_micro_contrast(lre_ctx *ctx, const float *in_rgb, float *out_rgb, int width,
+                           int height, float threshold, float strength, float target_contrast,
+                           const float *mask);
+
 /* Accelerator::defringe (core/accelerator.h:344-351, core/cpu_accelerator.cpp:3939-5064),
  * defaults radius 10, edge_threshold 0.1, chroma_threshold 0.15, strength 10,
  * defringe_green 0, green_strength_scale 0.3.  in_rgb == out_rgb selects the
@@ -141,6 +151,10 @@ int lre_hip_dev_tone_mapping(lre_ctx *ctx, const float *d_in, float *d_out, int 
                              int height, float after_scale, void *stream);
 int lre_hip_dev_detail_tonemap(lre_ctx *ctx, const float *d_in, float *d_out, int width,
                                int height, void *stream);
+/* Micro-contrast on resident buffers; d_mask is a device plane or NULL. */
+int lre_hip_dev_micro_contrast(lre_ctx *ctx, const float *d_in, float *d_out, int width,
+                               int height, float threshold, float strength,
+                               float target_contrast, const float *d_mask, void *stream);
 /* Defringe on resident buffers; d_in == d_out selects the in-place semantics. */
 int lre_hip_dev_defringe(lre_ctx *ctx, const float *d_in, float *d_out, int width, int height,
                          float radius, float edge_threshold, float chroma_threshold,

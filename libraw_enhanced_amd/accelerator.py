@@ -55,6 +55,10 @@ _SIGS = {
     "lre_hip_tone_mapping": ([_vp, _f32p, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_float],
                              ctypes.c_int),
     "lre_hip_detail_tonemap": ([_vp, _f32p, _f32p, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "lre_hip_micro_contrast": ([_vp, _f32p, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                ctypes.c_float, ctypes.c_float, _f32p], ctypes.c_int),
+    "lre_hip_dev_micro_contrast": ([_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                    ctypes.c_float, ctypes.c_float, _vp, _vp], ctypes.c_int),
     "lre_hip_defringe": ([_vp, _f32p, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_float,
                           ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int,
                           ctypes.c_float], ctypes.c_int),
@@ -220,6 +224,23 @@ class HIPAccelerator:
         h, w, _ = src.shape
         out = np.empty_like(src)
         _check(native().lre_hip_detail_tonemap(self._h, _p(src), _p(out), w, h))
+        return out
+
+    def enhance_micro_contrast(self, rgb_input, threshold: float, strength: float = 8.0,
+                               target_contrast: float = 0.06, mask=None):
+        """Accelerator::enhance_micro_contrast (core/accelerator.h:338-341).  mask:
+        (H, W) float32 weights in [0, 1] blending input and result; None gates the
+        enhancement on green >= threshold."""
+        src = _rgb(rgb_input)
+        h, w, _ = src.shape
+        m = None
+        if mask is not None:
+            m = np.ascontiguousarray(mask, np.float32)
+            if m.shape != (h, w):
+                raise LibRawInvalidArgument(f"mask must have shape {(h, w)}, got {m.shape}")
+        out = np.empty_like(src)
+        _check(native().lre_hip_micro_contrast(self._h, _p(src), _p(out), w, h, threshold, strength,
+                                               target_contrast, None if m is None else _p(m)))
         return out
 
     def defringe(self, rgb_input, rgb_output=None, radius: float = 10.0,
@@ -395,6 +416,15 @@ class LibRawWrapper:
     def tone_mapping(self, image, after_scale: float = 1.0):
         # core/libraw_wrapper.cpp:3098-3131
         return self._acc.tone_mapping(_rgb(image), after_scale)
+
+    def enhance_micro_contrast(self, image, threshold: float = -1.0, strength: float = 8.0,
+                               target_contrast: float = 0.06):
+        # core/libraw_wrapper.cpp:3134-3173: threshold < 0 takes maximum /
+        # data_maximum of the loaded raw, which falls back to 1 when there is
+        # none (:3004-3011); no raw is ever loaded here
+        if threshold < 0.0:
+            threshold = 1.0
+        return self._acc.enhance_micro_contrast(_rgb(image), threshold, strength, target_contrast)
 
     def defringe(self, image, radius: float = 10.0, strength: float = 10.0,
                  defringe_green: bool = False):

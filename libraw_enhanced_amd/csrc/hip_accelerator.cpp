@@ -78,6 +78,14 @@ bool HIPAccelerator::apply_detail_preserving_tonemap(const ImageBufferFloat &in,
     return lre_hip_detail_tonemap(ctx_, fp(in), fp(out), (int)in.width, (int)in.height) == LRE_OK;
 }
 
+bool HIPAccelerator::enhance_micro_contrast(const ImageBufferFloat &in, ImageBufferFloat &out,
+                                            float threshold, float strength, float target_contrast,
+                                            const float *mask) {
+    if (!ctx_ || !same_size(in, out)) return false;
+    return lre_hip_micro_contrast(ctx_, fp(in), fp(out), (int)in.width, (int)in.height, threshold,
+                                  strength, target_contrast, mask) == LRE_OK;
+}
+
 bool HIPAccelerator::defringe(const ImageBufferFloat &in, ImageBufferFloat &out, float radius,
                               float edge_threshold, float chroma_threshold, float strength,
                               bool defringe_green, float green_strength_scale) {

@@ -42,6 +42,9 @@ public:
     // apply_detail_preserving_tonemap_gpu of the facade (core/accelerator.h:329-330)
     bool apply_detail_preserving_tonemap(const ImageBufferFloat &rgb_input,
                                          ImageBufferFloat &rgb_output);
+    bool enhance_micro_contrast(const ImageBufferFloat &rgb_input, ImageBufferFloat &rgb_output,
+                                float threshold, float strength, float target_contrast = 0.06f,
+                                const float *mask = nullptr);
     bool defringe(const ImageBufferFloat &rgb_input, ImageBufferFloat &rgb_output,
                   float radius = 10.0f, float edge_threshold = 0.1f,
                   float chroma_threshold = 0.15f, float strength = 10.0f,

@@ -240,6 +240,17 @@ int lre_hip_dev_detail_tonemap(lre_ctx *ctx, const float *d_in, float *d_out, in
     return lre_launch_detail_tonemap(d_in, d_out, w, h, ctx->scratch, lre_pick_stream(ctx, stream));
 }
 
+int lre_hip_dev_micro_contrast(lre_ctx *ctx, const float *d_in, float *d_out, int w, int h,
+                               float threshold, float strength, float target_contrast,
+                               const float *d_mask, void *stream) {
+    if (!ctx || !d_in || !d_out) return set_arg_error("null pointer");
+    if (!valid_dims(w, h)) return set_arg_error("image size");
+    int rc = lre_ensure_scratch(ctx, lre_micro_contrast_scratch_bytes(w, h));
+    if (rc) return rc;
+    return lre_launch_micro_contrast(d_in, d_out, w, h, threshold, strength, target_contrast, d_mask,
+                                     ctx->scratch, lre_pick_stream(ctx, stream));
+}
+
 int lre_hip_dev_defringe(lre_ctx *ctx, const float *d_in, float *d_out, int w, int h,
                          float radius, float edge_threshold, float chroma_threshold,
                          float strength, int defringe_green, float green_strength_scale,
@@ -374,6 +385,28 @@ int lre_hip_detail_tonemap(lre_ctx *ctx, const float *in_rgb, float *out_rgb, in
     if ((rc = copy_in(ctx, d, in_rgb, n * 4))) return rc;
     if ((rc = lre_launch_detail_tonemap(d, d, w, h, ws, ctx->stream))) return rc;
     return copy_out(ctx, out_rgb, d, n * 4);
+}
+
+int lre_hip_micro_contrast(lre_ctx *ctx, const float *in_rgb, float *out_rgb, int w, int h,
+                           float threshold, float strength, float target_contrast,
+                           const float *mask) {
+    if (!ctx || !in_rgb || !out_rgb) return set_arg_error("null pointer");
+    if (!valid_dims(w, h)) return set_arg_error("image size");
+    LRE_HIP_CHECK(hipSetDevice(ctx->device));
+    const size_t n = (size_t)w * h;
+    int rc = lre_ensure_scratch(ctx, aligned(n * 12) + (mask ? aligned(n * 4) : 0) +
+                                         lre_micro_contrast_scratch_bytes(w, h));
+    if (rc) return rc;
+    Carve c(ctx->scratch);
+    float *d = c.take<float>(n * 3);
+    float *d_mask = mask ? c.take<float>(n) : nullptr;
+    void *ws = static_cast<char *>(ctx->scratch) + c.off;
+    if ((rc = copy_in(ctx, d, in_rgb, n * 12))) return rc;
+    if (mask && (rc = copy_in(ctx, d_mask, mask, n * 4))) return rc;
+    if ((rc = lre_launch_micro_contrast(d, d, w, h, threshold, strength, target_contrast, d_mask, ws,
+                                        ctx->stream)))
+        return rc;
+    return copy_out(ctx, out_rgb, d, n * 12);
 }
 
 int lre_hip_defringe(lre_ctx *ctx, const float *in_rgb, float *out_rgb, int w, int h,

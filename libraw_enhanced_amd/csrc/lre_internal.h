@@ -31,6 +31,7 @@ inline hipStream_t lre_pick_stream(lre_ctx *ctx, void *stream) {
 // Bytes of device scratch each stage needs for an image of w x h.
 size_t lre_amaze_scratch_bytes(lre_ctx *ctx, int w, int h);
 size_t lre_detail_tm_scratch_bytes(int w, int h);
+size_t lre_micro_contrast_scratch_bytes(int w, int h);
 size_t lre_defringe_scratch_bytes(int w, int h);
 size_t lre_axial_scratch_bytes(int w, int h);
 size_t lre_xtrans_scratch_bytes(lre_ctx *ctx, int w, int h);
@@ -95,6 +96,9 @@ int lre_launch_tone_mapping(const float *d_in, float *d_out, size_t n, float aft
                             hipStream_t s);
 int lre_launch_detail_tonemap(const float *d_in, float *d_out, int w, int h, void *d_scratch,
                               hipStream_t s);
+int lre_launch_micro_contrast(const float *d_in, float *d_out, int w, int h, float threshold,
+                              float strength, float target_contrast, const float *d_mask,
+                              void *d_scratch, hipStream_t s);
 int lre_launch_defringe(const float *d_in, float *d_out, int w, int h, float radius,
                         float edge_threshold, float chroma_threshold, float strength,
                         int defringe_green, float green_strength_scale, void *d_scratch,

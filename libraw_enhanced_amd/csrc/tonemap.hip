@@ -237,6 +237,142 @@ int grid_for(size_t n, int block) {
     return (int)(b > 0 ? b : 1);
 }
 
+
+// ---- micro-contrast enhancement, core/cpu_accelerator.cpp:3135-3319 -------
+// Local Gaussian mean and deviation (9x9 taps of sigma 5/3.7, coordinates
+// clamped at the frame, :3077-3128), then a gain on the detail that grows as
+// the local contrast (deviation over its image maximum) falls below the
+// target.  Every output sums its 81 taps in the reference's (ky, kx) order, so
+// the means are bit-identical to the reference's.
+constexpr int MC_K = 9, MC_R = 4, MC_T = 32, MC_W = MC_T + 2 * MC_R, MC_RB = 4;
+constexpr int MC_BLOCK = MC_T * (MC_T / MC_RB);  // one 4-row strip of one column per thread
+
+struct McTaps {
+    float k[MC_K * MC_K];
+};
+
+__global__ void mc_init(unsigned *stats) {
+    if (threadIdx.x == 0) stats[0] = 0u;
+}
+
+// pass 1: means of I and I*I per channel, the deviation sqrt(max(var, 0)),
+// and the image maximum of the deviations (atomicMax on the bits of a
+// non-negative float)
+__global__ __launch_bounds__(MC_BLOCK) void mc_stats_kernel(const float *__restrict__ in, float *__restrict__ mean3,
+                                                            float *__restrict__ sd3, int W, int H, McTaps T,
+                                                            unsigned *stats) {
+    __shared__ float win[3][MC_W * MC_W];
+    const int x0 = blockIdx.x * MC_T, y0 = blockIdx.y * MC_T;
+    lre::stage2d<MC_BLOCK>(
+        MC_W, MC_W,
+        [&](int r, int c) {
+            const int y = min(max(y0 - MC_R + r, 0), H - 1), x = min(max(x0 - MC_R + c, 0), W - 1);
+            const float *p = in + ((size_t)y * W + x) * 3;
+            return make_float3(p[0], p[1], p[2]);
+        },
+        [&](int r, int c, float3 v) {
+            const int t = r * MC_W + c;
+            win[0][t] = v.x;
+            win[1][t] = v.y;
+            win[2][t] = v.z;
+        });
+    __syncthreads();
+    const int lx = threadIdx.x % MC_T, ly0 = (threadIdx.x / MC_T) * MC_RB;
+    float a[MC_RB][3], q[MC_RB][3];
+#pragma unroll
+    for (int u = 0; u < MC_RB; u++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) a[u][c] = q[u][c] = 0.f;
+    // window row ky feeds the outputs u with tap row ky - u in [0, 9): each
+    // output still takes its taps row by row, left to right
+#pragma unroll
+    for (int ky = 0; ky < MC_K + MC_RB - 1; ky++)
+#pragma unroll
+        for (int kx = 0; kx < MC_K; kx++) {
+            float v[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) v[c] = win[c][(ly0 + ky) * MC_W + lx + kx];
+#pragma unroll
+            for (int u = 0; u < MC_RB; u++) {
+                const int ty = ky - u;
+                if (ty < 0 || ty >= MC_K) continue;
+                const float wgt = T.k[ty * MC_K + kx];
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    a[u][c] += v[c] * wgt;
+                    q[u][c] += (v[c] * v[c]) * wgt;
+                }
+            }
+        }
+    float m = 0.f;
+    const int x = x0 + lx;
+#pragma unroll
+    for (int u = 0; u < MC_RB; u++) {
+        const int y = y0 + ly0 + u;
+        if (x >= W || y >= H) continue;
+        const size_t i = ((size_t)y * W + x) * 3;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float sd = sqrtf(lre::max_std(q[u][c] - a[u][c] * a[u][c], 0.f));
+            mean3[i + c] = a[u][c];
+            sd3[i + c] = sd;
+            m = lre::max_std(m, sd);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    __shared__ float red[MC_BLOCK / 64];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float v = red[0];
+        for (int k = 1; k < MC_BLOCK / 64; k++) v = fmaxf(v, red[k]);
+        const unsigned u = __float_as_uint(v);
+        if (u > __hip_atomic_load(stats, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(stats, u);
+    }
+}
+
+// pass 2, pointwise (:3212-3318); mask == nullptr selects the gate on green
+__global__ void mc_apply_kernel(const float *in, float *out, const float *__restrict__ mean3,
+                                const float *__restrict__ sd3, const float *__restrict__ mask, size_t n,
+                                float threshold, float strength, float target, const unsigned *stats) {
+    const float max_sd = __uint_as_float(stats[0]);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float s0 = in[3 * i], s1 = in[3 * i + 1], s2 = in[3 * i + 2];
+        if (!mask && !(s1 >= threshold)) {
+            out[3 * i] = s0;
+            out[3 * i + 1] = s1;
+            out[3 * i + 2] = s2;
+            continue;
+        }
+        const float s[3] = {s0, s1, s2};
+        float e[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float cm = sd3[3 * i + c] / max_sd;
+            const float f = (cm < target) ? strength * (target - cm) / target : 0.f;
+            const float hf = s[c] - mean3[3 * i + c];
+            e[c] = mean3[3 * i + c] + hf * (1.f + f);
+        }
+        float r = e[0], g = e[1], b = e[2];
+        const float maxc = lre::max_std(r, lre::max_std(g, b));
+        if (maxc > 1.f) {
+            const float scale = 1.f / maxc;
+            r *= scale;
+            g *= scale;
+            b *= scale;
+        }
+        if (mask) {
+            const float wt = lre::clamp_std(mask[i], 0.f, 1.f);
+            r = s0 + (r - s0) * wt;
+            g = s1 + (g - s1) * wt;
+            b = s2 + (b - s2) * wt;
+        }
+        out[3 * i] = r;
+        out[3 * i + 1] = g;
+        out[3 * i + 2] = b;
+    }
+}
+
 }  // namespace
 
 int lre_launch_tone_mapping(const float *d_in, float *d_out, size_t n, float after_scale,
@@ -285,6 +421,41 @@ int lre_launch_rgb3_to_cfa(const float *d_rgb3, float *d_cfa, int w, int h, uint
         for (int i = 0; i < 36; i++) xt.v[i] = xtrans[i];
     hipLaunchKernelGGL(rgb3_to_cfa_kernel, dim3(grid_for((size_t)w * h, 256)), dim3(256), 0, s,
                        d_rgb3, d_cfa, w, h, filters, xt);
+    LRE_HIP_CHECK(hipGetLastError());
+    return LRE_OK;
+}
+
+// the deviation and mean planes (3 floats per pixel each) and the maximum
+size_t lre_micro_contrast_scratch_bytes(int w, int h) {
+    return (((size_t)w * h * 12 + 255) & ~(size_t)255) * 2 + 256;
+}
+
+int lre_launch_micro_contrast(const float *d_in, float *d_out, int w, int h, float threshold,
+                              float strength, float target_contrast, const float *d_mask,
+                              void *d_scratch, hipStream_t s) {
+    const size_t n = (size_t)w * h, plane = ((n * 12 + 255) & ~(size_t)255);
+    float *mean3 = static_cast<float *>(d_scratch);
+    float *sd3 = reinterpret_cast<float *>(static_cast<char *>(d_scratch) + plane);
+    unsigned *stats = reinterpret_cast<unsigned *>(static_cast<char *>(d_scratch) + 2 * plane);
+    // create_gaussian_kernel(9, 5/3.7) (:3077-3099), in float with the C
+    // library's expf, weights summed in row order and divided by the sum
+    McTaps T;
+    {
+        const float sigma = 5.f / 3.7f;
+        float sum = 0.f;
+        for (int i = 0; i < MC_K; i++)
+            for (int j = 0; j < MC_K; j++) {
+                const float x = (float)(i - MC_R), y = (float)(j - MC_R);
+                T.k[i * MC_K + j] = expf(-(x * x + y * y) / (2.0f * sigma * sigma));
+                sum += T.k[i * MC_K + j];
+            }
+        for (int i = 0; i < MC_K * MC_K; i++) T.k[i] /= sum;
+    }
+    hipLaunchKernelGGL(mc_init, dim3(1), dim3(64), 0, s, stats);
+    hipLaunchKernelGGL(mc_stats_kernel, dim3((w + MC_T - 1) / MC_T, (h + MC_T - 1) / MC_T), dim3(MC_BLOCK), 0, s,
+                       d_in, mean3, sd3, w, h, T, stats);
+    hipLaunchKernelGGL(mc_apply_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, d_in, d_out, mean3, sd3, d_mask,
+                       n, threshold, strength, target_contrast, stats);
     LRE_HIP_CHECK(hipGetLastError());
     return LRE_OK;
 }

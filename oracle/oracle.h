@@ -38,6 +38,10 @@ int oracle_xtrans3(const float *raw3, float *rgb3, int w, int h, const signed ch
 void oracle_border_interpolate(float *rgb3, int w, int h, uint32_t filters, int border);
 int oracle_tone_mapping(const float *in3, float *out3, int w, int h, float after_scale);
 int oracle_detail_tonemap(float *rgb3, int w, int h);
+/* mask == NULL selects the threshold gate on green; in == out is allowed */
+int oracle_micro_contrast(const float *in3, float *out3, int w, int h, float threshold,
+                          float strength, float target_contrast, const float *mask);
+void oracle_mc_kernel(float k[81]);
 /* in == out selects the in-place semantics of the reference */
 int oracle_defringe(const float *in3, float *out3, int w, int h, float radius, float edge_threshold,
                     float chroma_threshold, float strength, int defringe_green,

@@ -39,6 +39,38 @@ def test_amaze_bit_exact(acc, h, w):
     assert bad == 0, f"{bad} values differ, max-abs {np.abs(got - want).max()}"
 
 
+# ---------------- micro-contrast ----------------
+from test_micro_contrast_cpu import _oracle_mc  # noqa: E402
+
+
+@pytest.mark.parametrize("kw", [dict(threshold=0.4), dict(threshold=0.7, strength=2.0, target=0.1),
+                                dict(threshold=1.3, target=0.04, mask=True)])
+def test_micro_contrast_bit_exact(acc, kw):
+    print(f"micro contrast {kw}", flush=True)
+    h, w = 480, 640
+    img = scene.render(h, w, scene.BASE_SEED + 31) * 1.3
+    mask = scene.render(h, w, scene.BASE_SEED + 32)[:, :, 1] if kw.get("mask") else None
+    args = (kw["threshold"], kw.get("strength", 8.0), kw.get("target", 0.06))
+    got = acc.enhance_micro_contrast(img, *args, mask=mask)
+    want = _oracle_mc(img, *args, mask=mask)
+    bad = np.count_nonzero(got != want)
+    assert bad == 0, f"{bad} values differ, max-abs {np.abs(got - want).max()}"
+
+
+@pytest.mark.parametrize("h,w", [(1, 1), (3, 5), (40, 3), (33, 47)])
+def test_micro_contrast_tiny_frames(acc, h, w):
+    print(f"micro contrast tiny {h}x{w}", flush=True)
+    img = scene.render(h, w, scene.BASE_SEED + h * w) * 1.1
+    assert np.array_equal(acc.enhance_micro_contrast(img, 0.2), _oracle_mc(img, 0.2))
+
+
+def test_micro_contrast_standalone_wrapper(acc):
+    from libraw_enhanced_amd.accelerator import LibRawWrapper
+    img = scene.render(64, 96, scene.BASE_SEED + 33) * 1.5
+    lw = LibRawWrapper(0)
+    assert np.array_equal(lw.enhance_micro_contrast(img), _oracle_mc(img, 1.0, 8.0, 0.06))
+
+
 def test_amaze_nyquist_patterns(acc):
     """Period-2 stripes, a checkerboard and a chirp up to 0.5 cycles/px: the
     content that AMaZE's Nyquist test flags, so its refinement paths run."""
