@@ -1,7 +1,7 @@
 // Edge-gated Gaussian defringe for gfx950 (core/cpu_accelerator.cpp:3939-5064).
 //
 // Pipeline (planes are float, H x W, in the context scratch):
-//   max(G) ─► guide / guide_signal / log R/G / log B/G ─► Sobel |∇| + max ─►
+//   max(G) ─► guide / log R/G / log B/G and the Sobel |∇| of guide_signal + max ─►
 //   5 Gaussian blurs (the support seed in the last one's epilogue) ─► blur ─►
 //   8-direction line link (the solo seeds in its epilogue) ─► 2 blurs ─►
 //   per-pixel correction.
@@ -136,45 +136,6 @@ __global__ __launch_bounds__(BLOCK) void max_green(const float *__restrict__ src
 
 LRE_DEV float max_guide_of(const unsigned *stats) { return mx(__uint_as_float(stats[0]), 1e-6f); }
 
-// guide planes and log chroma ratios (:3970-3991); out of place also seeds dst
-__global__ __launch_bounds__(BLOCK) void prep(const float *src, float *__restrict__ gl,
-                                              float *__restrict__ gs, float *__restrict__ lrg,
-                                              float *__restrict__ lbg, size_t n,
-                                              const unsigned *stats) {
-    const float max_guide = max_guide_of(stats);
-    const float ratio_eps = mx(max_guide * 1e-4f, 1e-6f);
-    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK) {
-        const float r = src[3 * i], g = src[3 * i + 1], b = src[3 * i + 2];
-        const float gg = mx(g, 0.f);
-        gl[i] = gg;
-        gs[i] = sqrtf(lre::clamp_std(gg / max_guide, 0.f, 1.f));
-        lrg[i] = logf_cr((mx(r, 0.f) + ratio_eps) / (gg + ratio_eps));
-        lbg[i] = logf_cr((mx(b, 0.f) + ratio_eps) / (gg + ratio_eps));
-    }
-}
-
-// 3x3 Sobel magnitude of guide_signal, 0 on the frame (:3996-4027)
-__global__ __launch_bounds__(BLOCK) void sobel(const float *__restrict__ s, float *__restrict__ er,
-                                               int W, int H, unsigned *stats) {
-    float m = 0.f;
-    const size_t n = (size_t)W * H;
-    const int ly = H > 1 ? H - 1 : 1, lx = W > 1 ? W - 1 : 1;
-    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK) {
-        const int y = (int)(i / W), x = (int)(i % W);
-        float mag = 0.f;
-        if (y >= 1 && y < ly && x >= 1 && x < lx) {
-            const size_t u = (size_t)(y - 1) * W, c = (size_t)y * W, d = (size_t)(y + 1) * W;
-            float gx = -s[u + x - 1] + s[u + x + 1] - 2.f * s[c + x - 1] + 2.f * s[c + x + 1] -
-                       s[d + x - 1] + s[d + x + 1];
-            float gy = -s[u + x - 1] - 2.f * s[u + x] - s[u + x + 1] + s[d + x - 1] +
-                       2.f * s[d + x] + s[d + x + 1];
-            mag = sqrtf(gx * gx + gy * gy);
-            if (mag > m) m = mag;
-        }
-        er[i] = mag;
-    }
-    block_max_to(m, &stats[1]);
-}
 
 struct Planes {
     const float *src;
@@ -217,6 +178,54 @@ LRE_DEV float support_px(const Planes &P, const Scal &C, const Gates &G, size_t 
     const float red_guard = 1.f - sstepc(0.08f, 0.25f, red_strength) * (1.f - red_purple);
     const float green_enable = (C.enable_green || is_purple) ? 1.f : 0.f;
     return c_edge * c_bright * c_hue * c_chroma * red_guard * green_enable;
+}
+
+// guide planes, log chroma ratios (:3970-3991) and the 3x3 Sobel magnitude of
+// guide_signal, 0 on the frame (:3996-4027), in one pass: a workgroup takes a
+// 64 x 16 tile and puts guide_signal of the tile and a one-pixel ring in LDS;
+// guide_signal itself is never stored.
+constexpr int PS_TX = 64, PS_TY = 16, PS_W = PS_TX + 2, PS_H = PS_TY + 2;
+
+__global__ __launch_bounds__(BLOCK) void prep_sobel(const float *src, float *__restrict__ gl,
+                                                    float *__restrict__ lrg, float *__restrict__ lbg,
+                                                    float *__restrict__ er, int W, int H, unsigned *stats) {
+    __shared__ float sgs[PS_H * PS_W];
+    const float max_guide = max_guide_of(stats);
+    const float ratio_eps = mx(max_guide * 1e-4f, 1e-6f);
+    const int x0 = blockIdx.x * PS_TX, y0 = blockIdx.y * PS_TY;
+    // ring cells outside the image only feed frame pixels, whose magnitude is 0
+    lre::stage2d<BLOCK>(
+        PS_H, PS_W,
+        [&](int r, int c) {
+            const int y = min(max(y0 - 1 + r, 0), H - 1), x = min(max(x0 - 1 + c, 0), W - 1);
+            return src[3 * ((size_t)y * W + x) + 1];
+        },
+        [&](int r, int c, float g) {
+            sgs[r * PS_W + c] = sqrtf(lre::clamp_std(mx(g, 0.f) / max_guide, 0.f, 1.f));
+        });
+    __syncthreads();
+    const int ly = H > 1 ? H - 1 : 1, lx = W > 1 ? W - 1 : 1;
+    float m = 0.f;
+    for (int t = threadIdx.x; t < PS_TX * PS_TY; t += BLOCK) {
+        const int ty = t / PS_TX, tx = t % PS_TX, y = y0 + ty, x = x0 + tx;
+        if (x >= W || y >= H) continue;
+        const size_t i = (size_t)y * W + x;
+        const float r = src[3 * i], g = src[3 * i + 1], b = src[3 * i + 2];
+        const float gg = mx(g, 0.f);
+        gl[i] = gg;
+        lrg[i] = logf_cr((mx(r, 0.f) + ratio_eps) / (gg + ratio_eps));
+        lbg[i] = logf_cr((mx(b, 0.f) + ratio_eps) / (gg + ratio_eps));
+        float mag = 0.f;
+        if (y >= 1 && y < ly && x >= 1 && x < lx) {
+            const float *u = sgs + ty * PS_W + tx, *c = u + PS_W, *d = c + PS_W;  // rows y-1, y, y+1 at x-1
+            float gx = -u[0] + u[2] - 2.f * c[0] + 2.f * c[2] - d[0] + d[2];
+            float gy = -u[0] - 2.f * u[1] - u[2] + d[0] + 2.f * d[1] + d[2];
+            mag = sqrtf(gx * gx + gy * gy);
+            if (mag > m) m = mag;
+        }
+        er[i] = mag;
+    }
+    block_max_to(m, &stats[1]);
 }
 
 // ---- separable Gaussian, clamp-to-edge, taps in the reference's order ----
@@ -807,8 +816,8 @@ int lre_launch_defringe(const float *d_in, float *d_out, int w, int h, float rad
     const int g = grid_for(n);
     hipLaunchKernelGGL(init_stats, dim3(1), dim3(64), 0, s, stats);
     hipLaunchKernelGGL(max_green, dim3(g), dim3(BLOCK), 0, s, d_in, n, stats);
-    hipLaunchKernelGGL(prep, dim3(g), dim3(BLOCK), 0, s, d_in, gl, gs, lrg, lbg, n, stats);
-    hipLaunchKernelGGL(sobel, dim3(g), dim3(BLOCK), 0, s, gs, er, w, h, stats);
+    hipLaunchKernelGGL(prep_sobel, dim3((w + PS_TX - 1) / PS_TX, (h + PS_TY - 1) / PS_TY), dim3(BLOCK), 0, s,
+                       d_in, gl, lrg, lbg, er, w, h, stats);
     LRE_HIP_CHECK(hipGetLastError());
 
     Planes P;
