@@ -190,6 +190,39 @@ int lre_hip_dev_green_matching(lre_ctx *ctx, const uint16_t *d_src, uint16_t *d_
 int lre_hip_remove_zeroes(lre_ctx *ctx, uint16_t *img4, int width, int height, uint32_t filters);
 int lre_hip_dev_remove_zeroes(lre_ctx *ctx, uint16_t *d_img4, int width, int height,
                               uint32_t filters, void *stream);
+/* The LibRaw fields that scale_colors (core/libraw_wrapper.cpp:602-862) reads
+ * and updates, outside the image itself. */
+typedef struct lre_wb_meta {
+    float user_mul[4];             /* params.user_mul; user_mul[0] == 0: not set */
+    int use_auto_wb;               /* params.use_auto_wb */
+    int use_camera_wb;             /* params.use_camera_wb */
+    int camerawb_daylight_fallback; /* rawparams.options has CAMERAWB_FALLBACK_TO_DAYLIGHT */
+    int as_shot_wb_applied;        /* color.as_shot_wb_applied (Nikon sRAW) */
+    int colors;                    /* idata.colors */
+    uint32_t filters;              /* idata.filters */
+    float cam_mul[4];              /* color.cam_mul */
+    float pre_mul[4];              /* color.pre_mul, updated */
+    uint16_t white[8][8];          /* color.white */
+    unsigned cblack[10];           /* color.cblack[0..9]; updated (pattern folded) */
+    unsigned black;                /* color.black */
+    unsigned maximum;              /* color.maximum; updated (black subtracted) */
+    unsigned warnings;             /* LRE_WARN_BAD_CAMERA_WB when camera WB is unusable */
+} lre_wb_meta;
+#define LRE_WARN_BAD_CAMERA_WB 0x4u /* LibRaw's LIBRAW_WARN_BAD_CAMERA_WB */
+
+/* Whether scale_colors takes the image-wide automatic multipliers
+ * (core/libraw_wrapper.cpp:620-629): then the caller passes the result of
+ * lre_hip_auto_wb to lre_scale_colors_meta.  Host logic only. */
+int lre_wb_needs_auto(const lre_wb_meta *meta);
+/* The multiplier part of scale_colors (core/libraw_wrapper.cpp:612-862 without
+ * the image-wide sums and the radial aberration): user, automatic and camera
+ * multipliers, the Nikon sRAW daylight case, pre_mul normalisation, maximum
+ * less black, scale_mul = pre_mul / min(pre_mul) (pre_mul itself on the
+ * automatic path), and the folding of a 1x1 / 2x2 cblack pattern into
+ * cblack[0..3].  auto_pre_mul: required when lre_wb_needs_auto, else ignored.
+ * Host logic only (no device work). */
+int lre_scale_colors_meta(lre_wb_meta *meta, const float auto_pre_mul[4], float scale_mul[4]);
+
 /* The automatic white-balance multipliers of scale_colors
  * (core/libraw_wrapper.cpp:621-663): the mean of every native channel over the
  * image (channel 3 folded into 1), pre_mul = (G/R, 1, G/B, 1). */

@@ -113,6 +113,21 @@ _SIGS = {
                               ctypes.c_float, ctypes.c_int, ctypes.c_int, _vp], ctypes.c_int),
 }
 
+
+
+class WbMeta(ctypes.Structure):
+    """lre_wb_meta: the LibRaw fields scale_colors reads and writes."""
+    _fields_ = [("user_mul", ctypes.c_float * 4), ("use_auto_wb", ctypes.c_int),
+                ("use_camera_wb", ctypes.c_int), ("camerawb_daylight_fallback", ctypes.c_int),
+                ("as_shot_wb_applied", ctypes.c_int), ("colors", ctypes.c_int), ("filters", ctypes.c_uint32),
+                ("cam_mul", ctypes.c_float * 4), ("pre_mul", ctypes.c_float * 4),
+                ("white", (ctypes.c_uint16 * 8) * 8), ("cblack", ctypes.c_uint * 10), ("black", ctypes.c_uint),
+                ("maximum", ctypes.c_uint), ("warnings", ctypes.c_uint)]
+
+
+_SIGS["lre_wb_needs_auto"] = ([ctypes.POINTER(WbMeta)], ctypes.c_int)
+_SIGS["lre_scale_colors_meta"] = ([ctypes.POINTER(WbMeta), _f32p, _f32p], ctypes.c_int)
+
 _lock = threading.Lock()
 
 
@@ -432,3 +447,50 @@ class LibRawWrapper:
         # distinct buffers, edge 0.1, chroma 0.15, green scale 0.3
         return self._acc.defringe(_rgb(image), None, radius, 0.1, 0.15, strength,
                                   defringe_green, 0.3)
+
+
+_WB_SCALARS = ("use_auto_wb", "use_camera_wb", "camerawb_daylight_fallback", "as_shot_wb_applied", "colors",
+               "filters", "black", "maximum", "warnings")
+
+
+def wb_meta(**fields) -> WbMeta:
+    """An lre_wb_meta from keyword fields (arrays as sequences); pre_mul
+    defaults to 1, colors to 3, everything else to 0."""
+    m = WbMeta()
+    m.colors = 3
+    for i in range(4):
+        m.pre_mul[i] = 1.0
+    for k, v in fields.items():
+        if k in ("user_mul", "cam_mul", "pre_mul", "cblack"):
+            arr = getattr(m, k)
+            for i, x in enumerate(v):
+                arr[i] = x
+        elif k == "white":
+            w = np.asarray(v, np.uint16).reshape(8, 8)
+            for r in range(8):
+                for c in range(8):
+                    m.white[r][c] = int(w[r, c])
+        elif k in _WB_SCALARS:
+            setattr(m, k, v)
+        else:
+            raise LibRawInvalidArgument(f"unknown white-balance field {k!r}")
+    return m
+
+
+def wb_needs_auto(meta: WbMeta) -> bool:
+    """True when scale_colors takes the image-wide multipliers of auto_wb
+    (core/libraw_wrapper.cpp:621-630)."""
+    return bool(native().lre_wb_needs_auto(ctypes.byref(meta)))
+
+
+def scale_colors_multipliers(meta: WbMeta, auto_pre_mul=None) -> np.ndarray:
+    """scale_colors' multiplier logic (core/libraw_wrapper.cpp:612-862): updates
+    meta's pre_mul, maximum, cblack and warnings in place and returns the four
+    scale_mul values.  auto_pre_mul (HIPAccelerator.auto_wb's result) is
+    required when wb_needs_auto(meta)."""
+    out = np.zeros(4, np.float32)
+    a = None if auto_pre_mul is None else np.ascontiguousarray(auto_pre_mul, np.float32)
+    if a is not None and a.size != 4:
+        raise LibRawInvalidArgument("auto_pre_mul needs four values")
+    _check(native().lre_scale_colors_meta(ctypes.byref(meta), None if a is None else _p(a), _p(out)))
+    return out

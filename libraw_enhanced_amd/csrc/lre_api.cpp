@@ -5,6 +5,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <string>
 
 #include "lre_common.h"
@@ -662,3 +663,70 @@ int lre_hip_white_balance(lre_ctx *ctx, const uint16_t *img4, float *rgb, int w,
 }
 
 }  // extern "C"
+
+// ---- scale_colors multiplier logic (core/libraw_wrapper.cpp:612-862) --------
+int lre_wb_needs_auto(const lre_wb_meta *m) {
+    if (!m) return 0;
+    return m->use_auto_wb ||
+           (m->use_camera_wb &&
+            (m->cam_mul[0] < -0.5 || (m->cam_mul[0] <= 0.00001f && !m->camerawb_daylight_fallback)));
+}
+
+int lre_scale_colors_meta(lre_wb_meta *m, const float auto_pre_mul[4], float scale_mul[4]) {
+    if (!m || !scale_mul) return set_arg_error("null pointer");
+    const bool use_auto = lre_wb_needs_auto(m) != 0;
+    if (use_auto && !auto_pre_mul) return set_arg_error("automatic WB needs the image-wide multipliers");
+    // 1. user multipliers (:612-615)
+    if (m->user_mul[0] != 0.f) std::memcpy(m->pre_mul, m->user_mul, sizeof(m->pre_mul));
+    // 2. automatic multipliers (:620-663), from the image-wide sums
+    if (use_auto) std::memcpy(m->pre_mul, auto_pre_mul, sizeof(m->pre_mul));
+    // 3. camera white balance from the white samples (:743-779)
+    if (m->use_camera_wb && m->cam_mul[0] > 0.00001f) {
+        unsigned sum[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (unsigned row = 0; row < 8; row++)
+            for (unsigned col = 0; col < 8; col++) {
+                const unsigned c = lre::bayer_native(m->filters, row, col);
+                // unsigned difference stored into an int, as in the reference
+                const int val = (int)((unsigned)m->white[row][col] - m->cblack[c]);
+                if (val > 0) sum[c] += (unsigned)val;
+                sum[c + 4]++;
+            }
+        if (m->as_shot_wb_applied) {
+            m->pre_mul[0] = m->pre_mul[1] = m->pre_mul[2] = m->pre_mul[3] = 1.0f;
+        } else if (sum[0] && sum[1] && sum[2] && sum[3]) {
+            for (int c = 0; c < 4; c++) m->pre_mul[c] = (float)sum[c + 4] / sum[c];
+        } else if (m->cam_mul[0] > 0.00001f && m->cam_mul[2] > 0.00001f) {
+            std::memcpy(m->pre_mul, m->cam_mul, sizeof(m->pre_mul));
+        } else {
+            m->warnings |= LRE_WARN_BAD_CAMERA_WB;
+        }
+    }
+    // 4. Nikon sRAW daylight (:784-795)
+    if (m->as_shot_wb_applied && !m->use_camera_wb && !m->use_auto_wb && m->cam_mul[0] > 0.00001f &&
+        m->cam_mul[1] > 0.00001f && m->cam_mul[2] > 0.00001f)
+        for (int c = 0; c < 3; c++) m->pre_mul[c] /= m->cam_mul[c];
+    // 5. normalisation (:800-806)
+    if (m->pre_mul[1] == 0) m->pre_mul[1] = 1;
+    if (m->pre_mul[3] == 0) m->pre_mul[3] = m->colors < 4 ? m->pre_mul[1] : 1;
+    // 7. scaling multipliers (:819-845); maximum is unsigned, as LibRaw's
+    m->maximum -= m->black;
+    if (!use_auto) {
+        double dmin = std::numeric_limits<double>::max();
+        for (int c = 0; c < 4; c++)
+            if (dmin > m->pre_mul[c]) dmin = m->pre_mul[c];
+        if (dmin > 0.00001 && m->maximum > 0)
+            for (int c = 0; c < 4; c++) scale_mul[c] = (float)(m->pre_mul[c] / dmin);
+        else
+            for (int c = 0; c < 4; c++) scale_mul[c] = 1.0f;
+    } else {
+        for (int c = 0; c < 4; c++) scale_mul[c] = m->pre_mul[c];
+    }
+    // 8. a 1x1 or 2x2 cblack pattern folded into the per-channel blacks (:850-862)
+    unsigned *cb = m->cblack;
+    if (m->filters > 1000 && (cb[4] + 1) / 2 == 1 && (cb[5] + 1) / 2 == 1) {
+        for (unsigned c = 0; c < 4; c++)
+            cb[lre::bayer_native(m->filters, c / 2, c % 2)] += cb[6 + c / 2 % cb[4] * cb[5] + c % 2 % cb[5]];
+        cb[4] = cb[5] = 0;
+    }
+    return LRE_OK;
+}
