@@ -96,6 +96,19 @@ int lre_hip_micro_contrast(lre_ctx *ctx, const float *in_rgb, float *out_rgb, in
                            int height, float threshold, float strength, float target_contrast,
                            const float *mask);
 
+/* LibRawWrapper::Impl::recover_highlights (core/libraw_wrapper.cpp:1095-1383),
+ * the float highlight recovery of postprocess for highlight_mode > 2
+ * (:1659-1683) and of recover_highlights_numpy (:3063-3095).
+ * saturation_threshold: the pipeline's maximum / data_maximum.  mask_out: w x h
+ * floats receiving the soft highlight mask of the input, or NULL (the
+ * reference computes it only when asked).  wb_rgb: the effective R, G, B white
+ * balance (it only changes the mask), or NULL.  max_val_out: the largest
+ * channel of the highlight pixels before the rewrite (the value whose
+ * non-zeroness the reference returns), or NULL.  in == out is allowed. */
+int lre_hip_recover_highlights(lre_ctx *ctx, const float *in_rgb, float *out_rgb, int width,
+                               int height, float saturation_threshold, float *mask_out,
+                               const float *wb_rgb, float *max_val_out);
+
 /* Accelerator::defringe (core/accelerator.h:344-351, core/cpu_accelerator.cpp:3939-5064),
  * defaults radius 10, edge_threshold 0.1, chroma_threshold 0.15, strength 10,
  * defringe_green 0, green_strength_scale 0.3.  in_rgb == out_rgb selects the
@@ -155,6 +168,11 @@ int lre_hip_dev_detail_tonemap(lre_ctx *ctx, const float *d_in, float *d_out, in
 int lre_hip_dev_micro_contrast(lre_ctx *ctx, const float *d_in, float *d_out, int width,
                                int height, float threshold, float strength,
                                float target_contrast, const float *d_mask, void *stream);
+/* Highlight recovery on resident buffers; d_mask and d_max_val are device
+ * pointers or NULL, wb_rgb a host array of 3 or NULL; d_in == d_out is allowed. */
+int lre_hip_dev_recover_highlights(lre_ctx *ctx, const float *d_in, float *d_out, int width,
+                                   int height, float saturation_threshold, float *d_mask,
+                                   const float *wb_rgb, float *d_max_val, void *stream);
 /* Defringe on resident buffers; d_in == d_out selects the in-place semantics. */
 int lre_hip_dev_defringe(lre_ctx *ctx, const float *d_in, float *d_out, int width, int height,
                          float radius, float edge_threshold, float chroma_threshold,

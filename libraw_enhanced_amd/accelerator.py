@@ -59,6 +59,10 @@ _SIGS = {
                                 ctypes.c_float, ctypes.c_float, _f32p], ctypes.c_int),
     "lre_hip_dev_micro_contrast": ([_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_float,
                                     ctypes.c_float, ctypes.c_float, _vp, _vp], ctypes.c_int),
+    "lre_hip_recover_highlights": ([_vp, _f32p, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_float, _f32p,
+                                    _f32p, _f32p], ctypes.c_int),
+    "lre_hip_dev_recover_highlights": ([_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, _vp, _f32p,
+                                        _vp, _vp], ctypes.c_int),
     "lre_hip_defringe": ([_vp, _f32p, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_float,
                           ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int,
                           ctypes.c_float], ctypes.c_int),
@@ -258,6 +262,27 @@ class HIPAccelerator:
                                                target_contrast, None if m is None else _p(m)))
         return out
 
+    def recover_highlights(self, rgb, saturation_threshold: float, want_mask: bool = False, wb_rgb=None):
+        """LibRawWrapper::Impl::recover_highlights (core/libraw_wrapper.cpp:1095-1383)
+        on a copy of rgb.  Returns (image, mask or None, max_val): the soft
+        highlight mask of the input is computed only when want_mask, wb_rgb (R, G,
+        B) only changes that mask, and max_val is the largest channel of the
+        highlight pixels before the rewrite."""
+        src = _rgb(rgb)
+        h, w, _ = src.shape
+        out = np.empty_like(src)
+        mask = np.empty((h, w), np.float32) if want_mask else None
+        wb = None
+        if wb_rgb is not None:
+            wb = np.ascontiguousarray(wb_rgb, np.float32)
+            if wb.size != 3:
+                raise LibRawInvalidArgument("wb_rgb needs three values")
+        mv = np.zeros(1, np.float32)
+        _check(native().lre_hip_recover_highlights(self._h, _p(src), _p(out), w, h, saturation_threshold,
+                                                   None if mask is None else _p(mask),
+                                                   None if wb is None else _p(wb), _p(mv)))
+        return out, mask, float(mv[0])
+
     def defringe(self, rgb_input, rgb_output=None, radius: float = 10.0,
                  edge_threshold: float = 0.1, chroma_threshold: float = 0.15,
                  strength: float = 10.0, defringe_green: bool = False,
@@ -440,6 +465,14 @@ class LibRawWrapper:
         if threshold < 0.0:
             threshold = 1.0
         return self._acc.enhance_micro_contrast(_rgb(image), threshold, strength, target_contrast)
+
+    def recover_highlights(self, image, threshold: float = -1.0):
+        # core/libraw_wrapper.cpp:3063-3095: a new array; threshold < 0 takes
+        # maximum / data_maximum of the loaded raw, 1 when there is none
+        # (:3004-3011); no raw is ever loaded here.  No mask, no white balance.
+        if threshold < 0.0:
+            threshold = 1.0
+        return self._acc.recover_highlights(_rgb(image), threshold)[0]
 
     def defringe(self, image, radius: float = 10.0, strength: float = 10.0,
                  defringe_green: bool = False):

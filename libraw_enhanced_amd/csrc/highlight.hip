@@ -1,0 +1,458 @@
+// Float highlight recovery for gfx950, core/libraw_wrapper.cpp:1095-1383
+// (Impl::recover_highlights).  Results equal the reference's serial order
+// bit for bit:
+//
+//  1. hr_scan     the soft highlight mask and, per chunk of HR_CHUNK pixels,
+//                 the number of pixels that enter the R/G, B/G ratio sums;
+//                 the maximum channel of the highlight pixels (:1149-1215).
+//  2. hr_sums     the ratio sums are serial float sums in raster order
+//                 (:1180-1190): one wave walks the non-empty chunks in order
+//                 and adds the quotients one by one.
+//  3. hr_rewrite  per 64x16 tile with a one-pixel halo: the highlight
+//                 rewrite (:1216-1241) of every staged pixel, then the
+//                 8-neighbour mean of blown interior pixels (:1258-1285),
+//                 which reads the rewritten neighbours and never another
+//                 mean.  Flags the pixels the LoCA pass may change.
+//  4. hr_loca     the LoCA green lift (:1287-1374) runs over the highlight
+//                 pixels in raster order, in place, so a pixel reads the
+//                 updated values of the pixels before it.  In the skewed
+//                 coordinate t = x + 3y every pixel that (y, x) reads and
+//                 that comes earlier in raster order has a smaller t, and
+//                 every later one a larger t.  So all rows can advance
+//                 together, one t per step.  The image is cut into tiles of
+//                 64 rows x HR_TW values of t (parallelograms), and tile
+//                 (band b, column k) needs only tiles (b, k-1), (b-1, k) and
+//                 (b-1, k-1).  Each anti-diagonal b + k = d is one launch,
+//                 and a tile with no flagged pixel returns at once.
+#include "lre_common.h"
+#include "lre_internal.h"
+
+#include <algorithm>
+#include <cfloat>
+
+namespace {
+
+constexpr int HR_CHUNK = 4096;  // pixels per chunk of the ratio scan
+constexpr int HR_BLOCK = 256;
+constexpr int RW_TX = 64, RW_TY = 16;  // rewrite tile
+constexpr int RW_SX = RW_TX + 2, RW_SY = RW_TY + 2;
+constexpr int HR_TR = 64;            // LoCA tile rows (one per lane of wave 0)
+constexpr int HR_TW = 160;           // LoCA tile width in t
+constexpr int HR_SR = HR_TR + 4;     // staged rows (two above, two below)
+constexpr int HR_LW = HR_TW + 17;    // staged columns: t - 8 .. t + W + 8, odd stride
+constexpr int HR_WORDS = (HR_TW + 31) / 32;
+constexpr size_t HR_LDS =
+    sizeof(float) * 3 * HR_SR * HR_LW + HR_TR * HR_TW + sizeof(unsigned) * HR_WORDS;
+
+struct HrState {
+    float grf, gbf;     // ratio means after the division by the count
+    unsigned max_bits;  // max_val as float bits (never negative)
+    unsigned total;     // pixels that enter the ratio sums
+};
+
+struct MaskConst {
+    float lo, inv, chroma_lo, chroma_inv, inv_wb_r, inv_wb_g, inv_wb_b;
+};
+
+LRE_DEV float max3(float r, float g, float b) { return lre::max_std(r, lre::max_std(g, b)); }
+
+// Green smoothstep times neutrality (:1152-1174)
+LRE_DEV float mask_value(float r, float g, float b, const MaskConst &M) {
+    float t = (g - M.lo) * M.inv;
+    if (t < 0.f) t = 0.f;
+    else if (t > 1.f) t = 1.f;
+    const float w_g = t * t * (3.f - 2.f * t);
+    const float r_n = r * M.inv_wb_r, g_n = g * M.inv_wb_g, b_n = b * M.inv_wb_b;
+    const float mx = lre::max_std(r_n, lre::max_std(g_n, b_n));
+    const float mn = lre::min_std(r_n, lre::min_std(g_n, b_n));
+    const float chroma = (mx > 1e-6f) ? (mx - mn) / mx : 0.f;
+    float ct = (chroma - M.chroma_lo) * M.chroma_inv;
+    if (ct < 0.f) ct = 0.f;
+    else if (ct > 1.f) ct = 1.f;
+    const float neutrality = 1.f - ct * ct * (3.f - 2.f * ct);
+    return w_g * neutrality;
+}
+
+LRE_DEV bool is_ratio_px(float r, float g, float b, float sat) {
+    return r >= sat && b >= sat && r < 0.95f && g >= sat && g < 0.95f && b < 0.95f;
+}
+
+__global__ void hr_init(HrState *st, unsigned *tile_count, int ntiles) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        st->grf = st->gbf = 1.f;
+        st->max_bits = 0u;
+        st->total = 0u;
+    }
+    for (int k = i; k < ntiles; k += gridDim.x * blockDim.x) tile_count[k] = 0u;
+}
+
+__global__ __launch_bounds__(HR_BLOCK) void hr_scan(const float *__restrict__ in, float *__restrict__ mask,
+                                                     size_t n, float sat, MaskConst M,
+                                                     unsigned *__restrict__ chunk_count, HrState *st) {
+    __shared__ unsigned s_cnt[HR_BLOCK / 64];
+    __shared__ float s_max[HR_BLOCK / 64];
+    const size_t base = (size_t)blockIdx.x * HR_CHUNK;
+    unsigned cnt = 0;
+    float mx = 0.f;
+#pragma unroll 4
+    for (int k = 0; k < HR_CHUNK; k += HR_BLOCK) {
+        const size_t i = base + k + threadIdx.x;
+        if (i >= n) break;
+        const float r = in[3 * i], g = in[3 * i + 1], b = in[3 * i + 2];
+        if (mask) mask[i] = mask_value(r, g, b, M);
+        if (r >= sat && b >= sat) {
+            // max_val over the channels of the highlight pixels (:1199-1203)
+            if (r > mx) mx = r;
+            if (g > mx) mx = g;
+            if (b > mx) mx = b;
+            cnt += is_ratio_px(r, g, b, sat);
+        }
+    }
+    for (int o = 32; o; o >>= 1) {
+        cnt += __shfl_xor(cnt, o);
+        const float other = __shfl_xor(mx, o);
+        mx = other > mx ? other : mx;
+    }
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        s_cnt[wv] = cnt;
+        s_max[wv] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned c = 0;
+        float m = 0.f;
+        for (int v = 0; v < HR_BLOCK / 64; v++) {
+            c += s_cnt[v];
+            m = s_max[v] > m ? s_max[v] : m;
+        }
+        chunk_count[blockIdx.x] = c;
+        if (c) atomicAdd(&st->total, c);
+        if (m > 0.f) atomicMax(&st->max_bits, __float_as_uint(m));
+    }
+}
+
+// One wave: the serial float sums of r/g and b/g over the ratio pixels in
+// raster order (:1180-1190), then the means (:1184-1190).  Every lane holds
+// the same running sums; readlane hands them the quotients one at a time.
+__global__ __launch_bounds__(64) void hr_sums(const float *__restrict__ in, size_t n,
+                                              const unsigned *__restrict__ chunk_count, int nchunks,
+                                              float sat, HrState *st) {
+    const int lane = threadIdx.x;
+    float grf = 0.f, gbf = 0.f, count = 0.f;
+    const int walk = st->total ? nchunks : 0;
+    for (int c0 = 0; c0 < walk; c0 += 64) {
+        const unsigned cnt = c0 + lane < nchunks ? chunk_count[c0 + lane] : 0u;
+        unsigned long long live = __ballot(cnt != 0u);
+        while (live) {
+            const int j = __ffsll(live) - 1;
+            live &= live - 1;
+            const size_t base = (size_t)(c0 + j) * HR_CHUNK;
+            const size_t end = base + HR_CHUNK < n ? base + HR_CHUNK : n;
+            for (size_t p = base; p < end; p += 64) {
+                const size_t i = p + lane;
+                bool q = false;
+                float qr = 0.f, qb = 0.f;
+                if (i < end) {
+                    const float r = in[3 * i], g = in[3 * i + 1], b = in[3 * i + 2];
+                    q = is_ratio_px(r, g, b, sat);
+                    if (q) {
+                        qr = r / g;
+                        qb = b / g;
+                    }
+                }
+                unsigned long long bal = __ballot(q);
+                while (bal) {
+                    const int k = __ffsll(bal) - 1;
+                    bal &= bal - 1;
+                    grf += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(qr), k));
+                    gbf += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(qb), k));
+                    count += 1.f;
+                }
+            }
+        }
+    }
+    if (lane == 0) {
+        if (count > 0.f) {
+            grf /= count;
+            gbf /= count;
+        } else {
+            grf = 1.f;
+            gbf = 1.f;
+        }
+        st->grf = grf;
+        st->gbf = gbf;
+    }
+}
+
+// Highlight rewrite of one pixel (:1216-1241); returns whether it is blown.
+LRE_DEV bool rewrite_px(float &r, float &g, float &b, float sat, float grf, float gbf) {
+    g = (r / grf + b / gbf) * 0.5f;
+    const float sp = (r < b) ? r : b;
+    const float sl = (lre::min_std(sp, 1.f) - sat) / (sat);
+    r = r * (1.f - sl) + (g * grf) * sl;
+    b = b * (1.f - sl) + (g * gbf) * sl;
+    g = (r / grf + b / gbf) * 0.5f;
+    return g >= 0.95f;
+}
+
+// Stages the rewritten 66 x 18 window, takes the 8-neighbour mean for blown
+// interior pixels, writes the tile and flags the LoCA candidates: highlight
+// pixels at least two from the frame whose own values pass the LoCA tests that
+// read only the pixel itself (the other tests read neighbours).
+__global__ __launch_bounds__(HR_BLOCK) void hr_rewrite(const float *__restrict__ in, float *__restrict__ out,
+                                                       int w, int h, float sat, const HrState *st,
+                                                       unsigned char *__restrict__ flags,
+                                                       unsigned *__restrict__ tile_count, int tcols) {
+    __shared__ float sR[RW_SY][RW_SX], sG[RW_SY][RW_SX], sB[RW_SY][RW_SX];
+    __shared__ unsigned char sW[RW_SY][RW_SX];  // bit 0 highlight, bit 1 blown
+    const int x0 = blockIdx.x * RW_TX, y0 = blockIdx.y * RW_TY;
+    const float grf = st->grf, gbf = st->gbf;
+    for (int e = threadIdx.x; e < RW_SY * RW_SX; e += HR_BLOCK) {
+        const int sy = e / RW_SX, sx = e - sy * RW_SX;
+        const int gy = y0 - 1 + sy, gx = x0 - 1 + sx;
+        float r = 0.f, g = 0.f, b = 0.f;
+        unsigned char f = 0;
+        if (gy >= 0 && gy < h && gx >= 0 && gx < w) {
+            const size_t i = (size_t)gy * w + gx;
+            r = in[3 * i];
+            g = in[3 * i + 1];
+            b = in[3 * i + 2];
+            if (r >= sat && b >= sat) f = rewrite_px(r, g, b, sat, grf, gbf) ? 3 : 1;
+        }
+        sR[sy][sx] = r;
+        sG[sy][sx] = g;
+        sB[sy][sx] = b;
+        sW[sy][sx] = f;
+    }
+    __syncthreads();
+    const float sat09 = sat * 0.90f;
+    for (int e = threadIdx.x; e < RW_TX * RW_TY; e += HR_BLOCK) {
+        const int ty = e / RW_TX, tx = e - ty * RW_TX;
+        const int gy = y0 + ty, gx = x0 + tx;
+        if (gy >= h || gx >= w) continue;
+        const int sy = ty + 1, sx = tx + 1;
+        float v[3] = {sR[sy][sx], sG[sy][sx], sB[sy][sx]};
+        const unsigned f = sW[sy][sx];
+        if ((f & 2) && gx >= 1 && gx <= w - 2 && gy >= 1 && gy <= h - 2) {
+            // neighbours summed in the reference's order, then times 1/8
+            auto mean8 = [&](const float(&S)[RW_SY][RW_SX]) {
+                return (S[sy - 1][sx - 1] + S[sy - 1][sx] + S[sy - 1][sx + 1] + S[sy][sx - 1] + S[sy][sx + 1] +
+                        S[sy + 1][sx - 1] + S[sy + 1][sx] + S[sy + 1][sx + 1]) *
+                       (1.f / 8.f);
+            };
+            v[0] = mean8(sR);
+            v[1] = mean8(sG);
+            v[2] = mean8(sB);
+        }
+        const size_t i = (size_t)gy * w + gx;
+        out[3 * i] = v[0];
+        out[3 * i + 1] = v[1];
+        out[3 * i + 2] = v[2];
+        const float l = max3(v[0], v[1], v[2]);
+        const float rb = 0.5f * (v[0] + v[2]);
+        const bool cand = (f & 1) && gx >= 2 && gx <= w - 3 && gy >= 2 && gy <= h - 3 && !(l < sat09) &&
+                          !(rb <= v[1] * 1.04f);
+        flags[i] = cand;
+        if (cand) atomicAdd(&tile_count[(gy / HR_TR) * tcols + (gx + 3 * gy) / HR_TW], 1u);
+    }
+}
+
+// LoCA green lift of one pixel at LDS index c (:1299-1372).  Neighbour
+// (dy, dx) sits at c + dy * (HR_LW + 3) + dx in the skewed layout.
+LRE_DEV void loca_px(float *R, float *G, float *B, int c, float sat, float max_val) {
+    const float p0 = R[c], p1 = G[c], p2 = B[c];
+    const float l = max3(p0, p1, p2);
+    if (l < sat * 0.90f) return;
+    constexpr int DY = HR_LW + 3;
+    float local_max = 0.0f, local_min = FLT_MAX;
+#pragma unroll
+    for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+        for (int dx = -1; dx <= 1; ++dx) {
+            const int o = c + dy * DY + dx;
+            const float nl = max3(R[o], G[o], B[o]);
+            local_max = lre::max_std(local_max, nl);
+            local_min = lre::min_std(local_min, nl);
+        }
+    if ((local_max - local_min) < 0.015f) return;
+    const float rb = 0.5f * (p0 + p2);
+    if (rb <= p1 * 1.04f) return;
+    double ratio_sum = 0.0, ratio_w = 0.0;
+    const float sat075 = sat * 0.75f;
+#pragma unroll
+    for (int dy = -2; dy <= 2; ++dy)
+#pragma unroll
+        for (int dx = -2; dx <= 2; ++dx) {
+            const int o = c + dy * DY + dx;
+            const float nr = R[o], ng = G[o], nb = B[o];
+            const float nrb = 0.5f * (nr + nb);
+            const float nl = max3(nr, ng, nb);
+            if (nl < sat075 || nrb <= 1e-6f) continue;
+            const float ratio = ng / nrb;
+            if (ratio < 0.70f || ratio > 1.35f) continue;
+            const double wt = 1.0 / (double)(1 + (dx < 0 ? -dx : dx) + (dy < 0 ? -dy : dy));
+            ratio_sum += ratio * wt;
+            ratio_w += wt;
+        }
+    const float local_ratio = (float)(ratio_w > 0.0 ? ratio_sum / ratio_w : 1.0);
+    const float desired_g = rb * lre::clamp_std(local_ratio, 0.95f, 1.08f);
+    const float g_missing = desired_g - p1;
+    if (g_missing <= 0.0f) return;
+    const float sat_w =
+        lre::clamp_std((l - sat) / lre::max_std(1e-6f, lre::max_std(1.0f, max_val) - sat), 0.0f, 1.0f);
+    const float edge_w = lre::clamp_std((local_max - local_min) / 0.08f, 0.0f, 1.0f);
+    const float alpha = lre::clamp_std(0.30f + 0.55f * sat_w * edge_w, 0.0f, 0.90f);
+    const float g_boost = alpha * lre::min_std(g_missing, (rb - p1) * 0.90f);
+    float q0 = p0, q1 = p1 + g_boost, q2 = p2;
+    const float rb_residual = lre::max_std(0.0f, 0.5f * (q0 + q2) - q1);
+    if (rb_residual > 0.0f) {
+        const float trim = lre::min_std(rb_residual * alpha * 0.45f, 0.5f * (q0 + q2) * 0.12f);
+        q0 = lre::max_std(0.0f, q0 - trim);
+        q2 = lre::max_std(0.0f, q2 - trim);
+    }
+    q1 = lre::max_std(0.0f, q1);
+    R[c] = q0;
+    G[c] = q1;
+    B[c] = q2;
+}
+
+// Tile (b_lo + blockIdx.x, d - b) of the anti-diagonal d.  Staged row sr holds
+// image row Y - 2 + sr, column j holds x = T - 3y - 8 + j, so the pixel of tile
+// row r at step s is at (r + 2, s + 8) for every row.
+__global__ __launch_bounds__(HR_BLOCK) void hr_loca(float *img, int w, int h, float sat, const HrState *st,
+                                                    const unsigned char *__restrict__ flags,
+                                                    const unsigned *__restrict__ tile_count, int tcols,
+                                                    int d, int b_lo) {
+    extern __shared__ __align__(16) float lds[];
+    float *R = lds, *G = R + HR_SR * HR_LW, *B = G + HR_SR * HR_LW;
+    unsigned char *F = reinterpret_cast<unsigned char *>(B + HR_SR * HR_LW);
+    unsigned *step_any = reinterpret_cast<unsigned *>(F + HR_TR * HR_TW);
+    const int b = b_lo + (int)blockIdx.x, k = d - b;
+    if (tile_count[b * tcols + k] == 0u) return;
+    const int Y = b * HR_TR, T = k * HR_TW;
+    if (threadIdx.x < HR_WORDS) step_any[threadIdx.x] = 0u;
+    for (int e = threadIdx.x; e < HR_SR * HR_LW; e += HR_BLOCK) {
+        const int sr = e / HR_LW, j = e - sr * HR_LW;
+        const int y = Y - 2 + sr, x = T - 3 * y - 8 + j;
+        float r = 0.f, g = 0.f, bb = 0.f;
+        if (y >= 0 && y < h && x >= 0 && x < w) {
+            const size_t i = (size_t)y * w + x;
+            r = img[3 * i];
+            g = img[3 * i + 1];
+            bb = img[3 * i + 2];
+        }
+        R[e] = r;
+        G[e] = g;
+        B[e] = bb;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < HR_TR * HR_TW; e += HR_BLOCK) {
+        const int r = e / HR_TW, s = e - r * HR_TW;
+        const int y = Y + r, x = T + s - 3 * y;
+        unsigned char f = 0;
+        if (y < h && x >= 0 && x < w) f = flags[(size_t)y * w + x];
+        F[e] = f;
+        if (f) atomicOr(&step_any[s >> 5], 1u << (s & 31));
+    }
+    __syncthreads();
+    const float max_val = __uint_as_float(st->max_bits);
+    const int r = threadIdx.x;
+    for (int wd = 0; wd < HR_WORDS; wd++) {
+        unsigned bits = step_any[wd];
+        while (bits) {
+            const int s = wd * 32 + __ffs(bits) - 1;
+            bits &= bits - 1;
+            if (r < HR_TR && F[r * HR_TW + s]) loca_px(R, G, B, (r + 2) * HR_LW + s + 8, sat, max_val);
+            __syncthreads();
+        }
+    }
+    for (int e = threadIdx.x; e < HR_TR * HR_TW; e += HR_BLOCK) {
+        if (!F[e]) continue;
+        const int rr = e / HR_TW, s = e - rr * HR_TW;
+        const int y = Y + rr, x = T + s - 3 * y;
+        const int c = (rr + 2) * HR_LW + s + 8;
+        const size_t i = (size_t)y * w + x;
+        img[3 * i] = R[c];
+        img[3 * i + 1] = G[c];
+        img[3 * i + 2] = B[c];
+    }
+}
+
+__global__ void hr_emit(const HrState *st, float *max_val) { *max_val = __uint_as_float(st->max_bits); }
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+struct HrLayout {
+    int nchunks, bands, tcols;
+    size_t o_counts, o_tiles, o_flags, bytes;
+};
+
+HrLayout hr_layout(int w, int h) {
+    HrLayout L;
+    const size_t n = (size_t)w * h;
+    L.nchunks = (int)((n + HR_CHUNK - 1) / HR_CHUNK);
+    L.bands = (h + HR_TR - 1) / HR_TR;
+    const long long tspan = (long long)w + 3LL * (h - 1);  // t = x + 3y < w + 3(h - 1)
+    L.tcols = (int)((tspan + HR_TW - 1) / HR_TW);
+    L.o_counts = 256;
+    L.o_tiles = L.o_counts + align256((size_t)L.nchunks * 4);
+    L.o_flags = L.o_tiles + align256((size_t)L.bands * L.tcols * 4);
+    L.bytes = L.o_flags + align256(n);
+    return L;
+}
+
+hipError_t hr_lds_attr() {
+    static const hipError_t rc =
+        hipFuncSetAttribute((const void *)hr_loca, hipFuncAttributeMaxDynamicSharedMemorySize, (int)HR_LDS);
+    return rc;
+}
+
+}  // namespace
+
+size_t lre_highlight_scratch_bytes(int w, int h) { return hr_layout(w, h).bytes; }
+
+int lre_launch_recover_highlights(const float *d_in, float *d_out, int w, int h, float sat, float *d_mask,
+                                  const float *wb_rgb, float *d_max_val, void *d_scratch, hipStream_t s) {
+    if (d_in == d_out) {
+        lre_set_error("recover_highlights launcher needs distinct input and output");
+        return LRE_ERR_ARG;
+    }
+    const HrLayout L = hr_layout(w, h);
+    const size_t n = (size_t)w * h;
+    char *base = static_cast<char *>(d_scratch);
+    HrState *st = reinterpret_cast<HrState *>(base);
+    unsigned *chunk_count = reinterpret_cast<unsigned *>(base + L.o_counts);
+    unsigned *tile_count = reinterpret_cast<unsigned *>(base + L.o_tiles);
+    unsigned char *flags = reinterpret_cast<unsigned char *>(base + L.o_flags);
+    // mask edges and the WB undo, in float as the reference computes them (:1135-1147)
+    const float mask_t_hi = 1.00f, mask_t_lo = 0.95f;
+    MaskConst M;
+    M.lo = mask_t_lo;
+    M.inv = 1.f / std::max(1e-6f, mask_t_hi - mask_t_lo);
+    M.chroma_lo = wb_rgb ? 0.30f : 0.50f;
+    const float chroma_hi = wb_rgb ? 0.60f : 0.70f;
+    M.chroma_inv = 1.f / std::max(1e-6f, chroma_hi - M.chroma_lo);
+    M.inv_wb_r = wb_rgb ? 1.f / std::max(1e-6f, wb_rgb[0]) : 1.f;
+    M.inv_wb_g = wb_rgb ? 1.f / std::max(1e-6f, wb_rgb[1]) : 1.f;
+    M.inv_wb_b = wb_rgb ? 1.f / std::max(1e-6f, wb_rgb[2]) : 1.f;
+    const int ntiles = L.bands * L.tcols;
+    LRE_HIP_CHECK(hr_lds_attr());
+    hipLaunchKernelGGL(hr_init, dim3(std::min(64, (ntiles + 255) / 256 + 1)), dim3(256), 0, s, st, tile_count,
+                       ntiles);
+    hipLaunchKernelGGL(hr_scan, dim3(L.nchunks), dim3(HR_BLOCK), 0, s, d_in, d_mask, n, sat, M, chunk_count, st);
+    hipLaunchKernelGGL(hr_sums, dim3(1), dim3(64), 0, s, d_in, n, chunk_count, L.nchunks, sat, st);
+    hipLaunchKernelGGL(hr_rewrite, dim3((w + RW_TX - 1) / RW_TX, (h + RW_TY - 1) / RW_TY), dim3(HR_BLOCK), 0, s,
+                       d_in, d_out, w, h, sat, st, flags, tile_count, L.tcols);
+    LRE_HIP_CHECK(hipGetLastError());
+    for (int d = 0; d <= L.bands + L.tcols - 2; d++) {
+        const int b_lo = std::max(0, d - L.tcols + 1), b_hi = std::min(L.bands - 1, d);
+        hipLaunchKernelGGL(hr_loca, dim3(b_hi - b_lo + 1), dim3(HR_BLOCK), HR_LDS, s, d_out, w, h, sat, st, flags,
+                           tile_count, L.tcols, d, b_lo);
+    }
+    LRE_HIP_CHECK(hipGetLastError());
+    if (d_max_val) hipLaunchKernelGGL(hr_emit, dim3(1), dim3(1), 0, s, st, d_max_val);
+    LRE_HIP_CHECK(hipGetLastError());
+    return LRE_OK;
+}

@@ -252,6 +252,27 @@ int lre_hip_dev_micro_contrast(lre_ctx *ctx, const float *d_in, float *d_out, in
                                      ctx->scratch, lre_pick_stream(ctx, stream));
 }
 
+int lre_hip_dev_recover_highlights(lre_ctx *ctx, const float *d_in, float *d_out, int w, int h,
+                                   float saturation_threshold, float *d_mask, const float *wb_rgb,
+                                   float *d_max_val, void *stream) {
+    if (!ctx || !d_in || !d_out) return set_arg_error("null pointer");
+    if (!valid_dims(w, h)) return set_arg_error("image size");
+    const size_t n = (size_t)w * h;
+    const bool inplace = d_in == d_out;
+    int rc = lre_ensure_scratch(ctx, (inplace ? aligned(n * 12) : 0) + lre_highlight_scratch_bytes(w, h));
+    if (rc) return rc;
+    hipStream_t s = lre_pick_stream(ctx, stream);
+    Carve c(ctx->scratch);
+    const float *src = d_in;
+    if (inplace) {
+        float *copy = c.take<float>(n * 3);
+        LRE_HIP_CHECK(hipMemcpyAsync(copy, d_in, n * 12, hipMemcpyDeviceToDevice, s));
+        src = copy;
+    }
+    return lre_launch_recover_highlights(src, d_out, w, h, saturation_threshold, d_mask, wb_rgb, d_max_val,
+                                         static_cast<char *>(ctx->scratch) + c.off, s);
+}
+
 int lre_hip_dev_defringe(lre_ctx *ctx, const float *d_in, float *d_out, int w, int h,
                          float radius, float edge_threshold, float chroma_threshold,
                          float strength, int defringe_green, float green_strength_scale,
@@ -408,6 +429,31 @@ int lre_hip_micro_contrast(lre_ctx *ctx, const float *in_rgb, float *out_rgb, in
                                         ctx->stream)))
         return rc;
     return copy_out(ctx, out_rgb, d, n * 12);
+}
+
+int lre_hip_recover_highlights(lre_ctx *ctx, const float *in_rgb, float *out_rgb, int w, int h,
+                               float saturation_threshold, float *mask_out, const float *wb_rgb,
+                               float *max_val_out) {
+    if (!ctx || !in_rgb || !out_rgb) return set_arg_error("null pointer");
+    if (!valid_dims(w, h)) return set_arg_error("image size");
+    LRE_HIP_CHECK(hipSetDevice(ctx->device));
+    const size_t n = (size_t)w * h;
+    int rc = lre_ensure_scratch(ctx, 2 * aligned(n * 12) + (mask_out ? aligned(n * 4) : 0) + 256 +
+                                         lre_highlight_scratch_bytes(w, h));
+    if (rc) return rc;
+    Carve c(ctx->scratch);
+    float *d_in = c.take<float>(n * 3);
+    float *d_out = c.take<float>(n * 3);
+    float *d_mask = mask_out ? c.take<float>(n) : nullptr;
+    float *d_max = c.take<float>(1);
+    void *ws = static_cast<char *>(ctx->scratch) + c.off;
+    if ((rc = copy_in(ctx, d_in, in_rgb, n * 12))) return rc;
+    if ((rc = lre_launch_recover_highlights(d_in, d_out, w, h, saturation_threshold, d_mask, wb_rgb, d_max, ws,
+                                            ctx->stream)))
+        return rc;
+    if (mask_out && (rc = copy_out(ctx, mask_out, d_mask, n * 4))) return rc;
+    if (max_val_out && (rc = copy_out(ctx, max_val_out, d_max, 4))) return rc;
+    return copy_out(ctx, out_rgb, d_out, n * 12);
 }
 
 int lre_hip_defringe(lre_ctx *ctx, const float *in_rgb, float *out_rgb, int w, int h,

@@ -32,6 +32,7 @@ inline hipStream_t lre_pick_stream(lre_ctx *ctx, void *stream) {
 size_t lre_amaze_scratch_bytes(lre_ctx *ctx, int w, int h);
 size_t lre_detail_tm_scratch_bytes(int w, int h);
 size_t lre_micro_contrast_scratch_bytes(int w, int h);
+size_t lre_highlight_scratch_bytes(int w, int h);
 size_t lre_defringe_scratch_bytes(int w, int h);
 size_t lre_axial_scratch_bytes(int w, int h);
 size_t lre_xtrans_scratch_bytes(lre_ctx *ctx, int w, int h);
@@ -96,6 +97,9 @@ int lre_launch_tone_mapping(const float *d_in, float *d_out, size_t n, float aft
                             hipStream_t s);
 int lre_launch_detail_tonemap(const float *d_in, float *d_out, int w, int h, void *d_scratch,
                               hipStream_t s);
+// d_in != d_out; wb_rgb is a host array of 3 or null; d_mask, d_max_val may be null
+int lre_launch_recover_highlights(const float *d_in, float *d_out, int w, int h, float sat, float *d_mask,
+                                  const float *wb_rgb, float *d_max_val, void *d_scratch, hipStream_t s);
 int lre_launch_micro_contrast(const float *d_in, float *d_out, int w, int h, float threshold,
                               float strength, float target_contrast, const float *d_mask,
                               void *d_scratch, hipStream_t s);

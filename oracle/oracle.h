@@ -42,6 +42,15 @@ int oracle_detail_tonemap(float *rgb3, int w, int h);
 int oracle_micro_contrast(const float *in3, float *out3, int w, int h, float threshold,
                           float strength, float target_contrast, const float *mask);
 void oracle_mc_kernel(float k[81]);
+/* in place; mask_out and wb_rgb may be NULL; counts3 = highlight, white and
+ * LoCA-adjusted pixel counts (may be NULL).  loca_order selects the reference's
+ * raster-order in-place LoCA pass or a snapshot reading of it. */
+#define ORACLE_LOCA_RASTER 0
+#define ORACLE_LOCA_SNAPSHOT 1
+#define ORACLE_LOCA_OFF 2
+int oracle_recover_highlights(float *rgb3, int w, int h, float sat, float *mask_out,
+                              const float *wb_rgb, float *max_val_out, long long *counts3,
+                              int loca_order);
 /* in == out selects the in-place semantics of the reference */
 int oracle_defringe(const float *in3, float *out3, int w, int h, float radius, float edge_threshold,
                     float chroma_threshold, float strength, int defringe_green,
