@@ -22,6 +22,9 @@ Other workloads, one JSON line each:
             2.1/1.0/1.5) through the request loop; p50/p99 latency and MP/s
   micro24   micro-contrast with a highlight mask on one resident 24 MP RGB
             image (the stage the pipeline runs for highlight_mode >= 4)
+  highlight24  highlight recovery with its mask and white balance on one
+            resident 24 MP RGB image (highlight_mode >= 4), magenta-cast
+            highlights so that the LoCA pass has work
 """
 from __future__ import annotations
 
@@ -125,7 +128,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--workload", default="chain24",
-                    choices=["chain24", "xtrans26", "batch64", "serve", "micro24", "copy"])
+                    choices=["chain24", "xtrans26", "batch64", "serve", "micro24", "highlight24", "copy"])
     ap.add_argument("--rates", default="4,8,16", help="serve: arrival rates (requests/s)")
     ap.add_argument("--duration", type=float, default=20.0, help="serve: seconds of arrivals per rate")
     ap.add_argument("--lanes", type=int, default=3, help="serve: requests in flight")
@@ -136,7 +139,7 @@ def main():
         sys.exit(2)
     if args.workload != "chain24":
         run = {"xtrans26": bench_xtrans26, "batch64": bench_batch64, "serve": bench_serve,
-               "micro24": bench_micro24, "copy": bench_copy}[args.workload]
+               "micro24": bench_micro24, "highlight24": bench_highlight24, "copy": bench_copy}[args.workload]
         print(json.dumps(run(args)), flush=True)
         return
 
@@ -387,6 +390,79 @@ def bench_micro24(args):
         dt = time.perf_counter() - t1
         line["cpu_baseline"] = {"value": round(mp / dt, 3), "unit": "MP/s", "cores": threads, "kind": "port",
                                 "sample": f"the whole {w}x{h} image ({dt:.2f} s)"}
+    acc.close()
+    return line
+
+
+def bench_highlight24(args):
+    """Highlight recovery (core/libraw_wrapper.cpp:1095-1383) on one resident
+    24 MP RGB image, as postprocess calls it for highlight_mode >= 4: with the
+    soft mask and the effective white balance (2.0, 1.0, 1.5), saturation
+    threshold 0.9.  The scene is scaled by 1.3 with R x 1.15 and B x 1.2, so
+    part of it is blown and the LoCA pass adjusts pixels.  The reference
+    function needs LibRaw, so the CPU baseline is the restatement ("port"),
+    serial as the reference is."""
+    from libraw_enhanced_amd import scene
+    from libraw_enhanced_amd.accelerator import HIPAccelerator, native
+    h, w = args.height, args.width
+    _log(f"generating {w}x{h} magenta-cast RGB scene")
+    img = scene.render(h, w, scene.BASE_SEED) * 1.3
+    img[..., 0] *= 1.15
+    img[..., 2] *= 1.2
+    img = np.ascontiguousarray(img, np.float32)
+    acc = HIPAccelerator(0)
+    lib = native()
+    d_in = acc.dev_from_host(img)
+    d_out, d_mask, d_max = acc.dev_alloc(img.nbytes), acc.dev_alloc(h * w * 4), acc.dev_alloc(4)
+    stream = lib.lre_hip_stream(acc.handle)
+    f32p = ctypes.POINTER(ctypes.c_float)
+    wb = np.float32([2.0, 1.0, 1.5])
+    sat = ctypes.c_float(0.9)
+
+    def step():
+        rc = lib.lre_hip_dev_recover_highlights(acc.handle, d_in.ptr, d_out.ptr, w, h, sat, d_mask.ptr,
+                                                wb.ctypes.data_as(f32p), d_max.ptr, stream)
+        assert rc == 0, rc
+    for _ in range(args.warmup):
+        step()
+    acc.synchronize()
+    ev = HipEvents()
+    marks = [(ev.create(), ev.create()) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for a, b in marks:
+        ev.record(a, stream)
+        step()
+        ev.record(b, stream)
+    acc.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms = float(np.mean([ev.elapsed_ms(a, b) for a, b in marks]))
+    mp = h * w / 1e6
+    alg = 28 * h * w  # RGB in 12 + mask 4 + RGB out 12 bytes per pixel
+    line = {"metric": "MP/s highlight recovery (24 MP)", "value": round(mp * args.steps / elapsed, 2),
+            "unit": "MP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(elapsed * 1e3 / args.steps, 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic (seeded scene x 1.3, R x 1.15, B x 1.2)",
+            "config": {"workload": f"recover_highlights with mask and WB on one {w}x{h} float32 RGB image",
+                       "image_mp": round(mp, 2), "stage_ms": round(ms, 4), "saturation_threshold": 0.9},
+            "roofline": {"kernel": "recover_highlights", "bound": "hbm",
+                         "achieved": round(alg / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                         "alg_bytes_per_launch": alg}}
+    if not args.no_cpu:
+        _log("CPU baseline (restatement, serial as the reference)")
+        olib = ctypes.CDLL(os.path.join(ROOT, "oracle", "liblre_oracle.so"))
+        out, mask = img.copy(), np.empty((h, w), np.float32)
+        mv, cnt = ctypes.c_float(0.0), (ctypes.c_longlong * 3)()
+        t1 = time.perf_counter()
+        olib.oracle_recover_highlights(out.ctypes.data_as(f32p), w, h, sat, mask.ctypes.data_as(f32p),
+                                       wb.ctypes.data_as(f32p), ctypes.byref(mv), cnt, 0)
+        dt = time.perf_counter() - t1
+        line["config"].update({"highlight_px": cnt[0], "blown_px": cnt[1], "loca_px": cnt[2]})
+        line["cpu_baseline"] = {"value": round(mp / dt, 3), "unit": "MP/s", "cores": 1, "kind": "port",
+                                "sample": f"the whole {w}x{h} image ({dt:.2f} s)"}
+        got = d_out.download((h, w, 3))
+        line["config"]["equal_to_port"] = bool(np.array_equal(got, out))
     acc.close()
     return line
 

@@ -34,15 +34,15 @@ namespace {
 
 constexpr int HR_CHUNK = 4096;  // pixels per chunk of the ratio scan
 constexpr int HR_BLOCK = 256;
-constexpr int RW_TX = 64, RW_TY = 16;  // rewrite tile
+constexpr int RW_TX = 64, RW_TY = 16;  // rewrite tile (RW_TY divides the LoCA band height)
 constexpr int RW_SX = RW_TX + 2, RW_SY = RW_TY + 2;
 constexpr int HR_TR = 64;            // LoCA tile rows (one per lane of wave 0)
-constexpr int HR_TW = 160;           // LoCA tile width in t
+constexpr int HR_TW = 104;           // LoCA tile width in t
 constexpr int HR_SR = HR_TR + 4;     // staged rows (two above, two below)
 constexpr int HR_LW = HR_TW + 17;    // staged columns: t - 8 .. t + W + 8, odd stride
 constexpr int HR_WORDS = (HR_TW + 31) / 32;
-constexpr size_t HR_LDS =
-    sizeof(float) * 3 * HR_SR * HR_LW + HR_TR * HR_TW + sizeof(unsigned) * HR_WORDS;
+constexpr size_t HR_LDS = sizeof(float) * (2 * HR_SR * HR_LW + 3 * HR_TR * HR_TW) + HR_TR * HR_TW +
+                          sizeof(unsigned) * HR_WORDS;
 
 struct HrState {
     float grf, gbf;     // ratio means after the division by the count
@@ -87,89 +87,109 @@ __global__ void hr_init(HrState *st, unsigned *tile_count, int ntiles) {
     for (int k = i; k < ntiles; k += gridDim.x * blockDim.x) tile_count[k] = 0u;
 }
 
+// Mask, ratio quotients and max_val over one chunk.  The quotients r/g, b/g of
+// the chunk's ratio pixels are written in raster order to the chunk's own list
+// (capacity HR_CHUNK), so the serial sum reads only them.
 __global__ __launch_bounds__(HR_BLOCK) void hr_scan(const float *__restrict__ in, float *__restrict__ mask,
                                                      size_t n, float sat, MaskConst M,
+                                                     float2 *__restrict__ ratios,
                                                      unsigned *__restrict__ chunk_count, HrState *st) {
     __shared__ unsigned s_cnt[HR_BLOCK / 64];
     __shared__ float s_max[HR_BLOCK / 64];
     const size_t base = (size_t)blockIdx.x * HR_CHUNK;
-    unsigned cnt = 0;
+    const int wv = threadIdx.x >> 6;
+    unsigned running = 0;
     float mx = 0.f;
-#pragma unroll 4
     for (int k = 0; k < HR_CHUNK; k += HR_BLOCK) {
         const size_t i = base + k + threadIdx.x;
-        if (i >= n) break;
-        const float r = in[3 * i], g = in[3 * i + 1], b = in[3 * i + 2];
-        if (mask) mask[i] = mask_value(r, g, b, M);
-        if (r >= sat && b >= sat) {
-            // max_val over the channels of the highlight pixels (:1199-1203)
-            if (r > mx) mx = r;
-            if (g > mx) mx = g;
-            if (b > mx) mx = b;
-            cnt += is_ratio_px(r, g, b, sat);
+        bool q = false;
+        float qr = 0.f, qb = 0.f;
+        if (i < n) {
+            const float r = in[3 * i], g = in[3 * i + 1], b = in[3 * i + 2];
+            if (mask) mask[i] = mask_value(r, g, b, M);
+            if (r >= sat && b >= sat) {
+                // max_val over the channels of the highlight pixels (:1199-1203)
+                if (r > mx) mx = r;
+                if (g > mx) mx = g;
+                if (b > mx) mx = b;
+                q = is_ratio_px(r, g, b, sat);
+                if (q) {
+                    qr = r / g;
+                    qb = b / g;
+                }
+            }
         }
+        const unsigned long long bal = __ballot(q);
+        if ((threadIdx.x & 63) == 0) s_cnt[wv] = (unsigned)__popcll(bal);
+        __syncthreads();
+        unsigned before = running, total = 0;
+#pragma unroll
+        for (int v = 0; v < HR_BLOCK / 64; v++) {
+            before += v < wv ? s_cnt[v] : 0u;
+            total += s_cnt[v];
+        }
+        if (q) {
+            // ratio pixels of this wave in lanes below this one
+            const unsigned rank =
+                __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
+            ratios[base + before + rank] = make_float2(qr, qb);
+        }
+        running += total;
+        __syncthreads();  // s_cnt is rewritten by the next round
     }
     for (int o = 32; o; o >>= 1) {
-        cnt += __shfl_xor(cnt, o);
         const float other = __shfl_xor(mx, o);
         mx = other > mx ? other : mx;
     }
-    const int wv = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
-        s_cnt[wv] = cnt;
-        s_max[wv] = mx;
-    }
+    if ((threadIdx.x & 63) == 0) s_max[wv] = mx;
     __syncthreads();
     if (threadIdx.x == 0) {
-        unsigned c = 0;
         float m = 0.f;
-        for (int v = 0; v < HR_BLOCK / 64; v++) {
-            c += s_cnt[v];
-            m = s_max[v] > m ? s_max[v] : m;
-        }
-        chunk_count[blockIdx.x] = c;
-        if (c) atomicAdd(&st->total, c);
+        for (int v = 0; v < HR_BLOCK / 64; v++) m = s_max[v] > m ? s_max[v] : m;
+        chunk_count[blockIdx.x] = running;
+        if (running) atomicAdd(&st->total, running);
         if (m > 0.f) atomicMax(&st->max_bits, __float_as_uint(m));
     }
 }
 
+constexpr int HR_PRE = 16;  // quotients per chunk fetched ahead
+
 // One wave: the serial float sums of r/g and b/g over the ratio pixels in
-// raster order (:1180-1190), then the means (:1184-1190).  Every lane holds
-// the same running sums; readlane hands them the quotients one at a time.
-__global__ __launch_bounds__(64) void hr_sums(const float *__restrict__ in, size_t n,
+// raster order (:1180-1190), then the means (:1184-1190).  Each lane fetches
+// the first HR_PRE quotients of one of 64 chunks; every lane then holds the
+// same running sums, and readlane hands them the quotients one at a time.
+__global__ __launch_bounds__(64) void hr_sums(const float2 *__restrict__ ratios,
                                               const unsigned *__restrict__ chunk_count, int nchunks,
-                                              float sat, HrState *st) {
+                                              HrState *st) {
     const int lane = threadIdx.x;
     float grf = 0.f, gbf = 0.f, count = 0.f;
     const int walk = st->total ? nchunks : 0;
+    auto add = [&](float2 v, int src) {
+        grf += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.x), src));
+        gbf += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.y), src));
+        count += 1.f;
+    };
     for (int c0 = 0; c0 < walk; c0 += 64) {
-        const unsigned cnt = c0 + lane < nchunks ? chunk_count[c0 + lane] : 0u;
+        const int cj = c0 + lane;
+        const unsigned cnt = cj < nchunks ? chunk_count[cj] : 0u;
         unsigned long long live = __ballot(cnt != 0u);
+        if (!live) continue;
+        const float2 *lst = ratios + (size_t)cj * HR_CHUNK;
+        float2 v[HR_PRE];
+#pragma unroll
+        for (int k = 0; k < HR_PRE; k++) v[k] = (unsigned)k < cnt ? lst[k] : make_float2(0.f, 0.f);
         while (live) {
             const int j = __ffsll(live) - 1;
             live &= live - 1;
-            const size_t base = (size_t)(c0 + j) * HR_CHUNK;
-            const size_t end = base + HR_CHUNK < n ? base + HR_CHUNK : n;
-            for (size_t p = base; p < end; p += 64) {
-                const size_t i = p + lane;
-                bool q = false;
-                float qr = 0.f, qb = 0.f;
-                if (i < end) {
-                    const float r = in[3 * i], g = in[3 * i + 1], b = in[3 * i + 2];
-                    q = is_ratio_px(r, g, b, sat);
-                    if (q) {
-                        qr = r / g;
-                        qb = b / g;
-                    }
-                }
-                unsigned long long bal = __ballot(q);
-                while (bal) {
-                    const int k = __ffsll(bal) - 1;
-                    bal &= bal - 1;
-                    grf += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(qr), k));
-                    gbf += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(qb), k));
-                    count += 1.f;
-                }
+            const unsigned m = (unsigned)__builtin_amdgcn_readlane((int)cnt, j);
+#pragma unroll
+            for (int k = 0; k < HR_PRE; k++)
+                if ((unsigned)k < m) add(v[k], j);
+            const float2 *lj = ratios + (size_t)(c0 + j) * HR_CHUNK;
+            for (unsigned k = HR_PRE; k < m; k += 64) {  // the rest of a long list, 64 at a time
+                const float2 u = k + lane < m ? lj[k + lane] : make_float2(0.f, 0.f);
+                const unsigned mm = m - k < 64u ? m - k : 64u;
+                for (unsigned i = 0; i < mm; i++) add(u, (int)i);
             }
         }
     }
@@ -207,7 +227,12 @@ __global__ __launch_bounds__(HR_BLOCK) void hr_rewrite(const float *__restrict__
                                                        unsigned *__restrict__ tile_count, int tcols) {
     __shared__ float sR[RW_SY][RW_SX], sG[RW_SY][RW_SX], sB[RW_SY][RW_SX];
     __shared__ unsigned char sW[RW_SY][RW_SX];  // bit 0 highlight, bit 1 blown
+    // LoCA candidates per LoCA tile: the 16 rows lie in one band of 64, and
+    // x + 3y spans fewer than 2 * HR_TW + 1 values, so at most 3 tiles
+    __shared__ unsigned s_tc[3];
     const int x0 = blockIdx.x * RW_TX, y0 = blockIdx.y * RW_TY;
+    const int k0 = (x0 + 3 * y0) / HR_TW;
+    if (threadIdx.x < 3) s_tc[threadIdx.x] = 0u;
     const float grf = st->grf, gbf = st->gbf;
     for (int e = threadIdx.x; e < RW_SY * RW_SX; e += HR_BLOCK) {
         const int sy = e / RW_SX, sx = e - sy * RW_SX;
@@ -255,14 +280,40 @@ __global__ __launch_bounds__(HR_BLOCK) void hr_rewrite(const float *__restrict__
         const bool cand = (f & 1) && gx >= 2 && gx <= w - 3 && gy >= 2 && gy <= h - 3 && !(l < sat09) &&
                           !(rb <= v[1] * 1.04f);
         flags[i] = cand;
-        if (cand) atomicAdd(&tile_count[(gy / HR_TR) * tcols + (gx + 3 * gy) / HR_TW], 1u);
+        if (cand) atomicAdd(&s_tc[(gx + 3 * gy) / HR_TW - k0], 1u);
     }
+    __syncthreads();
+    if (threadIdx.x < 3 && s_tc[threadIdx.x])
+        atomicAdd(&tile_count[(y0 / HR_TR) * tcols + k0 + (int)threadIdx.x], s_tc[threadIdx.x]);
 }
 
-// LoCA green lift of one pixel at LDS index c (:1299-1372).  Neighbour
-// (dy, dx) sits at c + dy * (HR_LW + 3) + dx in the skewed layout.
-LRE_DEV void loca_px(float *R, float *G, float *B, int c, float sat, float max_val) {
-    const float p0 = R[c], p1 = G[c], p2 = B[c];
+// What a pixel contributes to a LoCA window (:1311-1345): NL = max(R, G, B)
+// for the 3x3 contrast, RT = G / ((R + B) / 2) when the 5x5 ratio mean takes
+// it, else RT_SKIP (a value no taken ratio has: those lie in [0.70, 1.35] or
+// are NaN).  Both depend only on the pixel's own values, so the window reads
+// them instead of redoing the division for every tap.
+constexpr float RT_SKIP = -1.0f;
+
+LRE_DEV void tap_info(float r, float g, float b, float sat075, float &nl, float &rt) {
+    nl = max3(r, g, b);
+    const float nrb = 0.5f * (r + b);
+    rt = RT_SKIP;
+    if (nl < sat075 || nrb <= 1e-6f) return;
+    const float ratio = g / nrb;
+    if (ratio < 0.70f || ratio > 1.35f) return;
+    rt = ratio;
+}
+
+struct LocaTile {
+    float *NL, *RT;        // [HR_SR][HR_LW], skewed: pixel (r, s) of the tile at (r + 2) * HR_LW + s + 8
+    float *OR, *OG, *OB;   // the tile's own pixels [HR_TR][HR_TW]
+};
+
+// LoCA green lift of the tile pixel (r, s) (:1299-1372).  Neighbour (dy, dx)
+// sits at c + dy * (HR_LW + 3) + dx of the skewed planes.
+LRE_DEV void loca_px(const LocaTile &L, int r, int s, float sat, float sat075, float max_val) {
+    const int o = r * HR_TW + s, c = (r + 2) * HR_LW + s + 8;
+    const float p0 = L.OR[o], p1 = L.OG[o], p2 = L.OB[o];
     const float l = max3(p0, p1, p2);
     if (l < sat * 0.90f) return;
     constexpr int DY = HR_LW + 3;
@@ -271,30 +322,25 @@ LRE_DEV void loca_px(float *R, float *G, float *B, int c, float sat, float max_v
     for (int dy = -1; dy <= 1; ++dy)
 #pragma unroll
         for (int dx = -1; dx <= 1; ++dx) {
-            const int o = c + dy * DY + dx;
-            const float nl = max3(R[o], G[o], B[o]);
+            const float nl = L.NL[c + dy * DY + dx];
             local_max = lre::max_std(local_max, nl);
             local_min = lre::min_std(local_min, nl);
         }
     if ((local_max - local_min) < 0.015f) return;
     const float rb = 0.5f * (p0 + p2);
     if (rb <= p1 * 1.04f) return;
+    // The reference skips a tap; adding +0.0 instead leaves both sums unchanged
+    // (no taken term is -0.0).
     double ratio_sum = 0.0, ratio_w = 0.0;
-    const float sat075 = sat * 0.75f;
 #pragma unroll
     for (int dy = -2; dy <= 2; ++dy)
 #pragma unroll
         for (int dx = -2; dx <= 2; ++dx) {
-            const int o = c + dy * DY + dx;
-            const float nr = R[o], ng = G[o], nb = B[o];
-            const float nrb = 0.5f * (nr + nb);
-            const float nl = max3(nr, ng, nb);
-            if (nl < sat075 || nrb <= 1e-6f) continue;
-            const float ratio = ng / nrb;
-            if (ratio < 0.70f || ratio > 1.35f) continue;
+            const float rt = L.RT[c + dy * DY + dx];
+            const bool take = rt != RT_SKIP;
             const double wt = 1.0 / (double)(1 + (dx < 0 ? -dx : dx) + (dy < 0 ? -dy : dy));
-            ratio_sum += ratio * wt;
-            ratio_w += wt;
+            ratio_sum += take ? rt * wt : 0.0;
+            ratio_w += take ? wt : 0.0;
         }
     const float local_ratio = (float)(ratio_w > 0.0 ? ratio_sum / ratio_w : 1.0);
     const float desired_g = rb * lre::clamp_std(local_ratio, 0.95f, 1.08f);
@@ -313,49 +359,66 @@ LRE_DEV void loca_px(float *R, float *G, float *B, int c, float sat, float max_v
         q2 = lre::max_std(0.0f, q2 - trim);
     }
     q1 = lre::max_std(0.0f, q1);
-    R[c] = q0;
-    G[c] = q1;
-    B[c] = q2;
+    L.OR[o] = q0;
+    L.OG[o] = q1;
+    L.OB[o] = q2;
+    tap_info(q0, q1, q2, sat075, L.NL[c], L.RT[c]);
 }
 
 // Tile (b_lo + blockIdx.x, d - b) of the anti-diagonal d.  Staged row sr holds
 // image row Y - 2 + sr, column j holds x = T - 3y - 8 + j, so the pixel of tile
-// row r at step s is at (r + 2, s + 8) for every row.
+// row r at step s is at (r + 2, s + 8) for every row.  Lane r of wave 0 takes
+// tile row r; the other waves stage, then keep step with the barriers.
 __global__ __launch_bounds__(HR_BLOCK) void hr_loca(float *img, int w, int h, float sat, const HrState *st,
                                                     const unsigned char *__restrict__ flags,
                                                     const unsigned *__restrict__ tile_count, int tcols,
                                                     int d, int b_lo) {
     extern __shared__ __align__(16) float lds[];
-    float *R = lds, *G = R + HR_SR * HR_LW, *B = G + HR_SR * HR_LW;
-    unsigned char *F = reinterpret_cast<unsigned char *>(B + HR_SR * HR_LW);
+    LocaTile L;
+    L.NL = lds;
+    L.RT = L.NL + HR_SR * HR_LW;
+    L.OR = L.RT + HR_SR * HR_LW;
+    L.OG = L.OR + HR_TR * HR_TW;
+    L.OB = L.OG + HR_TR * HR_TW;
+    unsigned char *F = reinterpret_cast<unsigned char *>(L.OB + HR_TR * HR_TW);
     unsigned *step_any = reinterpret_cast<unsigned *>(F + HR_TR * HR_TW);
     const int b = b_lo + (int)blockIdx.x, k = d - b;
     if (tile_count[b * tcols + k] == 0u) return;
     const int Y = b * HR_TR, T = k * HR_TW;
+    const float sat075 = sat * 0.75f;
     if (threadIdx.x < HR_WORDS) step_any[threadIdx.x] = 0u;
-    for (int e = threadIdx.x; e < HR_SR * HR_LW; e += HR_BLOCK) {
-        const int sr = e / HR_LW, j = e - sr * HR_LW;
-        const int y = Y - 2 + sr, x = T - 3 * y - 8 + j;
-        float r = 0.f, g = 0.f, bb = 0.f;
-        if (y >= 0 && y < h && x >= 0 && x < w) {
-            const size_t i = (size_t)y * w + x;
-            r = img[3 * i];
-            g = img[3 * i + 1];
-            bb = img[3 * i + 2];
-        }
-        R[e] = r;
-        G[e] = g;
-        B[e] = bb;
-    }
+    // 8 global loads in flight per thread (lre::stage2d)
+    lre::stage2d<HR_BLOCK>(
+        HR_SR, HR_LW,
+        [&](int sr, int j) {
+            const int y = Y - 2 + sr, x = T - 3 * y - 8 + j;
+            float3 v = make_float3(0.f, 0.f, 0.f);
+            if (y >= 0 && y < h && x >= 0 && x < w) {
+                const size_t i = (size_t)y * w + x;
+                v = make_float3(img[3 * i], img[3 * i + 1], img[3 * i + 2]);
+            }
+            return v;
+        },
+        [&](int sr, int j, float3 v) {
+            const int e = sr * HR_LW + j;
+            tap_info(v.x, v.y, v.z, sat075, L.NL[e], L.RT[e]);
+            const int tr = sr - 2, ts = j - 8;
+            if (tr >= 0 && tr < HR_TR && ts >= 0 && ts < HR_TW) {
+                L.OR[tr * HR_TW + ts] = v.x;
+                L.OG[tr * HR_TW + ts] = v.y;
+                L.OB[tr * HR_TW + ts] = v.z;
+            }
+        });
+    lre::stage2d<HR_BLOCK>(
+        HR_TR, HR_TW,
+        [&](int r, int s) {
+            const int y = Y + r, x = T + s - 3 * y;
+            return (y < h && x >= 0 && x < w) ? flags[(size_t)y * w + x] : (unsigned char)0;
+        },
+        [&](int r, int s, unsigned char f) { F[r * HR_TW + s] = f; });
     __syncthreads();
-    for (int e = threadIdx.x; e < HR_TR * HR_TW; e += HR_BLOCK) {
-        const int r = e / HR_TW, s = e - r * HR_TW;
-        const int y = Y + r, x = T + s - 3 * y;
-        unsigned char f = 0;
-        if (y < h && x >= 0 && x < w) f = flags[(size_t)y * w + x];
-        F[e] = f;
-        if (f) atomicOr(&step_any[s >> 5], 1u << (s & 31));
-    }
+    for (int e = threadIdx.x; e < HR_TR * HR_TW; e += HR_BLOCK)
+        if (F[e]) atomicOr(&step_any[(e % HR_TW) >> 5], 1u << ((e % HR_TW) & 31));
     __syncthreads();
     const float max_val = __uint_as_float(st->max_bits);
     const int r = threadIdx.x;
@@ -364,7 +427,7 @@ __global__ __launch_bounds__(HR_BLOCK) void hr_loca(float *img, int w, int h, fl
         while (bits) {
             const int s = wd * 32 + __ffs(bits) - 1;
             bits &= bits - 1;
-            if (r < HR_TR && F[r * HR_TW + s]) loca_px(R, G, B, (r + 2) * HR_LW + s + 8, sat, max_val);
+            if (r < HR_TR && F[r * HR_TW + s]) loca_px(L, r, s, sat, sat075, max_val);
             __syncthreads();
         }
     }
@@ -372,11 +435,10 @@ __global__ __launch_bounds__(HR_BLOCK) void hr_loca(float *img, int w, int h, fl
         if (!F[e]) continue;
         const int rr = e / HR_TW, s = e - rr * HR_TW;
         const int y = Y + rr, x = T + s - 3 * y;
-        const int c = (rr + 2) * HR_LW + s + 8;
         const size_t i = (size_t)y * w + x;
-        img[3 * i] = R[c];
-        img[3 * i + 1] = G[c];
-        img[3 * i + 2] = B[c];
+        img[3 * i] = L.OR[e];
+        img[3 * i + 1] = L.OG[e];
+        img[3 * i + 2] = L.OB[e];
     }
 }
 
@@ -384,9 +446,13 @@ __global__ void hr_emit(const HrState *st, float *max_val) { *max_val = __uint_a
 
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
+static_assert(HR_LDS <= 160 * 1024, "LoCA tile exceeds the LDS of a workgroup");
+static_assert(HR_TR % RW_TY == 0 && RW_TX - 1 + 3 * (RW_TY - 1) < 2 * HR_TW,
+              "a rewrite tile must fall in one LoCA band and at most 3 LoCA tile columns");
+
 struct HrLayout {
     int nchunks, bands, tcols;
-    size_t o_counts, o_tiles, o_flags, bytes;
+    size_t o_counts, o_tiles, o_flags, o_ratios, bytes;
 };
 
 HrLayout hr_layout(int w, int h) {
@@ -399,7 +465,8 @@ HrLayout hr_layout(int w, int h) {
     L.o_counts = 256;
     L.o_tiles = L.o_counts + align256((size_t)L.nchunks * 4);
     L.o_flags = L.o_tiles + align256((size_t)L.bands * L.tcols * 4);
-    L.bytes = L.o_flags + align256(n);
+    L.o_ratios = L.o_flags + align256(n);
+    L.bytes = L.o_ratios + (size_t)L.nchunks * HR_CHUNK * sizeof(float2);
     return L;
 }
 
@@ -426,6 +493,7 @@ int lre_launch_recover_highlights(const float *d_in, float *d_out, int w, int h,
     unsigned *chunk_count = reinterpret_cast<unsigned *>(base + L.o_counts);
     unsigned *tile_count = reinterpret_cast<unsigned *>(base + L.o_tiles);
     unsigned char *flags = reinterpret_cast<unsigned char *>(base + L.o_flags);
+    float2 *ratios = reinterpret_cast<float2 *>(base + L.o_ratios);
     // mask edges and the WB undo, in float as the reference computes them (:1135-1147)
     const float mask_t_hi = 1.00f, mask_t_lo = 0.95f;
     MaskConst M;
@@ -441,8 +509,9 @@ int lre_launch_recover_highlights(const float *d_in, float *d_out, int w, int h,
     LRE_HIP_CHECK(hr_lds_attr());
     hipLaunchKernelGGL(hr_init, dim3(std::min(64, (ntiles + 255) / 256 + 1)), dim3(256), 0, s, st, tile_count,
                        ntiles);
-    hipLaunchKernelGGL(hr_scan, dim3(L.nchunks), dim3(HR_BLOCK), 0, s, d_in, d_mask, n, sat, M, chunk_count, st);
-    hipLaunchKernelGGL(hr_sums, dim3(1), dim3(64), 0, s, d_in, n, chunk_count, L.nchunks, sat, st);
+    hipLaunchKernelGGL(hr_scan, dim3(L.nchunks), dim3(HR_BLOCK), 0, s, d_in, d_mask, n, sat, M, ratios, chunk_count,
+                       st);
+    hipLaunchKernelGGL(hr_sums, dim3(1), dim3(64), 0, s, ratios, chunk_count, L.nchunks, st);
     hipLaunchKernelGGL(hr_rewrite, dim3((w + RW_TX - 1) / RW_TX, (h + RW_TY - 1) / RW_TY), dim3(HR_BLOCK), 0, s,
                        d_in, d_out, w, h, sat, st, flags, tile_count, L.tcols);
     LRE_HIP_CHECK(hipGetLastError());
