@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <mutex>
 
+#include "lre_epilogue.h"
 #include "lre_hip.h"
 
 struct lre_ctx {
@@ -73,6 +74,10 @@ int lre_launch_xtrans_multi(const lre_demosaic_job *jobs, int n, const void *h_t
                             int total_tiles, int slots, void *d_scratch, hipStream_t s);
 // Output epilogue: optional 3x4 matrix (transform12 NULL skips it), optional gamma
 // (output_color_space < 0 skips it), then float (bits 32) or quantized (8, 16) store.
+// The matrix / gamma part of the epilogue as one parameter set (transform12
+// NULL: no matrix; output_color_space < 0: no gamma).
+lre::Epilogue lre_make_epilogue(const float *transform12, float gamma_power, float gamma_slope,
+                                int output_color_space);
 int lre_launch_epilogue(const float *d_in, void *d_out, size_t npx, const float *transform12,
                         float gamma_power, float gamma_slope, int output_color_space, int bits,
                         hipStream_t s);
@@ -96,7 +101,7 @@ int lre_launch_border_interpolate(const float *dummy, float *d_rgb, int w, int h
 int lre_launch_tone_mapping(const float *d_in, float *d_out, size_t n, float after_scale,
                             hipStream_t s);
 int lre_launch_detail_tonemap(const float *d_in, float *d_out, int w, int h, void *d_scratch,
-                              hipStream_t s);
+                              hipStream_t s, const lre::Epilogue *epi = nullptr);
 // d_in != d_out; wb_rgb is a host array of 3 or null; d_mask, d_max_val may be null
 int lre_launch_recover_highlights(const float *d_in, float *d_out, int w, int h, float sat, float *d_mask,
                                   const float *wb_rgb, float *d_max_val, void *d_scratch, hipStream_t s);

@@ -112,6 +112,42 @@ int run_post_stage(int stage, const lre_image &im, const lre_chain_params &p, vo
     }
 }
 
+// The stages from index `from` on.  A colour matrix and/or gamma stage that
+// directly follows the detail tone map runs inside its store, and a gamma
+// stage that directly follows the colour matrix shares its pass.  The results
+// are the same bits as one pass per stage (same float operations on the same
+// values); the fused forms save a 12 B/px read and write per stage.
+int run_post_chain(const int *stages, int from, int ns, const lre_image &im, const lre_chain_params &p,
+                   void *scratch, hipStream_t s) {
+    for (int si = from; si < ns; si++) {
+        const int st = stages[si];
+        if (st == LRE_STAGE_DETAIL_TM || st == LRE_STAGE_COLOR_SPACE) {
+            int j = si + 1;
+            bool matrix = st == LRE_STAGE_COLOR_SPACE;
+            if (st == LRE_STAGE_DETAIL_TM && j < ns && stages[j] == LRE_STAGE_COLOR_SPACE) {
+                matrix = true;
+                j++;
+            }
+            const bool gamma = j < ns && stages[j] == LRE_STAGE_GAMMA;
+            if (gamma) j++;
+            if (j > si + 1) {
+                const lre::Epilogue E = lre_make_epilogue(matrix ? p.color_transform : nullptr, p.gamma_power,
+                                                          p.gamma_slope, gamma ? p.output_color_space : -1);
+                const size_t npx = (size_t)im.width * im.height;
+                const int rc = st == LRE_STAGE_DETAIL_TM
+                                   ? lre_launch_detail_tonemap(im.d_rgb, im.d_rgb, im.width, im.height, scratch, s, &E)
+                                   : lre_launch_epilogue(im.d_rgb, im.d_rgb, npx, p.color_transform, p.gamma_power,
+                                                         p.gamma_slope, p.output_color_space, 32, s);
+                if (rc) return rc;
+                si = j - 1;
+                continue;
+            }
+        }
+        if (const int rc = run_post_stage(st, im, p, scratch, s)) return rc;
+    }
+    return LRE_OK;
+}
+
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
@@ -411,9 +447,8 @@ int enqueue(lre_batch *b, const Plan &P, const lre_image *imgs, int n, const int
         auto plan = assign_lanes(imgs, n, lanes);
         for (int k = 0; k < lanes; k++)
             for (int i : plan[k])
-                for (int si = first; si < ns; si++)
-                    if ((rc = run_post_stage(stages[si], imgs[i], p, L[k].scratch.p, L[k].stream)))
-                        return rc;
+                if ((rc = run_post_chain(stages, first, ns, imgs[i], p, L[k].scratch.p, L[k].stream)))
+                    return rc;
     }
     for (int k = 1; k < lanes; k++) {
         LRE_HIP_CHECK(hipEventRecord(L[k].done, L[k].stream));
@@ -791,8 +826,9 @@ int serve_one(lre_server *S, Lane &L, const lre_request &rq, char *dev, const Fo
     }
     lre_image out = im;
     out.d_rgb = d_rgb;
-    for (size_t si = has_demosaic ? 1 : 0; si < S->stages.size(); si++)
-        if ((rc = run_post_stage(S->stages[si], out, S->params, d_scr, s))) return rc;
+    if ((rc = run_post_chain(S->stages.data(), has_demosaic ? 1 : 0, (int)S->stages.size(), out, S->params,
+                             d_scr, s)))
+        return rc;
     LRE_HIP_CHECK(hipMemcpyAsync(rq.h_rgb, d_rgb, px * 12, hipMemcpyDeviceToHost, s));
     LRE_HIP_CHECK(hipStreamSynchronize(s));
     return LRE_OK;

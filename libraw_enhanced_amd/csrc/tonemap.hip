@@ -1,6 +1,7 @@
 // Pointwise and box-filter stages for gfx950: ACES tone map, the
 // detail-preserving tone map, white balance and CFA-plane extraction.
 #include "lre_common.h"
+#include "lre_epilogue.h"
 #include "lre_internal.h"
 
 namespace {
@@ -140,11 +141,14 @@ __global__ __launch_bounds__(DBLOCK) void dtm_ab_kernel(const float *__restrict_
     });
 }
 
-// pass 2: box(a), box(b); base = max(1e-6, A*guide + B); RGB *= 1 + (gain-1) w
+// pass 2: box(a), box(b); base = max(1e-6, A*guide + B); RGB *= 1 + (gain-1) w.
+// EPI: the colour matrix and gamma stages that follow in a stage list run on
+// the result before it is stored (the same float operations as their own pass).
+template <bool EPI>
 __global__ __launch_bounds__(DBLOCK) void dtm_apply_kernel(const float *__restrict__ pa,
                                                            const float *__restrict__ pb,
                                                            const float *in, float *out, int W,
-                                                           int H) {
+                                                           int H, lre::Epilogue E) {
     extern __shared__ float dsm[];
     float *win = dsm, *hs0 = win + DIY * DIXP, *hs1 = hs0 + DIY * DTXP;
     const int x0 = blockIdx.x * DTX, y0 = blockIdx.y * DTY;
@@ -178,6 +182,7 @@ __global__ __launch_bounds__(DBLOCK) void dtm_apply_kernel(const float *__restri
             g *= gain;
             b *= gain;
         }
+        if (EPI) lre::epilogue_rgb(r, g, b, E);
         out[i] = r;
         out[i + 1] = g;
         out[i + 2] = b;
@@ -224,8 +229,11 @@ hipError_t dtm_lds_attr() {
         hipError_t e = hipFuncSetAttribute((const void *)dtm_ab_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                            (int)DTM_LDS);
         if (e == hipSuccess)
-            e = hipFuncSetAttribute((const void *)dtm_apply_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)DTM_LDS);
+            e = hipFuncSetAttribute((const void *)dtm_apply_kernel<false>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)DTM_LDS);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute((const void *)dtm_apply_kernel<true>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)DTM_LDS);
         return e;
     }();
     return rc;
@@ -389,14 +397,18 @@ int lre_launch_tone_mapping(const float *d_in, float *d_out, size_t n, float aft
 size_t lre_detail_tm_scratch_bytes(int w, int h) { return (size_t)w * h * sizeof(float) * 2; }
 
 int lre_launch_detail_tonemap(const float *d_in, float *d_out, int w, int h, void *d_scratch,
-                              hipStream_t s) {
+                              hipStream_t s, const lre::Epilogue *epi) {
     float *pa = static_cast<float *>(d_scratch);
     float *pb = pa + (size_t)w * h;
     dim3 grid((w + DTX - 1) / DTX, (h + DTY - 1) / DTY);
     LRE_HIP_CHECK(dtm_lds_attr());
     hipLaunchKernelGGL(dtm_ab_kernel, grid, dim3(DBLOCK), DTM_LDS, s, d_in, pa, pb, w, h);
     LRE_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(dtm_apply_kernel, grid, dim3(DBLOCK), DTM_LDS, s, pa, pb, d_in, d_out, w, h);
+    if (epi && epi->active())
+        hipLaunchKernelGGL(dtm_apply_kernel<true>, grid, dim3(DBLOCK), DTM_LDS, s, pa, pb, d_in, d_out, w, h, *epi);
+    else
+        hipLaunchKernelGGL(dtm_apply_kernel<false>, grid, dim3(DBLOCK), DTM_LDS, s, pa, pb, d_in, d_out, w, h,
+                           lre::Epilogue{});
     LRE_HIP_CHECK(hipGetLastError());
     return LRE_OK;
 }

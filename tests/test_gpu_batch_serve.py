@@ -159,9 +159,13 @@ def test_batch_other_stage_lists(acc):
     p = dv.ChainParams.defaults(defringe_green=1, axial_radius=3, output_color_space=2)
     p.color_transform[:] = [1.62, -0.49, -0.13, 0.0, -0.18, 1.35, -0.17, 0.0, 0.02, -0.31, 1.29, 0.0]
     spec = _spec()[:3]
+    # the detail tone map followed by the matrix and/or gamma runs them inside
+    # its store, and the matrix followed by gamma shares one pass: same bits
     for stages in [("demosaic",), ("demosaic", "lateral_ca", "tone_mapping"),
                    ("demosaic", "defringe", "axial_ca", "detail_tm"),
-                   ("demosaic", "detail_tm", "color_space", "gamma")]:
+                   ("demosaic", "detail_tm", "color_space", "gamma"),
+                   ("demosaic", "detail_tm", "gamma"), ("demosaic", "detail_tm", "color_space"),
+                   ("demosaic", "color_space", "gamma"), ("demosaic", "gamma", "color_space")]:
         print(f"stages {stages}", flush=True)
         S = _Set(acc, spec)
         b = dv.BatchDeveloper(0, 2)
@@ -170,6 +174,20 @@ def test_batch_other_stage_lists(acc):
         for it, g in zip(S.items, S.results()):
             assert np.array_equal(g, _single(acc, it, stages, p)), (stages, it["kind"])
         b.close()
+
+
+def test_fused_epilogue_srgb(acc):
+    from libraw_enhanced_amd import develop as dv
+    p = dv.ChainParams.defaults(output_color_space=1)
+    p.color_transform[:] = [1.2, -0.1, -0.1, 0.0, -0.05, 1.1, -0.05, 0.0, 0.0, -0.2, 1.2, 0.0]
+    stages = ("demosaic", "axial_ca", "detail_tm", "color_space", "gamma")
+    S = _Set(acc, _spec()[3:5])
+    b = dv.BatchDeveloper(0, 1)
+    b.run(S.images(), stages, p)
+    b.synchronize()
+    for it, g in zip(S.items, S.results()):
+        assert np.array_equal(g, _single(acc, it, stages, p)), it["kind"]
+    b.close()
 
 
 def test_batch_graph_replays(acc):
