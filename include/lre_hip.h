@@ -308,7 +308,15 @@ enum {
     LRE_STAGE_DETAIL_TM = 5,
     LRE_STAGE_TONE_MAPPING = 6,
     LRE_STAGE_COLOR_SPACE = 7,
-    LRE_STAGE_GAMMA = 8
+    LRE_STAGE_GAMMA = 8,
+    /* recover_highlights (core/libraw_wrapper.cpp:1095-1383) with the image's
+     * soft highlight mask kept for a later micro-contrast stage; WB for the
+     * mask: the image's wb when it is ingested from uint16, else none */
+    LRE_STAGE_HIGHLIGHT = 9,
+    /* enhance_micro_contrast as postprocess calls it (:1698-1705): threshold
+     * highlight_threshold + 0.3, micro_strength, micro_target_contrast, and the
+     * highlight mask when a highlight stage came earlier in the list */
+    LRE_STAGE_MICRO_CONTRAST = 10
 };
 
 /* Parameters of the post-demosaic stages. */
@@ -325,6 +333,8 @@ typedef struct lre_chain_params {
     float color_transform[12];   /* row-major 3x4 (identity by default) */
     float gamma_power, gamma_slope;
     int output_color_space;
+    float highlight_threshold;   /* maximum / data_maximum of the raw (1 when unknown) */
+    float micro_strength, micro_target_contrast;
 } lre_chain_params;
 
 /* The reference's defaults (core/accelerator.h:344-367, :322-323, :315-319). */

@@ -21,7 +21,19 @@ constexpr int MAX_LANES = 8;
 
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-bool valid_stage(int s) { return s >= LRE_STAGE_DEMOSAIC && s <= LRE_STAGE_GAMMA; }
+bool valid_stage(int s) { return s >= LRE_STAGE_DEMOSAIC && s <= LRE_STAGE_MICRO_CONTRAST; }
+
+bool has_stage(const int *stages, int ns, int st) {
+    for (int i = 0; i < ns; i++)
+        if (stages[i] == st) return true;
+    return false;
+}
+
+// A stage list with a highlight stage keeps the image's highlight mask at the
+// start of the lane scratch, below what every stage uses.
+size_t mask_bytes(const lre_image &im, const int *stages, int ns) {
+    return has_stage(stages, ns, LRE_STAGE_HIGHLIGHT) ? align256((size_t)im.width * im.height * 4) : 0;
+}
 
 int check_stages(const int *stages, int ns) {
     if (ns < 0 || (ns > 0 && !stages)) {
@@ -73,11 +85,15 @@ size_t post_scratch(const lre_image &im, const int *stages, int ns, const lre_ch
         case LRE_STAGE_AXIAL_CA: b = lre_axial_scratch_bytes(im.width, im.height); break;
         case LRE_STAGE_DEFRINGE: b = lre_defringe_scratch_bytes(im.width, im.height); break;
         case LRE_STAGE_DETAIL_TM: b = lre_detail_tm_scratch_bytes(im.width, im.height); break;
+        case LRE_STAGE_HIGHLIGHT:  // a copy of the input, then the stage's own scratch
+            b = align256((size_t)im.width * im.height * 12) + lre_highlight_scratch_bytes(im.width, im.height);
+            break;
+        case LRE_STAGE_MICRO_CONTRAST: b = lre_micro_contrast_scratch_bytes(im.width, im.height); break;
         default: break;
         }
         m = std::max(m, b);
     }
-    return m;
+    return mask_bytes(im, stages, ns) + m;
 }
 
 // one post-demosaic stage, in place on the image's RGB buffer (the develop
@@ -118,9 +134,35 @@ int run_post_stage(int stage, const lre_image &im, const lre_chain_params &p, vo
 // are the same bits as one pass per stage (same float operations on the same
 // values); the fused forms save a 12 B/px read and write per stage.
 int run_post_chain(const int *stages, int from, int ns, const lre_image &im, const lre_chain_params &p,
-                   void *scratch, hipStream_t s) {
+                   void *lane_scratch, hipStream_t s) {
+    const size_t mb = mask_bytes(im, stages, ns), npx = (size_t)im.width * im.height;
+    float *mask = mb ? static_cast<float *>(lane_scratch) : nullptr;
+    void *scratch = static_cast<char *>(lane_scratch) + mb;
+    bool mask_ready = false;
     for (int si = from; si < ns; si++) {
         const int st = stages[si];
+        if (st == LRE_STAGE_HIGHLIGHT) {
+            // in place: the input is copied once, as the reference rewrites its buffer
+            float *copy = static_cast<float *>(scratch);
+            LRE_HIP_CHECK(hipMemcpyAsync(copy, im.d_rgb, npx * 12, hipMemcpyDeviceToDevice, s));
+            const bool u16 = !im.d_cfa && im.d_raw;
+            if (const int rc = lre_launch_recover_highlights(copy, im.d_rgb, im.width, im.height,
+                                                             p.highlight_threshold, mask, u16 ? im.wb : nullptr,
+                                                             nullptr, static_cast<char *>(scratch) +
+                                                                          align256(npx * 12),
+                                                             s))
+                return rc;
+            mask_ready = true;
+            continue;
+        }
+        if (st == LRE_STAGE_MICRO_CONTRAST) {
+            if (const int rc = lre_launch_micro_contrast(im.d_rgb, im.d_rgb, im.width, im.height,
+                                                         p.highlight_threshold + 0.3f, p.micro_strength,
+                                                         p.micro_target_contrast, mask_ready ? mask : nullptr,
+                                                         scratch, s))
+                return rc;
+            continue;
+        }
         if (st == LRE_STAGE_DETAIL_TM || st == LRE_STAGE_COLOR_SPACE) {
             int j = si + 1;
             bool matrix = st == LRE_STAGE_COLOR_SPACE;
@@ -495,6 +537,9 @@ void lre_chain_params_default(lre_chain_params *p) {
     p->gamma_power = 0.0f;  // ProcessingParams defaults (core/accelerator.h:106, :123-124)
     p->gamma_slope = 0.0f;
     p->output_color_space = 1;
+    p->highlight_threshold = 1.0f;  // compute_default_threshold's fallback (core/libraw_wrapper.cpp:3004-3011)
+    p->micro_strength = 8.0f;       // postprocess's call (:1686, :1698-1705)
+    p->micro_target_contrast = 0.04f;
 }
 
 int lre_hip_batch_create(lre_batch **out, int device, int lanes) {

@@ -30,11 +30,14 @@ STAGE_DETAIL_TM = 5
 STAGE_TONE_MAPPING = 6
 STAGE_COLOR_SPACE = 7
 STAGE_GAMMA = 8
+STAGE_HIGHLIGHT = 9
+STAGE_MICRO_CONTRAST = 10
 
 STAGE_NAMES = {
     "demosaic": STAGE_DEMOSAIC, "lateral_ca": STAGE_LATERAL_CA, "axial_ca": STAGE_AXIAL_CA,
     "defringe": STAGE_DEFRINGE, "detail_tm": STAGE_DETAIL_TM, "tone_mapping": STAGE_TONE_MAPPING,
     "color_space": STAGE_COLOR_SPACE, "gamma": STAGE_GAMMA,
+    "highlight": STAGE_HIGHLIGHT, "micro_contrast": STAGE_MICRO_CONTRAST,
 }
 
 # demosaic -> axial CA -> defringe -> detail tone map
@@ -52,6 +55,7 @@ class ChainParams(ctypes.Structure):
         ("tone_after_scale", _f),
         ("color_transform", _f * 12), ("gamma_power", _f), ("gamma_slope", _f),
         ("output_color_space", _i),
+        ("highlight_threshold", _f), ("micro_strength", _f), ("micro_target_contrast", _f),
     ]
 
     @classmethod

@@ -24,6 +24,7 @@ int main(void) {
     F(lre_chain_params, axial_radius) F(lre_chain_params, lateral_levels)
     F(lre_chain_params, tone_after_scale) F(lre_chain_params, color_transform)
     F(lre_chain_params, gamma_power) F(lre_chain_params, output_color_space)
+    F(lre_chain_params, highlight_threshold) F(lre_chain_params, micro_target_contrast)
     F(lre_request, image) F(lre_request, h_raw) F(lre_request, h_rgb)
     F(lre_completion, status) F(lre_completion, t_submit) F(lre_completion, t_done)
     return 0;

@@ -99,6 +99,7 @@ def _single(acc, it, stages, p):
         cam = np.array(WB, np.float32)
     else:
         d_cfa, maxv, cam = it["d_in"], 1.0, np.ones(4, np.float32)
+    d_mask = None
     for s in stages:
         if s == "demosaic":
             if filters == 9:
@@ -129,6 +130,18 @@ def _single(acc, it, stages, p):
         elif s == "gamma":
             rc = lib.lre_hip_dev_gamma_correct(ctx, d_rgb.ptr, d_rgb.ptr, w, h, p.gamma_power,
                                                p.gamma_slope, p.output_color_space, None)
+        elif s == "highlight":
+            d_mask = acc.dev_alloc(h * w * 4)
+            wb3 = np.array(WB[:3], np.float32)
+            wbp = wb3.ctypes.data_as(ct.POINTER(ct.c_float)) if it["inp"] == "u16" else None
+            rc = lib.lre_hip_dev_recover_highlights(ctx, d_rgb.ptr, d_rgb.ptr, w, h,
+                                                    ct.c_float(p.highlight_threshold), d_mask.ptr, wbp, None,
+                                                    None)
+        elif s == "micro_contrast":
+            thr = np.float32(p.highlight_threshold) + np.float32(0.3)  # float, as the runtime adds it
+            rc = lib.lre_hip_dev_micro_contrast(ctx, d_rgb.ptr, d_rgb.ptr, w, h, ct.c_float(thr),
+                                                ct.c_float(p.micro_strength), ct.c_float(p.micro_target_contrast),
+                                                d_mask.ptr if d_mask is not None else None, None)
         assert rc == 0, s
     acc.synchronize()
     return d_rgb.download((h, w, 3))
@@ -188,6 +201,27 @@ def test_fused_epilogue_srgb(acc):
     for it, g in zip(S.items, S.results()):
         assert np.array_equal(g, _single(acc, it, stages, p)), it["kind"]
     b.close()
+
+
+def test_batch_highlight_stages(acc):
+    """postprocess's highlight_mode order on the device: recovery (keeping its
+    mask), tone map, micro-contrast blended by that mask; and micro-contrast
+    alone, gated on green."""
+    from libraw_enhanced_amd import develop as dv
+    p = dv.ChainParams.defaults(highlight_threshold=0.8)
+    for stages in [("demosaic", "highlight", "detail_tm", "micro_contrast"),
+                   ("demosaic", "highlight", "tone_mapping", "micro_contrast"),
+                   ("demosaic", "micro_contrast"), ("demosaic", "highlight")]:
+        print(f"stages {stages}", flush=True)
+        S = _Set(acc, _spec()[:5])
+        b = dv.BatchDeveloper(0, 2)
+        b.run(S.images(), stages, p)
+        b.synchronize()
+        for it, g in zip(S.items, S.results()):
+            want = _single(acc, it, stages, p)
+            bad = int(np.count_nonzero(g != want))
+            assert bad == 0, f"{stages} {it['kind']} {it['inp']}: {bad} values differ"
+        b.close()
 
 
 def test_batch_graph_replays(acc):
