@@ -41,7 +41,8 @@ constexpr int HR_TW = 104;           // LoCA tile width in t
 constexpr int HR_SR = HR_TR + 4;     // staged rows (two above, two below)
 constexpr int HR_LW = HR_TW + 17;    // staged columns: t - 8 .. t + W + 8, odd stride
 constexpr int HR_WORDS = (HR_TW + 31) / 32;
-constexpr size_t HR_LDS = sizeof(float) * (2 * HR_SR * HR_LW + 3 * HR_TR * HR_TW) + HR_TR * HR_TW +
+constexpr int HR_OW = HR_TW + 1;     // row stride of the tile's own pixels (odd: no LDS bank conflicts)
+constexpr size_t HR_LDS = sizeof(float) * (2 * HR_SR * HR_LW + 3 * HR_TR * HR_OW) + HR_TR * HR_TW +
                           sizeof(unsigned) * HR_WORDS;
 
 struct HrState {
@@ -306,13 +307,13 @@ LRE_DEV void tap_info(float r, float g, float b, float sat075, float &nl, float 
 
 struct LocaTile {
     float *NL, *RT;        // [HR_SR][HR_LW], skewed: pixel (r, s) of the tile at (r + 2) * HR_LW + s + 8
-    float *OR, *OG, *OB;   // the tile's own pixels [HR_TR][HR_TW]
+    float *OR, *OG, *OB;   // the tile's own pixels [HR_TR][HR_OW]
 };
 
 // LoCA green lift of the tile pixel (r, s) (:1299-1372).  Neighbour (dy, dx)
 // sits at c + dy * (HR_LW + 3) + dx of the skewed planes.
 LRE_DEV void loca_px(const LocaTile &L, int r, int s, float sat, float sat075, float max_val) {
-    const int o = r * HR_TW + s, c = (r + 2) * HR_LW + s + 8;
+    const int o = r * HR_OW + s, c = (r + 2) * HR_LW + s + 8;
     const float p0 = L.OR[o], p1 = L.OG[o], p2 = L.OB[o];
     const float l = max3(p0, p1, p2);
     if (l < sat * 0.90f) return;
@@ -378,9 +379,9 @@ __global__ __launch_bounds__(HR_BLOCK) void hr_loca(float *img, int w, int h, fl
     L.NL = lds;
     L.RT = L.NL + HR_SR * HR_LW;
     L.OR = L.RT + HR_SR * HR_LW;
-    L.OG = L.OR + HR_TR * HR_TW;
-    L.OB = L.OG + HR_TR * HR_TW;
-    unsigned char *F = reinterpret_cast<unsigned char *>(L.OB + HR_TR * HR_TW);
+    L.OG = L.OR + HR_TR * HR_OW;
+    L.OB = L.OG + HR_TR * HR_OW;
+    unsigned char *F = reinterpret_cast<unsigned char *>(L.OB + HR_TR * HR_OW);
     unsigned *step_any = reinterpret_cast<unsigned *>(F + HR_TR * HR_TW);
     const int b = b_lo + (int)blockIdx.x, k = d - b;
     if (tile_count[b * tcols + k] == 0u) return;
@@ -404,9 +405,9 @@ __global__ __launch_bounds__(HR_BLOCK) void hr_loca(float *img, int w, int h, fl
             tap_info(v.x, v.y, v.z, sat075, L.NL[e], L.RT[e]);
             const int tr = sr - 2, ts = j - 8;
             if (tr >= 0 && tr < HR_TR && ts >= 0 && ts < HR_TW) {
-                L.OR[tr * HR_TW + ts] = v.x;
-                L.OG[tr * HR_TW + ts] = v.y;
-                L.OB[tr * HR_TW + ts] = v.z;
+                L.OR[tr * HR_OW + ts] = v.x;
+                L.OG[tr * HR_OW + ts] = v.y;
+                L.OB[tr * HR_OW + ts] = v.z;
             }
         });
     lre::stage2d<HR_BLOCK>(
@@ -422,12 +423,22 @@ __global__ __launch_bounds__(HR_BLOCK) void hr_loca(float *img, int w, int h, fl
     __syncthreads();
     const float max_val = __uint_as_float(st->max_bits);
     const int r = threadIdx.x;
+    // this lane's flags as bits, so the step loop reads no LDS to decide
+    unsigned fb[HR_WORDS];
+#pragma unroll
+    for (int wd = 0; wd < HR_WORDS; wd++) {
+        fb[wd] = 0u;
+        if (r < HR_TR)
+            for (int q = 0; q < 32 && wd * 32 + q < HR_TW; q++)
+                fb[wd] |= (unsigned)(F[r * HR_TW + wd * 32 + q] != 0) << q;
+    }
+#pragma unroll
     for (int wd = 0; wd < HR_WORDS; wd++) {
         unsigned bits = step_any[wd];
         while (bits) {
-            const int s = wd * 32 + __ffs(bits) - 1;
+            const int q = __ffs(bits) - 1;
             bits &= bits - 1;
-            if (r < HR_TR && F[r * HR_TW + s]) loca_px(L, r, s, sat, sat075, max_val);
+            if ((fb[wd] >> q) & 1u) loca_px(L, r, wd * 32 + q, sat, sat075, max_val);
             __syncthreads();
         }
     }
@@ -436,9 +447,10 @@ __global__ __launch_bounds__(HR_BLOCK) void hr_loca(float *img, int w, int h, fl
         const int rr = e / HR_TW, s = e - rr * HR_TW;
         const int y = Y + rr, x = T + s - 3 * y;
         const size_t i = (size_t)y * w + x;
-        img[3 * i] = L.OR[e];
-        img[3 * i + 1] = L.OG[e];
-        img[3 * i + 2] = L.OB[e];
+        const int o = rr * HR_OW + s;
+        img[3 * i] = L.OR[o];
+        img[3 * i + 1] = L.OG[o];
+        img[3 * i + 2] = L.OB[o];
     }
 }
 
