@@ -18,6 +18,13 @@
  *       the drop-in for the corresponding Accelerator method;
  *   lre_hip_dev_<stage>(ctx, device pointers..., stream)  enqueue only, on the
  *       given HIP stream (NULL = the context's stream); inputs stay in HBM.
+ *
+ * The calls on one context share its device scratch.  A call on a different
+ * stream from the previous call waits on the host for that call's work first,
+ * so calls on several streams are correct but do not overlap; use one context
+ * per stream for overlap.  Work captured into a HIP graph uses the scratch
+ * whenever the graph is launched: order such launches against other calls on
+ * the context yourself.
  */
 #ifndef LRE_HIP_H
 #define LRE_HIP_H
@@ -374,7 +381,9 @@ int lre_hip_batch_run(lre_batch *b, const lre_image *images, int n, const int *s
 int lre_hip_batch_synchronize(lre_batch *b);
 /* Record the batch (same arguments as lre_hip_batch_run) into a HIP graph after
  * one eager run that sizes every buffer; lre_hip_batch_replay launches it.  The
- * graph holds kernel nodes only and refers to the images' device buffers. */
+ * graph holds kernel nodes only and refers to the images' device buffers.  A
+ * later lre_hip_batch_run that needs larger arenas or scratch reallocates them
+ * and drops the graph: replay then fails until the batch is captured again. */
 int lre_hip_batch_capture(lre_batch *b, const lre_image *images, int n, const int *stages,
                           int nstages, const lre_chain_params *params);
 int lre_hip_batch_replay(lre_batch *b);
@@ -393,7 +402,11 @@ int lre_hip_batch_autotune(lre_batch *b, const lre_image *images, int n, const i
  * lre_hip_host_alloc gives pinned buffers for the fastest copies.  A request is
  * admitted when one of `lanes` worker streams is free and its device footprint
  * (input, float CFA, RGB and stage scratch) fits in hbm_budget_bytes beside the
- * requests already admitted; admitted requests run concurrently on the device. */
+ * requests already admitted; admitted requests run concurrently on the device.
+ * The stage list must start with LRE_STAGE_DEMOSAIC (a request carries a raw
+ * plane); the parameters of its stages are checked at creation.  A request
+ * that fails posts its error status and leaves the others unaffected: its
+ * completion is posted only after all of its device work has finished. */
 typedef struct lre_server lre_server;
 typedef struct lre_request {
     uint64_t id;

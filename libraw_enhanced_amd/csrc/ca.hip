@@ -534,13 +534,18 @@ int lateral_levels(int W0, int H0, int levels) {
 
 size_t lre_axial_scratch_bytes(int w, int h) { return align256((size_t)w * h * 4) * 5 + 256; }
 
-int lre_launch_axial_ca(const float *d_in, float *d_out, int w, int h, int radius, float epsilon,
-                        float strength, void *d_scratch, hipStream_t s) {
-    if (radius < 1) radius = 1;
+int lre_check_axial_params(int radius) {
     if (radius > AX_RMAX) {
         lre_set_error("axial CA radius %d above the supported %d", radius, AX_RMAX);
         return LRE_ERR_ARG;
     }
+    return LRE_OK;
+}
+
+int lre_launch_axial_ca(const float *d_in, float *d_out, int w, int h, int radius, float epsilon,
+                        float strength, void *d_scratch, hipStream_t s) {
+    if (const int rc = lre_check_axial_params(radius)) return rc;
+    if (radius < 1) radius = 1;
     strength = strength < 0.f ? 0.f : (1.f < strength ? 1.f : strength);
     const size_t n = (size_t)w * h, plane = align256(n * 4);
     char *base = static_cast<char *>(d_scratch);

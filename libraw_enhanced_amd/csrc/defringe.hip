@@ -731,10 +731,20 @@ size_t lre_defringe_scratch_bytes(int w, int h) {
     return plane * 20 + 4096;
 }
 
+int lre_check_defringe_params(float radius) {
+    const float sigma = (radius / 3.f < 0.5f) ? 0.5f : radius / 3.f;
+    if (!std::isfinite(radius) || 2.f * std::ceil(3.f * sigma) + 1.f > (float)MAX_TAPS) {
+        lre_set_error("defringe radius %g not supported (finite, at most %d taps)", radius, MAX_TAPS);
+        return LRE_ERR_ARG;
+    }
+    return LRE_OK;
+}
+
 int lre_launch_defringe(const float *d_in, float *d_out, int w, int h, float radius,
                         float edge_threshold, float chroma_threshold, float strength,
                         int defringe_green, float green_strength_scale, void *d_scratch,
                         hipStream_t s) {
+    if (const int rc = lre_check_defringe_params(radius)) return rc;
     const size_t n = (size_t)w * h;
     const size_t plane = ((n * 4) + 255) & ~(size_t)255;
     char *base = static_cast<char *>(d_scratch);
@@ -750,10 +760,6 @@ int lre_launch_defringe(const float *d_in, float *d_out, int w, int h, float rad
     green_strength_scale = green_strength_scale < 0.f ? 0.f : (1.f < green_strength_scale ? 1.f : green_strength_scale);
     const float sigma = (radius / 3.f < 0.5f) ? 0.5f : radius / 3.f;
     const int ks = (int)std::ceil(3.f * sigma);
-    if (2 * ks + 1 > MAX_TAPS) {
-        lre_set_error("defringe radius %g too large (at most %d taps)", radius, MAX_TAPS);
-        return LRE_ERR_ARG;
-    }
     Taps taps{};
     {
         float ksum = 0.f;

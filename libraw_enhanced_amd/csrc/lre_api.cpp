@@ -72,8 +72,9 @@ int rgb_stage_host(lre_ctx *ctx, const float *in_rgb, float *out_rgb, int w, int
     if (!valid_dims(w, h)) return set_arg_error("image size");
     LRE_HIP_CHECK(hipSetDevice(ctx->device));
     const size_t n = (size_t)w * h;
-    int rc = lre_ensure_scratch(ctx, aligned(n * 12) * 2 + stage_scratch);
+    int rc = lre_ensure_scratch(ctx, aligned(n * 12) * 2 + stage_scratch, ctx->stream);
     if (rc) return rc;
+    lre_scratch_guard scratch_use{ctx, ctx->stream};
     Carve c(ctx->scratch);
     float *d_in = c.take<float>(n * 3), *d_out = c.take<float>(n * 3);
     void *ws = static_cast<char *>(ctx->scratch) + c.off;
@@ -93,7 +94,19 @@ void lre_set_error(const char *fmt, ...) {
     g_last_error = buf;
 }
 
-int lre_ensure_scratch(lre_ctx *ctx, size_t bytes) {
+void lre_scratch_release(lre_ctx *ctx, hipStream_t s) {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    ctx->scratch_pending = hipStreamIsCapturing(s, &st) == hipSuccess && st == hipStreamCaptureStatusNone &&
+                           hipEventRecord(ctx->scratch_done, s) == hipSuccess;
+    ctx->scratch_stream = s;
+}
+
+int lre_ensure_scratch(lre_ctx *ctx, size_t bytes, hipStream_t s) {
+    if (ctx->scratch_stream != s && ctx->scratch_pending) {
+        // the previous call ran on another stream and may still use the scratch
+        LRE_HIP_CHECK(hipEventSynchronize(ctx->scratch_done));
+        ctx->scratch_pending = false;
+    }
     if (ctx->scratch_bytes >= bytes) return LRE_OK;
     if (ctx->scratch) {
         // work on any stream may still use the old scratch
@@ -137,7 +150,9 @@ int lre_hip_create(lre_ctx **out, int device) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
     hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->scratch_done, hipEventDisableTiming);
     if (e != hipSuccess) {
+        if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
         delete ctx;
         lre_set_error("hipStreamCreate failed: %s", hipGetErrorString(e));
         return LRE_ERR_HIP;
@@ -152,6 +167,7 @@ int lre_hip_destroy(lre_ctx *ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->scratch_done) (void)hipEventDestroy(ctx->scratch_done);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return LRE_OK;
@@ -207,8 +223,9 @@ int lre_hip_dev_amaze(lre_ctx *ctx, const float *d_cfa, float *d_rgb, int w, int
     if (!valid_dims(w, h)) return set_arg_error("image size");
     if (!(maximum_value > 0.f)) return set_arg_error("maximum_value must be > 0");
     if (filters == 9) return set_arg_error("AMaZE needs a Bayer pattern");
-    int rc = lre_ensure_scratch(ctx, lre_amaze_scratch_bytes(ctx, w, h));
+    int rc = lre_ensure_scratch(ctx, lre_amaze_scratch_bytes(ctx, w, h), lre_pick_stream(ctx, stream));
     if (rc) return rc;
+    lre_scratch_guard scratch_use{ctx, lre_pick_stream(ctx, stream)};
     return lre_launch_amaze(ctx, d_cfa, d_rgb, w, h, filters, cam_mul, maximum_value,
                             ctx->scratch, lre_pick_stream(ctx, stream));
 }
@@ -218,8 +235,9 @@ int lre_hip_dev_xtrans3(lre_ctx *ctx, const float *d_cfa, float *d_rgb, int w, i
     if (!ctx || !d_cfa || !d_rgb || !xtrans) return set_arg_error("null pointer");
     if (!valid_dims(w, h)) return set_arg_error("image size");
     if (!(maximum_value > 0.f)) return set_arg_error("maximum_value must be > 0");
-    int rc = lre_ensure_scratch(ctx, lre_xtrans_scratch_bytes(ctx, w, h));
+    int rc = lre_ensure_scratch(ctx, lre_xtrans_scratch_bytes(ctx, w, h), lre_pick_stream(ctx, stream));
     if (rc) return rc;
+    lre_scratch_guard scratch_use{ctx, lre_pick_stream(ctx, stream)};
     return lre_launch_xtrans3(ctx, d_cfa, d_rgb, w, h, xtrans, maximum_value, ctx->scratch,
                               lre_pick_stream(ctx, stream));
 }
@@ -236,8 +254,9 @@ int lre_hip_dev_detail_tonemap(lre_ctx *ctx, const float *d_in, float *d_out, in
                                void *stream) {
     if (!ctx || !d_in || !d_out) return set_arg_error("null pointer");
     if (!valid_dims(w, h)) return set_arg_error("image size");
-    int rc = lre_ensure_scratch(ctx, lre_detail_tm_scratch_bytes(w, h));
+    int rc = lre_ensure_scratch(ctx, lre_detail_tm_scratch_bytes(w, h), lre_pick_stream(ctx, stream));
     if (rc) return rc;
+    lre_scratch_guard scratch_use{ctx, lre_pick_stream(ctx, stream)};
     return lre_launch_detail_tonemap(d_in, d_out, w, h, ctx->scratch, lre_pick_stream(ctx, stream));
 }
 
@@ -246,8 +265,9 @@ int lre_hip_dev_micro_contrast(lre_ctx *ctx, const float *d_in, float *d_out, in
                                const float *d_mask, void *stream) {
     if (!ctx || !d_in || !d_out) return set_arg_error("null pointer");
     if (!valid_dims(w, h)) return set_arg_error("image size");
-    int rc = lre_ensure_scratch(ctx, lre_micro_contrast_scratch_bytes(w, h));
+    int rc = lre_ensure_scratch(ctx, lre_micro_contrast_scratch_bytes(w, h), lre_pick_stream(ctx, stream));
     if (rc) return rc;
+    lre_scratch_guard scratch_use{ctx, lre_pick_stream(ctx, stream)};
     return lre_launch_micro_contrast(d_in, d_out, w, h, threshold, strength, target_contrast, d_mask,
                                      ctx->scratch, lre_pick_stream(ctx, stream));
 }
@@ -259,8 +279,9 @@ int lre_hip_dev_recover_highlights(lre_ctx *ctx, const float *d_in, float *d_out
     if (!valid_dims(w, h)) return set_arg_error("image size");
     const size_t n = (size_t)w * h;
     const bool inplace = d_in == d_out;
-    int rc = lre_ensure_scratch(ctx, (inplace ? aligned(n * 12) : 0) + lre_highlight_scratch_bytes(w, h));
+    int rc = lre_ensure_scratch(ctx, (inplace ? aligned(n * 12) : 0) + lre_highlight_scratch_bytes(w, h), lre_pick_stream(ctx, stream));
     if (rc) return rc;
+    lre_scratch_guard scratch_use{ctx, lre_pick_stream(ctx, stream)};
     hipStream_t s = lre_pick_stream(ctx, stream);
     Carve c(ctx->scratch);
     const float *src = d_in;
@@ -279,8 +300,9 @@ int lre_hip_dev_defringe(lre_ctx *ctx, const float *d_in, float *d_out, int w, i
                          void *stream) {
     if (!ctx || !d_in || !d_out) return set_arg_error("null pointer");
     if (!valid_dims(w, h)) return set_arg_error("image size");
-    int rc = lre_ensure_scratch(ctx, lre_defringe_scratch_bytes(w, h));
+    int rc = lre_ensure_scratch(ctx, lre_defringe_scratch_bytes(w, h), lre_pick_stream(ctx, stream));
     if (rc) return rc;
+    lre_scratch_guard scratch_use{ctx, lre_pick_stream(ctx, stream)};
     return lre_launch_defringe(d_in, d_out, w, h, radius, edge_threshold, chroma_threshold,
                                strength, defringe_green, green_strength_scale, ctx->scratch,
                                lre_pick_stream(ctx, stream));
@@ -290,8 +312,9 @@ int lre_hip_dev_axial_ca(lre_ctx *ctx, const float *d_in, float *d_out, int w, i
                          float epsilon, float strength, void *stream) {
     if (!ctx || !d_in || !d_out) return set_arg_error("null pointer");
     if (!valid_dims(w, h)) return set_arg_error("image size");
-    int rc = lre_ensure_scratch(ctx, lre_axial_scratch_bytes(w, h));
+    int rc = lre_ensure_scratch(ctx, lre_axial_scratch_bytes(w, h), lre_pick_stream(ctx, stream));
     if (rc) return rc;
+    lre_scratch_guard scratch_use{ctx, lre_pick_stream(ctx, stream)};
     return lre_launch_axial_ca(d_in, d_out, w, h, radius, epsilon, strength, ctx->scratch,
                                lre_pick_stream(ctx, stream));
 }
@@ -301,8 +324,9 @@ int lre_hip_dev_lateral_ca(lre_ctx *ctx, const float *d_in, float *d_out, int w,
                            float min_confidence, int pyramid_levels, void *stream) {
     if (!ctx || !d_in || !d_out) return set_arg_error("null pointer");
     if (!valid_dims(w, h)) return set_arg_error("image size");
-    int rc = lre_ensure_scratch(ctx, lre_lateral_scratch_bytes(w, h, cell_size));
+    int rc = lre_ensure_scratch(ctx, lre_lateral_scratch_bytes(w, h, cell_size), lre_pick_stream(ctx, stream));
     if (rc) return rc;
+    lre_scratch_guard scratch_use{ctx, lre_pick_stream(ctx, stream)};
     return lre_launch_lateral_ca(d_in, d_out, w, h, cell_size, max_iterations, max_shift,
                                  min_confidence, pyramid_levels, ctx->scratch,
                                  lre_pick_stream(ctx, stream));
@@ -341,8 +365,9 @@ int lre_hip_amaze(lre_ctx *ctx, const float *raw_rgb, float *rgb, int w, int h,
     LRE_HIP_CHECK(hipSetDevice(ctx->device));
     const size_t n = (size_t)w * h;
     const size_t io = aligned(n * 3 * 4) * 2 + aligned(n * 4);
-    int rc = lre_ensure_scratch(ctx, io + lre_amaze_scratch_bytes(ctx, w, h));
+    int rc = lre_ensure_scratch(ctx, io + lre_amaze_scratch_bytes(ctx, w, h), ctx->stream);
     if (rc) return rc;
+    lre_scratch_guard scratch_use{ctx, ctx->stream};
     Carve c(ctx->scratch);
     float *d_in = c.take<float>(n * 3), *d_out = c.take<float>(n * 3), *d_cfa = c.take<float>(n);
     void *d_ws = static_cast<char *>(ctx->scratch) + c.off;
@@ -365,8 +390,9 @@ int lre_hip_xtrans3(lre_ctx *ctx, const float *raw_rgb, float *rgb, int w, int h
     LRE_HIP_CHECK(hipSetDevice(ctx->device));
     const size_t n = (size_t)w * h;
     const size_t io = aligned(n * 3 * 4) * 2 + aligned(n * 4);
-    int rc = lre_ensure_scratch(ctx, io + lre_xtrans_scratch_bytes(ctx, w, h));
+    int rc = lre_ensure_scratch(ctx, io + lre_xtrans_scratch_bytes(ctx, w, h), ctx->stream);
     if (rc) return rc;
+    lre_scratch_guard scratch_use{ctx, ctx->stream};
     Carve c(ctx->scratch);
     float *d_in = c.take<float>(n * 3), *d_out = c.take<float>(n * 3), *d_cfa = c.take<float>(n);
     void *d_ws = static_cast<char *>(ctx->scratch) + c.off;
@@ -387,8 +413,9 @@ int lre_hip_tone_mapping(lre_ctx *ctx, const float *in_rgb, float *out_rgb, int 
     if (!valid_dims(w, h)) return set_arg_error("image size");
     LRE_HIP_CHECK(hipSetDevice(ctx->device));
     const size_t n = (size_t)w * h * 3;
-    int rc = lre_ensure_scratch(ctx, aligned(n * 4));
+    int rc = lre_ensure_scratch(ctx, aligned(n * 4), ctx->stream);
     if (rc) return rc;
+    lre_scratch_guard scratch_use{ctx, ctx->stream};
     float *d = static_cast<float *>(ctx->scratch);
     if ((rc = copy_in(ctx, d, in_rgb, n * 4))) return rc;
     if ((rc = lre_launch_tone_mapping(d, d, n, after_scale, ctx->stream))) return rc;
@@ -400,8 +427,9 @@ int lre_hip_detail_tonemap(lre_ctx *ctx, const float *in_rgb, float *out_rgb, in
     if (!valid_dims(w, h)) return set_arg_error("image size");
     LRE_HIP_CHECK(hipSetDevice(ctx->device));
     const size_t n = (size_t)w * h * 3;
-    int rc = lre_ensure_scratch(ctx, aligned(n * 4) + lre_detail_tm_scratch_bytes(w, h));
+    int rc = lre_ensure_scratch(ctx, aligned(n * 4) + lre_detail_tm_scratch_bytes(w, h), ctx->stream);
     if (rc) return rc;
+    lre_scratch_guard scratch_use{ctx, ctx->stream};
     float *d = static_cast<float *>(ctx->scratch);
     void *ws = static_cast<char *>(ctx->scratch) + aligned(n * 4);
     if ((rc = copy_in(ctx, d, in_rgb, n * 4))) return rc;
@@ -417,8 +445,9 @@ int lre_hip_micro_contrast(lre_ctx *ctx, const float *in_rgb, float *out_rgb, in
     LRE_HIP_CHECK(hipSetDevice(ctx->device));
     const size_t n = (size_t)w * h;
     int rc = lre_ensure_scratch(ctx, aligned(n * 12) + (mask ? aligned(n * 4) : 0) +
-                                         lre_micro_contrast_scratch_bytes(w, h));
+                                         lre_micro_contrast_scratch_bytes(w, h), ctx->stream);
     if (rc) return rc;
+    lre_scratch_guard scratch_use{ctx, ctx->stream};
     Carve c(ctx->scratch);
     float *d = c.take<float>(n * 3);
     float *d_mask = mask ? c.take<float>(n) : nullptr;
@@ -439,8 +468,9 @@ int lre_hip_recover_highlights(lre_ctx *ctx, const float *in_rgb, float *out_rgb
     LRE_HIP_CHECK(hipSetDevice(ctx->device));
     const size_t n = (size_t)w * h;
     int rc = lre_ensure_scratch(ctx, 2 * aligned(n * 12) + (mask_out ? aligned(n * 4) : 0) + 256 +
-                                         lre_highlight_scratch_bytes(w, h));
+                                         lre_highlight_scratch_bytes(w, h), ctx->stream);
     if (rc) return rc;
+    lre_scratch_guard scratch_use{ctx, ctx->stream};
     Carve c(ctx->scratch);
     float *d_in = c.take<float>(n * 3);
     float *d_out = c.take<float>(n * 3);
@@ -464,8 +494,9 @@ int lre_hip_defringe(lre_ctx *ctx, const float *in_rgb, float *out_rgb, int w, i
     LRE_HIP_CHECK(hipSetDevice(ctx->device));
     const size_t n = (size_t)w * h;
     const bool inplace = in_rgb == out_rgb;
-    int rc = lre_ensure_scratch(ctx, aligned(n * 12) * 2 + lre_defringe_scratch_bytes(w, h));
+    int rc = lre_ensure_scratch(ctx, aligned(n * 12) * 2 + lre_defringe_scratch_bytes(w, h), ctx->stream);
     if (rc) return rc;
+    lre_scratch_guard scratch_use{ctx, ctx->stream};
     Carve c(ctx->scratch);
     float *d_in = c.take<float>(n * 3), *d_out = c.take<float>(n * 3);
     void *ws = static_cast<char *>(ctx->scratch) + c.off;
@@ -521,8 +552,9 @@ int lre_hip_quantize(lre_ctx *ctx, const float *in, void *out, size_t count, int
     if (count % 3) return set_arg_error("count must be a multiple of 3 (RGB samples)");
     LRE_HIP_CHECK(hipSetDevice(ctx->device));
     const size_t ob = count * (size_t)(bits / 8);
-    int rc = lre_ensure_scratch(ctx, aligned(count * 4) + aligned(ob));
+    int rc = lre_ensure_scratch(ctx, aligned(count * 4) + aligned(ob), ctx->stream);
     if (rc) return rc;
+    lre_scratch_guard scratch_use{ctx, ctx->stream};
     Carve c(ctx->scratch);
     float *d_in = c.take<float>(count);
     void *d_out = c.take<char>(ob);
@@ -571,8 +603,9 @@ int lre_hip_green_matching(lre_ctx *ctx, uint16_t *img4, int w, int h, uint32_t 
     if (filters == 9) return LRE_OK;
     LRE_HIP_CHECK(hipSetDevice(ctx->device));
     const size_t n = (size_t)w * h;
-    int rc = lre_ensure_scratch(ctx, aligned(n * 8) * 2);
+    int rc = lre_ensure_scratch(ctx, aligned(n * 8) * 2, ctx->stream);
     if (rc) return rc;
+    lre_scratch_guard scratch_use{ctx, ctx->stream};
     Carve c(ctx->scratch);
     uint16_t *d_src = c.take<uint16_t>(n * 4), *d_dst = c.take<uint16_t>(n * 4);
     if ((rc = copy_in(ctx, d_src, img4, n * 8))) return rc;
@@ -594,8 +627,9 @@ int lre_hip_remove_zeroes(lre_ctx *ctx, uint16_t *img4, int w, int h, uint32_t f
     if (!valid_dims(w, h)) return set_arg_error("image size");
     LRE_HIP_CHECK(hipSetDevice(ctx->device));
     const size_t n = (size_t)w * h;
-    int rc = lre_ensure_scratch(ctx, aligned(n * 8) + lre_remove_zeroes_scratch_bytes(w, h));
+    int rc = lre_ensure_scratch(ctx, aligned(n * 8) + lre_remove_zeroes_scratch_bytes(w, h), ctx->stream);
     if (rc) return rc;
+    lre_scratch_guard scratch_use{ctx, ctx->stream};
     Carve c(ctx->scratch);
     uint16_t *d = c.take<uint16_t>(n * 4);
     void *ws = static_cast<char *>(ctx->scratch) + c.off;
@@ -609,8 +643,9 @@ int lre_hip_dev_remove_zeroes(lre_ctx *ctx, uint16_t *d_img4, int w, int h, uint
     if (!ctx || !d_img4) return set_arg_error("null pointer");
     if (!valid_dims(w, h)) return set_arg_error("image size");
     LRE_HIP_CHECK(hipSetDevice(ctx->device));
-    int rc = lre_ensure_scratch(ctx, lre_remove_zeroes_scratch_bytes(w, h));
+    int rc = lre_ensure_scratch(ctx, lre_remove_zeroes_scratch_bytes(w, h), lre_pick_stream(ctx, stream));
     if (rc) return rc;
+    lre_scratch_guard scratch_use{ctx, lre_pick_stream(ctx, stream)};
     return lre_launch_remove_zeroes(d_img4, w, h, filters, ctx->scratch, lre_pick_stream(ctx, stream));
 }
 
@@ -620,8 +655,9 @@ int lre_hip_auto_wb(lre_ctx *ctx, const uint16_t *img4, int w, int h, uint32_t f
     if (!valid_dims(w, h)) return set_arg_error("image size");
     LRE_HIP_CHECK(hipSetDevice(ctx->device));
     const size_t n = (size_t)w * h;
-    int rc = lre_ensure_scratch(ctx, aligned(n * 8) + 256 + 256);
+    int rc = lre_ensure_scratch(ctx, aligned(n * 8) + 256 + 256, ctx->stream);
     if (rc) return rc;
+    lre_scratch_guard scratch_use{ctx, ctx->stream};
     Carve c(ctx->scratch);
     uint16_t *d = c.take<uint16_t>(n * 4);
     float *d_pm = c.take<float>(4);
@@ -636,8 +672,9 @@ int lre_hip_dev_auto_wb(lre_ctx *ctx, const uint16_t *d_img4, int w, int h, uint
     if (!ctx || !d_img4 || !d_pre_mul || (filters == 9 && !xtrans)) return set_arg_error("null pointer");
     if (!valid_dims(w, h)) return set_arg_error("image size");
     LRE_HIP_CHECK(hipSetDevice(ctx->device));
-    int rc = lre_ensure_scratch(ctx, 256);
+    int rc = lre_ensure_scratch(ctx, 256, lre_pick_stream(ctx, stream));
     if (rc) return rc;
+    lre_scratch_guard scratch_use{ctx, lre_pick_stream(ctx, stream)};
     return lre_launch_auto_wb(d_img4, w, h, filters, xtrans, d_pre_mul, ctx->scratch,
                               lre_pick_stream(ctx, stream));
 }
@@ -647,8 +684,9 @@ int lre_hip_scale_aber(lre_ctx *ctx, uint16_t *img4, int w, int h, const float a
     if (!valid_dims(w, h)) return set_arg_error("image size");
     LRE_HIP_CHECK(hipSetDevice(ctx->device));
     const size_t n = (size_t)w * h;
-    int rc = lre_ensure_scratch(ctx, aligned(n * 8) + lre_scale_aber_scratch_bytes(w, h));
+    int rc = lre_ensure_scratch(ctx, aligned(n * 8) + lre_scale_aber_scratch_bytes(w, h), ctx->stream);
     if (rc) return rc;
+    lre_scratch_guard scratch_use{ctx, ctx->stream};
     Carve c(ctx->scratch);
     uint16_t *d = c.take<uint16_t>(n * 4);
     void *ws = static_cast<char *>(ctx->scratch) + c.off;
@@ -662,8 +700,9 @@ int lre_hip_dev_scale_aber(lre_ctx *ctx, uint16_t *d_img4, int w, int h, const f
     if (!ctx || !d_img4 || !aber) return set_arg_error("null pointer");
     if (!valid_dims(w, h)) return set_arg_error("image size");
     LRE_HIP_CHECK(hipSetDevice(ctx->device));
-    int rc = lre_ensure_scratch(ctx, lre_scale_aber_scratch_bytes(w, h));
+    int rc = lre_ensure_scratch(ctx, lre_scale_aber_scratch_bytes(w, h), lre_pick_stream(ctx, stream));
     if (rc) return rc;
+    lre_scratch_guard scratch_use{ctx, lre_pick_stream(ctx, stream)};
     return lre_launch_scale_aber(d_img4, w, h, aber, ctx->scratch, lre_pick_stream(ctx, stream));
 }
 
@@ -675,8 +714,9 @@ int lre_hip_black_level(lre_ctx *ctx, uint16_t *img4, int w, int h, const int cb
     if (positional && (size_t)pos_rows * pos_cols > 4096) return set_arg_error("positional table size");
     LRE_HIP_CHECK(hipSetDevice(ctx->device));
     const size_t n = (size_t)w * h;
-    int rc = lre_ensure_scratch(ctx, aligned(n * 8) + aligned(4096 * 4));
+    int rc = lre_ensure_scratch(ctx, aligned(n * 8) + aligned(4096 * 4), ctx->stream);
     if (rc) return rc;
+    lre_scratch_guard scratch_use{ctx, ctx->stream};
     Carve c(ctx->scratch);
     uint16_t *d_img = c.take<uint16_t>(n * 4);
     int *d_pos = c.take<int>(4096);
@@ -695,8 +735,9 @@ int lre_hip_white_balance(lre_ctx *ctx, const uint16_t *img4, float *rgb, int w,
     if (filters == 9 && !xtrans) return set_arg_error("X-Trans table");
     LRE_HIP_CHECK(hipSetDevice(ctx->device));
     const size_t n = (size_t)w * h;
-    int rc = lre_ensure_scratch(ctx, aligned(n * 8) + aligned(n * 12));
+    int rc = lre_ensure_scratch(ctx, aligned(n * 8) + aligned(n * 12), ctx->stream);
     if (rc) return rc;
+    lre_scratch_guard scratch_use{ctx, ctx->stream};
     Carve c(ctx->scratch);
     uint16_t *d_in = c.take<uint16_t>(n * 4);
     float *d_out = c.take<float>(n * 3);

@@ -16,15 +16,31 @@ struct lre_ctx {
     // host thread at a time (the same contract as the reference's Accelerator).
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
+    // The stream of the last call that used the scratch, and an event recorded
+    // there after its work (scratch_pending: the event is set and not yet waited
+    // for).  A call on another stream first waits for that event.
+    hipStream_t scratch_stream = nullptr;
+    hipEvent_t scratch_done = nullptr;
+    bool scratch_pending = false;
     // Pinned staging for the host-pointer entry points.
     void *pinned = nullptr;
     size_t pinned_bytes = 0;
     int num_cus = 0;
 };
 
-// Scratch helpers (stream-ordered: the caller must not reuse scratch across
-// streams without synchronising).
-int lre_ensure_scratch(lre_ctx *ctx, size_t bytes);
+// Scratch of a call that enqueues on stream s: waits (on the host) for the work
+// of an earlier call on another stream, then grows the scratch to `bytes`.
+int lre_ensure_scratch(lre_ctx *ctx, size_t bytes, hipStream_t s);
+// Marks the end of a call's use of the scratch on stream s: records the
+// context's event there (not while s is being captured into a graph: a graph
+// uses the scratch when it is launched, and its launches are the caller's to
+// order against other calls on the context).
+void lre_scratch_release(lre_ctx *ctx, hipStream_t s);
+struct lre_scratch_guard {
+    lre_ctx *ctx;
+    hipStream_t s;
+    ~lre_scratch_guard() { lre_scratch_release(ctx, s); }
+};
 inline hipStream_t lre_pick_stream(lre_ctx *ctx, void *stream) {
     return stream ? static_cast<hipStream_t>(stream) : ctx->stream;
 }
@@ -51,6 +67,11 @@ struct lre_demosaic_job {
     float maximum_value;
     const float *d_maxv;
 };
+
+// The parameter checks the launchers make (LRE_OK or LRE_ERR_ARG with the
+// message set), for callers that validate a stage list before enqueueing.
+int lre_check_axial_params(int radius);
+int lre_check_defringe_params(float radius);
 
 // Kernel launchers (device pointers; scratch must already be large enough).
 int lre_launch_amaze(lre_ctx *ctx, const float *d_cfa, float *d_rgb, int w, int h,

@@ -53,6 +53,17 @@ int check_stages(const int *stages, int ns) {
     return LRE_OK;
 }
 
+// The parameters of the stages in the list, checked before anything is
+// enqueued (a launcher that rejects them later would leave a partial chain).
+int check_params(const int *stages, int ns, const lre_chain_params &p) {
+    int rc = LRE_OK;
+    for (int i = 0; i < ns && !rc; i++) {
+        if (stages[i] == LRE_STAGE_AXIAL_CA) rc = lre_check_axial_params(p.axial_radius);
+        else if (stages[i] == LRE_STAGE_DEFRINGE) rc = lre_check_defringe_params(p.defringe_radius);
+    }
+    return rc;
+}
+
 int check_image(const lre_image &im, bool demosaic) {
     if (im.width < 1 || im.height < 1 || !im.d_rgb) {
         lre_set_error("image needs a positive size and an output buffer");
@@ -336,6 +347,7 @@ struct Plan {
 int make_plan(lre_batch *b, const lre_image *imgs, int n, const int *stages, int ns,
               const lre_chain_params &p, Plan &P) {
     int rc = check_stages(stages, ns);
+    if (!rc) rc = check_params(stages, ns, p);
     if (rc) return rc;
     if (n < 0 || (n > 0 && !imgs)) {
         lre_set_error("invalid image list");
@@ -386,6 +398,9 @@ int ensure_buffers(lre_batch *b, const Plan &P, int n) {
     if (!grow) return LRE_OK;
     int rc = batch_sync_all(b);
     if (rc) return rc;
+    // A captured graph holds the arenas, workspace and lane scratch as kernel
+    // arguments: once they are reallocated it must not be launched again.
+    drop_graph(b);
     for (int k = 0; k < b->lanes; k++)
         if ((rc = b->lane[k].scratch.ensure(P.lane_scratch))) return rc;
     if (P.demosaic) {
@@ -820,9 +835,10 @@ size_t demosaic_arena(const lre_image &im, int num_cus, int per_cu, int *slots) 
     return align256(lre_amaze_multi_scratch_bytes(*slots)) + align256(lre_amaze_multi_table_bytes(1));
 }
 
-// One request on one stream, start to finish.  Returns an LRE status.
-int serve_one(lre_server *S, Lane &L, const lre_request &rq, char *dev, const Footprint &f,
-              size_t arena_bytes, int slots, void *h_tab) {
+// One request on one stream, start to finish.  Returns an LRE status.  May
+// return early with work still queued on the lane's stream: serve_one waits.
+int serve_one_enqueue(lre_server *S, Lane &L, const lre_request &rq, char *dev, const Footprint &f,
+                      size_t arena_bytes, int slots, void *h_tab) {
     const lre_image &im = rq.image;
     hipStream_t s = L.stream;
     const size_t px = (size_t)im.width * im.height;
@@ -875,7 +891,21 @@ int serve_one(lre_server *S, Lane &L, const lre_request &rq, char *dev, const Fo
                              d_scr, s)))
         return rc;
     LRE_HIP_CHECK(hipMemcpyAsync(rq.h_rgb, d_rgb, px * 12, hipMemcpyDeviceToHost, s));
-    LRE_HIP_CHECK(hipStreamSynchronize(s));
+    return LRE_OK;
+}
+
+// Every exit waits for the lane's stream, so that neither the device block
+// (returned to the pool) nor the caller's host buffers are still in use by a
+// copy or kernel of this request when its completion is posted.
+int serve_one(lre_server *S, Lane &L, const lre_request &rq, char *dev, const Footprint &f,
+              size_t arena_bytes, int slots, void *h_tab) {
+    const int rc = serve_one_enqueue(S, L, rq, dev, f, arena_bytes, slots, h_tab);
+    const hipError_t e = hipStreamSynchronize(L.stream);
+    if (rc) return rc;
+    if (e != hipSuccess) {
+        lre_set_error("request stream: %s", hipGetErrorString(e));
+        return LRE_ERR_HIP;
+    }
     return LRE_OK;
 }
 
@@ -960,14 +990,22 @@ int lre_hip_serve_create(lre_server **out, int device, int lanes, size_t budget,
     }
     int rc = check_stages(stages, ns);
     if (rc) return rc;
+    // requests carry a uint16 CFA plane only: the chain must start by demosaicing it
+    if (ns < 1 || stages[0] != LRE_STAGE_DEMOSAIC) {
+        lre_set_error("a served stage list must start with the demosaic stage");
+        return LRE_ERR_ARG;
+    }
+    lre_chain_params p;
+    if (params) p = *params;
+    else lre_chain_params_default(&p);
+    if ((rc = check_params(stages, ns, p))) return rc;
     LRE_HIP_CHECK(hipSetDevice(device));
     auto *S = new lre_server();
     S->device = device;
     S->lanes = lanes;
     S->budget = budget;
     S->stages.assign(stages, stages + ns);
-    if (params) S->params = *params;
-    else lre_chain_params_default(&S->params);
+    S->params = p;
     S->t0 = Clock::now();
     S->lane.resize(lanes);
     for (int k = 0; k < lanes && !rc; k++) rc = lane_open(S->lane[k]);

@@ -75,11 +75,15 @@ class DevelopChain:
         return self.rgb.download((self.h, self.w, 3), np.float32)
 
 
+CHAIN24 = ("amaze", "defringe", "detail_tm")
+
+
 def available_chain():
-    """The 24 MP Bayer chain, limited to the stages the loaded library exports."""
+    """The 24 MP Bayer chain: AMaZE -> defringe -> detail tone map.  Raises if
+    the loaded library lacks one of its entry points (a timed chain must never
+    silently run fewer stages)."""
     lib = native()
-    chain = ["amaze"]
-    if hasattr(lib, "lre_hip_dev_defringe"):
-        chain.append("defringe")
-    chain.append("detail_tm")
-    return chain
+    for fn in ("lre_hip_dev_amaze", "lre_hip_dev_defringe", "lre_hip_dev_detail_tonemap"):
+        if not hasattr(lib, fn):
+            raise RuntimeError(f"liblre_hip.so lacks {fn}: rebuild it (python __graft_entry__.py)")
+    return list(CHAIN24)

@@ -329,3 +329,104 @@ def test_serve_alone_and_in_the_mix(acc):
     for k, g in enumerate(S.results()):
         assert np.array_equal(g, solo[k]), f"request {k}: batch and served results differ"
     b.close()
+
+
+def test_batch_graph_dropped_when_buffers_grow(acc):
+    """A captured graph holds the arenas and lane scratch as kernel arguments:
+    an eager run that must reallocate them drops the graph, so replay fails
+    instead of launching on freed memory; a new capture replays bit-exact."""
+    from libraw_enhanced_amd import develop as dv
+    from libraw_enhanced_amd.accelerator import LibRawInvalidArgument
+    small = _Set(acc, _spec()[:2])
+    b = dv.BatchDeveloper(0, 2)
+    b.run(small.images(), CHAIN)
+    b.synchronize()
+    eager = small.results()
+    b.capture(small.images(), CHAIN)
+    b.replay()
+    b.synchronize()
+    for e, g in zip(eager, small.results()):
+        assert np.array_equal(e, g)
+    # larger images: more lane scratch and more demosaic tiles than the graph's
+    big = _Set(acc, [("bayer", 640, 960, RGGB, "f32"), ("xtrans", 600, 900, 9, "u16")])
+    print("eager batch that grows the buffers", flush=True)
+    b.run(big.images(), CHAIN)
+    b.synchronize()
+    with pytest.raises(LibRawInvalidArgument):
+        b.replay()
+    for it in small.items:
+        it["d_rgb"].upload(np.full((it["h"], it["w"], 3), np.nan, np.float32))
+    acc.synchronize()
+    b.capture(small.images(), CHAIN)
+    b.replay()
+    b.synchronize()
+    for e, g in zip(eager, small.results()):
+        assert np.array_equal(e, g)
+    b.close()
+
+
+def test_serve_rejects_bad_lists_and_isolates_requests(acc):
+    """Stage lists a request cannot run, and stage parameters the launchers
+    reject, fail at creation; an invalid request fails at submission; the
+    requests around it come out as they do alone."""
+    from libraw_enhanced_amd import develop as dv
+    from libraw_enhanced_amd.accelerator import LibRawInvalidArgument
+    for stages, kw in [(("axial_ca", "defringe"), {}), ((), {}),
+                       (CHAIN, {"axial_radius": 17}), (CHAIN, {"defringe_radius": float("nan")})]:
+        with pytest.raises(LibRawInvalidArgument):
+            dv.DevelopServer(0, lanes=1, hbm_budget_bytes=1 << 30, stages=stages,
+                             params=dv.ChainParams.defaults(**kw))
+    kind, h, w, filters, _ = _spec()[0]
+    raw, xt = _inputs(0, kind, h, w, filters, "u16")
+    raw = np.ascontiguousarray(raw)
+    im = dv.make_image(w, h, filters=filters, xtrans=xt, cam_mul=WB, black=BLACK, white=WHITE, wb=WB)
+    bad = dv.make_image(w, h, filters=filters, xtrans=xt, cam_mul=WB, black=WHITE, white=BLACK, wb=WB)
+    outs = [np.full((h, w, 3), np.nan, np.float32) for _ in range(3)]
+    server = dv.DevelopServer(0, lanes=2, hbm_budget_bytes=1 << 30)
+    server.submit(0, im, raw.ctypes.data, outs[0].ctypes.data, keep=(raw, outs[0]))
+    with pytest.raises(LibRawInvalidArgument):
+        server.submit(1, bad, raw.ctypes.data, outs[1].ctypes.data)
+    server.submit(2, im, raw.ctypes.data, outs[2].ctypes.data, keep=(raw, outs[2]))
+    done = {}
+    while len(done) < 2:
+        for c in server.poll(timeout_ms=1000):
+            done[c["id"]] = c["status"]
+    server.close()
+    assert done == {0: 0, 2: 0}
+    assert np.isnan(outs[1]).all()
+    assert np.array_equal(outs[0], outs[2]) and not np.isnan(outs[0]).any()
+
+
+def test_context_scratch_across_streams(acc):
+    """Device-resident calls of one context on two streams share its scratch:
+    the second call waits for the first, and both results equal the same calls
+    on one stream."""
+    from libraw_enhanced_amd.accelerator import native
+    hip = ct.CDLL("libamdhip64.so")
+    hip.hipStreamCreate.argtypes = [ct.POINTER(ct.c_void_p)]
+    hip.hipStreamDestroy.argtypes = [ct.c_void_p]
+    other = ct.c_void_p()
+    assert hip.hipStreamCreate(ct.byref(other)) == 0
+    lib, ctx = native(), acc.handle
+    h, w = 300, 420
+    img = np.ascontiguousarray(scene.render(h, w, scene.BASE_SEED + 7), np.float32)
+    d_a, d_b = acc.dev_from_host(img), acc.dev_from_host(img)
+    d_x, d_y = acc.dev_alloc(img.nbytes), acc.dev_alloc(img.nbytes)
+    args = (ct.c_float(10.0), ct.c_float(0.1), ct.c_float(0.15), ct.c_float(10.0), 0, ct.c_float(0.3))
+    # reference: both on the context stream
+    assert lib.lre_hip_dev_defringe(ctx, d_a.ptr, d_x.ptr, w, h, *args, None) == 0
+    assert lib.lre_hip_dev_axial_ca(ctx, d_b.ptr, d_y.ptr, w, h, 6, ct.c_float(1e-4), ct.c_float(0.3), None) == 0
+    acc.synchronize()
+    want_x, want_y = d_x.download((h, w, 3)), d_y.download((h, w, 3))
+    for d in (d_x, d_y):
+        d.upload(np.full((h, w, 3), np.nan, np.float32))
+    acc.synchronize()
+    # the same two calls on two streams, back to back
+    assert lib.lre_hip_dev_defringe(ctx, d_a.ptr, d_x.ptr, w, h, *args, other) == 0
+    assert lib.lre_hip_dev_axial_ca(ctx, d_b.ptr, d_y.ptr, w, h, 6, ct.c_float(1e-4), ct.c_float(0.3), None) == 0
+    acc.synchronize()
+    hip.hipStreamSynchronize.argtypes = [ct.c_void_p]
+    assert hip.hipStreamSynchronize(other) == 0
+    assert np.array_equal(d_x.download((h, w, 3)), want_x)
+    assert np.array_equal(d_y.download((h, w, 3)), want_y)
+    assert hip.hipStreamDestroy(other) == 0
