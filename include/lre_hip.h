@@ -164,6 +164,20 @@ int lre_hip_white_balance(lre_ctx *ctx, const uint16_t *img4, float *rgb, int wi
 int lre_hip_dev_amaze(lre_ctx *ctx, const float *d_cfa, float *d_rgb, int width, int height,
                       uint32_t filters, const float cam_mul[4], float maximum_value,
                       void *stream);
+/* No counterpart in the reference.  lre_hip_dev_amaze for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): same arguments, same output bits, and no
+ * allocation of any kind.  Its tile arenas and tile-queue counter live in a
+ * fixed device array of the library, 48 MiB per GPU (32 tile arenas), set up
+ * by lre_hip_create and held until the process ends.  With 32 arenas at most
+ * 32 workgroups run, so it is slower than lre_hip_dev_amaze on large images.
+ * The arena pool does not depend on the image: every size lre_hip_dev_amaze
+ * accepts is accepted (width * height below 2^31).  All calls on one GPU share
+ * that GPU's array, whatever the context or stream: calls that overlap in time
+ * on one GPU race and must be serialised by the caller. */
+int lre_hip_dev_amaze_noalloc(lre_ctx *ctx, const float *d_cfa, float *d_rgb, int width,
+                              int height, uint32_t filters, const float cam_mul[4],
+                              float maximum_value, void *stream);
 /* X-Trans 3-pass from a float CFA plane to interleaved RGB. */
 int lre_hip_dev_xtrans3(lre_ctx *ctx, const float *d_cfa, float *d_rgb, int width, int height,
                         const signed char xtrans[36], float maximum_value, void *stream);

@@ -88,6 +88,8 @@ _SIGS = {
                                ctypes.c_uint32, _i8p], ctypes.c_int),
     "lre_hip_dev_amaze": ([_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, _f32p,
                            ctypes.c_float, _vp], ctypes.c_int),
+    "lre_hip_dev_amaze_noalloc": ([_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, _f32p,
+                                   ctypes.c_float, _vp], ctypes.c_int),
     "lre_hip_dev_tone_mapping": ([_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, _vp],
                                  ctypes.c_int),
     "lre_hip_dev_detail_tonemap": ([_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp], ctypes.c_int),

@@ -949,7 +949,7 @@ static int amaze_slots(lre_ctx *ctx, int w, int h) {
 }
 
 // the slots' arenas, one spare arena of padding behind the last, the queue counter
-static size_t arenas_bytes(int slots) {
+static constexpr size_t arenas_bytes(int slots) {
     return (size_t)(slots + 1) * ARENA_FLOATS * sizeof(float) + 256;
 }
 
@@ -989,6 +989,41 @@ int lre_launch_amaze(lre_ctx *ctx, const float *d_cfa, float *d_rgb, int w, int 
     A.arenas = static_cast<float *>(d_scratch);
     const int slots = amaze_slots(ctx, w, h);
     LRE_HIP_CHECK(amaze_lds_attr());
+    hipLaunchKernelGGL(queue_reset, dim3(1), dim3(64), 0, s, A.arenas, slots);
+    hipLaunchKernelGGL(amaze_kernel, dim3(slots), dim3(BLOCK), LDS_BYTES, s, A);
+    LRE_HIP_CHECK(hipGetLastError());
+    return lre_launch_border_interpolate(nullptr, d_rgb, w, h, filters, 4, s);
+}
+
+// ---- the variant that allocates nothing -------------------------------------
+// Its tile arenas and queue counter live in this fixed device array (one copy
+// per GPU, held until the process ends): LRE_AMAZE_NOALLOC_SLOTS arenas, the
+// spare arena behind them and the counter, laid out as arenas_bytes() lays out
+// an allocated scratch.  Every call on a GPU shares that GPU's copy.
+static_assert(arenas_bytes(LRE_AMAZE_NOALLOC_SLOTS) <= LRE_AMAZE_NOALLOC_BYTES, "no-alloc arena size");
+static __device__ __align__(256) float scratch[LRE_AMAZE_NOALLOC_BYTES / sizeof(float)];
+
+int lre_amaze_noalloc_setup(void **d_arenas) {
+    // the device copy of `scratch` (the host-side name is not a device address)
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        *d_arenas = nullptr;
+        lre_set_error("device address of the no-alloc AMaZE arenas: %s", hipGetErrorString(e));
+        return LRE_ERR_HIP;
+    }
+    *d_arenas = ptr;
+    return LRE_OK;
+}
+
+int lre_launch_amaze_noalloc(const float *d_cfa, float *d_rgb, int w, int h, uint32_t filters,
+                             const float cam_mul[4], float maximum_value, void *d_arenas, hipStream_t s) {
+    Args A;
+    amaze_args(A, d_cfa, d_rgb, w, h, filters, cam_mul, maximum_value, nullptr);
+    A.arenas = static_cast<float *>(d_arenas);
+    const int slots = A.ntiles < LRE_AMAZE_NOALLOC_SLOTS ? (A.ntiles > 0 ? A.ntiles : 1) : LRE_AMAZE_NOALLOC_SLOTS;
+    LRE_HIP_CHECK(amaze_lds_attr());
+    // the counter is cleared by a kernel (no memset node if the caller captures)
     hipLaunchKernelGGL(queue_reset, dim3(1), dim3(64), 0, s, A.arenas, slots);
     hipLaunchKernelGGL(amaze_kernel, dim3(slots), dim3(BLOCK), LDS_BYTES, s, A);
     LRE_HIP_CHECK(hipGetLastError());

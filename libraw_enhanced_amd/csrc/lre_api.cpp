@@ -5,6 +5,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <limits>
 #include <string>
 
@@ -83,6 +84,18 @@ int rgb_stage_host(lre_ctx *ctx, const float *in_rgb, float *out_rgb, int w, int
     return copy_out(ctx, out_rgb, d_out, n * 12);
 }
 
+// This device's address of the no-alloc AMaZE arenas, looked up once per
+// device (the caller has made that device current).
+void *noalloc_arenas_for(int device) {
+    constexpr int MAX_DEVICES = 64;
+    static std::mutex mu;
+    static void *addr[MAX_DEVICES] = {};
+    if (device < 0 || device >= MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!addr[device] && lre_amaze_noalloc_setup(&addr[device]) != LRE_OK) addr[device] = nullptr;
+    return addr[device];
+}
+
 }  // namespace
 
 void lre_set_error(const char *fmt, ...) {
@@ -157,6 +170,8 @@ int lre_hip_create(lre_ctx **out, int device) {
         lre_set_error("hipStreamCreate failed: %s", hipGetErrorString(e));
         return LRE_ERR_HIP;
     }
+    // set-up of the no-alloc entry point; the context works without it
+    ctx->noalloc_arenas = noalloc_arenas_for(device);
     *out = ctx;
     return LRE_OK;
 }
@@ -228,6 +243,22 @@ int lre_hip_dev_amaze(lre_ctx *ctx, const float *d_cfa, float *d_rgb, int w, int
     lre_scratch_guard scratch_use{ctx, lre_pick_stream(ctx, stream)};
     return lre_launch_amaze(ctx, d_cfa, d_rgb, w, h, filters, cam_mul, maximum_value,
                             ctx->scratch, lre_pick_stream(ctx, stream));
+}
+
+int lre_hip_dev_amaze_noalloc(lre_ctx *ctx, const float *d_cfa, float *d_rgb, int w, int h,
+                              uint32_t filters, const float cam_mul[4], float maximum_value,
+                              void *stream) {
+    if (!ctx || !d_cfa || !d_rgb || !cam_mul) return set_arg_error("null pointer");
+    if (!valid_dims(w, h)) return set_arg_error("image size");
+    if (!(maximum_value > 0.f)) return set_arg_error("maximum_value must be > 0");
+    if (filters == 9) return set_arg_error("AMaZE needs a Bayer pattern");
+    if (!ctx->noalloc_arenas) {
+        lre_set_error("the no-alloc AMaZE arenas are not available on device %d", ctx->device);
+        return LRE_ERR_HIP;
+    }
+    LRE_HIP_CHECK(hipSetDevice(ctx->device));
+    return lre_launch_amaze_noalloc(d_cfa, d_rgb, w, h, filters, cam_mul, maximum_value,
+                                    ctx->noalloc_arenas, lre_pick_stream(ctx, stream));
 }
 
 int lre_hip_dev_xtrans3(lre_ctx *ctx, const float *d_cfa, float *d_rgb, int w, int h,

@@ -20,6 +20,8 @@ struct lre_ctx {
     // there after its work (scratch_pending: the event is set and not yet waited
     // for).  A call on another stream first waits for that event.
     hipStream_t scratch_stream = nullptr;
+    // this device's address of the no-alloc AMaZE arenas (null if unavailable)
+    void *noalloc_arenas = nullptr;
     hipEvent_t scratch_done = nullptr;
     bool scratch_pending = false;
     // Pinned staging for the host-pointer entry points.
@@ -77,6 +79,14 @@ int lre_check_defringe_params(float radius);
 int lre_launch_amaze(lre_ctx *ctx, const float *d_cfa, float *d_rgb, int w, int h,
                      uint32_t filters, const float cam_mul[4], float maximum_value,
                      void *d_scratch, hipStream_t s, const float *d_maxv = nullptr);
+// AMaZE without allocation: a fixed pool of LRE_AMAZE_NOALLOC_SLOTS tile
+// arenas in a device array of the library.  _setup gives the current device's
+// address of that array (once per device, that device current).
+constexpr int LRE_AMAZE_NOALLOC_SLOTS = 32;
+constexpr size_t LRE_AMAZE_NOALLOC_BYTES = (size_t)48 << 20;
+int lre_amaze_noalloc_setup(void **d_arenas);
+int lre_launch_amaze_noalloc(const float *d_cfa, float *d_rgb, int w, int h, uint32_t filters,
+                             const float cam_mul[4], float maximum_value, void *d_arenas, hipStream_t s);
 // Multi-image tile queues.  *_fill writes the per-image table on the host and
 // returns the total tile count (<0 on error); the caller copies it to d_table
 // before the launch.  Table and arena sizes come from the *_bytes helpers.
