@@ -73,11 +73,39 @@ class HipEvents:
         return float(ms.value)
 
 
-def cpu_baseline(raw3, stages, threads):
+def host_cpu_info():
+    """Model name of the host CPU and the logical CPUs this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "logical_cpus_in_affinity": len(os.sched_getaffinity(0)),
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
+
+
+def cpu_threads(args):
+    """Threads of the CPU leg: --cpu-threads, else OMP_NUM_THREADS (the CPU share
+    the host allots this job), else every CPU in the affinity mask."""
+    if args.cpu_threads:
+        return args.cpu_threads
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return int(env) if env.isdigit() and int(env) > 0 else len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(raw3, stages, threads, runs=5):
     """The reference's CPU code (oracle/_ref, built from its sources) on the same
-    image and stage list; falls back to the CPU restatement if absent."""
+    image and stage list, called as the develop pipeline calls it (defringe and
+    the tone map in place); the CPU restatement if the build is absent.  One
+    warm-up pass, then the median of `runs` timed passes.  Returns the
+    cpu_baseline object and the output of the last pass."""
     f32p = ctypes.POINTER(ctypes.c_float)
     h, w, _ = raw3.shape
+    host = host_cpu_info()
     ref_path = os.path.join(ROOT, "oracle", "_ref", "libref_cpu.so")
     kind = "reference" if os.path.exists(ref_path) else "port"
     lib = ctypes.CDLL(ref_path if kind == "reference" else os.path.join(ROOT, "oracle", "liblre_oracle.so"))
@@ -85,29 +113,62 @@ def cpu_baseline(raw3, stages, threads):
     if kind == "reference":
         lib.ref_set_threads(threads)
     rgb = np.zeros_like(raw3)
-    tmp = np.zeros_like(raw3)
     cm = np.ones(4, np.float32)
-    t0 = time.perf_counter()
-    for s in stages:
-        if s == "amaze":
-            if kind == "reference":
-                lib.ref_amaze(raw3.ctypes.data_as(f32p), rgb.ctypes.data_as(f32p), w, h,
-                              ctypes.c_uint32(0x94949494), cm.ctypes.data_as(f32p), ctypes.c_float(1.0))
-            else:
-                lib.oracle_amaze(raw3.ctypes.data_as(f32p), rgb.ctypes.data_as(f32p), w, h,
-                                 ctypes.c_uint32(0x94949494), cm.ctypes.data_as(f32p),
-                                 ctypes.c_float(1.0), 0)
-        elif s == "defringe":
-            fn = lib.ref_defringe if kind == "reference" else lib.oracle_defringe
-            fn(rgb.ctypes.data_as(f32p), tmp.ctypes.data_as(f32p), w, h, ctypes.c_float(10.0),
-               ctypes.c_float(0.1), ctypes.c_float(0.15), ctypes.c_float(10.0), 0, ctypes.c_float(0.3))
-            rgb, tmp = tmp, rgb
-        elif s == "detail_tm":
-            fn = lib.ref_detail_tonemap if kind == "reference" else lib.oracle_detail_tonemap
-            fn(rgb.ctypes.data_as(f32p), w, h)
-    dt = time.perf_counter() - t0
-    return {"value": round(h * w / 1e6 / dt, 3), "unit": "MP/s", "cores": threads, "kind": kind,
-            "sample": f"one {w}x{h} RGGB image through {' -> '.join(stages)} ({dt:.2f} s)"}
+    P = lambda a: a.ctypes.data_as(f32p)
+
+    def one_pass():
+        for s in stages:
+            if s == "amaze":
+                if kind == "reference":
+                    lib.ref_amaze(P(raw3), P(rgb), w, h, ctypes.c_uint32(0x94949494), P(cm), ctypes.c_float(1.0))
+                else:
+                    lib.oracle_amaze(P(raw3), P(rgb), w, h, ctypes.c_uint32(0x94949494), P(cm),
+                                     ctypes.c_float(1.0), 0)
+            elif s == "defringe":
+                # in place, as postprocess calls it (core/libraw_wrapper.cpp:1810)
+                fn = lib.ref_defringe if kind == "reference" else lib.oracle_defringe
+                fn(P(rgb), P(rgb), w, h, ctypes.c_float(10.0), ctypes.c_float(0.1), ctypes.c_float(0.15),
+                   ctypes.c_float(10.0), 0, ctypes.c_float(0.3))
+            elif s == "detail_tm":
+                fn = lib.ref_detail_tonemap if kind == "reference" else lib.oracle_detail_tonemap
+                fn(P(rgb), w, h)
+
+    one_pass()  # warm-up
+    times = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        one_pass()
+        times.append(time.perf_counter() - t0)
+    dt = float(np.median(times))
+    line = {"value": round(h * w / 1e6 / dt, 3), "unit": "MP/s", "cores": threads, "kind": kind,
+            "sample": f"one {w}x{h} RGGB image through {' -> '.join(stages)}: 1 warm-up pass, median of "
+                      f"{runs} passes ({dt:.2f} s each, {threads} OpenMP threads)"}
+    line.update(host)
+    return line, rgb
+
+
+def edge_band_mask(h, w, band):
+    """True within `band` px of the frame."""
+    m = np.zeros((h, w), bool)
+    m[:band, :] = m[-band:, :] = True
+    m[:, :band] = m[:, -band:] = True
+    return m
+
+
+def parity_block(got, want, band):
+    """max-abs, PSNR and differing pixels of got against want, over the whole
+    image and split by the band of edge tiles where the reference's output
+    depends on its thread schedule."""
+    d = np.abs(got.astype(np.float64) - want.astype(np.float64))
+    px_diff = np.any(got != want, axis=2)
+    edge = edge_band_mask(*got.shape[:2], band)
+    out = {}
+    for name, sel in (("all", np.ones_like(edge)), ("inside_edge_band", edge), ("outside_edge_band", ~edge)):
+        dd = d[sel]
+        mse = float(np.mean(dd * dd))
+        out[name] = {"max_abs": float(dd.max()), "psnr_db": (None if mse == 0 else round(10 * np.log10(1.0 / mse), 2)),
+                     "px_differ": int(np.count_nonzero(px_diff[sel])), "px": int(sel.sum())}
+    return out
 
 
 def load_traffic(stage):
@@ -206,11 +267,20 @@ def main():
                      "traffic": load_traffic(dominant),
                      "alg_bytes_per_launch": alg_bytes},
     }
+    gpu_out = chain.result()  # the last timed step's output
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+        threads = cpu_threads(args)
         print(f"bench: CPU baseline on {threads} threads", file=sys.stderr, flush=True)
         raw3 = scene.cfa_to_rgb3(cfa)
-        line["cpu_baseline"] = cpu_baseline(raw3, stages, threads)
+        line["cpu_baseline"], cpu_out = cpu_baseline(raw3, stages, threads)
+        # Parity at full size against that output.  The reference's AMaZE
+        # output within its edge tiles depends on its thread schedule; the band
+        # is one tile (160 px) widened by the later stages' reach (defringe
+        # 10 + 10 + 14 + 10, detail tone map 12 + 12).
+        band = 160 + 44 + 24
+        line["parity"] = {"against": f"{line['cpu_baseline']['kind']} CPU output of the same image",
+                          "edge_band_px": band, "tolerance": {"max_abs": 2e-4, "psnr_db": 70.0},
+                          **parity_block(gpu_out, cpu_out, band)}
     else:
         line["cpu_baseline"] = None
     acc.close()
@@ -225,7 +295,7 @@ def _log(msg):
 
 
 def _threads(args):
-    return args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+    return cpu_threads(args)
 
 
 def _ref_lib(threads):
