@@ -105,6 +105,17 @@ struct Gates {
 LRE_DEV float mx(float a, float b) { return lre::max_std(a, b); }
 LRE_DEV float mn(float a, float b) { return lre::min_std(a, b); }
 LRE_DEV float expf_cr(float x) { return (float)exp((double)x); }
+
+// Exact shortcuts.  Where a pixel's edge gate is 0 the reference multiplies
+// it into a product of factors that are all finite numbers in [0, 1] as long
+// as the pixel's inputs are of moderate size, so the product is +0 (support
+// and solo seeds), or the pixel is left as it is (correction).  A wavefront
+// skips the rest of the work only when every one of its pixels qualifies;
+// otherwise it computes every pixel in full, so the results are the same bits.
+// moderate: |x| <= 1e30 (NaN fails), so differences and quotients by the
+// reference's floors (1e-4, 0.05) stay finite.
+LRE_DEV bool moderate(float x) { return fabsf(x) <= 1e30f; }
+LRE_DEV bool wave_all(bool p) { return __all(p); }
 LRE_DEV float logf_cr(float x) { return (float)log((double)x); }
 
 // block-wide max of non-negative floats folded into *dst
@@ -148,11 +159,16 @@ struct Planes {
 template <bool FAST>
 LRE_DEV float support_px(const Planes &P, const Scal &C, const Gates &G, size_t i, float ber) {
     const auto gate = [&](Gate k, float x) { return sstep<FAST>(G.g[k], x); };
-    const float rg = P.lrg[i], bg = P.lbg[i];
-    const float dr = rg - P.brg[i], db = bg - P.bbg[i];
-    const float r0 = mx(P.src[3 * i], 0.f), g0 = mx(P.src[3 * i + 1], 0.f), b0 = mx(P.src[3 * i + 2], 0.f);
+    const float rg = P.lrg[i], bg = P.lbg[i], brg = P.brg[i], bbg = P.bbg[i], bgd = P.bg[i];
+    const float s0 = P.src[3 * i], s1 = P.src[3 * i + 1], s2 = P.src[3 * i + 2];
+    const float c_edge = gate(G_EDGE, P.be[i]) * gate(G_EDGE_RATIO, ber);
+    if (wave_all(c_edge == 0.f && moderate(rg) && moderate(bg) && moderate(brg) && moderate(bbg) &&
+                 moderate(bgd) && moderate(s0) && moderate(s1) && moderate(s2)))
+        return 0.f;
+    const float dr = rg - brg, db = bg - bbg;
+    const float r0 = mx(s0, 0.f), g0 = mx(s1, 0.f), b0 = mx(s2, 0.f);
     const float purple_excess = mn(mx(0.f, dr), mx(0.f, db));
-    const float denom = mx(P.bg[i], 0.05f);
+    const float denom = mx(bgd, 0.05f);
     const float g_over_min = mx(0.f, g0 - mn(r0, b0)) / denom;
     const float green_dom = sstepc(0.010f, 0.060f, g_over_min);
     const float green_excess = mx(mx(0.f, -dr), mx(0.f, -db)) * green_dom;
@@ -161,9 +177,7 @@ LRE_DEV float support_px(const Planes &P, const Scal &C, const Gates &G, size_t 
     const float min_dir = is_purple ? mx(0.f, mn(rg, bg)) : mx(0.f, mn(-rg, -bg));
     const float max_dir = is_purple ? mx(1e-4f, mx(rg, bg)) : mx(1e-4f, mx(-rg, -bg));
     const float balance = min_dir / max_dir;
-    const float c_edge = gate(G_EDGE, P.be[i]) *
-                         gate(G_EDGE_RATIO, ber);
-    const float c_bright = gate(G_BRIGHT, P.bg[i]);
+    const float c_bright = gate(G_BRIGHT, bgd);
     const float c_chroma = gate(G_CHROMA, excess);
     const float red_strength = mx(0.f, r0 - mx(g0, b0)) / mx(r0, 1e-4f);
     const float rb_balance = b0 / mx(r0, 1e-4f);
@@ -338,6 +352,7 @@ __global__ __launch_bounds__(FB_BLOCK) void blur2d(BlurSet S, Taps taps, int W, 
     for (int p = 0; p < S.count; p++) {
         const float *in = S.in[p];
         const bool scaled = S.scale_by_edge[p] != 0;
+        bool zero = true;
         __syncthreads();
         lre::stage2d<FB_BLOCK>(
             IH, IW,
@@ -347,8 +362,22 @@ __global__ __launch_bounds__(FB_BLOCK) void blur2d(BlurSet S, Taps taps, int W, 
                 xx = xx < 0 ? 0 : (xx > W - 1 ? W - 1 : xx);
                 return in[(size_t)yy * W + xx];
             },
-            [&](int r, int c, float v) { win[r * PW + c] = scaled ? v * inv_edge : v; });
-        __syncthreads();
+            [&](int r, int c, float v) {
+                const float x = scaled ? v * inv_edge : v;
+                win[r * PW + c] = x;
+                zero = zero && x == 0.f;
+            });
+        // A window of zeros blurs to +0 everywhere (taps times zeros, summed
+        // from +0): the sparse seed planes have many such tiles.  The support
+        // seeds of the last plane need the pass itself.
+        if (__syncthreads_and(zero) && !(SUP && p == S.count - 1)) {
+            float *out = S.out[p];
+            for (int t = threadIdx.x; t < FB_TY * FB_TX; t += FB_BLOCK) {
+                const int x = x0 + t % FB_TX, y = y0 + t / FB_TX;
+                if (x < W && y < H) out[(size_t)y * W + x] = 0.f;
+            }
+            continue;
+        }
         for (int t = threadIdx.x; t < IH * (FB_TX / FB_RB); t += FB_BLOCK) {
             const int r = t % IH, c = (t / IH) * FB_RB;
             const float *row = win + r * PW + c;
@@ -424,15 +453,23 @@ template <bool FAST>
 LRE_DEV void solo_px(const Planes &P, const Scal &C, const Gates &G, size_t i, float lfs) {
     const auto gate = [&](Gate k, float x) { return sstep<FAST>(G.g[k], x); };
     const float s0 = P.src[3 * i], s1 = P.src[3 * i + 1], s2 = P.src[3 * i + 2];
+    const float bgd = P.bg[i], bfs = P.bfs[i];
+    const float c_edge = gate(G_EDGE, P.be[i]) * gate(G_EDGE_RATIO, P.ber[i]);
+    // solo_r and solo_b are finite whatever the exponentials give (mx(0, x)
+    // maps -inf and NaN to 0), so with the edge gate at 0 both seeds are +0
+    if (wave_all(c_edge == 0.f && moderate(s0) && moderate(s1) && moderate(s2) && moderate(bgd) &&
+                 moderate(bfs) && moderate(lfs))) {
+        P.sr[i] = 0.f;
+        P.sb[i] = 0.f;
+        return;
+    }
     const float g = mx(s1, 0.f);
     const float tr = g * expf_cr(P.brg[i]);
     const float tb = g * expf_cr(P.bbg[i]);
-    const float solo_r = mx(0.f, s0 - tr) / mx(P.bg[i], 0.05f);
-    const float solo_b = mx(0.f, s2 - tb) / mx(P.bg[i], 0.05f);
-    const float linked = sstepc(0.004f, 0.045f, mx(P.bfs[i], lfs * 0.90f));
-    const float c_edge = gate(G_EDGE, P.be[i]) *
-                         gate(G_EDGE_RATIO, P.ber[i]);
-    const float c_bright = gate(G_BRIGHT, P.bg[i]);
+    const float solo_r = mx(0.f, s0 - tr) / mx(bgd, 0.05f);
+    const float solo_b = mx(0.f, s2 - tb) / mx(bgd, 0.05f);
+    const float linked = sstepc(0.004f, 0.045f, mx(bfs, lfs * 0.90f));
+    const float c_bright = gate(G_BRIGHT, bgd);
     const float r0 = mx(s0, 0.f), g0 = mx(s1, 0.f), b0 = mx(s2, 0.f);
     const float red_strength = mx(0.f, r0 - mx(g0, b0)) / mx(r0, 1e-4f);
     const float red_guard = 1.f - 0.90f * sstepc(0.12f, 0.32f, red_strength);
@@ -452,6 +489,7 @@ __global__ __launch_bounds__(BLOCK) void line_link(Planes P, Scal C, Gates G) {
     const int W = C.W, H = C.H;
     __shared__ float s_ls[LL_R * LL_R], s_fs[LL_R * LL_R];
     const int x0 = blockIdx.x * LL_T, y0 = blockIdx.y * LL_T;
+    bool flat = true;
     lre::stage2d<BLOCK>(
         LL_R, LL_R,
         [&](int r, int c) {
@@ -462,10 +500,15 @@ __global__ __launch_bounds__(BLOCK) void line_link(Planes P, Scal C, Gates G) {
             return make_float2(P.fs[i], P.bfs[i]);
         },
         [&](int r, int c, float2 v) {
+            const float ls = mx(v.x, v.y * 0.55f);
             s_fs[r * LL_R + c] = v.x;
-            s_ls[r * LL_R + c] = mx(v.x, v.y * 0.55f);
+            s_ls[r * LL_R + c] = ls;
+            flat = flat && ls == 0.f;
         });
-    __syncthreads();
+    // With every staged ls at 0 each direction's `along` is +0, so along * thin
+    // is +0 or NaN (0 / 0 in the smoothstep), which mx(best, .) ignores: every
+    // output of the tile is +0 (:4296-4323)
+    flat = __syncthreads_and(flat);
     // direction d: (tx, ty) = (1,0) (0,1) (1,1) (1,-1) (2,1) (1,2) (2,-1) (1,-2)
     for (int o = threadIdx.x; o < LL_T * LL_T; o += BLOCK) {
         const int ly = o / LL_T, lx = o % LL_T;
@@ -473,6 +516,7 @@ __global__ __launch_bounds__(BLOCK) void line_link(Planes P, Scal C, Gates G) {
         if (x >= W || y >= H) continue;
         const int c = (ly + LL_HALO) * LL_R + (lx + LL_HALO);
         float best = 0.f;
+        if (!flat)
 #pragma unroll
         for (int d = 0; d < 8; d++) {
             const int tx = d == 1 ? 0 : ((d == 4 || d == 6) ? 2 : 1);
@@ -507,7 +551,24 @@ __global__ __launch_bounds__(BLOCK) void correct(Planes P, Scal C, Gates G, cons
     for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK) {
         const float s0 = P.src[3 * i], s1 = P.src[3 * i + 1], s2 = P.src[3 * i + 2];
         const float bguide = P.bg[i], brg = P.brg[i], bbg = P.bbg[i], bfs = P.bfs[i], lfs = P.lfs[i];
-        const float rg = P.lrg[i], bg = P.lbg[i];
+        const float rg = P.lrg[i], bg = P.lbg[i], be = P.be[i], ber = P.ber[i];
+        const float c_edge_core = gate(G_EDGE, be) * gate(G_EDGE_RATIO, ber);
+        const float c_edge_low = gate(G_EDGE_LOW, be) * gate(G_RATIO_LOW, ber);
+        // With both edge gates at 0 (and the green correction off) both masks
+        // are +0, so the main correction does not run; the solo masks are
+        // products of srb / sbb with two factors in [0, 1], so below the cutoff
+        // when those are.  The pixel is then returned as it came in.
+        if (wave_all(!eg && c_edge_core == 0.f && c_edge_low == 0.f &&
+                     (!(C.solo_enable > 0.f) || (P.srb[i] <= mask_cutoff && P.sbb[i] <= mask_cutoff)) &&
+                     moderate(s0) && moderate(s1) && moderate(s2) && moderate(bguide) && moderate(brg) &&
+                     moderate(bbg) && moderate(bfs) && moderate(lfs) && moderate(rg) && moderate(bg))) {
+            if (P.dst != P.src) {
+                P.dst[3 * i] = s0;
+                P.dst[3 * i + 1] = s1;
+                P.dst[3 * i + 2] = s2;
+            }
+            continue;
+        }
         const float dr = rg - brg, db = bg - bbg;
         const float r0 = mx(s0, 0.f), g0 = mx(s1, 0.f), b0 = mx(s2, 0.f);
         const float rb_avg = 0.5f * (r0 + b0);
@@ -521,10 +582,6 @@ __global__ __launch_bounds__(BLOCK) void correct(Planes P, Scal C, Gates G, cons
         const float min_dir = is_purple ? mx(0.f, mn(rg, bg)) : mx(0.f, mn(-rg, -bg));
         const float max_dir = is_purple ? mx(1e-4f, mx(rg, bg)) : mx(1e-4f, mx(-rg, -bg));
         const float balance = min_dir / max_dir;
-        const float c_edge_core = gate(G_EDGE, P.be[i]) *
-                                  gate(G_EDGE_RATIO, P.ber[i]);
-        const float c_edge_low = gate(G_EDGE_LOW, P.be[i]) *
-                                 gate(G_RATIO_LOW, P.ber[i]);
         const float linked = sstepc(0.004f, 0.045f, mx(bfs, lfs * 0.90f));
         const float c_bright = gate(G_BRIGHT, bguide);
         const float c_chroma = gate(G_CHROMA, excess);
