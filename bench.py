@@ -171,6 +171,30 @@ def parity_block(got, want, band):
     return out
 
 
+def valu_roofline(kernel, stage, ms, px):
+    """The VALU issue roofline of a demosaic kernel: its VALU instructions per
+    launch (rocprofv3 SQ_INSTS_VALU on the same workload, profiles/
+    valu_demosaic.json) over the launch time measured here, against one wave64
+    instruction per CU per clock; plus the stage's FP32 operations per pixel
+    (the oracle's op count, profiles/opcount_demosaic.json) as a rate."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "valu_demosaic.json")) as f:
+            v = json.load(f)
+        with open(os.path.join(ROOT, "profiles", "opcount_demosaic.json")) as f:
+            o = json.load(f)["stages"][stage]
+    except (OSError, KeyError, ValueError):
+        return None
+    insts = v["kernels"][kernel]["SQ_INSTS_VALU"]
+    peak = v["peak_wave_insts_per_s"]
+    rate = insts / (ms * 1e-3)
+    ops = o["fp32_arith_per_px"] * px
+    return {"kernel": kernel, "bound": "valu", "achieved": round(rate / 1e9, 1), "peak": round(peak / 1e9, 1),
+            "unit": "G wave64 VALU instructions/s", "frac": round(rate / peak, 4),
+            "insts_per_launch": int(insts), "lane_insts_per_px": round(insts * 64 / px, 1),
+            "fp32_ops_per_px": o["fp32_arith_per_px"], "fp32_tops": round(ops / (ms * 1e-3) / 1e12, 2),
+            "fp32_frac_of_lane_issue": round(ops / (ms * 1e-3) / (peak * 64), 4)}
+
+
 def load_traffic(stage):
     path = os.path.join(ROOT, "profiles", f"traffic_{stage}.json")
     if os.path.exists(path):
@@ -266,6 +290,8 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": load_traffic(dominant),
                      "alg_bytes_per_launch": alg_bytes},
+        # AMaZE does not stream: its time goes to VALU issue
+        "valu_roofline": valu_roofline("amaze_kernel", "amaze", stage_ms["amaze"], h * w),
     }
     gpu_out = chain.result()  # the last timed step's output
     if not args.no_cpu:
@@ -388,7 +414,8 @@ def bench_xtrans26(args):
             "scaling": "weak", "vs_baseline": None, "dtype": "f32",
             "data": "synthetic (seeded scene, radial lateral CA pre-warp, Poisson-Gaussian noise)",
             "config": {"workload": f"{' -> '.join(stages)} on one {w}x{h} X-Trans float32 mosaic",
-                       "image_mp": round(mp, 2), "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()}}}
+                       "image_mp": round(mp, 2), "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()}},
+            "valu_roofline": valu_roofline("xtrans_kernel", "xtrans3", stage_ms["xtrans3"], h * w)}
     if not args.no_cpu:
         threads = _threads(args)
         _log(f"CPU baseline on {threads} threads")
