@@ -147,6 +147,25 @@ def cpu_baseline(raw3, stages, threads, runs=5):
     return line, rgb
 
 
+def fp64_post_chain(rgb, stages, threads):
+    """An FP64 evaluation of the post-demosaic stages (the restatement compiled
+    with double in place of float, oracle/fp64.h) from a demosaic output; the
+    checker for the parity block, not a timed leg."""
+    lib = ctypes.CDLL(os.path.join(ROOT, "oracle", "liblre_oracle.so"))
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    f64p = ctypes.POINTER(ctypes.c_double)
+    D = ctypes.c_double
+    a = np.ascontiguousarray(rgb, np.float64).copy()
+    h, w, _ = a.shape
+    P = a.ctypes.data_as(f64p)
+    for st in stages:
+        if st == "defringe":
+            assert lib.oracle64_defringe(P, P, w, h, D(10.0), D(0.1), D(0.15), D(10.0), 0, D(0.3)) == 0
+        elif st == "detail_tm":
+            assert lib.oracle64_detail_tonemap(P, w, h) == 0
+    return a
+
+
 def edge_band_mask(h, w, band):
     """True within `band` px of the frame."""
     m = np.zeros((h, w), bool)
@@ -307,6 +326,20 @@ def main():
         line["parity"] = {"against": f"{line['cpu_baseline']['kind']} CPU output of the same image",
                           "edge_band_px": band, "tolerance": {"max_abs": 2e-4, "psnr_db": 70.0},
                           **parity_block(gpu_out, cpu_out, band)}
+        # error of both outputs against an FP64 evaluation of defringe -> detail
+        # TM from the HIP AMaZE output (bit-identical to the restated AMaZE)
+        chain.run_stage("amaze", d_cfa)
+        acc.synchronize()
+        print("bench: FP64 evaluation of the post-demosaic stages", file=sys.stderr, flush=True)
+        f64 = fp64_post_chain(chain.result(), stages[1:], threads)
+        edge = edge_band_mask(h, w, band)
+        def err(x):
+            d = np.abs(x.astype(np.float64) - f64)
+            return {"max_abs": float(d.max()), "max_abs_outside_edge_band": float(d[~edge].max())}
+        line["parity"]["vs_fp64"] = {"against": "FP64 evaluation of " + " -> ".join(stages[1:]) +
+                                                " from the HIP AMaZE output", "hip": err(gpu_out),
+                                     line["cpu_baseline"]["kind"]: err(cpu_out)}
+        del f64
     else:
         line["cpu_baseline"] = None
     acc.close()

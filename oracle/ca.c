@@ -49,7 +49,7 @@ static inline double box_mean(const double *sat, int W, int H, int x, int y, int
 static void guided(const float *I, const float *p, float *out, int W, int H, int r, float eps,
                    const double *satI, const double *satII) { /* :3725-3777 */
     const size_t N = (size_t)W * H;
-    float *Ip = malloc(N * 4), *ap = malloc(N * 4), *bp = malloc(N * 4);
+    float *Ip = malloc(N * sizeof(float)), *ap = malloc(N * sizeof(float)), *bp = malloc(N * sizeof(float));
     for (size_t i = 0; i < N; i++) Ip[i] = I[i] * p[i];
     double *satp = build_sat(p, W, H), *satIp = build_sat(Ip, W, H);
 #pragma omp parallel for schedule(static)
@@ -91,8 +91,8 @@ int oracle_axial_ca(const float *src, float *dst, int W, int H, int radius, floa
     }
     const float norm_ref = maxf_(max_g, 1e-6f);
     const float inv_norm = 1.f / norm_ref;
-    float *R = malloc(N * 4), *G = malloc(N * 4), *B = malloc(N * 4), *II = malloc(N * 4);
-    float *Rf = malloc(N * 4), *Bf = malloc(N * 4), *eraw = calloc(N, 4), *ew = malloc(N * 4);
+    float *R = malloc(N * sizeof(float)), *G = malloc(N * sizeof(float)), *B = malloc(N * sizeof(float)), *II = malloc(N * sizeof(float));
+    float *Rf = malloc(N * sizeof(float)), *Bf = malloc(N * sizeof(float)), *eraw = calloc(N, sizeof(float)), *ew = malloc(N * sizeof(float));
     for (size_t i = 0; i < N; i++) {
         R[i] = src[3 * i] * inv_norm;
         G[i] = src[3 * i + 1] * inv_norm;
@@ -150,7 +150,7 @@ static float bilinear(const float *plane, int W, int H, float x, float y) { /* :
 static float *downsample(const float *in, int W, int H, int *W2, int *H2) { /* :3369-3384 */
     *W2 = W / 2;
     *H2 = H / 2;
-    float *out = calloc((size_t)*W2 * *H2 + 1, 4);
+    float *out = calloc((size_t)*W2 * *H2 + 1, sizeof(float));
     for (int y = 0; y < *H2; y++) {
         const int y0 = 2 * y, y1 = mini_(y0 + 1, H - 1);
         for (int x = 0; x < *W2; x++) {
@@ -214,7 +214,7 @@ static void lk_level(const float *G, const float *T, int W, int H, int mw, int m
 }
 
 static void smooth3(float *m, int mw, int mh) { /* :3474-3492 */
-    float *o = calloc((size_t)mw * mh, 4);
+    float *o = calloc((size_t)mw * mh, sizeof(float));
     for (int cy = 0; cy < mh; cy++)
         for (int cx = 0; cx < mw; cx++) {
             float acc = 0.f;
@@ -228,7 +228,7 @@ static void smooth3(float *m, int mw, int mh) { /* :3474-3492 */
                 }
             o[cy * mw + cx] = n > 0 ? acc / n : 0.f;
         }
-    memcpy(m, o, (size_t)mw * mh * 4);
+    memcpy(m, o, (size_t)mw * mh * sizeof(float));
     free(o);
 }
 
@@ -270,7 +270,7 @@ int oracle_lateral_ca(const float *src, float *dst, int W0, int H0, int cell_siz
     while (L > 1 && ((W0 >> (L - 1)) < 32 || (H0 >> (L - 1)) < 32)) L--;
     float *Rl[16], *Gl[16], *Bl[16];
     int Wl[16], Hl[16];
-    Rl[0] = malloc(N0 * 4); Gl[0] = malloc(N0 * 4); Bl[0] = malloc(N0 * 4);
+    Rl[0] = malloc(N0 * sizeof(float)); Gl[0] = malloc(N0 * sizeof(float)); Bl[0] = malloc(N0 * sizeof(float));
     for (size_t i = 0; i < N0; i++) {
         Rl[0][i] = src[3 * i];
         Gl[0][i] = src[3 * i + 1];
@@ -286,8 +286,8 @@ int oracle_lateral_ca(const float *src, float *dst, int W0, int H0, int cell_siz
     const int mw = maxi_(2, (W0 + cell_size - 1) / cell_size);
     const int mh = maxi_(2, (H0 + cell_size - 1) / cell_size);
     const int nc = mw * mh;
-    float *dxr = calloc(nc, 4), *dyr = calloc(nc, 4), *cr = calloc(nc, 4);
-    float *dxb = calloc(nc, 4), *dyb = calloc(nc, 4), *cb = calloc(nc, 4);
+    float *dxr = calloc(nc, sizeof(float)), *dyr = calloc(nc, sizeof(float)), *cr = calloc(nc, sizeof(float));
+    float *dxb = calloc(nc, sizeof(float)), *dyb = calloc(nc, sizeof(float)), *cb = calloc(nc, sizeof(float));
     for (int l = L - 1; l >= 0; l--) {
         const float down = 1.f / (float)(1 << l);
         for (int i = 0; i < nc; i++) { dxr[i] *= down; dyr[i] *= down; dxb[i] *= down; dyb[i] *= down; }

@@ -74,8 +74,8 @@ int oracle_defringe(const float *src, float *dst, int W, int H, float radius, fl
     }
     max_guide = maxf_(max_guide, 1e-6f);
     const float ratio_eps = maxf_(max_guide * 1e-4f, 1e-6f);
-    float *guide_linear = malloc(N * 4), *guide_signal = malloc(N * 4);
-    float *log_rg = malloc(N * 4), *log_bg = malloc(N * 4);
+    float *guide_linear = malloc(N * sizeof(float)), *guide_signal = malloc(N * sizeof(float));
+    float *log_rg = malloc(N * sizeof(float)), *log_bg = malloc(N * sizeof(float));
 #pragma omp parallel for schedule(static)
     for (size_t i = 0; i < N; i++) {
         const float r = src[3 * i], g = src[3 * i + 1], b = src[3 * i + 2];
@@ -92,7 +92,7 @@ int oracle_defringe(const float *src, float *dst, int W, int H, float radius, fl
     }
 
     /* step 2: Sobel */
-    float *edge = calloc(N, 4), *edge_raw = calloc(N, 4);
+    float *edge = calloc(N, sizeof(float)), *edge_raw = calloc(N, sizeof(float));
     {
         float max_edge = 0.f;
         const int ly = H > 1 ? H - 1 : 1, lx = W > 1 ? W - 1 : 1;
@@ -119,7 +119,7 @@ int oracle_defringe(const float *src, float *dst, int W, int H, float radius, fl
     /* step 3: Gaussian taps, :4033-4048 */
     const float sigma = maxf_(radius / 3.f, 0.5f);
     const int ks = (int)ceilf(3.f * sigma);
-    blur_t B = {W, H, ks, malloc((2 * ks + 1) * 4), calloc(N, 4)};
+    blur_t B = {W, H, ks, malloc((2 * ks + 1) * sizeof(float)), calloc(N, sizeof(float))};
     {
         float ksum = 0.f;
         for (int k = -ks; k <= ks; k++) {
@@ -129,8 +129,8 @@ int oracle_defringe(const float *src, float *dst, int W, int H, float radius, fl
         }
         for (int k = 0; k < 2 * ks + 1; k++) B.kernel[k] /= ksum;
     }
-    float *blur_guide = malloc(N * 4), *blur_rg = malloc(N * 4), *blur_bg = malloc(N * 4);
-    float *blur_edge = malloc(N * 4), *blur_edge_raw = malloc(N * 4);
+    float *blur_guide = malloc(N * sizeof(float)), *blur_rg = malloc(N * sizeof(float)), *blur_bg = malloc(N * sizeof(float));
+    float *blur_edge = malloc(N * sizeof(float)), *blur_edge_raw = malloc(N * sizeof(float));
     gblur(&B, guide_linear, blur_guide);
     gblur(&B, log_rg, blur_rg);
     gblur(&B, log_bg, blur_bg);
@@ -143,7 +143,7 @@ int oracle_defringe(const float *src, float *dst, int W, int H, float radius, fl
     const float scene_gate = sstep(0.55f, 0.75f, max_guide);
 
     /* support seed, :4162-4248 */
-    float *fs = calloc(N, 4), *bfs = calloc(N, 4);
+    float *fs = calloc(N, sizeof(float)), *bfs = calloc(N, sizeof(float));
 #pragma omp parallel for schedule(static)
     for (size_t i = 0; i < N; i++) {
         const float rg = log_rg[i], bg = log_bg[i];
@@ -181,7 +181,7 @@ int oracle_defringe(const float *src, float *dst, int W, int H, float radius, fl
     gblur(&B, fs, bfs);
 
     /* line link, :4266-4323 */
-    float *ls = malloc(N * 4), *lfs = calloc(N, 4);
+    float *ls = malloc(N * sizeof(float)), *lfs = calloc(N, sizeof(float));
     for (size_t i = 0; i < N; i++) ls[i] = maxf_(fs[i], bfs[i] * 0.55f);
     static const int dirs[8][2] = {{1, 0}, {0, 1}, {1, 1}, {1, -1}, {2, 1}, {1, 2}, {2, -1}, {1, -2}};
 #define SAMPLE(map, X, Y) \
@@ -207,7 +207,7 @@ int oracle_defringe(const float *src, float *dst, int W, int H, float radius, fl
 #undef SAMPLE
 
     /* solo seeds, :4325-4376 */
-    float *sr = calloc(N, 4), *sb = calloc(N, 4), *srb = calloc(N, 4), *sbb = calloc(N, 4);
+    float *sr = calloc(N, sizeof(float)), *sb = calloc(N, sizeof(float)), *srb = calloc(N, sizeof(float)), *sbb = calloc(N, sizeof(float));
     const float solo_enable = sstep(1.25f, 3.0f, strength);
 #pragma omp parallel for schedule(static)
     for (size_t i = 0; i < N; i++) {

@@ -110,3 +110,52 @@ def place_native_u16(u16_plane, filters=RGGB, xtrans=None):
         native = (filters >> ((((r << 1) & 14) | (c & 1)) << 1)) & 3
     np.put_along_axis(img4, native[..., None].astype(np.int64), u16_plane[..., None], axis=2)
     return img4
+
+
+# ---- FP64 evaluations of the float stages (oracle/fp64.h: the oracle's own
+# source compiled with double in place of float); arrays of float64 ----------
+F64P = ctypes.POINTER(ctypes.c_double)
+
+
+def _d(a):
+    return np.ascontiguousarray(a, np.float64).copy()
+
+
+def fp64_defringe(img, inplace, radius=10.0, edge=0.1, chroma=0.15, strength=10.0, green=False, gscale=0.3):
+    h, w, _ = img.shape
+    a = _d(img)
+    out = a if inplace else np.zeros_like(a)
+    D = ctypes.c_double
+    assert oracle().oracle64_defringe(p(a, F64P), p(out, F64P), w, h, D(radius), D(edge), D(chroma), D(strength),
+                                      int(green), D(gscale)) == 0
+    return out
+
+
+def fp64_detail_tonemap(img):
+    a = _d(img)
+    h, w, _ = a.shape
+    assert oracle().oracle64_detail_tonemap(p(a, F64P), w, h) == 0
+    return a
+
+
+def fp64_axial_ca(img, inplace=False, radius=6, eps=1e-4, strength=0.3):
+    h, w, _ = img.shape
+    a = _d(img)
+    out = a if inplace else np.zeros_like(a)
+    D = ctypes.c_double
+    assert oracle().oracle64_axial_ca(p(a, F64P), p(out, F64P), w, h, radius, D(eps), D(strength)) == 0
+    return out
+
+
+def fp64_lateral_ca(img, inplace=False, cell=96, iters=3, shift=6.0, conf=0.02, levels=3):
+    h, w, _ = img.shape
+    a = _d(img)
+    out = a if inplace else np.zeros_like(a)
+    D = ctypes.c_double
+    assert oracle().oracle64_lateral_ca(p(a, F64P), p(out, F64P), w, h, cell, iters, D(shift), D(conf),
+                                        levels) == 0
+    return out
+
+
+def max_abs(a, b):
+    return float(np.max(np.abs(np.asarray(a, np.float64) - np.asarray(b, np.float64))))

@@ -9,12 +9,16 @@ lateral-CA scene, against the oracle chain on the GPU.
 * the same X-Trans chain at 1040 x 1560.
 
 The demosaics must be bit-exact; every later stage output within max-abs 2e-4
-and PSNR 70 dB of the oracle chain.
+and PSNR 70 dB of the oracle chain.  Each float stage is also compared with
+an FP64 evaluation of the chain (oracle/fp64.h) from the same demosaic
+output: the HIP chain must be no further from it than the oracle chain is,
+give or take the tolerance.
 """
 import numpy as np
 import pytest
 
-from _common import oracle, oracle_amaze, oracle_detail_tonemap, psnr
+from _common import (fp64_axial_ca, fp64_defringe, fp64_detail_tonemap, fp64_lateral_ca, max_abs, oracle,
+                     oracle_amaze, oracle_detail_tonemap, psnr)
 from libraw_enhanced_amd import scene
 
 pytestmark = pytest.mark.gpu
@@ -36,6 +40,12 @@ def _report(name, got, want):
     ndiff = int(np.count_nonzero(np.any(got != want, axis=2)))
     print(f"{name}: max-abs {err:.3g}, PSNR {q:.1f} dB, {ndiff} px differ", flush=True)
     return err, q
+
+
+def _fp64(name, got, want, f64):
+    eh, eo = max_abs(got, f64), max_abs(want, f64)
+    print(f"{name}: max-abs vs FP64 chain: HIP {eh:.3g}, oracle {eo:.3g}", flush=True)
+    assert eh <= eo + TOL
 
 
 def _run_stages(acc, chain, d_cfa):
@@ -66,6 +76,10 @@ def test_chain24_full_size(acc):
     o_tm = oracle_detail_tonemap(o_def)
     err, q = _report("chain24 detail TM", g_tm, o_tm)
     assert err <= TOL and q >= MIN_PSNR
+    print("chain24: FP64 chain", flush=True)
+    f_def = fp64_defringe(o_amaze, True)
+    _fp64("chain24 defringe", g_def, o_def, f_def)
+    _fp64("chain24 detail TM", g_tm, o_tm, fp64_detail_tonemap(f_def))
 
 
 @pytest.mark.parametrize("h,w", [(1040, 1560), (4160, 6240)])
@@ -88,3 +102,6 @@ def test_xtrans_lateral_axial_chain(acc, h, w):
     o_ax = axial(oracle(), "oracle_axial_ca", o_lat, inplace=True)
     err, q = _report(f"xtrans {w}x{h} axial CA", g_ax, o_ax)
     assert err <= TOL and q >= MIN_PSNR
+    f_lat = fp64_lateral_ca(o_x, inplace=True)
+    _fp64(f"xtrans {w}x{h} lateral CA", g_lat, o_lat, f_lat)
+    _fp64(f"xtrans {w}x{h} axial CA", g_ax, o_ax, fp64_axial_ca(f_lat, inplace=True))
