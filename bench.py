@@ -6,9 +6,12 @@ image, float32, from the seeded scene generator (gradient + 200 polygons +
 zone plate, Poisson-Gaussian noise), developed by AMaZE demosaic -> defringe
 (radius 10, strength 10, green off) -> detail tone map.  The CFA plane is
 resident in HBM before timing; a step is one pass of the chain over the image.
-Prints one JSON line: MP/s (value), ms per step, the roofline of the dominant
-kernel (HIP events on the stream the stages run on), and the reference's own
-CPU code timed on this host's cores on the same image.
+Prints one JSON line: MP/s (value), ms per step, the rooflines of the dominant
+kernel (HBM and VALU issue; HIP events on the stream the stages run on), the
+reference's own CPU code timed on this host's cores on the same image, parity
+of the 24 MP output against that CPU output and against an FP64 evaluation,
+and the metric's second half: p50/p99 latency of the request loop at 4, 8 and
+16 requests/s (--serve-duration seconds each) beside the plain per-image path.
 
 Other workloads, one JSON line each:
   xtrans26  one 26 MP (4160 x 6240) X-Trans image, pre-warped with radial
@@ -235,6 +238,8 @@ def main():
                     choices=["chain24", "xtrans26", "batch64", "serve", "micro24", "highlight24", "copy"])
     ap.add_argument("--rates", default="4,8,16", help="serve: arrival rates (requests/s)")
     ap.add_argument("--duration", type=float, default=20.0, help="serve: seconds of arrivals per rate")
+    ap.add_argument("--serve-duration", type=float, default=15.0,
+                    help="chain24: seconds of arrivals per rate of the served-loop leg (0 skips it)")
     ap.add_argument("--lanes", type=int, default=3, help="serve: requests in flight")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -290,7 +295,7 @@ def main():
     alg_bytes = STAGE_BYTES_PER_PX[dominant] * h * w
     achieved = alg_bytes / (stage_ms[dominant] * 1e-3) / 1e9
     line = {
-        "metric": "MP/s full develop chain (24 MP Bayer)",
+        "metric": "MP/s full develop chain (24 MP Bayer) + p99 latency at fixed QPS vs plain path",
         "value": round(value, 2),
         "unit": "MP/s",
         "n_gpus": 1,
@@ -343,6 +348,11 @@ def main():
     else:
         line["cpu_baseline"] = None
     acc.close()
+    if args.serve_duration > 0:
+        # the metric's second half: p50/p99 latency of the request loop at fixed
+        # arrival rates, beside the plain per-image path
+        rates = [float(r) for r in args.rates.split(",") if r]
+        line["serve"] = serve_leg(rates, args.serve_duration, args.lanes, burst=False)
     print(json.dumps(line), flush=True)
 
 
@@ -729,12 +739,15 @@ SERVE_MIX = [(0.4, 4000, 6000, "bayer"), (0.3, 5504, 8256, "bayer"), (0.2, 6336,
 WB = (2.1, 1.0, 1.5, 1.0)
 
 
-def bench_serve(args):
+def serve_leg(rates, duration, lanes, burst=True):
+    """The request loop at fixed arrival rates: Poisson arrivals of uint16
+    requests (SERVE_MIX) for `duration` seconds per rate, p50/p99 latency per
+    rate; a closed burst for the capacity; and the plain per-image path (host
+    arrays in and out of every Accelerator method) on one request per class."""
     from libraw_enhanced_amd import develop as dv
     from libraw_enhanced_amd import scene
     from libraw_enhanced_amd.accelerator import HIPAccelerator
-    rates = [float(r) for r in args.rates.split(",") if r]
-    _log("generating uint16 requests (2 scenes per size class) into pinned memory")
+    _log("serve: generating uint16 requests (2 scenes per size class) into pinned memory")
     inputs, outputs = {}, {}
     for ci, (_, h, w, kind) in enumerate(SERVE_MIX):
         for v in range(2):
@@ -759,7 +772,7 @@ def bench_serve(args):
         return dv.make_image(w, h, filters=9, xtrans=scene.XTRANS, cam_mul=WB, black=512, white=16383, wb=WB)
 
     acc = HIPAccelerator(0)  # device init before the server threads
-    server = dv.DevelopServer(0, lanes=args.lanes, hbm_budget_bytes=96 << 30)
+    server = dv.DevelopServer(0, lanes=lanes, hbm_budget_bytes=96 << 30)
     # warm-up: one request of each class (pool and code objects)
     for ci in range(len(SERVE_MIX)):
         server.submit(10_000 + ci, image_for(ci), inputs[(ci, 0)].ptr, outputs[ci].ptr)
@@ -768,7 +781,7 @@ def bench_serve(args):
         got += len(server.poll(timeout_ms=1000))
     results = []
     for rate in rates:
-        n = max(1, int(round(rate * args.duration)))
+        n = max(1, int(round(rate * duration)))
         arr = dv.poisson_schedule(rate, n, seed=int(rate * 1000))
         rng = np.random.Generator(np.random.PCG64(int(rate * 7)))
         classes = np.searchsorted(shares, rng.random(n), side="right").clip(0, len(SERVE_MIX) - 1)
@@ -779,30 +792,40 @@ def bench_serve(args):
             _, h, w, _ = SERVE_MIX[ci]
             return image_for(ci), inputs[(ci, int(variants[k]))].ptr, outputs[ci].ptr, h * w / 1e6, None
 
-        _log(f"rate {rate}/s: {n} requests over {arr[-1]:.1f} s")
-        wall, lat, mp_total = dv.run_open_loop(server, arr, make)
+        _log(f"serve: rate {rate}/s, {n} requests over {arr[-1]:.1f} s")
+        progress = {"t": time.perf_counter(), "k": 0}
+
+        def on_done(c):
+            progress["k"] += 1
+            if time.perf_counter() - progress["t"] > 30:
+                progress["t"] = time.perf_counter()
+                _log(f"serve: rate {rate}/s, {progress['k']} of {n} requests done")
+
+        wall, lat, mp_total = dv.run_open_loop(server, arr, make, on_done=on_done)
         rep = dv.ServeReport(rate=rate, requests=n, duration_s=wall, mp_total=mp_total, latencies_s=lat)
         summ = rep.summary()
         summ["offered_mp_s"] = round(mp_total / arr[-1], 2)
         results.append(summ)
-        _log(f"rate {rate}/s: {summ}")
-    # capacity: a closed burst of requests, all queued at once
-    n_burst = max(len(SERVE_MIX), int(round(2 * args.duration)))
-    rng = np.random.Generator(np.random.PCG64(99))
-    classes = np.searchsorted(shares, rng.random(n_burst), side="right").clip(0, len(SERVE_MIX) - 1)
+        _log(f"serve: rate {rate}/s: {summ}")
+    capacity = n_burst = None
+    if burst:
+        # capacity: a closed burst of requests, all queued at once
+        n_burst = max(len(SERVE_MIX), int(round(2 * duration)))
+        rng = np.random.Generator(np.random.PCG64(99))
+        classes = np.searchsorted(shares, rng.random(n_burst), side="right").clip(0, len(SERVE_MIX) - 1)
 
-    def make_burst(k):
-        ci = int(classes[k])
-        _, h, w, _ = SERVE_MIX[ci]
-        return image_for(ci), inputs[(ci, k % 2)].ptr, outputs[ci].ptr, h * w / 1e6, None
+        def make_burst(k):
+            ci = int(classes[k])
+            _, h, w, _ = SERVE_MIX[ci]
+            return image_for(ci), inputs[(ci, k % 2)].ptr, outputs[ci].ptr, h * w / 1e6, None
 
-    _log(f"capacity: burst of {n_burst} requests")
-    wall, lat, mp_burst = dv.run_open_loop(server, np.zeros(n_burst), make_burst)
-    capacity = mp_burst / wall
+        _log(f"serve: capacity, burst of {n_burst} requests")
+        wall, lat, mp_burst = dv.run_open_loop(server, np.zeros(n_burst), make_burst)
+        capacity = mp_burst / wall
     _, peak = server.footprint()
     server.close()
     # the plain per-image path: one request of each class, host arrays through every stage
-    _log("timing the plain per-image path (one request per class)")
+    _log("serve: timing the plain per-image path (one request per class)")
     plain_lat = {}
     for ci, (_, h, w, kind) in enumerate(SERVE_MIX):
         raw = inputs[(ci, 0)].array
@@ -825,24 +848,33 @@ def bench_serve(args):
         rgb = acc.defringe(rgb, rgb)
         rgb = acc.apply_detail_preserving_tonemap(rgb)
         plain_lat[ci] = time.perf_counter() - t
+    acc.close()
+    for pa in list(inputs.values()) + list(outputs.values()):
+        pa.free()
     mean_mp = sum(m[0] * m[1] * m[2] / 1e6 for m in SERVE_MIX)
     mean_plain_s = sum(m[0] * plain_lat[ci] for ci, m in enumerate(SERVE_MIX))
     plain_mp_s = mean_mp / mean_plain_s
-    line = {"metric": "p99 latency of the served develop loop", "value": results[-1]["p99_ms"], "unit": "ms",
-            "n_gpus": 1, "steps": sum(r["requests"] for r in results), "warmup": len(SERVE_MIX),
+    out = {"workload": "ingest -> demosaic -> axial_ca -> defringe -> detail_tm, Poisson arrivals, "
+                       "40% 24 MP / 30% 45.4 MP / 20% 60.2 MP Bayer / 10% 39.8 MP X-Trans",
+           "rates": results, "lanes": lanes, "duration_s_per_rate": duration, "peak_device_bytes": int(peak),
+           "plain_per_image_ms": {f"{m[1]}x{m[2]}{'x' if m[3] == 'xtrans' else ''}":
+                                  round(plain_lat[ci] * 1e3, 1) for ci, m in enumerate(SERVE_MIX)},
+           "plain_per_image_mp_s": round(plain_mp_s, 2),
+           "plain_mean_request_ms": round(mean_plain_s * 1e3, 1)}
+    if capacity is not None:
+        out.update({"capacity_mp_s": round(capacity, 2), "capacity_requests": n_burst,
+                    "served_capacity_vs_plain": round(capacity / plain_mp_s, 2)})
+    return out
+
+
+def bench_serve(args):
+    rates = [float(r) for r in args.rates.split(",") if r]
+    leg = serve_leg(rates, args.duration, args.lanes)
+    return {"metric": "p99 latency of the served develop loop", "value": leg["rates"][-1]["p99_ms"], "unit": "ms",
+            "n_gpus": 1, "steps": sum(r["requests"] for r in leg["rates"]), "warmup": len(SERVE_MIX),
             "ms_per_step": None, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
             "dtype": "u16->f32", "data": "synthetic uint16 CFA (seeded scenes, black 512, white 16383)",
-            "config": {"workload": "ingest -> demosaic -> axial_ca -> defringe -> detail_tm, Poisson arrivals, "
-                                   "40% 24 MP / 30% 45.4 MP / 20% 60.2 MP Bayer / 10% 39.8 MP X-Trans",
-                       "rates": results, "lanes": args.lanes, "duration_s_per_rate": args.duration,
-                       "peak_device_bytes": int(peak),
-                       "plain_per_image_ms": {f"{m[1]}x{m[2]}{'x' if m[3] == 'xtrans' else ''}":
-                                              round(plain_lat[ci] * 1e3, 1) for ci, m in enumerate(SERVE_MIX)},
-                       "capacity_mp_s": round(capacity, 2), "capacity_requests": n_burst,
-                       "plain_per_image_mp_s": round(plain_mp_s, 2),
-                       "served_capacity_vs_plain": round(capacity / plain_mp_s, 2)}}
-    acc.close()
-    return line
+            "config": leg}
 
 
 if __name__ == "__main__":
