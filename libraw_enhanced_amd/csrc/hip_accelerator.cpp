@@ -13,7 +13,7 @@ const float *fp(const ImageBufferFloat &b) { return reinterpret_cast<const float
 float *fp(ImageBufferFloat &b) { return reinterpret_cast<float *>(b.image); }
 }  // namespace
 
-HIPAccelerator::HIPAccelerator() = default;
+HIPAccelerator::HIPAccelerator(int device) : device_(device) {}
 
 HIPAccelerator::~HIPAccelerator() {
     if (ctx_) lre_hip_destroy(ctx_);
@@ -22,7 +22,7 @@ HIPAccelerator::~HIPAccelerator() {
 bool HIPAccelerator::initialize() {
     if (ctx_) return true;
     if (!lre_hip_available()) return false;
-    if (lre_hip_create(&ctx_, 0) != LRE_OK) {
+    if (lre_hip_create(&ctx_, device_) != LRE_OK) {
         ctx_ = nullptr;
         return false;
     }

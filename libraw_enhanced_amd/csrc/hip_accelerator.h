@@ -20,7 +20,8 @@ namespace libraw_enhanced {
 
 class HIPAccelerator {
 public:
-    HIPAccelerator();
+    // device: the HIP device index the backend binds to in initialize()
+    explicit HIPAccelerator(int device = 0);
     ~HIPAccelerator();
     HIPAccelerator(const HIPAccelerator &) = delete;
     HIPAccelerator &operator=(const HIPAccelerator &) = delete;
@@ -64,6 +65,7 @@ public:
     std::string last_error() const;
 
 private:
+    int device_ = 0;
     lre_ctx *ctx_ = nullptr;
     std::string device_name_;
 };
