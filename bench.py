@@ -23,6 +23,10 @@ Other workloads, one JSON line each:
   serve     Poisson arrivals of uint16 requests (40% 24 MP, 30% 45 MP, 20%
             61 MP Bayer, 10% 40 MP X-Trans; black 512, white 16383, WB
             2.1/1.0/1.5) through the request loop; p50/p99 latency and MP/s
+  grouping  the batch's two post-stage modes (one launch per stage over a
+            group of images; each image's chain on one of several streams) on
+            batches of 256 x 0.3 MP, 64 x 3.1 MP and 16 x 12 MP Bayer images,
+            eager and replayed from a graph, 1-3 lanes
   micro24   micro-contrast with a highlight mask on one resident 24 MP RGB
             image (the stage the pipeline runs for highlight_mode >= 4)
   highlight24  highlight recovery with its mask and white balance on one
@@ -235,7 +239,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--workload", default="chain24",
-                    choices=["chain24", "xtrans26", "batch64", "serve", "micro24", "highlight24", "copy"])
+                    choices=["chain24", "xtrans26", "batch64", "grouping", "serve", "micro24", "highlight24",
+                             "copy"])
     ap.add_argument("--rates", default="4,8,16", help="serve: arrival rates (requests/s)")
     ap.add_argument("--duration", type=float, default=20.0, help="serve: seconds of arrivals per rate")
     ap.add_argument("--serve-duration", type=float, default=15.0,
@@ -247,7 +252,8 @@ def main():
               "so N GPUs would each run this same single-GPU workload", file=sys.stderr)
         sys.exit(2)
     if args.workload != "chain24":
-        run = {"xtrans26": bench_xtrans26, "batch64": bench_batch64, "serve": bench_serve,
+        run = {"xtrans26": bench_xtrans26, "batch64": bench_batch64, "grouping": bench_grouping,
+               "serve": bench_serve,
                "micro24": bench_micro24, "highlight24": bench_highlight24, "copy": bench_copy}[args.workload]
         print(json.dumps(run(args)), flush=True)
         return
@@ -682,14 +688,22 @@ def bench_batch64(args):
     b = dv.BatchDeveloper(0, 2)
     tune_set = [images[sum(c for c, *_ in BATCH64[:i])] for i in range(len(BATCH64))]
     tune_ms = b.autotune(tune_set, stages, reps=1)
-    cfg = b.config()
+    cfg = {**b.config(), **b.grouping()}
     _log(f"autotune: {cfg} ({tune_ms:.1f} ms for the 5-image sample)")
-    b.run(images, stages)  # eager warm-up (sizes every buffer)
-    b.synchronize()
-    eager_t0 = time.perf_counter()
-    b.run(images, stages)
-    b.synchronize()
-    eager_s = time.perf_counter() - eager_t0
+    # eager batch in both post-stage modes (group: one launch per stage over
+    # the batch; lanes: per image over streams), each after a warm-up run
+    eager = {}
+    picked = cfg["group_mode"]
+    for mode in (not picked, picked):
+        b.set_grouping(mode)
+        b.run(images, stages)
+        b.synchronize()
+        t = time.perf_counter()
+        b.run(images, stages)
+        b.synchronize()
+        eager["group" if mode else "lanes"] = round(time.perf_counter() - t, 3)
+    eager_s = eager["group" if picked else "lanes"]
+    _log(f"eager batch: {eager}")
     b.capture(images, stages)
     for _ in range(args.warmup):
         b.replay()
@@ -720,7 +734,8 @@ def bench_batch64(args):
             "data": "synthetic (seeded scenes base..base+63, Poisson-Gaussian noise)",
             "config": {"workload": "demosaic -> axial_ca -> defringe -> detail_tm over 16x12 MP + 16x24 MP + "
                                    "12x45.4 MP + 8x60.2 MP RGGB + 12x39.8 MP X-Trans, float32, one batch",
-                       "total_mp": round(total_mp, 1), "autotuned": cfg, "eager_batch_s": round(eager_s, 3),
+                       "total_mp": round(total_mp, 1), "autotuned": cfg, "eager_batch_s": eager_s,
+                       "eager_batch_s_by_mode": eager,
                        "plain_per_image_mp_s": round(plain, 2), "batched_vs_plain": round(value / plain, 2),
                        "plain_sample_mp": round(plain_mp, 1)}}
     if not args.no_cpu:
@@ -865,6 +880,58 @@ def serve_leg(rates, duration, lanes, burst=True):
         out.update({"capacity_mp_s": round(capacity, 2), "capacity_requests": n_burst,
                     "served_capacity_vs_plain": round(capacity / plain_mp_s, 2)})
     return out
+
+
+# (count, height, width) of the grouping batches
+GROUPING = [(256, 480, 640), (64, 1536, 2048), (16, 3000, 4000)]
+
+
+def bench_grouping(args):
+    from libraw_enhanced_amd import develop as dv
+    from libraw_enhanced_amd import scene
+    from libraw_enhanced_amd.accelerator import HIPAccelerator
+    acc = HIPAccelerator(0)
+    stages = list(dv.BATCH_CHAIN)
+    steps = max(1, args.steps)
+    rows = []
+    for n, h, w in GROUPING:
+        _log(f"grouping: {n} x {w}x{h}")
+        cfa = scene.bayer_scene(h, w, scene.BASE_SEED)
+        bufs = [(acc.dev_from_host(cfa), acc.dev_alloc(h * w * 12)) for _ in range(n)]
+        imgs = [dv.make_image(w, h, filters=scene.RGGB, d_cfa=c, maximum_value=1.0, d_rgb=r) for c, r in bufs]
+        b = dv.BatchDeveloper(0, 1)
+        row = {"images": n, "width": w, "height": h, "total_mp": round(n * h * w / 1e6, 1)}
+        for mode in (False, True):
+            for lanes in (1, 2, 3):
+                b.set_grouping(mode)
+                b.set_config(lanes, 2, 4)
+                b.run(imgs, stages)
+                b.synchronize()
+                t = time.perf_counter()
+                for _ in range(steps):
+                    b.run(imgs, stages)
+                b.synchronize()
+                eager = (time.perf_counter() - t) / steps
+                b.capture(imgs, stages)
+                b.replay()
+                b.synchronize()
+                t = time.perf_counter()
+                for _ in range(steps):
+                    b.replay()
+                b.synchronize()
+                graph = (time.perf_counter() - t) / steps
+                key = f"{'group' if mode else 'lanes'}_{lanes}"
+                row[key] = {"eager_ms": round(eager * 1e3, 2), "graph_ms": round(graph * 1e3, 2)}
+        best = {m: min(v["graph_ms"] for k, v in row.items() if k.startswith(m)) for m in ("group", "lanes")}
+        row["best_graph_ms"] = best
+        row["group_vs_lanes"] = round(best["lanes"] / best["group"], 2)
+        rows.append(row)
+        b.close()
+        del imgs, bufs
+    return {"metric": "batch post-stage modes (ms per batch)", "value": rows[0]["group_vs_lanes"],
+            "unit": "x (lanes time / group time, 256 x 0.3 MP)", "n_gpus": 1, "steps": steps,
+            "higher_is_better": True, "dtype": "f32", "data": "synthetic (one seeded scene per size)",
+            "config": {"workload": " -> ".join(stages) + " over batches of equal Bayer images", "batches": rows}}
 
 
 def bench_serve(args):

@@ -384,9 +384,17 @@ typedef struct lre_image {
 
 /* Batched develop over resident images.  One multi-image tile queue per
  * demosaic kind (all Bayer images in one AMaZE launch, all X-Trans images in
- * one X-Trans launch), then the remaining stages per image spread over `lanes`
- * HIP streams.  lre_hip_batch_run only enqueues; lre_hip_batch_synchronize
- * waits.  A batch object is used by one host thread at a time. */
+ * one X-Trans launch).  The remaining stages run in group mode when it is on
+ * (see lre_hip_batch_set_grouping) and every one of them is axial CA, defringe
+ * (when its blur kernel has at most 16 taps and every gate's ramp width lies
+ * in [2^-20, 2^100], as at the reference's defaults), detail tone map, tone
+ * mapping, colour matrix or gamma: the images are spread over `lanes` HIP
+ * streams, each stream's images form groups whose stage scratch fits its share
+ * of the post-stage budget, and each stage is one launch over a whole group.
+ * Otherwise each image runs its stages on one of the `lanes` streams.  Both
+ * give the same bits as the per-image entry points.  lre_hip_batch_run only
+ * enqueues; lre_hip_batch_synchronize waits.  A batch object is used by one
+ * host thread at a time. */
 typedef struct lre_batch lre_batch;
 int lre_hip_batch_create(lre_batch **out, int device, int lanes);
 int lre_hip_batch_destroy(lre_batch *b);
@@ -407,8 +415,16 @@ int lre_hip_batch_set_config(lre_batch *b, int lanes, int amaze_slots_per_cu,
                              int xtrans_slots_per_cu);
 int lre_hip_batch_get_config(lre_batch *b, int *lanes, int *amaze_slots_per_cu,
                              int *xtrans_slots_per_cu);
-/* Time `reps` runs of each candidate launch shape on this batch and keep the
- * fastest; *best_ms receives its time per batch. */
+/* Group mode off (0), on (1) or by image size (2, the default: on when the
+ * batch's mean image is below 2 MP, where one launch per stage is faster than
+ * per-image chains), and the device bytes the groups' stage scratch may take
+ * (0: half of the device's memory; an image that alone exceeds its lane's
+ * share forms a group of its own). */
+int lre_hip_batch_set_grouping(lre_batch *b, int group_mode, size_t post_budget_bytes);
+int lre_hip_batch_get_grouping(lre_batch *b, int *group_mode, size_t *post_budget_bytes);
+/* Time `reps` runs of each candidate launch shape (group mode on and off,
+ * lanes, slots per CU) on this batch and keep the fastest; *best_ms receives
+ * its time per batch. */
 int lre_hip_batch_autotune(lre_batch *b, const lre_image *images, int n, const int *stages,
                            int nstages, const lre_chain_params *params, int reps, float *best_ms);
 

@@ -48,6 +48,7 @@ __global__ __launch_bounds__(BLOCK) void axial_max_g(const float *__restrict__ s
 }
 
 constexpr int AX_TX = 64, AX_TY = 32, AX_RMAX = 16;
+constexpr int AX_CHUNK = BLOCK * 16;  // pixels per workgroup of the multi-image max(G)
 constexpr int AX_BLOCK = 512;  // one 4-row strip of one column per thread
 
 struct AxialArgs {
@@ -58,6 +59,30 @@ struct AxialArgs {
     float eps, strength;
     const unsigned *stats;
 };
+
+// several images: the max(G) statistics reset (one workgroup per image) and
+// folded over chunks of AX_CHUNK pixels
+__global__ void axial_stats_reset_many(lre::Many<AxialArgs> S) {
+    const lre::TileOf tl = S.tile();
+    if (threadIdx.x < 4) const_cast<unsigned *>(S.args(tl).stats)[threadIdx.x] = 0u;
+}
+__global__ __launch_bounds__(BLOCK) void axial_max_g_many(lre::Many<AxialArgs> S) {
+    const lre::TileOf tl = S.tile();
+    const AxialArgs &A = S.args(tl);
+    const size_t n = (size_t)A.W * A.H, b = (size_t)tl.tx * AX_CHUNK, e = b + AX_CHUNK < n ? b + AX_CHUNK : n;
+    float m = lre::green_max_range(A.src, b, e, 1e-6f, [](float v) { return v; });
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    __shared__ float red[BLOCK / 64];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float v = red[0];
+        for (int k = 1; k < BLOCK / 64; k++) v = fmaxf(v, red[k]);
+        unsigned *st = const_cast<unsigned *>(A.stats);
+        const unsigned u = __float_as_uint(v);
+        if (u > __hip_atomic_load(st, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(st, u);
+    }
+}
 
 // Horizontal FP64 window sums over LDS rows [0, RY) into hs[RY][AX_HP], for
 // output columns x0 .. x0+AX_TX-1; region column 0 is image column x0 - r - 1
@@ -121,9 +146,13 @@ size_t axial_ab_lds(int r) {
     return sizeof(float) * (RX | 1) * (4 * RY - 2) + sizeof(double) * (RY - 2) * AX_HP;
 }
 
-// pass 1: a, b for R and B, and the edge weight
-__global__ __launch_bounds__(AX_BLOCK) void axial_ab(AxialArgs A) {
+// pass 1: a, b for R and B, and the edge weight.  Src: lre::One<AxialArgs>
+// (one image, 2D grid) or lre::Many<AxialArgs> (the tiles of several images).
+template <class Src>
+__global__ __launch_bounds__(AX_BLOCK) void axial_ab(Src S) {
     extern __shared__ __align__(16) unsigned char smem[];
+    const lre::TileOf tl = S.tile();
+    const AxialArgs &A = S.args(tl);
     const int r = A.r;
     const int RX = AX_TX + 2 * r + 2, RY = AX_TY + 2 * r + 2;  // one extra ring for the gradient
     const int RXP = RX | 1;                              // odd LDS pitch
@@ -132,7 +161,7 @@ __global__ __launch_bounds__(AX_BLOCK) void axial_ab(AxialArgs A) {
     float *sB = sG + RXP * RY;
     float *sE = sB + RXP * RY;                           // gradient magnitude, rows 1 .. RY-2
     double *hs = reinterpret_cast<double *>(sE + RXP * (RY - 2));  // (RY-2) x AX_HP
-    const int x0 = blockIdx.x * AX_TX, y0 = blockIdx.y * AX_TY;
+    const int x0 = tl.tx * AX_TX, y0 = tl.ty * AX_TY;
     const int W = A.W, H = A.H;
     const float inv_norm = 1.f / mx(__uint_as_float(A.stats[0]), 1e-6f);
     lre::stage2d<AX_BLOCK>(
@@ -236,14 +265,17 @@ size_t axial_apply_lds(int r) {
 }
 
 // pass 2: box means of a, b; q = ma * G + mb; blend by strength * edge weight
-__global__ __launch_bounds__(AX_BLOCK) void axial_apply(AxialArgs A) {
+template <class Src>
+__global__ __launch_bounds__(AX_BLOCK) void axial_apply(Src S) {
     extern __shared__ __align__(16) unsigned char smem[];
+    const lre::TileOf tl = S.tile();
+    const AxialArgs &A = S.args(tl);
     const int r = A.r;
     const int RX = AX_TX + 2 * r + 2, RY = AX_TY + 2 * r;
     const int RXP = RX | 1;
     float *q = reinterpret_cast<float *>(smem);             // 4 planes, RY x RXP each
     double *hs = reinterpret_cast<double *>(q + 4 * RXP * RY);
-    const int x0 = blockIdx.x * AX_TX, y0 = blockIdx.y * AX_TY;
+    const int x0 = tl.tx * AX_TX, y0 = tl.ty * AX_TY;
     const int W = A.W, H = A.H;
     const float norm_ref = mx(__uint_as_float(A.stats[0]), 1e-6f);
     const float inv_norm = 1.f / norm_ref;
@@ -542,11 +574,25 @@ int lre_check_axial_params(int radius) {
     return LRE_OK;
 }
 
-int lre_launch_axial_ca(const float *d_in, float *d_out, int w, int h, int radius, float epsilon,
-                        float strength, void *d_scratch, hipStream_t s) {
-    if (const int rc = lre_check_axial_params(radius)) return rc;
-    if (radius < 1) radius = 1;
-    strength = strength < 0.f ? 0.f : (1.f < strength ? 1.f : strength);
+namespace {
+
+// raise the dynamic LDS limit of both passes once per process and source type
+template <class Src>
+hipError_t axial_lds_attr() {
+    static const hipError_t rc = [] {
+        hipError_t e = hipFuncSetAttribute((const void *)axial_ab<Src>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)axial_ab_lds(AX_RMAX));
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute((const void *)axial_apply<Src>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)axial_apply_lds(AX_RMAX));
+        return e;
+    }();
+    return rc;
+}
+
+// the arguments of one image; planes carved from its scratch (5 planes + stats)
+AxialArgs axial_args(const float *d_in, float *d_out, int w, int h, int radius, float epsilon, float strength,
+                     void *d_scratch) {
     const size_t n = (size_t)w * h, plane = align256(n * 4);
     char *base = static_cast<char *>(d_scratch);
     AxialArgs A;
@@ -556,24 +602,70 @@ int lre_launch_axial_ca(const float *d_in, float *d_out, int w, int h, int radiu
     A.aB = reinterpret_cast<float *>(base + 2 * plane);
     A.bB = reinterpret_cast<float *>(base + 3 * plane);
     A.ew = reinterpret_cast<float *>(base + 4 * plane);
-    unsigned *stats = reinterpret_cast<unsigned *>(base + 5 * plane);
-    A.stats = stats;
-    A.W = w; A.H = h; A.r = radius; A.eps = epsilon; A.strength = strength;
+    A.stats = reinterpret_cast<unsigned *>(base + 5 * plane);
+    A.W = w; A.H = h; A.r = radius < 1 ? 1 : radius; A.eps = epsilon;
+    A.strength = strength < 0.f ? 0.f : (1.f < strength ? 1.f : strength);
+    return A;
+}
+
+}  // namespace
+
+int lre_launch_axial_ca(const float *d_in, float *d_out, int w, int h, int radius, float epsilon,
+                        float strength, void *d_scratch, hipStream_t s) {
+    if (const int rc = lre_check_axial_params(radius)) return rc;
+    const AxialArgs A = axial_args(d_in, d_out, w, h, radius, epsilon, strength, d_scratch);
+    const size_t n = (size_t)w * h;
+    unsigned *stats = const_cast<unsigned *>(A.stats);
     hipLaunchKernelGGL(init_u32, dim3(1), dim3(64), 0, s, stats, 4);
     hipLaunchKernelGGL(axial_max_g, dim3(grid_for(n)), dim3(BLOCK), 0, s, d_in, n, stats);
     dim3 grid((w + AX_TX - 1) / AX_TX, (h + AX_TY - 1) / AX_TY);
-    // raise the dynamic LDS limit once per process (thread-safe static initialisation)
-    static const hipError_t attr_rc = [] {
-        hipError_t e = hipFuncSetAttribute((const void *)axial_ab, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)axial_ab_lds(AX_RMAX));
-        if (e == hipSuccess)
-            e = hipFuncSetAttribute((const void *)axial_apply, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)axial_apply_lds(AX_RMAX));
-        return e;
-    }();
-    LRE_HIP_CHECK(attr_rc);
-    hipLaunchKernelGGL(axial_ab, grid, dim3(AX_BLOCK), axial_ab_lds(radius), s, A);
-    hipLaunchKernelGGL(axial_apply, grid, dim3(AX_BLOCK), axial_apply_lds(radius), s, A);
+    using One = lre::One<AxialArgs>;
+    LRE_HIP_CHECK(axial_lds_attr<One>());
+    hipLaunchKernelGGL(axial_ab<One>, grid, dim3(AX_BLOCK), axial_ab_lds(A.r), s, One{A});
+    hipLaunchKernelGGL(axial_apply<One>, grid, dim3(AX_BLOCK), axial_apply_lds(A.r), s, One{A});
+    LRE_HIP_CHECK(hipGetLastError());
+    return LRE_OK;
+}
+
+// Many images: three tables over the same per-image arguments (one workgroup
+// per image for the statistics reset, max(G) chunks, the window tiles).
+namespace {
+using AxTable = lre::ManyTable<AxialArgs>;
+size_t ax_part(int n) { return align256(AxTable::bytes(n)); }
+}  // namespace
+
+size_t lre_axial_multi_table_bytes(int n) { return 3 * ax_part(n); }
+
+int lre_axial_multi_fill(const lre_post_job *jobs, int n, int radius, float epsilon, float strength,
+                         void *h_table, int grids[3]) {
+    if (const int rc = lre_check_axial_params(radius)) return rc;
+    auto args = [&](int i) {
+        return axial_args(jobs[i].d_in, jobs[i].d_out, jobs[i].w, jobs[i].h, radius, epsilon, strength,
+                          jobs[i].d_scratch);
+    };
+    char *h = static_cast<char *>(h_table);
+    grids[0] = AxTable::fill(h, n, [](int) { return make_int2(1, 1); }, args);
+    grids[1] = AxTable::fill(h + ax_part(n), n, [&](int i) {
+        return make_int2((int)(((size_t)jobs[i].w * jobs[i].h + AX_CHUNK - 1) / AX_CHUNK), 1);
+    }, args);
+    grids[2] = AxTable::fill(h + 2 * ax_part(n), n, [&](int i) {
+        return make_int2((jobs[i].w + AX_TX - 1) / AX_TX, (jobs[i].h + AX_TY - 1) / AX_TY);
+    }, args);
+    return LRE_OK;
+}
+
+int lre_launch_axial_multi(const void *d_table, int n, int radius, const int grids[3], hipStream_t s) {
+    if (n == 0) return LRE_OK;
+    if (const int rc = lre_check_axial_params(radius)) return rc;
+    if (radius < 1) radius = 1;
+    const char *d = static_cast<const char *>(d_table);
+    using Many = lre::Many<AxialArgs>;
+    LRE_HIP_CHECK(axial_lds_attr<Many>());
+    hipLaunchKernelGGL(axial_stats_reset_many, dim3(grids[0]), dim3(64), 0, s, AxTable::view(d, n));
+    hipLaunchKernelGGL(axial_max_g_many, dim3(grids[1]), dim3(BLOCK), 0, s, AxTable::view(d + ax_part(n), n));
+    const Many tiles = AxTable::view(d + 2 * ax_part(n), n);
+    hipLaunchKernelGGL(axial_ab<Many>, dim3(grids[2]), dim3(AX_BLOCK), axial_ab_lds(radius), s, tiles);
+    hipLaunchKernelGGL(axial_apply<Many>, dim3(grids[2]), dim3(AX_BLOCK), axial_apply_lds(radius), s, tiles);
     LRE_HIP_CHECK(hipGetLastError());
     return LRE_OK;
 }

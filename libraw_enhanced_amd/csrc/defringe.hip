@@ -194,19 +194,61 @@ LRE_DEV float support_px(const Planes &P, const Scal &C, const Gates &G, size_t 
     return c_edge * c_bright * c_hue * c_chroma * red_guard * green_enable;
 }
 
+// ---- separable Gaussian, clamp-to-edge, taps in the reference's order ----
+struct BlurSet {
+    const float *in[5];
+    float *tmp[5];
+    float *out[5];
+    int scale_by_edge[5];  // multiply the input by 1/max(edge) (the normalised edge map)
+    int count;
+};
+
+// normalised taps, passed by value (no host-to-device copy per call)
+struct Taps {
+    float v[MAX_TAPS];
+};
+
+// Everything per image of a defringe launch: its planes, the plane sets of
+// the three blur passes, its two global maxima and its size.  The kernels
+// take it through lre::One<DfImg> (one image) or lre::Many<DfImg> (several);
+// the parameters common to the batch (Scal without the size, Gates, Taps) are
+// kernel arguments.
+struct DfImg {
+    Planes P;
+    float *gl, *er;
+    BlurSet sets[3];
+    unsigned *stats;
+    int W, H, inplace;
+};
+// the batch-wide scalars with one image's size and aliasing filled in
+LRE_DEV Scal scal_of(const Scal &C, const DfImg &I) {
+    Scal c = C;
+    c.W = I.W;
+    c.H = I.H;
+    c.inplace = I.inplace;
+    return c;
+}
+constexpr int DF_CHUNK = BLOCK * 8;  // pixels per workgroup of the pointwise passes
+
 // guide planes, log chroma ratios (:3970-3991) and the 3x3 Sobel magnitude of
 // guide_signal, 0 on the frame (:3996-4027), in one pass: a workgroup takes a
 // 64 x 16 tile and puts guide_signal of the tile and a one-pixel ring in LDS;
 // guide_signal itself is never stored.
 constexpr int PS_TX = 64, PS_TY = 16, PS_W = PS_TX + 2, PS_H = PS_TY + 2;
 
-__global__ __launch_bounds__(BLOCK) void prep_sobel(const float *src, float *__restrict__ gl,
-                                                    float *__restrict__ lrg, float *__restrict__ lbg,
-                                                    float *__restrict__ er, int W, int H, unsigned *stats) {
+template <class Src>
+__global__ __launch_bounds__(BLOCK) void prep_sobel(Src S) {
+    const lre::TileOf tl = S.tile();
+    const DfImg &I = S.args(tl);
+    const float *src = I.P.src;
+    float *__restrict__ gl = I.gl, *__restrict__ lrg = I.P.lrg, *__restrict__ lbg = I.P.lbg;
+    float *__restrict__ er = I.er;
+    const int W = I.W, H = I.H;
+    unsigned *stats = I.stats;
     __shared__ float sgs[PS_H * PS_W];
     const float max_guide = max_guide_of(stats);
     const float ratio_eps = mx(max_guide * 1e-4f, 1e-6f);
-    const int x0 = blockIdx.x * PS_TX, y0 = blockIdx.y * PS_TY;
+    const int x0 = tl.tx * PS_TX, y0 = tl.ty * PS_TY;
     // ring cells outside the image only feed frame pixels, whose magnitude is 0
     lre::stage2d<BLOCK>(
         PS_H, PS_W,
@@ -241,20 +283,6 @@ __global__ __launch_bounds__(BLOCK) void prep_sobel(const float *src, float *__r
     }
     block_max_to(m, &stats[1]);
 }
-
-// ---- separable Gaussian, clamp-to-edge, taps in the reference's order ----
-struct BlurSet {
-    const float *in[5];
-    float *tmp[5];
-    float *out[5];
-    int scale_by_edge[5];  // multiply the input by 1/max(edge) (the normalised edge map)
-    int count;
-};
-
-// normalised taps, passed by value (no host-to-device copy per call)
-struct Taps {
-    float v[MAX_TAPS];
-};
 
 constexpr int HB_W = 256;  // outputs per block along x
 
@@ -333,10 +361,16 @@ static_assert(FB_KSMAX <= (MAX_TAPS - 1) / 2, "fused blur radius within the tap 
 // output of it also yields the fringe support seed of its pixel: the thread
 // that writes it wrote the pixel's other blurred planes in the earlier plane
 // iterations (barriers between them).
-template <int KS, bool SUP>
-__global__ __launch_bounds__(FB_BLOCK) void blur2d(BlurSet S, Taps taps, int W, int H,
-                                                const unsigned *stats, Planes P, Scal C, Gates G) {
+template <int KS, bool SUP, class Src>
+__global__ __launch_bounds__(FB_BLOCK) void blur2d(Src SR, int which, Taps taps, Scal Cb, Gates G) {
     extern __shared__ float fb[];
+    const lre::TileOf tl = SR.tile();
+    const DfImg &I = SR.args(tl);
+    const BlurSet &S = I.sets[which];
+    const Planes &P = I.P;
+    const Scal C = scal_of(Cb, I);
+    const int W = I.W, H = I.H;
+    const unsigned *stats = I.stats;
     constexpr int ks = KS, nt = 2 * KS + 1;
     constexpr int IW = FB_TX + 2 * ks, IH = FB_TY + 2 * ks;
     // odd row pitches: lanes that walk down rows hit distinct LDS banks
@@ -348,7 +382,7 @@ __global__ __launch_bounds__(FB_BLOCK) void blur2d(BlurSet S, Taps taps, int W, 
 #pragma unroll
     for (int t = 0; t < nt; t++) kt[t] = taps.v[t];
     const float inv_edge = 1.f / mx(__uint_as_float(stats[1]), 1e-6f);
-    const int x0 = blockIdx.x * FB_TX, y0 = blockIdx.y * FB_TY;
+    const int x0 = tl.tx * FB_TX, y0 = tl.ty * FB_TY;
     for (int p = 0; p < S.count; p++) {
         const float *in = S.in[p];
         const bool scaled = S.scale_by_edge[p] != 0;
@@ -431,10 +465,10 @@ __global__ __launch_bounds__(FB_BLOCK) void blur2d(BlurSet S, Taps taps, int W, 
 }
 
 // the LDS window of blur2d<KS> exceeds 64 KiB: raise the limit once per radius
-template <int KS, bool SUP>
+template <int KS, bool SUP, class Src>
 hipError_t blur2d_lds_attr() {
     static const hipError_t rc = hipFuncSetAttribute(
-        (const void *)blur2d<KS, SUP>, hipFuncAttributeMaxDynamicSharedMemorySize,
+        (const void *)blur2d<KS, SUP, Src>, hipFuncAttributeMaxDynamicSharedMemorySize,
         (int)(sizeof(float) * ((FB_TY + 2 * KS) * ((FB_TX + 2 * KS) | 1) + (FB_TY + 2 * KS) * (FB_TX + 1))));
     return rc;
 }
@@ -484,11 +518,15 @@ LRE_DEV void solo_px(const Planes &P, const Scal &C, const Gates &G, size_t i, f
 // every sample).
 constexpr int LL_T = 32, LL_HALO = 14, LL_R = LL_T + 2 * LL_HALO;
 
-template <bool FAST>
-__global__ __launch_bounds__(BLOCK) void line_link(Planes P, Scal C, Gates G) {
+template <bool FAST, class Src>
+__global__ __launch_bounds__(BLOCK) void line_link(Src SR, Scal Cb, Gates G) {
+    const lre::TileOf tl = SR.tile();
+    const DfImg &I = SR.args(tl);
+    const Planes &P = I.P;
+    const Scal C = scal_of(Cb, I);
     const int W = C.W, H = C.H;
     __shared__ float s_ls[LL_R * LL_R], s_fs[LL_R * LL_R];
-    const int x0 = blockIdx.x * LL_T, y0 = blockIdx.y * LL_T;
+    const int x0 = tl.tx * LL_T, y0 = tl.ty * LL_T;
     bool flat = true;
     lre::stage2d<BLOCK>(
         LL_R, LL_R,
@@ -540,15 +578,21 @@ __global__ __launch_bounds__(BLOCK) void line_link(Planes P, Scal C, Gates G) {
 }
 
 // main correction and solo cleanup (:4377-4987)
-template <bool FAST>
-__global__ __launch_bounds__(BLOCK) void correct(Planes P, Scal C, Gates G, const unsigned *stats) {
+template <bool FAST, class Src>
+__global__ __launch_bounds__(BLOCK) void correct(Src SR, Scal Cb, Gates G) {
+    const lre::TileOf tl = SR.tile();
+    const DfImg &I = SR.args(tl);
+    const Planes &P = I.P;
+    const Scal C = scal_of(Cb, I);
+    const unsigned *stats = I.stats;
     const size_t n = (size_t)C.W * C.H;
+    const size_t chunk_end = ((size_t)tl.tx + 1) * DF_CHUNK < n ? ((size_t)tl.tx + 1) * DF_CHUNK : n;
     const float strength = C.strength;
     const auto gate = [&](Gate k, float x) { return sstep<FAST>(G.g[k], x); };
     const bool eg = C.enable_green != 0;
     const float scene_gate = sstepc(0.55f, 0.75f, max_guide_of(stats));
     const float mask_cutoff = C.mask_cutoff;
-    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK) {
+    for (size_t i = (size_t)tl.tx * DF_CHUNK + threadIdx.x; i < chunk_end; i += BLOCK) {
         const float s0 = P.src[3 * i], s1 = P.src[3 * i + 1], s2 = P.src[3 * i + 2];
         const float bguide = P.bg[i], brg = P.brg[i], bbg = P.bbg[i], bfs = P.bfs[i], lfs = P.lfs[i];
         const float rg = P.lrg[i], bg = P.lbg[i], be = P.be[i], ber = P.ber[i];
@@ -797,40 +841,38 @@ int lre_check_defringe_params(float radius) {
     return LRE_OK;
 }
 
-int lre_launch_defringe(const float *d_in, float *d_out, int w, int h, float radius,
-                        float edge_threshold, float chroma_threshold, float strength,
-                        int defringe_green, float green_strength_scale, void *d_scratch,
-                        hipStream_t s) {
-    if (const int rc = lre_check_defringe_params(radius)) return rc;
-    const size_t n = (size_t)w * h;
-    const size_t plane = ((n * 4) + 255) & ~(size_t)255;
-    char *base = static_cast<char *>(d_scratch);
-    auto take = [&](int k) { return reinterpret_cast<float *>(base + plane * k); };
-    float *gl = take(0), *gs = take(1), *lrg = take(2), *lbg = take(3), *er = take(4);
-    float *t0 = take(5), *t1 = take(6), *t2 = take(7), *t3 = take(8), *t4 = take(9);
-    float *bg = take(10), *brg = take(11), *bbg = take(12), *be = take(13), *ber = take(14);
-    float *fs = take(15), *bfs = take(16), *lfs = take(17);
-    float *sr = gl, *sb = gs, *srb = t0, *sbb = t1;  // reuse: the guide planes are dead by then
-    unsigned *stats = reinterpret_cast<unsigned *>(base + plane * 20);
+namespace {
 
+// the parameters common to every image of a call
+struct DfParams {
+    Taps taps;
+    Scal C;  // W, H and inplace are taken per image
+    Gates G;
+    int ks;
+    bool fast;  // every ramp width inside the quotient shortcut's range
+};
+
+DfParams df_params(float radius, float edge_threshold, float chroma_threshold, float strength, int defringe_green,
+                   float green_strength_scale) {
+    DfParams D{};
     // Gaussian taps (:4033-4048), computed on the host with the C library's expf
     green_strength_scale = green_strength_scale < 0.f ? 0.f : (1.f < green_strength_scale ? 1.f : green_strength_scale);
     const float sigma = (radius / 3.f < 0.5f) ? 0.5f : radius / 3.f;
     const int ks = (int)std::ceil(3.f * sigma);
-    Taps taps{};
+    D.ks = ks;
     {
         float ksum = 0.f;
         for (int k = -ks; k <= ks; k++) {
             float v = std::exp(-0.5f * (float)(k * k) / (sigma * sigma));
-            taps.v[k + ks] = v;
+            D.taps.v[k + ks] = v;
             ksum += v;
         }
-        for (int k = 0; k < 2 * ks + 1; k++) taps.v[k] /= ksum;
+        for (int k = 0; k < 2 * ks + 1; k++) D.taps.v[k] /= ksum;
     }
-
-    Scal C;
-    C.W = w; C.H = h; C.ks = ks;
-    C.inplace = d_in == d_out;
+    Scal &C = D.C;
+    C.W = C.H = 0;
+    C.ks = ks;
+    C.inplace = 0;
     C.enable_green = defringe_green ? 1 : 0;
     C.edge_thr = edge_threshold; C.chroma_thr = chroma_threshold; C.strength = strength;
     C.green_scale = green_strength_scale;
@@ -848,14 +890,13 @@ int lre_launch_defringe(const float *d_in, float *d_out, int w, int h, float rad
     }
     // parameter-dependent smoothstep edges; the quotient shortcut needs every
     // ramp width inside [2^-20, 2^100] (else the kernels divide)
-    Gates G;
-    bool fast = true;
+    D.fast = true;
     {
         const float i = C.isens, et = edge_threshold, ct = chroma_threshold, mc = C.mask_cutoff;
         auto set = [&](Gate k, float e0, float e1) {
             const float d = e1 - e0;
-            G.g[k] = Edge{e0, d, 1.0f / d};
-            if (!(d >= 0x1p-20f && d <= 0x1p100f)) fast = false;
+            D.G.g[k] = Edge{e0, d, 1.0f / d};
+            if (!(d >= 0x1p-20f && d <= 0x1p100f)) D.fast = false;
         };
         set(G_EDGE, et * 0.35f * i, et * 1.50f * i);
         set(G_EDGE_RATIO, 0.55f * i, 1.35f * i);
@@ -875,79 +916,222 @@ int lre_launch_defringe(const float *d_in, float *d_out, int w, int h, float rad
         set(G_HALO, ct * 0.55f * i, ct * 1.65f * i);
         set(G_MASK, mc, mc * 5.0f);
     }
+    return D;
+}
 
-    const int g = grid_for(n);
-    hipLaunchKernelGGL(init_stats, dim3(1), dim3(64), 0, s, stats);
-    hipLaunchKernelGGL(max_green, dim3(g), dim3(BLOCK), 0, s, d_in, n, stats);
-    hipLaunchKernelGGL(prep_sobel, dim3((w + PS_TX - 1) / PS_TX, (h + PS_TY - 1) / PS_TY), dim3(BLOCK), 0, s,
-                       d_in, gl, lrg, lbg, er, w, h, stats);
-    LRE_HIP_CHECK(hipGetLastError());
-
-    Planes P;
-    P.src = d_in; P.dst = d_out;
-    P.lrg = lrg; P.lbg = lbg; P.bg = bg; P.brg = brg; P.bbg = bbg; P.be = be; P.ber = ber;
-    P.fs = fs; P.bfs = bfs; P.lfs = lfs; P.sr = sr; P.sb = sb; P.srb = srb; P.sbb = sbb;
-    // blur of a plane set; with seed, also the support seeds (the fused form
-    // exists for the fused radii and the quotient smoothsteps only)
-    auto blur = [&](BlurSet S, bool seed) -> int {
-        if (ks <= FB_KSMAX) {
-            const size_t lds = sizeof(float) * ((size_t)(FB_TY + 2 * ks) * ((FB_TX + 2 * ks) | 1) +
-                                                (size_t)(FB_TY + 2 * ks) * (FB_TX + 1));
-            dim3 gb((w + FB_TX - 1) / FB_TX, (h + FB_TY - 1) / FB_TY);
-            switch (ks) {
-#define LRE_BLUR2D(K) \
-    case K: \
-        if (seed) { \
-            LRE_HIP_CHECK((blur2d_lds_attr<K, true>())); \
-            hipLaunchKernelGGL((blur2d<K, true>), gb, dim3(FB_BLOCK), lds, s, S, taps, w, h, stats, P, C, G); \
-        } else { \
-            LRE_HIP_CHECK((blur2d_lds_attr<K, false>())); \
-            hipLaunchKernelGGL((blur2d<K, false>), gb, dim3(FB_BLOCK), lds, s, S, taps, w, h, stats, P, C, G); \
-        } \
-        break;
-                LRE_BLUR2D(1) LRE_BLUR2D(2) LRE_BLUR2D(3) LRE_BLUR2D(4) LRE_BLUR2D(5) LRE_BLUR2D(6)
-                LRE_BLUR2D(7) LRE_BLUR2D(8) LRE_BLUR2D(9) LRE_BLUR2D(10) LRE_BLUR2D(11)
-                LRE_BLUR2D(12) LRE_BLUR2D(13) LRE_BLUR2D(14) LRE_BLUR2D(15) LRE_BLUR2D(16)
-#undef LRE_BLUR2D
-            default: break;
-            }
-            LRE_HIP_CHECK(hipGetLastError());
-            return LRE_OK;
-        }
-        dim3 gh((w + HB_W - 1) / HB_W, h);
-        hipLaunchKernelGGL(blur_h, gh, dim3(BLOCK), 0, s, S, taps, w, h, ks, stats);
-        dim3 gv((w + VB_W - 1) / VB_W, (h + VB_H - 1) / VB_H);
-        hipLaunchKernelGGL(blur_v, gv, dim3(BLOCK), sizeof(float) * (VB_H + 2 * ks) * VB_W, s, S, taps, w, h, ks);
-        LRE_HIP_CHECK(hipGetLastError());
-        return LRE_OK;
-    };
-    BlurSet S5{};
+// One image's planes, carved from its scratch: 20 planes and the statistics,
+// or with `compact` (the fused blur only, which needs no separable
+// temporaries) 15 planes: the temporaries t2..t4 then alias planes the fused
+// blur never touches through them.
+DfImg df_image(const float *d_in, float *d_out, int w, int h, void *d_scratch, bool compact = false) {
+    const size_t n = (size_t)w * h;
+    const size_t plane = ((n * 4) + 255) & ~(size_t)255;
+    char *base = static_cast<char *>(d_scratch);
+    auto take = [&](int k) { return reinterpret_cast<float *>(base + plane * k); };
+    const int o = compact ? 3 : 0;  // planes 10.. move down by three
+    float *gl = take(0), *gs = take(1), *lrg = take(2), *lbg = take(3), *er = take(4);
+    float *t0 = take(5), *t1 = take(6);
+    float *t2 = compact ? t0 : take(7), *t3 = compact ? t0 : take(8), *t4 = compact ? t0 : take(9);
+    float *bg = take(10 - o), *brg = take(11 - o), *bbg = take(12 - o), *be = take(13 - o), *ber = take(14 - o);
+    float *fs = take(15 - o), *bfs = take(16 - o), *lfs = take(17 - o);
+    float *sr = gl, *sb = gs, *srb = t0, *sbb = t1;  // reuse: the guide planes are dead by then
+    DfImg I{};
+    I.P.src = d_in; I.P.dst = d_out;
+    I.P.lrg = lrg; I.P.lbg = lbg; I.P.bg = bg; I.P.brg = brg; I.P.bbg = bbg; I.P.be = be; I.P.ber = ber;
+    I.P.fs = fs; I.P.bfs = bfs; I.P.lfs = lfs; I.P.sr = sr; I.P.sb = sb; I.P.srb = srb; I.P.sbb = sbb;
+    I.gl = gl;
+    I.er = er;
+    I.stats = reinterpret_cast<unsigned *>(base + plane * (compact ? 15 : 20));
+    I.W = w; I.H = h; I.inplace = d_in == d_out;
+    // the five blurred guide planes (the normalised edge map scaled on load),
+    // the support seeds, the two solo seed planes
+    BlurSet &S5 = I.sets[0];
     S5.count = 5;
     const float *ins[5] = {gl, lrg, lbg, er, er};
     float *tms[5] = {t0, t1, t2, t3, t4};
     float *outs[5] = {bg, brg, bbg, be, ber};
     for (int k = 0; k < 5; k++) { S5.in[k] = ins[k]; S5.tmp[k] = tms[k]; S5.out[k] = outs[k]; S5.scale_by_edge[k] = 0; }
-    S5.scale_by_edge[3] = 1;  // blur of the normalised edge map
-    const bool seed_in_blur = fast && ks <= FB_KSMAX;
-    int rc = blur(S5, seed_in_blur);
-    if (rc) return rc;
-    if (!seed_in_blur) {
-        if (fast) hipLaunchKernelGGL(support<true>, dim3(g), dim3(BLOCK), 0, s, P, C, G);
-        else hipLaunchKernelGGL(support<false>, dim3(g), dim3(BLOCK), 0, s, P, C, G);
-    }
-    BlurSet S1{};
+    S5.scale_by_edge[3] = 1;
+    BlurSet &S1 = I.sets[1];
     S1.count = 1; S1.in[0] = fs; S1.tmp[0] = t2; S1.out[0] = bfs;
-    if ((rc = blur(S1, false))) return rc;
-    const dim3 gl_grid((w + LL_T - 1) / LL_T, (h + LL_T - 1) / LL_T);
-    if (fast) hipLaunchKernelGGL(line_link<true>, gl_grid, dim3(BLOCK), 0, s, P, C, G);
-    else hipLaunchKernelGGL(line_link<false>, gl_grid, dim3(BLOCK), 0, s, P, C, G);
-    BlurSet S2{};
+    BlurSet &S2 = I.sets[2];
     S2.count = 2;
     S2.in[0] = sr; S2.tmp[0] = t3; S2.out[0] = srb;
     S2.in[1] = sb; S2.tmp[1] = t4; S2.out[1] = sbb;
-    if ((rc = blur(S2, false))) return rc;
-    if (fast) hipLaunchKernelGGL(correct<true>, dim3(g), dim3(BLOCK), 0, s, P, C, G, stats);
-    else hipLaunchKernelGGL(correct<false>, dim3(g), dim3(BLOCK), 0, s, P, C, G, stats);
+    return I;
+}
+
+// the fused blur of set `which` over a source (seed: the support seeds in its
+// last plane); fused radii only
+template <class Src>
+int blur_fused(Src src, dim3 grid, int which, bool seed, const DfParams &D, hipStream_t s) {
+    const int ks = D.ks;
+    const size_t lds = sizeof(float) * ((size_t)(FB_TY + 2 * ks) * ((FB_TX + 2 * ks) | 1) +
+                                        (size_t)(FB_TY + 2 * ks) * (FB_TX + 1));
+    switch (ks) {
+#define LRE_BLUR2D(K)                                                                                         \
+    case K:                                                                                                   \
+        if (seed) {                                                                                           \
+            LRE_HIP_CHECK((blur2d_lds_attr<K, true, Src>()));                                                 \
+            hipLaunchKernelGGL((blur2d<K, true, Src>), grid, dim3(FB_BLOCK), lds, s, src, which, D.taps, D.C, \
+                               D.G);                                                                          \
+        } else {                                                                                              \
+            LRE_HIP_CHECK((blur2d_lds_attr<K, false, Src>()));                                                \
+            hipLaunchKernelGGL((blur2d<K, false, Src>), grid, dim3(FB_BLOCK), lds, s, src, which, D.taps,     \
+                               D.C, D.G);                                                                     \
+        }                                                                                                     \
+        break;
+        LRE_BLUR2D(1) LRE_BLUR2D(2) LRE_BLUR2D(3) LRE_BLUR2D(4) LRE_BLUR2D(5) LRE_BLUR2D(6)
+        LRE_BLUR2D(7) LRE_BLUR2D(8) LRE_BLUR2D(9) LRE_BLUR2D(10) LRE_BLUR2D(11)
+        LRE_BLUR2D(12) LRE_BLUR2D(13) LRE_BLUR2D(14) LRE_BLUR2D(15) LRE_BLUR2D(16)
+#undef LRE_BLUR2D
+    default:
+        lre_set_error("defringe: radius %d has no fused blur", ks);
+        return LRE_ERR_ARG;
+    }
+    LRE_HIP_CHECK(hipGetLastError());
+    return LRE_OK;
+}
+
+// Grids of the tiled passes of one w x h image
+dim3 grid_prep(int w, int h) { return dim3((w + PS_TX - 1) / PS_TX, (h + PS_TY - 1) / PS_TY); }
+dim3 grid_blur(int w, int h) { return dim3((w + FB_TX - 1) / FB_TX, (h + FB_TY - 1) / FB_TY); }
+dim3 grid_link(int w, int h) { return dim3((w + LL_T - 1) / LL_T, (h + LL_T - 1) / LL_T); }
+int chunks_of(int w, int h) { return (int)(((size_t)w * h + DF_CHUNK - 1) / DF_CHUNK); }
+
+// several images: statistics reset (one workgroup per image) and max(G)
+// folded over DF_CHUNK-pixel chunks
+__global__ void init_stats_many(lre::Many<DfImg> S) {
+    const lre::TileOf tl = S.tile();
+    if (threadIdx.x < 4) S.args(tl).stats[threadIdx.x] = 0u;
+}
+__global__ __launch_bounds__(BLOCK) void max_green_many(lre::Many<DfImg> S) {
+    const lre::TileOf tl = S.tile();
+    const DfImg &I = S.args(tl);
+    const size_t n = (size_t)I.W * I.H, b = (size_t)tl.tx * DF_CHUNK, e = b + DF_CHUNK < n ? b + DF_CHUNK : n;
+    const float m = lre::green_max_range(I.P.src, b, e, 0.f, [](float v) { return mx(0.f, v); });
+    block_max_to(m, &I.stats[0]);
+}
+
+}  // namespace
+
+int lre_launch_defringe(const float *d_in, float *d_out, int w, int h, float radius,
+                        float edge_threshold, float chroma_threshold, float strength,
+                        int defringe_green, float green_strength_scale, void *d_scratch,
+                        hipStream_t s) {
+    if (const int rc = lre_check_defringe_params(radius)) return rc;
+    const size_t n = (size_t)w * h;
+    const DfParams D = df_params(radius, edge_threshold, chroma_threshold, strength, defringe_green,
+                                 green_strength_scale);
+    const DfImg I = df_image(d_in, d_out, w, h, d_scratch);
+    using One = lre::One<DfImg>;
+    const One one{I};
+    const int ks = D.ks;
+    const Scal C = [&] {
+        Scal c = D.C;
+        c.W = w; c.H = h; c.inplace = I.inplace;
+        return c;
+    }();
+    const int g = grid_for(n);
+    hipLaunchKernelGGL(init_stats, dim3(1), dim3(64), 0, s, I.stats);
+    hipLaunchKernelGGL(max_green, dim3(g), dim3(BLOCK), 0, s, d_in, n, I.stats);
+    hipLaunchKernelGGL(prep_sobel<One>, grid_prep(w, h), dim3(BLOCK), 0, s, one);
+    LRE_HIP_CHECK(hipGetLastError());
+    // blur of a plane set; with seed, also the support seeds (the fused form
+    // exists for the fused radii and the quotient smoothsteps only)
+    auto blur = [&](int which, bool seed) -> int {
+        if (ks <= FB_KSMAX) return blur_fused(one, grid_blur(w, h), which, seed, D, s);
+        const BlurSet &S = I.sets[which];
+        dim3 gh((w + HB_W - 1) / HB_W, h);
+        hipLaunchKernelGGL(blur_h, gh, dim3(BLOCK), 0, s, S, D.taps, w, h, ks, I.stats);
+        dim3 gv((w + VB_W - 1) / VB_W, (h + VB_H - 1) / VB_H);
+        hipLaunchKernelGGL(blur_v, gv, dim3(BLOCK), sizeof(float) * (VB_H + 2 * ks) * VB_W, s, S, D.taps, w, h, ks);
+        LRE_HIP_CHECK(hipGetLastError());
+        return LRE_OK;
+    };
+    const bool seed_in_blur = D.fast && ks <= FB_KSMAX;
+    int rc = blur(0, seed_in_blur);
+    if (rc) return rc;
+    if (!seed_in_blur) {
+        if (D.fast) hipLaunchKernelGGL(support<true>, dim3(g), dim3(BLOCK), 0, s, I.P, C, D.G);
+        else hipLaunchKernelGGL(support<false>, dim3(g), dim3(BLOCK), 0, s, I.P, C, D.G);
+    }
+    if ((rc = blur(1, false))) return rc;
+    if (D.fast) hipLaunchKernelGGL((line_link<true, One>), grid_link(w, h), dim3(BLOCK), 0, s, one, D.C, D.G);
+    else hipLaunchKernelGGL((line_link<false, One>), grid_link(w, h), dim3(BLOCK), 0, s, one, D.C, D.G);
+    if ((rc = blur(2, false))) return rc;
+    const dim3 gc(chunks_of(w, h));
+    if (D.fast) hipLaunchKernelGGL((correct<true, One>), gc, dim3(BLOCK), 0, s, one, D.C, D.G);
+    else hipLaunchKernelGGL((correct<false, One>), gc, dim3(BLOCK), 0, s, one, D.C, D.G);
+    LRE_HIP_CHECK(hipGetLastError());
+    return LRE_OK;
+}
+
+// ---- many images: one launch per pass over all of them ----------------------
+// Six tables over the same per-image arguments, one per workgroup geometry:
+// the statistics reset (one per image), max(G) chunks, the prep / Sobel tiles,
+// the blur tiles, the line-link tiles and the correction chunks.
+namespace {
+using DfTable = lre::ManyTable<DfImg>;
+size_t df_part(int n) { return (DfTable::bytes(n) + 255) & ~(size_t)255; }
+}  // namespace
+
+size_t lre_defringe_multi_table_bytes(int n) { return 6 * df_part(n); }
+
+// the compact plane layout of the multi-image pass
+size_t lre_defringe_multi_scratch_bytes(int w, int h) {
+    return ((((size_t)w * h * 4) + 255) & ~(size_t)255) * 15 + 256;
+}
+
+int lre_defringe_multi_supported(float radius, float edge_threshold, float chroma_threshold, float strength,
+                                 int defringe_green, float green_strength_scale) {
+    if (lre_check_defringe_params(radius) != LRE_OK) return 0;
+    const DfParams D = df_params(radius, edge_threshold, chroma_threshold, strength, defringe_green,
+                                 green_strength_scale);
+    return D.fast && D.ks <= FB_KSMAX;
+}
+
+int lre_defringe_multi_fill(const lre_post_job *jobs, int n, void *h_table, int grids[6]) {
+    auto args = [&](int i) {
+        return df_image(jobs[i].d_in, jobs[i].d_out, jobs[i].w, jobs[i].h, jobs[i].d_scratch, true);
+    };
+    char *h = static_cast<char *>(h_table);
+    auto as2 = [](dim3 g) { return make_int2((int)g.x, (int)g.y); };
+    grids[0] = DfTable::fill(h, n, [](int) { return make_int2(1, 1); }, args);
+    grids[1] = DfTable::fill(h + df_part(n), n, [&](int i) { return make_int2(chunks_of(jobs[i].w, jobs[i].h), 1); },
+                             args);
+    grids[2] = DfTable::fill(h + 2 * df_part(n), n, [&](int i) { return as2(grid_prep(jobs[i].w, jobs[i].h)); }, args);
+    grids[3] = DfTable::fill(h + 3 * df_part(n), n, [&](int i) { return as2(grid_blur(jobs[i].w, jobs[i].h)); }, args);
+    grids[4] = DfTable::fill(h + 4 * df_part(n), n, [&](int i) { return as2(grid_link(jobs[i].w, jobs[i].h)); }, args);
+    grids[5] = DfTable::fill(h + 5 * df_part(n), n, [&](int i) { return make_int2(chunks_of(jobs[i].w, jobs[i].h), 1); },
+                             args);
+    return LRE_OK;
+}
+
+int lre_launch_defringe_multi(const void *d_table, int n, const int grids[6], float radius, float edge_threshold,
+                              float chroma_threshold, float strength, int defringe_green,
+                              float green_strength_scale, hipStream_t s) {
+    if (n == 0) return LRE_OK;
+    if (!lre_defringe_multi_supported(radius, edge_threshold, chroma_threshold, strength, defringe_green,
+                                      green_strength_scale)) {
+        lre_set_error("defringe: these parameters have no multi-image pass");
+        return LRE_ERR_ARG;
+    }
+    const DfParams D = df_params(radius, edge_threshold, chroma_threshold, strength, defringe_green,
+                                 green_strength_scale);
+    const char *d = static_cast<const char *>(d_table);
+    auto view = [&](int k) { return DfTable::view(d + k * df_part(n), n); };
+    using Many = lre::Many<DfImg>;
+    hipLaunchKernelGGL(init_stats_many, dim3(grids[0]), dim3(64), 0, s, view(0));
+    hipLaunchKernelGGL(max_green_many, dim3(grids[1]), dim3(BLOCK), 0, s, view(1));
+    hipLaunchKernelGGL(prep_sobel<Many>, dim3(grids[2]), dim3(BLOCK), 0, s, view(2));
+    LRE_HIP_CHECK(hipGetLastError());
+    int rc;
+    if ((rc = blur_fused(view(3), dim3(grids[3]), 0, true, D, s))) return rc;
+    if ((rc = blur_fused(view(3), dim3(grids[3]), 1, false, D, s))) return rc;
+    hipLaunchKernelGGL((line_link<true, Many>), dim3(grids[4]), dim3(BLOCK), 0, s, view(4), D.C, D.G);
+    if ((rc = blur_fused(view(3), dim3(grids[3]), 2, false, D, s))) return rc;
+    hipLaunchKernelGGL((correct<true, Many>), dim3(grids[5]), dim3(BLOCK), 0, s, view(5), D.C, D.G);
     LRE_HIP_CHECK(hipGetLastError());
     return LRE_OK;
 }

@@ -133,6 +133,17 @@ LRE_DEV float green_max(const float *src, size_t n, float m, G g) {
     return m;
 }
 
+// Maximum over the green samples of pixels [b, e) of an interleaved RGB
+// image, by the threads of one workgroup: m = max(m, g(G)).
+template <class G>
+LRE_DEV float green_max_range(const float *src, size_t b, size_t e, float m, G g) {
+    for (size_t i = b + threadIdx.x; i < e; i += blockDim.x) {
+        const float x = g(src[3 * i + 1]);
+        m = x > m ? x : m;
+    }
+    return m;
+}
+
 // Pixels of the frame of width `border` around a w x h image, enumerated as
 // the full rows at the top, then at the bottom, then the left and right strips
 // of the rows between (on images narrower than two strips a pixel may appear
@@ -160,6 +171,76 @@ LRE_DEV void frame_pixel(int t, int w, int h, int border, int &row, int &col) {
         col = s < strip ? s : w - strip + (s - strip);
     }
 }
+
+// ---- one launch over many images ------------------------------------------
+// A tiled kernel finds its image and tile through a source S: S.tile() gives
+// (image, tx, ty) and S.args(t) the image's arguments.  One<A>: a single
+// image, its arguments passed by value and the tile from the 2D grid.  Many<A>:
+// a flat grid over several images; workgroup b belongs to the last image whose
+// first workgroup first[i] is <= b (binary search; first[n] is the grid size)
+// and takes tile b - first[i] of that image's row-major grid of tiles_x[i]
+// columns.  The per-image arguments and both index arrays live in one device
+// table.
+struct TileOf {
+    int image, tx, ty;
+};
+template <class A>
+struct One {
+    A a;
+    LRE_DEV TileOf tile() const { return {0, (int)blockIdx.x, (int)blockIdx.y}; }
+    LRE_DEV const A &args(const TileOf &) const { return a; }
+};
+template <class A>
+struct Many {
+    const A *a;
+    const int *first;
+    const int *tiles_x;
+    int n;
+    LRE_DEV TileOf tile() const {
+        const int b = (int)blockIdx.x;
+        int lo = 0, hi = n - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (first[mid] <= b) lo = mid;
+            else hi = mid - 1;
+        }
+        const int lt = b - first[lo], tx = tiles_x[lo];
+        return {lo, lt % tx, lt / tx};
+    }
+    LRE_DEV const A &args(const TileOf &t) const { return a[t.image]; }
+};
+
+// Host side of a Many<A> table: the per-image arguments, then first[n + 1] and
+// tiles_x[n], in one buffer.  fill() writes the host copy and returns the grid
+// size; the device copy (same layout) is what view() points into.
+template <class A>
+struct ManyTable {
+    static size_t bytes(int n) {
+        return ((sizeof(A) * (size_t)n + 15) & ~(size_t)15) + sizeof(int) * (2 * (size_t)n + 1);
+    }
+    // tiles(i) -> (tiles_x, tiles_y) of image i; args(i) -> its arguments
+    template <class TilesF, class ArgsF>
+    static int fill(void *h, int n, TilesF tiles, ArgsF args) {
+        A *a = static_cast<A *>(h);
+        int *first = reinterpret_cast<int *>(static_cast<char *>(h) + ((sizeof(A) * (size_t)n + 15) & ~(size_t)15));
+        int *tx = first + n + 1;
+        int total = 0;
+        for (int i = 0; i < n; i++) {
+            a[i] = args(i);
+            const int2 t = tiles(i);
+            first[i] = total;
+            tx[i] = t.x;
+            total += t.x * t.y;
+        }
+        first[n] = total;
+        return total;
+    }
+    static Many<A> view(const void *d, int n) {
+        const char *c = static_cast<const char *>(d);
+        const int *first = reinterpret_cast<const int *>(c + ((sizeof(A) * (size_t)n + 15) & ~(size_t)15));
+        return Many<A>{reinterpret_cast<const A *>(c), first, first + n + 1, n};
+    }
+};
 
 // Tile queue of a persistent grid: thread 0 takes the next tile with one
 // atomic add on *counter (zeroed before the launch) and the workgroup reads it

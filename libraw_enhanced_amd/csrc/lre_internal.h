@@ -70,6 +70,39 @@ struct lre_demosaic_job {
     const float *d_maxv;
 };
 
+// One image of a multi-image post-demosaic pass: its RGB in and out (the
+// same buffer for in place) and its own scratch for the stage.
+struct lre_post_job {
+    const float *d_in;
+    float *d_out;
+    int w, h;
+    void *d_scratch;
+};
+// Multi-image passes: *_multi_table_bytes sizes the host and device table,
+// *_multi_fill writes the host table (grid sizes out), the caller copies it
+// to the device, *_launch_*_multi enqueues one launch per kernel over all
+// images.  Parameters are common to the batch.
+size_t lre_detail_tm_multi_table_bytes(int n);
+int lre_detail_tm_multi_fill(const lre_post_job *jobs, int n, void *h_table, int grids[2]);
+int lre_launch_detail_tm_multi(const void *d_table, int n, const int grids[2], hipStream_t s,
+                               const lre::Epilogue *epi = nullptr);
+
+size_t lre_axial_multi_table_bytes(int n);
+int lre_axial_multi_fill(const lre_post_job *jobs, int n, int radius, float epsilon, float strength,
+                         void *h_table, int grids[3]);
+int lre_launch_axial_multi(const void *d_table, int n, int radius, const int grids[3], hipStream_t s);
+
+size_t lre_defringe_multi_table_bytes(int n);
+size_t lre_defringe_multi_scratch_bytes(int w, int h);
+// 1 when the parameters have a multi-image pass (fused blur radius and the
+// quotient smoothsteps), else the images take the per-image launcher
+int lre_defringe_multi_supported(float radius, float edge_threshold, float chroma_threshold, float strength,
+                                 int defringe_green, float green_strength_scale);
+int lre_defringe_multi_fill(const lre_post_job *jobs, int n, void *h_table, int grids[6]);
+int lre_launch_defringe_multi(const void *d_table, int n, const int grids[6], float radius, float edge_threshold,
+                              float chroma_threshold, float strength, int defringe_green,
+                              float green_strength_scale, hipStream_t s);
+
 // The parameter checks the launchers make (LRE_OK or LRE_ERR_ARG with the
 // message set), for callers that validate a stage list before enqueueing.
 int lre_check_axial_params(int radius);

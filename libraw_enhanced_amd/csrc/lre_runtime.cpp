@@ -318,6 +318,13 @@ struct lre_batch {
     hipGraphExec_t exec = nullptr;
     // tables of the captured graph (kept apart from those of eager runs)
     DevBuf g_tab_a, g_tab_x;
+    // post-demosaic stages: one multi-image pass per stage over groups of
+    // images whose scratch fits post_budget (group mode), or per image over
+    // `lanes` streams
+    int group_mode = 2;      // 0 off, 1 on, 2 by image size
+    size_t post_budget = 0;  // 0: half of the device's memory
+    DevBuf post_scratch, d_tab_post, g_tab_post;
+    HostBuf h_tab_post;
 };
 
 namespace {
@@ -335,6 +342,29 @@ void drop_graph(lre_batch *b) {
     b->graph = nullptr;
 }
 
+// group mode 2 groups batches whose mean image is below this many pixels
+// (measured on the default chain: 256 x 0.3 MP, grouped 1.7x faster than the
+// best lane count; 64 x 3.1 MP and 16 x 12 MP, grouped 2% and 5% slower)
+constexpr size_t GROUP_AUTO_PX = 2000000;
+
+// one multi-image pass of a group: where its table sits and its grid sizes
+struct GroupPass {
+    int stage;               // LRE_STAGE_AXIAL_CA, _DEFRINGE or _DETAIL_TM
+    size_t tab_off = 0;      // into the post table buffer
+    int grids[6] = {0, 0, 0, 0, 0, 0};
+    bool epi = false;        // detail TM with a fused matrix / gamma
+    bool matrix = false, gamma = false;
+};
+// images whose post-demosaic scratch is held at once, run on one lane
+struct Group {
+    int lane = 0;
+    size_t base = 0;                  // the lane's region of the post scratch
+    std::vector<int> imgs;
+    std::vector<size_t> scratch_off;  // per image, from base
+    size_t scratch_bytes = 0;
+    std::vector<GroupPass> passes;
+};
+
 struct Plan {
     std::vector<lre_demosaic_job> bayer, xtrans;
     int tiles_a = 0, tiles_x = 0, slots_a = 0, slots_x = 0;
@@ -342,7 +372,113 @@ struct Plan {
     std::vector<size_t> ingest_off;  // per image offset into the ingest workspace (or SIZE_MAX)
     size_t ingest_bytes = 0;
     size_t lane_scratch = 0;
+    bool grouped = false;            // post stages as multi-image passes
+    std::vector<Group> groups;
+    size_t post_scratch = 0, post_tab_bytes = 0;
 };
+
+// Whether every post-demosaic stage of the list has a multi-image pass (or is
+// a pointwise stage, run per image in the group's stream order).
+bool groupable(const int *stages, int from, int ns, const lre_chain_params &p) {
+    for (int i = from; i < ns; i++) {
+        switch (stages[i]) {
+        case LRE_STAGE_AXIAL_CA: case LRE_STAGE_DETAIL_TM: case LRE_STAGE_TONE_MAPPING:
+        case LRE_STAGE_COLOR_SPACE: case LRE_STAGE_GAMMA:
+            break;
+        case LRE_STAGE_DEFRINGE:
+            if (!lre_defringe_multi_supported(p.defringe_radius, p.defringe_edge_threshold,
+                                              p.defringe_chroma_threshold, p.defringe_strength, p.defringe_green,
+                                              p.defringe_green_scale))
+                return false;
+            break;
+        default:
+            return false;
+        }
+    }
+    return true;
+}
+
+// scratch one image needs in group mode (the largest of its passes)
+size_t group_scratch(const lre_image &im, const int *stages, int from, int ns) {
+    size_t m = 0;
+    for (int i = from; i < ns; i++) {
+        size_t b = 0;
+        switch (stages[i]) {
+        case LRE_STAGE_AXIAL_CA: b = lre_axial_scratch_bytes(im.width, im.height); break;
+        case LRE_STAGE_DEFRINGE: b = lre_defringe_multi_scratch_bytes(im.width, im.height); break;
+        case LRE_STAGE_DETAIL_TM: b = lre_detail_tm_scratch_bytes(im.width, im.height); break;
+        default: break;
+        }
+        m = std::max(m, b);
+    }
+    return align256(m);
+}
+
+size_t device_memory(int device) {
+    size_t free_b = 0, total = 0;
+    (void)hipSetDevice(device);
+    if (hipMemGetInfo(&free_b, &total) != hipSuccess) return (size_t)64 << 30;
+    return total;
+}
+
+// The images are spread over the lanes as in per-image mode (largest first,
+// to the least loaded lane); each lane's images, in image order, form groups
+// as large as the lane's share of the budget allows (an image larger than it
+// forms a group of its own).  Groups of one lane run one after another and
+// share the lane's region of the post scratch.
+void plan_groups(const lre_image *imgs, int n, const int *stages, int from, int ns, size_t budget, int lanes,
+                 Plan &P) {
+    P.groups.clear();
+    P.post_scratch = P.post_tab_bytes = 0;
+    const size_t share = budget / (size_t)lanes;
+    Group g;
+    size_t lane_max = 0;
+    auto close = [&]() {
+        if (g.imgs.empty()) return;
+        const int m = (int)g.imgs.size();
+        for (int si = from; si < ns; si++) {
+            const int st = stages[si];
+            GroupPass gp;
+            gp.stage = st;
+            if (st == LRE_STAGE_DETAIL_TM) {
+                // a colour matrix and/or gamma right after it run in its store
+                int j = si + 1;
+                if (j < ns && stages[j] == LRE_STAGE_COLOR_SPACE) { gp.matrix = true; j++; }
+                if (j < ns && stages[j] == LRE_STAGE_GAMMA) { gp.gamma = true; j++; }
+                gp.epi = gp.matrix || gp.gamma;
+                si = j - 1;
+            }
+            size_t tb = 0;
+            if (st == LRE_STAGE_AXIAL_CA) tb = lre_axial_multi_table_bytes(m);
+            else if (st == LRE_STAGE_DEFRINGE) tb = lre_defringe_multi_table_bytes(m);
+            else if (st == LRE_STAGE_DETAIL_TM) tb = lre_detail_tm_multi_table_bytes(m);
+            gp.tab_off = P.post_tab_bytes;
+            P.post_tab_bytes += align256(tb);
+            g.passes.push_back(gp);
+        }
+        lane_max = std::max(lane_max, g.scratch_bytes);
+        P.groups.push_back(std::move(g));
+        g = Group();
+    };
+    auto per_lane = assign_lanes(imgs, n, lanes);
+    for (int k = 0; k < lanes; k++) {
+        std::sort(per_lane[k].begin(), per_lane[k].end());
+        lane_max = 0;
+        for (int i : per_lane[k]) {
+            const size_t b = group_scratch(imgs[i], stages, from, ns);
+            if (!g.imgs.empty() && g.scratch_bytes + b > share) close();
+            if (g.imgs.empty()) {
+                g.lane = k;
+                g.base = P.post_scratch;
+            }
+            g.imgs.push_back(i);
+            g.scratch_off.push_back(g.scratch_bytes);
+            g.scratch_bytes += b;
+        }
+        close();
+        P.post_scratch += lane_max;
+    }
+}
 
 int make_plan(lre_batch *b, const lre_image *imgs, int n, const int *stages, int ns,
               const lre_chain_params &p, Plan &P) {
@@ -379,6 +515,20 @@ int make_plan(lre_batch *b, const lre_image *imgs, int n, const int *stages, int
         }
         (im.filters == 9 ? P.xtrans : P.bayer).push_back(j);
     }
+    const int from = P.demosaic ? 1 : 0;
+    // By image size: one launch per stage over many images pays where images
+    // are small (launch and tail bound); large images fill the device alone
+    // and keep their stages' data warm in the caches when each one runs its
+    // whole chain before the next on a lane.
+    size_t px = 0;
+    for (int i = 0; i < n; i++) px += (size_t)imgs[i].width * imgs[i].height;
+    const bool want = b->group_mode == 1 || (b->group_mode == 2 && n > 0 && px / (size_t)n < GROUP_AUTO_PX);
+    P.grouped = want && ns > from && n > 0 && groupable(stages, from, ns, p);
+    if (P.grouped) {
+        const size_t budget = b->post_budget ? b->post_budget : device_memory(b->device) / 2;
+        plan_groups(imgs, n, stages, from, ns, budget, b->lanes, P);
+        P.lane_scratch = 0;
+    }
     for (auto &j : P.bayer) P.tiles_a += lre_amaze_tiles(j.w, j.h);
     for (auto &j : P.xtrans) P.tiles_x += lre_xtrans_tiles(j.w, j.h);
     P.slots_a = std::max(1, std::min(P.tiles_a, b->num_cus * b->amaze_per_cu));
@@ -395,6 +545,8 @@ int ensure_buffers(lre_batch *b, const Plan &P, int n) {
                                lre_amaze_multi_table_bytes(std::max(n, 1)) > b->d_tab_a.bytes ||
                                lre_xtrans_multi_table_bytes(std::max(n, 1)) > b->d_tab_x.bytes);
     for (int k = 0; k < b->lanes; k++) grow = grow || P.lane_scratch > b->lane[k].scratch.bytes;
+    grow = grow || (P.grouped && (P.post_scratch > b->post_scratch.bytes || P.post_tab_bytes > b->d_tab_post.bytes ||
+                                  P.post_tab_bytes > b->h_tab_post.bytes));
     if (!grow) return LRE_OK;
     int rc = batch_sync_all(b);
     if (rc) return rc;
@@ -403,6 +555,11 @@ int ensure_buffers(lre_batch *b, const Plan &P, int n) {
     drop_graph(b);
     for (int k = 0; k < b->lanes; k++)
         if ((rc = b->lane[k].scratch.ensure(P.lane_scratch))) return rc;
+    if (P.grouped) {
+        if ((rc = b->post_scratch.ensure(P.post_scratch))) return rc;
+        if ((rc = b->d_tab_post.ensure(P.post_tab_bytes))) return rc;
+        if ((rc = b->h_tab_post.ensure(P.post_tab_bytes))) return rc;
+    }
     if (P.demosaic) {
         if ((rc = b->amaze_arena.ensure(need_a))) return rc;
         if ((rc = b->xtrans_arena.ensure(need_x))) return rc;
@@ -416,15 +573,15 @@ int ensure_buffers(lre_batch *b, const Plan &P, int n) {
 }
 
 // Fill the per-image tables and copy them to d_a / d_x (stream-ordered on lane 0).
-int upload_tables(lre_batch *b, Plan &P, const lre_image *imgs, int n, void *d_a, void *d_x,
-                  bool synchronous) {
+int upload_tables(lre_batch *b, Plan &P, const lre_image *imgs, int n, void *d_a, void *d_x, void *d_post,
+                  const lre_chain_params *params, bool synchronous) {
     // the pinned tables may still be read by the previous upload
     if (b->tables_pending) LRE_HIP_CHECK(hipEventSynchronize(b->ev_tables));
     b->tables_pending = false;
     // point u16 images at their workspace planes
     char *ws = static_cast<char *>(b->ingest_ws.p);
     size_t ib = 0, ix = 0;
-    for (int i = 0; i < n; i++) {
+    for (int i = 0; i < n && P.demosaic; i++) {
         lre_demosaic_job &j = imgs[i].filters == 9 ? P.xtrans[ix++] : P.bayer[ib++];
         if (P.ingest_off[i] == SIZE_MAX) continue;
         // the float CFA plane, then {data maximum, demosaic maximum}
@@ -432,16 +589,45 @@ int upload_tables(lre_batch *b, Plan &P, const lre_image *imgs, int n, void *d_a
         j.d_maxv = reinterpret_cast<float *>(ws + P.ingest_off[i] + align256((size_t)j.w * j.h * 4)) + 1;
     }
     hipStream_t s = b->lane[0].stream;
-    if (!P.bayer.empty()) {
+    if (P.demosaic && !P.bayer.empty()) {
         lre_amaze_multi_fill(P.bayer.data(), (int)P.bayer.size(), b->h_tab_a.p);
         LRE_HIP_CHECK(hipMemcpyAsync(d_a, b->h_tab_a.p, lre_amaze_multi_table_bytes((int)P.bayer.size()),
                                      hipMemcpyHostToDevice, s));
     }
-    if (!P.xtrans.empty()) {
+    if (P.demosaic && !P.xtrans.empty()) {
         int t = lre_xtrans_multi_fill(P.xtrans.data(), (int)P.xtrans.size(), b->h_tab_x.p);
         if (t < 0) return t;
         LRE_HIP_CHECK(hipMemcpyAsync(d_x, b->h_tab_x.p, lre_xtrans_multi_table_bytes((int)P.xtrans.size()),
                                      hipMemcpyHostToDevice, s));
+    }
+    if (P.grouped && P.post_tab_bytes) {
+        if (!d_post) {
+            lre_set_error("no device buffer for the post-stage tables");
+            return LRE_ERR_ARG;
+        }
+        char *h = static_cast<char *>(b->h_tab_post.p);
+        char *scr = static_cast<char *>(b->post_scratch.p);
+        const lre_chain_params &p = *params;
+        for (Group &g : P.groups) {
+            std::vector<lre_post_job> jobs(g.imgs.size());
+            for (size_t k = 0; k < g.imgs.size(); k++) {
+                const lre_image &im = imgs[g.imgs[k]];
+                jobs[k] = lre_post_job{im.d_rgb, im.d_rgb, im.width, im.height, scr + g.base + g.scratch_off[k]};
+            }
+            const int m = (int)jobs.size();
+            for (GroupPass &gp : g.passes) {
+                int rc = LRE_OK;
+                if (gp.stage == LRE_STAGE_AXIAL_CA)
+                    rc = lre_axial_multi_fill(jobs.data(), m, p.axial_radius, p.axial_epsilon, p.axial_strength,
+                                              h + gp.tab_off, gp.grids);
+                else if (gp.stage == LRE_STAGE_DEFRINGE)
+                    rc = lre_defringe_multi_fill(jobs.data(), m, h + gp.tab_off, gp.grids);
+                else if (gp.stage == LRE_STAGE_DETAIL_TM)
+                    rc = lre_detail_tm_multi_fill(jobs.data(), m, h + gp.tab_off, gp.grids);
+                if (rc) return rc;
+            }
+        }
+        LRE_HIP_CHECK(hipMemcpyAsync(d_post, b->h_tab_post.p, P.post_tab_bytes, hipMemcpyHostToDevice, s));
     }
     LRE_HIP_CHECK(hipEventRecord(b->ev_tables, s));
     b->tables_pending = true;
@@ -453,8 +639,58 @@ int upload_tables(lre_batch *b, Plan &P, const lre_image *imgs, int n, void *d_a
 }
 
 // Enqueue the whole batch.  Tables must already be on the device (d_a, d_x).
+// the post-demosaic stages of group mode: per group, on its lane's stream,
+// one multi-image pass per stage (pointwise stages per image)
+int enqueue_groups(const Plan &P, const lre_image *imgs, const lre_chain_params &p, const void *d_post,
+                   const Lane *L) {
+    const char *d = static_cast<const char *>(d_post);
+    for (const Group &g : P.groups) {
+        const int m = (int)g.imgs.size();
+        hipStream_t s = L[g.lane].stream;
+        for (const GroupPass &gp : g.passes) {
+            int rc = LRE_OK;
+            switch (gp.stage) {
+            case LRE_STAGE_AXIAL_CA:
+                rc = lre_launch_axial_multi(d + gp.tab_off, m, p.axial_radius, gp.grids, s);
+                break;
+            case LRE_STAGE_DEFRINGE:
+                rc = lre_launch_defringe_multi(d + gp.tab_off, m, gp.grids, p.defringe_radius,
+                                               p.defringe_edge_threshold, p.defringe_chroma_threshold,
+                                               p.defringe_strength, p.defringe_green, p.defringe_green_scale, s);
+                break;
+            case LRE_STAGE_DETAIL_TM: {
+                const lre::Epilogue E = lre_make_epilogue(gp.matrix ? p.color_transform : nullptr, p.gamma_power,
+                                                          p.gamma_slope, gp.gamma ? p.output_color_space : -1);
+                rc = lre_launch_detail_tm_multi(d + gp.tab_off, m, gp.grids, s, gp.epi ? &E : nullptr);
+                break;
+            }
+            default:
+                // pointwise stages: per image (a matrix followed by gamma shares one pass,
+                // as in the per-image chain)
+                for (int i : g.imgs) {
+                    const lre_image &im = imgs[i];
+                    const size_t npx = (size_t)im.width * im.height;
+                    if (gp.stage == LRE_STAGE_COLOR_SPACE && &gp + 1 < g.passes.data() + g.passes.size() &&
+                        (&gp + 1)->stage == LRE_STAGE_GAMMA)
+                        rc = lre_launch_epilogue(im.d_rgb, im.d_rgb, npx, p.color_transform, p.gamma_power,
+                                                 p.gamma_slope, p.output_color_space, 32, s);
+                    else if (gp.stage == LRE_STAGE_GAMMA && &gp > g.passes.data() &&
+                             (&gp - 1)->stage == LRE_STAGE_COLOR_SPACE)
+                        rc = LRE_OK;  // done with the matrix
+                    else
+                        rc = run_post_stage(gp.stage, im, p, nullptr, s);
+                    if (rc) return rc;
+                }
+                break;
+            }
+            if (rc) return rc;
+        }
+    }
+    return LRE_OK;
+}
+
 int enqueue(lre_batch *b, const Plan &P, const lre_image *imgs, int n, const int *stages, int ns,
-            const lre_chain_params &p, const void *d_a, void *d_x) {
+            const lre_chain_params &p, const void *d_a, void *d_x, const void *d_post) {
     Lane *L = b->lane;
     const int lanes = b->lanes;
     hipStream_t s0 = L[0].stream;
@@ -500,7 +736,9 @@ int enqueue(lre_batch *b, const Plan &P, const lre_image *imgs, int n, const int
         }
     }
     const int first = P.demosaic ? 1 : 0;
-    if (ns > first) {
+    if (P.grouped) {
+        if ((rc = enqueue_groups(P, imgs, p, d_post, L))) return rc;
+    } else if (ns > first) {
         auto plan = assign_lanes(imgs, n, lanes);
         for (int k = 0; k < lanes; k++)
             for (int i : plan[k])
@@ -523,8 +761,10 @@ int batch_run_eager(lre_batch *b, const lre_image *imgs, int n, const int *stage
     int rc = make_plan(b, imgs, n, stages, ns, p, P);
     if (rc) return rc;
     if ((rc = ensure_buffers(b, P, n))) return rc;
-    if (P.demosaic && (rc = upload_tables(b, P, imgs, n, b->d_tab_a.p, b->d_tab_x.p, false))) return rc;
-    return enqueue(b, P, imgs, n, stages, ns, p, b->d_tab_a.p, b->d_tab_x.p);
+    if ((P.demosaic || P.grouped) &&
+        (rc = upload_tables(b, P, imgs, n, b->d_tab_a.p, b->d_tab_x.p, b->d_tab_post.p, &p, false)))
+        return rc;
+    return enqueue(b, P, imgs, n, stages, ns, p, b->d_tab_a.p, b->d_tab_x.p, b->d_tab_post.p);
 }
 
 }  // namespace
@@ -594,10 +834,11 @@ int lre_hip_batch_destroy(lre_batch *b) {
     drop_graph(b);
     for (int k = 0; k < MAX_LANES; k++) lane_close(b->lane[k]);
     DevBuf *bufs[] = {&b->amaze_arena, &b->xtrans_arena, &b->ingest_ws, &b->d_tab_a, &b->d_tab_x,
-                      &b->g_tab_a, &b->g_tab_x};
+                      &b->g_tab_a, &b->g_tab_x, &b->post_scratch, &b->d_tab_post, &b->g_tab_post};
     for (auto *d : bufs) d->release();
     b->h_tab_a.release();
     b->h_tab_x.release();
+    b->h_tab_post.release();
     hipEvent_t evs[] = {b->ev_fork, b->ev_ingest, b->ev_demosaic[0], b->ev_demosaic[1], b->ev_tables};
     for (auto e : evs)
         if (e) (void)hipEventDestroy(e);
@@ -632,15 +873,16 @@ int lre_hip_batch_capture(lre_batch *b, const lre_image *imgs, int n, const int 
     Plan P;
     if ((rc = make_plan(b, imgs, n, stages, ns, p, P))) return rc;
     if ((rc = ensure_buffers(b, P, n))) return rc;
-    if (P.demosaic) {
+    if (P.demosaic || P.grouped) {
         if ((rc = b->g_tab_a.ensure(lre_amaze_multi_table_bytes(std::max(n, 1))))) return rc;
         if ((rc = b->g_tab_x.ensure(lre_xtrans_multi_table_bytes(std::max(n, 1))))) return rc;
+        if (P.grouped && (rc = b->g_tab_post.ensure(std::max<size_t>(P.post_tab_bytes, 256)))) return rc;
         // the graph's own tables, uploaded now and read by every replay
-        if ((rc = upload_tables(b, P, imgs, n, b->g_tab_a.p, b->g_tab_x.p, true))) return rc;
+        if ((rc = upload_tables(b, P, imgs, n, b->g_tab_a.p, b->g_tab_x.p, b->g_tab_post.p, &p, true))) return rc;
     }
     hipStream_t s0 = b->lane[0].stream;
     LRE_HIP_CHECK(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
-    rc = enqueue(b, P, imgs, n, stages, ns, p, b->g_tab_a.p, b->g_tab_x.p);
+    rc = enqueue(b, P, imgs, n, stages, ns, p, b->g_tab_a.p, b->g_tab_x.p, b->g_tab_post.p);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(s0, &g);
     if (rc) {
@@ -687,6 +929,27 @@ int lre_hip_batch_set_config(lre_batch *b, int lanes, int amaze_per_cu, int xtra
     return LRE_OK;
 }
 
+int lre_hip_batch_set_grouping(lre_batch *b, int group_mode, size_t post_budget_bytes) {
+    if (!b) return LRE_ERR_ARG;
+    if (group_mode < 0 || group_mode > 2) {
+        lre_set_error("group_mode must be 0, 1 or 2");
+        return LRE_ERR_ARG;
+    }
+    int rc = batch_sync_all(b);
+    if (rc) return rc;
+    drop_graph(b);
+    b->group_mode = group_mode;
+    b->post_budget = post_budget_bytes;
+    return LRE_OK;
+}
+
+int lre_hip_batch_get_grouping(lre_batch *b, int *group_mode, size_t *post_budget_bytes) {
+    if (!b) return LRE_ERR_ARG;
+    if (group_mode) *group_mode = b->group_mode;
+    if (post_budget_bytes) *post_budget_bytes = b->post_budget;
+    return LRE_OK;
+}
+
 int lre_hip_batch_get_config(lre_batch *b, int *lanes, int *amaze_per_cu, int *xtrans_per_cu) {
     if (!b) return LRE_ERR_ARG;
     if (lanes) *lanes = b->lanes;
@@ -700,18 +963,20 @@ int lre_hip_batch_autotune(lre_batch *b, const lre_image *imgs, int n, const int
     if (!b) return LRE_ERR_ARG;
     LRE_HIP_CHECK(hipSetDevice(b->device));
     if (reps < 1) reps = 1;
-    struct Cand { int lanes, a, x; };
+    struct Cand { int group, lanes, a, x; };
     std::vector<Cand> cands;
-    for (int lanes : {1, 2, 3, 4})
-        for (int a : {1, 2, 4})
-            cands.push_back({lanes, a, 2 * a});
+    for (int group : {1, 0})
+        for (int lanes : {1, 2, 3, 4})
+            for (int a : {1, 2, 4}) cands.push_back({group, lanes, a, 2 * a});
     hipEvent_t e0, e1;
     LRE_HIP_CHECK(hipEventCreate(&e0));
     LRE_HIP_CHECK(hipEventCreate(&e1));
     float best = 1e30f;
-    Cand pick = {b->lanes, b->amaze_per_cu, b->xtrans_per_cu};
+    Cand pick = {b->group_mode, b->lanes, b->amaze_per_cu, b->xtrans_per_cu};
+    const size_t budget = b->post_budget;
     int rc = LRE_OK;
     for (const Cand &c : cands) {
+        if ((rc = lre_hip_batch_set_grouping(b, c.group, budget))) break;
         if ((rc = lre_hip_batch_set_config(b, c.lanes, c.a, c.x))) break;
         // warm run (sizes buffers), then timed runs
         if ((rc = batch_run_eager(b, imgs, n, stages, ns, params))) break;
@@ -735,6 +1000,7 @@ int lre_hip_batch_autotune(lre_batch *b, const lre_image *imgs, int n, const int
     (void)hipEventDestroy(e1);
     if (rc) return rc;
     if (best_ms) *best_ms = best;
+    if ((rc = lre_hip_batch_set_grouping(b, pick.group, budget))) return rc;
     return lre_hip_batch_set_config(b, pick.lanes, pick.a, pick.x);
 }
 

@@ -44,6 +44,14 @@ __global__ void aces_kernel(const float4 *__restrict__ in, float4 *__restrict__ 
 constexpr int DR = 12;
 constexpr float DTM_DARK = 0.5499f;
 
+// one image of a detail tone map launch
+struct DtmImg {
+    const float *guide;
+    const float *in;
+    float *out;
+    int W, H;
+};
+
 // Geometry of an output tile of side T: the guide window (side T + 4 DR), the
 // mean / a / b window (side T + 2 DR), odd LDS pitches, and LDS in floats: a
 // and b first (the guide window shares their space; it is dead when they are
@@ -104,6 +112,19 @@ LRE_DEV void box_run2(V val, int p0, int len, F emit) {
     }
 }
 
+// the guide planes of several images, one chunk of DTM_CHUNK pixels per workgroup
+constexpr int DTM_CHUNK = 256 * 16;
+__global__ __launch_bounds__(256) void dtm_guide_many(lre::Many<DtmImg> src) {
+    const lre::TileOf tl = src.tile();
+    const DtmImg &I = src.args(tl);
+    const size_t n = (size_t)I.W * I.H, b = (size_t)tl.tx * DTM_CHUNK;
+    const size_t e = b + DTM_CHUNK < n ? b + DTM_CHUNK : n;
+    for (size_t t = b + threadIdx.x; t < e; t += 256) {
+        const float *p = I.in + 3 * t;
+        const_cast<float *>(I.guide)[t] = lre::max_std(0.0f, lre::max_std(p[0], lre::max_std(p[1], p[2])));
+    }
+}
+
 // n4: pixel quads read as three float4 (0 when the image is not 16-byte aligned)
 __global__ void dtm_guide(const float *__restrict__ rgb, float *__restrict__ guide, size_t n, size_t n4) {
     const float4 *v = reinterpret_cast<const float4 *>(rgb);
@@ -120,9 +141,10 @@ __global__ void dtm_guide(const float *__restrict__ rgb, float *__restrict__ gui
 
 // EPI: the colour matrix and gamma stages that follow in a stage list run on
 // the result before it is stored (the same float operations as their own pass).
-template <int T, int NT, bool EPI>
-__global__ __launch_bounds__(NT) void dtm_tile(const float *__restrict__ guide, const float *in, float *out,
-                                               int W, int H, lre::Epilogue E) {
+// Src: lre::One<DtmImg> (one image, 2D grid) or lre::Many<DtmImg> (a flat grid
+// over the tiles of several images)
+template <int T, int NT, bool EPI, class Src>
+__global__ __launch_bounds__(NT) void dtm_tile(Src src, lre::Epilogue E) {
     using D = DtmGeo<T>;
     constexpr int G = D::G, M = D::M, GP = D::GP, MP = D::MP, TP = D::TP;
     extern __shared__ float dsm[];
@@ -130,7 +152,13 @@ __global__ __launch_bounds__(NT) void dtm_tile(const float *__restrict__ guide, 
     float *pa = dsm, *pb = dsm + M * MP;           // M x M a and b (pitch MP)
     float *hg = dsm + D::L_AB, *hq = hg + G * MP;  // G x M row means of g and g*g
     float *ha = dsm + D::L_AB, *hb = ha + M * TP;  // M x T row means of a and b
-    const int x0 = blockIdx.x * T, y0 = blockIdx.y * T;
+    const lre::TileOf tl = src.tile();
+    const DtmImg &I = src.args(tl);
+    const float *__restrict__ guide = I.guide;
+    const float *in = I.in;
+    float *out = I.out;
+    const int W = I.W, H = I.H;
+    const int x0 = tl.tx * T, y0 = tl.ty * T;
     const int tid = threadIdx.x;
     bool dark = true;
     lre::stage2d<NT>(
@@ -267,24 +295,21 @@ __global__ void rgb3_to_cfa_kernel(const float *__restrict__ rgb3, float *__rest
 // (one workgroup per CU, 142 KiB of LDS) measured faster on 24 MP than 32 x 32
 // tiles of 512 (two per CU): 0.44 against 0.54 ms, the smaller tiles redoing
 // half again as many box means in their rings.
-template <int T, int NT>
-int dtm_launch(const float *guide, const float *d_in, float *d_out, int w, int h, hipStream_t s,
-               const lre::Epilogue *epi) {
+template <int T, int NT, class Src>
+int dtm_launch(Src src, dim3 grid, hipStream_t s, const lre::Epilogue *epi) {
     static const hipError_t attr = [] {
-        hipError_t e = hipFuncSetAttribute((const void *)dtm_tile<T, NT, false>,
+        hipError_t e = hipFuncSetAttribute((const void *)dtm_tile<T, NT, false, Src>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)DtmGeo<T>::LDS);
         if (e == hipSuccess)
-            e = hipFuncSetAttribute((const void *)dtm_tile<T, NT, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)DtmGeo<T>::LDS);
+            e = hipFuncSetAttribute((const void *)dtm_tile<T, NT, true, Src>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)DtmGeo<T>::LDS);
         return e;
     }();
     LRE_HIP_CHECK(attr);
-    dim3 grid((w + T - 1) / T, (h + T - 1) / T);
     if (epi && epi->active())
-        hipLaunchKernelGGL((dtm_tile<T, NT, true>), grid, dim3(NT), DtmGeo<T>::LDS, s, guide, d_in, d_out, w, h, *epi);
+        hipLaunchKernelGGL((dtm_tile<T, NT, true, Src>), grid, dim3(NT), DtmGeo<T>::LDS, s, src, *epi);
     else
-        hipLaunchKernelGGL((dtm_tile<T, NT, false>), grid, dim3(NT), DtmGeo<T>::LDS, s, guide, d_in, d_out, w, h,
-                           lre::Epilogue{});
+        hipLaunchKernelGGL((dtm_tile<T, NT, false, Src>), grid, dim3(NT), DtmGeo<T>::LDS, s, src, lre::Epilogue{});
     LRE_HIP_CHECK(hipGetLastError());
     return LRE_OK;
 }
@@ -453,7 +478,38 @@ int lre_launch_detail_tonemap(const float *d_in, float *d_out, int w, int h, voi
     const size_t n = (size_t)w * h;
     const size_t n4 = (reinterpret_cast<uintptr_t>(d_in) % 16 == 0) ? n / 4 : 0;
     hipLaunchKernelGGL(dtm_guide, dim3(grid_for(n4 > 0 ? n4 : n, 256)), dim3(256), 0, s, d_in, guide, n, n4);
-    return dtm_launch<64, 1024>(guide, d_in, d_out, w, h, s, epi);
+    constexpr int T = 64;
+    const lre::One<DtmImg> one{DtmImg{guide, d_in, d_out, w, h}};
+    return dtm_launch<T, 1024>(one, dim3((w + T - 1) / T, (h + T - 1) / T), s, epi);
+}
+
+// Many images: two tables (guide chunks, then tiles) over the same per-image
+// arguments, each guide plane at the start of the image's scratch.
+using DtmTable = lre::ManyTable<DtmImg>;
+size_t lre_detail_tm_multi_table_bytes(int n) { return 2 * ((DtmTable::bytes(n) + 255) & ~(size_t)255); }
+
+int lre_detail_tm_multi_fill(const lre_post_job *jobs, int n, void *h_table, int grids[2]) {
+    const size_t half = (DtmTable::bytes(n) + 255) & ~(size_t)255;
+    auto args = [&](int i) {
+        return DtmImg{static_cast<const float *>(jobs[i].d_scratch), jobs[i].d_in, jobs[i].d_out, jobs[i].w, jobs[i].h};
+    };
+    grids[0] = DtmTable::fill(h_table, n, [&](int i) {
+        return make_int2((int)(((size_t)jobs[i].w * jobs[i].h + DTM_CHUNK - 1) / DTM_CHUNK), 1);
+    }, args);
+    grids[1] = DtmTable::fill(static_cast<char *>(h_table) + half, n, [&](int i) {
+        return make_int2((jobs[i].w + 63) / 64, (jobs[i].h + 63) / 64);
+    }, args);
+    return LRE_OK;
+}
+
+int lre_launch_detail_tm_multi(const void *d_table, int n, const int grids[2], hipStream_t s,
+                               const lre::Epilogue *epi) {
+    if (n == 0 || grids[1] == 0) return LRE_OK;
+    const size_t half = (DtmTable::bytes(n) + 255) & ~(size_t)255;
+    hipLaunchKernelGGL(dtm_guide_many, dim3(grids[0]), dim3(256), 0, s, DtmTable::view(d_table, n));
+    LRE_HIP_CHECK(hipGetLastError());
+    return dtm_launch<64, 1024>(DtmTable::view(static_cast<const char *>(d_table) + half, n), dim3(grids[1]), s,
+                                epi);
 }
 
 int lre_launch_white_balance(const uint16_t *d_img4, float *d_rgb, int w, int h,

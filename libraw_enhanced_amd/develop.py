@@ -99,6 +99,8 @@ _SIGS = {
     "lre_hip_batch_replay": ([_vp], _i),
     "lre_hip_batch_set_config": ([_vp, _i, _i, _i], _i),
     "lre_hip_batch_get_config": ([_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)], _i),
+    "lre_hip_batch_set_grouping": ([_vp, _i, ctypes.c_size_t], _i),
+    "lre_hip_batch_get_grouping": ([_vp, ctypes.POINTER(_i), ctypes.POINTER(ctypes.c_size_t)], _i),
     "lre_hip_batch_autotune": ([_vp, ctypes.POINTER(Image), _i, ctypes.POINTER(_i), _i,
                                 ctypes.POINTER(ChainParams), _i, ctypes.POINTER(_f)], _i),
     "lre_hip_serve_create": ([ctypes.POINTER(_vp), _i, _i, ctypes.c_size_t, ctypes.POINTER(_i), _i,
@@ -194,6 +196,21 @@ class BatchDeveloper:
         a, b, c = _i(), _i(), _i()
         _check(lib().lre_hip_batch_get_config(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
         return {"lanes": a.value, "amaze_slots_per_cu": b.value, "xtrans_slots_per_cu": c.value}
+
+    GROUP_MODES = {False: 0, True: 1, "auto": 2}
+
+    def set_grouping(self, group_mode="auto", post_budget_bytes=0):
+        """Group mode (True, False or "auto": on when the mean image is below
+        2 MP): each post-demosaic stage as one launch over a group of images
+        whose scratch fits post_budget_bytes (0: half the device)."""
+        mode = self.GROUP_MODES[group_mode if isinstance(group_mode, str) else bool(group_mode)]
+        _check(lib().lre_hip_batch_set_grouping(self._h, mode, int(post_budget_bytes)))
+
+    def grouping(self):
+        g, b = _i(), ctypes.c_size_t()
+        _check(lib().lre_hip_batch_get_grouping(self._h, ctypes.byref(g), ctypes.byref(b)))
+        mode = {0: False, 1: True, 2: "auto"}[g.value]
+        return {"group_mode": mode, "post_budget_bytes": b.value}
 
     def autotune(self, images, stages=BATCH_CHAIN, params=None, reps=1):
         arr, n, st, ns, p = self._args(images, stages, params)

@@ -167,6 +167,86 @@ def test_batch_matches_per_image(acc, lanes):
     b.close()
 
 
+@pytest.mark.parametrize("group,budget", [(False, 0), (True, 0), (True, 1), ("auto", 0)])
+def test_batch_grouping_modes(acc, group, budget):
+    """Group mode (one launch per stage over a group), with the default budget
+    (one group) and a 1-byte budget (one image per group), and per-image lanes:
+    all bit-identical to the per-image entry points, also for a stage list
+    without the demosaic run over images already in RGB."""
+    from libraw_enhanced_amd import develop as dv
+    print(f"grouping group={group} budget={budget}", flush=True)
+    p = dv.ChainParams.defaults(defringe_green=1)
+    S = _Set(acc)
+    b = dv.BatchDeveloper(0, 2)
+    assert b.grouping() == {"group_mode": "auto", "post_budget_bytes": 0}
+    b.set_grouping(group, budget)
+    assert b.grouping() == {"group_mode": group, "post_budget_bytes": budget}
+    stages = ("demosaic", "axial_ca", "defringe", "detail_tm", "color_space", "gamma")
+    b.run(S.images(), stages, p)
+    b.synchronize()
+    got = S.results()
+    for it, g in zip(S.items, got):
+        want = _single(acc, it, stages, p)
+        bad = int(np.count_nonzero(g != want))
+        assert bad == 0, f"{it['kind']} {it['h']}x{it['w']} {it['inp']}: {bad} values differ"
+    # post stages only, in place over the demosaicked images
+    post = ("defringe", "axial_ca", "tone_mapping", "detail_tm")
+    b.run(S.images(), ("demosaic",), p)
+    b.synchronize()
+    base = S.results()
+    b.run(S.images(), post, p)
+    b.synchronize()
+    from libraw_enhanced_amd.accelerator import native
+    for it, x, g in zip(S.items, base, S.results()):
+        h, w = it["h"], it["w"]
+        d = acc.dev_from_host(np.ascontiguousarray(x))
+        lib, ctx = native(), acc.handle
+        for st in post:
+            if st == "defringe":
+                rc = lib.lre_hip_dev_defringe(ctx, d.ptr, d.ptr, w, h, p.defringe_radius,
+                                              p.defringe_edge_threshold, p.defringe_chroma_threshold,
+                                              p.defringe_strength, p.defringe_green, p.defringe_green_scale, None)
+            elif st == "axial_ca":
+                rc = lib.lre_hip_dev_axial_ca(ctx, d.ptr, d.ptr, w, h, p.axial_radius, p.axial_epsilon,
+                                              p.axial_strength, None)
+            elif st == "tone_mapping":
+                rc = lib.lre_hip_dev_tone_mapping(ctx, d.ptr, d.ptr, w, h, p.tone_after_scale, None)
+            else:
+                rc = lib.lre_hip_dev_detail_tonemap(ctx, d.ptr, d.ptr, w, h, None)
+            assert rc == 0, st
+        acc.synchronize()
+        want = d.download((h, w, 3))
+        assert np.array_equal(g, want), (post, it["kind"], it["h"], it["w"])
+    # a graph of the grouped batch replays to the same bits
+    b.capture(S.images(), stages, p)
+    for it in S.items:
+        it["d_rgb"].upload(np.full((it["h"], it["w"], 3), np.nan, np.float32))
+    acc.synchronize()
+    b.replay()
+    b.synchronize()
+    for e, g in zip(got, S.results()):
+        assert np.array_equal(e, g)
+    b.close()
+
+
+@pytest.mark.parametrize("kw", [dict(defringe_edge_threshold=1e-7), dict(defringe_radius=20.0)])
+def test_batch_defringe_without_group_pass(acc, kw):
+    """Defringe parameters outside the multi-image pass's range (a ramp width
+    below 2^-20; a blur of more than 16 taps) fall back to per-image lanes,
+    same bits."""
+    from libraw_enhanced_amd import develop as dv
+    print(f"defringe fallback {kw}", flush=True)
+    p = dv.ChainParams.defaults(**kw)
+    S = _Set(acc, _spec()[:2])
+    b = dv.BatchDeveloper(0, 2)
+    stages = ("demosaic", "defringe", "detail_tm")
+    b.run(S.images(), stages, p)
+    b.synchronize()
+    for it, g in zip(S.items, S.results()):
+        assert np.array_equal(g, _single(acc, it, stages, p)), it["kind"]
+    b.close()
+
+
 def test_batch_other_stage_lists(acc):
     from libraw_enhanced_amd import develop as dv
     p = dv.ChainParams.defaults(defringe_green=1, axial_radius=3, output_color_space=2)
