@@ -196,8 +196,9 @@ struct Many {
     const int *first;
     const int *tiles_x;
     int n;
-    LRE_DEV TileOf tile() const {
-        const int b = (int)blockIdx.x;
+    LRE_DEV TileOf tile() const { return tile_at((int)blockIdx.x); }
+    // tile b of the flat list (persistent grids walk it themselves)
+    LRE_DEV TileOf tile_at(int b) const {
         int lo = 0, hi = n - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;

@@ -56,13 +56,18 @@ struct DtmImg {
 // mean / a / b window (side T + 2 DR), odd LDS pitches, and LDS in floats: a
 // and b first (the guide window shares their space; it is dead when they are
 // written), then the row means of g and g*g, whose space later holds the row
-// means of a and b.  Each thread sums runs of consecutive outputs: a line of
-// M = T + 2 DR outputs in 8 runs, a line of T outputs in 8 runs.
+// means of a and b.  Each thread sums runs of consecutive outputs: the row
+// passes a line of M = T + 2 DR outputs in 8 runs and a line of T outputs in 8
+// runs, the column passes runs of RUNC and RUND outputs.
 template <int T>
 struct DtmGeo {
     static constexpr int G = T + 4 * DR, M = T + 2 * DR;
     static constexpr int GP = G + 1, MP = M + 1, TP = T + 1;
     static constexpr int RUN1 = M / 8, RUN2 = T / 8;
+    // the column passes: pass 2 sums NC runs of RUNC rows per column, pass 4 ND
+    // runs of RUND (shorter runs for more threads measured no faster)
+    static constexpr int RUNC = RUN1, NC = 8, RUND = RUN2, ND = 8;
+    static_assert(NC * RUNC == M && ND * RUND == T, "column runs tile the windows");
     static constexpr int L_AB = 2 * M * MP > G * GP ? 2 * M * MP : G * GP, L_H = 2 * G * MP;
     static constexpr size_t LDS = sizeof(float) * (L_AB + L_H);
     static_assert(RUN1 * 8 == M && RUN2 * 8 == T, "runs tile the windows");
@@ -139,52 +144,70 @@ __global__ void dtm_guide(const float *__restrict__ rgb, float *__restrict__ gui
         guide[t] = gd(rgb[3 * t], rgb[3 * t + 1], rgb[3 * t + 2]);
 }
 
-// EPI: the colour matrix and gamma stages that follow in a stage list run on
-// the result before it is stored (the same float operations as their own pass).
-// Src: lre::One<DtmImg> (one image, 2D grid) or lre::Many<DtmImg> (a flat grid
-// over the tiles of several images)
-template <int T, int NT, bool EPI, class Src>
-__global__ __launch_bounds__(NT) void dtm_tile(Src src, lre::Epilogue E) {
+// One output tile whose guide window is in LDS (dark: no window value exceeds
+// the dark limit, the same for every thread).  Returns with every LDS read of
+// the window's space done before a barrier that all threads passed, and the
+// last pass reading only ha/hb.
+template <int T, int NT, bool EPI>
+LRE_DEV void dtm_tile_one(const DtmImg &I, int x0, int y0, bool dark, float *dsm, const lre::Epilogue &E) {
     using D = DtmGeo<T>;
     constexpr int G = D::G, M = D::M, GP = D::GP, MP = D::MP, TP = D::TP;
-    extern __shared__ float dsm[];
     float *gw = dsm;                               // G x G guide window (pitch GP)
     float *pa = dsm, *pb = dsm + M * MP;           // M x M a and b (pitch MP)
     float *hg = dsm + D::L_AB, *hq = hg + G * MP;  // G x M row means of g and g*g
     float *ha = dsm + D::L_AB, *hb = ha + M * TP;  // M x T row means of a and b
-    const lre::TileOf tl = src.tile();
-    const DtmImg &I = src.args(tl);
-    const float *__restrict__ guide = I.guide;
     const float *in = I.in;
     float *out = I.out;
     const int W = I.W, H = I.H;
-    const int x0 = tl.tx * T, y0 = tl.ty * T;
     const int tid = threadIdx.x;
-    bool dark = true;
-    lre::stage2d<NT>(
-        G, G,
-        [&](int r, int c) {
-            const int y = y0 - 2 * DR + r, x = x0 - 2 * DR + c;
-            return (x >= 0 && x < W && y >= 0 && y < H) ? guide[(size_t)y * W + x] : 0.f;
-        },
-        [&](int r, int c, float g) {
-            gw[r * GP + c] = g;
-            dark = dark && g <= DTM_DARK;  // NaN is not dark
-        });
     // the reference leaves every pixel of a dark tile unchanged
-    if (__syncthreads_and(dark)) {
+    if (dark) {
         if (!EPI && in == out) return;
-        for (int t = tid; t < T * T; t += NT) {
-            const int x = x0 + (t % T), y = y0 + (t / T);
-            if (x >= W || y >= H) continue;
-            const size_t i = (size_t)y * W + x;
-            float r = in[3 * i], g = in[3 * i + 1], b = in[3 * i + 2];
-            if (EPI) lre::epilogue_rgb(r, g, b, E);
-            out[3 * i] = r;
-            out[3 * i + 1] = g;
-            out[3 * i + 2] = b;
+        // each thread reads and then writes its own pixels (safe in place)
+        constexpr int PT = (T * T + NT - 1) / NT;
+        float v[PT][3];
+#pragma unroll
+        for (int k = 0; k < PT; k++) {
+            const int t = tid + k * NT, x = x0 + (t % T), y = y0 + (t / T);
+            if (t < T * T && x < W && y < H) {
+                const size_t i = ((size_t)y * W + x) * 3;
+                v[k][0] = in[i];
+                v[k][1] = in[i + 1];
+                v[k][2] = in[i + 2];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < PT; k++) {
+            const int t = tid + k * NT, x = x0 + (t % T), y = y0 + (t / T);
+            if (t < T * T && x < W && y < H) {
+                const size_t i = ((size_t)y * W + x) * 3;
+                float r = v[k][0], g = v[k][1], b = v[k][2];
+                if (EPI) lre::epilogue_rgb(r, g, b, E);
+                out[i] = r;
+                out[i + 1] = g;
+                out[i + 2] = b;
+            }
         }
         return;
+    }
+    // This thread's pixels of the last pass (a run of RUND rows of one column)
+    // are loaded now, so that their latency passes behind the box means: loaded
+    // in the last pass, each would wait for its own round trip between the
+    // stores, which may alias the input (in place).
+    static_assert(T * D::ND <= NT, "one run of the last pass per thread");
+    const int c4 = tid % T, run4 = tid / T, x4 = x0 + c4;
+    float px[D::RUND][3];
+    if (tid < T * D::ND) {
+#pragma unroll
+        for (int j = 0; j < D::RUND; j++) {
+            const int y = y0 + run4 * D::RUND + j;
+            if (x4 < W && y < H) {
+                const size_t i = ((size_t)y * W + x4) * 3;
+                px[j][0] = in[i];
+                px[j][1] = in[i + 1];
+                px[j][2] = in[i + 2];
+            }
+        }
     }
     // row means of g and g*g: rows [0, G), output columns [DR, DR + M)
     for (int t = tid; t < G * 8; t += NT) {
@@ -198,13 +221,13 @@ __global__ __launch_bounds__(NT) void dtm_tile(Src src, lre::Epilogue E) {
     }
     __syncthreads();
     // column means -> mean, mean of squares -> a, b over the M x M window
-    for (int t = tid; t < M * 8; t += NT) {
+    for (int t = tid; t < M * D::NC; t += NT) {
         const int c = t % M, run = t / M;
         const int x = x0 - DR + c;
-        const float *v0 = hg + run * D::RUN1 * MP + c, *v1 = hq + run * D::RUN1 * MP + c;
-        box_run2<D::RUN1>([&](int k) { return make_float2(v0[k * MP], v1[k * MP]); }, y0 - DR + run * D::RUN1, H,
+        const float *v0 = hg + run * D::RUNC * MP + c, *v1 = hq + run * D::RUNC * MP + c;
+        box_run2<D::RUNC>([&](int k) { return make_float2(v0[k * MP], v1[k * MP]); }, y0 - DR + run * D::RUNC, H,
                           [&](int j, float mean, float msq) {
-                              const int rr = run * D::RUN1 + j, y = y0 - DR + rr;
+                              const int rr = run * D::RUNC + j, y = y0 - DR + rr;
                               float a = 0.f, b = 0.f;  // outside the image: no taps
                               if (x >= 0 && x < W && y >= 0 && y < H) {
                                   const float var = lre::max_std(0.0f, msq - mean * mean);
@@ -228,16 +251,16 @@ __global__ __launch_bounds__(NT) void dtm_tile(Src src, lre::Epilogue E) {
     }
     __syncthreads();
     // column means -> A, B; base = max(1e-6, A guide + B); RGB *= 1 + (gain - 1) w
-    for (int t = tid; t < T * 8; t += NT) {
-        const int c = t % T, run = t / T;
-        const int x = x0 + c;
-        const float *v0 = ha + run * D::RUN2 * TP + c, *v1 = hb + run * D::RUN2 * TP + c;
-        box_run2<D::RUN2>([&](int k) { return make_float2(v0[k * TP], v1[k * TP]); }, y0 + run * D::RUN2, H,
+    if (tid < T * D::ND) {
+        const int c = c4, run = run4;
+        const int x = x4;
+        const float *v0 = ha + run * D::RUND * TP + c, *v1 = hb + run * D::RUND * TP + c;
+        box_run2<D::RUND>([&](int k) { return make_float2(v0[k * TP], v1[k * TP]); }, y0 + run * D::RUND, H,
                           [&](int j, float ma, float mb) {
-                              const int y = y0 + run * D::RUN2 + j;
+                              const int y = y0 + run * D::RUND + j;
                               if (x >= W || y >= H) return;
                               const size_t i = ((size_t)y * W + x) * 3;
-                              float r = in[i], g = in[i + 1], b = in[i + 2];
+                              float r = px[j][0], g = px[j][1], b = px[j][2];
                               const float gd = lre::max_std(0.0f, lre::max_std(r, lre::max_std(g, b)));
                               const float base = lre::max_std(1e-6f, ma * gd + mb);
                               float gain = lre::clamp_std(lre::aces(base) / base, 0.0f, 1.0f);
@@ -256,6 +279,107 @@ __global__ __launch_bounds__(NT) void dtm_tile(Src src, lre::Epilogue E) {
                               out[i + 1] = g;
                               out[i + 2] = b;
                           });
+    }
+}
+
+// The guide window of a tile, as NT threads hold it in registers: thread
+// (c, r0) = (tid % G, tid / G) holds column c of rows r0, r0 + PER, ...
+template <int T, int NT>
+struct DtmWin {
+    static constexpr int G = DtmGeo<T>::G, PER = NT / G, K = (G + PER - 1) / PER;
+    static_assert(PER >= 1, "a window row fits the workgroup");
+    float v[K];
+    LRE_DEV void load(const DtmImg &I, int x0, int y0) {
+        const int tid = threadIdx.x, c = tid % G, r0 = tid / G;
+        const int x = x0 - 2 * DR + c;
+        const bool xin = x >= 0 && x < I.W && r0 < PER;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int r = r0 + k * PER, y = y0 - 2 * DR + r;
+            v[k] = (xin && r < G && y >= 0 && y < I.H) ? I.guide[(size_t)y * I.W + x] : 0.f;
+        }
+    }
+    // into the LDS window (pitch G + 1); false if any value exceeds the dark limit
+    LRE_DEV bool store(float *gw) const {
+        const int tid = threadIdx.x, c = tid % G, r0 = tid / G;
+        bool dark = true;
+        if (r0 < PER) {
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const int r = r0 + k * PER;
+                if (r < G) {
+                    gw[r * (G + 1) + c] = v[k];
+                    dark = dark && v[k] <= DTM_DARK;  // NaN is not dark
+                }
+            }
+        }
+        return dark;
+    }
+};
+
+// Tile t of a launch: a flat index over the tiles of one image (row-major,
+// tiles_x from the width) or of several (the table's binary search).
+template <int T>
+LRE_DEV lre::TileOf dtm_tile_at(const lre::One<DtmImg> &s, int t) {
+    const int tx = (s.a.W + T - 1) / T;
+    return {0, t % tx, t / tx};
+}
+template <int T>
+LRE_DEV lre::TileOf dtm_tile_at(const lre::Many<DtmImg> &s, int t) { return s.tile_at(t); }
+
+// The tiles of workgroup w, XCD by XCD: workgroups are dealt round-robin to the
+// 8 XCDs, so the workgroups of one XCD (w % 8) take one contiguous range of
+// tiles in row-major order between them, and neighbouring tiles, which share
+// most of their guide windows, meet in that XCD's L2.
+struct DtmOrder {
+    int t, step, end;
+    LRE_DEV DtmOrder(int ntiles) {
+        constexpr int XCDS = 8;
+        const int g = (int)gridDim.x, w = (int)blockIdx.x, x = w % XCDS;
+        int before = 0;
+        for (int k = 0; k < x; k++) before += (g - k + XCDS - 1) / XCDS;
+        const int mine = (g - x + XCDS - 1) / XCDS;
+        const long long b0 = (long long)ntiles * before / g, b1 = (long long)ntiles * (before + mine) / g;
+        t = (int)b0 + w / XCDS;
+        step = mine;
+        end = (int)b1;
+    }
+};
+
+// EPI: the colour matrix and gamma stages that follow in a stage list run on
+// the result before it is stored (the same float operations as their own pass).
+// Src: lre::One<DtmImg> (one image) or lre::Many<DtmImg> (the tiles of several
+// images).  The grid is persistent: each workgroup walks its tiles (DtmOrder)
+// and loads the next tile's guide window into registers while it works on the
+// current one, so the window's latency passes behind the box means.
+template <int T, int NT, bool EPI, class Src>
+__global__ __launch_bounds__(NT) void dtm_tile(Src src, int ntiles, lre::Epilogue E) {
+    using D = DtmGeo<T>;
+    constexpr int G = D::G, M = D::M, MP = D::MP, TP = D::TP;
+    static_assert(D::GP == G + 1, "window pitch");
+    extern __shared__ float dsm[];
+    float *gw = dsm;                               // G x G guide window (pitch GP)
+    float *pa = dsm, *pb = dsm + M * MP;           // M x M a and b (pitch MP)
+    float *hg = dsm + D::L_AB, *hq = hg + G * MP;  // G x M row means of g and g*g
+    float *ha = dsm + D::L_AB, *hb = ha + M * TP;  // M x T row means of a and b
+    const int tid = threadIdx.x;
+    DtmOrder ord(ntiles);
+    DtmWin<T, NT> win;
+    if (ord.t < ord.end) {
+        const lre::TileOf tl = dtm_tile_at<T>(src, ord.t);
+        win.load(src.args(tl), tl.tx * T, tl.ty * T);
+    }
+    for (; ord.t < ord.end; ord.t += ord.step) {
+        const lre::TileOf tl = dtm_tile_at<T>(src, ord.t);
+        const DtmImg &I = src.args(tl);
+        // The previous tile's last pass reads only ha/hb, which the window does
+        // not overlap; its passes that read the window's space ended at a barrier.
+        const bool dark = win.store(gw);
+        if (ord.t + ord.step < ord.end) {
+            const lre::TileOf nt = dtm_tile_at<T>(src, ord.t + ord.step);
+            win.load(src.args(nt), nt.tx * T, nt.ty * T);
+        }
+        dtm_tile_one<T, NT, EPI>(I, tl.tx * T, tl.ty * T, __syncthreads_and(dark), dsm, E);
     }
 }
 
@@ -291,12 +415,29 @@ __global__ void rgb3_to_cfa_kernel(const float *__restrict__ rgb3, float *__rest
     }
 }
 
-// The tile kernel at side T with NT threads.  64 x 64 tiles of 1024 threads
-// (one workgroup per CU, 142 KiB of LDS) measured faster on 24 MP than 32 x 32
-// tiles of 512 (two per CU): 0.44 against 0.54 ms, the smaller tiles redoing
-// half again as many box means in their rings.
+// Compute units of the current device (cached per device).
+int current_cus() {
+    constexpr int MAXD = 64;
+    static int cus[MAXD] = {};
+    int d = 0;
+    if (hipGetDevice(&d) != hipSuccess || d < 0 || d >= MAXD) return 256;
+    if (!cus[d]) {
+        int c = 0;
+        if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, d) != hipSuccess || c < 1) c = 256;
+        cus[d] = c;
+    }
+    return cus[d];
+}
+
+// The tile kernel at side T with NT threads over ntiles tiles, one persistent
+// workgroup per CU.  64 x 64 tiles of 1024 threads (one workgroup per CU,
+// 142 KiB of LDS) measured faster on 24 MP than 32 x 32 tiles of 512 (two per
+// CU): 0.44 against 0.54 ms, the smaller tiles redoing half again as many box
+// means in their rings.
 template <int T, int NT, class Src>
-int dtm_launch(Src src, dim3 grid, hipStream_t s, const lre::Epilogue *epi) {
+int dtm_launch(Src src, int ntiles, hipStream_t s, const lre::Epilogue *epi) {
+    if (ntiles < 1) return LRE_OK;
+    const dim3 grid(ntiles < current_cus() ? ntiles : current_cus());
     static const hipError_t attr = [] {
         hipError_t e = hipFuncSetAttribute((const void *)dtm_tile<T, NT, false, Src>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)DtmGeo<T>::LDS);
@@ -307,9 +448,10 @@ int dtm_launch(Src src, dim3 grid, hipStream_t s, const lre::Epilogue *epi) {
     }();
     LRE_HIP_CHECK(attr);
     if (epi && epi->active())
-        hipLaunchKernelGGL((dtm_tile<T, NT, true, Src>), grid, dim3(NT), DtmGeo<T>::LDS, s, src, *epi);
+        hipLaunchKernelGGL((dtm_tile<T, NT, true, Src>), grid, dim3(NT), DtmGeo<T>::LDS, s, src, ntiles, *epi);
     else
-        hipLaunchKernelGGL((dtm_tile<T, NT, false, Src>), grid, dim3(NT), DtmGeo<T>::LDS, s, src, lre::Epilogue{});
+        hipLaunchKernelGGL((dtm_tile<T, NT, false, Src>), grid, dim3(NT), DtmGeo<T>::LDS, s, src, ntiles,
+                           lre::Epilogue{});
     LRE_HIP_CHECK(hipGetLastError());
     return LRE_OK;
 }
@@ -480,7 +622,7 @@ int lre_launch_detail_tonemap(const float *d_in, float *d_out, int w, int h, voi
     hipLaunchKernelGGL(dtm_guide, dim3(grid_for(n4 > 0 ? n4 : n, 256)), dim3(256), 0, s, d_in, guide, n, n4);
     constexpr int T = 64;
     const lre::One<DtmImg> one{DtmImg{guide, d_in, d_out, w, h}};
-    return dtm_launch<T, 1024>(one, dim3((w + T - 1) / T, (h + T - 1) / T), s, epi);
+    return dtm_launch<T, 1024>(one, ((w + T - 1) / T) * ((h + T - 1) / T), s, epi);
 }
 
 // Many images: two tables (guide chunks, then tiles) over the same per-image
@@ -508,8 +650,7 @@ int lre_launch_detail_tm_multi(const void *d_table, int n, const int grids[2], h
     const size_t half = (DtmTable::bytes(n) + 255) & ~(size_t)255;
     hipLaunchKernelGGL(dtm_guide_many, dim3(grids[0]), dim3(256), 0, s, DtmTable::view(d_table, n));
     LRE_HIP_CHECK(hipGetLastError());
-    return dtm_launch<64, 1024>(DtmTable::view(static_cast<const char *>(d_table) + half, n), dim3(grids[1]), s,
-                                epi);
+    return dtm_launch<64, 1024>(DtmTable::view(static_cast<const char *>(d_table) + half, n), grids[1], s, epi);
 }
 
 int lre_launch_white_balance(const uint16_t *d_img4, float *d_rgb, int w, int h,
