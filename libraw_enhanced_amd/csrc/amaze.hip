@@ -65,11 +65,35 @@ LRE_DEV float half(float x) { return x * 0.5f; }
 LRE_DEV float twice(float x) { return x + x; }
 LRE_DEV int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// Index t = t0, t0 + step, ... < n of a row-major range with rows of `cols`
+// entries, as (row, column): one division at the start, then the quotient and
+// remainder advance by those of the step (an integer division per iteration
+// costs some thirty VALU instructions).
+struct RowWalk {
+    int t, n, cols, q, r, dq, dr, step;
+    LRE_DEV RowWalk(int t0, int n_, int cols_, int step_) : t(t0), n(n_), cols(cols_), step(step_) {
+        q = t0 / cols_;
+        r = t0 - q * cols_;
+        dq = step_ / cols_;
+        dr = step_ - dq * cols_;
+    }
+    LRE_DEV bool more() const { return t < n; }
+    LRE_DEV void next() {
+        t += step;
+        q += dq;
+        r += dr;
+        if (r >= cols) {
+            r -= cols;
+            q++;
+        }
+    }
+};
+
 template <class F>
 LRE_DEV void for_rect(int r0, int r1, int c0, int c1, F f) {
     const int nc = c1 - c0, nr = r1 - r0;
     if (nc <= 0 || nr <= 0) return;
-    for (int t = threadIdx.x; t < nr * nc; t += BLOCK) f(r0 + t / nc, c0 + t % nc);
+    for (RowWalk w((int)threadIdx.x, nr * nc, nc, BLOCK); w.more(); w.next()) f(r0 + w.q, c0 + w.r);
 }
 
 // Non-green ("site") positions of rows [r0,r1): cc = c0 + parity(row) + 2j < c1,
@@ -77,10 +101,11 @@ LRE_DEV void for_rect(int r0, int r1, int c0, int c1, F f) {
 template <class F>
 LRE_DEV void for_sites_on(const Args &A, int r0, int r1, int c0, int c1, int tid0, int nthr, F f) {
     const int ns = (c1 - c0 + 1) / 2, nr = r1 - r0;
-    if (ns <= 0 || nr <= 0) return;
-    for (int t = (int)threadIdx.x - tid0; t < nr * ns; t += nthr) {
-        int rr = r0 + t / ns;
-        int cc = c0 + (int)(A.cf[rr & 1][0] & 1) + 2 * (t % ns);
+    const int t0 = (int)threadIdx.x - tid0;
+    if (ns <= 0 || nr <= 0 || t0 < 0) return;
+    for (RowWalk w(t0, nr * ns, ns, nthr); w.more(); w.next()) {
+        const int rr = r0 + w.q;
+        const int cc = c0 + (int)(A.cf[rr & 1][0] & 1) + 2 * w.r;
         if (cc < c1) f(rr, cc);
     }
 }
@@ -112,8 +137,8 @@ LRE_DEV void for_site_pairs(const Args &A, int r0, int r1, int c0, int c1, C com
     const int ns = (c1 - c0 + 1) / 2, nr = r1 - r0;
     if (ns <= 0 || nr <= 0) return;
     const int n = ((nr + 3) / 4) * 2 * ns;
-    for (int t = threadIdx.x; t < n; t += BLOCK) {
-        const int g = t / (2 * ns), rem = t - g * 2 * ns;
+    for (RowWalk w((int)threadIdx.x, n, 2 * ns, BLOCK); w.more(); w.next()) {
+        const int g = w.q, rem = w.r;
         const int p = rem >= ns ? 1 : 0, j = rem - p * ns;
         const int ra = r0 + 4 * g + p, rb = ra + 2;
         const int cc = c0 + (int)(A.cf[ra & 1][0] & 1) + 2 * j;
@@ -283,18 +308,27 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
         const int n_lo = rrmax < rr1 ? 16 * nc : 0;
         const int n_le = ccmin > 0 ? 16 * nr : 0;
         const int n_all = n_up + n_in + n_lo + n_le;
-        // source pixel and arena position of fill write t
+        // source pixel and arena position of fill write t (quotients by nc
+        // through its float reciprocal: exact after one correction, t < 2^16)
+        const float inv_nc = nc > 0 ? 1.0f / (float)nc : 0.f;
+        auto qr = [&](int t, int &q, int &r) { lre::divmod(t, nc, inv_nc, q, r); };
         auto locate = [&](int t, int &row, int &col, int &L) {
             if (t < n_up) {
-                const int rr = t / nc, cc = ccmin + t % nc;
+                int rr, cm;
+                qr(t, rr, cm);
+                const int cc = ccmin + cm;
                 row = clampi(32 - rr + top, 0, H - 1); col = clampi(cc + left, 0, W - 1);
                 L = rr * TS + cc;
             } else if (t < n_up + n_in) {
-                const int u = t - n_up, rr = rrmin + u / nc, cc = ccmin + u % nc;
+                int q, cm;
+                qr(t - n_up, q, cm);
+                const int rr = rrmin + q, cc = ccmin + cm;
                 row = rr + top; col = cc + left;
                 L = rr * TS + cc;
             } else if (t < n_up + n_in + n_lo) {
-                const int u = t - n_up - n_in, k = u / nc, cc = ccmin + u % nc;
+                int k, cm;
+                qr(t - n_up - n_in, k, cm);
+                const int cc = ccmin + cm;
                 row = clampi(H - k - 2, 0, H - 1); col = clampi(left + cc, 0, W - 1);
                 L = (rrmax + k) * TS + cc;
             } else {
