@@ -49,7 +49,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per pixel of each stage (fp32 operator I/O)
-STAGE_BYTES_PER_PX = {"amaze": 4 + 12, "defringe": 12 + 12, "detail_tm": 12 + 12,
+STAGE_BYTES_PER_PX = {"amaze": 4 + 12, "xtrans3": 4 + 12, "defringe": 12 + 12, "detail_tm": 12 + 12,
                       "tone_mapping": 12 + 12}
 
 
@@ -457,6 +457,9 @@ def bench_xtrans26(args):
     stage_ms = {s: float(np.mean([ev.elapsed_ms(marks[k][i], marks[k][i + 1]) for k in range(args.steps)]))
                 for i, s in enumerate(stages)}
     mp = h * w / 1e6
+    # HBM roofline of the X-Trans kernel, the largest launch: 4 B CFA in, 12 B RGB out per pixel
+    alg_bytes = STAGE_BYTES_PER_PX["xtrans3"] * h * w
+    achieved = alg_bytes / (stage_ms["xtrans3"] * 1e-3) / 1e9
     line = {"metric": "MP/s X-Trans develop chain (26 MP)", "value": round(mp * args.steps / elapsed, 2),
             "unit": "MP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": round(elapsed * 1e3 / args.steps, 3), "higher_is_better": True,
@@ -464,6 +467,9 @@ def bench_xtrans26(args):
             "data": "synthetic (seeded scene, radial lateral CA pre-warp, Poisson-Gaussian noise)",
             "config": {"workload": f"{' -> '.join(stages)} on one {w}x{h} X-Trans float32 mosaic",
                        "image_mp": round(mp, 2), "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()}},
+            "roofline": {"kernel": "xtrans3", "bound": "hbm", "achieved": round(achieved, 1),
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                         "traffic": load_traffic("xtrans"), "alg_bytes_per_launch": alg_bytes},
             "valu_roofline": valu_roofline("xtrans_kernel", "xtrans3", stage_ms["xtrans3"], h * w)}
     if not args.no_cpu:
         threads = _threads(args)
