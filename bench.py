@@ -694,7 +694,7 @@ def bench_batch64(args):
     b = dv.BatchDeveloper(0, 2)
     tune_set = [images[sum(c for c, *_ in BATCH64[:i])] for i in range(len(BATCH64))]
     tune_ms = b.autotune(tune_set, stages, reps=1)
-    cfg = {**b.config(), **b.grouping()}
+    cfg = {**b.config(), **b.grouping(), **b.tiles()}
     _log(f"autotune: {cfg} ({tune_ms:.1f} ms for the 5-image sample)")
     # eager batch in both post-stage modes (group: one launch per stage over
     # the batch; lanes: per image over streams), each after a warm-up run

@@ -422,9 +422,15 @@ int lre_hip_batch_get_config(lre_batch *b, int *lanes, int *amaze_slots_per_cu,
  * share forms a group of its own). */
 int lre_hip_batch_set_grouping(lre_batch *b, int group_mode, size_t post_budget_bytes);
 int lre_hip_batch_get_grouping(lre_batch *b, int *group_mode, size_t *post_budget_bytes);
+/* Tile height of the defringe blur: 48 or 96 rows, or 0 (the default) to pick
+ * it by image size (48 while 64 x 96 tiles would give fewer than four per CU).
+ * Both heights give the same bits. */
+int lre_hip_batch_set_tiles(lre_batch *b, int defringe_blur_rows);
+int lre_hip_batch_get_tiles(lre_batch *b, int *defringe_blur_rows);
 /* Time `reps` runs of each candidate launch shape (group mode on and off,
- * lanes, slots per CU) on this batch and keep the fastest; *best_ms receives
- * its time per batch. */
+ * lanes, slots per CU) on this batch and keep the fastest, then the defringe
+ * blur tile height (48, 96) on that shape; *best_ms receives the time per batch
+ * of what is kept. */
 int lre_hip_batch_autotune(lre_batch *b, const lre_image *images, int n, const int *stages,
                            int nstages, const lre_chain_params *params, int reps, float *best_ms);
 

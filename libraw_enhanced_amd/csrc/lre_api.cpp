@@ -5,6 +5,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <atomic>
 #include <mutex>
 #include <limits>
 #include <string>
@@ -97,6 +98,19 @@ void *noalloc_arenas_for(int device) {
 }
 
 }  // namespace
+
+int lre_current_cus() {
+    constexpr int MAXD = 64;
+    static std::atomic<int> cus[MAXD] = {};
+    int d = 0;
+    if (hipGetDevice(&d) != hipSuccess || d < 0 || d >= MAXD) return 256;
+    int c = cus[d].load(std::memory_order_relaxed);
+    if (c < 1) {
+        if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, d) != hipSuccess || c < 1) c = 256;
+        cus[d].store(c, std::memory_order_relaxed);
+    }
+    return c;
+}
 
 void lre_set_error(const char *fmt, ...) {
     char buf[512];

@@ -98,10 +98,19 @@ size_t lre_defringe_multi_scratch_bytes(int w, int h);
 // quotient smoothsteps), else the images take the per-image launcher
 int lre_defringe_multi_supported(float radius, float edge_threshold, float chroma_threshold, float strength,
                                  int defringe_green, float green_strength_scale);
-int lre_defringe_multi_fill(const lre_post_job *jobs, int n, void *h_table, int grids[6]);
+// rows: the blur tile height, as lre_defringe_blur_rows returns it (the same
+// value for the fill and the launch)
+int lre_defringe_multi_fill(const lre_post_job *jobs, int n, void *h_table, int grids[6], int rows);
 int lre_launch_defringe_multi(const void *d_table, int n, const int grids[6], float radius, float edge_threshold,
                               float chroma_threshold, float strength, int defringe_green,
-                              float green_strength_scale, hipStream_t s);
+                              float green_strength_scale, int rows, hipStream_t s);
+// Blur tile height of defringe: blur_rows if it is one of the two heights
+// (48, 96), else chosen by the number of 64 x 96 tiles to cover (short tiles
+// below four per CU).  Both heights give the same bits.
+int lre_defringe_blur_rows(int blur_rows, size_t tall_tiles);
+size_t lre_defringe_tall_tiles(int w, int h);
+// compute units of the current device (cached per device)
+int lre_current_cus();
 
 // The parameter checks the launchers make (LRE_OK or LRE_ERR_ARG with the
 // message set), for callers that validate a stage list before enqueueing.
@@ -175,7 +184,7 @@ int lre_launch_micro_contrast(const float *d_in, float *d_out, int w, int h, flo
 int lre_launch_defringe(const float *d_in, float *d_out, int w, int h, float radius,
                         float edge_threshold, float chroma_threshold, float strength,
                         int defringe_green, float green_strength_scale, void *d_scratch,
-                        hipStream_t s);
+                        hipStream_t s, int blur_rows = 0);
 int lre_launch_xtrans3(lre_ctx *ctx, const float *d_cfa, float *d_rgb, int w, int h,
                        const signed char xtrans36[36], float maximum_value, void *d_scratch,
                        hipStream_t s);

@@ -110,7 +110,7 @@ size_t post_scratch(const lre_image &im, const int *stages, int ns, const lre_ch
 // one post-demosaic stage, in place on the image's RGB buffer (the develop
 // pipeline calls every one of them in place, core/libraw_wrapper.cpp:1776-1818)
 int run_post_stage(int stage, const lre_image &im, const lre_chain_params &p, void *scratch,
-                   hipStream_t s) {
+                   hipStream_t s, int blur_rows = 0) {
     float *rgb = im.d_rgb;
     const int w = im.width, h = im.height;
     switch (stage) {
@@ -124,7 +124,7 @@ int run_post_stage(int stage, const lre_image &im, const lre_chain_params &p, vo
     case LRE_STAGE_DEFRINGE:
         return lre_launch_defringe(rgb, rgb, w, h, p.defringe_radius, p.defringe_edge_threshold,
                                    p.defringe_chroma_threshold, p.defringe_strength,
-                                   p.defringe_green, p.defringe_green_scale, scratch, s);
+                                   p.defringe_green, p.defringe_green_scale, scratch, s, blur_rows);
     case LRE_STAGE_DETAIL_TM:
         return lre_launch_detail_tonemap(rgb, rgb, w, h, scratch, s);
     case LRE_STAGE_TONE_MAPPING:
@@ -145,7 +145,7 @@ int run_post_stage(int stage, const lre_image &im, const lre_chain_params &p, vo
 // are the same bits as one pass per stage (same float operations on the same
 // values); the fused forms save a 12 B/px read and write per stage.
 int run_post_chain(const int *stages, int from, int ns, const lre_image &im, const lre_chain_params &p,
-                   void *lane_scratch, hipStream_t s) {
+                   void *lane_scratch, hipStream_t s, int blur_rows = 0) {
     const size_t mb = mask_bytes(im, stages, ns), npx = (size_t)im.width * im.height;
     float *mask = mb ? static_cast<float *>(lane_scratch) : nullptr;
     void *scratch = static_cast<char *>(lane_scratch) + mb;
@@ -196,7 +196,7 @@ int run_post_chain(const int *stages, int from, int ns, const lre_image &im, con
                 continue;
             }
         }
-        if (const int rc = run_post_stage(st, im, p, scratch, s)) return rc;
+        if (const int rc = run_post_stage(st, im, p, scratch, s, blur_rows)) return rc;
     }
     return LRE_OK;
 }
@@ -322,6 +322,7 @@ struct lre_batch {
     // images whose scratch fits post_budget (group mode), or per image over
     // `lanes` streams
     int group_mode = 2;      // 0 off, 1 on, 2 by image size
+    int blur_rows = 0;       // defringe blur tile height: 48, 96 or 0 (by image size)
     size_t post_budget = 0;  // 0: half of the device's memory
     DevBuf post_scratch, d_tab_post, g_tab_post;
     HostBuf h_tab_post;
@@ -343,8 +344,8 @@ void drop_graph(lre_batch *b) {
 }
 
 // group mode 2 groups batches whose mean image is below this many pixels
-// (measured on the default chain: 256 x 0.3 MP, grouped 1.7x faster than the
-// best lane count; 64 x 3.1 MP and 16 x 12 MP, grouped 2% and 5% slower)
+// (measured on the default chain: 256 x 0.3 MP, grouped 1.6x faster than the
+// best lane count; 64 x 3.1 MP equal; 16 x 12 MP, grouped 6% slower)
 constexpr size_t GROUP_AUTO_PX = 2000000;
 
 // one multi-image pass of a group: where its table sits and its grid sizes
@@ -354,6 +355,7 @@ struct GroupPass {
     int grids[6] = {0, 0, 0, 0, 0, 0};
     bool epi = false;        // detail TM with a fused matrix / gamma
     bool matrix = false, gamma = false;
+    int rows = 0;            // defringe: blur tile height
 };
 // images whose post-demosaic scratch is held at once, run on one lane
 struct Group {
@@ -427,7 +429,7 @@ size_t device_memory(int device) {
 // forms a group of its own).  Groups of one lane run one after another and
 // share the lane's region of the post scratch.
 void plan_groups(const lre_image *imgs, int n, const int *stages, int from, int ns, size_t budget, int lanes,
-                 Plan &P) {
+                 int blur_rows, Plan &P) {
     P.groups.clear();
     P.post_scratch = P.post_tab_bytes = 0;
     const size_t share = budget / (size_t)lanes;
@@ -447,6 +449,11 @@ void plan_groups(const lre_image *imgs, int n, const int *stages, int from, int 
                 if (j < ns && stages[j] == LRE_STAGE_GAMMA) { gp.gamma = true; j++; }
                 gp.epi = gp.matrix || gp.gamma;
                 si = j - 1;
+            }
+            if (st == LRE_STAGE_DEFRINGE) {
+                size_t tiles = 0;
+                for (int i : g.imgs) tiles += lre_defringe_tall_tiles(imgs[i].width, imgs[i].height);
+                gp.rows = lre_defringe_blur_rows(blur_rows, tiles);
             }
             size_t tb = 0;
             if (st == LRE_STAGE_AXIAL_CA) tb = lre_axial_multi_table_bytes(m);
@@ -526,7 +533,7 @@ int make_plan(lre_batch *b, const lre_image *imgs, int n, const int *stages, int
     P.grouped = want && ns > from && n > 0 && groupable(stages, from, ns, p);
     if (P.grouped) {
         const size_t budget = b->post_budget ? b->post_budget : device_memory(b->device) / 2;
-        plan_groups(imgs, n, stages, from, ns, budget, b->lanes, P);
+        plan_groups(imgs, n, stages, from, ns, budget, b->lanes, b->blur_rows, P);
         P.lane_scratch = 0;
     }
     for (auto &j : P.bayer) P.tiles_a += lre_amaze_tiles(j.w, j.h);
@@ -621,7 +628,7 @@ int upload_tables(lre_batch *b, Plan &P, const lre_image *imgs, int n, void *d_a
                     rc = lre_axial_multi_fill(jobs.data(), m, p.axial_radius, p.axial_epsilon, p.axial_strength,
                                               h + gp.tab_off, gp.grids);
                 else if (gp.stage == LRE_STAGE_DEFRINGE)
-                    rc = lre_defringe_multi_fill(jobs.data(), m, h + gp.tab_off, gp.grids);
+                    rc = lre_defringe_multi_fill(jobs.data(), m, h + gp.tab_off, gp.grids, gp.rows);
                 else if (gp.stage == LRE_STAGE_DETAIL_TM)
                     rc = lre_detail_tm_multi_fill(jobs.data(), m, h + gp.tab_off, gp.grids);
                 if (rc) return rc;
@@ -656,7 +663,8 @@ int enqueue_groups(const Plan &P, const lre_image *imgs, const lre_chain_params 
             case LRE_STAGE_DEFRINGE:
                 rc = lre_launch_defringe_multi(d + gp.tab_off, m, gp.grids, p.defringe_radius,
                                                p.defringe_edge_threshold, p.defringe_chroma_threshold,
-                                               p.defringe_strength, p.defringe_green, p.defringe_green_scale, s);
+                                               p.defringe_strength, p.defringe_green, p.defringe_green_scale, gp.rows,
+                                               s);
                 break;
             case LRE_STAGE_DETAIL_TM: {
                 const lre::Epilogue E = lre_make_epilogue(gp.matrix ? p.color_transform : nullptr, p.gamma_power,
@@ -742,7 +750,8 @@ int enqueue(lre_batch *b, const Plan &P, const lre_image *imgs, int n, const int
         auto plan = assign_lanes(imgs, n, lanes);
         for (int k = 0; k < lanes; k++)
             for (int i : plan[k])
-                if ((rc = run_post_chain(stages, first, ns, imgs[i], p, L[k].scratch.p, L[k].stream)))
+                if ((rc = run_post_chain(stages, first, ns, imgs[i], p, L[k].scratch.p, L[k].stream,
+                                         b->blur_rows)))
                     return rc;
     }
     for (int k = 1; k < lanes; k++) {
@@ -943,6 +952,25 @@ int lre_hip_batch_set_grouping(lre_batch *b, int group_mode, size_t post_budget_
     return LRE_OK;
 }
 
+int lre_hip_batch_set_tiles(lre_batch *b, int defringe_blur_rows) {
+    if (!b) return LRE_ERR_ARG;
+    if (defringe_blur_rows != 0 && defringe_blur_rows != 48 && defringe_blur_rows != 96) {
+        lre_set_error("defringe blur rows must be 0, 48 or 96");
+        return LRE_ERR_ARG;
+    }
+    int rc = batch_sync_all(b);
+    if (rc) return rc;
+    drop_graph(b);
+    b->blur_rows = defringe_blur_rows;
+    return LRE_OK;
+}
+
+int lre_hip_batch_get_tiles(lre_batch *b, int *defringe_blur_rows) {
+    if (!b) return LRE_ERR_ARG;
+    if (defringe_blur_rows) *defringe_blur_rows = b->blur_rows;
+    return LRE_OK;
+}
+
 int lre_hip_batch_get_grouping(lre_batch *b, int *group_mode, size_t *post_budget_bytes) {
     if (!b) return LRE_ERR_ARG;
     if (group_mode) *group_mode = b->group_mode;
@@ -963,45 +991,61 @@ int lre_hip_batch_autotune(lre_batch *b, const lre_image *imgs, int n, const int
     if (!b) return LRE_ERR_ARG;
     LRE_HIP_CHECK(hipSetDevice(b->device));
     if (reps < 1) reps = 1;
+    hipEvent_t e0, e1;
+    LRE_HIP_CHECK(hipEventCreate(&e0));
+    LRE_HIP_CHECK(hipEventCreate(&e1));
+    // a warm run (sizes the buffers), then `reps` timed runs of the current shape
+    auto time_it = [&](float &ms) -> int {
+        int rc = batch_run_eager(b, imgs, n, stages, ns, params);
+        if (!rc) rc = batch_sync_all(b);
+        if (rc) return rc;
+        if (hipEventRecord(e0, b->lane[0].stream) != hipSuccess) return LRE_ERR_HIP;
+        for (int r = 0; r < reps && !rc; r++) rc = batch_run_eager(b, imgs, n, stages, ns, params);
+        if (rc) return rc;
+        if (hipEventRecord(e1, b->lane[0].stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
+            return LRE_ERR_HIP;
+        ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        ms /= (float)reps;
+        return LRE_OK;
+    };
+    // launch shapes: group mode, lanes, workgroups per CU of the demosaic queues
     struct Cand { int group, lanes, a, x; };
     std::vector<Cand> cands;
     for (int group : {1, 0})
         for (int lanes : {1, 2, 3, 4})
             for (int a : {1, 2, 4}) cands.push_back({group, lanes, a, 2 * a});
-    hipEvent_t e0, e1;
-    LRE_HIP_CHECK(hipEventCreate(&e0));
-    LRE_HIP_CHECK(hipEventCreate(&e1));
     float best = 1e30f;
     Cand pick = {b->group_mode, b->lanes, b->amaze_per_cu, b->xtrans_per_cu};
     const size_t budget = b->post_budget;
     int rc = LRE_OK;
     for (const Cand &c : cands) {
+        float ms = 0.f;
         if ((rc = lre_hip_batch_set_grouping(b, c.group, budget))) break;
         if ((rc = lre_hip_batch_set_config(b, c.lanes, c.a, c.x))) break;
-        // warm run (sizes buffers), then timed runs
-        if ((rc = batch_run_eager(b, imgs, n, stages, ns, params))) break;
-        if ((rc = batch_sync_all(b))) break;
-        if (hipEventRecord(e0, b->lane[0].stream) != hipSuccess) { rc = LRE_ERR_HIP; break; }
-        for (int r = 0; r < reps && !rc; r++) rc = batch_run_eager(b, imgs, n, stages, ns, params);
-        if (rc) break;
-        if (hipEventRecord(e1, b->lane[0].stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
-            rc = LRE_ERR_HIP;
-            break;
-        }
-        float ms = 0.f;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        ms /= (float)reps;
+        if ((rc = time_it(ms))) break;
         if (ms < best) {
             best = ms;
             pick = c;
+        }
+    }
+    if (!rc) rc = lre_hip_batch_set_grouping(b, pick.group, budget);
+    if (!rc) rc = lre_hip_batch_set_config(b, pick.lanes, pick.a, pick.x);
+    // then the defringe blur tile height on that shape (same bits either way)
+    int rows = b->blur_rows;
+    for (int cand : {48, 96}) {
+        float ms = 0.f;
+        if (rc || (rc = lre_hip_batch_set_tiles(b, cand)) || (rc = time_it(ms))) break;
+        if (ms < best) {
+            best = ms;
+            rows = cand;
         }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;
     if (best_ms) *best_ms = best;
-    if ((rc = lre_hip_batch_set_grouping(b, pick.group, budget))) return rc;
-    return lre_hip_batch_set_config(b, pick.lanes, pick.a, pick.x);
+    return lre_hip_batch_set_tiles(b, rows);
 }
 
 }  // extern "C"

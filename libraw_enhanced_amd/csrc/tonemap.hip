@@ -415,20 +415,6 @@ __global__ void rgb3_to_cfa_kernel(const float *__restrict__ rgb3, float *__rest
     }
 }
 
-// Compute units of the current device (cached per device).
-int current_cus() {
-    constexpr int MAXD = 64;
-    static int cus[MAXD] = {};
-    int d = 0;
-    if (hipGetDevice(&d) != hipSuccess || d < 0 || d >= MAXD) return 256;
-    if (!cus[d]) {
-        int c = 0;
-        if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, d) != hipSuccess || c < 1) c = 256;
-        cus[d] = c;
-    }
-    return cus[d];
-}
-
 // The tile kernel at side T with NT threads over ntiles tiles, one persistent
 // workgroup per CU.  64 x 64 tiles of 1024 threads (one workgroup per CU,
 // 142 KiB of LDS) measured faster on 24 MP than 32 x 32 tiles of 512 (two per
@@ -437,7 +423,8 @@ int current_cus() {
 template <int T, int NT, class Src>
 int dtm_launch(Src src, int ntiles, hipStream_t s, const lre::Epilogue *epi) {
     if (ntiles < 1) return LRE_OK;
-    const dim3 grid(ntiles < current_cus() ? ntiles : current_cus());
+    const int cus = lre_current_cus();
+    const dim3 grid(ntiles < cus ? ntiles : cus);
     static const hipError_t attr = [] {
         hipError_t e = hipFuncSetAttribute((const void *)dtm_tile<T, NT, false, Src>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)DtmGeo<T>::LDS);

@@ -101,6 +101,8 @@ _SIGS = {
     "lre_hip_batch_get_config": ([_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)], _i),
     "lre_hip_batch_set_grouping": ([_vp, _i, ctypes.c_size_t], _i),
     "lre_hip_batch_get_grouping": ([_vp, ctypes.POINTER(_i), ctypes.POINTER(ctypes.c_size_t)], _i),
+    "lre_hip_batch_set_tiles": ([_vp, _i], _i),
+    "lre_hip_batch_get_tiles": ([_vp, ctypes.POINTER(_i)], _i),
     "lre_hip_batch_autotune": ([_vp, ctypes.POINTER(Image), _i, ctypes.POINTER(_i), _i,
                                 ctypes.POINTER(ChainParams), _i, ctypes.POINTER(_f)], _i),
     "lre_hip_serve_create": ([ctypes.POINTER(_vp), _i, _i, ctypes.c_size_t, ctypes.POINTER(_i), _i,
@@ -211,6 +213,15 @@ class BatchDeveloper:
         _check(lib().lre_hip_batch_get_grouping(self._h, ctypes.byref(g), ctypes.byref(b)))
         mode = {0: False, 1: True, 2: "auto"}[g.value]
         return {"group_mode": mode, "post_budget_bytes": b.value}
+
+    def set_tiles(self, defringe_blur_rows=0):
+        """Defringe blur tile height: 48, 96 or 0 (by image size); same bits."""
+        _check(lib().lre_hip_batch_set_tiles(self._h, int(defringe_blur_rows)))
+
+    def tiles(self):
+        r = _i()
+        _check(lib().lre_hip_batch_get_tiles(self._h, ctypes.byref(r)))
+        return {"defringe_blur_rows": r.value}
 
     def autotune(self, images, stages=BATCH_CHAIN, params=None, reps=1):
         arr, n, st, ns, p = self._args(images, stages, params)

@@ -229,6 +229,31 @@ def test_batch_grouping_modes(acc, group, budget):
     b.close()
 
 
+@pytest.mark.parametrize("group", [False, True])
+def test_batch_defringe_blur_tiles(acc, group):
+    """The defringe blur tile height (48 or 96 rows, or chosen by size) leaves
+    the bits unchanged, in lane and in group mode."""
+    from libraw_enhanced_amd import develop as dv
+    p = dv.ChainParams.defaults(defringe_green=1)
+    stages = ("demosaic", "defringe", "detail_tm")
+    S = _Set(acc, _spec()[:3])
+    want = [_single(acc, it, stages, p) for it in S.items]
+    b = dv.BatchDeveloper(0, 2)
+    b.set_grouping(group)
+    for rows in (48, 96, 0):
+        print(f"blur rows {rows} group={group}", flush=True)
+        b.set_tiles(rows)
+        assert b.tiles() == {"defringe_blur_rows": rows}
+        b.run(S.images(), stages, p)
+        b.synchronize()
+        for it, g, w in zip(S.items, S.results(), want):
+            assert np.array_equal(g, w), (rows, it["kind"])
+    from libraw_enhanced_amd.accelerator import LibRawInvalidArgument
+    with pytest.raises(LibRawInvalidArgument):
+        b.set_tiles(64)
+    b.close()
+
+
 @pytest.mark.parametrize("kw", [dict(defringe_edge_threshold=1e-7), dict(defringe_radius=20.0)])
 def test_batch_defringe_without_group_pass(acc, kw):
     """Defringe parameters outside the multi-image pass's range (a ramp width
@@ -334,6 +359,7 @@ def test_batch_autotune_and_config(acc):
     ms = b.autotune(S.images(), CHAIN)
     cfg = b.config()
     assert ms > 0 and 1 <= cfg["lanes"] <= 4 and cfg["amaze_slots_per_cu"] in (1, 2, 4)
+    assert b.tiles()["defringe_blur_rows"] in (0, 48, 96)
     from libraw_enhanced_amd.accelerator import LibRawInvalidArgument
     with pytest.raises(LibRawInvalidArgument):
         b.set_config(0, 8, 6)
