@@ -418,8 +418,8 @@ int lre_hip_batch_get_config(lre_batch *b, int *lanes, int *amaze_slots_per_cu,
 /* Group mode off (0), on (1) or by image size (2, the default: on when the
  * batch's mean image is below 2 MP, where one launch per stage is faster than
  * per-image chains), and the device bytes the groups' stage scratch may take
- * (0: half of the device's memory; an image that alone exceeds its lane's
- * share forms a group of its own). */
+ * (0: 8 GiB, or a quarter of the device's memory if less; an image that alone
+ * exceeds its lane's share forms a group of its own). */
 int lre_hip_batch_set_grouping(lre_batch *b, int group_mode, size_t post_budget_bytes);
 int lre_hip_batch_get_grouping(lre_batch *b, int *group_mode, size_t *post_budget_bytes);
 /* Tile height of the defringe blur: 48 or 96 rows, or 0 (the default) to pick

@@ -323,7 +323,7 @@ struct lre_batch {
     // `lanes` streams
     int group_mode = 2;      // 0 off, 1 on, 2 by image size
     int blur_rows = 0;       // defringe blur tile height: 48, 96 or 0 (by image size)
-    size_t post_budget = 0;  // 0: half of the device's memory
+    size_t post_budget = 0;  // 0: DEFAULT_POST_BUDGET, at most a quarter of the device's memory
     DevBuf post_scratch, d_tab_post, g_tab_post;
     HostBuf h_tab_post;
 };
@@ -342,6 +342,11 @@ void drop_graph(lre_batch *b) {
     b->exec = nullptr;
     b->graph = nullptr;
 }
+
+// Default stage-scratch budget of group mode.  A group's launches already fill
+// the device at a fraction of it (defringe, the largest, takes 60 B per pixel:
+// 8 GiB holds some 140 MP of images at once).
+constexpr size_t DEFAULT_POST_BUDGET = (size_t)8 << 30;
 
 // group mode 2 groups batches whose mean image is below this many pixels
 // (measured on the default chain: 256 x 0.3 MP, grouped 1.6x faster than the
@@ -532,7 +537,8 @@ int make_plan(lre_batch *b, const lre_image *imgs, int n, const int *stages, int
     const bool want = b->group_mode == 1 || (b->group_mode == 2 && n > 0 && px / (size_t)n < GROUP_AUTO_PX);
     P.grouped = want && ns > from && n > 0 && groupable(stages, from, ns, p);
     if (P.grouped) {
-        const size_t budget = b->post_budget ? b->post_budget : device_memory(b->device) / 2;
+        const size_t budget =
+            b->post_budget ? b->post_budget : std::min(DEFAULT_POST_BUDGET, device_memory(b->device) / 4);
         plan_groups(imgs, n, stages, from, ns, budget, b->lanes, b->blur_rows, P);
         P.lane_scratch = 0;
     }

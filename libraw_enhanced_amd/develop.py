@@ -204,7 +204,7 @@ class BatchDeveloper:
     def set_grouping(self, group_mode="auto", post_budget_bytes=0):
         """Group mode (True, False or "auto": on when the mean image is below
         2 MP): each post-demosaic stage as one launch over a group of images
-        whose scratch fits post_budget_bytes (0: half the device)."""
+        whose scratch fits post_budget_bytes (0: 8 GiB, at most a quarter of the device)."""
         mode = self.GROUP_MODES[group_mode if isinstance(group_mode, str) else bool(group_mode)]
         _check(lib().lre_hip_batch_set_grouping(self._h, mode, int(post_budget_bytes)))
 
