@@ -113,9 +113,11 @@ LRE_DEV void hsum(Q q, int RY, int r, double *hs) {
 }
 
 // Vertical sums of hs for output rows oy .. oy+3 of column cx (a full window,
-// then entering minus leaving taps), divided by the true tap count of the
-// clipped window.
-LRE_DEV void vmean(const double *hs, int r, int cx, int oy, int x0, int y0, int W, int H,
+// then entering minus leaving taps), over the true tap count of the clipped
+// window: times 1 / (2r+1)^2 where the window is whole (inv_full, one FP64
+// division per thread instead of one per mean; the two differ by at most an
+// ulp of the double mean), divided where it is clipped.
+LRE_DEV void vmean(const double *hs, int r, int cx, int oy, int x0, int y0, int W, int H, double inv_full,
                    double out[AX_RB]) {
     const int nt = 2 * r + 1;
     const double *col = hs + oy * AX_HP + cx;
@@ -130,12 +132,13 @@ LRE_DEV void vmean(const double *hs, int r, int cx, int oy, int x0, int y0, int 
     }
     const int x = x0 + cx;
     const int xl = max(0, x - r), xr = min(W - 1, x + r);
+    const int full = (2 * r + 1) * (2 * r + 1);
 #pragma unroll
     for (int u = 0; u < AX_RB; u++) {
         const int y = y0 + oy + u;
         const int yl = max(0, y - r), yb = min(H - 1, y + r);
         const int area = (xr - xl + 1) * (yb - yl + 1);
-        out[u] = area > 0 ? s[u] / (double)area : 0.0;
+        out[u] = area == full ? s[u] * inv_full : (area > 0 ? s[u] / (double)area : 0.0);
     }
 }
 
@@ -154,6 +157,7 @@ __global__ __launch_bounds__(AX_BLOCK) void axial_ab(Src S) {
     const lre::TileOf tl = S.tile();
     const AxialArgs &A = S.args(tl);
     const int r = A.r;
+    const double inv_full = 1.0 / (double)((2 * r + 1) * (2 * r + 1));
     const int RX = AX_TX + 2 * r + 2, RY = AX_TY + 2 * r + 2;  // one extra ring for the gradient
     const int RXP = RX | 1;                              // odd LDS pitch
     float *sR = reinterpret_cast<float *>(smem);
@@ -216,7 +220,7 @@ __global__ __launch_bounds__(AX_BLOCK) void axial_ab(Src S) {
             },
             RYi, r, hs);
         __syncthreads();
-        vmean(hs, r, cx, oy, x0, y0, W, H, out);
+        vmean(hs, r, cx, oy, x0, y0, W, H, inv_full, out);
         __syncthreads();
     };
     auto index_of = [&](int k, size_t &i) {
@@ -271,6 +275,7 @@ __global__ __launch_bounds__(AX_BLOCK) void axial_apply(Src S) {
     const lre::TileOf tl = S.tile();
     const AxialArgs &A = S.args(tl);
     const int r = A.r;
+    const double inv_full = 1.0 / (double)((2 * r + 1) * (2 * r + 1));
     const int RX = AX_TX + 2 * r + 2, RY = AX_TY + 2 * r;
     const int RXP = RX | 1;
     float *q = reinterpret_cast<float *>(smem);             // 4 planes, RY x RXP each
@@ -304,7 +309,7 @@ __global__ __launch_bounds__(AX_BLOCK) void axial_apply(Src S) {
         const float *pl = q + qi * RXP * RY;
         hsum([&](int ry, int c) { return pl[ry * RXP + c]; }, RY, r, hs);
         __syncthreads();
-        vmean(hs, r, cx, oy, x0, y0, W, H, m[qi]);
+        vmean(hs, r, cx, oy, x0, y0, W, H, inv_full, m[qi]);
         __syncthreads();
     }
     for (int k = 0; k < PER; k++) {
