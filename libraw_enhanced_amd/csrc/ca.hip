@@ -344,8 +344,10 @@ __global__ void split_rgb(const float *__restrict__ src, float *__restrict__ R, 
 __global__ void down2(const float *R, const float *G, const float *B, float *R2, float *G2, float *B2,
                       int W, int H, int W2, int H2) {
     const size_t n = (size_t)W2 * H2;
-    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK) {
-        const int y = (int)(i / W2), x = (int)(i % W2);
+    for (lre::GridWalk gw((size_t)blockIdx.x * BLOCK + threadIdx.x, n, (size_t)gridDim.x * BLOCK, W2); gw.more();
+         gw.next()) {
+        const size_t i = gw.i;
+        const int y = gw.row, x = gw.col;
         const int y0 = 2 * y, y1 = min(y0 + 1, H - 1), x0 = 2 * x, x1 = min(x0 + 1, W - 1);
         const size_t a = (size_t)y0 * W + x0, b = (size_t)y0 * W + x1, c = (size_t)y1 * W + x0, d = (size_t)y1 * W + x1;
         R2[i] = 0.25f * (R[a] + R[b] + R[c] + R[d]);
@@ -398,8 +400,18 @@ __global__ __launch_bounds__(BLOCK) void lk_level(LkArgs A) {
         const int npix = iw > 0 && ih > 0 ? iw * ih : 0;
         for (int it = 0; it < A.iters; it++) {
             double a00 = 0, a01 = 0, a11 = 0, bx = 0, by = 0;
+            // pixel t of the interior at (t / iw, t % iw): the quotient and
+            // remainder advance by those of the stride (no division per pixel)
+            const int dq = BLOCK / iw, dr = BLOCK - dq * iw;
+            int q = (int)threadIdx.x / iw, rm = (int)threadIdx.x - q * iw;
             for (int t = threadIdx.x; t < npix; t += BLOCK) {
-                const int y = y0 + 1 + t / iw, x = x0 + 1 + t % iw;
+                const int y = y0 + 1 + q, x = x0 + 1 + rm;
+                q += dq;
+                rm += dr;
+                if (rm >= iw) {
+                    rm -= iw;
+                    q++;
+                }
                 const size_t i = (size_t)y * W + x;
                 const float gx = G[i + 1] - G[i - 1];
                 const float gy = G[i + W] - G[i - W];
@@ -539,8 +551,10 @@ LRE_DEV void sample_shift(const WarpArgs &A, int ch, float px, float py, float &
 
 __global__ __launch_bounds__(BLOCK) void lateral_warp(WarpArgs A) {
     const size_t n = (size_t)A.W * A.H;
-    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK) {
-        const int y = (int)(i / A.W), x = (int)(i % A.W);
+    for (lre::GridWalk gw((size_t)blockIdx.x * BLOCK + threadIdx.x, n, (size_t)gridDim.x * BLOCK, A.W); gw.more();
+         gw.next()) {
+        const size_t i = gw.i;
+        const int y = gw.row, x = gw.col;
         float drx, dry, dbx, dby;
         sample_shift(A, 0, (float)x, (float)y, drx, dry);
         sample_shift(A, 1, (float)x, (float)y, dbx, dby);

@@ -52,8 +52,10 @@ __global__ __launch_bounds__(BLOCK) void ingest_kernel(const unsigned short *__r
                                                       unsigned *dmax) {
     const size_t n = (size_t)W * H;
     float m = 0.f;
-    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK) {
-        const unsigned row = (unsigned)(i / W), col = (unsigned)(i % W);
+    for (lre::GridWalk gw((size_t)blockIdx.x * BLOCK + threadIdx.x, n, (size_t)gridDim.x * BLOCK, W); gw.more();
+         gw.next()) {
+        const size_t i = gw.i;
+        const unsigned row = (unsigned)gw.row, col = (unsigned)gw.col;
         unsigned c4, c3;
         if (filters == 9) {
             c4 = c3 = (unsigned)xt.v[(row % 6) * 6 + col % 6];
@@ -159,8 +161,10 @@ __global__ __launch_bounds__(BLOCK) void zero_classify_kernel(const unsigned sho
                                                              unsigned char *__restrict__ dep,
                                                              unsigned *__restrict__ rowflag) {
     const size_t n = (size_t)W * H;
-    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK) {
-        const int row = (int)(i / W), col = (int)(i % W);
+    for (lre::GridWalk gw((size_t)blockIdx.x * BLOCK + threadIdx.x, n, (size_t)gridDim.x * BLOCK, W); gw.more();
+         gw.next()) {
+        const size_t i = gw.i;
+        const int row = (int)gw.row, col = (int)gw.col;
         const unsigned ch = chan(filters, row, col);
         unsigned char d = 0;
         if (sample(img4, W, row, col, ch) == 0) {
@@ -195,9 +199,11 @@ __global__ __launch_bounds__(BLOCK) void zero_fill_independent(unsigned short *i
                                                               uint32_t filters,
                                                               const unsigned char *__restrict__ dep) {
     const size_t n = (size_t)W * H;
-    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK) {
+    for (lre::GridWalk gw((size_t)blockIdx.x * BLOCK + threadIdx.x, n, (size_t)gridDim.x * BLOCK, W); gw.more();
+         gw.next()) {
+        const size_t i = gw.i;
         if (dep[i] != 1) continue;
-        const int row = (int)(i / W), col = (int)(i % W);
+        const int row = (int)gw.row, col = (int)gw.col;
         const unsigned ch = chan(filters, row, col);
         bool any;
         const unsigned short v = window_mean(img4, W, H, filters, row, col, ch, any);
@@ -243,8 +249,10 @@ __global__ __launch_bounds__(BLOCK) void awb_sum_kernel(const unsigned short *__
                                                        unsigned long long *acc) {
     unsigned long long s[4] = {0, 0, 0, 0}, k[4] = {0, 0, 0, 0};
     const size_t n = (size_t)W * H;
-    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK) {
-        const int row = (int)(i / W), col = (int)(i % W);
+    for (lre::GridWalk gw((size_t)blockIdx.x * BLOCK + threadIdx.x, n, (size_t)gridDim.x * BLOCK, W); gw.more();
+         gw.next()) {
+        const size_t i = gw.i;
+        const int row = (int)gw.row, col = (int)gw.col;
         const unsigned cc = filters == 9 ? (unsigned)xt.v[((row + 6) % 6) * 6 + (col + 6) % 6]
                                          : lre::bayer_native(filters, (unsigned)row, (unsigned)col);
         const unsigned v = img4[i * 4 + cc];
@@ -305,8 +313,10 @@ __global__ __launch_bounds__(BLOCK) void aber_warp_kernel(unsigned short *img4, 
                                                          int c, int W, int H, float scale) {
     const size_t n = (size_t)W * H;
     const float cy = (float)H * 0.5f, cx = (float)W * 0.5f;
-    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK) {
-        const int row = (int)(i / W), col = (int)(i % W);
+    for (lre::GridWalk gw((size_t)blockIdx.x * BLOCK + threadIdx.x, n, (size_t)gridDim.x * BLOCK, W); gw.more();
+         gw.next()) {
+        const size_t i = gw.i;
+        const int row = (int)gw.row, col = (int)gw.col;
         const float sc = ((float)col - cx) * scale + cx;
         const float sr = ((float)row - cy) * scale + cy;
         const unsigned ur = f2u_x86(sr);

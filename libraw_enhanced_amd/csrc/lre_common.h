@@ -44,6 +44,32 @@ LRE_DEV void divmod(int t, int d, float inv_d, int &q, int &r) {
     else if (r >= d) { q++; r -= d; }
 }
 
+// A grid-stride walk over the n = rows * cols entries of a row-major image:
+// i = first, first + stride, ... < n, with (row, col) = (i / cols, i % cols)
+// advanced by the stride's quotient and remainder (one 64-bit division per
+// thread instead of one per entry).
+struct GridWalk {
+    size_t i, n, stride;
+    int row, col, cols, drow, dcol;
+    LRE_DEV GridWalk(size_t first, size_t n_, size_t stride_, int cols_) : i(first), n(n_), stride(stride_), cols(cols_) {
+        const size_t q = first / (size_t)cols_, dq = stride_ / (size_t)cols_;
+        row = (int)q;
+        col = (int)(first - q * (size_t)cols_);
+        drow = (int)dq;
+        dcol = (int)(stride_ - dq * (size_t)cols_);
+    }
+    LRE_DEV bool more() const { return i < n; }
+    LRE_DEV void next() {
+        i += stride;
+        row += drow;
+        col += dcol;
+        if (col >= cols) {
+            col -= cols;
+            row++;
+        }
+    }
+};
+
 // Stages a rows x cols window into LDS: element (r, c) = load(r, c), written by
 // store(r, c, value).  Each thread issues U independent global loads before
 // its first LDS store, so their latencies overlap.

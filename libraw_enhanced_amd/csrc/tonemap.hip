@@ -389,9 +389,10 @@ struct XTransTable { signed char v[36]; };
 __global__ void wb_kernel(const ushort4 *__restrict__ img4, float *__restrict__ rgb, int W, int H,
                           float4 wb, uint32_t filters, XTransTable xt) {
     const size_t n = (size_t)W * H;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x) {
-        const unsigned row = (unsigned)(i / W), col = (unsigned)(i % W);
+    for (lre::GridWalk gw((size_t)blockIdx.x * blockDim.x + threadIdx.x, n, (size_t)gridDim.x * blockDim.x, W);
+         gw.more(); gw.next()) {
+        const size_t i = gw.i;
+        const unsigned row = (unsigned)gw.row, col = (unsigned)gw.col;
         unsigned c = filters == 9 ? (unsigned)xt.v[(row % 6) * 6 + col % 6]
                                   : lre::bayer_native(filters, row, col);
         const ushort4 s = img4[i];
@@ -406,9 +407,10 @@ __global__ void wb_kernel(const ushort4 *__restrict__ img4, float *__restrict__ 
 __global__ void rgb3_to_cfa_kernel(const float *__restrict__ rgb3, float *__restrict__ cfa, int W,
                                    int H, uint32_t filters, XTransTable xt) {
     const size_t n = (size_t)W * H;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x) {
-        const unsigned row = (unsigned)(i / W), col = (unsigned)(i % W);
+    for (lre::GridWalk gw((size_t)blockIdx.x * blockDim.x + threadIdx.x, n, (size_t)gridDim.x * blockDim.x, W);
+         gw.more(); gw.next()) {
+        const size_t i = gw.i;
+        const unsigned row = (unsigned)gw.row, col = (unsigned)gw.col;
         const unsigned c = filters == 9 ? (unsigned)xt.v[(row % 6) * 6 + col % 6]
                                         : lre::bayer3(filters, row, col);
         cfa[i] = rgb3[i * 3 + c];
