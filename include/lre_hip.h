@@ -320,7 +320,13 @@ int lre_hip_dev_epilogue(lre_ctx *ctx, const float *d_rgb, void *d_out, int widt
  * and defaults) over many device-resident images. */
 
 /* Stage codes of an ordered stage list.  LRE_STAGE_DEMOSAIC, when present, must
- * come first: AMaZE for Bayer images, X-Trans 3-pass for filters == 9. */
+ * come first (AMaZE for Bayer images, X-Trans 3-pass for filters == 9), after
+ * the raw-preprocessing stages when those are listed: LRE_STAGE_REMOVE_ZEROES
+ * and LRE_STAGE_GREEN_MATCHING, in that order, each at most once, followed by
+ * LRE_STAGE_DEMOSAIC.  They act on Bayer images with uint16 input in the
+ * reference's order (core/libraw_wrapper.cpp:1496-1547): zero removal on the
+ * raw samples, black subtraction and clip, green matching with the white level
+ * as maximum, then white balance; float-CFA and X-Trans images skip them. */
 enum {
     LRE_STAGE_DEMOSAIC = 1,
     LRE_STAGE_LATERAL_CA = 2,
@@ -337,7 +343,11 @@ enum {
     /* enhance_micro_contrast as postprocess calls it (:1698-1705): threshold
      * highlight_threshold + 0.3, micro_strength, micro_target_contrast, and the
      * highlight mask when a highlight stage came earlier in the list */
-    LRE_STAGE_MICRO_CONTRAST = 10
+    LRE_STAGE_MICRO_CONTRAST = 10,
+    /* remove_zeroes (core/libraw_wrapper.cpp:150-212), as lre_hip_remove_zeroes */
+    LRE_STAGE_REMOVE_ZEROES = 11,
+    /* apply_green_matching (:496-585), as lre_hip_green_matching */
+    LRE_STAGE_GREEN_MATCHING = 12
 };
 
 /* Parameters of the post-demosaic stages. */

@@ -116,32 +116,55 @@ __global__ void adjust_maximum_kernel(const float *d_data_max, float maximum, fl
     *d_maxv = m;
 }
 
+// The uint16 raw image of the preprocessing stages, as the reference holds it
+// (uint16 x 4 per pixel, only the channel of the pixel's Bayer colour set) or
+// as one sample per site.  get/set address channel ch of pixel (r, c); in the
+// plane, a channel other than the site's colour reads 0 and is not stored,
+// which is what the four-channel image shows to every later stage (they read
+// a pixel's own channel only).
+struct Img4View {
+    unsigned short *p;
+    int W;
+    LRE_DEV unsigned short get(int r, int c, unsigned ch) const { return p[((size_t)r * W + c) * 4 + ch]; }
+    LRE_DEV void set(int r, int c, unsigned ch, unsigned short v) const { p[((size_t)r * W + c) * 4 + ch] = v; }
+};
+struct PlaneView {
+    unsigned short *p;
+    int W;
+    uint32_t filters;
+    LRE_DEV bool own(int r, int c, unsigned ch) const {
+        return lre::bayer_native(filters, (unsigned)r, (unsigned)c) == ch;
+    }
+    LRE_DEV unsigned short get(int r, int c, unsigned ch) const { return own(r, c, ch) ? p[(size_t)r * W + c] : 0; }
+    LRE_DEV void set(int r, int c, unsigned ch, unsigned short v) const {
+        if (own(r, c, ch)) p[(size_t)r * W + c] = v;
+    }
+};
+
 // ---- green matching (:496-585): reads src, writes the channel-3 samples it
 // corrects into dst (a copy of src) ----
-__global__ __launch_bounds__(BLOCK) void green_match_kernel(const ushort4 *__restrict__ src,
-                                                           ushort4 *__restrict__ dst, int W, int H,
-                                                           int maximum) {
+template <class V>
+__global__ __launch_bounds__(BLOCK) void green_match_kernel(V src, V dst, int W, int H, int maximum) {
     const int nj = (H - 3 - 3 + 1) / 2, ni = (W - 3 - 2 + 1) / 2;  // j = 3,5,.. < H-3; i = 2,4,.. < W-3
     if (nj <= 0 || ni <= 0) return;
     const size_t n = (size_t)nj * ni;
     const float thr = 0.01f;
     for (size_t t = (size_t)blockIdx.x * BLOCK + threadIdx.x; t < n; t += (size_t)gridDim.x * BLOCK) {
         const int j = 3 + 2 * (int)(t / ni), i = 2 + 2 * (int)(t % ni);
-        const int o1_1 = src[(size_t)(j - 1) * W + i - 1].y, o1_2 = src[(size_t)(j - 1) * W + i + 1].y;
-        const int o1_3 = src[(size_t)(j + 1) * W + i - 1].y, o1_4 = src[(size_t)(j + 1) * W + i + 1].y;
-        const int o2_1 = src[(size_t)(j - 2) * W + i].w, o2_2 = src[(size_t)(j + 2) * W + i].w;
-        const int o2_3 = src[(size_t)j * W + i - 2].w, o2_4 = src[(size_t)j * W + i + 2].w;
+        const int o1_1 = src.get(j - 1, i - 1, 1), o1_2 = src.get(j - 1, i + 1, 1);
+        const int o1_3 = src.get(j + 1, i - 1, 1), o1_4 = src.get(j + 1, i + 1, 1);
+        const int o2_1 = src.get(j - 2, i, 3), o2_2 = src.get(j + 2, i, 3);
+        const int o2_3 = src.get(j, i - 2, 3), o2_4 = src.get(j, i + 2, 3);
         const double m1 = (o1_1 + o1_2 + o1_3 + o1_4) / 4.0;
         const double m2 = (o2_1 + o2_2 + o2_3 + o2_4) / 4.0;
         const double c1 = (abs(o1_1 - o1_2) + abs(o1_1 - o1_3) + abs(o1_1 - o1_4) + abs(o1_2 - o1_3) +
                            abs(o1_3 - o1_4) + abs(o1_2 - o1_4)) / 6.0;
         const double c2 = (abs(o2_1 - o2_2) + abs(o2_1 - o2_3) + abs(o2_1 - o2_4) + abs(o2_2 - o2_3) +
                            abs(o2_3 - o2_4) + abs(o2_2 - o2_4)) / 6.0;
-        const int g2 = src[(size_t)j * W + i].w;
+        const int g2 = src.get(j, i, 3);
         if ((g2 < maximum * 0.95) && (c1 < maximum * thr) && (c2 < maximum * thr) && (m2 > 0.1)) {
             const float correction = (float)(g2 * m1 / m2);
-            dst[(size_t)j * W + i].w =
-                correction > maximum ? (unsigned short)maximum : (unsigned short)correction;
+            dst.set(j, i, 3, correction > maximum ? (unsigned short)maximum : (unsigned short)correction);
         }
     }
 }
@@ -152,12 +175,9 @@ __global__ __launch_bounds__(BLOCK) void green_match_kernel(const ushort4 *__res
 // parallel: their windows hold no other independent zero.  The rest are
 // replaced afterwards by one wavefront in raster order. ----
 LRE_DEV unsigned chan(uint32_t filters, int r, int c) { return lre::bayer_native(filters, (unsigned)r, (unsigned)c); }
-LRE_DEV unsigned short sample(const unsigned short *img4, int W, int r, int c, unsigned ch) {
-    return img4[((size_t)r * W + c) * 4 + ch];
-}
 
-__global__ __launch_bounds__(BLOCK) void zero_classify_kernel(const unsigned short *__restrict__ img4,
-                                                             int W, int H, uint32_t filters,
+template <class V>
+__global__ __launch_bounds__(BLOCK) void zero_classify_kernel(V img, int W, int H, uint32_t filters,
                                                              unsigned char *__restrict__ dep,
                                                              unsigned *__restrict__ rowflag) {
     const size_t n = (size_t)W * H;
@@ -167,13 +187,13 @@ __global__ __launch_bounds__(BLOCK) void zero_classify_kernel(const unsigned sho
         const int row = (int)gw.row, col = (int)gw.col;
         const unsigned ch = chan(filters, row, col);
         unsigned char d = 0;
-        if (sample(img4, W, row, col, ch) == 0) {
+        if (img.get(row, col, ch) == 0) {
             d = 1;  // independent zero
             for (int r = row - 2; r <= row && d == 1; r++)
                 for (int c = col - 2; c <= col + 2; c++) {
                     if (r == row && c >= col) break;
                     if (r < 0 || c < 0 || c >= W) continue;
-                    if (chan(filters, r, c) == ch && sample(img4, W, r, c, ch) == 0) { d = 2; break; }
+                    if (chan(filters, r, c) == ch && img.get(r, c, ch) == 0) { d = 2; break; }
                 }
             if (d == 2) rowflag[row] = 1u;
         }
@@ -181,22 +201,23 @@ __global__ __launch_bounds__(BLOCK) void zero_classify_kernel(const unsigned sho
     }
 }
 
-LRE_DEV unsigned short window_mean(const unsigned short *img4, int W, int H, uint32_t filters,
-                                   int row, int col, unsigned ch, bool &any) {
+template <class V>
+LRE_DEV unsigned short window_mean(const V &img, int W, int H, uint32_t filters, int row, int col, unsigned ch,
+                                   bool &any) {
     unsigned sum = 0, cnt = 0;
     for (int r = row - 2; r <= row + 2; r++)
         for (int c = col - 2; c <= col + 2; c++) {
             if (r < 0 || r >= H || c < 0 || c >= W) continue;
             const unsigned nch = chan(filters, r, c);
-            const unsigned short v = sample(img4, W, r, c, nch);
+            const unsigned short v = img.get(r, c, nch);
             if (nch == ch && v != 0) { sum += v; cnt++; }
         }
     any = cnt > 0;
     return any ? (unsigned short)(sum / cnt) : 0;
 }
 
-__global__ __launch_bounds__(BLOCK) void zero_fill_independent(unsigned short *img4, int W, int H,
-                                                              uint32_t filters,
+template <class V>
+__global__ __launch_bounds__(BLOCK) void zero_fill_independent(V img, int W, int H, uint32_t filters,
                                                               const unsigned char *__restrict__ dep) {
     const size_t n = (size_t)W * H;
     for (lre::GridWalk gw((size_t)blockIdx.x * BLOCK + threadIdx.x, n, (size_t)gridDim.x * BLOCK, W); gw.more();
@@ -206,15 +227,15 @@ __global__ __launch_bounds__(BLOCK) void zero_fill_independent(unsigned short *i
         const int row = (int)gw.row, col = (int)gw.col;
         const unsigned ch = chan(filters, row, col);
         bool any;
-        const unsigned short v = window_mean(img4, W, H, filters, row, col, ch, any);
-        if (any) img4[i * 4 + ch] = v;
+        const unsigned short v = window_mean(img, W, H, filters, row, col, ch, any);
+        if (any) img.set(row, col, ch, v);
     }
 }
 
 // one wavefront: rows in order; within a flagged row, 64 columns at a time,
 // the flagged zeros of the chunk one after another (lowest column first)
-__global__ __launch_bounds__(64) void zero_fill_dependent(unsigned short *img4, int W, int H,
-                                                         uint32_t filters,
+template <class V>
+__global__ __launch_bounds__(64) void zero_fill_dependent(V img, int W, int H, uint32_t filters,
                                                          const unsigned char *__restrict__ dep,
                                                          const unsigned *__restrict__ rowflag) {
     const int lane = threadIdx.x;
@@ -231,8 +252,8 @@ __global__ __launch_bounds__(64) void zero_fill_dependent(unsigned short *img4, 
                 if (lane == 0) {
                     const unsigned ch = chan(filters, row, col);
                     bool any;
-                    const unsigned short v = window_mean(img4, W, H, filters, row, col, ch, any);
-                    if (any) img4[((size_t)row * W + col) * 4 + ch] = v;
+                    const unsigned short v = window_mean(img, W, H, filters, row, col, ch, any);
+                    if (any) img.set(row, col, ch, v);
                 }
                 // the next zero of the chunk reads this one's value
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
@@ -341,6 +362,49 @@ __global__ void zero_bytes(unsigned *p, size_t n4) {
 
 }  // namespace
 
+namespace {
+
+// green matching over a view; dst already holds a copy of src
+template <class V>
+int green_match_on(V src, V dst, int w, int h, int maximum, hipStream_t s) {
+    hipLaunchKernelGGL((green_match_kernel<V>), dim3(grid_for((size_t)w * h / 4)), dim3(BLOCK), 0, s, src, dst,
+                       w, h, maximum);
+    LRE_HIP_CHECK(hipGetLastError());
+    return LRE_OK;
+}
+
+template <class V>
+int remove_zeroes_on(V img, int w, int h, uint32_t filters, void *d_scratch, hipStream_t s) {
+    unsigned char *dep = static_cast<unsigned char *>(d_scratch);
+    unsigned *rowflag = reinterpret_cast<unsigned *>(dep + (((size_t)w * h + 255) & ~(size_t)255));
+    const size_t nflag = ((size_t)h * 4 + 255) / 4;
+    hipLaunchKernelGGL(zero_bytes, dim3(grid_for(nflag)), dim3(BLOCK), 0, s, rowflag, nflag);
+    const int g = grid_for((size_t)w * h);
+    hipLaunchKernelGGL((zero_classify_kernel<V>), dim3(g), dim3(BLOCK), 0, s, img, w, h, filters, dep, rowflag);
+    hipLaunchKernelGGL((zero_fill_independent<V>), dim3(g), dim3(BLOCK), 0, s, img, w, h, filters, dep);
+    hipLaunchKernelGGL((zero_fill_dependent<V>), dim3(1), dim3(64), 0, s, img, w, h, filters, dep, rowflag);
+    LRE_HIP_CHECK(hipGetLastError());
+    return LRE_OK;
+}
+
+// black subtraction and clip of a one-sample-per-site plane, in place: the
+// channel of each site as the fused ingest takes it (:401-491)
+__global__ __launch_bounds__(BLOCK) void black_plane_kernel(unsigned short *plane, int W, int H, uint32_t filters,
+                                                           XT xt, Black B) {
+    const size_t n = (size_t)W * H;
+    for (lre::GridWalk gw((size_t)blockIdx.x * BLOCK + threadIdx.x, n, (size_t)gridDim.x * BLOCK, W); gw.more();
+         gw.next()) {
+        const unsigned row = (unsigned)gw.row, col = (unsigned)gw.col;
+        const unsigned c4 = filters == 9 ? (unsigned)xt.v[(row % 6) * 6 + col % 6]
+                                         : lre::bayer_native(filters, row, col);
+        int v = plane[gw.i];
+        v -= B.any ? B.c[c4] : B.black;
+        plane[gw.i] = (unsigned short)clampi(v, 0, B.maximum);
+    }
+}
+
+}  // namespace
+
 int lre_launch_green_matching(const uint16_t *d_src, uint16_t *d_dst, int w, int h, uint32_t filters,
                               int maximum, hipStream_t s) {
     if (d_src != d_dst)
@@ -350,11 +414,22 @@ int lre_launch_green_matching(const uint16_t *d_src, uint16_t *d_dst, int w, int
         lre_set_error("green matching reads the unmodified image: source and destination must differ");
         return LRE_ERR_ARG;
     }
-    hipLaunchKernelGGL(green_match_kernel, dim3(grid_for((size_t)w * h / 4)), dim3(BLOCK), 0, s,
-                       reinterpret_cast<const ushort4 *>(d_src), reinterpret_cast<ushort4 *>(d_dst), w,
-                       h, maximum);
-    LRE_HIP_CHECK(hipGetLastError());
-    return LRE_OK;
+    const Img4View src{const_cast<unsigned short *>(reinterpret_cast<const unsigned short *>(d_src)), w};
+    const Img4View dst{reinterpret_cast<unsigned short *>(d_dst), w};
+    return green_match_on(src, dst, w, h, maximum, s);
+}
+
+int lre_launch_green_matching_plane(const uint16_t *d_src, uint16_t *d_dst, int w, int h, uint32_t filters,
+                                    int maximum, hipStream_t s) {
+    if (d_src == d_dst) {
+        lre_set_error("green matching reads the unmodified image: source and destination must differ");
+        return LRE_ERR_ARG;
+    }
+    LRE_HIP_CHECK(hipMemcpyAsync(d_dst, d_src, (size_t)w * h * 2, hipMemcpyDeviceToDevice, s));
+    if (filters == 9) return LRE_OK;
+    const PlaneView src{const_cast<unsigned short *>(reinterpret_cast<const unsigned short *>(d_src)), w, filters};
+    const PlaneView dst{reinterpret_cast<unsigned short *>(d_dst), w, filters};
+    return green_match_on(src, dst, w, h, maximum, s);
 }
 
 size_t lre_remove_zeroes_scratch_bytes(int w, int h) {
@@ -363,14 +438,23 @@ size_t lre_remove_zeroes_scratch_bytes(int w, int h) {
 
 int lre_launch_remove_zeroes(uint16_t *d_img4, int w, int h, uint32_t filters, void *d_scratch,
                              hipStream_t s) {
-    unsigned char *dep = static_cast<unsigned char *>(d_scratch);
-    unsigned *rowflag = reinterpret_cast<unsigned *>(dep + (((size_t)w * h + 255) & ~(size_t)255));
-    const size_t nflag = ((size_t)h * 4 + 255) / 4;
-    hipLaunchKernelGGL(zero_bytes, dim3(grid_for(nflag)), dim3(BLOCK), 0, s, rowflag, nflag);
-    const int g = grid_for((size_t)w * h);
-    hipLaunchKernelGGL(zero_classify_kernel, dim3(g), dim3(BLOCK), 0, s, d_img4, w, h, filters, dep, rowflag);
-    hipLaunchKernelGGL(zero_fill_independent, dim3(g), dim3(BLOCK), 0, s, d_img4, w, h, filters, dep);
-    hipLaunchKernelGGL(zero_fill_dependent, dim3(1), dim3(64), 0, s, d_img4, w, h, filters, dep, rowflag);
+    return remove_zeroes_on(Img4View{reinterpret_cast<unsigned short *>(d_img4), w}, w, h, filters, d_scratch, s);
+}
+
+int lre_launch_remove_zeroes_plane(uint16_t *d_plane, int w, int h, uint32_t filters, void *d_scratch,
+                                   hipStream_t s) {
+    return remove_zeroes_on(PlaneView{reinterpret_cast<unsigned short *>(d_plane), w, filters}, w, h, filters,
+                            d_scratch, s);
+}
+
+int lre_launch_black_plane(uint16_t *d_plane, int w, int h, uint32_t filters, const signed char *xtrans,
+                           const int cblack[4], int black, int maximum, hipStream_t s) {
+    XT xt{};
+    if (xtrans)
+        for (int i = 0; i < 36; i++) xt.v[i] = xtrans[i];
+    hipLaunchKernelGGL(black_plane_kernel, dim3(grid_for((size_t)w * h)), dim3(BLOCK), 0, s,
+                       reinterpret_cast<unsigned short *>(d_plane), w, h, filters, xt,
+                       make_black(cblack, black, maximum));
     LRE_HIP_CHECK(hipGetLastError());
     return LRE_OK;
 }

@@ -158,6 +158,15 @@ int lre_launch_epilogue(const float *d_in, void *d_out, size_t npx, const float 
 int lre_launch_green_matching(const uint16_t *d_src, uint16_t *d_dst, int w, int h, uint32_t filters,
                               int maximum, hipStream_t s);
 size_t lre_remove_zeroes_scratch_bytes(int w, int h);
+// The same two stages on a one-sample-per-site uint16 plane (Bayer), and the
+// black subtraction and clip of such a plane in place; the zero removal
+// scratch is lre_remove_zeroes_scratch_bytes.
+int lre_launch_remove_zeroes_plane(uint16_t *d_plane, int w, int h, uint32_t filters, void *d_scratch,
+                                   hipStream_t s);
+int lre_launch_green_matching_plane(const uint16_t *d_src, uint16_t *d_dst, int w, int h, uint32_t filters,
+                                    int maximum, hipStream_t s);
+int lre_launch_black_plane(uint16_t *d_plane, int w, int h, uint32_t filters, const signed char *xtrans,
+                           const int cblack[4], int black, int maximum, hipStream_t s);
 int lre_launch_remove_zeroes(uint16_t *d_img4, int w, int h, uint32_t filters, void *d_scratch,
                              hipStream_t s);
 // Automatic white balance (d_scratch: 64 bytes) and legacy radial aberration.

@@ -21,7 +21,15 @@ constexpr int MAX_LANES = 8;
 
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-bool valid_stage(int s) { return s >= LRE_STAGE_DEMOSAIC && s <= LRE_STAGE_MICRO_CONTRAST; }
+bool valid_stage(int s) { return s >= LRE_STAGE_DEMOSAIC && s <= LRE_STAGE_GREEN_MATCHING; }
+bool pre_stage(int s) { return s == LRE_STAGE_REMOVE_ZEROES || s == LRE_STAGE_GREEN_MATCHING; }
+
+// leading raw-preprocessing stages (zero removal, green matching) of a list
+int pre_count(const int *stages, int ns) {
+    int k = 0;
+    while (k < ns && pre_stage(stages[k])) k++;
+    return k;
+}
 
 bool has_stage(const int *stages, int ns, int st) {
     for (int i = 0; i < ns; i++)
@@ -45,12 +53,69 @@ int check_stages(const int *stages, int ns) {
             lre_set_error("unknown stage code %d", stages[i]);
             return LRE_ERR_ARG;
         }
-        if (stages[i] == LRE_STAGE_DEMOSAIC && i != 0) {
-            lre_set_error("the demosaic stage must come first");
+    }
+    // [zero removal] [green matching] demosaic ..., or a list without these
+    const int k = pre_count(stages, ns);
+    if (k == 2 && !(stages[0] == LRE_STAGE_REMOVE_ZEROES && stages[1] == LRE_STAGE_GREEN_MATCHING)) {
+        lre_set_error("zero removal comes before green matching, once each");
+        return LRE_ERR_ARG;
+    }
+    if (k > 2 || (k > 0 && (k == ns || stages[k] != LRE_STAGE_DEMOSAIC))) {
+        lre_set_error("zero removal and green matching go first, once each, followed by the demosaic");
+        return LRE_ERR_ARG;
+    }
+    for (int i = k + 1; i < ns; i++)
+        if (stages[i] == LRE_STAGE_DEMOSAIC || pre_stage(stages[i])) {
+            lre_set_error("the demosaic stage must come first, after zero removal and green matching");
             return LRE_ERR_ARG;
         }
+    if (k == 0 && ns > 0 && stages[0] != LRE_STAGE_DEMOSAIC && has_stage(stages, ns, LRE_STAGE_DEMOSAIC)) {
+        lre_set_error("the demosaic stage must come first");
+        return LRE_ERR_ARG;
     }
     return LRE_OK;
+}
+
+// Raw preprocessing of one uint16 image before the demosaic
+// (core/libraw_wrapper.cpp:1496-1547): zero removal on the raw samples, black
+// subtraction and clip, green matching, then white balance into the float CFA
+// plane with the data maximum, and adjust_maximum.  Without zero removal and
+// green matching the first three run fused in the ingest kernel.  work: a
+// uint16 plane (the raw samples are copied there unless it is d_raw itself),
+// work2: a second plane for green matching, zscr: zero removal scratch.
+// Bayer only for the two optional stages (X-Trans images skip them: green
+// matching is a no-op there in the reference).
+int enqueue_ingest(const lre_image &im, const uint16_t *d_raw, bool zero, bool green, uint16_t *work,
+                   uint16_t *work2, void *zscr, float *cfa, float *stat, hipStream_t s) {
+    const int w = im.width, h = im.height;
+    const signed char *xt = im.filters == 9 ? im.xtrans : nullptr;
+    const bool pre = (zero || green) && im.filters != 9;
+    int rc;
+    if (!pre) {
+        rc = lre_launch_ingest_u16(d_raw, cfa, w, h, im.filters, xt, im.cblack, im.black, im.white, im.wb, stat, s);
+    } else {
+        if (work != d_raw)
+            LRE_HIP_CHECK(hipMemcpyAsync(work, d_raw, (size_t)w * h * 2, hipMemcpyDeviceToDevice, s));
+        if (zero && (rc = lre_launch_remove_zeroes_plane(work, w, h, im.filters, zscr, s))) return rc;
+        if ((rc = lre_launch_black_plane(work, w, h, im.filters, xt, im.cblack, im.black, im.white, s))) return rc;
+        const uint16_t *src = work;
+        if (green) {
+            if ((rc = lre_launch_green_matching_plane(work, work2, w, h, im.filters, im.white, s))) return rc;
+            src = work2;
+        }
+        const int no_black[4] = {0, 0, 0, 0};
+        rc = lre_launch_ingest_u16(src, cfa, w, h, im.filters, xt, no_black, 0, im.white, im.wb, stat, s);
+    }
+    if (rc) return rc;
+    return lre_launch_adjust_maximum(stat, (float)(im.white - im.black), im.adjust_threshold, stat + 1, s);
+}
+
+// device bytes of that preprocessing beyond the raw samples (0 without it)
+size_t pre_bytes(const lre_image &im, bool zero, bool green, bool own_raw) {
+    if (!(zero || green) || im.filters == 9) return 0;
+    const size_t plane = align256((size_t)im.width * im.height * 2);
+    return (own_raw ? 0 : plane) + (green ? plane : 0) +
+           (zero ? align256(lre_remove_zeroes_scratch_bytes(im.width, im.height)) : 0);
 }
 
 // The parameters of the stages in the list, checked before anything is
@@ -378,6 +443,8 @@ struct Plan {
     bool demosaic = false;
     std::vector<size_t> ingest_off;  // per image offset into the ingest workspace (or SIZE_MAX)
     size_t ingest_bytes = 0;
+    int first = 0;                   // index of the first post-demosaic stage
+    bool zero = false, green = false;  // raw preprocessing stages
     size_t lane_scratch = 0;
     bool grouped = false;            // post stages as multi-image passes
     std::vector<Group> groups;
@@ -501,7 +568,11 @@ int make_plan(lre_batch *b, const lre_image *imgs, int n, const int *stages, int
         lre_set_error("invalid image list");
         return LRE_ERR_ARG;
     }
-    P.demosaic = ns > 0 && stages[0] == LRE_STAGE_DEMOSAIC;
+    const int k = pre_count(stages, ns);
+    P.demosaic = k < ns && stages[k] == LRE_STAGE_DEMOSAIC;
+    P.first = P.demosaic ? k + 1 : 0;
+    P.zero = has_stage(stages, k, LRE_STAGE_REMOVE_ZEROES);
+    P.green = has_stage(stages, k, LRE_STAGE_GREEN_MATCHING);
     P.ingest_off.assign(n, SIZE_MAX);
     for (int i = 0; i < n; i++) {
         if ((rc = check_image(imgs[i], P.demosaic))) return rc;
@@ -521,13 +592,14 @@ int make_plan(lre_batch *b, const lre_image *imgs, int n, const int *stages, int
             j.d_maxv = nullptr;
         } else {
             // workspace: float CFA plane + {data_max, maximum}
+            // float CFA plane + {data_max, maximum}, then the preprocessing planes
             P.ingest_off[i] = P.ingest_bytes;
-            P.ingest_bytes += align256((size_t)im.width * im.height * 4) + 256;
+            P.ingest_bytes += align256((size_t)im.width * im.height * 4) + 256 + pre_bytes(im, P.zero, P.green, false);
             j.maximum_value = (float)(im.white - im.black);
         }
         (im.filters == 9 ? P.xtrans : P.bayer).push_back(j);
     }
-    const int from = P.demosaic ? 1 : 0;
+    const int from = P.first;
     // By image size: one launch per stage over many images pays where images
     // are small (launch and tail bound); large images fill the device alone
     // and keep their stages' data warm in the caches when each one runs its
@@ -712,20 +784,22 @@ int enqueue(lre_batch *b, const Plan &P, const lre_image *imgs, int n, const int
     for (int k = 1; k < lanes; k++) LRE_HIP_CHECK(hipStreamWaitEvent(L[k].stream, b->ev_fork, 0));
     int rc;
     if (P.demosaic) {
-        // uint16 ingest + device-side maximum (lane 0)
+        // uint16 preprocessing + device-side maximum (lane 0)
         char *ws = static_cast<char *>(b->ingest_ws.p);
         for (int i = 0; i < n; i++) {
             if (P.ingest_off[i] == SIZE_MAX) continue;
             const lre_image &im = imgs[i];
-            float *cfa = reinterpret_cast<float *>(ws + P.ingest_off[i]);
-            float *stat = reinterpret_cast<float *>(ws + P.ingest_off[i] + align256((size_t)im.width * im.height * 4));
-            if ((rc = lre_launch_ingest_u16(im.d_raw, cfa, im.width, im.height, im.filters,
-                                            im.filters == 9 ? im.xtrans : nullptr, im.cblack,
-                                            im.black, im.white, im.wb, stat, s0)))
-                return rc;
-            if ((rc = lre_launch_adjust_maximum(stat, (float)(im.white - im.black),
-                                                im.adjust_threshold, stat + 1, s0)))
-                return rc;
+            const size_t px = (size_t)im.width * im.height;
+            char *base = ws + P.ingest_off[i];
+            float *cfa = reinterpret_cast<float *>(base);
+            float *stat = reinterpret_cast<float *>(base + align256(px * 4));
+            // preprocessing region: the working copy of the raw samples, the
+            // green matching output, the zero removal scratch
+            char *pre = base + align256(px * 4) + 256;
+            uint16_t *work = reinterpret_cast<uint16_t *>(pre);
+            uint16_t *work2 = reinterpret_cast<uint16_t *>(pre + align256(px * 2));
+            void *zscr = pre + align256(px * 2) * (P.green ? 2 : 1);
+            if ((rc = enqueue_ingest(im, im.d_raw, P.zero, P.green, work, work2, zscr, cfa, stat, s0))) return rc;
         }
         hipStream_t sx = lanes > 1 ? L[1].stream : s0;
         if (!P.xtrans.empty() && sx != s0) {
@@ -749,7 +823,7 @@ int enqueue(lre_batch *b, const Plan &P, const lre_image *imgs, int n, const int
                 LRE_HIP_CHECK(hipStreamWaitEvent(L[k].stream, b->ev_demosaic[1], 0));
         }
     }
-    const int first = P.demosaic ? 1 : 0;
+    const int first = P.first;
     if (P.grouped) {
         if ((rc = enqueue_groups(P, imgs, p, d_post, L))) return rc;
     } else if (ns > first) {
@@ -1124,7 +1198,7 @@ struct lre_server {
 namespace {
 
 struct Footprint {
-    size_t raw, cfa, rgb, scratch, total;
+    size_t raw, cfa, rgb, scratch, pre, total;
 };
 
 Footprint footprint(const lre_server *S, const lre_image &im) {
@@ -1134,7 +1208,11 @@ Footprint footprint(const lre_server *S, const lre_image &im) {
     f.cfa = align256(px * 4) + 256;
     f.rgb = align256(px * 12);
     f.scratch = align256(post_scratch(im, S->stages.data(), (int)S->stages.size(), S->params));
-    f.total = f.raw + f.cfa + f.rgb + f.scratch;
+    const int k = pre_count(S->stages.data(), (int)S->stages.size());
+    // the raw samples are the request's own copy: preprocessed in place
+    f.pre = pre_bytes(im, has_stage(S->stages.data(), k, LRE_STAGE_REMOVE_ZEROES),
+                      has_stage(S->stages.data(), k, LRE_STAGE_GREEN_MATCHING), true);
+    f.total = f.raw + f.cfa + f.rgb + f.scratch + f.pre;
     return f;
 }
 
@@ -1168,13 +1246,13 @@ int serve_one_enqueue(lre_server *S, Lane &L, const lre_request &rq, char *dev, 
                                                                     : lre_amaze_multi_table_bytes(1)));
     int rc;
     LRE_HIP_CHECK(hipMemcpyAsync(d_raw, rq.h_raw, px * 2, hipMemcpyHostToDevice, s));
-    if ((rc = lre_launch_ingest_u16(d_raw, d_cfa, im.width, im.height, im.filters,
-                                    im.filters == 9 ? im.xtrans : nullptr, im.cblack, im.black,
-                                    im.white, im.wb, d_stat, s)))
-        return rc;
-    if ((rc = lre_launch_adjust_maximum(d_stat, (float)(im.white - im.black), im.adjust_threshold,
-                                        d_stat + 1, s)))
-        return rc;
+    const int k = pre_count(S->stages.data(), (int)S->stages.size());
+    const bool zero = has_stage(S->stages.data(), k, LRE_STAGE_REMOVE_ZEROES);
+    const bool green = has_stage(S->stages.data(), k, LRE_STAGE_GREEN_MATCHING);
+    char *d_pre = dev + f.raw + f.cfa + f.rgb + f.scratch;  // green matching output, then zero removal scratch
+    uint16_t *work2 = reinterpret_cast<uint16_t *>(d_pre);
+    void *zscr = d_pre + (green ? align256(px * 2) : 0);
+    if ((rc = enqueue_ingest(im, d_raw, zero, green, d_raw, work2, zscr, d_cfa, d_stat, s))) return rc;
     lre_demosaic_job j{};
     j.d_cfa = d_cfa;
     j.d_rgb = d_rgb;
@@ -1185,7 +1263,7 @@ int serve_one_enqueue(lre_server *S, Lane &L, const lre_request &rq, char *dev, 
     std::memcpy(j.cam_mul, im.cam_mul, sizeof(j.cam_mul));
     j.maximum_value = (float)(im.white - im.black);
     j.d_maxv = d_stat + 1;
-    const bool has_demosaic = !S->stages.empty() && S->stages[0] == LRE_STAGE_DEMOSAIC;
+    const bool has_demosaic = k < (int)S->stages.size() && S->stages[k] == LRE_STAGE_DEMOSAIC;
     if (has_demosaic) {
         if (im.filters == 9) {
             const int t = lre_xtrans_multi_fill(&j, 1, h_tab);
@@ -1203,7 +1281,7 @@ int serve_one_enqueue(lre_server *S, Lane &L, const lre_request &rq, char *dev, 
     }
     lre_image out = im;
     out.d_rgb = d_rgb;
-    if ((rc = run_post_chain(S->stages.data(), has_demosaic ? 1 : 0, (int)S->stages.size(), out, S->params,
+    if ((rc = run_post_chain(S->stages.data(), has_demosaic ? k + 1 : 0, (int)S->stages.size(), out, S->params,
                              d_scr, s)))
         return rc;
     LRE_HIP_CHECK(hipMemcpyAsync(rq.h_rgb, d_rgb, px * 12, hipMemcpyDeviceToHost, s));
@@ -1306,8 +1384,10 @@ int lre_hip_serve_create(lre_server **out, int device, int lanes, size_t budget,
     }
     int rc = check_stages(stages, ns);
     if (rc) return rc;
-    // requests carry a uint16 CFA plane only: the chain must start by demosaicing it
-    if (ns < 1 || stages[0] != LRE_STAGE_DEMOSAIC) {
+    // requests carry a uint16 CFA plane only: the chain must demosaic it first
+    // (after zero removal and green matching, when listed)
+    const int k = pre_count(stages, ns);
+    if (k >= ns || stages[k] != LRE_STAGE_DEMOSAIC) {
         lre_set_error("a served stage list must start with the demosaic stage");
         return LRE_ERR_ARG;
     }

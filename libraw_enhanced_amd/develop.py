@@ -32,12 +32,15 @@ STAGE_COLOR_SPACE = 7
 STAGE_GAMMA = 8
 STAGE_HIGHLIGHT = 9
 STAGE_MICRO_CONTRAST = 10
+STAGE_REMOVE_ZEROES = 11
+STAGE_GREEN_MATCHING = 12
 
 STAGE_NAMES = {
     "demosaic": STAGE_DEMOSAIC, "lateral_ca": STAGE_LATERAL_CA, "axial_ca": STAGE_AXIAL_CA,
     "defringe": STAGE_DEFRINGE, "detail_tm": STAGE_DETAIL_TM, "tone_mapping": STAGE_TONE_MAPPING,
     "color_space": STAGE_COLOR_SPACE, "gamma": STAGE_GAMMA,
     "highlight": STAGE_HIGHLIGHT, "micro_contrast": STAGE_MICRO_CONTRAST,
+    "remove_zeroes": STAGE_REMOVE_ZEROES, "green_matching": STAGE_GREEN_MATCHING,
 }
 
 # demosaic -> axial CA -> defringe -> detail tone map

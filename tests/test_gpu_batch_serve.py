@@ -88,7 +88,10 @@ def _single(acc, it, stages, p):
         d_max = acc.dev_alloc(4)
         wb = np.array(WB, np.float32)
         cb = (ct.c_int * 4)(0, 0, 0, 0)
-        assert lib.lre_hip_dev_ingest_u16(ctx, it["d_in"].ptr, d_cfa.ptr, w, h, filters, xtp, cb, BLACK,
+        # an input already black-subtracted (preprocessed) is ingested with black 0
+        d_raw = it.get("d_pre", it["d_in"])
+        black_in = 0 if "d_pre" in it else BLACK
+        assert lib.lre_hip_dev_ingest_u16(ctx, d_raw.ptr, d_cfa.ptr, w, h, filters, xtp, cb, black_in,
                                           WHITE, wb.ctypes.data_as(ct.POINTER(ct.c_float)), d_max.ptr,
                                           None) == 0
         acc.synchronize()
@@ -254,6 +257,111 @@ def test_batch_defringe_blur_tiles(acc, group):
     b.close()
 
 
+FOUR = 0xB4B4B4B4  # R G / G2 B: the pattern green matching acts on
+
+
+def _preprocessed(raw, filters, zero, green):
+    """The reference's order on its four-channel raw image, by the oracle
+    (core/libraw_wrapper.cpp:1496-1547): zero removal, black subtraction and
+    clip, green matching; the sites' own channel back as a plane."""
+    from test_ingest_cpu import black_level, oracle_green_matching, oracle_remove_zeroes
+    h, w = raw.shape
+    r, c = np.arange(h)[:, None], np.arange(w)[None, :]
+    ch = ((filters >> ((((r << 1) & 14) | (c & 1)) << 1)) & 3).astype(np.int64)
+    img4 = np.zeros((h, w, 4), np.uint16)
+    np.put_along_axis(img4, ch[..., None], raw[..., None], axis=2)
+    if zero:
+        img4 = oracle_remove_zeroes(img4, filters)
+    img4 = black_level(img4, black=BLACK, maximum=WHITE)
+    if green:
+        img4 = oracle_green_matching(img4, filters, WHITE)
+    return np.ascontiguousarray(np.take_along_axis(img4, ch[..., None], axis=2)[..., 0])
+
+
+def _with_zeros(raw, seed):
+    """Dead sites: single zeros, and runs of zeros of one channel close enough
+    that a later one's window holds an earlier one (raster-order refill)."""
+    out = raw.copy()
+    rng = np.random.Generator(np.random.PCG64(seed))
+    h, w = out.shape
+    for _ in range(40):
+        out[rng.integers(0, h), rng.integers(0, w)] = 0
+    for _ in range(6):
+        y, x = int(rng.integers(4, h - 8)), int(rng.integers(4, w - 8))
+        out[y, x:x + 6:2] = 0
+        out[y + 2, x:x + 6:2] = 0
+    return out
+
+
+@pytest.mark.parametrize("stages", [("remove_zeroes", "demosaic", "axial_ca", "defringe", "detail_tm"),
+                                    ("green_matching", "demosaic", "defringe", "detail_tm"),
+                                    ("remove_zeroes", "green_matching", "demosaic", "detail_tm")])
+def test_batch_and_serve_raw_preprocessing(acc, stages):
+    """Zero removal and green matching ahead of the demosaic, in the batch (lane
+    and group mode, graph replay) and the request loop, against the oracle's
+    preprocessing on the four-channel image followed by the per-image entry
+    points: bit-identical.  X-Trans images skip both stages."""
+    from libraw_enhanced_amd import develop as dv
+    print(f"preprocessing {stages}", flush=True)
+    spec = [("bayer", 240, 320, FOUR, "u16"), ("bayer", 256, 384, RGGB, "u16"), ("xtrans", 252, 330, 9, "u16"),
+            ("bayer", 200, 260, FOUR, "f32")]
+    S = _Set(acc, spec)
+    zero, green = "remove_zeroes" in stages, "green_matching" in stages
+    for k, it in enumerate(S.items):
+        if it["inp"] != "u16":
+            continue
+        raw = _with_zeros(it["data"], 500 + k) if zero else it["data"]
+        it["d_in"].upload(raw)
+        it["raw"] = raw
+        if it["filters"] != 9:  # X-Trans images skip both stages
+            it["d_pre"] = acc.dev_from_host(_preprocessed(raw, it["filters"], zero, green))
+    post = tuple(s for s in stages if s not in ("remove_zeroes", "green_matching"))
+    want = [_single(acc, it, post, dv.ChainParams.defaults()) for it in S.items]
+    # the stages act on these images (the comparison is not vacuous)
+    four = S.items[0]
+    if zero:
+        assert np.count_nonzero(four["raw"] == 0) > 0
+    if green:
+        assert not np.array_equal(_preprocessed(four["raw"], FOUR, zero, True),
+                                  _preprocessed(four["raw"], FOUR, zero, False))
+    b = dv.BatchDeveloper(0, 2)
+    for group in (False, True):
+        b.set_grouping(group)
+        b.run(S.images(), stages)
+        b.synchronize()
+        for it, g, w in zip(S.items, S.results(), want):
+            assert np.array_equal(g, w), (stages, group, it["kind"], it["filters"], it["inp"])
+    b.capture(S.images(), stages)
+    for it in S.items:
+        it["d_rgb"].upload(np.full((it["h"], it["w"], 3), np.nan, np.float32))
+    acc.synchronize()
+    b.replay()
+    b.synchronize()
+    for it, g, w in zip(S.items, S.results(), want):
+        assert np.array_equal(g, w), (stages, "replay", it["kind"])
+    b.close()
+    # the request loop (uint16 requests only)
+    server = dv.DevelopServer(0, lanes=2, hbm_budget_bytes=8 << 30, stages=stages)
+    outs = {}
+    for k, it in enumerate(S.items):
+        if it["inp"] != "u16":
+            continue
+        im = dv.make_image(it["w"], it["h"], filters=it["filters"], xtrans=it["xt"], cam_mul=WB, black=BLACK,
+                           white=WHITE, wb=WB)
+        src = np.ascontiguousarray(it["raw"])
+        out = np.zeros((it["h"], it["w"], 3), np.float32)
+        outs[k] = (src, out)
+        server.submit(k, im, src.ctypes.data, out.ctypes.data)
+    done = 0
+    while done < len(outs):
+        for c in server.poll(timeout_ms=2000):
+            assert c["status"] == 0, c
+            done += 1
+    server.close()
+    for k, (_, out) in outs.items():
+        assert np.array_equal(out, want[k]), (stages, "served", S.items[k]["kind"])
+
+
 @pytest.mark.parametrize("kw", [dict(defringe_edge_threshold=1e-7), dict(defringe_radius=20.0)])
 def test_batch_defringe_without_group_pass(acc, kw):
     """Defringe parameters outside the multi-image pass's range (a ramp width
@@ -365,6 +473,12 @@ def test_batch_autotune_and_config(acc):
         b.set_config(0, 8, 6)
     with pytest.raises(LibRawInvalidArgument):
         b.run(S.images(), ("axial_ca", "demosaic"))
+    # raw preprocessing: before the demosaic, zero removal first, once each
+    for bad in [("green_matching", "remove_zeroes", "demosaic"), ("remove_zeroes",),
+                ("remove_zeroes", "axial_ca"), ("demosaic", "green_matching"),
+                ("remove_zeroes", "remove_zeroes", "demosaic")]:
+        with pytest.raises(LibRawInvalidArgument):
+            b.run(S.images(), bad)
     b.close()
 
 
