@@ -151,6 +151,47 @@ LRE_DEV void for_site_pairs(const Args &A, int r0, int r1, int c0, int c1, C com
     }
 }
 
+// The sites of for_sites whose bit is set in the half-index mask `flags`, packed
+// 64 to a wavefront; f(i) runs once per such site (i = rr * TS + cc).  The
+// waves [wave0, wave0 + nwaves) walk the range 64 sites at a time, interleaved.
+// A chunk's flagged sites move to the lanes behind those still pending from
+// earlier chunks: lane l sends its site to lane (fill + rank) mod 64, flagged
+// sites ranked among the flagged, the others after them, so the sends form a
+// permutation of the 64 lanes.  Each full set of 64 runs at once and the rest
+// at the end.  Flagged sites are sparse and clustered (the Nyquist flags), so
+// this runs their long bodies on full waves instead of on every wave that
+// holds one.  Called by whole waves, with all their lanes active.
+template <class F>
+LRE_DEV void for_flagged_sites(const Args &A, const unsigned *flags, int r0, int r1, int c0, int c1, int wave0,
+                               int nwaves, F f) {
+    const int ns = (c1 - c0 + 1) / 2, nr = r1 - r0;
+    const int wave = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+    if (ns <= 0 || nr <= 0 || wave < wave0 || wave >= wave0 + nwaves) return;
+    const int n = nr * ns, stride = 64 * nwaves;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    int pend = 0, fill = 0;  // lanes [0, fill) of pend hold sites not yet run
+    RowWalk w((wave - wave0) * 64 + lane, n, ns, stride);
+    for (int base = (wave - wave0) * 64; base < n; base += stride, w.next()) {
+        const int rr = r0 + w.q, cc = c0 + (int)(A.cf[rr & 1][0] & 1) + 2 * w.r;
+        const int i = rr * TS + cc;
+        const bool on = w.more() && cc < c1 && ((flags[i >> 6] >> ((i >> 1) & 31)) & 1u);
+        const unsigned long long m = __ballot(on);
+        if (m == 0ull) continue;
+        const int cnt = __popcll(m);
+        const int dst = on ? fill + __popcll(m & below) : fill + cnt + __popcll(~m & below);
+        const int got = __builtin_amdgcn_ds_permute((dst & 63) << 2, i);
+        if (fill + cnt < 64) {
+            if (lane >= fill && lane < fill + cnt) pend = got;
+            fill += cnt;
+        } else {
+            f(lane < fill ? pend : got);
+            pend = got;
+            fill += cnt - 64;
+        }
+    }
+    if (lane < fill) f(pend);
+}
+
 // Lane exchange inside one wavefront: lane l receives lane l-1's value
 // (wave_shr:1) or lane l+1's (wave_shl:1); the lane without a source gets 0.
 LRE_DEV float from_lane_below(float v) {
@@ -590,9 +631,7 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
             if (f) atomicOr(&nqm2[k >> 5], 1u << (k & 31));
         });
         __syncthreads();
-        for_sites(A, 8, rr1 - 8, 8, cc1 - 8, [&](int rr, int cc) {
-            int i = rr * TS + cc;
-            if (!nbit(nqm2, i >> 1)) return;
+        for_flagged_sites(A, nqm2, 8, rr1 - 8, 8, cc1 - 8, 0, BLOCK / 64, [&](int i) {
             float scfa = 0, sh = 0, sv = 0, sqh = 0, sqv = 0, area = 0;
             for (int a = -6; a < 7; a += 2)
                 for (int b = -6; b < 7; b += 2) {
@@ -683,9 +722,8 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
         pmwt[k] = w;
     };
     // green refinement at Nyquist sites (:1449-1481)
-    auto nyq_refine = [&](int rr, int cc) {
-        int i = rr * TS + cc, k = i >> 1;
-        if (!nbit(nqm2, k)) return;
+    auto nyq_refine = [&](int i) {
+        const int k = i >> 1;
 #define D2H(o) d2[2 * ((i + (o)) >> 1)]
 #define D2V(o) d2[2 * ((i + (o)) >> 1) + 1]
         float gvh = EPSSQ + (GQUINC[0] * D2H(0) + GQUINC[1] * (D2H(-m1) + D2H(p1) + D2H(-p1) + D2H(m1)) +
@@ -740,7 +778,7 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
     // pmwt, so it is computed in a parallel pass afterwards
     smooth_rows(A, sm, 10, rr1, cc1, [&] {
         // the Nyquist refinement reads and writes neither the LDS plane nor the stage-8 planes
-        if (any_nyq) for_sites_on(A, 8, rr1 - 8, 8, cc1 - 8, 64, BLOCK - 64, nyq_refine);
+        if (any_nyq) for_flagged_sites(A, nqm2, 8, rr1 - 8, 8, cc1 - 8, 1, BLOCK / 64 - 1, nyq_refine);
     });
     for_sites(A, 10, rr1 - 10, 10, cc1 - 10, [&](int rr, int cc) {
         const int i = rr * TS + cc, k = i >> 1;
