@@ -44,7 +44,7 @@ struct Args {
     const float *cfa_plane;
     float *rgb;
     int W, H;
-    unsigned cf[2][2];
+    unsigned cfw;  // CFA colour of (r & 1, c & 1) in bits 16 (r & 1) + 8 (c & 1)
     int ex, ey;
     float clip_pt, clip_pt8, scale;
     const float *d_maxv;  // when set, the demosaic maximum is read from the device
@@ -64,6 +64,11 @@ __device__ const float GQUINC[4] = {0.169917f, 0.108947f, 0.069855f, 0.0287182f}
 LRE_DEV float half(float x) { return x * 0.5f; }
 LRE_DEV float twice(float x) { return x + x; }
 LRE_DEV int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// CFA colour at row r, column c of the image or of a tile (tile origins are
+// even).  The four colours sit in one word: an indexed array in the kernel
+// arguments would cost a memory load per lookup.
+LRE_DEV unsigned fct(const Args &A, int r, int c) { return (A.cfw >> (((r & 1) << 4) | ((c & 1) << 3))) & 0xffu; }
 
 // Index t = t0, t0 + step, ... < n of a row-major range with rows of `cols`
 // entries, as (row, column): one division at the start, then the quotient and
@@ -105,7 +110,7 @@ LRE_DEV void for_sites_on(const Args &A, int r0, int r1, int c0, int c1, int tid
     if (ns <= 0 || nr <= 0 || t0 < 0) return;
     for (RowWalk w(t0, nr * ns, ns, nthr); w.more(); w.next()) {
         const int rr = r0 + w.q;
-        const int cc = c0 + (int)(A.cf[rr & 1][0] & 1) + 2 * w.r;
+        const int cc = c0 + (int)(fct(A, rr, 0) & 1) + 2 * w.r;
         if (cc < c1) f(rr, cc);
     }
 }
@@ -114,26 +119,23 @@ LRE_DEV void for_sites(const Args &A, int r0, int r1, int c0, int c1, F f) {
     for_sites_on(A, r0, r1, c0, c1, 0, BLOCK, f);
 }
 
-LRE_DEV unsigned fct(const Args &A, int r, int c) { return A.cf[r & 1][c & 1]; }
 
-// for_rect / for_sites with the work split into compute(rr, cc) -> R and
-// store(rr, cc, R), one pixel or site per iteration.
-template <class C, class S>
-LRE_DEV void for_rect_split(int r0, int r1, int c0, int c1, C compute, S store) {
-    for_rect(r0, r1, c0, c1, [&](int rr, int cc) { store(rr, cc, compute(rr, cc)); });
-}
+// for_sites with the work split into compute(rr, cc) -> R and store(rr, cc, R),
+// one site per iteration.
 template <class C, class S>
 LRE_DEV void for_sites_split(const Args &A, int r0, int r1, int c0, int c1, C compute, S store) {
     for_sites(A, r0, r1, c0, c1, [&](int rr, int cc) { store(rr, cc, compute(rr, cc)); });
 }
 
 // The same sites as for_sites, two per thread: (rr, cc) and (rr + 2, cc) share
-// a column and a colour.  compute(rr, cc) only loads and computes; both results
-// are stored after both computations, so the loads of the two sites overlap
-// and the loads they have in common are issued once.  Rows go in groups of
-// four: pairs (r, r+2) and (r+1, r+3).
-template <class C, class S>
-LRE_DEV void for_site_pairs(const Args &A, int r0, int r1, int c0, int c1, C compute, S store) {
+// a column and a colour.  Each site's work is load(rr, cc) -> L, which only
+// loads, then finish(rr, cc, L) -> R; both sites' loads are issued before
+// either finish runs (and may branch), and both results are stored after both
+// computations, so the loads of the two sites overlap and the loads they have
+// in common are issued once.  Rows go in groups of four: pairs (r, r+2) and
+// (r+1, r+3).
+template <class LD, class C, class S>
+LRE_DEV void for_site_pairs_ld(const Args &A, int r0, int r1, int c0, int c1, LD load, C finish, S store) {
     const int ns = (c1 - c0 + 1) / 2, nr = r1 - r0;
     if (ns <= 0 || nr <= 0) return;
     const int n = ((nr + 3) / 4) * 2 * ns;
@@ -141,14 +143,23 @@ LRE_DEV void for_site_pairs(const Args &A, int r0, int r1, int c0, int c1, C com
         const int g = w.q, rem = w.r;
         const int p = rem >= ns ? 1 : 0, j = rem - p * ns;
         const int ra = r0 + 4 * g + p, rb = ra + 2;
-        const int cc = c0 + (int)(A.cf[ra & 1][0] & 1) + 2 * j;
+        const int cc = c0 + (int)(fct(A, ra, 0) & 1) + 2 * j;
         if (ra >= r1 || cc >= c1) continue;
         const bool vb = rb < r1;
-        const auto Ra = compute(ra, cc);
-        const auto Rb = compute(vb ? rb : ra, cc);  // a row past the range repeats the first site
+        const int rb2 = vb ? rb : ra;  // a row past the range repeats the first site
+        const auto La = load(ra, cc);
+        const auto Lb = load(rb2, cc);
+        const auto Ra = finish(ra, cc, La);
+        const auto Rb = finish(rb2, cc, Lb);
         store(ra, cc, Ra);
         if (vb) store(rb, cc, Rb);
     }
+}
+// for_site_pairs_ld with compute(rr, cc) -> R doing both parts
+template <class C, class S>
+LRE_DEV void for_site_pairs(const Args &A, int r0, int r1, int c0, int c1, C compute, S store) {
+    for_site_pairs_ld(A, r0, r1, c0, c1, [](int, int) { return 0; },
+                      [&](int rr, int cc, int) { return compute(rr, cc); }, store);
 }
 
 // The sites of for_sites whose bit is set in the half-index mask `flags`, packed
@@ -172,7 +183,7 @@ LRE_DEV void for_flagged_sites(const Args &A, const unsigned *flags, int r0, int
     int pend = 0, fill = 0;  // lanes [0, fill) of pend hold sites not yet run
     RowWalk w((wave - wave0) * 64 + lane, n, ns, stride);
     for (int base = (wave - wave0) * 64; base < n; base += stride, w.next()) {
-        const int rr = r0 + w.q, cc = c0 + (int)(A.cf[rr & 1][0] & 1) + 2 * w.r;
+        const int rr = r0 + w.q, cc = c0 + (int)(fct(A, rr, 0) & 1) + 2 * w.r;
         const int i = rr * TS + cc;
         const bool on = w.more() && cc < c1 && ((flags[i >> 6] >> ((i >> 1) & 31)) & 1u);
         const unsigned long long m = __ballot(on);
@@ -228,7 +239,7 @@ LRE_DEV void smooth_rows(const Args &A, float *sm, int b, int rr1, int cc1, F ot
         const int h = b / 2 - 2 + 2 * (int)threadIdx.x, h1 = h + 1;
         const int last = rr1 - b;  // last row read (lower neighbours of the last row)
         const int om = min(h - 1, TSH - 1), o0 = min(h, TSH - 1), o1 = min(h1, TSH - 1), o2 = min(h + 2, TSH - 1);
-        const int pe = (int)(A.cf[0][0] & 1), po = (int)(A.cf[1][0] & 1);
+        const int pe = (int)(fct(A, 0, 0) & 1), po = (int)(fct(A, 1, 0) & 1);
         const bool in0 = h >= b / 2, in1 = h1 >= b / 2;
         const bool ok0e = in0 && 2 * h + pe < cc1 - b, ok1e = in1 && 2 * h1 + pe < cc1 - b;
         const bool ok0o = in0 && 2 * h + po < cc1 - b, ok1o = in1 && 2 * h1 + po < cc1 - b;
@@ -526,30 +537,24 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
                     float hv = 3.f * (lre::sq(prev) + lre::sq(cur) + lre::sq(nxt)) - lre::sq(prev + cur + nxt);
                     float hav = 3.f * (lre::sq(a0) + lre::sq(a1) + lre::sq(a2)) - lre::sq(a0 + a1 + a2);
                     float h = hav < hv ? a1 : cur;
-                    float gh;
                     const int rr = horiz ? base / TS : pos, cc = horiz ? pos : base;
-                    const bool is_g = fct(A, rr, cc) & 1;
                     const float lf = cfa[i - s1], rt = cfa[i + s1];
-                    if (is_g) {
-                        gh = -h + c0;
-                        if (h > 0) {
-                            if (3.f * h > (gh + c0)) h = -lre::med3(gh, lf, rt) + c0;
-                            else {
-                                float w2 = 1.f - 3.f * h / (EPS + gh + c0);
-                                h = w2 * h + (1.f - w2) * (-lre::med3(gh, lf, rt) + c0);
-                            }
-                        }
-                    } else {
-                        gh = h + c0;
-                        if (h < 0) {
-                            if (3.f * h < -(gh + c0)) h = lre::med3(gh, lf, rt) - c0;
-                            else {
-                                float w2 = 1.f + 3.f * h / (EPS + gh + c0);
-                                h = w2 * h + (1.f - w2) * (lre::med3(gh, lf, rt) - c0);
-                            }
+                    // the green and the R/B forms of the reference differ only in
+                    // signs: with sg = -1 at greens and +1 elsewhere each operation
+                    // below is the reference's own (-x is exact, and a + (-b) and
+                    // a - b round alike), so one body serves both colours
+                    const float sg = (fct(A, rr, cc) & 1) ? -1.f : 1.f;
+                    const float gh = sg * h + c0;  // -h + c0 / h + c0
+                    const float mg = lre::med3(gh, lf, rt);
+                    const float hm = sg * mg - sg * c0;  // -mg + c0 / mg - c0
+                    if (sg * h < 0.f) {  // h > 0 / h < 0
+                        if (sg * (3.f * h) < -(gh + c0)) h = hm;  // 3h > gh + c0 / 3h < -(gh + c0)
+                        else {
+                            const float w2 = 1.f + sg * (3.f * h / (EPS + gh + c0));  // 1 - q / 1 + q
+                            h = w2 * h + (1.f - w2) * hm;
                         }
                     }
-                    if (gh > A.clip_pt) h = (is_g ? -1.f : 1.f) * (lre::med3(gh, lf, rt) - c0);
+                    if (gh > A.clip_pt) h = sg * (mg - c0);
                     cd[i] = h;
                     prev = h;
                     a0 = a1; a1 = a2; cur = nxt;
@@ -791,33 +796,46 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
     // dirwts1 of green cell j as the refinement reads it: where the cell lies in
     // the plane's first half and its pair's site in stage 8's range, pmwt has
     // replaced it (arena); elsewhere it holds the stage-1 value
+    // (the arena cell is loaded whether or not it is taken, so that the load
+    // does not wait behind a branch)
     auto wh_at = [&](int j) {
+        const float a = wh[j];
+        bool arena = false;
         if (j < N2 / 2) {
             const int e = 2 * j, er = e / TS, ec = e - er * TS;
             const int sc = (fct(A, er, ec) & 1) ? ec + 1 : ec;  // the site of the pair (same row)
-            if (er >= 8 && er < rr1 - 8 && sc >= 8 && sc < cc1 - 8) return wh[j];
+            arena = er >= 8 && er < rr1 - 8 && sc >= 8 && sc < cc1 - 8;
         }
-        return f_wh(j);
+        return arena ? a : f_wh(j);
     };
     struct SR {
         bool on;
         float g, d0;
     };
-    for_sites_split(A, 12, rr1 - 12, 12, cc1 - 12, [&](int rr, int cc) {
+    // the arena values of a site, loaded ahead of the gate (sm holds the
+    // smoothed pmwt)
+    struct RL {
+        float pk, hk, rb, ru, rd, rl, rr, wl, wr;
+    };
+    for_site_pairs_ld(A, 12, rr1 - 12, 12, cc1 - 12, [&](int rr, int cc) {
         const int i = rr * TS + cc, k = i >> 1;
+        return RL{sm[k], hvwt[k], rbint[k], rbint[k - v1], rbint[k + v1], rbint[k - 1], rbint[k + 1],
+                  wh_at(i - 1), wh_at(i + 1)};
+    }, [&](int rr, int cc, const RL &L) {
+        const int i = rr * TS + cc;
         SR R{false, 0.f, 0.f};
-        if (fabsf(0.5f - sm[k]) >= fabsf(0.5f - hvwt[k])) return R;  // sm holds the smoothed pmwt
-        float rb = rbint[k];
-        float ru = twice(cfa[i - v1]) / (EPS + rb + rbint[k - v1]);
-        float rd = twice(cfa[i + v1]) / (EPS + rb + rbint[k + v1]);
-        float rl = twice(cfa[i - 1]) / (EPS + rb + rbint[k - 1]);
-        float rr_ = twice(cfa[i + 1]) / (EPS + rb + rbint[k + 1]);
-        float gu = fabsf(1.f - ru) < ARTHRESH ? rb * ru : cfa[i - v1] + half(rb - rbint[k - v1]);
-        float gd = fabsf(1.f - rd) < ARTHRESH ? rb * rd : cfa[i + v1] + half(rb - rbint[k + v1]);
-        float gl = fabsf(1.f - rl) < ARTHRESH ? rb * rl : cfa[i - 1] + half(rb - rbint[k - 1]);
-        float gr = fabsf(1.f - rr_) < ARTHRESH ? rb * rr_ : cfa[i + 1] + half(rb - rbint[k + 1]);
+        if (fabsf(0.5f - L.pk) >= fabsf(0.5f - L.hk)) return R;
+        const float rb = L.rb;
+        float ru = twice(cfa[i - v1]) / (EPS + rb + L.ru);
+        float rd = twice(cfa[i + v1]) / (EPS + rb + L.rd);
+        float rl = twice(cfa[i - 1]) / (EPS + rb + L.rl);
+        float rr_ = twice(cfa[i + 1]) / (EPS + rb + L.rr);
+        float gu = fabsf(1.f - ru) < ARTHRESH ? rb * ru : cfa[i - v1] + half(rb - L.ru);
+        float gd = fabsf(1.f - rd) < ARTHRESH ? rb * rd : cfa[i + v1] + half(rb - L.rd);
+        float gl = fabsf(1.f - rl) < ARTHRESH ? rb * rl : cfa[i - 1] + half(rb - L.rl);
+        float gr = fabsf(1.f - rr_) < ARTHRESH ? rb * rr_ : cfa[i + 1] + half(rb - L.rr);
         float gv = (f_wv(i - v1) * gd + f_wv(i + v1) * gu) / (f_wv(i + v1) + f_wv(i - v1));
-        const float wl = wh_at(i - 1), wr = wh_at(i + 1);
+        const float wl = L.wl, wr = L.wr;
         float gh = (wl * gr + wr * gl) / (wl + wr);
         const float up = cfa[i - v1], dn = cfa[i + v1], lf = cfa[i - 1], rt = cfa[i + 1];
         if (gv < rb) {
@@ -836,7 +854,7 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
         }
         if (gh > A.clip_pt) gh = lre::med3(gh, lf, rt);
         if (gv > A.clip_pt) gv = lre::med3(gv, up, dn);
-        float hk = hvwt[k];
+        const float hk = L.hk;
         float g = gh * (1.f - hk) + gv * hk;
         R.on = true;
         R.g = g;
@@ -895,41 +913,70 @@ __device__ void amaze_tile(const Args &A, float *ar, int top, int left) {
     });
     __syncthreads();
 
-    // stage 10 (:1597-1625)
+    // stage 10 (:1597-1625), two horizontally adjacent pixels per thread.  In a
+    // Bayer row one of the two is green and the other is not, so each thread
+    // runs each of the two interpolations once, where a wavefront of single
+    // pixels would run both for every pixel; other pairs (the last column of
+    // an odd range) take each pixel on its own.
     struct SO {
         float r, g, b;
     };
-    for_rect_split(16, rr1 - 16, 16, cc1 - 16, [&](int rr, int cc) {
-        const int row = rr + top, col = cc + left;
-        const int t = rr * TS + cc;
-        float r, g, b;
-        // Dgrb0 / Dgrb1 of the site cell k in tile row q: on B rows X is Dgrb1
-        // and the pass's output Dgrb0, on R rows the other way round
-        auto d0 = [&](int q, int k) { return ((q - r0b) & 1) == 0 ? dg0[k] : sm[k]; };
-        auto d1 = [&](int q, int k) { return ((q - r0b) & 1) == 0 ? sm[k] : dg1[k]; };
-        if (fct(A, row, col) == 1) {
-            g = cfa[t];
-            float hu = hvwt[(t - v1) >> 1], hr = hvwt[(t + 1) >> 1], hl = hvwt[(t - 1) >> 1], hd = hvwt[(t + v1) >> 1];
-            float inv = 1.0f / (hu + 2.f - hr - hl + hd);
-            const int ku = (t - v1) >> 1, kr = (t + 1) >> 1, kl = (t - 1) >> 1, kd = (t + v1) >> 1;
-            float rd = (hu * d0(rr - 1, ku) + (1.f - hr) * d0(rr, kr) + (1.f - hl) * d0(rr, kl) + hd * d0(rr + 1, kd)) * inv;
-            float bd = (hu * d1(rr - 1, ku) + (1.f - hr) * d1(rr, kr) + (1.f - hl) * d1(rr, kl) + hd * d1(rr + 1, kd)) * inv;
-            r = g - rd;
-            b = g - bd;
-        } else {
-            g = green[t];
-            r = g - d0(rr, t >> 1);
-            b = g - d1(rr, t >> 1);
-        }
-        return SO{r, g, b};
-    }, [&](int rr, int cc, const SO &R) {
+    // Dgrb0 / Dgrb1 of a site cell: on B rows (rr - r0b even) X is Dgrb1 and the
+    // pass's output Dgrb0, on R rows the other way round.  X is in the LDS half
+    // plane, the output in the arena plane g_row(rr); every value is loaded
+    // unconditionally and picked by the row parity, so that the loads of a pixel
+    // are all in flight together.
+    auto even_row = [&](int rr) { return ((rr - r0b) & 1) == 0; };
+    auto g_row = [&](int rr) { return even_row(rr) ? dg0 : dg1; };
+    auto at_green = [&](int rr, int t) {
+        const float g = cfa[t];
+        const int ku = (t - v1) >> 1, kr = (t + 1) >> 1, kl = (t - 1) >> 1, kd = (t + v1) >> 1;
+        const float hu = hvwt[ku], hr = hvwt[kr], hl = hvwt[kl], hd = hvwt[kd];
+        const float *gv = g_row(rr + 1), *gh = g_row(rr);  // rows rr - 1 and rr + 1 share a parity
+        const float au = gv[ku], ad = gv[kd], ar_ = gh[kr], al = gh[kl];
+        const float xu = sm[ku], xd = sm[kd], xr = sm[kr], xl = sm[kl];
+        const bool e = even_row(rr);
+        // d0 / d1 of the four neighbours: on an even row the vertical ones' d0
+        // is X and the horizontal ones' d0 the arena value; the reverse for d1
+        const float u0 = e ? xu : au, d0_ = e ? xd : ad, r0 = e ? ar_ : xr, l0 = e ? al : xl;
+        const float u1 = e ? au : xu, d1_ = e ? ad : xd, r1 = e ? xr : ar_, l1 = e ? xl : al;
+        float inv = 1.0f / (hu + 2.f - hr - hl + hd);
+        float rd = (hu * u0 + (1.f - hr) * r0 + (1.f - hl) * l0 + hd * d0_) * inv;
+        float bd = (hu * u1 + (1.f - hr) * r1 + (1.f - hl) * l1 + hd * d1_) * inv;
+        return SO{g - rd, g, g - bd};
+    };
+    auto at_site = [&](int rr, int t) {
+        const float g = green[t];
+        const int k = t >> 1;
+        const float a = g_row(rr)[k], x = sm[k];
+        const bool e = even_row(rr);
+        return SO{g - (e ? a : x), g, g - (e ? x : a)};
+    };
+    auto store = [&](int rr, int cc, const SO &R) {
         const int row = rr + top, col = cc + left;
         if (row < 0 || row >= H || col < 0 || col >= W) return;
         float *o = A.rgb + ((size_t)row * W + col) * 3;
         o[0] = (0.f < R.r) ? R.r : 0.f;
         o[1] = (0.f < R.g) ? R.g : 0.f;
         o[2] = (0.f < R.b) ? R.b : 0.f;
-    });
+    };
+    {
+        const int c1 = cc1 - 16, nr = rr1 - 32, np = (c1 - 16 + 1) / 2;
+        if (nr > 0 && np > 0)
+            for (RowWalk w((int)threadIdx.x, nr * np, np, BLOCK); w.more(); w.next()) {
+                const int rr = 16 + w.q, cc = 16 + 2 * w.r, t = rr * TS + cc;
+                const bool two = cc + 1 < c1;
+                const bool ga = fct(A, rr + top, cc + left) == 1, gb = fct(A, rr + top, cc + 1 + left) == 1;
+                if (two && ga != gb) {
+                    const SO G = at_green(rr, ga ? t : t + 1), S = at_site(rr, ga ? t + 1 : t);
+                    store(rr, cc, ga ? G : S);
+                    store(rr, cc + 1, ga ? S : G);
+                } else {
+                    store(rr, cc, ga ? at_green(rr, t) : at_site(rr, t));
+                    if (two) store(rr, cc + 1, gb ? at_green(rr, t + 1) : at_site(rr, t + 1));
+                }
+            }
+    }
     __syncthreads();
 }
 
@@ -1039,12 +1086,17 @@ static void amaze_args(Args &A, const float *d_cfa, float *d_rgb, int w, int h, 
     A.clip_pt = 1.0f / gain;
     A.clip_pt8 = 0.8f / gain;
     A.scale = 1.0f / maximum_value;
+    unsigned cf[2][2];
+    A.cfw = 0;
     for (int r = 0; r < 2; r++)
-        for (int c = 0; c < 2; c++) A.cf[r][c] = lre::bayer3(filters, r, c);
-    if (A.cf[0][0] == 1) {
-        if (A.cf[0][1] == 0) { A.ey = 0; A.ex = 1; } else { A.ey = 1; A.ex = 0; }
+        for (int c = 0; c < 2; c++) {
+            cf[r][c] = lre::bayer3(filters, r, c);
+            A.cfw |= cf[r][c] << (16 * r + 8 * c);
+        }
+    if (cf[0][0] == 1) {
+        if (cf[0][1] == 0) { A.ey = 0; A.ex = 1; } else { A.ey = 1; A.ex = 0; }
     } else {
-        if (A.cf[0][0] == 0) { A.ey = 0; A.ex = 0; } else { A.ey = 1; A.ex = 1; }
+        if (cf[0][0] == 0) { A.ey = 0; A.ex = 0; } else { A.ey = 1; A.ex = 1; }
     }
     A.d_maxv = d_maxv;
     A.ntc = tiles_for(w);
