@@ -25,96 +25,396 @@ __global__ void aces_kernel(const float4 *__restrict__ in, float4 *__restrict__ 
 }
 
 // ---- detail-preserving tone map, core/libraw_wrapper.cpp:1398-1471 ---------
-// Box means of radius 12 with the window clipped at the frame and divided by
-// the true tap count, rows first then columns (core/libraw_wrapper.cpp:45-116),
-// evaluated as the reference does: a running sum that drops the leaving tap
-// and adds the entering one.  Each thread restarts the sum for its own run of
-// outputs, so the rounding differs from the reference's whole-row runs by
-// ~1e-7 relative.
+// The guide g = max(0, R, G, B), box means of radius 12 of g and g*g, the
+// gain coefficients a = var / (var + eps) and b = mean (1 - a), their box
+// means A and B, and per pixel base = max(1e-6, A g + B), whose ACES gain is
+// blended in by a smoothstep over [0.55, 0.90] (the pixel is left as it is
+// where that weight is 0).  Box means clip the window at the frame and divide
+// by the true tap count (core/libraw_wrapper.cpp:45-116).
 //
-// Two kernels.  dtm_guide writes the guide plane max(0, R, G, B).  dtm_tile
-// takes a 64 x 64 output tile and stages the guide over the tile and a 24-px
-// ring (the reach of two chained radius-12 boxes) in LDS; the means of the
-// guide and its square, a and b, their means and the per-pixel gain all stay
-// on chip.  Around the tile only the guide plane is read, so the image itself
-// can be rewritten in place.  A tile whose staged guide never exceeds 0.5499
-// is left as it is: there base <= max(guide) < 0.55 (b is a mean times 1 - a,
-// so a g + b is a convex combination of values no larger than that maximum),
-// the smoothstep weight is 0 and the reference leaves every pixel unchanged.
+// One workgroup of two wavefronts develops a strip of 64 output columns over a
+// segment of rows, walking down the rows with every intermediate on chip, 8
+// rows (a batch) at a time:
+//   wave 0  V1: vertical running sums of g and g*g over the 25 rows of the
+//               window, per column of the 112 guide columns (the strip and a
+//               24-column ring on each side); each lane keeps its two columns'
+//               last 32 guide values in registers for the value that leaves;
+//           H1: horizontal running sums of those over 25 columns, giving the
+//               means and a, b on the 88 columns the second box reads (lane:
+//               a run of 11 of one row of the batch);
+//   wave 1  H2: horizontal running sums of a and b (lane: a run of 8 of one
+//               row);
+//           V2: vertical running sums of those, per output column, with its
+//               last 32 rows in registers, then the output row 12 rows behind.
+// Wave 1 works on the batch before the one wave 0 works on; a and b pass
+// between them through two LDS buffers, one barrier per batch.  The two waves'
+// rings and sums share registers.  The first box sums columns then rows and
+// the second rows then columns; the reference sums rows then columns in both,
+// and the two orders differ by the rounding of the running sums only (the
+// reference's own runs span whole rows and columns).  A segment starts 36 rows
+// above its first output row with empty sums and rings: the first three
+// batches only load the guide.
+//
+// The guide comes from a guide plane written beforehand (safe in place: the
+// kernel reads the image only at its own output pixels, each lane reading its
+// pixel before writing it), or, when the output is another buffer, straight
+// from the input pixels.
 constexpr int DR = 12;
-constexpr float DTM_DARK = 0.5499f;
+constexpr int DS_W = 64;               // output columns per strip, one per lane
+constexpr int DS_GX = DS_W + 4 * DR;   // guide columns of a strip
+constexpr int DS_MX = DS_W + 2 * DR;   // a, b columns of a strip
+constexpr int DS_RB = 8;               // rows per batch
+constexpr int DS_RING = 32;            // ring of past rows (>= 2 DR + 2, a multiple of DS_RB)
+// LDS planes (one float per value): pitches that spread each pass's 64
+// lanes over the banks, lane 8 r + k (row r, run k) in H1 and 8 k + r in H2.
+// V1 means: H1 reads runs of 11 (no conflict); a, b: written in runs of 11,
+// read in runs of 8 (at most two lanes per bank); H2 means: written in runs of
+// 8 (no conflict).  V1 and V2 touch consecutive columns.
+constexpr int DS_VP = 120, DS_AP = 90, DS_HP = 65;
+static_assert(DS_VP >= DS_GX && DS_AP >= DS_MX && DS_HP >= DS_W, "LDS pitches hold a row");
+static_assert(DS_RING >= 2 * DR + 2 && DS_RING == 4 * DS_RB, "a ring cycle is 4 batches");
+static_assert(DS_MX == 8 * 11 && DS_W == 8 * 8 && DS_RB * 8 == 64, "a batch is 8 rows x 8 runs, one per lane");
+static_assert(DS_GX - 64 <= 64, "two guide columns per lane");
 
 // one image of a detail tone map launch
 struct DtmImg {
-    const float *guide;
+    const float *guide;  // guide plane; null: computed from `in` (only when in != out)
     const float *in;
     float *out;
     int W, H;
+    int seg;             // output rows per segment (32 k - 48)
 };
 
-// Geometry of an output tile of side T: the guide window (side T + 4 DR), the
-// mean / a / b window (side T + 2 DR), odd LDS pitches, and LDS in floats: a
-// and b first (the guide window shares their space; it is dead when they are
-// written), then the row means of g and g*g, whose space later holds the row
-// means of a and b.  Each thread sums runs of consecutive outputs: the row
-// passes a line of M = T + 2 DR outputs in 8 runs and a line of T outputs in 8
-// runs, the column passes runs of RUNC and RUND outputs.
-template <int T>
-struct DtmGeo {
-    static constexpr int G = T + 4 * DR, M = T + 2 * DR;
-    static constexpr int GP = G + 1, MP = M + 1, TP = T + 1;
-    static constexpr int RUN1 = M / 8, RUN2 = T / 8;
-    // the column passes: pass 2 sums NC runs of RUNC rows per column, pass 4 ND
-    // runs of RUND (shorter runs for more threads measured no faster)
-    static constexpr int RUNC = RUN1, NC = 8, RUND = RUN2, ND = 8;
-    static_assert(NC * RUNC == M && ND * RUND == T, "column runs tile the windows");
-    static constexpr int L_AB = 2 * M * MP > G * GP ? 2 * M * MP : G * GP, L_H = 2 * G * MP;
-    static constexpr size_t LDS = sizeof(float) * (L_AB + L_H);
-    static_assert(RUN1 * 8 == M && RUN2 * 8 == T, "runs tile the windows");
-    static_assert(G * GP <= L_AB && 2 * M * TP <= L_H, "LDS regions");
+LRE_DEV float dtm_gd(float r, float g, float b) { return lre::max_std(0.0f, lre::max_std(r, lre::max_std(g, b))); }
+
+// 1 / (taps of the radius-DR window at p in [0, len)); positions outside have
+// no use for the value and get 1.  A window inside the frame has 2 DR + 1 taps.
+LRE_DEV float dtm_inv_count(int p, int len) {
+    if (p >= DR && p + DR < len) return 1.0f / (2 * DR + 1);
+    const int c = min(len - 1, p + DR) - max(0, p - DR) + 1;
+    return __builtin_amdgcn_rcpf((float)(c > 0 ? c : 1));
+}
+
+// n / d from the hardware reciprocal (within 2 ulp; d > 0 here)
+LRE_DEV float dtm_div(float n, float d) { return n * __builtin_amdgcn_rcpf(d); }
+
+// LDS writes of this wavefront's lanes are seen by its later LDS reads
+LRE_DEV void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// the LDS of a strip workgroup
+struct DtmLds {
+    float vg[DS_RB * DS_VP], vq[DS_RB * DS_VP];          // wave 0: V1 means of g, g*g
+    float a[2][DS_RB * DS_AP], b[2][DS_RB * DS_AP];      // a, b from wave 0 to wave 1, by batch parity
+    float ha[DS_RB * DS_HP], hb[DS_RB * DS_HP];          // wave 1: H2 row means
 };
 
-// One thread's run of N box means along a line: tap k of the line is val(k)
-// (two quantities at once), output j covers taps j .. j + 2 DR and sits at
-// image coordinate p0 + j of an extent len (taps outside the image are 0).
-// The mean is the sum times 1 / (tap count): for a run whose windows all lie
-// inside the image the count is 2 DR + 1 for every output (a constant), else
-// it is clipped per output and inverted by v_rcp_f32.  The reference divides;
-// the two differ by an ulp or so, well inside the running-sum rounding.
-template <int N, class V, class F>
-LRE_DEV void box_run2(V val, int p0, int len, F emit) {
-    float s0 = 0.f, s1 = 0.f;
+template <bool FUSED, bool EPI>
+struct DtmStrip {
+    // raw values in flight: wave 0 the next batch's guide samples (a pixel per
+    // sample when fused), wave 1 the next batch's output pixels
+    static constexpr int LD = FUSED ? 3 * 2 * DS_RB : 3 * DS_RB;
+    DtmImg I;
+    int x0, y0, yend, lane;
+    int gx0, gx1, ox;
+    bool in0, in1, oin;
+    float ring0[DS_RING], ring1[DS_RING];  // wave 0: g at gx0, gx1; wave 1: row means of a, b
+    float acc[4];                          // wave 0: sums of g, g*g at gx0, gx1; wave 1: of a, b
+    float ld[LD];
+    float gc[DS_RB][2];                    // wave 0: guide values of the batch's V1 rows
+
+    // wave 0: raw guide samples of guide rows yb + DR .. yb + DR + 7 (0 outside
+    // the frame)
+    LRE_DEV void load_guide(int yb) {
 #pragma unroll
-    for (int k = 0; k <= 2 * DR; k++) {
-        const float2 v = val(k);
-        s0 += v.x;
-        s1 += v.y;
+        for (int r = 0; r < DS_RB; r++)
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                const int y = yb + r + DR, x = c ? gx1 : gx0;
+                const bool ok = (c ? in1 : in0) && y >= 0 && y < I.H;
+                const size_t i = ok ? (size_t)y * I.W + x : 0;
+                if (FUSED) {
+                    float *v = ld + 3 * (2 * r + c);
+                    v[0] = v[1] = v[2] = 0.f;
+                    if (ok) {
+                        v[0] = I.in[3 * i];
+                        v[1] = I.in[3 * i + 1];
+                        v[2] = I.in[3 * i + 2];
+                    }
+                } else {
+                    ld[2 * r + c] = ok ? I.guide[i] : 0.f;
+                }
+            }
     }
-    const auto slide = [&](int j) {
-        const float2 o = val(j), e = val(j + 2 * DR + 1);
-        s0 -= o.x;
-        s1 -= o.y;
-        s0 += e.x;
-        s1 += e.y;
-    };
-    if (p0 >= DR && p0 + N - 1 + DR <= len - 1) {
-        constexpr float ic = 1.0f / (2 * DR + 1);
+    LRE_DEV void take_guide() {
 #pragma unroll
-        for (int j = 0; j < N; j++) {
-            emit(j, s0 * ic, s1 * ic);
-            if (j + 1 < N) slide(j);
+        for (int r = 0; r < DS_RB; r++)
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                if (FUSED) {
+                    const float *v = ld + 3 * (2 * r + c);
+                    gc[r][c] = dtm_gd(v[0], v[1], v[2]);
+                } else {
+                    gc[r][c] = ld[2 * r + c];
+                }
+            }
+    }
+    // wave 1: the output pixels of rows yb - DR .. yb - DR + 7 (rows outside
+    // the segment are not used: they read a clamped row)
+    LRE_DEV void load_pixels(int yb) {
+        const int cx = min(ox, I.W - 1);
+#pragma unroll
+        for (int r = 0; r < DS_RB; r++) {
+            const int y = min(max(yb + r - DR, 0), I.H - 1);
+            const float *p = I.in + 3 * ((size_t)y * I.W + cx);
+            ld[3 * r] = p[0];
+            ld[3 * r + 1] = p[1];
+            ld[3 * r + 2] = p[2];
         }
+    }
+
+    // wave 0, batch in ring quarter Q: a, b rows yb .. yb + 7 into ab
+    template <int Q>
+    LRE_DEV void first_box(int yb, bool warm, float *s_vg, float *s_vq, float *s_a, float *s_b) {
+        // V1: add guide row y + DR, drop row y - DR - 1 (ring steps u + 12, u - 13)
+#pragma unroll
+        for (int r = 0; r < DS_RB; r++) {
+            const int u = Q * DS_RB + r, y = yb + r;
+            const float g0 = gc[r][0], g1 = gc[r][1];
+            const float o0 = ring0[(u + DS_RING - DR - 1) % DS_RING], o1 = ring1[(u + DS_RING - DR - 1) % DS_RING];
+            ring0[(u + DR) % DS_RING] = g0;
+            ring1[(u + DR) % DS_RING] = g1;
+            acc[0] = (acc[0] - o0) + g0;
+            acc[1] = (acc[1] - o0 * o0) + g0 * g0;
+            acc[2] = (acc[2] - o1) + g1;
+            acc[3] = (acc[3] - o1 * o1) + g1 * g1;
+            if (!warm) {
+                const float ic = dtm_inv_count(y, I.H);
+                s_vg[r * DS_VP + lane] = acc[0] * ic;
+                s_vq[r * DS_VP + lane] = acc[1] * ic;
+                if (lane < DS_GX - 64) {
+                    s_vg[r * DS_VP + 64 + lane] = acc[2] * ic;
+                    s_vq[r * DS_VP + 64 + lane] = acc[3] * ic;
+                }
+            }
+        }
+        // the next batch's guide rows load behind this batch's row sums
+        load_guide(yb + DS_RB);
+        if (!warm) {
+            wave_lds_sync();
+            // H1: lane (row r, run k) -> a, b at a, b columns 11 k .. 11 k + 10
+            const int r = lane >> 3, j0 = 11 * (lane & 7), y = yb + r;
+            const float *vg = s_vg + r * DS_VP + j0, *vq = s_vq + r * DS_VP + j0;
+            float s0 = 0.f, s1 = 0.f;
+#pragma unroll 5
+            for (int t = 0; t <= 2 * DR; t++) {
+                s0 += vg[t];
+                s1 += vq[t];
+            }
+            const bool yin = y >= 0 && y < I.H;
+            const int xa = x0 - DR + j0;  // the run's first column
+            const bool inner = xa >= DR && xa + 10 + DR < I.W;
+#pragma unroll 2
+            for (int j = 0; j < 11; j++) {
+                const int x = xa + j;
+                const float ic = inner ? 1.0f / (2 * DR + 1) : dtm_inv_count(x, I.W);
+                const float mean = s0 * ic, msq = s1 * ic;
+                float a = 0.f, b = 0.f;  // outside the image: no taps
+                if (yin && x >= 0 && x < I.W) {
+                    const float var = lre::max_std(0.0f, msq - mean * mean);
+                    a = dtm_div(var, var + 0.0025f);
+                    b = mean * (1.0f - a);
+                }
+                s_a[r * DS_AP + j0 + j] = a;
+                s_b[r * DS_AP + j0 + j] = b;
+                if (j < 10) {
+                    s0 = (s0 - vg[j]) + vg[j + 2 * DR + 1];
+                    s1 = (s1 - vq[j]) + vq[j + 2 * DR + 1];
+                }
+            }
+        }
+        take_guide();
+    }
+
+    // wave 1, batch in ring quarter Q: a, b rows yb .. yb + 7 from ab, output
+    // rows yb - DR .. yb - DR + 7
+    template <int Q>
+    LRE_DEV void second_box(int yb, const float *s_a, const float *s_b, float *s_ha, float *s_hb,
+                               const lre::Epilogue &E) {
+        {
+            // H2: lane (run k, row r) -> row means of a, b at output columns 8 k .. 8 k + 7
+            const int r = lane & 7, j0 = 8 * (lane >> 3);
+            const float *va = s_a + r * DS_AP + j0, *vb = s_b + r * DS_AP + j0;
+            float s0 = 0.f, s1 = 0.f;
+#pragma unroll 5
+            for (int t = 0; t <= 2 * DR; t++) {
+                s0 += va[t];
+                s1 += vb[t];
+            }
+            const bool inner = x0 + j0 >= DR && x0 + j0 + 7 + DR < I.W;
+#pragma unroll 2
+            for (int j = 0; j < 8; j++) {
+                const float ic = inner ? 1.0f / (2 * DR + 1) : dtm_inv_count(x0 + j0 + j, I.W);
+                s_ha[r * DS_HP + j0 + j] = s0 * ic;
+                s_hb[r * DS_HP + j0 + j] = s1 * ic;
+                if (j < 7) {
+                    s0 = (s0 - va[j]) + va[j + 2 * DR + 1];
+                    s1 = (s1 - vb[j]) + vb[j + 2 * DR + 1];
+                }
+            }
+        }
+        wave_lds_sync();
+        // V2 and the output rows: add row u, drop row u - 25
+        const bool inplace = I.in == I.out;
+#pragma unroll
+        for (int r = 0; r < DS_RB; r++) {
+            const int u = Q * DS_RB + r, yo = yb + r - DR;
+            const float2 h = make_float2(s_ha[r * DS_HP + lane], s_hb[r * DS_HP + lane]);
+            const float oa = ring0[(u + DS_RING - 2 * DR - 1) % DS_RING];
+            const float ob = ring1[(u + DS_RING - 2 * DR - 1) % DS_RING];
+            ring0[u % DS_RING] = h.x;
+            ring1[u % DS_RING] = h.y;
+            acc[0] = (acc[0] - oa) + h.x;
+            acc[1] = (acc[1] - ob) + h.y;
+            if (!oin || yo < y0 || yo >= yend) continue;
+            const float ic = dtm_inv_count(yo, I.H);
+            const float A = acc[0] * ic, B = acc[1] * ic;
+            float R = ld[3 * r], G = ld[3 * r + 1], Bl = ld[3 * r + 2];
+            const float base = lre::max_std(1e-6f, A * dtm_gd(R, G, Bl) + B);
+            // aces(base) / base with the common factor base cancelled
+            float gain = dtm_div(2.51f * base + 0.03f, base * (2.43f * base + 0.59f) + 0.14f);
+            gain = lre::clamp_std(gain, 0.0f, 1.0f);
+            float w = lre::clamp_std((base - 0.55f) * (1.0f / (0.90f - 0.55f)), 0.0f, 1.0f);
+            w = w * w * (3.0f - 2.0f * w);
+            // where the weight is 0 the pixel stays as it is (times exactly 1)
+            const float f = w > 0.0f ? 1.0f + (gain - 1.0f) * w : 1.0f;
+            R *= f;
+            G *= f;
+            Bl *= f;
+            if (EPI) lre::epilogue_rgb(R, G, Bl, E);
+            if (EPI || !inplace || w > 0.0f) {  // unchanged in place: nothing to store
+                float *o = I.out + 3 * ((size_t)yo * I.W + ox);
+                o[0] = R;
+                o[1] = G;
+                o[2] = Bl;
+            }
+            // one row's output at a time: interleaving the rows' arithmetic
+            // would hold more registers than the occupancy leaves
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        load_pixels(yb + DS_RB);
+    }
+
+    // step Q of a 32-row iteration: wave 0 on batch bn, wave 1 on batch bn - 1
+    template <int Q>
+    LRE_DEV void step(int bn, int nb, int ystart, DtmLds &L, const lre::Epilogue &E) {
+        if (threadIdx.x < 64) {
+            if (bn < nb) first_box<Q>(ystart + DS_RB * bn, bn < 3, L.vg, L.vq, L.a[Q & 1], L.b[Q & 1]);
+        } else if (bn >= 4 && bn <= nb) {
+            second_box<(Q + 3) % 4>(ystart + DS_RB * (bn - 1), L.a[(Q + 1) & 1], L.b[(Q + 1) & 1], L.ha, L.hb, E);
+        }
+        __syncthreads();
+    }
+};
+
+// The strip segment of workgroup p of n.  Workgroups are dealt to the 8 XCDs
+// round-robin; each XCD's workgroups take one contiguous range of the
+// row-major segment list, so the strips beside each other in a segment row,
+// which read 48 of each other's guide columns, run on one XCD and share its L2.
+LRE_DEV int dtm_xcd_tile(int p, int n) {
+    const int x = p & 7, slot = p >> 3;
+    return x * (n >> 3) + min(x, n & 7) + slot;
+}
+LRE_DEV lre::TileOf dtm_tile_of(const lre::One<DtmImg> &s, int t) {
+    const int strips = (s.a.W + DS_W - 1) / DS_W;
+    return {0, t % strips, t / strips};
+}
+LRE_DEV lre::TileOf dtm_tile_of(const lre::Many<DtmImg> &s, int t) { return s.tile_at(t); }
+
+// Src: lre::One<DtmImg> or lre::Many<DtmImg> (the strip segments of several
+// images); a 1D grid of one workgroup of 128 threads per strip segment.
+template <bool FUSED, bool EPI, class Src>
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(EPI ? 2 : 3))) void dtm_strip(Src src,
+                                                                                              lre::Epilogue E) {
+    __shared__ DtmLds L;
+    const lre::TileOf tl = dtm_tile_of(src, dtm_xcd_tile((int)blockIdx.x, (int)gridDim.x));
+    DtmStrip<FUSED, EPI> S;
+    S.I = src.args(tl);
+    const int SH = S.I.seg;
+    S.lane = (int)threadIdx.x & 63;
+    S.x0 = tl.tx * DS_W;
+    S.y0 = tl.ty * SH;
+    S.yend = min(S.y0 + SH, S.I.H);
+    S.gx0 = S.x0 - 2 * DR + S.lane;
+    S.gx1 = S.gx0 + 64;
+    S.ox = S.x0 + S.lane;
+    S.in0 = S.gx0 >= 0 && S.gx0 < S.I.W;
+    S.in1 = S.lane < DS_GX - 64 && S.gx1 >= 0 && S.gx1 < S.I.W;
+    S.oin = S.ox < S.I.W;
+#pragma unroll
+    for (int k = 0; k < DS_RING; k++) S.ring0[k] = S.ring1[k] = 0.f;
+#pragma unroll
+    for (int k = 0; k < 4; k++) S.acc[k] = 0.f;
+    // a, b rows from y0 - 3 DR: 24 lead-in rows fill the first box's window;
+    // wave 1 starts at batch 3, the first with a, b rows
+    const int ystart = S.y0 - 3 * DR, iters = (SH + 4 * DR) / DS_RING, nb = 4 * iters;
+    if (threadIdx.x < 64) {
+        S.load_guide(ystart);
+        S.take_guide();
     } else {
-#pragma unroll
-        for (int j = 0; j < N; j++) {
-            const int p = p0 + j;
-            const int cnt = min(len - 1, p + DR) - max(0, p - DR) + 1;
-            // cnt < 1 only outside the image, where the mean is not used
-            const float ic = __builtin_amdgcn_rcpf((float)(cnt > 0 ? cnt : 1));
-            emit(j, s0 * ic, s1 * ic);
-            if (j + 1 < N) slide(j);
+        S.load_pixels(ystart + 3 * DS_RB);
+    }
+    for (int it = 0; it <= iters; it++) {
+        const int b = 4 * it;
+        S.template step<0>(b, nb, ystart, L, E);
+        if (it == iters) break;  // wave 1's last batch ran in step 0
+        S.template step<1>(b + 1, nb, ystart, L, E);
+        S.template step<2>(b + 2, nb, ystart, L, E);
+        S.template step<3>(b + 3, nb, ystart, L, E);
+    }
+}
+
+// Output rows per segment: 32 k - 48 for a whole number k of 32-row
+// iterations, each segment reading 48 rows of lead-in.  Chosen from the frame
+// size only (the batch path must match the per-image path bit for bit): the
+// height whose rounds of resident workgroups times segment length is least,
+// for an MI355X holding 1536 of these workgroups at once (256 CUs x 6: three
+// waves per SIMD).  On 24 MP that is one round of 272-row segments; a frame
+// too small to fill a round takes the shortest segments.
+constexpr int DTM_RESIDENT = 1536;
+int dtm_seg_rows(int w, int h) {
+    const long long strips = (w + DS_W - 1) / DS_W;
+    int best = 80;
+    long long best_cost = -1;
+    for (int k = 4; k <= 16; k++) {
+        const int sh = 32 * k - 48;
+        const long long wg = strips * ((h + sh - 1) / sh);
+        const long long cost = ((wg + DTM_RESIDENT - 1) / DTM_RESIDENT) * (sh + 48);
+        if (best_cost < 0 || cost < best_cost) {
+            best_cost = cost;
+            best = sh;
         }
     }
+    return best;
+}
+
+template <class Src>
+int dtm_strip_launch(Src src, dim3 grid, bool fused, hipStream_t s, const lre::Epilogue *epi) {
+    if (grid.x == 0 || grid.y == 0) return LRE_OK;
+    const bool e = epi && epi->active();
+    const lre::Epilogue E = e ? *epi : lre::Epilogue{};
+    if (fused) {
+        if (e) hipLaunchKernelGGL((dtm_strip<true, true, Src>), grid, dim3(128), 0, s, src, E);
+        else hipLaunchKernelGGL((dtm_strip<true, false, Src>), grid, dim3(128), 0, s, src, E);
+    } else {
+        if (e) hipLaunchKernelGGL((dtm_strip<false, true, Src>), grid, dim3(128), 0, s, src, E);
+        else hipLaunchKernelGGL((dtm_strip<false, false, Src>), grid, dim3(128), 0, s, src, E);
+    }
+    LRE_HIP_CHECK(hipGetLastError());
+    return LRE_OK;
 }
 
 // the guide planes of several images, one chunk of DTM_CHUNK pixels per workgroup
@@ -142,245 +442,6 @@ __global__ void dtm_guide(const float *__restrict__ rgb, float *__restrict__ gui
     }
     for (size_t t = n4 * 4 + (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride)
         guide[t] = gd(rgb[3 * t], rgb[3 * t + 1], rgb[3 * t + 2]);
-}
-
-// One output tile whose guide window is in LDS (dark: no window value exceeds
-// the dark limit, the same for every thread).  Returns with every LDS read of
-// the window's space done before a barrier that all threads passed, and the
-// last pass reading only ha/hb.
-template <int T, int NT, bool EPI>
-LRE_DEV void dtm_tile_one(const DtmImg &I, int x0, int y0, bool dark, float *dsm, const lre::Epilogue &E) {
-    using D = DtmGeo<T>;
-    constexpr int G = D::G, M = D::M, GP = D::GP, MP = D::MP, TP = D::TP;
-    float *gw = dsm;                               // G x G guide window (pitch GP)
-    float *pa = dsm, *pb = dsm + M * MP;           // M x M a and b (pitch MP)
-    float *hg = dsm + D::L_AB, *hq = hg + G * MP;  // G x M row means of g and g*g
-    float *ha = dsm + D::L_AB, *hb = ha + M * TP;  // M x T row means of a and b
-    const float *in = I.in;
-    float *out = I.out;
-    const int W = I.W, H = I.H;
-    const int tid = threadIdx.x;
-    // the reference leaves every pixel of a dark tile unchanged
-    if (dark) {
-        if (!EPI && in == out) return;
-        // each thread reads and then writes its own pixels (safe in place)
-        constexpr int PT = (T * T + NT - 1) / NT;
-        float v[PT][3];
-#pragma unroll
-        for (int k = 0; k < PT; k++) {
-            const int t = tid + k * NT, x = x0 + (t % T), y = y0 + (t / T);
-            if (t < T * T && x < W && y < H) {
-                const size_t i = ((size_t)y * W + x) * 3;
-                v[k][0] = in[i];
-                v[k][1] = in[i + 1];
-                v[k][2] = in[i + 2];
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < PT; k++) {
-            const int t = tid + k * NT, x = x0 + (t % T), y = y0 + (t / T);
-            if (t < T * T && x < W && y < H) {
-                const size_t i = ((size_t)y * W + x) * 3;
-                float r = v[k][0], g = v[k][1], b = v[k][2];
-                if (EPI) lre::epilogue_rgb(r, g, b, E);
-                out[i] = r;
-                out[i + 1] = g;
-                out[i + 2] = b;
-            }
-        }
-        return;
-    }
-    // This thread's pixels of the last pass (a run of RUND rows of one column)
-    // are loaded now, so that their latency passes behind the box means: loaded
-    // in the last pass, each would wait for its own round trip between the
-    // stores, which may alias the input (in place).
-    static_assert(T * D::ND <= NT, "one run of the last pass per thread");
-    const int c4 = tid % T, run4 = tid / T, x4 = x0 + c4;
-    float px[D::RUND][3];
-    if (tid < T * D::ND) {
-#pragma unroll
-        for (int j = 0; j < D::RUND; j++) {
-            const int y = y0 + run4 * D::RUND + j;
-            if (x4 < W && y < H) {
-                const size_t i = ((size_t)y * W + x4) * 3;
-                px[j][0] = in[i];
-                px[j][1] = in[i + 1];
-                px[j][2] = in[i + 2];
-            }
-        }
-    }
-    // row means of g and g*g: rows [0, G), output columns [DR, DR + M)
-    for (int t = tid; t < G * 8; t += NT) {
-        const int r = t % G, run = t / G;
-        const float *v = gw + r * GP + run * D::RUN1;
-        box_run2<D::RUN1>([&](int k) { const float g = v[k]; return make_float2(g, g * g); },
-                          x0 - DR + run * D::RUN1, W, [&](int j, float m0, float m1) {
-                              hg[r * MP + run * D::RUN1 + j] = m0;
-                              hq[r * MP + run * D::RUN1 + j] = m1;
-                          });
-    }
-    __syncthreads();
-    // column means -> mean, mean of squares -> a, b over the M x M window
-    for (int t = tid; t < M * D::NC; t += NT) {
-        const int c = t % M, run = t / M;
-        const int x = x0 - DR + c;
-        const float *v0 = hg + run * D::RUNC * MP + c, *v1 = hq + run * D::RUNC * MP + c;
-        box_run2<D::RUNC>([&](int k) { return make_float2(v0[k * MP], v1[k * MP]); }, y0 - DR + run * D::RUNC, H,
-                          [&](int j, float mean, float msq) {
-                              const int rr = run * D::RUNC + j, y = y0 - DR + rr;
-                              float a = 0.f, b = 0.f;  // outside the image: no taps
-                              if (x >= 0 && x < W && y >= 0 && y < H) {
-                                  const float var = lre::max_std(0.0f, msq - mean * mean);
-                                  a = var / (var + 0.0025f);
-                                  b = mean * (1.0f - a);
-                              }
-                              pa[rr * MP + c] = a;
-                              pb[rr * MP + c] = b;
-                          });
-    }
-    __syncthreads();
-    // row means of a and b: rows [0, M), output columns [DR, DR + T)
-    for (int t = tid; t < M * 8; t += NT) {
-        const int r = t % M, run = t / M;
-        const float *v0 = pa + r * MP + run * D::RUN2, *v1 = pb + r * MP + run * D::RUN2;
-        box_run2<D::RUN2>([&](int k) { return make_float2(v0[k], v1[k]); }, x0 + run * D::RUN2, W,
-                          [&](int j, float m0, float m1) {
-                              ha[r * TP + run * D::RUN2 + j] = m0;
-                              hb[r * TP + run * D::RUN2 + j] = m1;
-                          });
-    }
-    __syncthreads();
-    // column means -> A, B; base = max(1e-6, A guide + B); RGB *= 1 + (gain - 1) w
-    if (tid < T * D::ND) {
-        const int c = c4, run = run4;
-        const int x = x4;
-        const float *v0 = ha + run * D::RUND * TP + c, *v1 = hb + run * D::RUND * TP + c;
-        box_run2<D::RUND>([&](int k) { return make_float2(v0[k * TP], v1[k * TP]); }, y0 + run * D::RUND, H,
-                          [&](int j, float ma, float mb) {
-                              const int y = y0 + run * D::RUND + j;
-                              if (x >= W || y >= H) return;
-                              const size_t i = ((size_t)y * W + x) * 3;
-                              float r = px[j][0], g = px[j][1], b = px[j][2];
-                              const float gd = lre::max_std(0.0f, lre::max_std(r, lre::max_std(g, b)));
-                              const float base = lre::max_std(1e-6f, ma * gd + mb);
-                              float gain = lre::clamp_std(lre::aces(base) / base, 0.0f, 1.0f);
-                              float w = lre::clamp_std((base - 0.55f) / (0.90f - 0.55f), 0.0f, 1.0f);
-                              w = w * w * (3.0f - 2.0f * w);
-                              if (w > 0.0f) {
-                                  gain = 1.0f + (gain - 1.0f) * w;
-                                  r *= gain;
-                                  g *= gain;
-                                  b *= gain;
-                              } else if (!EPI && in == out) {
-                                  return;  // unchanged in place: nothing to store
-                              }
-                              if (EPI) lre::epilogue_rgb(r, g, b, E);
-                              out[i] = r;
-                              out[i + 1] = g;
-                              out[i + 2] = b;
-                          });
-    }
-}
-
-// The guide window of a tile, as NT threads hold it in registers: thread
-// (c, r0) = (tid % G, tid / G) holds column c of rows r0, r0 + PER, ...
-template <int T, int NT>
-struct DtmWin {
-    static constexpr int G = DtmGeo<T>::G, PER = NT / G, K = (G + PER - 1) / PER;
-    static_assert(PER >= 1, "a window row fits the workgroup");
-    float v[K];
-    LRE_DEV void load(const DtmImg &I, int x0, int y0) {
-        const int tid = threadIdx.x, c = tid % G, r0 = tid / G;
-        const int x = x0 - 2 * DR + c;
-        const bool xin = x >= 0 && x < I.W && r0 < PER;
-#pragma unroll
-        for (int k = 0; k < K; k++) {
-            const int r = r0 + k * PER, y = y0 - 2 * DR + r;
-            v[k] = (xin && r < G && y >= 0 && y < I.H) ? I.guide[(size_t)y * I.W + x] : 0.f;
-        }
-    }
-    // into the LDS window (pitch G + 1); false if any value exceeds the dark limit
-    LRE_DEV bool store(float *gw) const {
-        const int tid = threadIdx.x, c = tid % G, r0 = tid / G;
-        bool dark = true;
-        if (r0 < PER) {
-#pragma unroll
-            for (int k = 0; k < K; k++) {
-                const int r = r0 + k * PER;
-                if (r < G) {
-                    gw[r * (G + 1) + c] = v[k];
-                    dark = dark && v[k] <= DTM_DARK;  // NaN is not dark
-                }
-            }
-        }
-        return dark;
-    }
-};
-
-// Tile t of a launch: a flat index over the tiles of one image (row-major,
-// tiles_x from the width) or of several (the table's binary search).
-template <int T>
-LRE_DEV lre::TileOf dtm_tile_at(const lre::One<DtmImg> &s, int t) {
-    const int tx = (s.a.W + T - 1) / T;
-    return {0, t % tx, t / tx};
-}
-template <int T>
-LRE_DEV lre::TileOf dtm_tile_at(const lre::Many<DtmImg> &s, int t) { return s.tile_at(t); }
-
-// The tiles of workgroup w, XCD by XCD: workgroups are dealt round-robin to the
-// 8 XCDs, so the workgroups of one XCD (w % 8) take one contiguous range of
-// tiles in row-major order between them, and neighbouring tiles, which share
-// most of their guide windows, meet in that XCD's L2.
-struct DtmOrder {
-    int t, step, end;
-    LRE_DEV DtmOrder(int ntiles) {
-        constexpr int XCDS = 8;
-        const int g = (int)gridDim.x, w = (int)blockIdx.x, x = w % XCDS;
-        int before = 0;
-        for (int k = 0; k < x; k++) before += (g - k + XCDS - 1) / XCDS;
-        const int mine = (g - x + XCDS - 1) / XCDS;
-        const long long b0 = (long long)ntiles * before / g, b1 = (long long)ntiles * (before + mine) / g;
-        t = (int)b0 + w / XCDS;
-        step = mine;
-        end = (int)b1;
-    }
-};
-
-// EPI: the colour matrix and gamma stages that follow in a stage list run on
-// the result before it is stored (the same float operations as their own pass).
-// Src: lre::One<DtmImg> (one image) or lre::Many<DtmImg> (the tiles of several
-// images).  The grid is persistent: each workgroup walks its tiles (DtmOrder)
-// and loads the next tile's guide window into registers while it works on the
-// current one, so the window's latency passes behind the box means.
-template <int T, int NT, bool EPI, class Src>
-__global__ __launch_bounds__(NT) void dtm_tile(Src src, int ntiles, lre::Epilogue E) {
-    using D = DtmGeo<T>;
-    constexpr int G = D::G, M = D::M, MP = D::MP, TP = D::TP;
-    static_assert(D::GP == G + 1, "window pitch");
-    extern __shared__ float dsm[];
-    float *gw = dsm;                               // G x G guide window (pitch GP)
-    float *pa = dsm, *pb = dsm + M * MP;           // M x M a and b (pitch MP)
-    float *hg = dsm + D::L_AB, *hq = hg + G * MP;  // G x M row means of g and g*g
-    float *ha = dsm + D::L_AB, *hb = ha + M * TP;  // M x T row means of a and b
-    const int tid = threadIdx.x;
-    DtmOrder ord(ntiles);
-    DtmWin<T, NT> win;
-    if (ord.t < ord.end) {
-        const lre::TileOf tl = dtm_tile_at<T>(src, ord.t);
-        win.load(src.args(tl), tl.tx * T, tl.ty * T);
-    }
-    for (; ord.t < ord.end; ord.t += ord.step) {
-        const lre::TileOf tl = dtm_tile_at<T>(src, ord.t);
-        const DtmImg &I = src.args(tl);
-        // The previous tile's last pass reads only ha/hb, which the window does
-        // not overlap; its passes that read the window's space ended at a barrier.
-        const bool dark = win.store(gw);
-        if (ord.t + ord.step < ord.end) {
-            const lre::TileOf nt = dtm_tile_at<T>(src, ord.t + ord.step);
-            win.load(src.args(nt), nt.tx * T, nt.ty * T);
-        }
-        dtm_tile_one<T, NT, EPI>(I, tl.tx * T, tl.ty * T, __syncthreads_and(dark), dsm, E);
-    }
 }
 
 // ---- white balance, core/cpu_accelerator.cpp:2813-2869 ---------------------
@@ -415,34 +476,6 @@ __global__ void rgb3_to_cfa_kernel(const float *__restrict__ rgb3, float *__rest
                                         : lre::bayer3(filters, row, col);
         cfa[i] = rgb3[i * 3 + c];
     }
-}
-
-// The tile kernel at side T with NT threads over ntiles tiles, one persistent
-// workgroup per CU.  64 x 64 tiles of 1024 threads (one workgroup per CU,
-// 142 KiB of LDS) measured faster on 24 MP than 32 x 32 tiles of 512 (two per
-// CU): 0.44 against 0.54 ms, the smaller tiles redoing half again as many box
-// means in their rings.
-template <int T, int NT, class Src>
-int dtm_launch(Src src, int ntiles, hipStream_t s, const lre::Epilogue *epi) {
-    if (ntiles < 1) return LRE_OK;
-    const int cus = lre_current_cus();
-    const dim3 grid(ntiles < cus ? ntiles : cus);
-    static const hipError_t attr = [] {
-        hipError_t e = hipFuncSetAttribute((const void *)dtm_tile<T, NT, false, Src>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)DtmGeo<T>::LDS);
-        if (e == hipSuccess)
-            e = hipFuncSetAttribute((const void *)dtm_tile<T, NT, true, Src>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)DtmGeo<T>::LDS);
-        return e;
-    }();
-    LRE_HIP_CHECK(attr);
-    if (epi && epi->active())
-        hipLaunchKernelGGL((dtm_tile<T, NT, true, Src>), grid, dim3(NT), DtmGeo<T>::LDS, s, src, ntiles, *epi);
-    else
-        hipLaunchKernelGGL((dtm_tile<T, NT, false, Src>), grid, dim3(NT), DtmGeo<T>::LDS, s, src, ntiles,
-                           lre::Epilogue{});
-    LRE_HIP_CHECK(hipGetLastError());
-    return LRE_OK;
 }
 
 int grid_for(size_t n, int block) {
@@ -600,35 +633,44 @@ int lre_launch_tone_mapping(const float *d_in, float *d_out, size_t n, float aft
     return LRE_OK;
 }
 
-// the guide plane
+// the guide plane (used when the stage runs in place)
 size_t lre_detail_tm_scratch_bytes(int w, int h) { return (size_t)w * h * sizeof(float); }
 
+// In place, the guide plane is written first; with a separate output the strip
+// kernel takes the guide from the input pixels and the plane is not used.
 int lre_launch_detail_tonemap(const float *d_in, float *d_out, int w, int h, void *d_scratch,
                               hipStream_t s, const lre::Epilogue *epi) {
-    float *guide = static_cast<float *>(d_scratch);
-    const size_t n = (size_t)w * h;
-    const size_t n4 = (reinterpret_cast<uintptr_t>(d_in) % 16 == 0) ? n / 4 : 0;
-    hipLaunchKernelGGL(dtm_guide, dim3(grid_for(n4 > 0 ? n4 : n, 256)), dim3(256), 0, s, d_in, guide, n, n4);
-    constexpr int T = 64;
-    const lre::One<DtmImg> one{DtmImg{guide, d_in, d_out, w, h}};
-    return dtm_launch<T, 1024>(one, ((w + T - 1) / T) * ((h + T - 1) / T), s, epi);
+    if (w < 1 || h < 1) return LRE_OK;
+    const bool fused = d_in != d_out;
+    float *guide = nullptr;
+    if (!fused) {
+        guide = static_cast<float *>(d_scratch);
+        const size_t n = (size_t)w * h;
+        const size_t n4 = (reinterpret_cast<uintptr_t>(d_in) % 16 == 0) ? n / 4 : 0;
+        hipLaunchKernelGGL(dtm_guide, dim3(grid_for(n4 > 0 ? n4 : n, 256)), dim3(256), 0, s, d_in, guide, n, n4);
+    }
+    const int seg = dtm_seg_rows(w, h);
+    const lre::One<DtmImg> one{DtmImg{guide, d_in, d_out, w, h, seg}};
+    return dtm_strip_launch(one, dim3(((w + DS_W - 1) / DS_W) * ((h + seg - 1) / seg)), fused, s, epi);
 }
 
-// Many images: two tables (guide chunks, then tiles) over the same per-image
-// arguments, each guide plane at the start of the image's scratch.
+// Many images: two tables (guide chunks, then strip segments) over the same
+// per-image arguments, each guide plane at the start of the image's scratch.
 using DtmTable = lre::ManyTable<DtmImg>;
 size_t lre_detail_tm_multi_table_bytes(int n) { return 2 * ((DtmTable::bytes(n) + 255) & ~(size_t)255); }
 
 int lre_detail_tm_multi_fill(const lre_post_job *jobs, int n, void *h_table, int grids[2]) {
     const size_t half = (DtmTable::bytes(n) + 255) & ~(size_t)255;
     auto args = [&](int i) {
-        return DtmImg{static_cast<const float *>(jobs[i].d_scratch), jobs[i].d_in, jobs[i].d_out, jobs[i].w, jobs[i].h};
+        return DtmImg{static_cast<const float *>(jobs[i].d_scratch), jobs[i].d_in, jobs[i].d_out, jobs[i].w,
+                      jobs[i].h, dtm_seg_rows(jobs[i].w, jobs[i].h)};
     };
     grids[0] = DtmTable::fill(h_table, n, [&](int i) {
         return make_int2((int)(((size_t)jobs[i].w * jobs[i].h + DTM_CHUNK - 1) / DTM_CHUNK), 1);
     }, args);
     grids[1] = DtmTable::fill(static_cast<char *>(h_table) + half, n, [&](int i) {
-        return make_int2((jobs[i].w + 63) / 64, (jobs[i].h + 63) / 64);
+        const int seg = dtm_seg_rows(jobs[i].w, jobs[i].h);
+        return make_int2((jobs[i].w + DS_W - 1) / DS_W, (jobs[i].h + seg - 1) / seg);
     }, args);
     return LRE_OK;
 }
@@ -639,7 +681,8 @@ int lre_launch_detail_tm_multi(const void *d_table, int n, const int grids[2], h
     const size_t half = (DtmTable::bytes(n) + 255) & ~(size_t)255;
     hipLaunchKernelGGL(dtm_guide_many, dim3(grids[0]), dim3(256), 0, s, DtmTable::view(d_table, n));
     LRE_HIP_CHECK(hipGetLastError());
-    return dtm_launch<64, 1024>(DtmTable::view(static_cast<const char *>(d_table) + half, n), grids[1], s, epi);
+    return dtm_strip_launch(DtmTable::view(static_cast<const char *>(d_table) + half, n), dim3(grids[1]), false, s,
+                            epi);
 }
 
 int lre_launch_white_balance(const uint16_t *d_img4, float *d_rgb, int w, int h,

@@ -36,6 +36,10 @@ class DevelopChain:
         n = height * width
         self.rgb = acc.dev_alloc(n * 12)
         self.stages = list(stages or available_chain())
+        # the detail tone map writes into a second buffer, which then becomes the
+        # image: with a separate output its guide comes straight from the input
+        # pixels (no guide-plane pass), with the same result as in place
+        self.rgb_alt = acc.dev_alloc(n * 12) if "detail_tm" in self.stages else None
 
     def run_stage(self, name: str, d_cfa, stream=None):
         lib, ctx, h, w = native(), self.acc.handle, self.h, self.w
@@ -61,7 +65,10 @@ class DevelopChain:
                                             p["strength"], int(p["defringe_green"]),
                                             p["green_strength_scale"], stream))
         elif name == "detail_tm":
-            _check(lib.lre_hip_dev_detail_tonemap(ctx, self.rgb.ptr, self.rgb.ptr, w, h, stream))
+            if self.rgb_alt is None:
+                self.rgb_alt = self.acc.dev_alloc(h * w * 12)
+            _check(lib.lre_hip_dev_detail_tonemap(ctx, self.rgb.ptr, self.rgb_alt.ptr, w, h, stream))
+            self.rgb, self.rgb_alt = self.rgb_alt, self.rgb
         elif name == "tone_mapping":
             _check(lib.lre_hip_dev_tone_mapping(ctx, self.rgb.ptr, self.rgb.ptr, w, h, 1.0, stream))
         else:

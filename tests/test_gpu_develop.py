@@ -120,6 +120,28 @@ def test_detail_tonemap_tolerance(acc, h, w):
     assert psnr(got, want) >= 70.0
 
 
+@pytest.mark.parametrize("h,w", [(1, 1), (5, 70), (130, 64), (300, 129), (481, 641), (1000, 300)])
+def test_detail_tonemap_out_of_place(acc, h, w):
+    """With a separate output the guide comes from the input pixels instead of
+    a guide plane: the same bits as in place, within tolerance of the oracle
+    (frames across strip and segment edges)."""
+    from libraw_enhanced_amd.accelerator import native
+    print(f"detail TM out of place {h}x{w}", flush=True)
+    rgb = scene.render(h, w, scene.BASE_SEED + h + 3 * w) * 1.6
+    lib = native()
+    d_in = acc.dev_from_host(rgb)
+    d_out = acc.dev_alloc(rgb.nbytes)
+    assert lib.lre_hip_dev_detail_tonemap(acc.handle, d_in.ptr, d_out.ptr, w, h, None) == 0
+    acc.synchronize()
+    got = d_out.download((h, w, 3))
+    assert np.array_equal(d_in.download((h, w, 3)), rgb)  # the input is left as it was
+    inplace = acc.apply_detail_preserving_tonemap(rgb)
+    assert np.array_equal(got, inplace), f"{np.count_nonzero(got != inplace)} values differ from in place"
+    want = oracle_detail_tonemap(rgb)
+    assert float(np.abs(got - want).max()) <= 2e-4
+    assert psnr(got, want) >= 70.0
+
+
 def test_white_balance_bit_exact(acc):
     h, w = 123, 211
     cfa = scene.bayer_scene(h, w, scene.BASE_SEED)
