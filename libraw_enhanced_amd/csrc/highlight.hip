@@ -43,7 +43,7 @@ constexpr int HR_LW = HR_TW + 17;    // staged columns: t - 8 .. t + W + 8, odd 
 constexpr int HR_WORDS = (HR_TW + 31) / 32;
 constexpr int HR_OW = HR_TW + 1;     // row stride of the tile's own pixels (odd: no LDS bank conflicts)
 constexpr size_t HR_LDS = sizeof(float) * (2 * HR_SR * HR_LW + 3 * HR_TR * HR_OW) + HR_TR * HR_TW +
-                          sizeof(unsigned) * HR_WORDS;
+                          sizeof(unsigned) * HR_WORDS * (1 + HR_TR);
 
 struct HrState {
     float grf, gbf;     // ratio means after the division by the count
@@ -311,36 +311,53 @@ struct LocaTile {
 };
 
 // LoCA green lift of the tile pixel (r, s) (:1299-1372).  Neighbour (dy, dx)
-// sits at c + dy * (HR_LW + 3) + dx of the skewed planes.
-LRE_DEV void loca_px(const LocaTile &L, int r, int s, float sat, float sat075, float max_val) {
+// sits at c + dy * (HR_LW + 3) + dx of the skewed planes.  Every LDS read of
+// the step is issued first, so the step waits for the LDS once.  The pixel is
+// a candidate: its own values, which no earlier step changes, already pass
+// l >= 0.9 sat and rb > 1.04 G (:1308-1310, :1327-1330).
+// A changed pixel is stored to the image at once (index gi): no later step of
+// this tile reads it from there, and later tiles stage it after this launch.
+LRE_DEV void loca_px(const LocaTile &L, int r, int s, float sat, float sat075, float max_val, float *img,
+                     size_t gi) {
     const int o = r * HR_OW + s, c = (r + 2) * HR_LW + s + 8;
-    const float p0 = L.OR[o], p1 = L.OG[o], p2 = L.OB[o];
-    const float l = max3(p0, p1, p2);
-    if (l < sat * 0.90f) return;
     constexpr int DY = HR_LW + 3;
-    float local_max = 0.0f, local_min = FLT_MAX;
+    const float p0 = L.OR[o], p1 = L.OG[o], p2 = L.OB[o];
+    float nl[9], rt[25];
 #pragma unroll
     for (int dy = -1; dy <= 1; ++dy)
 #pragma unroll
-        for (int dx = -1; dx <= 1; ++dx) {
-            const float nl = L.NL[c + dy * DY + dx];
-            local_max = lre::max_std(local_max, nl);
-            local_min = lre::min_std(local_min, nl);
-        }
+        for (int dx = -1; dx <= 1; ++dx) nl[(dy + 1) * 3 + dx + 1] = L.NL[c + dy * DY + dx];
+#pragma unroll
+    for (int dy = -2; dy <= 2; ++dy)
+#pragma unroll
+        for (int dx = -2; dx <= 2; ++dx) rt[(dy + 2) * 5 + dx + 2] = L.RT[c + dy * DY + dx];
+    const float l = max3(p0, p1, p2);
+    // The reference's running max from 0 and min from FLT_MAX never take a
+    // NaN (a comparison with NaN is false), so they are the maximum and
+    // minimum over the numbers and either order gives them: a tree of the
+    // hardware max/min, which skip NaN.  (Only the sign of a zero extreme can
+    // differ, and a zero spread returns below either way.)
+    float mx4[4], mn4[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        mx4[k] = fmaxf(nl[2 * k], nl[2 * k + 1]);
+        mn4[k] = fminf(nl[2 * k], nl[2 * k + 1]);
+    }
+    const float local_max = fmaxf(fmaxf(fmaxf(mx4[0], mx4[1]), fmaxf(mx4[2], mx4[3])), fmaxf(nl[8], 0.0f));
+    const float local_min = fminf(fminf(fminf(mn4[0], mn4[1]), fminf(mn4[2], mn4[3])), fminf(nl[8], FLT_MAX));
     if ((local_max - local_min) < 0.015f) return;
     const float rb = 0.5f * (p0 + p2);
-    if (rb <= p1 * 1.04f) return;
     // The reference skips a tap; adding +0.0 instead leaves both sums unchanged
-    // (no taken term is -0.0).
+    // (no taken term is -0.0).  The sums run in the reference's order.
     double ratio_sum = 0.0, ratio_w = 0.0;
 #pragma unroll
     for (int dy = -2; dy <= 2; ++dy)
 #pragma unroll
         for (int dx = -2; dx <= 2; ++dx) {
-            const float rt = L.RT[c + dy * DY + dx];
-            const bool take = rt != RT_SKIP;
+            const float t = rt[(dy + 2) * 5 + dx + 2];
+            const bool take = t != RT_SKIP;
             const double wt = 1.0 / (double)(1 + (dx < 0 ? -dx : dx) + (dy < 0 ? -dy : dy));
-            ratio_sum += take ? rt * wt : 0.0;
+            ratio_sum += take ? t * wt : 0.0;
             ratio_w += take ? wt : 0.0;
         }
     const float local_ratio = (float)(ratio_w > 0.0 ? ratio_sum / ratio_w : 1.0);
@@ -364,6 +381,18 @@ LRE_DEV void loca_px(const LocaTile &L, int r, int s, float sat, float sat075, f
     L.OG[o] = q1;
     L.OB[o] = q2;
     tap_info(q0, q1, q2, sat075, L.NL[c], L.RT[c]);
+    img[3 * gi] = q0;
+    img[3 * gi + 1] = q1;
+    img[3 * gi + 2] = q2;
+}
+
+// LDS writes of this wavefront's lanes are seen by its later LDS reads (the
+// wave runs in order; this keeps the compiler from moving a read of another
+// lane's cell above that lane's write)
+LRE_DEV void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // Tile (b_lo + blockIdx.x, d - b) of the anti-diagonal d.  Staged row sr holds
@@ -418,39 +447,38 @@ __global__ __launch_bounds__(HR_BLOCK) void hr_loca(float *img, int w, int h, fl
         },
         [&](int r, int s, unsigned char f) { F[r * HR_TW + s] = f; });
     __syncthreads();
-    for (int e = threadIdx.x; e < HR_TR * HR_TW; e += HR_BLOCK)
-        if (F[e]) atomicOr(&step_any[(e % HR_TW) >> 5], 1u << ((e % HR_TW) & 31));
+    // each row's flags as bits, one word per thread (HR_TR x HR_WORDS threads),
+    // and their union over the rows per step
+    unsigned *msk = step_any + HR_WORDS;
+    static_assert(HR_TR * HR_WORDS <= HR_BLOCK, "one mask word per thread");
+    if (threadIdx.x < HR_TR * HR_WORDS) {
+        const int rr = threadIdx.x / HR_WORDS, wd = threadIdx.x % HR_WORDS;
+        unsigned m = 0u;
+        for (int q = 0; q < 32 && wd * 32 + q < HR_TW; q++) m |= (unsigned)(F[rr * HR_TW + wd * 32 + q] != 0) << q;
+        msk[threadIdx.x] = m;
+        if (m) atomicOr(&step_any[wd], m);
+    }
     __syncthreads();
+    // One wavefront runs the steps: lane r takes tile row r, and only the LDS
+    // orders one step's writes before the next step's reads.
+    if (threadIdx.x >= 64) return;
     const float max_val = __uint_as_float(st->max_bits);
     const int r = threadIdx.x;
-    // this lane's flags as bits, so the step loop reads no LDS to decide
+    const int y = Y + r;
     unsigned fb[HR_WORDS];
 #pragma unroll
-    for (int wd = 0; wd < HR_WORDS; wd++) {
-        fb[wd] = 0u;
-        if (r < HR_TR)
-            for (int q = 0; q < 32 && wd * 32 + q < HR_TW; q++)
-                fb[wd] |= (unsigned)(F[r * HR_TW + wd * 32 + q] != 0) << q;
-    }
+    for (int wd = 0; wd < HR_WORDS; wd++) fb[wd] = msk[r * HR_WORDS + wd];
 #pragma unroll
     for (int wd = 0; wd < HR_WORDS; wd++) {
         unsigned bits = step_any[wd];
         while (bits) {
             const int q = __ffs(bits) - 1;
             bits &= bits - 1;
-            if ((fb[wd] >> q) & 1u) loca_px(L, r, wd * 32 + q, sat, sat075, max_val);
-            __syncthreads();
+            const int sidx = wd * 32 + q;
+            if ((fb[wd] >> q) & 1u)
+                loca_px(L, r, sidx, sat, sat075, max_val, img, (size_t)y * w + (T + sidx - 3 * y));
+            wave_lds_sync();
         }
-    }
-    for (int e = threadIdx.x; e < HR_TR * HR_TW; e += HR_BLOCK) {
-        if (!F[e]) continue;
-        const int rr = e / HR_TW, s = e - rr * HR_TW;
-        const int y = Y + rr, x = T + s - 3 * y;
-        const size_t i = (size_t)y * w + x;
-        const int o = rr * HR_OW + s;
-        img[3 * i] = L.OR[o];
-        img[3 * i + 1] = L.OG[o];
-        img[3 * i + 2] = L.OB[o];
     }
 }
 
