@@ -290,15 +290,15 @@ __global__ __launch_bounds__(HR_BLOCK) void hr_rewrite(const float *__restrict__
 
 // What a pixel contributes to a LoCA window (:1311-1345): NL = max(R, G, B)
 // for the 3x3 contrast, RT = G / ((R + B) / 2) when the 5x5 ratio mean takes
-// it, else RT_SKIP (a value no taken ratio has: those lie in [0.70, 1.35] or
-// are NaN).  Both depend only on the pixel's own values, so the window reads
-// them instead of redoing the division for every tap.
-constexpr float RT_SKIP = -1.0f;
+// it, else 0 (no taken ratio is 0: those lie in [0.70, 1.35] or are NaN).
+// Both depend only on the pixel's own values, so the window reads them
+// instead of redoing the division for every tap, and a skipped tap adds
+// 0 * w = +0 to the ratio sum as it stands.
 
 LRE_DEV void tap_info(float r, float g, float b, float sat075, float &nl, float &rt) {
     nl = max3(r, g, b);
     const float nrb = 0.5f * (r + b);
-    rt = RT_SKIP;
+    rt = 0.0f;
     if (nl < sat075 || nrb <= 1e-6f) return;
     const float ratio = g / nrb;
     if (ratio < 0.70f || ratio > 1.35f) return;
@@ -355,10 +355,9 @@ LRE_DEV void loca_px(const LocaTile &L, int r, int s, float sat, float sat075, f
 #pragma unroll
         for (int dx = -2; dx <= 2; ++dx) {
             const float t = rt[(dy + 2) * 5 + dx + 2];
-            const bool take = t != RT_SKIP;
             const double wt = 1.0 / (double)(1 + (dx < 0 ? -dx : dx) + (dy < 0 ? -dy : dy));
-            ratio_sum += take ? t * wt : 0.0;
-            ratio_w += take ? wt : 0.0;
+            ratio_sum += t * wt;
+            ratio_w += t != 0.0f ? wt : 0.0;
         }
     const float local_ratio = (float)(ratio_w > 0.0 ? ratio_sum / ratio_w : 1.0);
     const float desired_g = rb * lre::clamp_std(local_ratio, 0.95f, 1.08f);
