@@ -183,6 +183,10 @@ int lre_hip_dev_xtrans3(lre_ctx *ctx, const float *d_cfa, float *d_rgb, int widt
                         const signed char xtrans[36], float maximum_value, void *stream);
 int lre_hip_dev_tone_mapping(lre_ctx *ctx, const float *d_in, float *d_out, int width,
                              int height, float after_scale, void *stream);
+/* Detail-preserving tone map on resident buffers.  In place (d_in == d_out)
+ * it first writes a guide plane into the context scratch; with a separate
+ * output it reads the guide from d_in directly and needs no scratch (faster,
+ * same result).  d_in and d_out must not otherwise overlap. */
 int lre_hip_dev_detail_tonemap(lre_ctx *ctx, const float *d_in, float *d_out, int width,
                                int height, void *stream);
 /* Micro-contrast on resident buffers; d_mask is a device plane or NULL. */

@@ -299,6 +299,8 @@ int lre_hip_dev_detail_tonemap(lre_ctx *ctx, const float *d_in, float *d_out, in
                                void *stream) {
     if (!ctx || !d_in || !d_out) return set_arg_error("null pointer");
     if (!valid_dims(w, h)) return set_arg_error("image size");
+    // with a separate output the guide is read from the input: no scratch
+    if (d_in != d_out) return lre_launch_detail_tonemap(d_in, d_out, w, h, nullptr, lre_pick_stream(ctx, stream));
     int rc = lre_ensure_scratch(ctx, lre_detail_tm_scratch_bytes(w, h), lre_pick_stream(ctx, stream));
     if (rc) return rc;
     lre_scratch_guard scratch_use{ctx, lre_pick_stream(ctx, stream)};

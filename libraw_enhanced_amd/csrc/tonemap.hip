@@ -170,9 +170,9 @@ struct DtmStrip {
         for (int r = 0; r < DS_RB; r++) {
             const int y = min(max(yb + r - DR, 0), I.H - 1);
             const float *p = I.in + 3 * ((size_t)y * I.W + cx);
-            ld[3 * r] = p[0];
-            ld[3 * r + 1] = p[1];
-            ld[3 * r + 2] = p[2];
+            ld[3 * r] = __builtin_nontemporal_load(p);
+            ld[3 * r + 1] = __builtin_nontemporal_load(p + 1);
+            ld[3 * r + 2] = __builtin_nontemporal_load(p + 2);
         }
     }
 
@@ -297,9 +297,9 @@ struct DtmStrip {
             if (EPI) lre::epilogue_rgb(R, G, Bl, E);
             if (EPI || !inplace || w > 0.0f) {  // unchanged in place: nothing to store
                 float *o = I.out + 3 * ((size_t)yo * I.W + ox);
-                o[0] = R;
-                o[1] = G;
-                o[2] = Bl;
+                __builtin_nontemporal_store(R, o);
+                __builtin_nontemporal_store(G, o + 1);
+                __builtin_nontemporal_store(Bl, o + 2);
             }
             // one row's output at a time: interleaving the rows' arithmetic
             // would hold more registers than the occupancy leaves
