@@ -86,11 +86,11 @@ def _single(acc, it, stages, p):
     if it["inp"] == "u16":
         d_cfa = acc.dev_alloc(h * w * 4)
         d_max = acc.dev_alloc(4)
-        wb = np.array(WB, np.float32)
-        cb = (ct.c_int * 4)(0, 0, 0, 0)
+        wb = np.array(it.get("wb", WB), np.float32)
         # an input already black-subtracted (preprocessed) is ingested with black 0
         d_raw = it.get("d_pre", it["d_in"])
         black_in = 0 if "d_pre" in it else BLACK
+        cb = (ct.c_int * 4)(*((0, 0, 0, 0) if "d_pre" in it else it.get("cblack", (0, 0, 0, 0))))
         assert lib.lre_hip_dev_ingest_u16(ctx, d_raw.ptr, d_cfa.ptr, w, h, filters, xtp, cb, black_in,
                                           WHITE, wb.ctypes.data_as(ct.POINTER(ct.c_float)), d_max.ptr,
                                           None) == 0
@@ -167,6 +167,52 @@ def test_batch_matches_per_image(acc, lanes):
         want = _single(acc, it, CHAIN, p)
         bad = int(np.count_nonzero(g != want))
         assert bad == 0, f"{it['kind']} {it['h']}x{it['w']} {it['inp']}: {bad} values differ"
+    b.close()
+
+
+def test_batch_and_serve_ingest_levels(acc):
+    """Bayer uint16 input through the batch and the request loop: per-colour
+    black levels, a fourth colour (second green) with its own multiplier, and
+    samples below black and above white, bit-identical to the per-image ingest
+    and AMaZE entry points."""
+    from libraw_enhanced_amd import develop as dv
+    from test_ingest_cpu import FOUR_COLOR
+    wb4 = (2.1, 1.0, 1.5, 0.95)
+    cases = [(RGGB, (0, 0, 0, 0), 250, 330), (RGGB, (3, 0, 7, 0), 161, 517),
+             (FOUR_COLOR, (2, 5, 1, 9), 210, 300), (0x16161616, (4, 0, 0, 6), 97, 131)]
+    rng = np.random.default_rng(scene.BASE_SEED + 7)
+    items = []
+    for k, (filters, cblack, h, w) in enumerate(cases):
+        seed = scene.BASE_SEED + 200 + k
+        raw = scene.quantize_u16(scene.mosaic(scene.render(h, w, seed), seed, filters), filters)
+        flat = raw.reshape(-1)
+        pick = rng.choice(flat.size, flat.size // 50, replace=False)
+        flat[pick[: pick.size // 2]] = rng.integers(0, BLACK, pick.size // 2, dtype=np.uint16)
+        flat[pick[pick.size // 2:]] = rng.integers(WHITE - 40, 65536, pick.size - pick.size // 2, dtype=np.uint16)
+        print(f"ingest levels filters={filters:#x} cblack={cblack} {h}x{w}", flush=True)
+        d_in, d_rgb = acc.dev_from_host(raw), acc.dev_alloc(h * w * 12)
+        im = dv.make_image(w, h, filters=filters, cam_mul=WB, d_raw=d_in, black=BLACK, white=WHITE,
+                           cblack=cblack, wb=wb4, d_rgb=d_rgb)
+        items.append(dict(kind="bayer", h=h, w=w, filters=filters, inp="u16", data=raw, xt=None, d_in=d_in,
+                          d_rgb=d_rgb, im=im, cblack=cblack, wb=wb4))
+    b = dv.BatchDeveloper(0, 2)
+    b.run([it["im"] for it in items], ("demosaic",))
+    b.synchronize()
+    server = dv.DevelopServer(0, lanes=2, hbm_budget_bytes=1 << 30, stages=("demosaic",))
+    outs = [np.full((it["h"], it["w"], 3), np.nan, np.float32) for it in items]
+    for k, it in enumerate(items):
+        server.submit(k, it["im"], it["data"].ctypes.data, outs[k].ctypes.data, keep=(it["data"], outs[k]))
+    done = 0
+    while done < len(items):
+        for c in server.poll(timeout_ms=1000):
+            assert c["status"] == 0, c
+            done += 1
+    server.close()
+    for k, it in enumerate(items):
+        want = _single(acc, it, ("demosaic",), None)
+        got = it["d_rgb"].download((it["h"], it["w"], 3))
+        assert np.array_equal(got, want), f"case {k}: batch differs from the per-image path"
+        assert np.array_equal(outs[k], want), f"case {k}: served result differs from the per-image path"
     b.close()
 
 
