@@ -666,26 +666,29 @@ __global__ __launch_bounds__(BLOCK) void correct(Src SR, Scal Cb, Gates G) {
         const bool correct_purple = use_direct ? abs_is_purple : is_purple;
         if (correct_purple) {
             mask *= gate(G_LOCAL, purple_abs);
+        } else if (!eg) {
+            // the reference computes the green terms below and then sets the
+            // mask to 0 without the green correction (:4698-4700): nothing else
+            // reads them
+            mask = 0.f;
         } else {
             const float local_gate = gate(G_LOCAL, green_abs);
-            if (eg) {
-                const float peak_chroma = gate(G_PEAK, green_abs);
-                const float peak_nb = sstepc(0.005f, 0.030f, bfs);
-                mask = mx(mask, peak_chroma * peak_nb * c_bright * c_hue * scene_gate);
-            }
+            const float peak_chroma = gate(G_PEAK, green_abs);
+            const float peak_nb = sstepc(0.005f, 0.030f, bfs);
+            mask = mx(mask, peak_chroma * peak_nb * c_bright * c_hue * scene_gate);
             const float hl_protect = 1.f - 0.85f * sstepc(0.62f, 0.88f, bguide);
             const float rb_to_green = rb_avg / mx(g0, 1e-4f);
             const float natural_hl = sstepc(0.22f, 0.46f, rb_to_green) * sstepc(0.30f, 0.55f, g0);
             const float nat_suppress = 1.f - 0.99f * natural_hl;
             const float nat_suppress_sq = nat_suppress * nat_suppress;
             const float halo = gate(G_HALO, green_abs);
-            const float green_gain = (eg ? 0.60f : 0.35f) + (eg ? 1.10f : 0.95f) * halo;
+            const float green_gain = 0.60f + 1.10f * halo;  // with the green correction
             const float green_nb = sstepc(0.02f, 0.10f, mx(0.f, -mn(brg, bbg)));
             const float smax = mx(r0, mx(g0, b0)), smin = mn(r0, mn(g0, b0));
             const float ssat = (smax - smin) / mx(smax, 1e-4f);
             const float whiteish = 1.f - sstepc(0.05f, 0.13f, ssat);
             const float foliage_guard = green_nb * whiteish * sstepc(0.35f, 0.75f, bguide) * (1.f - halo);
-            const float strength_release = eg ? sstepc(2.0f, 6.0f, strength) : 0.f;
+            const float strength_release = sstepc(2.0f, 6.0f, strength);
             const float line_evidence = sstepc(0.004f, 0.030f, lfs) * sstepc(0.20f, 0.55f, halo) * sstepc(0.05f, 0.15f, ssat);
             const float release = strength_release * line_evidence * (1.f - natural_hl);
             const float rel_hl = hl_protect + (1.f - hl_protect) * release;
@@ -696,7 +699,6 @@ __global__ __launch_bounds__(BLOCK) void correct(Src SR, Scal Cb, Gates G) {
             const float kill_evidence = sstepc(0.04f, 0.15f, lfs);
             const float natural_blob = sqrtf(mx(0.f, natural_hl)) * (1.f - kill_evidence) * green_nb;
             mask *= 1.f - 0.999f * natural_blob;
-            if (!eg) mask = 0.f;
         }
 
         float o0 = s0, o1 = s1, o2 = s2;
