@@ -453,43 +453,74 @@ __device__ void xtrans_tile(const XArgs &A, char *ar, int top, int left) {
         // reference gives the same), their maximum less an eighth, and the
         // average of the most homogeneous directions, per pixel
         const int rows = max(mrow - 8 - rs, 0), cols = max(mcol - 8 - cs, 0);
-        for (int t = threadIdx.x; t < rows * cols; t += BLOCK) {
-            const int row = rs + t / cols, col = cs + t % cols;
-            unsigned s0 = 0, s1 = 0;
-            for (int v = -2; v <= 2; v++)
+        // AVU pixels per thread: the selected directions' samples of both are
+        // loaded before any is summed (the sums keep the direction order)
+        constexpr int AVU = 2;
+        for (int base = threadIdx.x; base < rows * cols; base += BLOCK * AVU) {
+            unsigned sel[AVU];
+            int prow[AVU], pcol[AVU];
+            float p[AVU][8][3];
 #pragma unroll
-                for (int h = -2; h <= 2; h++) {
-                    const int k = (row + v) * TS + col + h;
-                    s0 += hw[k];
-                    s1 += hw[TS * TS + k];
+            for (int u = 0; u < AVU; u++) {
+                const int t = base + u * BLOCK;
+                sel[u] = 0u;
+                prow[u] = pcol[u] = 0;
+                if (t >= rows * cols) continue;
+                const int row = rs + t / cols, col = cs + t % cols;
+                prow[u] = row;
+                pcol[u] = col;
+                unsigned s0 = 0, s1 = 0;
+                for (int v = -2; v <= 2; v++)
+#pragma unroll
+                    for (int h = -2; h <= 2; h++) {
+                        const int k = (row + v) * TS + col + h;
+                        s0 += hw[k];
+                        s1 += hw[TS * TS + k];
+                    }
+                unsigned char hm[8];
+#pragma unroll
+                for (int d = 0; d < 4; d++) {
+                    hm[d] = (unsigned char)(s0 >> (8 * d));
+                    hm[d + 4] = (unsigned char)(s1 >> (8 * d));
                 }
-            unsigned char hm[8];
+                unsigned char mx = hm[0];
 #pragma unroll
-            for (int d = 0; d < 4; d++) {
-                hm[d] = (unsigned char)(s0 >> (8 * d));
-                hm[d + 4] = (unsigned char)(s1 >> (8 * d));
-            }
-            unsigned char mx = hm[0];
+                for (int d = 1; d < 8; d++) mx = mx < hm[d] ? hm[d] : mx;
+                mx -= mx >> 3;
 #pragma unroll
-            for (int d = 1; d < 8; d++) mx = mx < hm[d] ? hm[d] : mx;
-            mx -= mx >> 3;
-#pragma unroll
-            for (int d = 4; d < 8; d++) {
-                if (hm[d - 4] < hm[d]) hm[d - 4] = 0;
-                else if (hm[d - 4] > hm[d]) hm[d] = 0;
-            }
-            float avg[4] = {0.f, 0.f, 0.f, 0.f};
-            for (int d = 0; d < 8; d++)
-                if (hm[d] >= mx) {
-                    const float *p = RGB(rgb0, d, row, col);
-                    for (int c = 0; c < 3; c++) avg[c] += p[c];
-                    avg[3] += 1.f;
+                for (int d = 4; d < 8; d++) {
+                    if (hm[d - 4] < hm[d]) hm[d - 4] = 0;
+                    else if (hm[d - 4] > hm[d]) hm[d] = 0;
                 }
-            float *o = A.out + ((size_t)(row + top) * W + col + left) * 3;
-            const float a0 = avg[0] / avg[3], a1 = avg[1] / avg[3], a2 = avg[2] / avg[3];
-            o[0] = (0.f < a0) ? a0 : 0.f;
-            o[1] = (0.f < a1) ? a1 : 0.f;
-            o[2] = (0.f < a2) ? a2 : 0.f;
+#pragma unroll
+                for (int d = 0; d < 8; d++) sel[u] |= (hm[d] >= mx ? 1u : 0u) << d;
+            }
+#pragma unroll
+            for (int u = 0; u < AVU; u++)
+#pragma unroll
+                for (int d = 0; d < 8; d++)
+                    if ((sel[u] >> d) & 1u) {
+                        const float *q = RGB(rgb0, d, prow[u], pcol[u]);
+#pragma unroll
+                        for (int c = 0; c < 3; c++) p[u][d][c] = q[c];
+                    }
+#pragma unroll
+            for (int u = 0; u < AVU; u++) {
+                if (base + u * BLOCK >= rows * cols) continue;
+                float avg[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int d = 0; d < 8; d++)
+                    if ((sel[u] >> d) & 1u) {
+#pragma unroll
+                        for (int c = 0; c < 3; c++) avg[c] += p[u][d][c];
+                        avg[3] += 1.f;
+                    }
+                float *o = A.out + ((size_t)(prow[u] + top) * W + pcol[u] + left) * 3;
+                const float a0 = avg[0] / avg[3], a1 = avg[1] / avg[3], a2 = avg[2] / avg[3];
+                o[0] = (0.f < a0) ? a0 : 0.f;
+                o[1] = (0.f < a1) ? a1 : 0.f;
+                o[2] = (0.f < a2) ? a2 : 0.f;
+            }
         }
     }
     __syncthreads();
