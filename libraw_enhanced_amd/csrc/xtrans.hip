@@ -106,8 +106,39 @@ LRE_DEV uint2 make_walk(const XArgs &A, int row, int c0, int lim) {
     for (int q = 0; q < 4; q++) w.y |= (unsigned)fcolx(A, row, walk_col(ls, coff, q)) << (8 * q);
     return w;
 }
-// bound on the number of sites a column walk visits in one tile row
-constexpr int KW = (2 * TS) / 3 + 2;
+// bound on the number of sites a column walk visits in one tile row, and in
+// a row whose walk has a stride of 3
+constexpr int KW = (2 * TS) / 3 + 2, K3 = TS / 3 + 1;
+
+// Slots of a loop over the column walks of rows r0 .. r1-1: KW per row, but
+// K3 for a row with a stride-3 walk (two rows of three in the X-Trans
+// pattern), laid out in groups of three rows, so fewer lanes idle on slots
+// past the end of a short walk.  Every site still has exactly one slot.
+struct WalkSlots {
+    int r0, n, g, c1, c2;
+    float inv_g;
+};
+LRE_DEV WalkSlots walk_slots(const XArgs &A, int r0, int r1) {
+    WalkSlots s;
+    const int k0 = A.rshift[r0 % 3] == 1 ? K3 : KW, k1 = A.rshift[(r0 + 1) % 3] == 1 ? K3 : KW,
+              k2 = A.rshift[(r0 + 2) % 3] == 1 ? K3 : KW;
+    const int rows = r1 > r0 ? r1 - r0 : 0, rem = rows % 3;
+    s.r0 = r0;
+    s.g = k0 + k1 + k2;
+    s.c1 = k0;
+    s.c2 = k0 + k1;
+    s.n = rows / 3 * s.g + (rem > 0 ? k0 : 0) + (rem > 1 ? k1 : 0);
+    s.inv_g = 1.0f / (float)s.g;
+    return s;
+}
+// slot t -> (row, k-th site of the row's walk)
+LRE_DEV void walk_slot(const WalkSlots &s, int t, int &row, int &k) {
+    int q, o;
+    lre::divmod(t, s.g, s.inv_g, q, o);
+    const int i = o >= s.c2 ? 2 : (o >= s.c1 ? 1 : 0);
+    row = s.r0 + 3 * q + i;
+    k = o - (i == 2 ? s.c2 : (i == 1 ? s.c1 : 0));
+}
 
 __device__ void xtrans_tile(const XArgs &A, char *ar, int top, int left) {
     const int W = A.W, H = A.H;
@@ -130,8 +161,10 @@ __device__ void xtrans_tile(const XArgs &A, char *ar, int top, int left) {
     // then each pair of adjacent sites fills both its slots with the bounds of
     // its first site, and a last lone site may follow.  No two units of a
     // row share a slot with different values, so the units run in parallel.
-    for (int t = threadIdx.x; t < nr * KW; t += BLOCK) {
-        const int row = top + t / KW, u = t % KW;
+    const WalkSlots sl_all = walk_slots(A, top, top + nr);
+    for (int t = threadIdx.x; t < sl_all.n; t += BLOCK) {
+        int row, u;
+        walk_slot(sl_all, t, row, u);
         const int ls = first_nongreen(A, row, left, mcol0);
         const int coff = A.rshift[row % 3] == 1 ? 3 : 1 + (fcolx(A, row, ls + 1) & 1);
         auto mmhex = [&](int col, const short *hx) {
@@ -194,10 +227,13 @@ __device__ void xtrans_tile(const XArgs &A, char *ar, int top, int left) {
     int nrow22 = 0;
     while (topstart + (3 * nrow22 + adj22) / 2 < mrow0 - 2) nrow22++;
 
+    const WalkSlots sl_pg = walk_slots(A, top + 2, mrow0 - 2), sl_rb = walk_slots(A, top + 3, mrow0 - 3);
+
     // green along four directions (:2075-2128), the value that lands in buffer j
     auto green4 = [&](int j) {
-        for (int t = threadIdx.x; t < nr * KW; t += BLOCK) {
-            const int row = top + t / KW, k = t % KW;
+        for (int t = threadIdx.x; t < sl_all.n; t += BLOCK) {
+            int row, k;
+            walk_slot(sl_all, t, row, k);
             const uint2 w = walks[row - top];
             const int coff0 = walk_coff(w);
             const int col = walk_col(walk_ls(w), coff0, k);
@@ -223,9 +259,9 @@ __device__ void xtrans_tile(const XArgs &A, char *ar, int top, int left) {
     // green from closer pixels (:2141-2182), passes 1 and 2: the d of 3..5
     // whose buffer is j
     auto pass_green = [&](int j) {
-        const int r0 = top + 2, r1 = mrow0 - 2;
-        for (int t = threadIdx.x; t < max(r1 - r0, 0) * KW; t += BLOCK) {
-            const int row = r0 + t / KW, k = t % KW;
+        for (int t = threadIdx.x; t < sl_pg.n; t += BLOCK) {
+            int row, k;
+            walk_slot(sl_pg, t, row, k);
             const uint2 w = walks[TS + row - top];
             const int coff0 = walk_coff(w);
             const int d = coff0 == 3 ? j + 2 : (j ^ 1) + 2;
@@ -284,9 +320,9 @@ __device__ void xtrans_tile(const XArgs &A, char *ar, int top, int left) {
 
     // R at B sites and B at R sites (:2224-2266), buffer j of the pass
     auto r_at_b = [&](int j) {
-        const int r0 = top + 3, r1 = mrow0 - 3;
-        for (int t = threadIdx.x; t < max(r1 - r0, 0) * KW; t += BLOCK) {
-            const int row = r0 + t / KW, k = t % KW;
+        for (int t = threadIdx.x; t < sl_rb.n; t += BLOCK) {
+            int row, k;
+            walk_slot(sl_rb, t, row, k);
             const uint2 w = walks[2 * TS + row - top];
             const int col = walk_col(walk_ls(w), walk_coff(w), k);
             if (col >= mcol0 - 3) continue;
