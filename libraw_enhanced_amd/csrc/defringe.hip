@@ -578,9 +578,15 @@ __global__ __launch_bounds__(BLOCK) void line_link(Src SR, Scal Cb, Gates G) {
                 along = mx(along, s_ls[c + step * s]);
                 along = mx(along, s_ls[c - step * s]);
             }
-            const float across = mx(s_fs[c + nstep * 2], s_fs[c - nstep * 2]);
-            const float thin = 1.f - sstep(along * 0.45f, along * 0.95f, across);
-            best = mx(best, along * thin);
+            // along starts at +0 and takes only larger values, so it is +0 or
+            // positive; at +0, along * thin is +0 or NaN (0 / 0 in the
+            // smoothstep) and mx(best, .) keeps best either way: the
+            // direction's division is skipped
+            if (along > 0.f) {
+                const float across = mx(s_fs[c + nstep * 2], s_fs[c - nstep * 2]);
+                const float thin = 1.f - sstep(along * 0.45f, along * 0.95f, across);
+                best = mx(best, along * thin);
+            }
         }
         const size_t i = (size_t)y * W + x;
         P.lfs[i] = best;
