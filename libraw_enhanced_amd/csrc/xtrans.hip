@@ -391,7 +391,10 @@ __device__ void xtrans_tile(const XArgs &A, char *ar, int top, int left) {
     auto store_buffer = [&](int d) {
         const float4 *src = reinterpret_cast<const float4 *>(lds);
         float4 *dst = reinterpret_cast<float4 *>(RGB(rgb0, d, 0, 0));
-        for (int t = q0 + threadIdx.x; t < q1; t += BLOCK) dst[t] = src[t];
+        typedef float f4 __attribute__((ext_vector_type(4)));
+        const f4 *s4 = reinterpret_cast<const f4 *>(src);
+        f4 *d4 = reinterpret_cast<f4 *>(dst);
+        for (int t = q0 + threadIdx.x; t < q1; t += BLOCK) __builtin_nontemporal_store(s4[t], &d4[t]);
     };
 
     for (int j = 0; j < 4; j++) {
@@ -538,7 +541,7 @@ __device__ void xtrans_tile(const XArgs &A, char *ar, int top, int left) {
                     if ((sel[u] >> d) & 1u) {
                         const float *q = RGB(rgb0, d, prow[u], pcol[u]);
 #pragma unroll
-                        for (int c = 0; c < 3; c++) p[u][d][c] = q[c];
+                        for (int c = 0; c < 3; c++) p[u][d][c] = __builtin_nontemporal_load(q + c);
                     }
 #pragma unroll
             for (int u = 0; u < AVU; u++) {
