@@ -248,8 +248,8 @@ __global__ __launch_bounds__(AX_BLOCK) void axial_ab(Src S) {
             size_t i;
             if (!index_of(k, i)) continue;
             const double a = (mIP[k] - mI[k] * mP[k]) / (var[k] + e);
-            (ch ? A.aB : A.aR)[i] = (float)a;
-            (ch ? A.bB : A.bR)[i] = (float)(mP[k] - a * mI[k]);
+            __builtin_nontemporal_store((float)a, &(ch ? A.aB : A.aR)[i]);
+            __builtin_nontemporal_store((float)(mP[k] - a * mI[k]), &(ch ? A.bB : A.bR)[i]);
         }
     }
     double mE[PER];
@@ -259,7 +259,7 @@ __global__ __launch_bounds__(AX_BLOCK) void axial_ab(Src S) {
         size_t i;
         if (!index_of(k, i)) continue;
         const float t = lre::clamp_std(((float)mE[k] - 0.01f) / 0.05f, 0.f, 1.f);
-        A.ew[i] = t * t * (3.f - 2.f * t);
+        __builtin_nontemporal_store(t * t * (3.f - 2.f * t), &A.ew[i]);
     }
 }
 
