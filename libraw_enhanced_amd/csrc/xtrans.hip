@@ -155,6 +155,25 @@ __device__ void xtrans_tile(const XArgs &A, char *ar, int top, int left) {
 #define PX(r, c) (lds + ((r) * TS + (c)) * 3)
 #define GM(r, c) gm[(size_t)(r) * (TS / 2) + (c)]
 
+    // tables: walks of green4 (from left), the pass green step (from left + 2)
+    // and the R-at-B step (from left + 3)
+    if (threadIdx.x < 2 * 9 * 8) hexl[threadIdx.x] = (&A.hex[0][0][0][0])[threadIdx.x];
+    if (threadIdx.x < 36) xtl[threadIdx.x] = A.xt[threadIdx.x / 6][threadIdx.x % 6];
+    for (int t = threadIdx.x; t < 3 * TS; t += BLOCK) {
+        const int step = t / TS, row = top + t % TS;
+        if (row >= mrow0) continue;
+        walks[t] = step == 0 ? make_walk(A, row, left, mcol0)
+                 : step == 1 ? make_walk(A, row, left + 2, mcol0 - 2)
+                             : make_walk(A, row, left + 3, mcol0 - 1);
+    }
+    __syncthreads();
+    auto walk_ls = [](uint2 w) { return (int)(w.x & 0xffffu); };
+    auto walk_coff = [](uint2 w) { return (int)((w.x >> 16) & 15u); };
+    auto walk_hex = [&](int set, uint2 w, int k) {
+        return hexl + (set * 9 + (int)((w.x >> ((k & 1) ? 24 : 20)) & 15u)) * 8;
+    };
+    auto walk_colour = [](uint2 w, int k) { return (int)((w.y >> (8 * (k & 3))) & 255u); };
+
     // green min/max (:1973-2059).  The reference walks each row: on rows with
     // one non-green site per three columns every site fills its own slot
     // (slots are column pairs); on the other rows a first lone site may come,
@@ -165,8 +184,9 @@ __device__ void xtrans_tile(const XArgs &A, char *ar, int top, int left) {
     for (int t = threadIdx.x; t < sl_all.n; t += BLOCK) {
         int row, u;
         walk_slot(sl_all, t, row, u);
-        const int ls = first_nongreen(A, row, left, mcol0);
-        const int coff = A.rshift[row % 3] == 1 ? 3 : 1 + (fcolx(A, row, ls + 1) & 1);
+        // the row's first site and stride, as its green-step walk holds them
+        const uint2 w = walks[row - top];
+        const int ls = walk_ls(w), coff = walk_coff(w);
         auto mmhex = [&](int col, const short *hx) {
             float lo = FLT_MAX, hi = 0.f;
             for (int c = 0; c < 6; c++) {
@@ -196,24 +216,7 @@ __device__ void xtrans_tile(const XArgs &A, char *ar, int top, int left) {
         GM(row - top, (col - left) >> 1) = m;
         if (col < mcol0 - 1) GM(row - top, (col + 1 - left) >> 1) = m;
     }
-    // tables: walks of green4 (from left), the pass green step (from left + 2)
-    // and the R-at-B step (from left + 3)
-    if (threadIdx.x < 2 * 9 * 8) hexl[threadIdx.x] = (&A.hex[0][0][0][0])[threadIdx.x];
-    if (threadIdx.x < 36) xtl[threadIdx.x] = A.xt[threadIdx.x / 6][threadIdx.x % 6];
-    for (int t = threadIdx.x; t < 3 * TS; t += BLOCK) {
-        const int step = t / TS, row = top + t % TS;
-        if (row >= mrow0) continue;
-        walks[t] = step == 0 ? make_walk(A, row, left, mcol0)
-                 : step == 1 ? make_walk(A, row, left + 2, mcol0 - 2)
-                             : make_walk(A, row, left + 3, mcol0 - 1);
-    }
-    __syncthreads();
-    auto walk_ls = [](uint2 w) { return (int)(w.x & 0xffffu); };
-    auto walk_coff = [](uint2 w) { return (int)((w.x >> 16) & 15u); };
-    auto walk_hex = [&](int set, uint2 w, int k) {
-        return hexl + (set * 9 + (int)((w.x >> ((k & 1) ? 24 : 20)) & 15u)) * 8;
-    };
-    auto walk_colour = [](uint2 w, int k) { return (int)((w.y >> (8 * (k & 3))) & 255u); };
+    __syncthreads();  // the green bounds, before the green step reads them
 
     // rows of the R/B steps for 2x2 green blocks: those with (row - sgrow) % 3 != 0
     int topstart = top + 2;
